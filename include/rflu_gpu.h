@@ -1,0 +1,143 @@
+/*
+ * rflu_gpu.h -- C ABI of the MI355X (gfx950) RocfluMP explicit residual path.
+ *
+ * One handle = one Rocflu region resident in HBM.  Every entry point returns
+ * 0 on success and a non-zero RFLU_GPU_E* code on failure; the message is in
+ * rflu_gpu_last_error(h) (h may be NULL for errors raised before a handle
+ * exists).  No entry point falls back to the CPU: without a usable GPU,
+ * rflu_gpu_create fails with RFLU_GPU_ENODEV.
+ *
+ * Host arrays use plain pointers and sizes.  Cell arrays are SoA with the
+ * variable index slowest: cv[k*nCells + i] for k = rho, rho u, rho v, rho w,
+ * rho E; dv[k*nCells + i] for k = p, T, c.  Gradients are returned in the
+ * reference's gradCell(3,5,N) order (grad[(i*5+k)*3+d]).
+ *
+ * Each call below names the reference routine it replaces (file:line in the
+ * Rocflu sources).  The handle's stream serialises all work; calls that
+ * return data synchronise on it.
+ */
+#ifndef RFLU_GPU_H
+#define RFLU_GPU_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RFLU_GPU_OK 0
+#define RFLU_GPU_EINVAL 1     /* bad argument / unsupported option */
+#define RFLU_GPU_ENODEV 2     /* no HIP device */
+#define RFLU_GPU_EHIP 3       /* HIP runtime error */
+#define RFLU_GPU_EMESH 4      /* mesh pipeline rejected the grid */
+#define RFLU_GPU_ENAN 5       /* RFLU_CheckValidity: NaN in the state */
+#define RFLU_GPU_ENEGATIVE 6  /* RFLU_CheckPositivity: rho or p <= 0 */
+
+/* spaceDiscr values (modfloflu/ModParameters.F90:663-666) */
+#define RFLU_DISCR_ROE 1
+#define RFLU_DISCR_HLLC 3
+#define RFLU_DISCR_AUSM 4
+/* reconst values (modfloflu/ModParameters.F90:682) */
+#define RFLU_RECONST_NONE 0
+#define RFLU_RECONST_VENKAT 11
+/* boundary kinds on the path (rocflu/RFLU_CentralFirstPatch.F90:575,790) */
+#define RFLU_BC_SLIPWALL 1
+#define RFLU_BC_OUTFLOW 2
+#define RFLU_BCOPT_SUPERSONIC 0
+#define RFLU_BCOPT_SUBSONIC 1
+#define RFLU_BCOPT_MIXED 2
+
+typedef struct rflu_handle rflu_handle;
+
+/* Grid as the reference reads it (RFLU_ReadGridWrapper): node coordinates,
+ * cell-to-vertex lists and per-patch boundary-face vertex lists. */
+typedef struct rflu_mesh_desc {
+  int nVert;
+  const double *xyz;      /* 3*nVert, (x,y,z) per node */
+  int nvc;                /* 4: all tets, 8: all hexes */
+  int nCells;
+  const int *c2v;         /* nvc*nCells, 0-based, Rocflu local vertex order */
+  int nPatches;
+  const int *bcType;      /* RFLU_BC_* per patch */
+  const int *bcOpt;       /* RFLU_BCOPT_* per patch (outflow) */
+  const double *pout;     /* outflow static pressure per patch */
+  const int *nBFaces;     /* faces per patch */
+  int nvf;                /* vertices per boundary face (3 or 4) */
+  const int *bf2v;        /* nvf per face, patches concatenated */
+} rflu_mesh_desc;
+
+/* The input-deck values the path reads (NUMERICS, TIMESTEP, REFERENCE,
+ * ACCELERATION sections; libfloflu/Read*Section.F90). */
+typedef struct rflu_params {
+  int spaceDiscr;         /* RFLU_DISCR_* */
+  int spaceOrder;         /* 1 or 2 */
+  int reconst;            /* RFLU_RECONST_* (order 2) */
+  int patchOrder;         /* ORDERBF: 1 (default) or 2 */
+  double cfl;             /* capped at 2 for RK3 (RFLU_DerivedInputValues.F90:395) */
+  double dtImposed;       /* TIMESTEP; used while time <= 1e-8 */
+  double dtMinLimit;      /* DTMINLIMIT */
+  double dTimeSystem;     /* run length for the dt clip (<= 0: none) */
+  double refCp;
+  double refGamma;
+  double epsentr;         /* ENTROPY (Roe) */
+  double dissFact;        /* DISSFACT (Roe) */
+  int accelOn;
+  double accel[3];
+} rflu_params;
+
+/* RFLU_InitFlowSolver grid/geometry/stencil set-up
+ * (rocflu/RFLU_InitFlowSolver.F90:606-607,769-770,1008-1068). */
+int rflu_gpu_create(const rflu_mesh_desc *mesh, const rflu_params *prm, rflu_handle **out);
+int rflu_gpu_destroy(rflu_handle *h);
+const char *rflu_gpu_last_error(const rflu_handle *h);
+
+int rflu_gpu_sizes(const rflu_handle *h, int *nCells, int *nFaces, int *nBFaces, int *nStencil);
+/* Mesh products of the set-up (for parity checks); any pointer may be NULL.
+ * f2c[2F], fn[4(F+B)], fc[3(F+B)] (interior then boundary faces), bf2c[B],
+ * vol[N], cofg[3N], stencil CSR off[N+1], memb[nStencil], moms[10N]. */
+int rflu_gpu_get_mesh(const rflu_handle *h, int *f2c, double *fn, double *fc, int *bf2c,
+                      double *vol, double *cofg);
+int rflu_gpu_get_stencils(const rflu_handle *h, int *off, int *memb, double *moms);
+
+/* Upload cv and set gv/dv (RFLU_SetVarsWrapper at init). */
+int rflu_gpu_set_state(rflu_handle *h, const double *cv);
+int rflu_gpu_get_state(rflu_handle *h, double *cv, double *dv);
+int rflu_gpu_get_rhs(rflu_handle *h, double *rhs);
+int rflu_gpu_get_grad(rflu_handle *h, double *grad);
+
+/* rocflu/RFLU_TimeStepInviscid.F90:75-335: per-cell dt and its minimum.
+ * dtCell may be NULL. */
+int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLocal);
+/* rocflu/RFLU_MinimumTimeStep.F90:78-365 (+ the run-length clip of
+ * rocflu/RFLU_TimeStepping.F90:489-493) on the device-held time. */
+int rflu_gpu_minimum_time_step(rflu_handle *h, double *dtMin);
+
+/* libfloflu/RkInitMP.F90:74-185.  The device keeps cv untouched during a
+ * stage (round trips happen in registers), so cvOld == cv: nothing to copy. */
+int rflu_gpu_rk_init(rflu_handle *h, int istage);
+/* libfloflu/CellGradientsMP.F90:75-527: LSQ gradients + limiter. */
+int rflu_gpu_cell_gradients(rflu_handle *h);
+/* libfloflu/ConvectiveFluxesMP.F90:73-215: rhs = -diss + face fluxes. */
+int rflu_gpu_convective_fluxes(rflu_handle *h);
+/* libfloflu/SourceTermsMP.F90:73-218: gravity source (its cons/prim round
+ * trip only alters cv bits that RKUpdateMP overwrites). */
+int rflu_gpu_source_terms(rflu_handle *h);
+/* libfloflu/IntegrateSourceTermsMP.F90:75-154: a no-op in the reference. */
+int rflu_gpu_integrate_source_terms(rflu_handle *h);
+/* libfloflu/RkUpdateMP.F90:75-210 + RFLU_SetVarsContWrapper + AfterUpdateMP. */
+int rflu_gpu_rk_update(rflu_handle *h, int istage);
+/* One fused RK stage: gradients, fluxes, sources, update (same bits as the
+ * five calls above in sequence). */
+int rflu_gpu_rk_stage(rflu_handle *h, int istage);
+/* nSteps iterations of the RFLU_TimeStepping loop body (dt, 3 RK stages,
+ * time advance); useGraph = 1 replays one captured hipGraph per step. */
+int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph);
+int rflu_gpu_get_time(rflu_handle *h, double *currentTime, double *dtMin);
+int rflu_gpu_sync(rflu_handle *h);
+
+/* Mean kernel duration (ms) per launch of the named kernel over the last
+ * rflu_gpu_step call with profiling enabled; for measurement only. */
+int rflu_gpu_set_profiling(rflu_handle *h, int on);
+int rflu_gpu_kernel_time(rflu_handle *h, const char *name, double *msPerLaunch, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

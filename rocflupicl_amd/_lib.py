@@ -1,0 +1,102 @@
+"""ctypes binding of librflu_gpu.so (the C ABI declared in include/rflu_gpu.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the library is missing, or no HIP device is present, the calls
+raise.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librflu_gpu.so")
+
+# Every symbol include/rflu_gpu.h declares (checked by the CPU test suite).
+RFLU_SYMBOLS = [
+    "rflu_gpu_create", "rflu_gpu_destroy", "rflu_gpu_last_error", "rflu_gpu_sizes",
+    "rflu_gpu_get_mesh", "rflu_gpu_get_stencils", "rflu_gpu_set_state", "rflu_gpu_get_state",
+    "rflu_gpu_get_rhs", "rflu_gpu_get_grad", "rflu_gpu_time_step_inviscid",
+    "rflu_gpu_minimum_time_step", "rflu_gpu_rk_init", "rflu_gpu_cell_gradients",
+    "rflu_gpu_convective_fluxes", "rflu_gpu_source_terms", "rflu_gpu_integrate_source_terms",
+    "rflu_gpu_rk_update", "rflu_gpu_rk_stage", "rflu_gpu_step", "rflu_gpu_get_time",
+    "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time",
+]
+
+ERRORS = {1: "EINVAL", 2: "ENODEV", 3: "EHIP", 4: "EMESH", 5: "ENAN", 6: "ENEGATIVE"}
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nVert", C.c_int), ("xyz", C.c_void_p), ("nvc", C.c_int), ("nCells", C.c_int),
+                ("c2v", C.c_void_p), ("nPatches", C.c_int), ("bcType", C.c_void_p),
+                ("bcOpt", C.c_void_p), ("pout", C.c_void_p), ("nBFaces", C.c_void_p),
+                ("nvf", C.c_int), ("bf2v", C.c_void_p)]
+
+
+class Params(C.Structure):
+    _fields_ = [("spaceDiscr", C.c_int), ("spaceOrder", C.c_int), ("reconst", C.c_int),
+                ("patchOrder", C.c_int), ("cfl", C.c_double), ("dtImposed", C.c_double),
+                ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
+                ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
+                ("accelOn", C.c_int), ("accel", C.c_double * 3)]
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    PP = C.POINTER(C.c_void_p)
+    L.rflu_gpu_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), PP]
+    L.rflu_gpu_destroy.argtypes = [P]
+    L.rflu_gpu_last_error.argtypes = [P]
+    L.rflu_gpu_last_error.restype = C.c_char_p
+    L.rflu_gpu_sizes.argtypes = [P, P, P, P, P]
+    L.rflu_gpu_get_mesh.argtypes = [P, P, P, P, P, P, P]
+    L.rflu_gpu_get_stencils.argtypes = [P, P, P, P]
+    L.rflu_gpu_set_state.argtypes = [P, P]
+    L.rflu_gpu_get_state.argtypes = [P, P, P]
+    L.rflu_gpu_get_rhs.argtypes = [P, P]
+    L.rflu_gpu_get_grad.argtypes = [P, P]
+    L.rflu_gpu_time_step_inviscid.argtypes = [P, P, P]
+    L.rflu_gpu_minimum_time_step.argtypes = [P, P]
+    for n in ("rflu_gpu_rk_init", "rflu_gpu_rk_update", "rflu_gpu_rk_stage"):
+        getattr(L, n).argtypes = [P, I]
+    for n in ("rflu_gpu_cell_gradients", "rflu_gpu_convective_fluxes", "rflu_gpu_source_terms",
+              "rflu_gpu_integrate_source_terms", "rflu_gpu_sync"):
+        getattr(L, n).argtypes = [P]
+    L.rflu_gpu_step.argtypes = [P, I, I]
+    L.rflu_gpu_get_time.argtypes = [P, P, P]
+    L.rflu_gpu_set_profiling.argtypes = [P, I]
+    L.rflu_gpu_kernel_time.argtypes = [P, C.c_char_p, P, P]
+    for n in RFLU_SYMBOLS:
+        if n != "rflu_gpu_last_error":
+            getattr(L, n).restype = I
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the shared object exports (read with nm; no GPU needed)."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def check(rc, handle=None):
+    if rc != 0:
+        L = load()
+        raise GpuError(rc, L.rflu_gpu_last_error(handle).decode())

@@ -1,0 +1,60 @@
+"""Builds the gfx950 HIP library in-tree (rocflupicl_amd/librflu_gpu.so).
+
+Every source is compiled with -ffp-contract=off: the reference is compiled
+without FMA contraction, and the FP64 path is checked bit for bit against it.
+Packed FP32 instructions are disabled for the device code.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "librflu_gpu.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("ROCFLUPICL_ARCH", "gfx950")
+
+DEVICE_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+              "-I", os.path.join(ROOT, "include")]
+
+SOURCES_HIP = ["rflu_gpu.hip"]
+SOURCES_CPP = ["rflu_mesh.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+
+
+def build(verbose=False):
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for s in SOURCES_CPP:
+        o = os.path.join(bdir, s + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in SOURCES_HIP:
+        o = os.path.join(bdir, s + ".o")
+        _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,84 @@
+"""The workloads of the Rocflu hot path, built from seeds.
+
+* ``sod``: 3-D Sod shock tube, 200x4x4 hexes (h = 0.005) stored as an
+  unstructured grid, nodes jittered +-10% of h (interior freely, boundary
+  nodes within their plane), perfect gas gamma = 1.4 with Cp = 3.5 (R = 1),
+  rho/p = 1/1 left and 0.125/0.1 right of x = 0.5, first-order AUSM+,
+  Wray RK3, CFL 0.5; x-ends supersonic outflow, sides slip walls.
+* ``shock_tets``: [0,1]^3 box of n^3 hexes split into 5 tets each
+  (64^3 -> 1,310,720 tets), nodes jittered +-5%, quiescent air rho = 1.2,
+  p = 1e5, gamma = 1.4, Cp = 1004.64, with a Mach 1.5 shock at x = 0.25
+  moving +x (Rankine-Hugoniot post-shock state behind it); second order with
+  LSQ gradients + Venkatakrishnan limiter; Roe (ENTROPY 0.05) or HLLC.
+* ``curtain``: 400x80x80 hexes over 0.2 x 0.04 x 0.04 m, Mach 2 shock at
+  x = 0.02; the particle curtain lives in picl_configs.
+
+Each builder takes a scale argument so tests can run the same workload at
+sizes the CPU oracle finishes in seconds.
+"""
+import numpy as np
+
+from . import meshgen as mg
+
+AIR = dict(refCp=1004.64, refGamma=1.4)
+
+
+def sod(nx=200, ny=4, nz=4, jitter=0.1, seed=1, discr=4, order=1, **kw):
+    h = 0.005
+    mesh = mg.hex_box(nx, ny, nz, nx * h, ny * h, nz * h, jitter=jitter, seed=seed)
+    params = dict(spaceDiscr=discr, spaceOrder=order, refCp=3.5, refGamma=1.4, cfl=0.5,
+                  dtImposed=3.0e-4, dtMinLimit=0.0)
+    if order == 2:
+        params.update(reconst=11)
+    params.update(kw)
+    x0 = 0.5 * nx * h
+
+    def ic(cofg):
+        x = cofg[:, 0]
+        rho = np.where(x < x0, 1.0, 0.125)
+        p = np.where(x < x0, 1.0, 0.1)
+        z = np.zeros_like(rho)
+        return mg.prim_to_cons(rho, z, z, z, p, 1.4).T.copy()
+
+    return mesh, params, ic
+
+
+def shock_tets(n=64, jitter=0.05, seed=2, discr=1, mach=1.5, order=2, **kw):
+    mesh = mg.tet_box(n, n, n, 1.0, 1.0, 1.0, jitter=jitter, seed=seed)
+    params = dict(spaceDiscr=discr, spaceOrder=order, reconst=11 if order == 2 else 0,
+                  cfl=0.5, dtImposed=1.0e-7, dtMinLimit=0.0, epsentr=0.05, **AIR)
+    params.update(kw)
+    rho1, p1, g = 1.2, 1.0e5, 1.4
+    rho2, u2, p2 = mg.normal_shock_state(mach, rho1, p1, g)
+
+    def ic(cofg):
+        x = cofg[:, 0]
+        post = x < 0.25
+        rho = np.where(post, rho2, rho1)
+        u = np.where(post, u2, 0.0)
+        p = np.where(post, p2, p1)
+        z = np.zeros_like(rho)
+        return mg.prim_to_cons(rho, u, z, z, p, g).T.copy()
+
+    return mesh, params, ic
+
+
+def shock_hex(nx=400, ny=80, nz=80, lx=0.2, ly=0.04, lz=0.04, mach=2.0, x_shock=0.02,
+              discr=4, order=2, jitter=0.0, seed=3, **kw):
+    mesh = mg.hex_box(nx, ny, nz, lx, ly, lz, jitter=jitter, seed=seed)
+    params = dict(spaceDiscr=discr, spaceOrder=order, reconst=11 if order == 2 else 0,
+                  cfl=0.5, dtImposed=1.0e-8, dtMinLimit=0.0, **AIR)
+    params.update(kw)
+    rho1, p1, g = 1.2, 1.0e5, 1.4
+    rho2, u2, p2 = mg.normal_shock_state(mach, rho1, p1, g)
+
+    def ic(cofg):
+        x = cofg[:, 0]
+        post = x < x_shock
+        rho = np.where(post, rho2, rho1)
+        u = np.where(post, u2, 0.0)
+        p = np.where(post, p2, p1)
+        z = np.zeros_like(rho)
+        return mg.prim_to_cons(rho, u, z, z, p, g).T.copy()
+
+    return mesh, params, ic

@@ -1,0 +1,1208 @@
+// MI355X (gfx950) kernels and C ABI of the RocfluMP explicit residual path.
+//
+// Data layout in HBM (one region per handle):
+//   cv, rhsSum, rhs : SoA [5][nCells] doubles (coalesced cell streams)
+//   rec             : AoS [nCells][8] doubles, one 64-byte record per cell that
+//                     the gather kernels fetch in one segment:
+//                     order 1: (rho, rho u, rho v, rho w, rho E, p, T, c)
+//                     order 2: (rho, u, v, w, p, x, y, z)  (primitive + cofg)
+//   grad            : AoS [nCells][15] limited gradients (order 2)
+//   flux            : AoS [nFaces+nBFaces][5] face fluxes of the stage
+//   face geometry   : SoA nx, ny, nz, |S|, xc, yc, zc per face slot
+//                     (interior faces first, then boundary faces by patch)
+//
+// Bitwise parity with the reference's FP64 arithmetic rests on three rules:
+// the file is compiled with -ffp-contract=off (no FMA, like the reference);
+// expressions keep the Fortran evaluation order; and every per-cell sum runs
+// over the cell's faces in ascending face-slot order, which is the order the
+// reference's face loop and patch loop add into that cell.  Dependent
+// variables (p, T, c) and the in-place cons/prim round trips of
+// CellGradientsMP are recomputed in registers from the stored state, with the
+// same expressions, instead of being stored.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rflu_gpu.h"
+#include "rflu_mesh.hpp"
+
+#define BLK 256
+
+namespace {
+
+struct Gas {
+  double cp, g, gc;   // flux path: refCp, refGamma, R = cp - cp/g (MixtPerf_R_CpG)
+  double gD, gcD;     // dependent-variable path: R = 8314.3/(8314.3/R), g = cp/(cp-R)
+  double epsentr, dissFact;
+  double accel[3];
+  int accelOn;
+};
+
+struct Dev {
+  int nC = 0, nF = 0, nB = 0, S = 0, nBlk = 0;
+  int *f2c = nullptr, *bf2c = nullptr, *bkind = nullptr, *cfOff = nullptr, *cfIdx = nullptr;
+  int *csOff = nullptr, *csMem = nullptr;
+  double *fnx = nullptr, *fny = nullptr, *fnz = nullptr, *fnm = nullptr;
+  double *fcx = nullptr, *fcy = nullptr, *fcz = nullptr, *bpout = nullptr;
+  double *vol = nullptr, *epsq = nullptr, *rm = nullptr;
+  double *cv = nullptr, *rhsSum = nullptr, *rhs = nullptr, *rec = nullptr, *grad = nullptr;
+  double *flux = nullptr, *dtCell = nullptr, *dvOut = nullptr, *partial = nullptr, *scal = nullptr, *cofg = nullptr;
+  unsigned *counter = nullptr;
+  int *err = nullptr;
+};
+
+// scal[] slots
+enum { S_DTLOCAL = 0, S_DTMIN = 1, S_TIME = 2, S_TSR = 3 };
+// err bits
+enum { E_NAN = 1, E_NEG = 2, E_POSFIX = 4, E_BOUNDFIX = 8 };
+// boundary kinds on device: 0 slip wall, 1..3 outflow (1 + bcOpt)
+}  // namespace
+
+struct rflu_handle {
+  rflu_params prm;
+  Gas gas;
+  Dev d;
+  rflu::MeshHost mesh;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hipGraphExec_t graph = nullptr;
+  bool profiling = false;
+  struct Prof { double ms = 0; int n = 0; };
+  std::map<std::string, Prof> prof;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> evPool;
+};
+
+static std::string g_err;
+
+#define HIPCHK(h, x)                                                                \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      (h)->err = std::string(#x) + ": " + hipGetErrorString(e_);                    \
+      return RFLU_GPU_EHIP;                                                         \
+    }                                                                               \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers (thermodynamics: libfloflu/MixtPerf_*.F90)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_rec(const double *__restrict__ rec, int i, double r[8]) {
+  const double2 *p = reinterpret_cast<const double2 *>(rec + 8 * (size_t)i);
+  double2 a = p[0], b = p[1], c = p[2], e = p[3];
+  r[0] = a.x; r[1] = a.y; r[2] = b.x; r[3] = b.y; r[4] = c.x; r[5] = c.y; r[6] = e.x; r[7] = e.y;
+}
+__device__ __forceinline__ void store_rec(double *__restrict__ rec, int i, const double r[8]) {
+  double2 *p = reinterpret_cast<double2 *>(rec + 8 * (size_t)i);
+  p[0] = make_double2(r[0], r[1]);
+  p[1] = make_double2(r[2], r[3]);
+  p[2] = make_double2(r[4], r[5]);
+  p[3] = make_double2(r[6], r[7]);
+}
+
+// libflu/RFLU_SetDependentVars.F90:256-287 (TCPERF, ksg = 0)
+struct DV { double p, T, c; };
+__device__ __forceinline__ DV dep_vars(const Gas &G, double rho, double m1, double m2, double m3, double E) {
+  double ir = 1.0 / rho;
+  double Eo = ir * E;
+  double Vm2 = ir * ir * (m1 * m1 + m2 * m2 + m3 * m3);
+  DV d;
+  d.p = (G.gD - 1.0) * rho * (Eo - 0.5 * Vm2 - 0.0);
+  d.T = d.p / (rho * G.gcD);
+  d.c = sqrt(G.gD * G.gcD * d.T);
+  return d;
+}
+
+// libfloflu/MixtPerf_E.F90: Eo = P/(D(G-1)) + |u|^2/2 + ksg
+__device__ __forceinline__ double eo_dgpuvw(double D, double G, double P, double U, double V, double W) {
+  return P / (D * (G - 1.0)) + 0.5 * (U * U + V * V + W * W) + 0.0;
+}
+
+// Record of a cell from its conserved state (and dv as computed above).
+__device__ __forceinline__ void make_rec(int order, double rho, double m1, double m2, double m3,
+                                         double E, const DV &dv, double x, double y, double z,
+                                         double r[8]) {
+  if (order == 1) {
+    r[0] = rho; r[1] = m1; r[2] = m2; r[3] = m3; r[4] = E; r[5] = dv.p; r[6] = dv.T; r[7] = dv.c;
+  } else {
+    double ir = 1.0 / rho;  // modflu/RFLU_ModConvertCv.F90:221-231
+    r[0] = rho; r[1] = ir * m1; r[2] = ir * m2; r[3] = ir * m3; r[4] = dv.p; r[5] = x; r[6] = y; r[7] = z;
+  }
+}
+
+// Speed of sound of a cell from its record.
+__device__ __forceinline__ double rec_c(const Gas &G, int order, const double r[8]) {
+  if (order == 1) return r[7];
+  double T = r[4] / (r[0] * G.gcD);
+  return sqrt(G.gD * G.gcD * T);
+}
+// Velocity the reference's dt loop uses: cv(m)*(1/cv(rho)) == cons2prim u.
+__device__ __forceinline__ void rec_u(int order, const double r[8], double &u, double &v, double &w) {
+  if (order == 1) {
+    double ir = 1.0 / r[0];
+    u = r[1] * ir; v = r[2] * ir; w = r[3] * ir;
+  } else {
+    u = r[1]; v = r[2]; w = r[3];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Records from cv (RFLU_SetVarsWrapper at start-up)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_records(Dev d, Gas G, int order) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double rho = d.cv[i], m1 = d.cv[N + i], m2 = d.cv[2 * N + i], m3 = d.cv[3 * N + i], E = d.cv[4 * N + i];
+  DV dv = dep_vars(G, rho, m1, m2, m3, E);
+  double r[8];
+  make_rec(order, rho, m1, m2, m3, E, dv, d.cofg[i], d.cofg[N + i], d.cofg[2 * N + i], r);
+  store_rec(d.rec, i, r);
+}
+
+// ---------------------------------------------------------------------------
+// dt: RFLU_TimeStepInviscid (per cell, faces in reference order) + MINVAL +
+// RFLU_MinimumTimeStep in the last block to finish.
+// ---------------------------------------------------------------------------
+struct DtPar { double cfl, dtImposed, dtMinLimit, dTimeSystem; int finalize; };
+
+__device__ double minimum_time_step(const DtPar &P, double dtLocal, double t, double tsr) {
+  // rocflu/RFLU_MinimumTimeStep.F90:125-146,235-241
+  double cfl = P.cfl < 2.0 ? P.cfl : 2.0;
+  double dmin = cfl * dtLocal;
+  double wI = 0.0, wC = 0.0;
+  if (t <= 1.0e-8) wI = 1.0; else wC = 1.0;
+  dmin = wI / P.dtImposed + wC / dmin;
+  dmin = 1.0 / dmin;
+  if (dmin <= 0.0) dmin = 6.0e-12;
+  dmin = fmax(dmin, P.dtMinLimit);
+  if (P.dTimeSystem > 0.0 && tsr + dmin > P.dTimeSystem) dmin = P.dTimeSystem - tsr;
+  return dmin;
+}
+
+__global__ void __launch_bounds__(BLK) k_dt(Dev d, Gas G, int order, DtPar P, int storeCell) {
+  __shared__ double sm[BLK / 64];
+  __shared__ bool amLast;
+  int i = blockIdx.x * BLK + threadIdx.x;
+  double dt = __builtin_inf();
+  if (i < d.nC) {
+    double ri[8];
+    load_rec(d.rec, i, ri);
+    double ui, vi, wi;
+    rec_u(order, ri, ui, vi, wi);
+    double ai = rec_c(G, order, ri);
+    double sum = 0.0;
+    for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+      int code = d.cfIdx[e], s = code >> 1;
+      double nx = d.fnx[s], ny = d.fny[s], nz = d.fnz[s], nm = d.fnm[s];
+      double srad;
+      if (s < d.nF) {
+        int o = d.f2c[2 * s + ((code & 1) ? 0 : 1)];
+        double ro[8];
+        load_rec(d.rec, o, ro);
+        double uo, vo, wo;
+        rec_u(order, ro, uo, vo, wo);
+        double ao = rec_c(G, order, ro);
+        double vc = 0.5 * ((ui + uo) * nx + (vi + vo) * ny + (wi + wo) * nz);
+        double as = 0.5 * (ai + ao);
+        srad = (fabs(vc - 0.0) + as) * nm;
+      } else {
+        double vc = ui * nx + vi * ny + wi * nz;
+        srad = (fabs(vc - 0.0) + ai) * nm;
+      }
+      sum = sum + srad;
+    }
+    dt = d.vol[i] / sum;
+    if (storeCell) d.dtCell[i] = dt;
+  }
+  // block minimum (wavefront shuffles, then LDS)
+  for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sm[wv] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sm[0];
+    for (int q = 1; q < BLK / 64; q++) m = fmin(m, sm[q]);
+    d.partial[blockIdx.x] = m;
+    __threadfence();
+    unsigned prev = atomicAdd(d.counter, 1u);
+    amLast = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (amLast) {
+    __threadfence();
+    double m = __builtin_inf();
+    const volatile double *pv = d.partial;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += BLK) m = fmin(m, pv[q]);
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+    if (lane == 0) sm[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mm = sm[0];
+      for (int q = 1; q < BLK / 64; q++) mm = fmin(mm, sm[q]);
+      d.scal[S_DTLOCAL] = mm;
+      if (P.finalize) d.scal[S_DTMIN] = minimum_time_step(P, mm, d.scal[S_TIME], d.scal[S_TSR]);
+      *d.counter = 0u;  // ready for the next launch (no memset node in graphs)
+    }
+  }
+}
+
+__global__ void k_min_time_step(Dev d, DtPar P) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    d.scal[S_DTMIN] = minimum_time_step(P, d.scal[S_DTLOCAL], d.scal[S_TIME], d.scal[S_TSR]);
+}
+
+__global__ void k_advance_time(Dev d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    d.scal[S_TIME] = d.scal[S_TIME] + d.scal[S_DTMIN];
+    d.scal[S_TSR] = d.scal[S_TSR] + d.scal[S_DTMIN];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gradients + Venkatakrishnan limiter (order 2), one thread per cell.
+// modflu/RFLU_ModDifferentiationCells.F90:955-1174 (weights from the R-factor
+// each call, members in stencil order) and modflu/RFLU_ModLimiters.F90:339-532
+// (min/max over interior-face neighbours, one limiter term per interior face).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double ri[8];
+  load_rec(d.rec, i, ri);
+  double r11 = d.rm[i], r12 = d.rm[N + i], r22 = d.rm[2 * N + i];
+  double r13 = d.rm[3 * N + i], r23 = d.rm[4 * N + i], r33 = d.rm[5 * N + i];
+  double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+  double c12 = -(c11 * r12);
+  double c13 = -(c11 * r13 + c12 * c22 * r23);
+  double c23 = -(c22 * r23);
+  double c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
+  double g[15];
+#pragma unroll
+  for (int q = 0; q < 15; q++) g[q] = 0.0;
+  int sb = d.csOff[i], se = d.csOff[i + 1];
+  for (int s = sb; s < se; s++) {
+    int j = d.csMem[s];
+    double rj[8];
+    load_rec(d.rec, j, rj);
+    double dx = rj[5] - ri[5], dy = rj[6] - ri[6], dz = rj[7] - ri[7];
+    double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    dx = term * dx; dy = term * dy; dz = term * dz;
+    double term1 = c1111 * (dx);
+    double term2 = c2222 * (dy + c12 * dx);
+    double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+    double wtx = term * (term1 + c12 * term2 + c13 * term3);
+    double wty = term * (term2 + c23 * term3);
+    double wtz = term * (term3);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      double dVar = rj[k] - ri[k];
+      g[3 * k] = g[3 * k] + wtx * dVar;
+      g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
+      g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    }
+  }
+  if (venkat) {
+    const double TINY = 1.0e-12;
+    double vmin[5], vmax[5], lim[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) { vmin[k] = ri[k]; vmax[k] = ri[k]; lim[k] = 1.0; }
+    int eb = d.cfOff[i], ee = d.cfOff[i + 1];
+    for (int e = eb; e < ee; e++) {
+      int code = d.cfIdx[e], s = code >> 1;
+      if (s >= d.nF) break;  // boundary slots come last
+      int o = d.f2c[2 * s + ((code & 1) ? 0 : 1)];
+      double ro[8];
+      load_rec(d.rec, o, ro);
+#pragma unroll
+      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], ro[k]); vmax[k] = fmax(vmax[k], ro[k]); }
+    }
+    double epsq = d.epsq[i];
+    for (int e = eb; e < ee; e++) {
+      int s = d.cfIdx[e] >> 1;
+      if (s >= d.nF) break;
+      double dx1 = ri[5] - d.fcx[s], dy1 = ri[6] - d.fcy[s], dz1 = ri[7] - d.fcz[s];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        double d21 = g[3 * k] * dx1 + g[3 * k + 1] * dy1 + g[3 * k + 2] * dz1;
+        if (d21 > 0.0) {
+          double d1 = vmax[k] - ri[k];
+          double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
+          double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
+          lim[k] = fmin(numer / (denom + TINY), lim[k]);
+        } else if (d21 < 0.0) {
+          double d1 = vmin[k] - ri[k];
+          double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
+          double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
+          lim[k] = fmin(numer / (denom + TINY), lim[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      g[3 * k] = lim[k] * g[3 * k];
+      g[3 * k + 1] = lim[k] * g[3 * k + 1];
+      g[3 * k + 2] = lim[k] * g[3 * k + 2];
+    }
+  }
+  double *o = d.grad + 15 * (size_t)i;
+#pragma unroll
+  for (int q = 0; q < 15; q++) o[q] = g[q];
+}
+
+// ---------------------------------------------------------------------------
+// Face fluxes
+// ---------------------------------------------------------------------------
+// modflu/RFLU_ModAUSMFlux.F90:282-367
+__device__ __forceinline__ void ausm_flux(double nx, double ny, double nz, double nm,
+                                          double rl, double ul, double vl, double wl, double pl, double Hl, double al,
+                                          double rr, double ur, double vr, double wr, double pr, double Hr, double ar,
+                                          double f[5]) {
+  const double fs = 0.0;
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double qr = ur * nx + vr * ny + wr * nz - fs;
+  double af = 0.5 * (al + ar);
+  double ml = ql / af, mla = fabs(ml);
+  double mr = qr / af, mra = fabs(mr);
+  double mlp, wtl, mrm, wtr;
+  if (mla <= 1.0) {
+    mlp = 0.25 * (ml + 1.0) * (ml + 1.0) + 0.125 * (ml * ml - 1.0) * (ml * ml - 1.0);
+    wtl = 0.25 * (ml + 1.0) * (ml + 1.0) * (2.0 - ml) + 0.1875 * ml * (ml * ml - 1.0) * (ml * ml - 1.0);
+  } else {
+    mlp = 0.5 * (ml + mla);
+    wtl = 0.5 * (1.0 + ml / mla);
+  }
+  if (mra <= 1.0) {
+    mrm = -0.25 * (mr - 1.0) * (mr - 1.0) - 0.125 * (mr * mr - 1.0) * (mr * mr - 1.0);
+    wtr = 0.25 * (mr - 1.0) * (mr - 1.0) * (2.0 + mr) - 0.1875 * mr * (mr * mr - 1.0) * (mr * mr - 1.0);
+  } else {
+    mrm = 0.5 * (mr - mra);
+    wtr = 0.5 * (1.0 - mr / mra);
+  }
+  double mf = mlp + mrm, mfa = fabs(mf);
+  double mfp = 0.5 * (mf + mfa), mfm = 0.5 * (mf - mfa);
+  double pf = wtl * pl + wtr * pr;
+  f[0] = (af * (mfp * rl + mfm * rr)) * nm;
+  f[1] = (af * (mfp * rl * ul + mfm * rr * ur) + pf * nx) * nm;
+  f[2] = (af * (mfp * rl * vl + mfm * rr * vr) + pf * ny) * nm;
+  f[3] = (af * (mfp * rl * wl + mfm * rr * wr) + pf * nz) * nm;
+  f[4] = (af * (mfp * rl * Hl + mfm * rr * Hr) + pf * fs) * nm;
+}
+
+// modflu/RFLU_ModRoeFlux.F90:2783-2855 (+ RFLU_ModEntropyFixes.F90:95-106)
+__device__ __forceinline__ double hh_fix(double l, double dd) { return (l > dd) ? l : (l * l + dd * dd) / (2.0 * dd); }
+__device__ __forceinline__ void roe_flux(const Gas &G, double nx, double ny, double nz, double nm,
+                                         double rl, double ul, double vl, double wl, double pl, double Hl, double ql,
+                                         double rr, double ur, double vr, double wr, double pr, double Hr, double qr,
+                                         double f[5]) {
+  const double fs = 0.0;
+  double rh = sqrt(rl * rr);
+  double wt = rl / (rl + rh);
+  double uh = wt * ul + (1.0 - wt) * ur;
+  double vh = wt * vl + (1.0 - wt) * vr;
+  double wh = wt * wl + (1.0 - wt) * wr;
+  double Hh = wt * Hl + (1.0 - wt) * Hr;
+  double qh = uh * nx + vh * ny + wh * nz - fs;
+  double sh = 0.5 * (uh * uh + vh * vh + wh * wh);
+  double ah = sqrt((G.g - 1.0) * (Hh - 0.5 * (2.0 * sh)));
+  double l1 = fabs(qh - ah), l2 = fabs(qh), l5 = fabs(qh + ah);
+  double efc = G.epsentr * l5;
+  l1 = hh_fix(l1, efc);
+  l2 = hh_fix(l2, efc);
+  l5 = hh_fix(l5, efc);
+  double dr = rr - rl, du = ur - ul, de = vr - vl, dw = wr - wl, dp = pr - pl;
+  double dq = du * nx + de * ny + dw * nz;
+  double dv1 = (dp - rh * ah * dq) / (2.0 * ah * ah);
+  double dv5 = (dp + rh * ah * dq) / (2.0 * ah * ah);
+  double dv2 = dr - dp / (ah * ah);
+  double t1 = l1 * dv1, t2 = l2 * dv2, t3 = l2 * rh, t5 = l5 * dv5;
+  double df = G.dissFact;
+  f[0] = 0.5 * (ql * rl + qr * rr) * nm - 0.5 * df * (t1 + t2 + t5) * nm;
+  f[1] = 0.5 * (ql * rl * ul + pl * nx + qr * rr * ur + pr * nx) * nm -
+         0.5 * df * (t1 * (uh - nx * ah) + t2 * uh + t3 * (du - nx * dq) + t5 * (uh + nx * ah)) * nm;
+  f[2] = 0.5 * (ql * rl * vl + pl * ny + qr * rr * vr + pr * ny) * nm -
+         0.5 * df * (t1 * (vh - ny * ah) + t2 * vh + t3 * (de - ny * dq) + t5 * (vh + ny * ah)) * nm;
+  f[3] = 0.5 * (ql * rl * wl + pl * nz + qr * rr * wr + pr * nz) * nm -
+         0.5 * df * (t1 * (wh - nz * ah) + t2 * wh + t3 * (dw - nz * dq) + t5 * (wh + nz * ah)) * nm;
+  f[4] = 0.5 * (ql * rl * Hl + pl * fs + qr * rr * Hr + pr * fs) * nm -
+         0.5 * df * (t1 * (Hh - qh * ah) + t2 * sh + t3 * (uh * du + vh * de + wh * dw - qh * dq) + t5 * (Hh + qh * ah)) * nm;
+}
+
+// modflu/RFLU_ModHLLCFlux.F90:2338-2409
+__device__ __forceinline__ void hllc_flux(const Gas &G, double nx, double ny, double nz, double nm,
+                                          double rl, double ul, double vl, double wl, double pl, double el, double al, double ql, double Hl,
+                                          double rr, double ur, double vr, double wr, double pr, double er, double ar, double qr, double Hr,
+                                          double f[5]) {
+  const double fs = 0.0;
+  double wt1 = sqrt(rr / rl);
+  double wt2 = 1.0 / (1.0 + wt1);
+  double us = wt2 * (ul + wt1 * ur);
+  double vs = wt2 * (vl + wt1 * vr);
+  double ws = wt2 * (wl + wt1 * wr);
+  double qs = wt2 * (ql + wt1 * qr);
+  double Hs = wt2 * (Hl + wt1 * Hr);
+  double Vms = us * us + vs * vs + ws * ws;
+  double as = sqrt((G.g - 1.0) * (Hs - 0.5 * Vms));
+  double sl = fmin(ql - al, qs - as);
+  double sr = fmax(qr + ar, qs + as);
+  if (sl > 0.0) {
+    f[0] = ql * rl * nm;
+    f[1] = (ql * rl * ul + pl * nx) * nm;
+    f[2] = (ql * rl * vl + pl * ny) * nm;
+    f[3] = (ql * rl * wl + pl * nz) * nm;
+    f[4] = (ql * (rl * el + pl) + pl * fs) * nm;
+  } else if (sr < 0.0) {
+    f[0] = qr * rr * nm;
+    f[1] = (qr * rr * ur + pr * nx) * nm;
+    f[2] = (qr * rr * vr + pr * ny) * nm;
+    f[3] = (qr * rr * wr + pr * nz) * nm;
+    f[4] = (qr * (rr * er + pr) + pr * fs) * nm;
+  } else {
+    double sm = (rr * qr * (sr - qr) - rl * ql * (sl - ql) - pr + pl) / (rr * (sr - qr) - rl * (sl - ql));
+    double ps = pl + rl * (ql - sl) * (ql - sm);
+    double rs, rus, rvs, rws, res, term;
+    if (sm > 0.0) {
+      term = 1.0 / (sl - sm);
+      rs = term * (sl - ql) * rl;
+      rus = term * ((sl - ql) * rl * ul + (ps - pl) * nx);
+      rvs = term * ((sl - ql) * rl * vl + (ps - pl) * ny);
+      rws = term * ((sl - ql) * rl * wl + (ps - pl) * nz);
+      res = term * ((sl - ql) * rl * el + (ps * sm - pl * ql));
+    } else {
+      term = 1.0 / (sr - sm);
+      rs = term * (sr - qr) * rr;
+      rus = term * ((sr - qr) * rr * ur + (ps - pr) * nx);
+      rvs = term * ((sr - qr) * rr * vr + (ps - pr) * ny);
+      rws = term * ((sr - qr) * rr * wr + (ps - pr) * nz);
+      res = term * ((sr - qr) * rr * er + (ps * sm - pr * qr));
+    }
+    f[0] = sm * rs * nm;
+    f[1] = (sm * rus + ps * nx) * nm;
+    f[2] = (sm * rvs + ps * ny) * nm;
+    f[3] = (sm * rws + ps * nz) * nm;
+    f[4] = (sm * (res + ps) - ps * fs) * nm;
+  }
+}
+
+// Face state reconstructed at (xc,yc,zc) from an order-2 record and gradient;
+// u is taken through the reference's prim->cons->prim round trip
+// (RFLU_ModConvertCv.F90:554-592 then RFLU_ModAUSMFlux.F90:2735-2783).
+struct FaceState { double r, u, v, w, p, ir; };
+__device__ __forceinline__ FaceState reconstruct(const double rc[8], const double *__restrict__ gp,
+                                                 double xc, double yc, double zc) {
+  FaceState s;
+  double rho = rc[0];
+  double ir = 1.0 / rho;
+  double u = (rho * rc[1]) * ir, v = (rho * rc[2]) * ir, w = (rho * rc[3]) * ir, p = rc[4];
+  double dx = xc - rc[5], dy = yc - rc[6], dz = zc - rc[7];
+  double g[15];
+#pragma unroll
+  for (int q = 0; q < 15; q++) g[q] = gp[q];
+  s.r = rho + g[0] * dx + g[1] * dy + g[2] * dz;
+  s.u = u + g[3] * dx + g[4] * dy + g[5] * dz;
+  s.v = v + g[6] * dx + g[7] * dy + g[8] * dz;
+  s.w = w + g[9] * dx + g[10] * dy + g[11] * dz;
+  s.p = p + g[12] * dx + g[13] * dy + g[14] * dz;
+  s.ir = ir;
+  return s;
+}
+
+// libfloflu/BcondOutflowPerf.F90:88-160 (gas from gv: R = 8314.3/M path)
+__device__ __forceinline__ void bcond_outflow(const Gas &G, int opt, double pout, double sxn, double syn, double szn,
+                                              double rho, double rhou, double rhov, double rhow, double rhoe,
+                                              double press, double b[5]) {
+  double gamma = G.gD;
+  double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+  double csound = sqrt(gamma * press / rho);
+  double mach = sqrt(u * u + v * v + w * w) / csound;
+  if (mach < 1.0 && (opt == RFLU_BCOPT_SUBSONIC || opt == RFLU_BCOPT_MIXED)) {
+    double rrhoc = 1.0 / (rho * csound);
+    double deltp = press - pout;
+    double rhob = rho - deltp / (csound * csound);
+    double ub = u + sxn * deltp * rrhoc;
+    double vb = v + syn * deltp * rrhoc;
+    double wb = w + szn * deltp * rrhoc;
+    double vnd = ub * sxn + vb * syn + wb * szn;
+    if (vnd < 0.0) {
+      ub = copysign(1.0, u) * fmax(fabs(ub), fabs(u));
+      vb = copysign(1.0, v) * fmax(fabs(vb), fabs(v));
+      wb = copysign(1.0, w) * fmax(fabs(wb), fabs(w));
+    }
+    b[0] = rhob; b[1] = rhob * ub; b[2] = rhob * vb; b[3] = rhob * wb;
+    b[4] = rhob * eo_dgpuvw(rhob, gamma, pout, ub, vb, wb);
+  } else {
+    b[0] = rho; b[1] = rhou; b[2] = rhov; b[3] = rhow; b[4] = rhoe;
+  }
+}
+
+struct FluxPar { int order, discr, patchOrder; };
+
+__global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
+  int s = blockIdx.x * BLK + threadIdx.x;
+  if (s >= d.S) return;
+  const double fs = 0.0, ksg = 0.0;
+  double nx = d.fnx[s], ny = d.fny[s], nz = d.fnz[s], nm = d.fnm[s];
+  double f[5];
+  if (s < d.nF) {
+    int c1 = d.f2c[2 * s], c2 = d.f2c[2 * s + 1];
+    double L[8], R[8];
+    load_rec(d.rec, c1, L);
+    load_rec(d.rec, c2, R);
+    if (P.order == 1) {
+      double rl = L[0], irl = 1.0 / rl, ul = L[1] * irl, vl = L[2] * irl, wl = L[3] * irl;
+      double pl = L[5], tl = L[6], al = L[7];
+      double rr = R[0], irr = 1.0 / rr, ur = R[1] * irr, vr = R[2] * irr, wr = R[3] * irr;
+      double pr = R[5], tr = R[6], ar = R[7];
+      if (P.discr == RFLU_DISCR_AUSM) {
+        double Hl = L[4] * irl + pl / rl, Hr = R[4] * irr + pr / rr;
+        ausm_flux(nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, f);
+      } else if (P.discr == RFLU_DISCR_ROE) {
+        double Hl = G.cp * tl + 0.5 * (ul * ul + vl * vl + wl * wl) + ksg;
+        double Hr = G.cp * tr + 0.5 * (ur * ur + vr * vr + wr * wr) + ksg;
+        double ql = ul * nx + vl * ny + wl * nz - fs, qr = ur * nx + vr * ny + wr * nz - fs;
+        roe_flux(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, ql, rr, ur, vr, wr, pr, Hr, qr, f);
+      } else {
+        double el = L[4] * irl, er = R[4] * irr;
+        double ql = ul * nx + vl * ny + wl * nz - fs, qr = ur * nx + vr * ny + wr * nz - fs;
+        double Hl = el + irl * pl, Hr = er + irr * pr;
+        hllc_flux(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, el, al, ql, Hl, rr, ur, vr, wr, pr, er, ar, qr, Hr, f);
+      }
+    } else {
+      double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
+      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, xc, yc, zc);
+      FaceState B = reconstruct(R, d.grad + 15 * (size_t)c2, xc, yc, zc);
+      if (P.discr == RFLU_DISCR_AUSM) {
+        double tl = A.p / (A.r * G.gc), al = sqrt(G.g * G.gc * tl);
+        double Hl = G.cp * tl + 0.5 * (A.u * A.u + A.v * A.v + A.w * A.w) + ksg;
+        double tr = B.p / (B.r * G.gc), ar = sqrt(G.g * G.gc * tr);
+        double Hr = G.cp * tr + 0.5 * (B.u * B.u + B.v * B.v + B.w * B.w) + ksg;
+        ausm_flux(nx, ny, nz, nm, A.r, A.u, A.v, A.w, A.p, Hl, al, B.r, B.u, B.v, B.w, B.p, Hr, ar, f);
+      } else if (P.discr == RFLU_DISCR_ROE) {
+        double tl = A.p / (A.r * G.gc);
+        double Hl = G.cp * tl + 0.5 * (A.u * A.u + A.v * A.v + A.w * A.w) + ksg;
+        double ql = A.u * nx + A.v * ny + A.w * nz - fs;
+        double tr = B.p / (B.r * G.gc);
+        double Hr = G.cp * tr + 0.5 * (B.u * B.u + B.v * B.v + B.w * B.w) + ksg;
+        double qr = B.u * nx + B.v * ny + B.w * nz - fs;
+        roe_flux(G, nx, ny, nz, nm, A.r, A.u, A.v, A.w, A.p, Hl, ql, B.r, B.u, B.v, B.w, B.p, Hr, qr, f);
+      } else {
+        double el = eo_dgpuvw(A.r, G.g, A.p, A.u, A.v, A.w), al = sqrt(G.g * A.p / A.r);
+        double ql = A.u * nx + A.v * ny + A.w * nz - fs, Hl = el + A.ir * A.p;
+        double er = eo_dgpuvw(B.r, G.g, B.p, B.u, B.v, B.w), ar = sqrt(G.g * B.p / B.r);
+        double qr = B.u * nx + B.v * ny + B.w * nz - fs, Hr = er + B.ir * B.p;
+        hllc_flux(G, nx, ny, nz, nm, A.r, A.u, A.v, A.w, A.p, el, al, ql, Hl, B.r, B.u, B.v, B.w, B.p, er, ar, qr, Hr, f);
+      }
+    }
+  } else {
+    // rocflu/RFLU_CentralFirstPatch.F90:575-909 / RFLU_CentralSecondPatch.F90:638-1035
+    int b = s - d.nF;
+    int c1 = d.bf2c[b], kind = d.bkind[b];
+    double L[8];
+    load_rec(d.rec, c1, L);
+    double rl, rul, rvl, rwl, rel, pl;
+    if (P.patchOrder == 1) {
+      if (P.order == 1) {
+        rl = L[0]; rul = L[1]; rvl = L[2]; rwl = L[3]; rel = L[4]; pl = L[5];
+      } else {  // cv after CellGradientsMP's prim->cons conversion
+        rl = L[0]; rul = L[0] * L[1]; rvl = L[0] * L[2]; rwl = L[0] * L[3]; pl = L[4];
+        rel = L[0] * eo_dgpuvw(L[0], G.gD, L[4], L[1], L[2], L[3]);
+      }
+    } else {
+      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, d.fcx[s], d.fcy[s], d.fcz[s]);
+      rl = A.r; pl = A.p;
+      rul = rl * A.u; rvl = rl * A.v; rwl = rl * A.w;
+      rel = (kind != 0) ? rl * eo_dgpuvw(rl, G.gD, pl, A.u, A.v, A.w) : 0.0;
+    }
+    if (kind == 0) {
+      f[0] = -0.0;  // slip wall adds nothing to the continuity residual
+      f[1] = pl * nx * nm;
+      f[2] = pl * ny * nm;
+      f[3] = pl * nz * nm;
+      f[4] = pl * fs * nm;
+    } else {
+      int opt = kind - 1;
+      double pr = (opt != RFLU_BCOPT_SUPERSONIC) ? d.bpout[b] : pl;
+      double bb[5];
+      bcond_outflow(G, opt, pr, nx, ny, nz, rl, rul, rvl, rwl, rel, pl, bb);
+      double rr = bb[0], rur = bb[1], rvr = bb[2], rwr = bb[3], rer = bb[4];
+      double ql = (rul * nx + rvl * ny + rwl * nz) / rl - fs;
+      double qr = (rur * nx + rvr * ny + rwr * nz) / rr - fs;
+      f[0] = 0.5 * (ql * rl + qr * rr) * nm;
+      f[1] = 0.5 * (ql * rul + pl * nx + qr * rur + pr * nx) * nm;
+      f[2] = 0.5 * (ql * rvl + pl * ny + qr * rvr + pr * ny) * nm;
+      f[3] = 0.5 * (ql * rwl + pl * nz + qr * rwr + pr * nz) * nm;
+      f[4] = 0.5 * (ql * (rel + pl) + pl * fs + qr * (rer + pr) + pr * fs) * nm;
+    }
+  }
+  double *o = d.flux + 5 * (size_t)s;
+#pragma unroll
+  for (int k = 0; k < 5; k++) o[k] = f[k];
+}
+
+// ---------------------------------------------------------------------------
+// Residual gather, sources, RK update, dependent variables and checks.
+// ---------------------------------------------------------------------------
+struct UpdPar { int order, istage, fused, storeRhs; double ark, grk; };
+
+// rhs = -diss (diss = 0) + face fluxes in ascending slot order
+// (libfloflu/ConvectiveFluxes.F90:133-141 and the face/patch loops).
+__device__ __forceinline__ void gather_rhs(const Dev &d, int i, double rhs[5]) {
+#pragma unroll
+  for (int k = 0; k < 5; k++) rhs[k] = -0.0;
+  for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+    int code = d.cfIdx[e];
+    const double *f = d.flux + 5 * (size_t)(code >> 1);
+    double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    if (code & 1) {
+      rhs[0] = rhs[0] - f0; rhs[1] = rhs[1] - f1; rhs[2] = rhs[2] - f2; rhs[3] = rhs[3] - f3; rhs[4] = rhs[4] - f4;
+    } else {
+      rhs[0] = rhs[0] + f0; rhs[1] = rhs[1] + f1; rhs[2] = rhs[2] + f2; rhs[3] = rhs[3] + f3; rhs[4] = rhs[4] + f4;
+    }
+  }
+}
+
+// libfloflu/SourceTerms.F90:127-136 on the cv that SourceTermsMP sees
+__device__ __forceinline__ void gravity(const Gas &G, int order, const double r[8], double vol, double rhs[5]) {
+  double rho = r[0], m1, m2, m3;
+  if (order == 1) { m1 = r[1]; m2 = r[2]; m3 = r[3]; }
+  else { m1 = rho * r[1]; m2 = rho * r[2]; m3 = rho * r[3]; }
+  double rhoVol = vol * rho;
+  rhs[1] = rhs[1] - G.accel[0] * rhoVol;
+  rhs[2] = rhs[2] - G.accel[1] * rhoVol;
+  rhs[3] = rhs[3] - G.accel[2] * rhoVol;
+  rhs[4] = rhs[4] - vol * (G.accel[0] * m1 + G.accel[1] * m2 + G.accel[2] * m3);
+}
+
+__global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double rhs[5];
+  double vol = d.vol[i];
+  if (P.fused) {
+    gather_rhs(d, i, rhs);
+    if (G.accelOn) {
+      double r[8];
+      load_rec(d.rec, i, r);
+      gravity(G, P.order, r, vol, rhs);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; k++) rhs[k] = d.rhs[k * N + i];
+  }
+  // libfloflu/RkUpdateGeneric.F90:285-314 (Wray RK3, static grid)
+  double fac = P.ark * d.scal[S_DTMIN];
+  double adtv = fac / vol;
+  double cv[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    double c0 = d.cv[k * N + i];
+    if (P.istage == 1) {
+      cv[k] = c0 - adtv * rhs[k];
+      d.rhsSum[k * N + i] = rhs[k];
+    } else if (P.istage == 2) {
+      double rs = d.rhsSum[k * N + i];
+      cv[k] = c0 - adtv * (rhs[k] - P.grk * rs);
+      d.rhsSum[k * N + i] = rhs[k];
+    } else {
+      double rs = d.rhsSum[k * N + i];
+      cv[k] = c0 - adtv * (rhs[k] - P.grk * rs);
+    }
+  }
+  // RFLU_SetDependentVars, then AfterUpdateMP: RFLU_CheckValidity,
+  // RFLU_EnforceBounds, RFLU_CheckPositivity (libflu/*.F90, TCPERF branches)
+  DV dv = dep_vars(G, cv[0], cv[1], cv[2], cv[3], cv[4]);
+  {
+    double rho = cv[0], rrho = 1.0 / rho;
+    double u = rrho * cv[1], v = rrho * cv[2], w = rrho * cv[3];
+    if (isnan(rho) || isnan(u) || isnan(v) || isnan(w) || isnan(dv.p) || isnan(dv.T)) atomicOr(d.err, E_NAN);
+  }
+  {
+    double rr = cv[0], ir = 1.0 / rr;
+    double u = ir * cv[1], v = ir * cv[2], w = ir * cv[3], Eo = ir * cv[4];
+    double Vm2 = u * u + v * v + w * w;
+    double p = (G.gD - 1.0) * rr * (Eo - 0.5 * Vm2 - 0.0);
+    if (p <= 0.0 && fabs(p) <= 1.0e-12) {
+      cv[4] = rr * eo_dgpuvw(rr, G.gD, 1.0e-12, u, v, w);
+      atomicOr(d.err, E_BOUNDFIX);
+    }
+  }
+  {
+    double rho = cv[0], rrho = 1.0 / rho;
+    double u = rrho * cv[1], v = rrho * cv[2], w = rrho * cv[3];
+    double Vm2 = u * u + v * v + w * w;
+    if (dv.p <= 0.0) {
+      double Eo = 1.0e4 + 0.5 * Vm2 + 0.0;
+      dv.p = (G.gD - 1.0) * rho * (Eo - 0.5 * Vm2 - 0.0);
+      dv.T = dv.p / (rho * G.gcD);
+      cv[4] = rho * Eo;
+      atomicOr(d.err, E_POSFIX);
+    }
+    if (rho <= 0.0 || dv.p <= 0.0) atomicOr(d.err, E_NEG);
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) d.cv[k * N + i] = cv[k];
+  double r[8];
+  make_rec(P.order, cv[0], cv[1], cv[2], cv[3], cv[4], dv, d.cofg[i], d.cofg[N + i], d.cofg[2 * N + i], r);
+  store_rec(d.rec, i, r);
+}
+
+// ConvectiveFluxesMP / SourceTermsMP as separate steps (rhs kept in HBM).
+__global__ void __launch_bounds__(BLK) k_gather(Dev d) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  double rhs[5];
+  gather_rhs(d, i, rhs);
+#pragma unroll
+  for (int k = 0; k < 5; k++) d.rhs[k * d.nC + i] = rhs[k];
+}
+
+__global__ void __launch_bounds__(BLK) k_source(Dev d, Gas G, int order) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double rhs[5], r[8];
+#pragma unroll
+  for (int k = 0; k < 5; k++) rhs[k] = d.rhs[k * N + i];
+  load_rec(d.rec, i, r);
+  gravity(G, order, r, d.vol[i], rhs);
+#pragma unroll
+  for (int k = 0; k < 5; k++) d.rhs[k * N + i] = rhs[k];
+}
+
+// dv for output
+__global__ void __launch_bounds__(BLK) k_dv_out(Dev d, Gas G, int order, double *dv) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double r[8];
+  load_rec(d.rec, i, r);
+  if (order == 1) {
+    dv[i] = r[5]; dv[N + i] = r[6]; dv[2 * N + i] = r[7];
+  } else {
+    double T = r[4] / (r[0] * G.gcD);
+    dv[i] = r[4]; dv[N + i] = T; dv[2 * N + i] = sqrt(G.gD * G.gcD * T);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+int set_err(rflu_handle *h, int code, const std::string &m) {
+  if (h) h->err = m; else g_err = m;
+  return code;
+}
+
+template <class T>
+int dalloc(rflu_handle *h, T **p, size_t n, const T *src = nullptr) {
+  if (n == 0) n = 1;
+  HIPCHK(h, hipMalloc((void **)p, n * sizeof(T)));
+  if (src) HIPCHK(h, hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+inline int nblk(int n) { return (n + BLK - 1) / BLK; }
+
+hipEvent_t ev_get(rflu_handle *h) {
+  if (!h->evPool.empty()) { hipEvent_t e = h->evPool.back(); h->evPool.pop_back(); return e; }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  rflu_handle *h; const char *name; hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(rflu_handle *hh, const char *n) : h(hh), name(n) {
+    if (h->profiling) { a = ev_get(h); b = ev_get(h); hipEventRecord(a, h->stream); }
+  }
+  ~ProfScope() {
+    if (h->profiling) { hipEventRecord(b, h->stream); h->pending.push_back({name, {a, b}}); }
+  }
+};
+
+void prof_collect(rflu_handle *h) {
+  for (auto &p : h->pending) {
+    float ms = 0.f;
+    hipEventSynchronize(p.second.second);
+    hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    auto &q = h->prof[p.first];
+    q.ms += ms;
+    q.n += 1;
+    h->evPool.push_back(p.second.first);
+    h->evPool.push_back(p.second.second);
+  }
+  h->pending.clear();
+}
+
+DtPar dtpar(const rflu_handle *h, int finalize) {
+  return DtPar{h->prm.cfl, h->prm.dtImposed, h->prm.dtMinLimit, h->prm.dTimeSystem, finalize};
+}
+
+int launch_dt(rflu_handle *h, int finalize, int storeCell) {
+  ProfScope ps(h, "k_dt");
+  hipLaunchKernelGGL(k_dt, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder,
+                     dtpar(h, finalize), storeCell);
+  return 0;
+}
+
+int launch_grad(rflu_handle *h) {
+  if (h->prm.spaceOrder < 2) return 0;
+  ProfScope ps(h, "k_grad");
+  hipLaunchKernelGGL(k_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d,
+                     h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0);
+  return 0;
+}
+
+int launch_flux(rflu_handle *h) {
+  ProfScope ps(h, "k_flux");
+  FluxPar P{h->prm.spaceOrder, h->prm.spaceDiscr, h->prm.patchOrder};
+  hipLaunchKernelGGL(k_flux, dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, P);
+  return 0;
+}
+
+UpdPar updpar(const rflu_handle *h, int istage, int fused) {
+  // libflu/RFLU_DerivedInputValues.F90:374-386
+  const double ark[3] = {8.0 / 15.0, 5.0 / 12.0, 0.75};
+  const double grk[3] = {0.0, 17.0 / 25.0, 5.0 / 9.0};
+  return UpdPar{h->prm.spaceOrder, istage, fused, 0, ark[istage - 1], grk[istage - 1]};
+}
+
+int launch_update(rflu_handle *h, int istage, int fused) {
+  ProfScope ps(h, "k_update");
+  hipLaunchKernelGGL(k_update, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, updpar(h, istage, fused));
+  return 0;
+}
+
+int enqueue_step(rflu_handle *h) {
+  launch_dt(h, 1, 0);
+  for (int st = 1; st <= 3; st++) {
+    launch_grad(h);
+    launch_flux(h);
+    launch_update(h, st, 1);
+  }
+  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, h->stream, h->d);
+  return 0;
+}
+
+int check_err(rflu_handle *h) {
+  int e = 0;
+  HIPCHK(h, hipMemcpyAsync(&e, h->d.err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (e & E_NAN) return set_err(h, RFLU_GPU_ENAN, "RFLU_CheckValidity: invalid (NaN) variables detected");
+  if (e & E_NEG) return set_err(h, RFLU_GPU_ENEGATIVE, "RFLU_CheckPositivity: negative positive-definite variables");
+  return 0;
+}
+
+}  // namespace (host helpers are file-local; C ABI below)
+
+extern "C" {
+
+const char *rflu_gpu_last_error(const rflu_handle *h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handle **out) {
+  if (!md || !prm || !out) return set_err(nullptr, RFLU_GPU_EINVAL, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(nullptr, RFLU_GPU_ENODEV, "no HIP device available (the library has no CPU path)");
+  if (prm->spaceOrder != 1 && prm->spaceOrder != 2) return set_err(nullptr, RFLU_GPU_EINVAL, "spaceOrder must be 1 or 2");
+  if (prm->spaceDiscr != RFLU_DISCR_ROE && prm->spaceDiscr != RFLU_DISCR_HLLC && prm->spaceDiscr != RFLU_DISCR_AUSM)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "spaceDiscr must be 1 (Roe), 3 (HLLC) or 4 (AUSM+)");
+  if (prm->patchOrder == 2 && prm->spaceOrder != 2)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "second-order patches need spaceOrder 2");
+  if (prm->reconst != RFLU_RECONST_NONE && prm->reconst != RFLU_RECONST_VENKAT)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "reconst must be 0 or 11 (Venkatakrishnan)");
+  for (int p = 0; p < md->nPatches; p++) {
+    if (md->bcType[p] != RFLU_BC_SLIPWALL && md->bcType[p] != RFLU_BC_OUTFLOW)
+      return set_err(nullptr, RFLU_GPU_EINVAL, "only slip-wall and outflow patches are supported");
+    if (md->bcType[p] == RFLU_BC_OUTFLOW && (md->bcOpt[p] < 0 || md->bcOpt[p] > 2))
+      return set_err(nullptr, RFLU_GPU_EINVAL, "outflow bcOpt must be 0, 1 or 2");
+  }
+  rflu_handle *h = new rflu_handle();
+  h->prm = *prm;
+  rflu::MeshInput in;
+  in.nVert = md->nVert; in.xyz = md->xyz; in.nvc = md->nvc; in.nCells = md->nCells; in.c2v = md->c2v;
+  in.nPatches = md->nPatches; in.bcType = md->bcType; in.bcOpt = md->bcOpt; in.pout = md->pout;
+  in.nBFaces = md->nBFaces; in.nvf = md->nvf; in.bf2v = md->bf2v;
+  std::string e = rflu::build_mesh(in, prm->spaceOrder > 1, h->mesh);
+  if (!e.empty()) { delete h; return set_err(nullptr, RFLU_GPU_EMESH, "mesh: " + e); }
+  const rflu::MeshHost &m = h->mesh;
+  Gas &G = h->gas;
+  G.cp = prm->refCp; G.g = prm->refGamma; G.gc = G.cp - G.cp / G.g;
+  double mol = 8314.3 / G.gc;          // RFLU_SetGasVars: gv(MOL)
+  G.gcD = 8314.3 / mol;                // MixtPerf_R_M
+  G.gD = G.cp / (G.cp - G.gcD);        // MixtPerf_G_CpR
+  G.epsentr = prm->epsentr; G.dissFact = prm->dissFact; G.accelOn = prm->accelOn;
+  for (int q = 0; q < 3; q++) G.accel[q] = prm->accel[q];
+  Dev &d = h->d;
+  d.nC = m.nCells; d.nF = m.nFaces; d.nB = m.nBFaces; d.S = m.nFaces + m.nBFaces; d.nBlk = nblk(d.nC);
+  if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed"); }
+  int rc = 0;
+  std::vector<int> bkind(m.nBFaces);
+  std::vector<double> bpout(m.nBFaces);
+  for (int b = 0; b < m.nBFaces; b++) {
+    int p = m.bPatch[b];
+    bkind[b] = md->bcType[p] == RFLU_BC_SLIPWALL ? 0 : 1 + md->bcOpt[p];
+    bpout[b] = md->pout[p];
+  }
+  const int S = d.S, N = d.nC;
+  std::vector<double> soa(7 * (size_t)S);
+  for (int s = 0; s < S; s++) {
+    for (int q = 0; q < 4; q++) soa[q * (size_t)S + s] = m.fn[4 * (size_t)s + q];
+    for (int q = 0; q < 3; q++) soa[(4 + q) * (size_t)S + s] = m.fc[3 * (size_t)s + q];
+  }
+  std::vector<double> cofgSoa(3 * (size_t)N), rmSoa(6 * (size_t)N, 0.0);
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < 3; q++) cofgSoa[q * (size_t)N + i] = m.cofg[3 * (size_t)i + q];
+  if (prm->spaceOrder > 1)
+    for (int i = 0; i < N; i++)
+      for (int q = 0; q < 6; q++) rmSoa[q * (size_t)N + i] = m.moms[10 * (size_t)i + q];
+  double *faceSoa = nullptr;
+  rc |= dalloc(h, &d.f2c, 2 * (size_t)d.nF, m.f2c.data());
+  rc |= dalloc(h, &d.bf2c, (size_t)d.nB, m.bf2c.data());
+  rc |= dalloc(h, &d.bkind, (size_t)d.nB, bkind.data());
+  rc |= dalloc(h, &d.bpout, (size_t)d.nB, bpout.data());
+  rc |= dalloc(h, &faceSoa, 7 * (size_t)S, soa.data());
+  rc |= dalloc(h, &d.cfOff, (size_t)N + 1, m.cfOff.data());
+  rc |= dalloc(h, &d.cfIdx, m.cfIdx.size(), m.cfIdx.data());
+  rc |= dalloc(h, &d.vol, (size_t)N, m.vol.data());
+  rc |= dalloc(h, &d.epsq, (size_t)N, m.epsq.data());
+  rc |= dalloc(h, &d.cofg, 3 * (size_t)N, cofgSoa.data());
+  if (prm->spaceOrder > 1) {
+    rc |= dalloc(h, &d.csOff, (size_t)N + 1, m.csOff.data());
+    rc |= dalloc(h, &d.csMem, m.csMem.size(), m.csMem.data());
+    rc |= dalloc(h, &d.rm, 6 * (size_t)N, rmSoa.data());
+    rc |= dalloc(h, &d.grad, 15 * (size_t)N);
+  }
+  rc |= dalloc(h, &d.cv, 5 * (size_t)N);
+  rc |= dalloc(h, &d.rhsSum, 5 * (size_t)N);
+  rc |= dalloc(h, &d.rhs, 5 * (size_t)N);
+  rc |= dalloc(h, &d.rec, 8 * (size_t)N);
+  rc |= dalloc(h, &d.flux, 5 * (size_t)S);
+  rc |= dalloc(h, &d.dtCell, (size_t)N);
+  rc |= dalloc(h, &d.dvOut, 3 * (size_t)N);
+  rc |= dalloc(h, &d.partial, (size_t)d.nBlk);
+  rc |= dalloc(h, &d.scal, 8);
+  rc |= dalloc(h, &d.counter, 1);
+  rc |= dalloc(h, &d.err, 1);
+  if (rc) { std::string msg = h->err; rflu_gpu_destroy(h); return set_err(nullptr, RFLU_GPU_EHIP, msg); }
+  d.fnx = faceSoa; d.fny = faceSoa + S; d.fnz = faceSoa + 2 * (size_t)S; d.fnm = faceSoa + 3 * (size_t)S;
+  d.fcx = faceSoa + 4 * (size_t)S; d.fcy = faceSoa + 5 * (size_t)S; d.fcz = faceSoa + 6 * (size_t)S;
+  hipMemset(d.scal, 0, 8 * sizeof(double));
+  hipMemset(d.counter, 0, sizeof(unsigned));
+  hipMemset(d.err, 0, sizeof(int));
+  hipMemset(d.rhsSum, 0, 5 * (size_t)N * sizeof(double));
+  hipDeviceSynchronize();
+  *out = h;
+  return 0;
+}
+
+int rflu_gpu_destroy(rflu_handle *h) {
+  if (!h) return 0;
+  Dev &d = h->d;
+  void *ptrs[] = {d.f2c, d.bf2c, d.bkind, d.bpout, d.fnx, d.cfOff, d.cfIdx, d.vol, d.epsq, d.cofg, d.csOff,
+                  d.csMem, d.rm, d.grad, d.cv, d.rhsSum, d.rhs, d.rec, d.flux, d.dtCell, d.dvOut, d.partial, d.scal,
+                  d.counter, d.err};
+  for (void *p : ptrs) if (p) hipFree(p);
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  for (auto e : h->evPool) hipEventDestroy(e);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int rflu_gpu_sizes(const rflu_handle *h, int *nC, int *nF, int *nB, int *nS) {
+  if (!h) return RFLU_GPU_EINVAL;
+  if (nC) *nC = h->mesh.nCells;
+  if (nF) *nF = h->mesh.nFaces;
+  if (nB) *nB = h->mesh.nBFaces;
+  if (nS) *nS = h->mesh.csOff.empty() ? 0 : h->mesh.csOff.back();
+  return 0;
+}
+
+int rflu_gpu_get_mesh(const rflu_handle *h, int *f2c, double *fn, double *fc, int *bf2c, double *vol, double *cofg) {
+  if (!h) return RFLU_GPU_EINVAL;
+  const rflu::MeshHost &m = h->mesh;
+  if (f2c) memcpy(f2c, m.f2c.data(), m.f2c.size() * sizeof(int));
+  if (fn) memcpy(fn, m.fn.data(), m.fn.size() * sizeof(double));
+  if (fc) memcpy(fc, m.fc.data(), m.fc.size() * sizeof(double));
+  if (bf2c) memcpy(bf2c, m.bf2c.data(), m.bf2c.size() * sizeof(int));
+  if (vol) memcpy(vol, m.vol.data(), m.vol.size() * sizeof(double));
+  if (cofg) memcpy(cofg, m.cofg.data(), m.cofg.size() * sizeof(double));
+  return 0;
+}
+
+int rflu_gpu_get_stencils(const rflu_handle *h, int *off, int *memb, double *moms) {
+  if (!h) return RFLU_GPU_EINVAL;
+  const rflu::MeshHost &m = h->mesh;
+  if (off && !m.csOff.empty()) memcpy(off, m.csOff.data(), m.csOff.size() * sizeof(int));
+  if (memb && !m.csMem.empty()) memcpy(memb, m.csMem.data(), m.csMem.size() * sizeof(int));
+  if (moms && !m.moms.empty()) memcpy(moms, m.moms.data(), m.moms.size() * sizeof(double));
+  return 0;
+}
+
+int rflu_gpu_set_state(rflu_handle *h, const double *cv) {
+  if (!h || !cv) return RFLU_GPU_EINVAL;
+  HIPCHK(h, hipMemcpyAsync(h->d.cv, cv, 5 * (size_t)h->d.nC * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_records, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_get_state(rflu_handle *h, double *cv, double *dv) {
+  if (!h) return RFLU_GPU_EINVAL;
+  size_t N = h->d.nC;
+  if (cv) HIPCHK(h, hipMemcpyAsync(cv, h->d.cv, 5 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (dv) {
+    hipLaunchKernelGGL(k_dv_out, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder, h->d.dvOut);
+    HIPCHK(h, hipMemcpyAsync(dv, h->d.dvOut, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_get_rhs(rflu_handle *h, double *rhs) {
+  if (!h || !rhs) return RFLU_GPU_EINVAL;
+  HIPCHK(h, hipMemcpyAsync(rhs, h->d.rhs, 5 * (size_t)h->d.nC * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_get_grad(rflu_handle *h, double *grad) {
+  if (!h || !grad) return RFLU_GPU_EINVAL;
+  if (h->prm.spaceOrder < 2) return set_err(h, RFLU_GPU_EINVAL, "no gradients at spaceOrder 1");
+  HIPCHK(h, hipMemcpyAsync(grad, h->d.grad, 15 * (size_t)h->d.nC * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLocal) {
+  if (!h) return RFLU_GPU_EINVAL;
+  launch_dt(h, 0, dtCell ? 1 : 0);
+  HIPCHK(h, hipGetLastError());
+  if (dtCell) HIPCHK(h, hipMemcpyAsync(dtCell, h->d.dtCell, (size_t)h->d.nC * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (dtMinLocal) HIPCHK(h, hipMemcpyAsync(dtMinLocal, h->d.scal + S_DTLOCAL, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_minimum_time_step(rflu_handle *h, double *dtMin) {
+  if (!h) return RFLU_GPU_EINVAL;
+  hipLaunchKernelGGL(k_min_time_step, dim3(1), dim3(64), 0, h->stream, h->d, dtpar(h, 1));
+  HIPCHK(h, hipGetLastError());
+  if (dtMin) HIPCHK(h, hipMemcpyAsync(dtMin, h->d.scal + S_DTMIN, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_rk_init(rflu_handle *h, int istage) {
+  if (!h || istage < 1 || istage > 3) return RFLU_GPU_EINVAL;
+  return 0;
+}
+
+int rflu_gpu_cell_gradients(rflu_handle *h) {
+  if (!h) return RFLU_GPU_EINVAL;
+  launch_grad(h);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int rflu_gpu_convective_fluxes(rflu_handle *h) {
+  if (!h) return RFLU_GPU_EINVAL;
+  launch_flux(h);
+  {
+    ProfScope ps(h, "k_gather");
+    hipLaunchKernelGGL(k_gather, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
+  }
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int rflu_gpu_source_terms(rflu_handle *h) {
+  if (!h) return RFLU_GPU_EINVAL;
+  if (h->gas.accelOn) hipLaunchKernelGGL(k_source, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int rflu_gpu_integrate_source_terms(rflu_handle *h) { return h ? 0 : RFLU_GPU_EINVAL; }
+
+int rflu_gpu_rk_update(rflu_handle *h, int istage) {
+  if (!h || istage < 1 || istage > 3) return RFLU_GPU_EINVAL;
+  launch_update(h, istage, 0);
+  HIPCHK(h, hipGetLastError());
+  return check_err(h);
+}
+
+int rflu_gpu_rk_stage(rflu_handle *h, int istage) {
+  if (!h || istage < 1 || istage > 3) return RFLU_GPU_EINVAL;
+  launch_grad(h);
+  launch_flux(h);
+  launch_update(h, istage, 1);
+  HIPCHK(h, hipGetLastError());
+  return check_err(h);
+}
+
+int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph) {
+  if (!h || nSteps < 0) return RFLU_GPU_EINVAL;
+  if (useGraph && !h->profiling) {
+    if (!h->graph) {
+      hipGraph_t g;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(h);
+      HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+      HIPCHK(h, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+      hipGraphDestroy(g);
+    }
+    for (int n = 0; n < nSteps; n++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+  } else {
+    for (int n = 0; n < nSteps; n++) enqueue_step(h);
+    HIPCHK(h, hipGetLastError());
+  }
+  int rc = check_err(h);
+  if (h->profiling) prof_collect(h);
+  return rc;
+}
+
+int rflu_gpu_get_time(rflu_handle *h, double *t, double *dtMin) {
+  if (!h) return RFLU_GPU_EINVAL;
+  double s[8];
+  HIPCHK(h, hipMemcpyAsync(s, h->d.scal, sizeof s, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (t) *t = s[S_TIME];
+  if (dtMin) *dtMin = s[S_DTMIN];
+  return 0;
+}
+
+int rflu_gpu_sync(rflu_handle *h) {
+  if (!h) return RFLU_GPU_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_set_profiling(rflu_handle *h, int on) {
+  if (!h) return RFLU_GPU_EINVAL;
+  h->profiling = on != 0;
+  h->prof.clear();
+  return 0;
+}
+
+int rflu_gpu_kernel_time(rflu_handle *h, const char *name, double *ms, int *n) {
+  if (!h || !name) return RFLU_GPU_EINVAL;
+  auto it = h->prof.find(name);
+  if (it == h->prof.end() || it->second.n == 0) { if (ms) *ms = 0; if (n) *n = 0; return 0; }
+  if (ms) *ms = it->second.ms / it->second.n;
+  if (n) *n = it->second.n;
+  return 0;
+}
+
+}  // extern "C"

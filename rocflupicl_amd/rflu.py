@@ -1,0 +1,182 @@
+"""Host-side mirror of the Rocflu residual-path call surface over the HIP C ABI.
+
+``Region`` plays the role of the reference's ``TYPE(t_region)`` for the hot
+path: it owns one region resident in HBM and exposes the reference's
+subroutines under their own names (see include/rflu_gpu.h for the file:line
+each one replaces).  Arrays cross the boundary in the reference's Fortran
+layout: cv[N,5] (cv(5,N)), dv[N,3], grad[N,5,3].
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .meshgen import BC_OUTFLOW, BC_SLIPWALL  # noqa: F401  (re-exported)
+
+DISCR_ROE, DISCR_HLLC, DISCR_AUSM = 1, 3, 4
+RECONST_NONE, RECONST_VENKAT = 0, 11
+
+DEFAULTS = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1,
+                cfl=0.5, dtImposed=1.0e-5, dtMinLimit=1.0e-10, dTimeSystem=0.0,
+                refCp=1004.5, refGamma=1.4, epsentr=0.05, dissFact=1.0, accelOn=0,
+                accel=(0.0, 0.0, 0.0))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(**kw):
+    d = dict(DEFAULTS)
+    unknown = set(kw) - set(d)
+    if unknown:
+        raise TypeError(f"unknown parameters {sorted(unknown)}")
+    d.update(kw)
+    p = _lib.Params()
+    for k, v in d.items():
+        if k == "accel":
+            for i in range(3):
+                p.accel[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Region:
+    """One Rocflu region on the GPU (RFLU_InitFlowSolver for the hot path)."""
+
+    def __init__(self, mesh, **params):
+        L = _lib.load()
+        self._L = L
+        self.params = make_params(**params)
+        xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
+        c2v = np.ascontiguousarray(mesh["c2v"], dtype=np.int32)
+        pts = mesh["patches"]
+        self._keep = [xyz, c2v]
+        bt = np.array([p["bcType"] for p in pts], np.int32)
+        bo = np.array([p["bcOpt"] for p in pts], np.int32)
+        po = np.array([p["pout"] for p in pts], np.float64)
+        nb = np.array([p["bf2v"].shape[0] for p in pts], np.int32)
+        bf = np.ascontiguousarray(np.concatenate([p["bf2v"] for p in pts]), np.int32)
+        md = _lib.MeshDesc(xyz.shape[0], _p(xyz), c2v.shape[1], c2v.shape[0], _p(c2v), len(pts),
+                           _p(bt), _p(bo), _p(po), _p(nb), bf.shape[1], _p(bf))
+        h = C.c_void_p()
+        rc = L.rflu_gpu_create(C.byref(md), C.byref(self.params), C.byref(h))
+        _lib.check(rc, None)
+        self.h = h
+        n = [C.c_int() for _ in range(4)]
+        L.rflu_gpu_sizes(self.h, *[C.byref(x) for x in n])
+        self.nCells, self.nFaces, self.nBFaces, self.nStencil = (x.value for x in n)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.rflu_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        _lib.check(rc, self.h)
+
+    # ---- mesh products -------------------------------------------------
+    def mesh_arrays(self):
+        F, B, N = self.nFaces, self.nBFaces, self.nCells
+        f2c = np.zeros((F, 2), np.int32)
+        fn = np.zeros((F + B, 4))
+        fc = np.zeros((F + B, 3))
+        bf2c = np.zeros(B, np.int32)
+        vol = np.zeros(N)
+        cofg = np.zeros((N, 3))
+        self._chk(self._L.rflu_gpu_get_mesh(self.h, _p(f2c), _p(fn), _p(fc), _p(bf2c), _p(vol), _p(cofg)))
+        return dict(f2c=f2c, fn=fn[:F], fc=fc[:F], bfn=fn[F:], bfc=fc[F:], bf2c=bf2c, vol=vol, cofg=cofg)
+
+    def stencils(self):
+        off = np.zeros(self.nCells + 1, np.int32)
+        mem = np.zeros(max(self.nStencil, 1), np.int32)
+        moms = np.zeros((self.nCells, 10))
+        self._chk(self._L.rflu_gpu_get_stencils(self.h, _p(off), _p(mem), _p(moms)))
+        return off, mem[: self.nStencil], moms
+
+    # ---- state ---------------------------------------------------------
+    def set_state(self, cv):
+        """cv as (N,5): rho, rho u, rho v, rho w, rho E per cell."""
+        cv = np.asarray(cv, dtype=np.float64)
+        assert cv.shape == (self.nCells, 5)
+        soa = np.ascontiguousarray(cv.T)
+        self._chk(self._L.rflu_gpu_set_state(self.h, _p(soa)))
+
+    def get_state(self):
+        cv = np.zeros((5, self.nCells))
+        dv = np.zeros((3, self.nCells))
+        self._chk(self._L.rflu_gpu_get_state(self.h, _p(cv), _p(dv)))
+        return cv.T.copy(), dv.T.copy()
+
+    def get_rhs(self):
+        r = np.zeros((5, self.nCells))
+        self._chk(self._L.rflu_gpu_get_rhs(self.h, _p(r)))
+        return r.T.copy()
+
+    def get_grad(self):
+        g = np.zeros((self.nCells, 5, 3))
+        self._chk(self._L.rflu_gpu_get_grad(self.h, _p(g)))
+        return g
+
+    # ---- the reference's routines --------------------------------------
+    def RFLU_TimeStepInviscid(self, per_cell=False):
+        dt = np.zeros(self.nCells) if per_cell else None
+        m = C.c_double()
+        self._chk(self._L.rflu_gpu_time_step_inviscid(self.h, _p(dt) if per_cell else None, C.byref(m)))
+        return (m.value, dt) if per_cell else m.value
+
+    def RFLU_MinimumTimeStep(self):
+        m = C.c_double()
+        self._chk(self._L.rflu_gpu_minimum_time_step(self.h, C.byref(m)))
+        return m.value
+
+    def RKInitMP(self, istage):
+        self._chk(self._L.rflu_gpu_rk_init(self.h, istage))
+
+    def CellGradientsMP(self):
+        self._chk(self._L.rflu_gpu_cell_gradients(self.h))
+
+    def ConvectiveFluxesMP(self):
+        self._chk(self._L.rflu_gpu_convective_fluxes(self.h))
+
+    def SourceTermsMP(self):
+        self._chk(self._L.rflu_gpu_source_terms(self.h))
+
+    def IntegrateSourceTermsMP(self):
+        self._chk(self._L.rflu_gpu_integrate_source_terms(self.h))
+
+    def RKUpdateMP(self, istage):
+        self._chk(self._L.rflu_gpu_rk_update(self.h, istage))
+
+    def rk_stage(self, istage):
+        self._chk(self._L.rflu_gpu_rk_stage(self.h, istage))
+
+    def RungeKuttaMP(self):
+        for s in (1, 2, 3):
+            self.rk_stage(s)
+
+    def RFLU_TimeStepping(self, nSteps, use_graph=True):
+        self._chk(self._L.rflu_gpu_step(self.h, int(nSteps), 1 if use_graph else 0))
+
+    def time(self):
+        t, dt = C.c_double(), C.c_double()
+        self._chk(self._L.rflu_gpu_get_time(self.h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def sync(self):
+        self._chk(self._L.rflu_gpu_sync(self.h))
+
+    def set_profiling(self, on):
+        self._chk(self._L.rflu_gpu_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_int()
+        self._chk(self._L.rflu_gpu_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
