@@ -80,6 +80,11 @@ typedef struct rflu_params {
   double dissFact;        /* DISSFACT (Roe) */
   int accelOn;
   double accel[3];
+  /* 0 (reference): Venkatakrishnan term = numer/(denom + 1e-12)
+   * (modflu/RFLU_ModLimiters.F90:459-505), which flips and amplifies the
+   * gradient when denom lies in (-1e-12, 0).  1: add 1e-12 with the sign of
+   * denom (a deliberate deviation, off by default). */
+  int limiterGuard;
 } rflu_params;
 
 /* RFLU_InitFlowSolver grid/geometry/stencil set-up

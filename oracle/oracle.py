@@ -22,7 +22,7 @@ class OrcParams(C.Structure):
                 ("patchOrder", C.c_int), ("cfl", C.c_double), ("dtImposed", C.c_double),
                 ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
                 ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
-                ("accelOn", C.c_int), ("accel", C.c_double * 3)]
+                ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int)]
 
 
 def build():
@@ -60,6 +60,10 @@ def lib():
         L.orc_last_dt.argtypes = [P]
         L.orc_last_dt.restype = D
         L.orc_flux_ausm.argtypes = [P, D, P, P, P]
+        L.orc_rk_stage.argtypes = [P, I]
+        L.orc_rk_stage.restype = I
+        L.orc_set_dtmin.argtypes = [P, D]
+        L.orc_flux_point.argtypes = [C.POINTER(OrcParams), I, P, P, P, P]
         _LIB = L
     return _LIB
 
@@ -72,7 +76,7 @@ def make_params(**kw):
     p = OrcParams()
     d = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1, cfl=0.5,
              dtImposed=1e-5, dtMinLimit=1e-10, dTimeSystem=0.0, refCp=1004.5, refGamma=1.4,
-             epsentr=0.05, dissFact=1.0, accelOn=0, accel=(0.0, 0.0, 0.0))
+             epsentr=0.05, dissFact=1.0, accelOn=0, accel=(0.0, 0.0, 0.0), limiterGuard=0)
     d.update(kw)
     for k, v in d.items():
         if k == "accel":
@@ -182,6 +186,10 @@ class Region:
         if e:
             raise RuntimeError("oracle step failed: " + lib().orc_last_error().decode())
 
+    def rk_stage(self, istage, dt):
+        lib().orc_set_dtmin(self.h, dt)
+        return lib().orc_rk_stage(self.h, istage)
+
     @property
     def time(self):
         return lib().orc_time(self.h)
@@ -197,4 +205,15 @@ def flux_ausm(n, fs, L, R):
     L = np.ascontiguousarray(L, np.float64)
     R = np.ascontiguousarray(R, np.float64)
     lib().orc_flux_ausm(_p(n), fs, _p(L), _p(R), _p(out))
+    return out
+
+
+def flux_point(discr, n, L, R, **params):
+    """Face flux F(L, R) through n = (nx, ny, nz, |S|); L, R = (rho, u, v, w, p)."""
+    out = np.zeros(5)
+    prm = make_params(**params)
+    n = np.ascontiguousarray(n, np.float64)
+    L = np.ascontiguousarray(L, np.float64)
+    R = np.ascontiguousarray(R, np.float64)
+    lib().orc_flux_point(C.byref(prm), discr, _p(n), _p(L), _p(R), _p(out))
     return out

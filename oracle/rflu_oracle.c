@@ -651,7 +651,9 @@ static void gradients(orc_region *r) {
 /* Venkatakrishnan: modflu/RFLU_ModLimiters.F90:339-532, 707-760 */
 static void limiter_venkat(orc_region *r) {
   int N = r->nCells;
-  const double THRD = 1.0 / 3.0, TINY = 1.0e-12, K = 1.0;
+  const double THRD = 1.0 / 3.0, K = 1.0;
+  const double TINYP = 1.0e-12, TINYN = r->p.limiterGuard ? -1.0e-12 : 1.0e-12;
+#define TINY (denom >= 0.0 ? TINYP : TINYN)
   double *vmin = malloc(sizeof(double) * 5 * N), *vmax = malloc(sizeof(double) * 5 * N);
   for (int i = 0; i < N; i++)
     for (int k = 0; k < 5; k++) {
@@ -709,6 +711,7 @@ static void limiter_venkat(orc_region *r) {
       }
     }
   }
+#undef TINY
   for (int i = 0; i < N; i++)
     for (int k = 0; k < 5; k++) {
       double l = r->lim[5 * (size_t)i + k];
@@ -869,6 +872,27 @@ static void hllc_flux(const orc_params *p, double nx, double ny, double nz, doub
     flx[2] = (sm * rvs + ps * ny) * nm;
     flx[3] = (sm * rws + ps * nz) * nm;
     flx[4] = (sm * (res + ps) - ps * fs) * nm;
+  }
+}
+
+/* Point fluxes for closed-form tests.  L/R = (rho,u,v,w,p); H, e and a are
+ * derived with the flux path's perfect-gas relations (refCp, refGamma). */
+void orc_flux_point(const orc_params *p, int discr, const double *n, const double *L, const double *R,
+                    double *flx) {
+  double cp = p->refCp, g = p->refGamma, gc = R_CpG(cp, g), fs = 0.0;
+  double tl = T_DPR(L[0], L[4], gc), tr = T_DPR(R[0], R[4], gc);
+  double Hl = Ho_CpTUVW(cp, tl, L[1], L[2], L[3], 0.0), Hr = Ho_CpTUVW(cp, tr, R[1], R[2], R[3], 0.0);
+  double ql = L[1] * n[0] + L[2] * n[1] + L[3] * n[2] - fs, qr = R[1] * n[0] + R[2] * n[1] + R[3] * n[2] - fs;
+  if (discr == ORC_DISCR_AUSM) {
+    ausm_flux_function(n[0], n[1], n[2], n[3], fs, L[0], L[1], L[2], L[3], L[4], Hl, C_GRT(g, gc, tl),
+                       R[0], R[1], R[2], R[3], R[4], Hr, C_GRT(g, gc, tr), flx);
+  } else if (discr == ORC_DISCR_ROE) {
+    roe_flux(p, n[0], n[1], n[2], n[3], fs, L[0], L[1], L[2], L[3], L[4], Hl, ql,
+             R[0], R[1], R[2], R[3], R[4], Hr, qr, flx);
+  } else {
+    double el = Eo_DGPUVW(L[0], g, L[4], L[1], L[2], L[3], 0.0), er = Eo_DGPUVW(R[0], g, R[4], R[1], R[2], R[3], 0.0);
+    hllc_flux(p, n[0], n[1], n[2], n[3], fs, L[0], L[1], L[2], L[3], L[4], el, C_DGP(L[0], g, L[4]), ql,
+              el + L[4] / L[0], R[0], R[1], R[2], R[3], R[4], er, C_DGP(R[0], g, R[4]), qr, er + R[4] / R[0], flx);
   }
 }
 
@@ -1160,6 +1184,17 @@ static int after_update(orc_region *r) {
   }
   return 0;
 }
+
+/* One RungeKuttaMP stage with the current dtMin (diagnostics). */
+int orc_rk_stage(orc_region *r, int istage) {
+  memcpy(r->cvOld, r->cv, sizeof(double) * 5 * r->nCells);
+  memset(r->diss, 0, sizeof(double) * 5 * r->nCells);
+  orc_residual(r);
+  rk_update(r, istage);
+  set_dependent_vars(r);
+  return after_update(r);
+}
+void orc_set_dtmin(orc_region *r, double dt) { r->dtMin = dt; }
 
 int orc_step(orc_region *r, int nSteps) {
   for (int it = 0; it < nSteps; it++) {

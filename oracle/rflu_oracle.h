@@ -38,6 +38,7 @@ typedef struct orc_params {
   double dissFact;
   int accelOn;
   double accel[3];
+  int limiterGuard;   /* 0: reference denom+TINY; 1: TINY with the sign of denom */
 } orc_params;
 
 typedef struct orc_region orc_region;
@@ -76,10 +77,15 @@ void orc_residual(orc_region *r);
 /* Full steps: dt, RFLU_MinimumTimeStep, RungeKuttaMP (3 Wray stages). */
 int orc_step(orc_region *r, int nSteps);
 double orc_time(const orc_region *r);
+/* One RK stage with the current dtMin, and a setter for dtMin (diagnostics). */
+int orc_rk_stage(orc_region *r, int istage);
+void orc_set_dtmin(orc_region *r, double dt);
 double orc_last_dt(const orc_region *r);
 
 /* Point flux functions (for closed-form tests). */
 void orc_flux_ausm(const double *nrm, double fs, const double *L, const double *R, double *flx);
+void orc_flux_point(const orc_params *p, int discr, const double *nrm, const double *L, const double *R,
+                    double *flx);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("patchOrder", C.c_int), ("cfl", C.c_double), ("dtImposed", C.c_double),
                 ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
                 ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
-                ("accelOn", C.c_int), ("accel", C.c_double * 3)]
+                ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int)]
 
 
 _lib = None

@@ -270,7 +270,7 @@ __global__ void k_advance_time(Dev d) {
 // each call, members in stencil order) and modflu/RFLU_ModLimiters.F90:339-532
 // (min/max over interior-face neighbours, one limiter term per interior face).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat) {
+__global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat, int guard) {
   int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat) {
     }
   }
   if (venkat) {
-    const double TINY = 1.0e-12;
+    const double TINYP = 1.0e-12, TINYN = guard ? -1.0e-12 : 1.0e-12;
     double vmin[5], vmax[5], lim[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) { vmin[k] = ri[k]; vmax[k] = ri[k]; lim[k] = 1.0; }
@@ -335,12 +335,12 @@ __global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat) {
           double d1 = vmax[k] - ri[k];
           double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
           double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
-          lim[k] = fmin(numer / (denom + TINY), lim[k]);
+          lim[k] = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim[k]);
         } else if (d21 < 0.0) {
           double d1 = vmin[k] - ri[k];
           double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
           double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
-          lim[k] = fmin(numer / (denom + TINY), lim[k]);
+          lim[k] = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim[k]);
         }
       }
     }
@@ -857,7 +857,7 @@ int launch_grad(rflu_handle *h) {
   if (h->prm.spaceOrder < 2) return 0;
   ProfScope ps(h, "k_grad");
   hipLaunchKernelGGL(k_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d,
-                     h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0);
+                     h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, h->prm.limiterGuard ? 1 : 0);
   return 0;
 }
 
@@ -1052,6 +1052,7 @@ int rflu_gpu_get_stencils(const rflu_handle *h, int *off, int *memb, double *mom
 int rflu_gpu_set_state(rflu_handle *h, const double *cv) {
   if (!h || !cv) return RFLU_GPU_EINVAL;
   HIPCHK(h, hipMemcpyAsync(h->d.cv, cv, 5 * (size_t)h->d.nC * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d.err, 0, sizeof(int), h->stream));
   hipLaunchKernelGGL(k_records, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -19,7 +19,7 @@ RECONST_NONE, RECONST_VENKAT = 0, 11
 DEFAULTS = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1,
                 cfl=0.5, dtImposed=1.0e-5, dtMinLimit=1.0e-10, dTimeSystem=0.0,
                 refCp=1004.5, refGamma=1.4, epsentr=0.05, dissFact=1.0, accelOn=0,
-                accel=(0.0, 0.0, 0.0))
+                accel=(0.0, 0.0, 0.0), limiterGuard=0)
 
 
 def _p(a):

@@ -129,6 +129,31 @@ def test_steps_match_oracle(kind, discr, order, oracle_mod):
     assert same_bits(gdv, o.dv()), "dv: " + first_diff(gdv, o.dv())
 
 
+@pytest.mark.parametrize("variant", ["guard", "patch2", "subsonic", "mixed"])
+def test_options_match_oracle(variant, oracle_mod):
+    """Venkatakrishnan sign-safe guard, second-order patches, and subsonic /
+    mixed outflow (BcondOutflowPerf's characteristic branch)."""
+    prm = dict(spaceDiscr=1, spaceOrder=2, reconst=11, refCp=1004.64, refGamma=1.4, dtImposed=2e-8, cfl=0.8)
+    bc = None
+    if variant == "guard":
+        prm["limiterGuard"] = 1
+    elif variant == "patch2":
+        prm["patchOrder"] = 2
+    else:
+        opt = mg.BCOPT_SUBSONIC if variant == "subsonic" else mg.BCOPT_MIXED
+        bc = [(mg.BC_OUTFLOW, opt, 1.1e5), (mg.BC_OUTFLOW, opt, 0.9e5)] + [(mg.BC_SLIPWALL, 0, 0.0)] * 4
+    mesh = mg.tet_box(5, 4, 4, 1.0, 0.8, 0.8, jitter=0.05, seed=12, bc=bc)
+    g, o, cofg = _pair(mesh, **prm)
+    cv = _shock_ic(cofg)
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.RFLU_TimeStepping(10)
+    o.step(10)
+    gcv, gdv = g.get_state()
+    assert same_bits(gcv, o.cv()), first_diff(gcv, o.cv())
+    assert same_bits(gdv, o.dv()), first_diff(gdv, o.dv())
+
+
 def test_fine_grained_calls_equal_fused_stage():
     mesh = MESHES["tet"]
     prm = dict(spaceDiscr=1, spaceOrder=2, reconst=11, refCp=1004.64, refGamma=1.4, dtImposed=2e-8,
@@ -170,7 +195,7 @@ def test_gravity_source_matches_oracle(oracle_mod):
 
 def test_graph_replays_equal_eager_steps():
     mesh = MESHES["hex"]
-    prm = dict(spaceDiscr=4, spaceOrder=1, refCp=3.5, refGamma=1.4, dtImposed=1e-4)
+    prm = dict(spaceDiscr=4, spaceOrder=1, refCp=1004.64, refGamma=1.4, dtImposed=2e-8)
     a = rflu.Region(mesh, **prm)
     b = rflu.Region(mesh, **prm)
     cv = _shock_ic(a.mesh_arrays()["cofg"])

@@ -1,0 +1,349 @@
+"""CPU pins of the oracle (no GPU): closed forms, invariants and brute force.
+
+The reference ships no tests for the Rocflu residual path, so each part of
+the oracle is pinned here by something the mathematics fixes, chosen so that
+a dropped term, a wrong sign or a transposed operand fails.
+"""
+import numpy as np
+import pytest
+
+from rocflupicl_amd import configs, meshgen as mg
+
+GAMMA = 1.4
+CP = 1004.64
+
+
+def euler_flux(n, U):
+    r, u, v, w, p = U
+    gc = CP - CP / GAMMA
+    T = p / (r * gc)
+    H = CP * T + 0.5 * (u * u + v * v + w * w)
+    q = u * n[0] + v * n[1] + w * n[2]
+    return np.array([r * q, r * u * q + p * n[0], r * v * q + p * n[1], r * w * q + p * n[2], r * H * q]) * n[3]
+
+
+def rand_state(rng, mach):
+    r = rng.uniform(0.5, 2.0)
+    p = rng.uniform(5e4, 2e5)
+    c = np.sqrt(GAMMA * p / r)
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    return np.array([r, *(mach * c * d), p])
+
+
+def rand_normal(rng):
+    n = rng.normal(size=3)
+    return np.array([*(n / np.linalg.norm(n)), rng.uniform(0.1, 3.0)])
+
+
+@pytest.mark.parametrize("discr", [1, 3, 4])
+@pytest.mark.parametrize("mach", [0.0, 0.3, 0.95, 1.7, 3.0])
+def test_flux_consistency_F_U_U_is_euler_flux(discr, mach, oracle_mod):
+    rng = np.random.default_rng(100 + discr + int(10 * mach))
+    for _ in range(20):
+        U = rand_state(rng, mach)
+        n = rand_normal(rng)
+        F = oracle_mod.flux_point(discr, n, U, U, refCp=CP, refGamma=GAMMA)
+        E = euler_flux(n, U)
+        scale = np.abs(E).max() + U[4] * n[3]
+        assert np.abs(F - E).max() <= 1e-13 * scale, (F, E)
+
+
+@pytest.mark.parametrize("discr", [1, 3, 4])
+def test_supersonic_flux_is_fully_upwind(discr, oracle_mod):
+    """Both states supersonic along +n: every scheme returns F(U_L)."""
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        L = rand_state(rng, 0.0)
+        R = rand_state(rng, 0.0)
+        n = rand_normal(rng)
+        for U in (L, R):
+            c = np.sqrt(GAMMA * U[4] / U[0])
+            U[1:4] = 2.5 * c * n[:3]
+        # make the Roe-average state supersonic too
+        F = oracle_mod.flux_point(discr, n, L, R, refCp=CP, refGamma=GAMMA)
+        E = euler_flux(n, L)
+        assert np.abs(F - E).max() <= 1e-11 * np.abs(E).max(), (discr, F, E)
+
+
+def test_flux_distinguishes_left_and_right(oracle_mod):
+    """Swapping L/R and flipping the normal negates the flux (antisymmetry).
+
+    Roe is tested without the entropy fix: its threshold eps*|q+a| is not
+    symmetric under the swap, which is the reference's own formulation."""
+    rng = np.random.default_rng(3)
+    for discr in (1, 3, 4):
+        for _ in range(10):
+            L, R = rand_state(rng, 0.6), rand_state(rng, 0.4)
+            n = rand_normal(rng)
+            nm = n.copy()
+            nm[:3] = -nm[:3]
+            kw = dict(refCp=CP, refGamma=GAMMA, epsentr=0.0)
+            F1 = oracle_mod.flux_point(discr, n, L, R, **kw)
+            F2 = oracle_mod.flux_point(discr, nm, R, L, **kw)
+            assert np.abs(F1 + F2).max() <= 1e-10 * np.abs(F1).max()
+
+
+def box_counts(nx, ny, nz):
+    nf = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
+    nb = 2 * (ny * nz + nx * nz + nx * ny)
+    return nf, nb
+
+
+def test_face_and_boundary_counts(oracle_mod):
+    o = oracle_mod.Region(mg.hex_box(5, 4, 3, 1.0, 1.0, 1.0), spaceOrder=1)
+    assert (o.nFaces, o.nBFaces) == box_counts(5, 4, 3)
+    t = oracle_mod.Region(mg.tet_box(4, 3, 3, 1.0, 1.0, 1.0, jitter=0.05, seed=1), spaceOrder=1)
+    nh = 4 * 3 * 3
+    nbt = 2 * 2 * (3 * 3 + 4 * 3 + 4 * 3)
+    assert t.nBFaces == nbt
+    assert t.nFaces == (20 * nh - nbt) // 2
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+def test_face_list_equals_brute_force_first_insertion(kind, oracle_mod):
+    mesh = (mg.hex_box(4, 3, 2, 1.0, 1.0, 1.0, jitter=0.1, seed=4) if kind == "hex"
+            else mg.tet_box(3, 2, 2, 1.0, 1.0, 1.0, jitter=0.1, seed=4))
+    table = mg.F2V_HEX if kind == "hex" else mg.F2V_TET
+    seen, order = {}, []
+    for c, verts in enumerate(mesh["c2v"]):
+        for lf, f in enumerate(table):
+            key = tuple(sorted(verts[f]))
+            if key in seen:
+                seen[key][1] = c
+            else:
+                seen[key] = [c, -1, lf]
+                order.append(key)
+    expect = [(seen[k][0], seen[k][1]) for k in order if seen[k][1] >= 0]
+    f2c, f2v, fn, fc = oracle_mod.Region(mesh, spaceOrder=1).faces()
+    assert [tuple(x) for x in f2c] == expect
+    # the face normal points out of c1
+    o = oracle_mod.Region(mesh, spaceOrder=1)
+    vol, cofg = o.cells()
+    d = fc - cofg[f2c[:, 0]]
+    assert np.all(np.einsum("ij,ij->i", d, fn[:, :3]) > 0)
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+def test_geometry_volume_closure_and_centroids(kind, oracle_mod):
+    if kind == "hex":
+        mesh = mg.hex_box(6, 5, 4, 0.6, 0.5, 0.4, jitter=0.15, seed=9)
+    else:
+        mesh = mg.tet_box(5, 4, 4, 0.5, 0.4, 0.4, jitter=0.1, seed=9)
+    o = oracle_mod.Region(mesh, spaceOrder=1)
+    vol, cofg = o.cells()
+    assert np.all(vol > 0)
+    box = 0.6 * 0.5 * 0.4 if kind == "hex" else 0.5 * 0.4 * 0.4
+    assert abs(vol.sum() - box) < 1e-13
+    f2c, _, fn, fc = o.faces()
+    bf2c, bfn, bfc = o.bfaces()
+    S = np.zeros((o.nCells, 3))
+    np.add.at(S, f2c[:, 0], fn[:, :3] * fn[:, 3:])
+    np.add.at(S, f2c[:, 1], -fn[:, :3] * fn[:, 3:])
+    np.add.at(S, bf2c, bfn[:, :3] * bfn[:, 3:])
+    assert np.abs(S).max() < 1e-15
+    # divergence theorem: the volume-weighted centroids sum to the box's first
+    # moment -- exactly for planar (triangular) faces, to the face-warping
+    # error for jittered (non-planar) quads
+    Lx = 0.6 if kind == "hex" else 0.5
+    V = vol.sum()
+    tol = 1e-13 if kind == "tet" else 1e-5
+    assert abs((vol * cofg[:, 0]).sum() - V * Lx / 2) < tol
+
+
+def test_unjittered_hex_geometry_closed_form(oracle_mod):
+    h = 0.25
+    o = oracle_mod.Region(mg.hex_box(4, 4, 4, 1.0, 1.0, 1.0), spaceOrder=1)
+    vol, cofg = o.cells()
+    assert np.allclose(vol, h ** 3, rtol=1e-14, atol=0)
+    ii = np.arange(64)
+    centre = np.stack([(ii % 4 + 0.5) * h, ((ii // 4) % 4 + 0.5) * h, (ii // 16 + 0.5) * h], axis=1)
+    assert np.abs(cofg - centre).max() < 1e-15
+
+
+def test_stencils_are_vertex_neighbours(oracle_mod):
+    """Interior hexes: all 26 vertex neighbours; tets: the 40 highest-sharing
+    vertex neighbours (every one when there are at most 40)."""
+    o = oracle_mod.Region(mg.hex_box(5, 5, 5, 1.0, 1.0, 1.0), spaceOrder=2, reconst=11)
+    off, mem, moms = o.stencils()
+    sizes = np.diff(off)
+    interior = [((i % 5) in (1, 2, 3)) and ((i // 5 % 5) in (1, 2, 3)) and ((i // 25) in (1, 2, 3))
+                for i in range(125)]
+    assert np.all(sizes[np.array(interior)] == 26)
+    assert sizes.min() == 7  # corner cells
+    mesh = mg.tet_box(4, 4, 4, 1.0, 1.0, 1.0, jitter=0.05, seed=2)
+    t = oracle_mod.Region(mesh, spaceOrder=2, reconst=11)
+    off, mem, _ = t.stencils()
+    c2v = mesh["c2v"]
+    v2c = {}
+    for c, vs in enumerate(c2v):
+        for v in vs:
+            v2c.setdefault(v, set()).add(c)
+    for c in range(len(c2v)):
+        cand = set().union(*(v2c[v] for v in c2v[c])) - {c}
+        share = {k: len(set(c2v[c]) & set(c2v[k])) for k in cand}
+        got = list(mem[off[c]:off[c + 1]])
+        assert got == sorted(got)
+        assert set(got) <= cand
+        if len(cand) <= 40:
+            assert set(got) == cand
+        else:
+            kth = sorted(share.values(), reverse=True)[39]
+            assert len(got) == 40
+            assert all(share[k] >= kth for k in got)
+            assert {k for k in cand if share[k] > kth} <= set(got)
+
+
+def test_moments_are_cholesky_of_weighted_normal_matrix(oracle_mod):
+    mesh = mg.tet_box(3, 3, 3, 1.0, 1.0, 1.0, jitter=0.1, seed=6)
+    o = oracle_mod.Region(mesh, spaceOrder=2, reconst=11)
+    off, mem, moms = o.stencils()
+    vol, cofg = o.cells()
+    for c in range(0, o.nCells, 7):
+        d = cofg[mem[off[c]:off[c + 1]]] - cofg[c]
+        wt = 1.0 / np.linalg.norm(d, axis=1)
+        A = np.column_stack([d * wt[:, None], wt])  # columns x, y, z, 1 (weighted)
+        M = A.T @ A
+        r11, r12, r22, r13, r23, r33, r14, r24, r34, r44 = moms[c]
+        R = np.array([[r11, r12, r13, r14], [0, r22, r23, r24], [0, 0, r33, r34], [0, 0, 0, r44]])
+        assert np.allclose(R.T @ R, M, rtol=1e-12, atol=1e-12 * np.abs(M).max())
+
+
+def linear_state(cofg, a):
+    x, y, z = cofg.T
+    rho = a[0] + a[1] * x + a[2] * y + a[3] * z
+    u = a[4] + a[5] * x - a[6] * y + a[7] * z
+    v = a[8] - a[5] * x + a[6] * y
+    w = a[9] + a[7] * z
+    p = a[10] + a[11] * x + a[12] * y - a[13] * z
+    return np.stack([rho, u, v, w, p], axis=1)
+
+
+def test_lsq_gradient_exact_for_linear_fields(oracle_mod):
+    mesh = mg.tet_box(4, 4, 4, 1.0, 1.0, 1.0, jitter=0.1, seed=8)
+    o = oracle_mod.Region(mesh, spaceOrder=2, reconst=0, refCp=CP, refGamma=GAMMA)
+    vol, cofg = o.cells()
+    a = np.array([1.2, 0.1, -0.2, 0.05, 10.0, 3.0, 2.0, -1.0, -5.0, 2.0, 1.0e5, 2.0e3, -1.0e3, 5.0e2])
+    P = linear_state(cofg, a)
+    o.set_cv(mg.prim_to_cons(*P.T, GAMMA).T.copy())
+    o.residual()
+    g = o.grad()
+    expect = np.array([[a[1], a[2], a[3]], [a[5], -a[6], a[7]], [-a[5], a[6], 0.0], [0.0, 0.0, a[7]],
+                       [a[11], a[12], -a[13]]])
+    for k in range(5):
+        scale = np.abs(expect[k]).max()
+        assert np.abs(g[:, k, :] - expect[k]).max() <= 1e-9 * scale, k
+
+
+def test_venkat_limiter_scales_gradients_into_unit_interval(oracle_mod):
+    mesh = mg.tet_box(4, 4, 4, 1.0, 1.0, 1.0, jitter=0.1, seed=8)
+    kw = dict(spaceOrder=2, refCp=CP, refGamma=GAMMA)
+    a0 = oracle_mod.Region(mesh, reconst=0, **kw)
+    a1 = oracle_mod.Region(mesh, reconst=11, **kw)
+    vol, cofg = a0.cells()
+    rng = np.random.default_rng(1)
+    P = np.stack([1.2 + 0.3 * rng.random(a0.nCells), 30 * rng.standard_normal(a0.nCells),
+                  30 * rng.standard_normal(a0.nCells), 30 * rng.standard_normal(a0.nCells),
+                  1e5 * (1 + 0.2 * rng.random(a0.nCells))], axis=1)
+    cv = mg.prim_to_cons(*P.T, GAMMA).T.copy()
+    a0.set_cv(cv)
+    a1.set_cv(cv)
+    a0.residual()
+    a1.residual()
+    g0, g1 = a0.grad(), a1.grad()
+    ratio = g1 / np.where(g0 == 0, 1, g0)
+    phi = ratio[..., 0]
+    assert np.all(phi > 0) and np.all(phi <= 1.0)
+    assert np.allclose(ratio[..., 1][g0[..., 1] != 0], phi[g0[..., 1] != 0], rtol=1e-12)
+    assert phi.min() < 0.5  # random data must actually be limited somewhere
+
+
+def test_timestep_closed_form_on_uniform_hexes(oracle_mod):
+    h = 0.1
+    o = oracle_mod.Region(mg.hex_box(5, 4, 3, 0.5, 0.4, 0.3), spaceOrder=1, refCp=CP, refGamma=GAMMA)
+    r, p = 1.2, 1.0e5
+    u = np.array([30.0, -20.0, 10.0])
+    cv = np.tile(mg.prim_to_cons(np.array([r]), *[np.array([x]) for x in u], np.array([p]), GAMMA).T, (o.nCells, 1))
+    o.set_cv(cv)
+    m, dt = o.timestep()
+    c = np.sqrt(GAMMA * p / r)
+    expect = h ** 3 / (h * h * (2 * (abs(u[0]) + c) + 2 * (abs(u[1]) + c) + 2 * (abs(u[2]) + c)))
+    assert np.allclose(dt, expect, rtol=1e-13, atol=0)
+    assert m == dt.min()
+
+
+def test_dt_rules_imposed_first_then_cfl(oracle_mod):
+    mesh, params, ic = configs.sod(nx=20)
+    o = oracle_mod.Region(mesh, **params)
+    vol, cofg = o.cells()
+    o.set_cv(ic(cofg))
+    o.step(1)
+    assert o.last_dt == 1.0 / (1.0 / params["dtImposed"])
+    dtl, _ = o.timestep()
+    o.step(1)
+    assert o.last_dt == 1.0 / (1.0 / (params["cfl"] * dtl))
+
+
+def test_closed_box_conserves_mass_and_energy(oracle_mod):
+    """All slip walls: mass and energy residuals sum to zero; momentum
+    residual sums to the wall pressure force."""
+    bc = [(mg.BC_SLIPWALL, 0, 0.0)] * 6
+    mesh = mg.tet_box(4, 4, 4, 1.0, 1.0, 1.0, jitter=0.1, seed=3, bc=bc)
+    for discr in (1, 3, 4):
+        o = oracle_mod.Region(mesh, spaceOrder=2, reconst=11, spaceDiscr=discr, refCp=CP, refGamma=GAMMA)
+        vol, cofg = o.cells()
+        rng = np.random.default_rng(2)
+        P = np.stack([1.2 + 0.1 * rng.random(o.nCells), 10 * rng.standard_normal(o.nCells),
+                      10 * rng.standard_normal(o.nCells), 10 * rng.standard_normal(o.nCells),
+                      1e5 * (1 + 0.1 * rng.random(o.nCells))], axis=1)
+        o.set_cv(mg.prim_to_cons(*P.T, GAMMA).T.copy())
+        rhs = o.residual()
+        tot = rhs.sum(axis=0)
+        assert abs(tot[0]) < 1e-12 * np.abs(rhs[:, 0]).max()
+        assert abs(tot[4]) < 1e-11 * np.abs(rhs[:, 4]).max()
+
+
+def exact_sod(x, t, x0=0.5, g=1.4):
+    """Exact Riemann solution of Sod's problem (density), Toro ch. 4."""
+    rl, pl, rr, pr = 1.0, 1.0, 0.125, 0.1
+    cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+
+    def f(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * ck / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+    lo, hi = 1e-8, 5.0
+    for _ in range(200):
+        pm = 0.5 * (lo + hi)
+        if f(pm, rl, pl, cl) + f(pm, rr, pr, cr) > 0:
+            hi = pm
+        else:
+            lo = pm
+    um = 0.5 * (f(pm, rr, pr, cr) - f(pm, rl, pl, cl))
+    rml = rl * (pm / pl) ** (1 / g)
+    q = (g - 1) / (g + 1)
+    rmr = rr * (pm / pr + q) / (q * pm / pr + 1)
+    S = cr * np.sqrt((g + 1) / (2 * g) * pm / pr + (g - 1) / (2 * g))
+    cml = cl * (pm / pl) ** ((g - 1) / (2 * g))
+    s = (x - x0) / t
+    fan = rl * (2 / (g + 1) - (g - 1) / ((g + 1) * cl) * s) ** (2 / (g - 1))
+    return np.where(s < -cl, rl, np.where(s < um - cml, fan, np.where(s < um, rml,
+                    np.where(s < S, rmr, rr))))
+
+
+def test_sod_first_order_ausm_approaches_exact_solution(oracle_mod):
+    mesh, params, ic = configs.sod()
+    o = oracle_mod.Region(mesh, **dict(params, dTimeSystem=0.15))
+    vol, cofg = o.cells()
+    o.set_cv(ic(cofg))
+    for _ in range(2000):
+        o.step(1)
+        if o.time >= 0.15 - 1e-15:
+            break
+    assert abs(o.time - 0.15) < 1e-12
+    rho = o.cv()[:, 0]
+    ex = exact_sod(cofg[:, 0], o.time)
+    l1 = np.sum(np.abs(rho - ex) * vol) / vol.sum()
+    assert l1 < 0.02, l1
+    assert np.isfinite(rho).all() and rho.min() > 0.1
