@@ -35,7 +35,7 @@ def algorithmic_bytes(nC, nF, nB, mbar, order):
     face geometry and connectivity, k_update the conserved state, volume and
     RK register.
     """
-    b = {"k_dt": 48.0 * nC + 40.0 * nF + 36.0 * nB}
+    b = {"k_dtmin": 48.0 * nC + 40.0 * nF + 36.0 * nB}
     if order == 2:
         b["k_grad"] = (24.0 + 48.0 + 4.0 * mbar + 4.0) * nC
         b["k_flux"] = 64.0 * nF + 60.0 * nB
@@ -43,8 +43,25 @@ def algorithmic_bytes(nC, nF, nB, mbar, order):
         b["k_grad"] = 0.0
         b["k_flux"] = 40.0 * nF + 36.0 * nB
     b["k_update"] = (40.0 + 40.0 + 8.0 + 160.0 / 3.0) * nC
-    b["step"] = b["k_dt"] + 3.0 * (b["k_grad"] + b["k_flux"] + b["k_update"])
+    b["step"] = b["k_dtmin"] + 3.0 * (b["k_grad"] + b["k_flux"] + b["k_update"])
     return b
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
+    calibration recorded there), or None when the summary lacks the kernel."""
+    path = os.path.join(ROOT, "profiles", "rflu_tets64_roe_kernels.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].startswith(kernel):
+            try:
+                return (float(cells[2]) + float(cells[4])) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def build_workload(name):
@@ -98,6 +115,8 @@ def main():
     ap.add_argument("--workload", default="tets64_roe")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time hipGraph replays (no per-kernel events)")
+    ap.add_argument("--copy-probe-mib", type=int, default=512,
+                    help="size of the plain-copy calibration kernel run after the timed region (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
@@ -131,7 +150,7 @@ def main():
 
     kern = {}
     if not args.graph:
-        for k in ("k_dt", "k_grad", "k_flux", "k_update"):
+        for k in ("k_dtmin", "k_grad", "k_flux", "k_update"):
             ms, n = reg.kernel_time(k)
             if n:
                 kern[k] = dict(ms=ms, launches=n, total_ms=ms * n)
@@ -139,9 +158,19 @@ def main():
     if kern:
         dom = max(kern, key=lambda k: kern[k]["total_ms"])
         ach = byt[dom] / (kern[dom]["ms"] * 1e-3) / 1e9
+        traffic = profiled_traffic("k_grad2" if dom == "k_grad" else dom) if args.workload == "tets64_roe" else None
         roof = dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None)
+                    frac=round(ach / HBM_PEAK_GBS, 4),
+                    traffic=None if traffic is None else int(traffic),
+                    algorithmic_bytes=int(byt[dom]), ms_per_launch=round(kern[dom]["ms"], 4))
     step_gbs = byt["step"] / (ms_step * 1e-3) / 1e9
+    copy_gbs = None
+    if args.copy_probe_mib > 0:
+        import ctypes
+        from rocflupicl_amd import _lib
+        g = ctypes.c_double()
+        _lib.check(_lib.load().rflu_gpu_bandwidth_probe(args.copy_probe_mib << 20, 5, ctypes.byref(g)))
+        copy_gbs = round(g.value, 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
     t_end, dt_end = reg.time()
     out = {
@@ -163,6 +192,7 @@ def main():
         "step_roofline": {"algorithmic_bytes_per_step": int(byt["step"]), "achieved_GBs": round(step_gbs, 1),
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels_ms": {k: round(v["ms"], 4) for k, v in kern.items()},
+        "copy_probe_GBs": copy_gbs,
         "cpu_baseline": cpu,
         "sim_time": t_end,
     }

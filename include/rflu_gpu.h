@@ -18,6 +18,7 @@
  */
 #ifndef RFLU_GPU_H
 #define RFLU_GPU_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -141,6 +142,9 @@ int rflu_gpu_sync(rflu_handle *h);
  * rflu_gpu_step call with profiling enabled; for measurement only. */
 int rflu_gpu_set_profiling(rflu_handle *h, int on);
 int rflu_gpu_kernel_time(rflu_handle *h, const char *name, double *msPerLaunch, int *launches);
+/* Streaming device-to-device copy of `bytes` (read + write counted), repeated
+ * `reps` times; reports GB/s.  A calibration aid for HBM traffic counters. */
+int rflu_gpu_bandwidth_probe(size_t bytes, int reps, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ RFLU_SYMBOLS = [
     "rflu_gpu_minimum_time_step", "rflu_gpu_rk_init", "rflu_gpu_cell_gradients",
     "rflu_gpu_convective_fluxes", "rflu_gpu_source_terms", "rflu_gpu_integrate_source_terms",
     "rflu_gpu_rk_update", "rflu_gpu_rk_stage", "rflu_gpu_step", "rflu_gpu_get_time",
-    "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time",
+    "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time", "rflu_gpu_bandwidth_probe",
 ]
 
 ERRORS = {1: "EINVAL", 2: "ENODEV", 3: "EHIP", 4: "EMESH", 5: "ENAN", 6: "ENEGATIVE"}
@@ -75,6 +75,7 @@ def load():
     L.rflu_gpu_get_time.argtypes = [P, P, P]
     L.rflu_gpu_set_profiling.argtypes = [P, I]
     L.rflu_gpu_kernel_time.argtypes = [P, C.c_char_p, P, P]
+    L.rflu_gpu_bandwidth_probe.argtypes = [C.c_size_t, I, P]
     for n in RFLU_SYMBOLS:
         if n != "rflu_gpu_last_error":
             getattr(L, n).restype = I

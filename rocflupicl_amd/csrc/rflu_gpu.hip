@@ -21,7 +21,10 @@
 // same expressions, instead of being stored.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -46,13 +49,13 @@ struct Gas {
 struct Dev {
   int nC = 0, nF = 0, nB = 0, S = 0, nBlk = 0;
   int *f2c = nullptr, *bf2c = nullptr, *bkind = nullptr, *cfOff = nullptr, *cfIdx = nullptr;
-  int *csOff = nullptr, *csMem = nullptr;
+  int *csOff = nullptr, *csMem = nullptr, *cfNInt = nullptr;
+  double *cfdx = nullptr, *cfdy = nullptr, *cfdz = nullptr;
   double *fnx = nullptr, *fny = nullptr, *fnz = nullptr, *fnm = nullptr;
   double *fcx = nullptr, *fcy = nullptr, *fcz = nullptr, *bpout = nullptr;
   double *vol = nullptr, *epsq = nullptr, *rm = nullptr;
   double *cv = nullptr, *rhsSum = nullptr, *rhs = nullptr, *rec = nullptr, *grad = nullptr;
-  double *flux = nullptr, *dtCell = nullptr, *dvOut = nullptr, *partial = nullptr, *scal = nullptr, *cofg = nullptr;
-  unsigned *counter = nullptr;
+  double *flux = nullptr, *srad = nullptr, *dtCell = nullptr, *dvOut = nullptr, *partial = nullptr, *scal = nullptr, *cofg = nullptr;
   int *err = nullptr;
 };
 
@@ -71,6 +74,9 @@ struct rflu_handle {
   hipStream_t stream = nullptr;
   std::string err;
   hipGraphExec_t graph = nullptr;
+  int gradVariant = 2;  // 2: trimmed one-thread-per-cell (default), 0: plain, 1: cooperative LDS
+  std::vector<int> perm, inv;  // device cell i holds reference cell perm[i]
+  std::vector<double> stage;   // host staging for (un)permuting cell arrays
   bool profiling = false;
   struct Prof { double ms = 0; int n = 0; };
   std::map<std::string, Prof> prof;
@@ -151,6 +157,19 @@ __device__ __forceinline__ void rec_u(int order, const double r[8], double &u, d
   }
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (hardware block b runs on
+// XCD b % 8).  Remap so each XCD walks one contiguous range of cells/faces:
+// neighbouring cells then share that XCD's L2 instead of being spread over
+// all eight.  Bijective on [0, gridDim.x) for any grid size.
+__device__ int g_xcdRemap = 1;
+__device__ __forceinline__ int xcd_block() {
+  if (!g_xcdRemap) return blockIdx.x;
+  const int nx = 8;
+  int nb = gridDim.x, b = blockIdx.x;
+  int xcd = b % nx, loc = b / nx, q = nb / nx, r = nb % nx;
+  return xcd * q + min(xcd, r) + loc;
+}
+
 // ---------------------------------------------------------------------------
 // Records from cv (RFLU_SetVarsWrapper at start-up)
 // ---------------------------------------------------------------------------
@@ -185,42 +204,54 @@ __device__ double minimum_time_step(const DtPar &P, double dtLocal, double t, do
   return dmin;
 }
 
-__global__ void __launch_bounds__(BLK) k_dt(Dev d, Gas G, int order, DtPar P, int storeCell) {
+// Spectral radius of one face (rocflu/RFLU_TimeStepInviscid.F90:165-197 for
+// interior faces, :264-285 for boundary faces; static grid, fs = 0).  The
+// expression is symmetric in the two cells, as in the reference.
+__device__ __forceinline__ double face_srad(const Gas &G, int order, const double L[8], const double *R,
+                                            double nx, double ny, double nz, double nm) {
+  double u1, v1, w1;
+  rec_u(order, L, u1, v1, w1);
+  double a1 = rec_c(G, order, L);
+  if (R) {
+    double u2, v2, w2;
+    rec_u(order, R, u2, v2, w2);
+    double a2 = rec_c(G, order, R);
+    double vc = 0.5 * ((u1 + u2) * nx + (v1 + v2) * ny + (w1 + w2) * nz);
+    double as = 0.5 * (a1 + a2);
+    return (fabs(vc - 0.0) + as) * nm;
+  }
+  double vc = u1 * nx + v1 * ny + w1 * nz;
+  return (fabs(vc - 0.0) + a1) * nm;
+}
+
+__global__ void __launch_bounds__(BLK) k_srad(Dev d, Gas G, int order) {
+  int s = xcd_block() * BLK + threadIdx.x;
+  if (s >= d.S) return;
+  double L[8], R[8];
+  if (s < d.nF) {
+    load_rec(d.rec, d.f2c[2 * s], L);
+    load_rec(d.rec, d.f2c[2 * s + 1], R);
+    d.srad[s] = face_srad(G, order, L, R, d.fnx[s], d.fny[s], d.fnz[s], d.fnm[s]);
+  } else {
+    load_rec(d.rec, d.bf2c[s - d.nF], L);
+    d.srad[s] = face_srad(G, order, L, nullptr, d.fnx[s], d.fny[s], d.fnz[s], d.fnm[s]);
+  }
+}
+
+// dt = vol / (sum of face spectral radii in face order) per cell, and each
+// workgroup's minimum.  The minimum over workgroups is taken by k_dtfinal in a
+// separate launch: the kernel boundary publishes the partial minima across
+// XCDs without a device-scope fence (an L2 write-back) in every workgroup.
+__global__ void __launch_bounds__(BLK) k_dtmin(Dev d, int storeCell) {
   __shared__ double sm[BLK / 64];
-  __shared__ bool amLast;
-  int i = blockIdx.x * BLK + threadIdx.x;
+  int i = xcd_block() * BLK + threadIdx.x;
   double dt = __builtin_inf();
   if (i < d.nC) {
-    double ri[8];
-    load_rec(d.rec, i, ri);
-    double ui, vi, wi;
-    rec_u(order, ri, ui, vi, wi);
-    double ai = rec_c(G, order, ri);
     double sum = 0.0;
-    for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
-      int code = d.cfIdx[e], s = code >> 1;
-      double nx = d.fnx[s], ny = d.fny[s], nz = d.fnz[s], nm = d.fnm[s];
-      double srad;
-      if (s < d.nF) {
-        int o = d.f2c[2 * s + ((code & 1) ? 0 : 1)];
-        double ro[8];
-        load_rec(d.rec, o, ro);
-        double uo, vo, wo;
-        rec_u(order, ro, uo, vo, wo);
-        double ao = rec_c(G, order, ro);
-        double vc = 0.5 * ((ui + uo) * nx + (vi + vo) * ny + (wi + wo) * nz);
-        double as = 0.5 * (ai + ao);
-        srad = (fabs(vc - 0.0) + as) * nm;
-      } else {
-        double vc = ui * nx + vi * ny + wi * nz;
-        srad = (fabs(vc - 0.0) + ai) * nm;
-      }
-      sum = sum + srad;
-    }
+    for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) sum = sum + d.srad[d.cfIdx[e] >> 1];
     dt = d.vol[i] / sum;
     if (storeCell) d.dtCell[i] = dt;
   }
-  // block minimum (wavefront shuffles, then LDS)
   for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) sm[wv] = dt;
@@ -229,26 +260,23 @@ __global__ void __launch_bounds__(BLK) k_dt(Dev d, Gas G, int order, DtPar P, in
     double m = sm[0];
     for (int q = 1; q < BLK / 64; q++) m = fmin(m, sm[q]);
     d.partial[blockIdx.x] = m;
-    __threadfence();
-    unsigned prev = atomicAdd(d.counter, 1u);
-    amLast = (prev == gridDim.x - 1);
   }
+}
+
+// MINVAL over the workgroup minima, then RFLU_MinimumTimeStep (one workgroup).
+__global__ void __launch_bounds__(1024) k_dtfinal(Dev d, DtPar P, int nPart) {
+  __shared__ double sm[1024 / 64];
+  double m = __builtin_inf();
+  for (int q = threadIdx.x; q < nPart; q += 1024) m = fmin(m, d.partial[q]);
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sm[wv] = m;
   __syncthreads();
-  if (amLast) {
-    __threadfence();
-    double m = __builtin_inf();
-    const volatile double *pv = d.partial;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += BLK) m = fmin(m, pv[q]);
-    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
-    if (lane == 0) sm[wv] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double mm = sm[0];
-      for (int q = 1; q < BLK / 64; q++) mm = fmin(mm, sm[q]);
-      d.scal[S_DTLOCAL] = mm;
-      if (P.finalize) d.scal[S_DTMIN] = minimum_time_step(P, mm, d.scal[S_TIME], d.scal[S_TSR]);
-      *d.counter = 0u;  // ready for the next launch (no memset node in graphs)
-    }
+  if (threadIdx.x == 0) {
+    double mm = sm[0];
+    for (int q = 1; q < 1024 / 64; q++) mm = fmin(mm, sm[q]);
+    d.scal[S_DTLOCAL] = mm;
+    if (P.finalize) d.scal[S_DTMIN] = minimum_time_step(P, mm, d.scal[S_TIME], d.scal[S_TSR]);
   }
 }
 
@@ -271,7 +299,7 @@ __global__ void k_advance_time(Dev d) {
 // (min/max over interior-face neighbours, one limiter term per interior face).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat, int guard) {
-  int i = blockIdx.x * BLK + threadIdx.x;
+  int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
   double ri[8];
@@ -288,7 +316,7 @@ __global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat, int guard) {
   for (int q = 0; q < 15; q++) g[q] = 0.0;
   int sb = d.csOff[i], se = d.csOff[i + 1];
   for (int s = sb; s < se; s++) {
-    int j = d.csMem[s];
+    int j = (int)((unsigned)d.csMem[s] & 0x7fffffffu);
     double rj[8];
     load_rec(d.rec, j, rj);
     double dx = rj[5] - ri[5], dy = rj[6] - ri[6], dz = rj[7] - ri[7];
@@ -354,6 +382,335 @@ __global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat, int guard) {
   double *o = d.grad + 15 * (size_t)i;
 #pragma unroll
   for (int q = 0; q < 15; q++) o[q] = g[q];
+}
+
+// Latency-trimmed variant (default).  Static geometry is prepared at set-up:
+// stencil entries carry a flag bit (CS_FACE) when the member is also an
+// interior-face neighbour, so the limiter's min/max comes out of the member
+// loop with no second gather; and each interior cell-face entry stores
+// cofg(cell) - fc(face), the same difference the reference forms, so the
+// limiter's face loop reads one contiguous stream.  The next member's record is
+// fetched while the current one is processed.
+#define CS_FACE 0x80000000u
+#define CS_IDX 0x7fffffffu
+template <int MINW>
+__global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guard) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double ri[8];
+  load_rec(d.rec, i, ri);
+  double r11 = d.rm[i], r12 = d.rm[N + i], r22 = d.rm[2 * N + i];
+  double r13 = d.rm[3 * N + i], r23 = d.rm[4 * N + i], r33 = d.rm[5 * N + i];
+  double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+  double c12 = -(c11 * r12);
+  double c13 = -(c11 * r13 + c12 * c22 * r23);
+  double c23 = -(c22 * r23);
+  double c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
+  double g[15], vmin[5], vmax[5];
+#pragma unroll
+  for (int q = 0; q < 15; q++) g[q] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; k++) { vmin[k] = ri[k]; vmax[k] = ri[k]; }
+  const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
+  int sb = d.csOff[i], se = d.csOff[i + 1];
+  unsigned jn = mem[sb];
+  double rn[8];
+  load_rec(d.rec, (int)(jn & CS_IDX), rn);
+  for (int s = sb; s < se; s++) {
+    double rj[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) rj[q] = rn[q];
+    unsigned j = jn;
+    if (s + 1 < se) {
+      jn = mem[s + 1];
+      load_rec(d.rec, (int)(jn & CS_IDX), rn);
+    }
+    double dx = rj[5] - ri[5], dy = rj[6] - ri[6], dz = rj[7] - ri[7];
+    double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    dx = term * dx; dy = term * dy; dz = term * dz;
+    double term1 = c1111 * (dx);
+    double term2 = c2222 * (dy + c12 * dx);
+    double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+    double wtx = term * (term1 + c12 * term2 + c13 * term3);
+    double wty = term * (term2 + c23 * term3);
+    double wtz = term * (term3);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      double dVar = rj[k] - ri[k];
+      g[3 * k] = g[3 * k] + wtx * dVar;
+      g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
+      g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    }
+    if (j & CS_FACE) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rj[k]); vmax[k] = fmax(vmax[k], rj[k]); }
+    }
+  }
+  if (venkat) {
+    const double TINYP = 1.0e-12, TINYN = guard ? -1.0e-12 : 1.0e-12;
+    double lim[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) lim[k] = 1.0;
+    double epsq = d.epsq[i];
+    int eb = d.cfOff[i], ee = eb + d.cfNInt[i];
+    for (int e = eb; e < ee; e++) {
+      double dx1 = d.cfdx[e], dy1 = d.cfdy[e], dz1 = d.cfdz[e];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        double d21 = g[3 * k] * dx1 + g[3 * k + 1] * dy1 + g[3 * k + 2] * dz1;
+        if (d21 > 0.0) {
+          double d1 = vmax[k] - ri[k];
+          double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
+          double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
+          lim[k] = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim[k]);
+        } else if (d21 < 0.0) {
+          double d1 = vmin[k] - ri[k];
+          double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
+          double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
+          lim[k] = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      g[3 * k] = lim[k] * g[3 * k];
+      g[3 * k + 1] = lim[k] * g[3 * k + 1];
+      g[3 * k + 2] = lim[k] * g[3 * k + 2];
+    }
+  }
+  double *o = d.grad + 15 * (size_t)i;
+#pragma unroll
+  for (int q = 0; q < 15; q++) o[q] = g[q];
+}
+
+// Cooperative variant 3: a workgroup of 256 lanes takes 16 cells.
+//  A: 16 lanes per cell fetch that cell's stencil members in parallel and
+//     stage each member's weights and variable differences in LDS (SoA, so
+//     consecutive lanes hit consecutive words); face-neighbour members also
+//     feed the limiter's min/max, reduced across the 16 lanes (exact).
+//  B: 15 lanes per cell, one per (variable, direction), accumulate over the
+//     members in stencil order -- the reference's sequence of roundings.
+//  C: 5 lanes per cell apply the limiter from the staged per-face geometry.
+#define G3C 16
+#ifndef GMAXM
+#define GMAXM 40
+#endif
+__global__ void __launch_bounds__(256) k_grad3(Dev d, int venkat, int guard) {
+  constexpr int NS = G3C * GMAXM + 1;  // +1 staggers the 8 arrays across LDS banks
+  __shared__ double sA[8][NS];        // wtx, wty, wtz, dVar[5]
+  __shared__ double sOwn[G3C][8];
+  __shared__ double sC[G3C][6];
+  __shared__ double sMin[G3C][5], sMax[G3C][5];
+  __shared__ double sG[G3C][15];
+  __shared__ int sOff[G3C + 1];
+  const int c0 = xcd_block() * G3C;
+  const int nCell = min(G3C, d.nC - c0);
+  const int tid = threadIdx.x;
+  const int N = d.nC;
+  if (tid < nCell) {
+    int i = c0 + tid;
+    double r[8];
+    load_rec(d.rec, i, r);
+#pragma unroll
+    for (int q = 0; q < 8; q++) sOwn[tid][q] = r[q];
+    double r11 = d.rm[i], r12 = d.rm[N + i], r22 = d.rm[2 * N + i];
+    double r13 = d.rm[3 * N + i], r23 = d.rm[4 * N + i], r33 = d.rm[5 * N + i];
+    double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+    double c12 = -(c11 * r12);
+    double c13 = -(c11 * r13 + c12 * c22 * r23);
+    double c23 = -(c22 * r23);
+    sC[tid][0] = c11 * c11; sC[tid][1] = c22 * c22; sC[tid][2] = c33 * c33;
+    sC[tid][3] = c12; sC[tid][4] = c13; sC[tid][5] = c23;
+  }
+  if (tid <= nCell) sOff[tid] = d.csOff[c0 + tid] - d.csOff[c0];
+  __syncthreads();
+  const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem) + d.csOff[c0];
+  {
+    const int ci = tid >> 4, l = tid & 15;
+    double vmn[5], vmx[5];
+    const bool live = ci < nCell;
+#pragma unroll
+    for (int k = 0; k < 5; k++) { vmn[k] = live ? sOwn[ci][k] : 0.0; vmx[k] = vmn[k]; }
+    if (live) {
+      const double xi = sOwn[ci][5], yi = sOwn[ci][6], zi = sOwn[ci][7];
+      const double c1111 = sC[ci][0], c2222 = sC[ci][1], c3333 = sC[ci][2];
+      const double c12 = sC[ci][3], c13 = sC[ci][4], c23 = sC[ci][5];
+      const int qb = sOff[ci], qe = sOff[ci + 1];
+      for (int q = qb + l; q < qe; q += 16) {
+        unsigned j = mem[q];
+        double rj[8];
+        load_rec(d.rec, (int)(j & CS_IDX), rj);
+        double dx = rj[5] - xi, dy = rj[6] - yi, dz = rj[7] - zi;
+        double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+        dx = term * dx; dy = term * dy; dz = term * dz;
+        double term1 = c1111 * (dx);
+        double term2 = c2222 * (dy + c12 * dx);
+        double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+        sA[0][q] = term * (term1 + c12 * term2 + c13 * term3);
+        sA[1][q] = term * (term2 + c23 * term3);
+        sA[2][q] = term * (term3);
+#pragma unroll
+        for (int k = 0; k < 5; k++) sA[3 + k][q] = rj[k] - sOwn[ci][k];
+        if (j & CS_FACE) {
+#pragma unroll
+          for (int k = 0; k < 5; k++) { vmn[k] = fmin(vmn[k], rj[k]); vmx[k] = fmax(vmx[k], rj[k]); }
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1)
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        vmn[k] = fmin(vmn[k], __shfl_xor(vmn[k], off, 16));
+        vmx[k] = fmax(vmx[k], __shfl_xor(vmx[k], off, 16));
+      }
+    if (live && l == 0)
+#pragma unroll
+      for (int k = 0; k < 5; k++) { sMin[ci][k] = vmn[k]; sMax[ci][k] = vmx[k]; }
+  }
+  __syncthreads();
+  if (tid < nCell * 15) {
+    int ci = tid / 15, comp = tid - 15 * ci, k = comp / 3, dd = comp - 3 * k;
+    const double *w = sA[dd], *v = sA[3 + k];
+    double g = 0.0;
+#pragma unroll 4
+    for (int q = sOff[ci]; q < sOff[ci + 1]; q++) g = g + w[q] * v[q];
+    sG[ci][comp] = g;
+  }
+  __syncthreads();
+  if (tid < nCell * 5) {
+    int ci = tid / 5, k = tid - 5 * ci, i = c0 + ci;
+    double gx = sG[ci][3 * k], gy = sG[ci][3 * k + 1], gz = sG[ci][3 * k + 2];
+    if (venkat) {
+      const double TINYP = 1.0e-12, TINYN = guard ? -1.0e-12 : 1.0e-12;
+      double vi = sOwn[ci][k], lim = 1.0;
+      double dmax = sMax[ci][k] - vi, dmin = sMin[ci][k] - vi;
+      double epsq = d.epsq[i];
+      int eb = d.cfOff[i], ee = eb + d.cfNInt[i];
+      for (int e = eb; e < ee; e++) {
+        double d21 = gx * d.cfdx[e] + gy * d.cfdy[e] + gz * d.cfdz[e];
+        if (d21 > 0.0) {
+          double numer = (dmax * dmax + epsq) * d21 + 2.0 * d21 * d21 * dmax;
+          double denom = d21 * (dmax * dmax + 2.0 * d21 * d21 + dmax * d21 + epsq);
+          lim = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim);
+        } else if (d21 < 0.0) {
+          double numer = (dmin * dmin + epsq) * d21 + 2.0 * d21 * d21 * dmin;
+          double denom = d21 * (dmin * dmin + 2.0 * d21 * d21 + dmin * d21 + epsq);
+          lim = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim);
+        }
+      }
+      gx = lim * gx; gy = lim * gy; gz = lim * gz;
+    }
+    double *o = d.grad + 15 * (size_t)i + 3 * k;
+    o[0] = gx; o[1] = gy; o[2] = gz;
+  }
+}
+
+// Cooperative variant: one workgroup takes GCPB consecutive cells.  All
+// stencil members of those cells are gathered at once (member-parallel), each
+// member's LSQ weights and variable differences are staged in LDS, and then
+// 15 lanes per cell each run one (variable, direction) accumulation over the
+// members in stencil order -- the same sequence of roundings per accumulator
+// as the reference's member loop.  The limiter then runs with 5 lanes per
+// cell (one per variable; min/max and the per-face minimum are exact, so their
+// order is free).
+#define GCPB 16
+__global__ void __launch_bounds__(BLK) k_grad_lds(Dev d, int venkat, int guard) {
+  __shared__ double sW[GCPB * GMAXM][8];  // wtx, wty, wtz, dVar[5]
+  __shared__ double sOwn[GCPB][8];        // own record
+  __shared__ double sC[GCPB][6];          // c1111, c2222, c3333, c12, c13, c23
+  __shared__ double sG[GCPB][15];
+  __shared__ int sOff[GCPB + 1];
+  const int c0 = xcd_block() * GCPB;
+  const int nCell = min(GCPB, d.nC - c0);
+  const int tid = threadIdx.x;
+  const int N = d.nC;
+  if (tid < nCell) {
+    int i = c0 + tid;
+    double r[8];
+    load_rec(d.rec, i, r);
+#pragma unroll
+    for (int q = 0; q < 8; q++) sOwn[tid][q] = r[q];
+    double r11 = d.rm[i], r12 = d.rm[N + i], r22 = d.rm[2 * N + i];
+    double r13 = d.rm[3 * N + i], r23 = d.rm[4 * N + i], r33 = d.rm[5 * N + i];
+    double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+    double c12 = -(c11 * r12);
+    double c13 = -(c11 * r13 + c12 * c22 * r23);
+    double c23 = -(c22 * r23);
+    sC[tid][0] = c11 * c11; sC[tid][1] = c22 * c22; sC[tid][2] = c33 * c33;
+    sC[tid][3] = c12; sC[tid][4] = c13; sC[tid][5] = c23;
+  }
+  if (tid <= nCell) sOff[tid] = d.csOff[c0 + tid] - d.csOff[c0];
+  __syncthreads();
+  const int base = d.csOff[c0];
+  const int nSlot = sOff[nCell];
+  for (int q = tid; q < nSlot; q += BLK) {
+    int ci = 0;
+    while (q >= sOff[ci + 1]) ci++;
+    int j = (int)((unsigned)d.csMem[base + q] & 0x7fffffffu);
+    double rj[8];
+    load_rec(d.rec, j, rj);
+    double dx = rj[5] - sOwn[ci][5], dy = rj[6] - sOwn[ci][6], dz = rj[7] - sOwn[ci][7];
+    double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    dx = term * dx; dy = term * dy; dz = term * dz;
+    double c1111 = sC[ci][0], c2222 = sC[ci][1], c3333 = sC[ci][2];
+    double c12 = sC[ci][3], c13 = sC[ci][4], c23 = sC[ci][5];
+    double term1 = c1111 * (dx);
+    double term2 = c2222 * (dy + c12 * dx);
+    double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+    sW[q][0] = term * (term1 + c12 * term2 + c13 * term3);
+    sW[q][1] = term * (term2 + c23 * term3);
+    sW[q][2] = term * (term3);
+#pragma unroll
+    for (int k = 0; k < 5; k++) sW[q][3 + k] = rj[k] - sOwn[ci][k];
+  }
+  __syncthreads();
+  if (tid < nCell * 15) {
+    int ci = tid / 15, comp = tid - 15 * ci, k = comp / 3, dd = comp - 3 * k;
+    double g = 0.0;
+    for (int q = sOff[ci]; q < sOff[ci + 1]; q++) g = g + sW[q][dd] * sW[q][3 + k];
+    sG[ci][comp] = g;
+  }
+  __syncthreads();
+  if (tid < nCell * 5) {
+    int ci = tid / 5, k = tid - 5 * ci, i = c0 + ci;
+    double gx = sG[ci][3 * k], gy = sG[ci][3 * k + 1], gz = sG[ci][3 * k + 2];
+    if (venkat) {
+      const double TINYP = 1.0e-12, TINYN = guard ? -1.0e-12 : 1.0e-12;
+      double vi = sOwn[ci][k], vmin = vi, vmax = vi, lim = 1.0;
+      int eb = d.cfOff[i], ee = d.cfOff[i + 1];
+      for (int e = eb; e < ee; e++) {
+        int code = d.cfIdx[e], s = code >> 1;
+        if (s >= d.nF) break;
+        int o = d.f2c[2 * s + ((code & 1) ? 0 : 1)];
+        double vo = d.rec[8 * (size_t)o + k];
+        vmin = fmin(vmin, vo);
+        vmax = fmax(vmax, vo);
+      }
+      double epsq = d.epsq[i];
+      for (int e = eb; e < ee; e++) {
+        int s = d.cfIdx[e] >> 1;
+        if (s >= d.nF) break;
+        double dx1 = sOwn[ci][5] - d.fcx[s], dy1 = sOwn[ci][6] - d.fcy[s], dz1 = sOwn[ci][7] - d.fcz[s];
+        double d21 = gx * dx1 + gy * dy1 + gz * dz1;
+        if (d21 > 0.0) {
+          double d1 = vmax - vi;
+          double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
+          double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
+          lim = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim);
+        } else if (d21 < 0.0) {
+          double d1 = vmin - vi;
+          double numer = (d1 * d1 + epsq) * d21 + 2.0 * d21 * d21 * d1;
+          double denom = d21 * (d1 * d1 + 2.0 * d21 * d21 + d1 * d21 + epsq);
+          lim = fmin(numer / (denom + (denom >= 0.0 ? TINYP : TINYN)), lim);
+        }
+      }
+      gx = lim * gx; gy = lim * gy; gz = lim * gz;
+    }
+    double *o = d.grad + 15 * (size_t)i + 3 * k;
+    o[0] = gx; o[1] = gy; o[2] = gz;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -541,10 +898,10 @@ __device__ __forceinline__ void bcond_outflow(const Gas &G, int opt, double pout
   }
 }
 
-struct FluxPar { int order, discr, patchOrder; };
+struct FluxPar { int order, discr, patchOrder, doSrad; };
 
 __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
-  int s = blockIdx.x * BLK + threadIdx.x;
+  int s = xcd_block() * BLK + threadIdx.x;
   if (s >= d.S) return;
   const double fs = 0.0, ksg = 0.0;
   double nx = d.fnx[s], ny = d.fny[s], nz = d.fnz[s], nm = d.fnm[s];
@@ -554,6 +911,7 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
     double L[8], R[8];
     load_rec(d.rec, c1, L);
     load_rec(d.rec, c2, R);
+    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, R, nx, ny, nz, nm);
     if (P.order == 1) {
       double rl = L[0], irl = 1.0 / rl, ul = L[1] * irl, vl = L[2] * irl, wl = L[3] * irl;
       double pl = L[5], tl = L[6], al = L[7];
@@ -605,6 +963,7 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
     int c1 = d.bf2c[b], kind = d.bkind[b];
     double L[8];
     load_rec(d.rec, c1, L);
+    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, nullptr, nx, ny, nz, nm);
     double rl, rul, rvl, rwl, rel, pl;
     if (P.patchOrder == 1) {
       if (P.order == 1) {
@@ -680,7 +1039,7 @@ __device__ __forceinline__ void gravity(const Gas &G, int order, const double r[
 }
 
 __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
-  int i = blockIdx.x * BLK + threadIdx.x;
+  int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
   double rhs[5];
@@ -755,7 +1114,7 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
 
 // ConvectiveFluxesMP / SourceTermsMP as separate steps (rhs kept in HBM).
 __global__ void __launch_bounds__(BLK) k_gather(Dev d) {
-  int i = blockIdx.x * BLK + threadIdx.x;
+  int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
   double rhs[5];
   gather_rhs(d, i, rhs);
@@ -774,6 +1133,12 @@ __global__ void __launch_bounds__(BLK) k_source(Dev d, Gas G, int order) {
   gravity(G, order, r, d.vol[i], rhs);
 #pragma unroll
   for (int k = 0; k < 5; k++) d.rhs[k * N + i] = rhs[k];
+}
+
+// Plain streaming copy; used only to calibrate HBM traffic counters and to
+// measure the copy bandwidth the board actually sustains.
+__global__ void __launch_bounds__(BLK) k_copy_probe(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLK) dst[i] = src[i];
 }
 
 // dv for output
@@ -795,6 +1160,35 @@ __global__ void __launch_bounds__(BLK) k_dv_out(Dev d, Gas G, int order, double 
 // Host side
 // ---------------------------------------------------------------------------
 namespace {
+
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1fffffull;
+  v = (v | v << 32) & 0x1f00000000ffffull;
+  v = (v | v << 16) & 0x1f0000ff0000ffull;
+  v = (v | v << 8) & 0x100f00f00f00f00full;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+
+// Order of n points (xyz triples) along a 3-D Morton curve (ties by index).
+void morton_order(const double *xyz, int n, std::vector<int> &order) {
+  order.resize(n);
+  if (n == 0) return;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], xyz[3 * (size_t)i + k]); hi[k] = std::max(hi[k], xyz[3 * (size_t)i + k]); }
+  double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+  double sc = ext > 0 ? (double)((1 << 21) - 1) / ext : 0.0;
+  std::vector<std::pair<uint64_t, int>> key(n);
+  for (int i = 0; i < n; i++) {
+    uint64_t c[3];
+    for (int k = 0; k < 3; k++) c[k] = (uint64_t)((xyz[3 * (size_t)i + k] - lo[k]) * sc);
+    key[i] = {spread3(c[0]) | spread3(c[1]) << 1 | spread3(c[2]) << 2, i};
+  }
+  std::sort(key.begin(), key.end());
+  for (int i = 0; i < n; i++) order[i] = key[i].second;
+}
 
 int set_err(rflu_handle *h, int code, const std::string &m) {
   if (h) h->err = m; else g_err = m;
@@ -846,24 +1240,40 @@ DtPar dtpar(const rflu_handle *h, int finalize) {
   return DtPar{h->prm.cfl, h->prm.dtImposed, h->prm.dtMinLimit, h->prm.dTimeSystem, finalize};
 }
 
-int launch_dt(rflu_handle *h, int finalize, int storeCell) {
-  ProfScope ps(h, "k_dt");
-  hipLaunchKernelGGL(k_dt, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder,
-                     dtpar(h, finalize), storeCell);
+int launch_dtmin(rflu_handle *h, int finalize, int storeCell) {
+  {
+    ProfScope ps(h, "k_dtmin");
+    hipLaunchKernelGGL(k_dtmin, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, storeCell);
+  }
+  ProfScope ps(h, "k_dtfinal");
+  hipLaunchKernelGGL(k_dtfinal, dim3(1), dim3(1024), 0, h->stream, h->d, dtpar(h, finalize), h->d.nBlk);
   return 0;
 }
 
 int launch_grad(rflu_handle *h) {
   if (h->prm.spaceOrder < 2) return 0;
   ProfScope ps(h, "k_grad");
-  hipLaunchKernelGGL(k_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d,
-                     h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, h->prm.limiterGuard ? 1 : 0);
+  int venkat = h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, guard = h->prm.limiterGuard ? 1 : 0;
+  if (h->gradVariant == 2)
+    hipLaunchKernelGGL(k_grad2<1>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  else if (h->gradVariant == 24)
+    hipLaunchKernelGGL(k_grad2<4>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  else if (h->gradVariant == 25)
+    hipLaunchKernelGGL(k_grad2<5>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  else if (h->gradVariant == 26)
+    hipLaunchKernelGGL(k_grad2<6>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  else if (h->gradVariant == 3)
+    hipLaunchKernelGGL(k_grad3, dim3((h->d.nC + G3C - 1) / G3C), dim3(256), 0, h->stream, h->d, venkat, guard);
+  else if (h->gradVariant == 0)
+    hipLaunchKernelGGL(k_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  else
+    hipLaunchKernelGGL(k_grad_lds, dim3((h->d.nC + GCPB - 1) / GCPB), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   return 0;
 }
 
-int launch_flux(rflu_handle *h) {
+int launch_flux(rflu_handle *h, int doSrad) {
   ProfScope ps(h, "k_flux");
-  FluxPar P{h->prm.spaceOrder, h->prm.spaceDiscr, h->prm.patchOrder};
+  FluxPar P{h->prm.spaceOrder, h->prm.spaceDiscr, h->prm.patchOrder, doSrad};
   hipLaunchKernelGGL(k_flux, dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, P);
   return 0;
 }
@@ -881,11 +1291,14 @@ int launch_update(rflu_handle *h, int istage, int fused) {
   return 0;
 }
 
+// One RFLU_TimeStepping iteration.  The stage-1 flux kernel reads exactly the
+// state RFLU_TimeStepInviscid sees, so it also writes the face spectral radii;
+// k_dtmin then forms dt and dtMin before the stage-1 update needs it.
 int enqueue_step(rflu_handle *h) {
-  launch_dt(h, 1, 0);
   for (int st = 1; st <= 3; st++) {
     launch_grad(h);
-    launch_flux(h);
+    launch_flux(h, st == 1);
+    if (st == 1) launch_dtmin(h, 1, 0);
     launch_update(h, st, 1);
   }
   hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, h->stream, h->d);
@@ -928,6 +1341,11 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   }
   rflu_handle *h = new rflu_handle();
   h->prm = *prm;
+  if (const char *gv = getenv("ROCFLUPICL_GRAD_VARIANT")) h->gradVariant = atoi(gv);
+  if (getenv("ROCFLUPICL_NO_XCD_REMAP")) {
+    int zero = 0;
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(g_xcdRemap), &zero, sizeof(int));
+  }
   rflu::MeshInput in;
   in.nVert = md->nVert; in.xyz = md->xyz; in.nvc = md->nvc; in.nCells = md->nCells; in.c2v = md->c2v;
   in.nPatches = md->nPatches; in.bcType = md->bcType; in.bcOpt = md->bcOpt; in.pout = md->pout;
@@ -946,39 +1364,118 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   d.nC = m.nCells; d.nF = m.nFaces; d.nB = m.nBFaces; d.S = m.nFaces + m.nBFaces; d.nBlk = nblk(d.nC);
   if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed"); }
   int rc = 0;
-  std::vector<int> bkind(m.nBFaces);
+  // Device numbering: cells and faces are renumbered along a Morton curve of
+  // their centroids so stencil and face neighbours sit close in HBM and L2.
+  // Only storage order changes: every cell keeps its face list and stencil
+  // list in the reference's order, so all sums round identically.
+  const int S = d.S, N = d.nC, F = d.nF;
+  morton_order(m.cofg.data(), N, h->perm);
+  if (getenv("ROCFLUPICL_NO_MORTON"))
+    for (int i = 0; i < N; i++) h->perm[i] = i;
+  h->inv.assign(N, 0);
+  for (int i = 0; i < N; i++) h->inv[h->perm[i]] = i;
+  std::vector<int> fperm, bperm;
+  morton_order(m.fc.data(), F, fperm);
+  morton_order(m.fc.data() + 3 * (size_t)F, m.nBFaces, bperm);
+  if (getenv("ROCFLUPICL_NO_MORTON")) {
+    for (int q = 0; q < F; q++) fperm[q] = q;
+    for (int q = 0; q < m.nBFaces; q++) bperm[q] = q;
+  }
+  for (int &x : bperm) x += F;
+  std::vector<int> slotNew2Old(fperm);
+  slotNew2Old.insert(slotNew2Old.end(), bperm.begin(), bperm.end());
+  std::vector<int> slotOld2New(S);
+  for (int q = 0; q < S; q++) slotOld2New[slotNew2Old[q]] = q;
+  std::vector<int> f2cD(2 * (size_t)F), bf2cD(m.nBFaces), bkind(m.nBFaces);
   std::vector<double> bpout(m.nBFaces);
+  for (int q = 0; q < F; q++) {
+    int o = slotNew2Old[q];
+    f2cD[2 * (size_t)q] = h->inv[m.f2c[2 * (size_t)o]];
+    f2cD[2 * (size_t)q + 1] = h->inv[m.f2c[2 * (size_t)o + 1]];
+  }
   for (int b = 0; b < m.nBFaces; b++) {
-    int p = m.bPatch[b];
+    int o = slotNew2Old[F + b] - F;
+    int p = m.bPatch[o];
+    bf2cD[b] = h->inv[m.bf2c[o]];
     bkind[b] = md->bcType[p] == RFLU_BC_SLIPWALL ? 0 : 1 + md->bcOpt[p];
     bpout[b] = md->pout[p];
   }
-  const int S = d.S, N = d.nC;
   std::vector<double> soa(7 * (size_t)S);
-  for (int s = 0; s < S; s++) {
-    for (int q = 0; q < 4; q++) soa[q * (size_t)S + s] = m.fn[4 * (size_t)s + q];
-    for (int q = 0; q < 3; q++) soa[(4 + q) * (size_t)S + s] = m.fc[3 * (size_t)s + q];
+  for (int q = 0; q < S; q++) {
+    int o = slotNew2Old[q];
+    for (int k = 0; k < 4; k++) soa[k * (size_t)S + q] = m.fn[4 * (size_t)o + k];
+    for (int k = 0; k < 3; k++) soa[(4 + k) * (size_t)S + q] = m.fc[3 * (size_t)o + k];
   }
-  std::vector<double> cofgSoa(3 * (size_t)N), rmSoa(6 * (size_t)N, 0.0);
-  for (int i = 0; i < N; i++)
-    for (int q = 0; q < 3; q++) cofgSoa[q * (size_t)N + i] = m.cofg[3 * (size_t)i + q];
-  if (prm->spaceOrder > 1)
-    for (int i = 0; i < N; i++)
-      for (int q = 0; q < 6; q++) rmSoa[q * (size_t)N + i] = m.moms[10 * (size_t)i + q];
+  std::vector<int> cfOffD(N + 1, 0), cfIdxD(m.cfIdx.size());
+  for (int i = 0; i < N; i++) {
+    int o = h->perm[i];
+    cfOffD[i + 1] = cfOffD[i] + (m.cfOff[o + 1] - m.cfOff[o]);
+    for (int e = m.cfOff[o], t = cfOffD[i]; e < m.cfOff[o + 1]; e++, t++)
+      cfIdxD[t] = (slotOld2New[m.cfIdx[e] >> 1] << 1) | (m.cfIdx[e] & 1);
+  }
+  std::vector<double> volD(N), epsqD(N), cofgSoa(3 * (size_t)N), rmSoa(6 * (size_t)N, 0.0);
+  for (int i = 0; i < N; i++) {
+    int o = h->perm[i];
+    volD[i] = m.vol[o];
+    epsqD[i] = m.epsq[o];
+    for (int k = 0; k < 3; k++) cofgSoa[k * (size_t)N + i] = m.cofg[3 * (size_t)o + k];
+    if (prm->spaceOrder > 1)
+      for (int k = 0; k < 6; k++) rmSoa[k * (size_t)N + i] = m.moms[10 * (size_t)o + k];
+  }
+  std::vector<int> csOffD, csMemD, cfNIntD(N, 0);
+  std::vector<double> cfd(3 * cfIdxD.size(), 0.0);
+  for (int i = 0; i < N; i++) {
+    int o = h->perm[i];
+    for (int e = m.cfOff[o], t = cfOffD[i]; e < m.cfOff[o + 1]; e++, t++) {
+      int slot = m.cfIdx[e] >> 1;
+      if (slot >= F) break;
+      cfNIntD[i]++;
+      for (int k = 0; k < 3; k++)  // cofg(cell) - fc(face), RFLU_ModLimiters.F90:443-448
+        cfd[k * cfIdxD.size() + t] = m.cofg[3 * (size_t)o + k] - m.fc[3 * (size_t)slot + k];
+    }
+  }
+  if (prm->spaceOrder > 1) {
+    csOffD.assign(N + 1, 0);
+    csMemD.resize(m.csMem.size());
+    std::vector<int> nb;
+    for (int i = 0; i < N; i++) {
+      int o = h->perm[i];
+      nb.clear();
+      for (int e = m.cfOff[o]; e < m.cfOff[o + 1]; e++) {
+        int slot = m.cfIdx[e] >> 1;
+        if (slot >= F) break;
+        nb.push_back(m.f2c[2 * (size_t)slot + ((m.cfIdx[e] & 1) ? 0 : 1)]);
+      }
+      int flagged = 0;
+      csOffD[i + 1] = csOffD[i] + (m.csOff[o + 1] - m.csOff[o]);
+      for (int e = m.csOff[o], t = csOffD[i]; e < m.csOff[o + 1]; e++, t++) {
+        unsigned v = (unsigned)h->inv[m.csMem[e]];
+        if (std::find(nb.begin(), nb.end(), m.csMem[e]) != nb.end()) { v |= 0x80000000u; flagged++; }
+        csMemD[t] = (int)v;
+      }
+      if (flagged != (int)nb.size()) {
+        std::string msg = "mesh: a face neighbour is missing from its cell's c2c stencil";
+        rflu_gpu_destroy(h);
+        return set_err(nullptr, RFLU_GPU_EMESH, msg);
+      }
+    }
+  }
   double *faceSoa = nullptr;
-  rc |= dalloc(h, &d.f2c, 2 * (size_t)d.nF, m.f2c.data());
-  rc |= dalloc(h, &d.bf2c, (size_t)d.nB, m.bf2c.data());
+  rc |= dalloc(h, &d.f2c, 2 * (size_t)d.nF, f2cD.data());
+  rc |= dalloc(h, &d.bf2c, (size_t)d.nB, bf2cD.data());
   rc |= dalloc(h, &d.bkind, (size_t)d.nB, bkind.data());
   rc |= dalloc(h, &d.bpout, (size_t)d.nB, bpout.data());
   rc |= dalloc(h, &faceSoa, 7 * (size_t)S, soa.data());
-  rc |= dalloc(h, &d.cfOff, (size_t)N + 1, m.cfOff.data());
-  rc |= dalloc(h, &d.cfIdx, m.cfIdx.size(), m.cfIdx.data());
-  rc |= dalloc(h, &d.vol, (size_t)N, m.vol.data());
-  rc |= dalloc(h, &d.epsq, (size_t)N, m.epsq.data());
+  rc |= dalloc(h, &d.cfOff, (size_t)N + 1, cfOffD.data());
+  rc |= dalloc(h, &d.cfIdx, cfIdxD.size(), cfIdxD.data());
+  rc |= dalloc(h, &d.vol, (size_t)N, volD.data());
+  rc |= dalloc(h, &d.epsq, (size_t)N, epsqD.data());
   rc |= dalloc(h, &d.cofg, 3 * (size_t)N, cofgSoa.data());
+  rc |= dalloc(h, &d.cfNInt, (size_t)N, cfNIntD.data());
+  rc |= dalloc(h, &d.cfdx, cfd.size(), cfd.data());
   if (prm->spaceOrder > 1) {
-    rc |= dalloc(h, &d.csOff, (size_t)N + 1, m.csOff.data());
-    rc |= dalloc(h, &d.csMem, m.csMem.size(), m.csMem.data());
+    rc |= dalloc(h, &d.csOff, (size_t)N + 1, csOffD.data());
+    rc |= dalloc(h, &d.csMem, csMemD.size(), csMemD.data());
     rc |= dalloc(h, &d.rm, 6 * (size_t)N, rmSoa.data());
     rc |= dalloc(h, &d.grad, 15 * (size_t)N);
   }
@@ -987,17 +1484,17 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   rc |= dalloc(h, &d.rhs, 5 * (size_t)N);
   rc |= dalloc(h, &d.rec, 8 * (size_t)N);
   rc |= dalloc(h, &d.flux, 5 * (size_t)S);
+  rc |= dalloc(h, &d.srad, (size_t)S);
   rc |= dalloc(h, &d.dtCell, (size_t)N);
   rc |= dalloc(h, &d.dvOut, 3 * (size_t)N);
   rc |= dalloc(h, &d.partial, (size_t)d.nBlk);
   rc |= dalloc(h, &d.scal, 8);
-  rc |= dalloc(h, &d.counter, 1);
   rc |= dalloc(h, &d.err, 1);
   if (rc) { std::string msg = h->err; rflu_gpu_destroy(h); return set_err(nullptr, RFLU_GPU_EHIP, msg); }
+  d.cfdy = d.cfdx + cfIdxD.size(); d.cfdz = d.cfdx + 2 * cfIdxD.size();
   d.fnx = faceSoa; d.fny = faceSoa + S; d.fnz = faceSoa + 2 * (size_t)S; d.fnm = faceSoa + 3 * (size_t)S;
   d.fcx = faceSoa + 4 * (size_t)S; d.fcy = faceSoa + 5 * (size_t)S; d.fcz = faceSoa + 6 * (size_t)S;
   hipMemset(d.scal, 0, 8 * sizeof(double));
-  hipMemset(d.counter, 0, sizeof(unsigned));
   hipMemset(d.err, 0, sizeof(int));
   hipMemset(d.rhsSum, 0, 5 * (size_t)N * sizeof(double));
   hipDeviceSynchronize();
@@ -1008,9 +1505,9 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
 int rflu_gpu_destroy(rflu_handle *h) {
   if (!h) return 0;
   Dev &d = h->d;
-  void *ptrs[] = {d.f2c, d.bf2c, d.bkind, d.bpout, d.fnx, d.cfOff, d.cfIdx, d.vol, d.epsq, d.cofg, d.csOff,
-                  d.csMem, d.rm, d.grad, d.cv, d.rhsSum, d.rhs, d.rec, d.flux, d.dtCell, d.dvOut, d.partial, d.scal,
-                  d.counter, d.err};
+  void *ptrs[] = {d.f2c, d.bf2c, d.bkind, d.bpout, d.fnx, d.cfOff, d.cfIdx, d.cfNInt, d.cfdx, d.vol, d.epsq, d.cofg, d.csOff,
+                  d.csMem, d.rm, d.grad, d.cv, d.rhsSum, d.rhs, d.rec, d.flux, d.srad, d.dtCell, d.dvOut, d.partial, d.scal,
+                  d.err};
   for (void *p : ptrs) if (p) hipFree(p);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (auto e : h->evPool) hipEventDestroy(e);
@@ -1049,9 +1546,34 @@ int rflu_gpu_get_stencils(const rflu_handle *h, int *off, int *memb, double *mom
   return 0;
 }
 
+// Host <-> device copies of SoA cell arrays with the cell renumbering applied.
+static int to_device_cells(rflu_handle *h, double *dst, const double *src, int nvar) {
+  size_t N = h->d.nC;
+  h->stage.resize(nvar * N);
+  for (int k = 0; k < nvar; k++)
+    for (size_t i = 0; i < N; i++) h->stage[k * N + i] = src[k * N + h->perm[i]];
+  HIPCHK(h, hipMemcpyAsync(dst, h->stage.data(), nvar * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+static int from_device_cells(rflu_handle *h, double *dst, const double *src, int nvar, bool aos) {
+  size_t N = h->d.nC;
+  h->stage.resize(nvar * N);
+  HIPCHK(h, hipMemcpyAsync(h->stage.data(), src, nvar * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < N; i++) {
+    size_t o = h->perm[i];
+    for (int k = 0; k < nvar; k++) {
+      if (aos) dst[o * nvar + k] = h->stage[i * nvar + k];
+      else dst[k * N + o] = h->stage[k * N + i];
+    }
+  }
+  return 0;
+}
+
 int rflu_gpu_set_state(rflu_handle *h, const double *cv) {
   if (!h || !cv) return RFLU_GPU_EINVAL;
-  HIPCHK(h, hipMemcpyAsync(h->d.cv, cv, 5 * (size_t)h->d.nC * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (int e = to_device_cells(h, h->d.cv, cv, 5)) return e;
   HIPCHK(h, hipMemsetAsync(h->d.err, 0, sizeof(int), h->stream));
   hipLaunchKernelGGL(k_records, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
   HIPCHK(h, hipGetLastError());
@@ -1061,36 +1583,36 @@ int rflu_gpu_set_state(rflu_handle *h, const double *cv) {
 
 int rflu_gpu_get_state(rflu_handle *h, double *cv, double *dv) {
   if (!h) return RFLU_GPU_EINVAL;
-  size_t N = h->d.nC;
-  if (cv) HIPCHK(h, hipMemcpyAsync(cv, h->d.cv, 5 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (cv)
+    if (int e = from_device_cells(h, cv, h->d.cv, 5, false)) return e;
   if (dv) {
     hipLaunchKernelGGL(k_dv_out, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder, h->d.dvOut);
-    HIPCHK(h, hipMemcpyAsync(dv, h->d.dvOut, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (int e = from_device_cells(h, dv, h->d.dvOut, 3, false)) return e;
   }
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 
 int rflu_gpu_get_rhs(rflu_handle *h, double *rhs) {
   if (!h || !rhs) return RFLU_GPU_EINVAL;
-  HIPCHK(h, hipMemcpyAsync(rhs, h->d.rhs, 5 * (size_t)h->d.nC * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return 0;
+  return from_device_cells(h, rhs, h->d.rhs, 5, false);
 }
 
 int rflu_gpu_get_grad(rflu_handle *h, double *grad) {
   if (!h || !grad) return RFLU_GPU_EINVAL;
   if (h->prm.spaceOrder < 2) return set_err(h, RFLU_GPU_EINVAL, "no gradients at spaceOrder 1");
-  HIPCHK(h, hipMemcpyAsync(grad, h->d.grad, 15 * (size_t)h->d.nC * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return 0;
+  return from_device_cells(h, grad, h->d.grad, 15, true);
 }
 
 int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLocal) {
   if (!h) return RFLU_GPU_EINVAL;
-  launch_dt(h, 0, dtCell ? 1 : 0);
+  {
+    ProfScope ps(h, "k_srad");
+    hipLaunchKernelGGL(k_srad, dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
+  }
+  launch_dtmin(h, 0, dtCell ? 1 : 0);
   HIPCHK(h, hipGetLastError());
-  if (dtCell) HIPCHK(h, hipMemcpyAsync(dtCell, h->d.dtCell, (size_t)h->d.nC * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (dtCell)
+    if (int e = from_device_cells(h, dtCell, h->d.dtCell, 1, false)) return e;
   if (dtMinLocal) HIPCHK(h, hipMemcpyAsync(dtMinLocal, h->d.scal + S_DTLOCAL, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -1119,7 +1641,7 @@ int rflu_gpu_cell_gradients(rflu_handle *h) {
 
 int rflu_gpu_convective_fluxes(rflu_handle *h) {
   if (!h) return RFLU_GPU_EINVAL;
-  launch_flux(h);
+  launch_flux(h, 0);
   {
     ProfScope ps(h, "k_gather");
     hipLaunchKernelGGL(k_gather, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
@@ -1147,7 +1669,7 @@ int rflu_gpu_rk_update(rflu_handle *h, int istage) {
 int rflu_gpu_rk_stage(rflu_handle *h, int istage) {
   if (!h || istage < 1 || istage > 3) return RFLU_GPU_EINVAL;
   launch_grad(h);
-  launch_flux(h);
+  launch_flux(h, 0);
   launch_update(h, istage, 1);
   HIPCHK(h, hipGetLastError());
   return check_err(h);
@@ -1187,6 +1709,34 @@ int rflu_gpu_get_time(rflu_handle *h, double *t, double *dtMin) {
 int rflu_gpu_sync(rflu_handle *h) {
   if (!h) return RFLU_GPU_EINVAL;
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_gpu_bandwidth_probe(size_t bytes, int reps, double *gbps) {
+  if (bytes < 16 || reps < 1 || !gbps) return set_err(nullptr, RFLU_GPU_EINVAL, "bad probe arguments");
+  size_t n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    return set_err(nullptr, RFLU_GPU_EHIP, "probe allocation failed");
+  }
+  (void)hipMemset(a, 0, n * 16);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int grid = 8192;
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(BLK), 0, 0, b, a, n);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(BLK), 0, 0, b, a, n);
+  (void)hipEventRecord(e1, 0);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *gbps = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }
 

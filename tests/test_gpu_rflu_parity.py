@@ -220,3 +220,20 @@ def test_sod_workload_matches_oracle_100_steps(oracle_mod):
     o.step(100)
     gcv, _ = g.get_state()
     assert same_bits(gcv, o.cv()), first_diff(gcv, o.cv())
+
+
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3"])
+def test_gradient_kernel_variants_match_oracle(variant, oracle_mod, monkeypatch):
+    """Both gradient kernels (one thread per cell; cooperative LDS) are
+    bit-identical to the oracle."""
+    monkeypatch.setenv("ROCFLUPICL_GRAD_VARIANT", variant)
+    mesh = MESHES["tet"]
+    prm = dict(spaceDiscr=3, spaceOrder=2, reconst=11, refCp=1004.64, refGamma=1.4, dtImposed=2e-8, cfl=0.8)
+    g, o, cofg = _pair(mesh, **prm)
+    cv = _shock_ic(cofg)
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.RFLU_TimeStepping(6)
+    o.step(6)
+    gcv, _ = g.get_state()
+    assert same_bits(gcv, o.cv()), first_diff(gcv, o.cv())
