@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every function its headers declare.
+
+CPU only: no compute call is made (there is no GPU here); the library is
+opened with ctypes and its dynamic symbol table is compared with the
+declarations parsed from include/*.h.
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+from rocflupicl_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def built_lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        from rocflupicl_amd import build
+        build.build()
+    return _lib.LIB_PATH
+
+
+def test_header_declarations_match_loader_list():
+    assert declared("rflu_gpu.h") == sorted(_lib.RFLU_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol(built_lib):
+    exported = _lib.exported_symbols()
+    missing = [s for s in declared("rflu_gpu.h") if s not in exported]
+    assert not missing, missing
+
+
+def test_library_opens_without_a_gpu(built_lib):
+    L = ctypes.CDLL(built_lib)
+    for s in declared("rflu_gpu.h"):
+        assert hasattr(L, s)
