@@ -25,6 +25,27 @@ class OrcParams(C.Structure):
                 ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int)]
 
 
+class OrcPiclParams(C.Structure):
+    _fields_ = [("rmuFlag", C.c_int), ("rmuRef", C.c_double), ("tref", C.c_double), ("suth", C.c_double),
+                ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
+                ("feedbackFlag", C.c_int)]
+
+
+PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
+                     amFlag=1, pgFlag=1, feedbackFlag=1)
+# rprop index (1-based, PPICLF_USER.h) of each interpolated field slot
+PICL_FIELD_RPROP = [2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39]
+
+
+def make_picl_params(**kw):
+    d = dict(PICL_DEFAULTS)
+    d.update(kw)
+    p = OrcPiclParams()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -64,6 +85,17 @@ def lib():
         L.orc_rk_stage.restype = I
         L.orc_set_dtmin.argtypes = [P, D]
         L.orc_flux_point.argtypes = [C.POINTER(OrcParams), I, P, P, P, P]
+        L.orc_picl_init.argtypes = [P, C.POINTER(OrcPiclParams), I, P, P]
+        L.orc_picl_init.restype = I
+        for n in ("orc_picl_npart", "orc_picl_nlost"):
+            getattr(L, n).argtypes = [P]
+            getattr(L, n).restype = I
+        L.orc_picl_get_particles.argtypes = [P, P, P, P, P, P]
+        L.orc_picl_get_rprop.argtypes = [P, P]
+        L.orc_picl_get_cells.argtypes = [P, P, P, P, P]
+        L.orc_picl_locate.argtypes = [P, I, P, P]
+        L.orc_picl_neighbors.argtypes = [P, P, P, P]
+        L.orc_picl_neighbors.restype = I
         _LIB = L
     return _LIB
 
@@ -189,6 +221,51 @@ class Region:
     def rk_stage(self, istage, dt):
         lib().orc_set_dtmin(self.h, dt)
         return lib().orc_rk_stage(self.h, istage)
+
+    # ---- ppiclF coupling ------------------------------------------------
+    def picl_init(self, y, pconst, **picl_params):
+        """PICL_TEMP_InitSolver: y (n,12) particle state, pconst (n,4) =
+        (rho_p, d_p, V_p, SPL).  cv must hold the pure-gas state."""
+        y = np.ascontiguousarray(y, np.float64)
+        pc = np.ascontiguousarray(pconst, np.float64)
+        self._picl_params = make_picl_params(**picl_params)
+        if lib().orc_picl_init(self.h, C.byref(self._picl_params), y.shape[0], _p(y), _p(pc)):
+            raise RuntimeError("oracle: PICL needs an all-hex second-order region")
+        self.nPart = y.shape[0]
+
+    def picl_particles(self):
+        n = self.nPart
+        y, y1, yd, yc = (np.zeros((n, 12)) for _ in range(4))
+        elem = np.zeros(n, np.int32)
+        lib().orc_picl_get_particles(self.h, _p(y), _p(y1), _p(yd), _p(yc), _p(elem))
+        return dict(y=y, y1=y1, ydot=yd, ydotc=yc, elem=elem)
+
+    def picl_rprop(self):
+        a = np.zeros((self.nPart, 48))
+        lib().orc_picl_get_rprop(self.h, _p(a))
+        return a
+
+    def picl_cells(self):
+        N = self.nCells
+        phi, g, pf, fld = np.zeros(N), np.zeros((N, 3)), np.zeros((N, 5)), np.zeros((N, 20))
+        lib().orc_picl_get_cells(self.h, _p(phi), _p(g), _p(pf), _p(fld))
+        return dict(phi=phi, gradRhog=g, proFld=pf, fields=fld)
+
+    def picl_nlost(self):
+        return lib().orc_picl_nlost(self.h)
+
+    def picl_locate(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros(xyz.shape[0], np.int32)
+        lib().orc_picl_locate(self.h, xyz.shape[0], _p(xyz), _p(out))
+        return out
+
+    def picl_neighbors(self, xp):
+        xp = np.ascontiguousarray(xp, np.float64)
+        ids = np.zeros(27, np.int32)
+        d2 = np.zeros(27)
+        n = lib().orc_picl_neighbors(self.h, _p(xp), _p(ids), _p(d2))
+        return ids[:n], d2[:n]
 
     @property
     def time(self):

@@ -1,0 +1,623 @@
+/*
+ * CPU oracle of the ppiclF coupling -- TEST INFRASTRUCTURE (picl_oracle.h).
+ *
+ * Only tests/ and bench.py's cpu_baseline leg load this, as the checker.
+ * One rank, one region, all-hex mesh.  Every expression keeps the Fortran
+ * evaluation order, and single-precision literals of the ppiclF user code are
+ * written as (double)<literal>f: the user files are compiled without
+ * -fdefault-real-8, so 0.15*re is (real*4 0.15 widened) * re.
+ * Integer powers follow gfortran's expansion: x**2 = x*x, x**3 = (x*x)*x,
+ * x**4 = (x*x)*(x*x).
+ *
+ * Not restated (the enabled models never read them): the feedback-force
+ * gradient and the vorticity / DOmega/Dt / p / div(tau) fields of
+ * PICL_TEMP_Runge (they feed only lift, torque, burn rate and the viscous
+ * models), and the 17 projected fields Rocflu does not read back.
+ */
+#include "rflu_oracle_int.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* PPICLF_USER.h indices, 0-based */
+enum { JX = 0, JY, JZ, JVX, JVY, JVZ, JT };
+enum {
+  R_JRHOP = 0, R_JRHOF = 1, R_JDP = 2, R_JVOLP = 3, R_JPHIP = 4, R_JUX = 5, R_JUY = 6, R_JUZ = 7,
+  R_JCS = 8, R_JDPDX = 9, R_JDPDY = 10, R_JDPDZ = 11, R_JSDRX = 12, R_JSDRY = 13, R_JSDRZ = 14,
+  R_JRHSR = 15, R_JPGCX = 16, R_JPGCY = 17, R_JPGCZ = 18, R_JSPL = 21, R_JT = 23,
+  R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
+};
+enum { P_JPHIP = 0, P_JFX, P_JFY, P_JFZ, P_JE, NPRO };
+#define LRS 12
+#define LRP 48
+
+const int orc_picl_field_rprop[ORC_PICL_NFLD] = {2, 6, 7, 8, 10, 11, 12, 9, 24, 5,
+                                                 13, 14, 15, 16, 17, 18, 19, 37, 38, 39};
+
+/* ppiclF node (i,j,k) of a Rocflu hex, n = i + 2j + 4k:
+ * hex2v 1,4,5,8,2,3,6,7 (rocpicl/PICL_TEMP_InitSolver.F90:562-597) */
+static const int NODE2V[8] = {0, 3, 4, 7, 1, 2, 5, 6};
+
+#define F32(x) ((double)(x##f))
+
+struct orc_picl {
+  orc_picl_params pp;
+  int n, ne;
+  double *y, *y1, *ydot, *ydotc, *rprop;
+  int *elem;
+  char *lost;
+  double *xm1b;    /* 24 per element: node n, coordinate d at [3n + d] */
+  double *centeri; /* LocalInterp element centroid */
+  double *bbox;    /* min xyz, max xyz */
+  double d2Max[3]; /* (1.5 * extent of the last element)^2 per axis */
+  double *rexp;    /* 1 / (12-tet hex volume) */
+  double *proFld;  /* NPRO per element, the value of all 8 nodes */
+  double *fld;     /* ORC_PICL_NFLD interpolation inputs per element */
+};
+
+void orc__picl_free(struct orc_picl *p) {
+  if (!p) return;
+  void *ptrs[] = {p->y, p->y1, p->ydot, p->ydotc, p->rprop, p->elem, p->lost, p->xm1b,
+                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld};
+  for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
+  free(p);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Overlap mesh                                                              */
+/* ------------------------------------------------------------------------ */
+/* Single-element box volume: ppiclf_solve.f:3285-3394 (12 tetrahedra from
+ * each face's two triangles to the element centroid). */
+static double hex_volume(const double *X) {
+  double centroid[3];
+  for (int ix = 0; ix < 3; ix++) centroid[ix] = 0.0;
+  for (int ix = 0; ix < 3; ix++)
+    for (int n = 0; n < 8; n++) centroid[ix] = centroid[ix] + X[3 * n + ix];
+  for (int ix = 0; ix < 3; ix++) centroid[ix] = centroid[ix] / 8.0;
+  double evol = 0.0;
+  for (int ix = 0; ix < 3; ix++) {
+    /* face_map: dimension ix takes the face side, ia = max(3-ix,1) the
+     * first in-face index, ib = min(5-ix,3) the second (1-based) */
+    int ia = (3 - (ix + 1) > 1 ? 3 - (ix + 1) : 1) - 1;
+    int ib = (5 - (ix + 1) < 3 ? 5 - (ix + 1) : 3) - 1;
+    for (int iface = 0; iface < 2; iface++) {
+      double face[2][2][3];
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+          int a[3];
+          a[ix] = iface; a[ia] = i; a[ib] = j;
+          int n = a[0] + 2 * a[1] + 4 * a[2];
+          for (int d = 0; d < 3; d++) face[i][j][d] = X[3 * n + d];
+        }
+      double v1[3], v2[3], v3[3], cross[3];
+      for (int d = 0; d < 3; d++) {
+        v1[d] = face[0][1][d] - face[1][0][d];
+        v2[d] = centroid[d] - face[1][0][d];
+      }
+      cross[0] = v1[1] * v2[2] - v1[2] * v2[1];
+      cross[1] = v1[2] * v2[0] - v1[0] * v2[2];
+      cross[2] = v1[0] * v2[1] - v1[1] * v2[0];
+      for (int inode = 0; inode < 2; inode++) {
+        for (int d = 0; d < 3; d++) v3[d] = face[inode][inode][d] - face[1][0][d];
+        double voltet = 0.0;
+        for (int d = 0; d < 3; d++) voltet = voltet + v3[d] * cross[d];
+        evol = evol + fabs(voltet);
+      }
+    }
+  }
+  return evol / 6.0;
+}
+
+static int setup_mesh(orc_region *r, struct orc_picl *p) {
+  int ne = r->nCells;
+  p->ne = ne;
+  p->xm1b = malloc(sizeof(double) * 24 * (size_t)ne);
+  p->centeri = malloc(sizeof(double) * 3 * (size_t)ne);
+  p->bbox = malloc(sizeof(double) * 6 * (size_t)ne);
+  p->rexp = malloc(sizeof(double) * (size_t)ne);
+  p->proFld = calloc(NPRO * (size_t)ne, sizeof(double));
+  p->fld = calloc(ORC_PICL_NFLD * (size_t)ne, sizeof(double));
+  for (int e = 0; e < ne; e++) {
+    double *X = p->xm1b + 24 * (size_t)e;
+    for (int n = 0; n < 8; n++)
+      for (int d = 0; d < 3; d++) X[3 * n + d] = r->xyz[3 * (size_t)r->c2v[8 * (size_t)e + NODE2V[n]] + d];
+    for (int d = 0; d < 3; d++) {
+      double lo = X[d], hi = X[d];
+      for (int n = 1; n < 8; n++) { lo = fmin(lo, X[3 * n + d]); hi = fmax(hi, X[3 * n + d]); }
+      p->bbox[6 * (size_t)e + d] = lo;
+      p->bbox[6 * (size_t)e + 3 + d] = hi;
+    }
+    p->rexp[e] = 1.0 / hex_volume(X);
+  }
+  /* LocalInterp set-up, ppiclf_solve.f:2253-2289: centroid = node mean; the
+   * extent threshold is reset for every element, so the last one's stays */
+  for (int e = 0; e < ne; e++) {
+    const double *X = p->xm1b + 24 * (size_t)e;
+    for (int l = 0; l < 3; l++) {
+      double c = 0.0, mx = -1.0e10, mn = 1.0e10;
+      for (int n = 0; n < 8; n++) {
+        c = c + X[3 * n + l];
+        if (X[3 * n + l] > mx) mx = X[3 * n + l];
+        if (X[3 * n + l] < mn) mn = X[3 * n + l];
+      }
+      double d2i = (mx - mn) * (mx - mn), d2m = 0.0;
+      if (d2i > d2m) d2m = d2i;
+      p->d2Max[l] = d2m;
+      p->centeri[3 * (size_t)e + l] = c / 8;
+    }
+  }
+  for (int l = 0; l < 3; l++) p->d2Max[l] = p->d2Max[l] * (1.5 * 1.5);
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Element location.  gslib findpts is not restated: the rule here is the    */
+/* lowest-numbered element whose trilinear map holds the point within        */
+/* |r|,|s|,|t| <= 1 + 1e-10 (Newton from the element centre).                */
+/* ------------------------------------------------------------------------ */
+static int trilinear_inside(const double *X, const double *xp) {
+  double r = 0.0, s = 0.0, t = 0.0;
+  for (int it = 0; it < 25; it++) {
+    double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
+    double b[2] = {0.5 * (1.0 - s), 0.5 * (1.0 + s)};
+    double c[2] = {0.5 * (1.0 - t), 0.5 * (1.0 + t)};
+    const double da[2] = {-0.5, 0.5};
+    double x[3] = {0.0, 0.0, 0.0}, J[3][3] = {{0.0}};
+    for (int n = 0; n < 8; n++) {
+      int i = n & 1, j = (n >> 1) & 1, k = n >> 2;
+      double w = a[i] * b[j] * c[k];
+      double wr = da[i] * b[j] * c[k], ws = a[i] * da[j] * c[k], wt = a[i] * b[j] * da[k];
+      for (int d = 0; d < 3; d++) {
+        double v = X[3 * n + d];
+        x[d] = x[d] + w * v;
+        J[d][0] = J[d][0] + wr * v;
+        J[d][1] = J[d][1] + ws * v;
+        J[d][2] = J[d][2] + wt * v;
+      }
+    }
+    double f0 = xp[0] - x[0], f1 = xp[1] - x[1], f2 = xp[2] - x[2];
+    double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double m01 = J[1][0] * J[2][2] - J[1][2] * J[2][0];
+    double m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * m00 - J[0][1] * m01 + J[0][2] * m02;
+    if (!(fabs(det) > 0.0)) return 0;
+    double dr = (f0 * m00 - J[0][1] * (f1 * J[2][2] - J[1][2] * f2) + J[0][2] * (f1 * J[2][1] - J[1][1] * f2)) / det;
+    double ds = (J[0][0] * (f1 * J[2][2] - J[1][2] * f2) - f0 * m01 + J[0][2] * (J[1][0] * f2 - f1 * J[2][0])) / det;
+    double dt = (J[0][0] * (J[1][1] * f2 - f1 * J[2][1]) - J[0][1] * (J[1][0] * f2 - f1 * J[2][0]) + f0 * m02) / det;
+    r = r + dr; s = s + ds; t = t + dt;
+    if (fabs(dr) + fabs(ds) + fabs(dt) < 1.0e-14) break;
+  }
+  const double lim = 1.0 + 1.0e-10;
+  return fabs(r) <= lim && fabs(s) <= lim && fabs(t) <= lim;
+}
+
+static int locate(const struct orc_picl *p, const double *xp) {
+  for (int e = 0; e < p->ne; e++) {
+    const double *bb = p->bbox + 6 * (size_t)e;
+    int in = 1;
+    for (int d = 0; d < 3; d++) {
+      double tol = 1.0e-9 * (bb[3 + d] - bb[d]);
+      if (xp[d] < bb[d] - tol || xp[d] > bb[3 + d] + tol) in = 0;
+    }
+    if (in && trilinear_inside(p->xm1b + 24 * (size_t)e, xp)) return e;
+  }
+  return -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LocalInterp: ppiclf_solve.f:2234-2409                                     */
+/* ------------------------------------------------------------------------ */
+static int nearest27(const struct orc_picl *p, const double *xp, int *inearest, double *d2) {
+  int nnearest = 0;
+  double d2s[28];
+  int ins[28];
+  for (int q = 0; q < 28; q++) { ins[q] = -1; d2s[q] = F32(1E20); }
+  for (int ie = 0; ie < p->ne; ie++) {
+    double d2i = 0.0;
+    int farAway = 0;
+    for (int l = 0; l < 3; l++) {
+      double t = p->centeri[3 * (size_t)ie + l] - xp[l];
+      double d2l = t * t;
+      d2i = d2i + d2l;
+      if (d2l > p->d2Max[l]) farAway = 1;
+    }
+    if (farAway) continue;
+    int added = 0;
+    for (int i = 1; i <= 27; i++) {
+      int j = 27 - i; /* 0-based slot of the reference's j = 27 - i + 1 */
+      if (d2i < d2s[j]) {
+        d2s[j + 1] = d2s[j];
+        ins[j + 1] = ins[j];
+        d2s[j] = d2i;
+        ins[j] = ie;
+        added = 1;
+      } else {
+        break;
+      }
+    }
+    if (added) nnearest = nnearest + 1;
+  }
+  if (nnearest > 27) nnearest = 27;
+  for (int q = 0; q < nnearest; q++) { inearest[q] = ins[q]; d2[q] = d2s[q]; }
+  return nnearest;
+}
+
+static void local_interp(struct orc_picl *p, int ip) {
+  const double eps = F32(1.0e-12);
+  int inear[27];
+  double d2[27];
+  const double *xp = p->y + LRS * (size_t)ip;
+  int nn = nearest27(p, xp, inear, d2);
+  if (nn < 1) return; /* unreachable for located particles */
+  for (int f = 0; f < ORC_PICL_NFLD; f++) {
+    int j = orc_picl_field_rprop[f] - 1;
+    double v = 0.0, wsum = 0.0;
+    for (int k = 0; k < nn; k++) {
+      double w = 1.0 / (sqrt(d2[k]) + eps);
+      v = v + w * p->fld[ORC_PICL_NFLD * (size_t)inear[k] + f];
+      wsum = wsum + w;
+    }
+    p->rprop[LRP * (size_t)ip + j] = v / wsum;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* InitSolve: locate, interpolate, project the map built from the current    */
+/* ydotc, zero ydotc.  ppiclf_solve.f:1954-1992; map:                        */
+/* ppiclf_user_MapProjPart.f:37-66 via ppiclf_comm.f:1190-1217; deposit:     */
+/* ppiclf_solve.f:3285-3394 (sngl_elem); gather :3431-3447.                  */
+/* ------------------------------------------------------------------------ */
+static void init_solve(struct orc_picl *p, int interp) {
+  for (int ip = 0; ip < p->n; ip++) {
+    if (p->lost[ip]) continue;
+    int e = locate(p, p->y + LRS * (size_t)ip);
+    p->elem[ip] = e;
+    if (e < 0) p->lost[ip] = 1;
+  }
+  if (interp)
+    for (int ip = 0; ip < p->n; ip++)
+      if (!p->lost[ip]) local_interp(p, ip);
+  double *fldb = calloc(NPRO * (size_t)p->ne, sizeof(double));
+  for (int ip = 0; ip < p->n; ip++) {
+    if (p->lost[ip]) continue;
+    const double *rp = p->rprop + LRP * (size_t)ip, *yc = p->ydotc + LRS * (size_t)ip;
+    double map[NPRO];
+    map[P_JPHIP] = rp[R_JVOLP] * rp[R_JSPL];
+    map[P_JFX] = yc[JVX] * rp[R_JSPL];
+    map[P_JFY] = yc[JVY] * rp[R_JSPL];
+    map[P_JFZ] = yc[JVZ] * rp[R_JSPL];
+    map[P_JE] = yc[JT] * rp[R_JSPL];
+    int e = p->elem[ip];
+    double rexp = p->rexp[e];
+    for (int jj = 0; jj < NPRO; jj++)
+      fldb[NPRO * (size_t)e + jj] = fldb[NPRO * (size_t)e + jj] + (map[jj] * 1.0) * rexp;
+  }
+  for (size_t q = 0; q < NPRO * (size_t)p->ne; q++) p->proFld[q] = 0.0 + fldb[q];
+  free(fldb);
+  memset(p->ydotc, 0, sizeof(double) * LRS * (size_t)p->n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Forces                                                                    */
+/* ------------------------------------------------------------------------ */
+/* ppiclf_user_QS_Parmar: libpicl/user_files/ppiclf_user_QuasiSteady.f:22-122 */
+static double qs_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = acos(-1.0);
+  double mp = fmax(rmachp, 0.01), re = fmax(rep, 0.1), rcd1;
+  if (re < F32(1E-14)) {
+    rcd1 = 1.0;
+  } else {
+    double rmacr = F32(0.6);
+    double rcd_mcr = (1.0 + F32(0.15) * pow(re, F32(0.684))) +
+                     (re / 24.0) * (F32(0.513) / (1.0 + 483.0 / pow(re, F32(0.669))));
+    if (mp <= rmacr) {
+      double rcd_std = (1.0 + F32(0.15) * pow(re, F32(0.687))) +
+                       (re / 24.0) * (F32(0.42) / (1.0 + 42500.0 / pow(re, F32(1.16))));
+      double rmach_rat = mp / rmacr;
+      rcd1 = rcd_std + (rcd_mcr - rcd_std) * rmach_rat;
+    } else if (mp <= 1.0) {
+      double rcd_M1 = (1.0 + F32(0.118) * pow(re, F32(0.813))) +
+                      (re / 24.0) * F32(0.69) / (1.0 + 3550.0 / pow(re, F32(.793)));
+      double C1 = F32(6.48), C2 = F32(9.28), C3 = F32(12.21);
+      double m2 = mp * mp, m3 = mp * mp * mp;
+      double f1M = -F32(1.884) + F32(8.422) * mp - F32(13.70) * m2 + F32(8.162) * m3;
+      double f2M = -F32(2.228) + F32(10.35) * mp - F32(16.96) * m2 + F32(9.840) * m3;
+      double f3M = F32(4.362) - F32(16.91) * mp + F32(19.84) * m2 - F32(6.296) * m3;
+      double lrep = log(re);
+      double factor = f1M * (lrep - C2) * (lrep - C3) / ((C1 - C2) * (C1 - C3)) +
+                      f2M * (lrep - C1) * (lrep - C3) / ((C2 - C1) * (C2 - C3)) +
+                      f3M * (lrep - C1) * (lrep - C2) / ((C3 - C1) * (C3 - C2));
+      rcd1 = rcd_mcr + (rcd_M1 - rcd_mcr) * factor;
+    } else if (mp < 1.75) {
+      double rcd_M1 = (1.0 + F32(0.118) * pow(re, F32(0.813))) +
+                      (re / 24.0) * F32(0.69) / (1.0 + 3550.0 / pow(re, F32(.793)));
+      double rcd_M2 = (1.0 + F32(0.107) * pow(re, F32(0.867))) +
+                      (re / 24.0) * F32(0.646) / (1.0 + 861.0 / pow(re, F32(.634)));
+      double C1 = F32(6.48), C2 = F32(8.93), C3 = F32(12.21);
+      double m2 = mp * mp, m3 = mp * mp * mp;
+      double f1M = -F32(2.963) + F32(4.392) * mp - F32(1.169) * m2 - F32(0.027) * m3 -
+                   F32(0.233) * exp((1.0 - mp) / F32(0.011));
+      double f2M = -F32(6.617) + F32(12.11) * mp - F32(6.501) * m2 + F32(1.182) * m3 -
+                   F32(0.174) * exp((1.0 - mp) / F32(0.010));
+      double f3M = -F32(5.866) + F32(11.57) * mp - F32(6.665) * m2 + F32(1.312) * m3 -
+                   F32(0.350) * exp((1.0 - mp) / F32(0.012));
+      double lrep = log(re);
+      double factor = f1M * (lrep - C2) * (lrep - C3) / ((C1 - C2) * (C1 - C3)) +
+                      f2M * (lrep - C1) * (lrep - C3) / ((C2 - C1) * (C2 - C3)) +
+                      f3M * (lrep - C1) * (lrep - C2) / ((C3 - C1) * (C3 - C2));
+      rcd1 = rcd_M1 + (rcd_M2 - rcd_M1) * factor;
+    } else {
+      rcd1 = (1.0 + F32(0.107) * pow(re, F32(0.867))) +
+             (re / 24.0) * F32(0.646) / (1.0 + 861.0 / pow(re, F32(.634)));
+    }
+  }
+  double phi_corr = 1.0 + F32(5.94) * fmin(rphip, 0.5);
+  double beta = rcd1 * 3.0 * rpi * rmu * dp;
+  return beta * phi_corr;
+}
+
+/* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
+ * particle with QS Parmar, AM Parmar (ppiclf_user_AddedMass.f:16-124), the
+ * pressure-gradient force, Euler flow (no viscous stress term), no
+ * collisions, lift, torque, heat transfer or burning. */
+static void set_ydot(struct orc_picl *p, int i) {
+  const orc_picl_params *P = &p->pp;
+  double *y = p->y + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i, *yc = p->ydotc + LRS * (size_t)i;
+  const double *rp = p->rprop + LRP * (size_t)i;
+  const double rcp_part = P->rcpPart;
+  double rmu;
+  if (P->rmuFlag == 0) {
+    rmu = P->rmuRef;
+  } else {
+    double temp = rp[R_JT];
+    rmu = P->rmuRef * sqrt(temp / P->tref) * (1.0 + P->suth / P->tref) / (1.0 + P->suth / temp);
+  }
+  double rmass = rp[R_JVOLP] * rp[R_JRHOP];
+  double vx = rp[R_JUX] - y[JVX];
+  double vy = rp[R_JUY] - y[JVY];
+  double vz = rp[R_JUZ] - y[JVZ];
+  double vmag = sqrt(vx * vx + vy * vy + vz * vz);
+  double rhof = rp[R_JRHOF];
+  double dp = rp[R_JDP];
+  double rep = vmag * dp * rhof / rmu;
+  double rphip = rp[R_JPHIP];
+  double rphif = 1.0 - rp[R_JPHIP];
+  double asndf = rp[R_JCS];
+  double rmachp = vmag / asndf;
+  if (vmag <= 1.0e-3) return; /* keeps the previous ydot; ydotc stays 0 */
+  rep = fmax(0.1, rep);
+  rphip = fmin(rphip, 0.62);
+  rphif = 1.0 - rphip;
+  double famx = 0.0, famy = 0.0, famz = 0.0, rmass_add = 0.0;
+  double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
+  const double fcx = 0.0, fcy = 0.0, fcz = 0.0, fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
+  const double liftx = 0.0, lifty = 0.0, liftz = 0.0, qq = 0.0;
+  double beta = qs_parmar(rmachp, rphip, rep, rmu, dp);
+  double fqsx = beta * vx, fqsy = beta * vy, fqsz = beta * vz;
+  fqsx = fqsx + 0.0; /* + fqs_fluct (zero) */
+  fqsy = fqsy + 0.0;
+  fqsz = fqsz + 0.0;
+  if (P->amFlag == 1) {
+    double rcd_am;
+    if (rmachp > F32(0.6)) {
+      /* 1.0 + 1.8*((0.6)**2) + 7.6*((0.6)**4): a real*4 constant expression
+       * that the compiler folds in single precision */
+      rcd_am = 0x1.5104d6p+1;
+    } else {
+      double m2 = rmachp * rmachp;
+      rcd_am = 1.0 + F32(1.8) * m2 + F32(7.6) * (m2 * m2);
+    }
+    rcd_am = rcd_am * 0.5;
+    rcd_am = rcd_am * (1.0 + F32(0.68) * (rphip * rphip));
+    rmass_add = rhof * rp[R_JVOLP] * rcd_am;
+    double SDrho = rp[R_JRHSR] + y[JVX] * rp[R_JPGCX] + y[JVY] * rp[R_JPGCY] + y[JVZ] * rp[R_JPGCZ];
+    SDrho = SDrho / rphif;
+    double vgradrho = vx * rp[R_JRHOGX] + vy * rp[R_JRHOGY] + vz * rp[R_JRHOGZ];
+    double ug = rp[R_JUX], vg = rp[R_JUY], wg = rp[R_JUZ];
+    famx = rcd_am * rp[R_JVOLP] * (vx * SDrho + rhof * rp[R_JSDRX] + ug * vgradrho);
+    famy = rcd_am * rp[R_JVOLP] * (vy * SDrho + rhof * rp[R_JSDRY] + vg * vgradrho);
+    famz = rcd_am * rp[R_JVOLP] * (vz * SDrho + rhof * rp[R_JSDRZ] + wg * vgradrho);
+  }
+  if (P->pgFlag == 1) {
+    fdpdx = -rp[R_JVOLP] * rp[R_JDPDX];
+    fdpdy = -rp[R_JVOLP] * rp[R_JDPDY];
+    fdpdz = -rp[R_JVOLP] * rp[R_JDPDZ];
+    fdpdx = fdpdx + 0.0; /* + fdpvdx (Euler flow) */
+    fdpdy = fdpdy + 0.0;
+    fdpdz = fdpdz + 0.0;
+  }
+  double rmass_therm = rmass * rcp_part;
+  double rmass_omega = rmass * dp * dp / 10.0;
+  const double taux = 0.0, tauy = 0.0, tauz = 0.0;
+  yd[JX] = y[JVX];
+  yd[JY] = y[JVY];
+  yd[JZ] = y[JVZ];
+  yd[JVX] = (fqsx + famx + fdpdx + fvux + liftx + fcx) / (rmass + rmass_add);
+  yd[JVY] = (fqsy + famy + fdpdy + fvuy + lifty + fcy) / (rmass + rmass_add);
+  yd[JVZ] = (fqsz + famz + fdpdz + fvuz + liftz + fcz) / (rmass + rmass_add);
+  yd[JT] = qq / rmass_therm;
+  yd[7] = taux / rmass_omega;
+  yd[8] = tauy / rmass_omega;
+  yd[9] = tauz / rmass_omega;
+  yd[10] = 0.0;
+  yd[11] = 0.0;
+  if (P->feedbackFlag == 1) {
+    const double tauxh = 0.0, tauyh = 0.0, tauzh = 0.0;
+    yc[JVX] = rp[R_JSPL] * (yd[JVX] * rmass - fcx);
+    yc[JVY] = rp[R_JSPL] * (yd[JVY] * rmass - fcy);
+    yc[JVZ] = rp[R_JSPL] * (yd[JVZ] * rmass - fcz);
+    yc[JT] = rp[R_JSPL] * ((fqsx + fvux + liftx) * y[JVX] + (fqsy + fvuy + lifty) * y[JVY] +
+                           (fqsz + fvuz + liftz) * y[JVZ] + famx * rp[R_JUX] + famy * rp[R_JUY] +
+                           famz * rp[R_JUZ] + tauxh * y[7] + tauyh * y[8] + tauzh * y[9] + qq);
+  } else {
+    yc[JVX] = yc[JVY] = yc[JVZ] = yc[JT] = 0.0;
+  }
+}
+
+/* ppiclf_solve_IntegrateRK3s_Rocflu + SetRK3Coeff (imethod 2):
+ * ppiclf_solve.f:1845-1936, 2774-2793 */
+static void integrate_particle(struct orc_picl *p, int istage, double dt) {
+  double c[3][3] = {{0.0, 1.0, dt * 8.0 / 15.0},
+                    {dt * 17.0 / 60.0, 1.0, dt * 5.0 / 12.0},
+                    {dt * 5.0 / 12.0, 1.0, dt * 3.0 / 4.0}};
+  init_solve(p, 1);
+  for (int i = 0; i < p->n; i++)
+    if (!p->lost[i]) set_ydot(p, i);
+  if (istage == 1) memset(p->y1, 0, sizeof(double) * LRS * (size_t)p->n);
+  const double *k = c[istage - 1];
+  for (int i = 0; i < p->n; i++) {
+    if (p->lost[i]) continue;
+    double *y = p->y + LRS * (size_t)i, *y1 = p->y1 + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i;
+    for (int j = 0; j < LRS; j++) y[j] = -(k[0] * y1[j]) + k[1] * y[j] + k[2] * yd[j];
+    for (int j = 0; j < LRS; j++) y1[j] = yd[j];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PICL_TEMP_Runge: rocpicl/PICL_TEMP_Runge.F90:623-1234                     */
+/* ------------------------------------------------------------------------ */
+static double sum8(double v) {
+  double s = 0.0;
+  for (int q = 0; q < 8; q++) s = s + v;
+  return s;
+}
+/* Rocflu's read-back of a projected volume fraction, :826-842 and :1217-1234 */
+static double phip_of(double vfP, double vol) {
+  double s = 0.0;
+  for (int q = 0; q < 8; q++) s = s + (0.125 * vfP) * vol;
+  s = s / vol;
+  if (s > F32(0.62)) s = F32(0.62);
+  return s;
+}
+
+int orc__picl_runge(orc_region *r, int istage) {
+  struct orc_picl *p = r->picl;
+  int N = r->nCells;
+  /* field assembly :712-842 (one value per element; LocalInterp's node
+   * average of the 8 identical copies, :2363-2373, is sum8(v)/8) */
+  for (int i = 0; i < N; i++) {
+    const double *cv = r->cv + 5 * (size_t)i, *dv = r->dv + 3 * (size_t)i;
+    const double *rhs = r->rhs + 5 * (size_t)i, *g = r->grad + 15 * (size_t)i;
+    double vol = r->vol[i];
+    double PhiP = phip_of(p->proFld[NPRO * (size_t)i + P_JPHIP], vol);
+    double ug[3] = {cv[1] / cv[0], cv[2] / cv[0], cv[3] / cv[0]};
+    double sdr[3];
+    for (int m = 0; m < 3; m++) {
+      double dot = 0.0; /* DOT_PRODUCT(ug, pGc(:,2+m,i)) */
+      for (int d = 0; d < 3; d++) dot = dot + ug[d] * g[3 * (1 + m) + d];
+      sdr[m] = (-(rhs[1 + m] / vol) + ug[m] * rhs[0] / vol) / cv[0] + dot;
+    }
+    double vFrac = 1.0 - r->phi[i];
+    double f[ORC_PICL_NFLD];
+    f[0] = cv[0] / vFrac;
+    f[1] = cv[1] / cv[0];
+    f[2] = cv[2] / cv[0];
+    f[3] = cv[3] / cv[0];
+    f[4] = g[12]; f[5] = g[13]; f[6] = g[14];
+    f[7] = dv[2];
+    f[8] = dv[1];
+    f[9] = PhiP;
+    f[10] = sdr[0]; f[11] = sdr[1]; f[12] = sdr[2];
+    f[13] = -(rhs[0] / vol);
+    f[14] = g[0]; f[15] = g[1]; f[16] = g[2];
+    f[17] = r->gradRhog[3 * (size_t)i];
+    f[18] = r->gradRhog[3 * (size_t)i + 1];
+    f[19] = r->gradRhog[3 * (size_t)i + 2];
+    for (int q = 0; q < ORC_PICL_NFLD; q++) p->fld[ORC_PICL_NFLD * (size_t)i + q] = sum8(f[q]) / 8;
+  }
+  integrate_particle(p, istage, r->dtMin);
+  if (p->pp.feedbackFlag == 1) { /* :926-1038 */
+    for (int i = 0; i < N; i++) {
+      const double *pf = p->proFld + NPRO * (size_t)i;
+      double vol = r->vol[i];
+      double jfx = sum8(pf[P_JFX]) * 0.125 * vol;
+      double jfy = sum8(pf[P_JFY]) * 0.125 * vol;
+      double jfz = sum8(pf[P_JFZ]) * 0.125 * vol;
+      double jfe = sum8(pf[P_JE]) * 0.125 * vol;
+      double *rhs = r->rhs + 5 * (size_t)i;
+      rhs[1] = rhs[1] + jfx;
+      rhs[2] = rhs[2] + jfy;
+      rhs[3] = rhs[3] + jfz;
+      rhs[4] = rhs[4] + jfe;
+    }
+  }
+  for (int i = 0; i < N; i++) r->phi[i] = phip_of(p->proFld[NPRO * (size_t)i + P_JPHIP], r->vol[i]);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const double *y,
+                  const double *pconst) {
+  if (r->nvc != 8 || r->p.spaceOrder != 2) return -1;
+  struct orc_picl *p = calloc(1, sizeof *p);
+  p->pp = *pp;
+  p->n = nPart;
+  size_t n = nPart > 0 ? (size_t)nPart : 1;
+  p->y = malloc(sizeof(double) * LRS * n);
+  memcpy(p->y, y, sizeof(double) * LRS * (size_t)nPart);
+  p->y1 = calloc(LRS * n, sizeof(double));
+  p->ydot = calloc(LRS * n, sizeof(double));
+  p->ydotc = calloc(LRS * n, sizeof(double));
+  p->rprop = calloc(LRP * n, sizeof(double));
+  p->elem = malloc(sizeof(int) * n);
+  p->lost = calloc(n, 1);
+  for (int i = 0; i < nPart; i++) {
+    double *rp = p->rprop + LRP * (size_t)i;
+    rp[R_JRHOP] = pconst[4 * (size_t)i];
+    rp[R_JDP] = pconst[4 * (size_t)i + 1];
+    rp[R_JVOLP] = pconst[4 * (size_t)i + 2];
+    rp[R_JSPL] = pconst[4 * (size_t)i + 3];
+  }
+  setup_mesh(r, p);
+  r->picl = p;
+  int N = r->nCells;
+  r->phi = calloc((size_t)N, sizeof(double));
+  r->gradRhog = calloc(3 * (size_t)N, sizeof(double));
+  init_solve(p, 0); /* ppiclf_comm_InitOverlapMesh -> InitSolve (no fields yet) */
+  /* PICL_TEMP_InitSolver.F90:789-835 */
+  for (int i = 0; i < N; i++) r->phi[i] = phip_of(p->proFld[NPRO * (size_t)i + P_JPHIP], r->vol[i]);
+  for (int i = 0; i < N; i++) {
+    double vFrac = 1.0 - r->phi[i];
+    for (int k = 0; k < 5; k++) r->cv[5 * (size_t)i + k] = vFrac * r->cv[5 * (size_t)i + k];
+  }
+  orc__set_dependent_vars(r); /* RFLU_SetVarsWrapper after the PICL set-up */
+  return 0;
+}
+
+int orc_picl_npart(const orc_region *r) { return r->picl ? r->picl->n : 0; }
+int orc_picl_nlost(const orc_region *r) {
+  int c = 0;
+  if (r->picl)
+    for (int i = 0; i < r->picl->n; i++) c += r->picl->lost[i];
+  return c;
+}
+void orc_picl_get_particles(const orc_region *r, double *y, double *y1, double *ydot, double *ydotc,
+                            int *elem) {
+  const struct orc_picl *p = r->picl;
+  size_t b = sizeof(double) * LRS * (size_t)p->n;
+  if (y) memcpy(y, p->y, b);
+  if (y1) memcpy(y1, p->y1, b);
+  if (ydot) memcpy(ydot, p->ydot, b);
+  if (ydotc) memcpy(ydotc, p->ydotc, b);
+  if (elem)
+    for (int i = 0; i < p->n; i++) elem[i] = p->lost[i] ? -1 : p->elem[i];
+}
+void orc_picl_get_rprop(const orc_region *r, double *rprop) {
+  memcpy(rprop, r->picl->rprop, sizeof(double) * LRP * (size_t)r->picl->n);
+}
+void orc_picl_get_cells(const orc_region *r, double *phi, double *gradRhog, double *proFld,
+                        double *fields) {
+  int N = r->nCells;
+  if (phi) memcpy(phi, r->phi, sizeof(double) * (size_t)N);
+  if (gradRhog) memcpy(gradRhog, r->gradRhog, sizeof(double) * 3 * (size_t)N);
+  if (proFld) memcpy(proFld, r->picl->proFld, sizeof(double) * NPRO * (size_t)N);
+  if (fields) memcpy(fields, r->picl->fld, sizeof(double) * ORC_PICL_NFLD * (size_t)N);
+}
+void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem) {
+  for (int i = 0; i < n; i++) elem[i] = locate(r->picl, xyz + 3 * (size_t)i);
+}
+int orc_picl_neighbors(const orc_region *r, const double *xp, int *ids, double *d2) {
+  return nearest27(r->picl, xp, ids, d2);
+}

@@ -1,0 +1,63 @@
+/*
+ * CPU oracle for the ppiclF coupling of RocfluMP -- TEST INFRASTRUCTURE.
+ *
+ * Restates PICL_TEMP_InitSolver / PICL_TEMP_Runge (rocpicl/) and the ppiclF
+ * routines they drive for one rank: element location, LocalInterp (IDW over
+ * the 27 nearest element centroids), ppiclf_user_SetYdot with QS Parmar,
+ * AM Parmar and the pressure-gradient force, IntegrateRK3s_Rocflu, and the
+ * single-element box projection with its read-back into rhs and piclVF.
+ * Only tests/ and bench.py's CPU-baseline leg use it, as the checker.
+ *
+ * Particle arrays use ppiclF's layout: y(PPICLF_LRS=12, n) -> y[12*p + j],
+ * rprop(PPICLF_LRP=48, n) -> rprop[48*p + j] (0-based j = index - 1).
+ */
+#ifndef PICL_ORACLE_H
+#define PICL_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_region orc_region;
+
+typedef struct orc_picl_params {
+  int rmuFlag;       /* 0: fixed viscosity rmuRef; 1: Sutherland law */
+  double rmuRef;     /* reference viscosity */
+  double tref;       /* Sutherland reference temperature */
+  double suth;       /* Sutherland constant */
+  double rcpPart;    /* particle heat capacity (ppiclf_rcp_part) */
+  int qsFlag;        /* 1: QS Parmar (the model restated here) */
+  int amFlag;        /* 0: off, 1: AM Parmar */
+  int pgFlag;        /* 0: off, 1: pressure-gradient force */
+  int feedbackFlag;  /* 0: one-way, 1: two-way coupling */
+} orc_picl_params;
+
+/* Interpolated fields, in slot order, with their ppiclF rprop index
+ * (1-based, libpicl/user_files/PPICLF_USER.h). */
+#define ORC_PICL_NFLD 20
+extern const int orc_picl_field_rprop[ORC_PICL_NFLD];
+
+/* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835) on a
+ * region whose cv holds the pure-gas state: locate the particles, project
+ * them (ydotc = 0), set piclVF, scale cv by (1 - piclVF) and recompute dv.
+ * y: 12 per particle; pconst: (rho_p, d_p, V_p, SPL) per particle.
+ * Requires an all-hex, second-order region.  Returns 0 or -1. */
+int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const double *y,
+                  const double *pconst);
+int orc_picl_npart(const orc_region *r);
+int orc_picl_nlost(const orc_region *r);
+void orc_picl_get_particles(const orc_region *r, double *y, double *y1, double *ydot, double *ydotc,
+                            int *elem);
+void orc_picl_get_rprop(const orc_region *r, double *rprop);
+/* Per cell: piclVF, limited grad(rho_g) (3), projected node values of
+ * (PHIP, FX, FY, FZ, E) (5), interpolation inputs of the last stage (20). */
+void orc_picl_get_cells(const orc_region *r, double *phi, double *gradRhog, double *proFld,
+                        double *fields);
+/* Element containing each point (-1: none), by the oracle's location rule. */
+void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
+/* Elements LocalInterp sums for point xp, nearest first; returns how many. */
+int orc_picl_neighbors(const orc_region *r, const double *xp, int *ids, double *d2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

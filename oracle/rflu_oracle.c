@@ -10,7 +10,7 @@
  *
  * Parity pins for each part are listed in oracle/README.md.
  */
-#include "rflu_oracle.h"
+#include "rflu_oracle_int.h"
 
 #include <math.h>
 #include <stdint.h>
@@ -27,31 +27,7 @@ static const int F2V_TET[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
 static const int F2V_HEX[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
                                   {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
 
-struct orc_region {
-  orc_params p;
-  int nVert, nvc, nCells;
-  double *xyz;
-  int *c2v;
-  /* interior faces (reference order) */
-  int nFaces;
-  int *f2c, *f2v;
-  double *fn, *fc;
-  /* boundary faces, patch by patch */
-  int nPatches, nvf, nBFaces;
-  int *pBcType, *pBcOpt, *pNBF, *pOff;
-  double *pPout;
-  int *bf2v, *bf2c;
-  double *bfn, *bfc;
-  /* cells */
-  double *vol, *cofg;
-  /* c2c stencils */
-  int *csOff, *csMem;
-  double *moms; /* 10 per cell: 11,12,22,13,23,33,14,24,34,44 */
-  /* state */
-  double *cv, *cvOld, *dv, *rhs, *rhsSum, *diss, *grad, *lim, *dt;
-  double cpGv, molGv; /* gv(CP), gv(MOL): constant for TCPERF */
-  double currentTime, timeSinceRestart, dtMin;
-};
+/* struct orc_region: rflu_oracle_int.h */
 
 /* ------------------------------------------------------------------------ */
 /* Sorting helpers                                                           */
@@ -536,6 +512,8 @@ static void set_dependent_vars(orc_region *r) {
     double ir = 1.0 / rr;
     double Eo = ir * cv[4];
     double Vm2 = ir * ir * (cv[1] * cv[1] + cv[2] * cv[2] + cv[3] * cv[3]);
+    /* PICL: p, T, c of the gas phase use rho/(1-phi) (:276-281) */
+    if (r->phi) rr = rr / (1.0 - r->phi[i]);
     double *dv = r->dv + 3 * (size_t)i;
     dv[0] = P_DEoGVm2(rr, Eo, g, Vm2, ksg);
     dv[1] = T_DPR(rr, dv[0], gc);
@@ -604,7 +582,9 @@ static void prim2cons(orc_region *r) {
     cv[3] = rr * w;
     double gc = R_M(r->molGv);
     double g = G_CpR(r->cpGv, gc);
-    cv[4] = rr * Eo_DGPUVW(rr, g, p, u, v, w, ksg);
+    /* PICL: energy from the gas-phase density r/(1-phi) (:584-586) */
+    double rPure = r->phi ? rr / (1.0 - r->phi[i]) : rr;
+    cv[4] = rr * Eo_DGPUVW(rPure, g, p, u, v, w, ksg);
   }
 }
 
@@ -721,13 +701,115 @@ static void limiter_venkat(orc_region *r) {
   free(vmin); free(vmax);
 }
 
-/* CellGradientsMP: libfloflu/CellGradientsMP.F90:158-253 */
+/* LSQ gradient of cv(1) alone: RFLU_ComputeGradCellsFast_3D with
+ * iBegVar = iEndVar = 1 (libfloflu/CellGradientsMP.F90:375-378) */
+static void gradient_var1(orc_region *r, double *out) {
+  for (int icg = 0; icg < r->nCells; icg++) {
+    const double *m = r->moms + 10 * (size_t)icg;
+    double r11 = m[0], r12 = m[1], r22 = m[2], r13 = m[3], r23 = m[4], r33 = m[5];
+    double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+    double c12 = -(c11 * r12);
+    double c13 = -(c11 * r13 + c12 * c22 * r23);
+    double c23 = -(c22 * r23);
+    double c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
+    double *g = out + 3 * (size_t)icg;
+    g[0] = g[1] = g[2] = 0.0;
+    const double *x0 = r->cofg + 3 * (size_t)icg;
+    double v0 = r->cv[5 * (size_t)icg];
+    for (int s = r->csOff[icg]; s < r->csOff[icg + 1]; s++) {
+      int j = r->csMem[s];
+      double dx = r->cofg[3 * (size_t)j] - x0[0];
+      double dy = r->cofg[3 * (size_t)j + 1] - x0[1];
+      double dz = r->cofg[3 * (size_t)j + 2] - x0[2];
+      double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      dx = term * dx; dy = term * dy; dz = term * dz;
+      double term1 = c1111 * (dx);
+      double term2 = c2222 * (dy + c12 * dx);
+      double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+      double wtx = term * (term1 + c12 * term2 + c13 * term3);
+      double wty = term * (term2 + c23 * term3);
+      double wtz = term * (term3);
+      double dVar = r->cv[5 * (size_t)j] - v0;
+      g[0] = g[0] + wtx * dVar;
+      g[1] = g[1] + wty * dVar;
+      g[2] = g[2] + wtz * dVar;
+    }
+  }
+}
+
+/* RFLU_WENOGradCellsXYZFast_3D, one gradient (modflu/RFLU_ModWENO.F90:1402-1588):
+ * per component, the average of the cell's own and its stencil members'
+ * gradients weighted by 1/(g^2 + 1e-15), accumulated self first, then the
+ * members in stencil order; all cells read the unmodified input. */
+static void weno_xyz(orc_region *r, double *grad) {
+  int N = r->nCells;
+  double *eno = malloc(sizeof(double) * 3 * (size_t)N);
+  for (int icg = 0; icg < N; icg++) {
+    for (int d = 0; d < 3; d++) {
+      double term = grad[3 * (size_t)icg + d];
+      double sum = 1.0 / (term * term + 1.0E-15);
+      double loc = sum * term;
+      for (int s = r->csOff[icg]; s < r->csOff[icg + 1]; s++) {
+        term = grad[3 * (size_t)r->csMem[s] + d];
+        double w = 1.0 / (term * term + 1.0E-15);
+        sum = sum + w;
+        loc = loc + w * term;
+      }
+      eno[3 * (size_t)icg + d] = loc / sum;
+    }
+  }
+  memcpy(grad, eno, sizeof(double) * 3 * (size_t)N);
+  free(eno);
+}
+
+/* RFLU_LimitGradCellsSimple, one VAR_INFO_POS variable
+ * (modflu/RFLU_ModLimiters.F90:793-921): the gradient is zeroed when the
+ * extrapolation to any face centroid of the cell (interior or boundary) is
+ * <= 0.  The face loop exits at the first such face and only then changes
+ * the gradient, so the outcome does not depend on the face order. */
+static void limit_simple(orc_region *r, double *grad) {
+  int N = r->nCells;
+  char *zero = calloc((size_t)N, 1);
+  for (int f = 0; f < r->nFaces + r->nBFaces; f++) {
+    int nc = f < r->nFaces ? 2 : 1;
+    const double *fc = f < r->nFaces ? r->fc + 3 * (size_t)f : r->bfc + 3 * (size_t)(f - r->nFaces);
+    for (int s = 0; s < nc; s++) {
+      int c = f < r->nFaces ? r->f2c[2 * f + s] : r->bf2c[f - r->nFaces];
+      const double *x = r->cofg + 3 * (size_t)c, *g = grad + 3 * (size_t)c;
+      double dx = fc[0] - x[0], dy = fc[1] - x[1], dz = fc[2] - x[2];
+      double varf = r->cv[5 * (size_t)c] + g[0] * dx + g[1] * dy + g[2] * dz;
+      if (varf <= 0.0) zero[c] = 1;
+    }
+  }
+  for (int i = 0; i < N; i++)
+    if (zero[i]) grad[3 * (size_t)i] = grad[3 * (size_t)i + 1] = grad[3 * (size_t)i + 2] = 0.0;
+  free(zero);
+}
+
+/* CellGradientsMP: libfloflu/CellGradientsMP.F90:158-253, and the PICL
+ * block :355-390 (gradient of rho_g = rho/(1-phi), WENO-XYZ, positivity
+ * limiter; cv(1) is divided and multiplied back in place, so it keeps the
+ * round-tripped bits) */
 static void cell_gradients(orc_region *r) {
-  if (r->p.spaceOrder <= 1) return;
-  cons2prim(r);
-  gradients(r);
-  if (r->p.reconst == ORC_RECONST_VENKAT) limiter_venkat(r);
-  prim2cons(r);
+  if (r->p.spaceOrder > 1) {
+    cons2prim(r);
+    gradients(r);
+    if (r->p.reconst == ORC_RECONST_VENKAT) limiter_venkat(r);
+    prim2cons(r);
+  }
+  if (r->phi) {
+    int N = r->nCells;
+    double *vfg = malloc(sizeof(double) * (size_t)N);
+    for (int i = 0; i < N; i++) {
+      vfg[i] = 1.0 - r->phi[i];
+      r->cv[5 * (size_t)i] = r->cv[5 * (size_t)i] / vfg[i];
+    }
+    gradient_var1(r, r->gradRhog);
+    weno_xyz(r, r->gradRhog);
+    limit_simple(r, r->gradRhog);
+    for (int i = 0; i < N; i++) r->cv[5 * (size_t)i] = r->cv[5 * (size_t)i] * vfg[i];
+    free(vfg);
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1190,11 +1272,16 @@ int orc_rk_stage(orc_region *r, int istage) {
   memcpy(r->cvOld, r->cv, sizeof(double) * 5 * r->nCells);
   memset(r->diss, 0, sizeof(double) * 5 * r->nCells);
   orc_residual(r);
+  if (r->picl) {
+    int e = orc__picl_runge(r, istage);
+    if (e) return e;
+  }
   rk_update(r, istage);
   set_dependent_vars(r);
   return after_update(r);
 }
 void orc_set_dtmin(orc_region *r, double dt) { r->dtMin = dt; }
+void orc__set_dependent_vars(orc_region *r) { set_dependent_vars(r); }
 
 int orc_step(orc_region *r, int nSteps) {
   for (int it = 0; it < nSteps; it++) {
@@ -1204,6 +1291,10 @@ int orc_step(orc_region *r, int nSteps) {
       memcpy(r->cvOld, r->cv, sizeof(double) * 5 * r->nCells);           /* RkInitGeneric */
       memset(r->diss, 0, sizeof(double) * 5 * r->nCells);
       orc_residual(r);
+      if (r->picl) {                                                       /* RungeKuttaMP.F90:243-247 */
+        int e = orc__picl_runge(r, istage);
+        if (e) return e;
+      }
       rk_update(r, istage);
       set_dependent_vars(r);
       int e = after_update(r);
@@ -1262,8 +1353,10 @@ void orc_destroy(orc_region *r) {
   if (!r) return;
   void *ptrs[] = {r->xyz, r->c2v, r->f2c, r->f2v, r->fn, r->fc, r->pBcType, r->pBcOpt, r->pNBF, r->pOff,
                   r->pPout, r->bf2v, r->bf2c, r->bfn, r->bfc, r->vol, r->cofg, r->csOff, r->csMem, r->moms,
-                  r->cv, r->cvOld, r->dv, r->rhs, r->rhsSum, r->diss, r->grad, r->lim, r->dt};
+                  r->cv, r->cvOld, r->dv, r->rhs, r->rhsSum, r->diss, r->grad, r->lim, r->dt,
+                  r->phi, r->gradRhog};
   for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
+  orc__picl_free(r->picl);
   free(r);
 }
 

@@ -82,3 +82,51 @@ def shock_hex(nx=400, ny=80, nz=80, lx=0.2, ly=0.04, lz=0.04, mach=2.0, x_shock=
         return mg.prim_to_cons(rho, u, z, z, p, g).T.copy()
 
     return mesh, params, ic
+
+
+PICL_AIR = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1, amFlag=1,
+                pgFlag=1, feedbackFlag=1)
+
+
+def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=None, x_curtain=None,
+            phi=0.2, d_median=115e-6, sigma_ln=0.2, rho_p=2500.0, discr=4, jitter=0.0, seed=4,
+            nparts_cap=None, **kw):
+    """Shock-particle-curtain: hex box nx*ny*nz of spacing h, Mach `mach`
+    shock at x_shock moving +x into quiescent air, and a slab of `npart`
+    particles (lognormal diameters, uniform positions) whose thickness gives
+    the volume fraction `phi` across the box section.
+
+    Returns (mesh, params, ic, particles, picl_params) where particles =
+    (y[n,12], pconst[n,4] = (rho_p, d_p, V_p, SPL)).  ic(cofg) gives the
+    pure-gas state; the PICL set-up scales it by (1 - phi).
+    """
+    mesh = mg.hex_box(nx, ny, nz, nx * h, ny * h, nz * h, jitter=jitter, seed=seed)
+    params = dict(spaceDiscr=discr, spaceOrder=2, reconst=11, cfl=0.5, dtImposed=1.0e-8,
+                  dtMinLimit=0.0, **AIR)
+    params.update(kw)
+    rho1, p1, g = 1.2, 1.0e5, 1.4
+    rho2, u2, p2 = mg.normal_shock_state(mach, rho1, p1, g)
+    xs = 0.1 * nx * h if x_shock is None else x_shock
+    x0 = xs + 2 * h if x_curtain is None else x_curtain
+
+    def ic(cofg):
+        x = cofg[:, 0]
+        post = x < xs
+        rho = np.where(post, rho2, rho1)
+        u = np.where(post, u2, 0.0)
+        p = np.where(post, p2, p1)
+        z = np.zeros_like(rho)
+        return mg.prim_to_cons(rho, u, z, z, p, g).T.copy()
+
+    rng = np.random.default_rng(seed + 1000)
+    dp = d_median * np.exp(sigma_ln * rng.standard_normal(npart))
+    vp = np.pi / 6.0 * dp ** 3
+    area = (ny * h) * (nz * h)
+    thick = vp.sum() / (phi * area)
+    y = np.zeros((npart, 12))
+    y[:, 0] = x0 + thick * rng.random(npart)
+    y[:, 1] = ny * h * rng.random(npart)
+    y[:, 2] = nz * h * rng.random(npart)
+    y[:, 6] = 300.0
+    pconst = np.stack([np.full(npart, rho_p), dp, vp, np.ones(npart)], axis=1)
+    return mesh, params, ic, (y, pconst), dict(PICL_AIR)
