@@ -1,0 +1,81 @@
+/*
+ * picl_gpu.h -- C ABI of the MI355X ppiclF coupling (particles in a Rocflu
+ * region).
+ *
+ * The particle cloud lives in HBM, attached to a Rocflu handle
+ * (rflu_gpu.h).  Once picl_gpu_init succeeds, every RK stage of that handle
+ * (rflu_gpu_rk_stage, rflu_gpu_step) runs the coupled stage of
+ * libfloflu/RungeKuttaMP.F90:219-280 with PICL_TEMP_Runge between
+ * SourceTermsMP and RKUpdateMP; picl_gpu_runge is that call on its own, for
+ * hosts that drive the stage routine by routine.
+ *
+ * Physics restated (rocpicl/, libpicl/): QS Parmar drag, AM Parmar added mass
+ * and the pressure-gradient force on an Euler flow, Sutherland or fixed
+ * viscosity, IDW-27 interpolation, single-element box projection, two-way
+ * feedback.  Requirements: all-hex grid, spaceOrder 2, patchOrder 1.
+ *
+ * Particle arrays use ppiclF's layout: y(PPICLF_LRS=12, n) -> y[12*p + j]
+ * (x, y, z, vx, vy, vz, T, ox, oy, oz, metal, oxide).  Element ids are Rocflu
+ * cell indices (0-based, the host's numbering); -1 marks a particle that lies
+ * in no element (ppiclF removes such particles; here they stay in place and
+ * are skipped).
+ */
+#ifndef PICL_GPU_H
+#define PICL_GPU_H
+#include "rflu_gpu.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The ppiclF user-physics switches the path reads
+ * (rocpicl/PICL_TEMP_InitSolver.F90:139-219, common /RFLU_ppiclF/). */
+typedef struct picl_params {
+  int rmuFlag;       /* 0: fixed viscosity rmuRef; 1: Sutherland */
+  double rmuRef;     /* reference viscosity */
+  double tref;       /* Sutherland reference temperature */
+  double suth;       /* Sutherland constant */
+  double rcpPart;    /* particle heat capacity */
+  int qsFlag;        /* 1: QS Parmar (the only one built) */
+  int amFlag;        /* 0 off, 1 AM Parmar */
+  int pgFlag;        /* 0 off, 1 pressure-gradient force */
+  int feedbackFlag;  /* 0 one-way, 1 two-way */
+} picl_params;
+
+/* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):
+ * ppiclf_solve_InitParticle (libpicl/ppiclF/source/ppiclf_solve.f:184-188)
+ * + ppiclf_comm_InitOverlapMesh (ppiclf_comm.f:926-998, its InitSolve
+ * projects the particles), then piclVF from the projection, cv scaled by
+ * (1 - piclVF) and dv recomputed.  The handle's state must hold the pure-gas
+ * cv (rflu_gpu_set_state).  y: 12 doubles per particle; pconst: (rho_p, d_p,
+ * V_p, SPL) per particle. */
+int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double *y, const double *pconst);
+
+/* PICL_TEMP_Runge (rocpicl/PICL_TEMP_Runge.F90:623-1234) for RK stage
+ * istage: field assembly, ppiclf_solve_IntegrateParticle
+ * (ppiclf_solve.f:1694-1936: locate, LocalInterp, CreateGhost map,
+ * ProjectParticleGrid, SetYdot, RK3), feedback into rhs and the new piclVF.
+ * Call after rflu_gpu_source_terms and before rflu_gpu_rk_update. */
+int picl_gpu_runge(rflu_handle *h, int istage);
+
+int picl_gpu_sizes(const rflu_handle *h, int *nPart, int *nLost);
+/* Particle state (any pointer may be NULL): y, y1, ydot, ydotc 12 per
+ * particle; elem one per particle (-1: lost). */
+int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, double *ydotc, int *elem);
+/* Per cell (any pointer may be NULL): piclVF; limited grad(rho_g) (3,
+ * piclgradRhog); projected node values of PHIP, FX, FY, FZ, E (5,
+ * ppiclf_pro_fld via GetProFldIJKEF, ppiclf_solve.f:3630-3647); the 20
+ * interpolation inputs of the last stage, in rprop order 2,6,7,8,10,11,12,9,
+ * 24,5,13,14,15,16,17,18,19,37,38,39 (PPICLF_USER.h). */
+int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *proFld, double *fields);
+/* Values LocalInterp produced at the last stage, 20 per particle in the
+ * order above (kept only while picl_gpu_set_debug(h, 1) is on). */
+int picl_gpu_get_interp(rflu_handle *h, double *rprop20);
+int picl_gpu_set_debug(rflu_handle *h, int on);
+/* Element holding each of n points (-1: none), by the library's location
+ * rule (lowest-numbered element whose trilinear map contains the point). */
+int picl_gpu_locate(rflu_handle *h, int n, const double *xyz, int *elem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

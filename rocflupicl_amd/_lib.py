@@ -21,6 +21,12 @@ RFLU_SYMBOLS = [
     "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time", "rflu_gpu_bandwidth_probe",
 ]
 
+# Every symbol include/picl_gpu.h declares.
+PICL_SYMBOLS = [
+    "picl_gpu_init", "picl_gpu_runge", "picl_gpu_sizes", "picl_gpu_get_particles", "picl_gpu_get_cells",
+    "picl_gpu_get_interp", "picl_gpu_set_debug", "picl_gpu_locate",
+]
+
 ERRORS = {1: "EINVAL", 2: "ENODEV", 3: "EHIP", 4: "EMESH", 5: "ENAN", 6: "ENEGATIVE"}
 
 
@@ -37,6 +43,12 @@ class Params(C.Structure):
                 ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
                 ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
                 ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int)]
+
+
+class PiclParams(C.Structure):
+    _fields_ = [("rmuFlag", C.c_int), ("rmuRef", C.c_double), ("tref", C.c_double), ("suth", C.c_double),
+                ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
+                ("feedbackFlag", C.c_int)]
 
 
 _lib = None
@@ -76,7 +88,15 @@ def load():
     L.rflu_gpu_set_profiling.argtypes = [P, I]
     L.rflu_gpu_kernel_time.argtypes = [P, C.c_char_p, P, P]
     L.rflu_gpu_bandwidth_probe.argtypes = [C.c_size_t, I, P]
-    for n in RFLU_SYMBOLS:
+    L.picl_gpu_init.argtypes = [P, C.POINTER(PiclParams), I, P, P]
+    L.picl_gpu_runge.argtypes = [P, I]
+    L.picl_gpu_sizes.argtypes = [P, P, P]
+    L.picl_gpu_get_particles.argtypes = [P, P, P, P, P, P]
+    L.picl_gpu_get_cells.argtypes = [P, P, P, P, P]
+    L.picl_gpu_get_interp.argtypes = [P, P]
+    L.picl_gpu_set_debug.argtypes = [P, I]
+    L.picl_gpu_locate.argtypes = [P, I, P, P]
+    for n in RFLU_SYMBOLS + PICL_SYMBOLS:
         if n != "rflu_gpu_last_error":
             getattr(L, n).restype = I
     _lib = L

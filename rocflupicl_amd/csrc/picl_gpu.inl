@@ -1,0 +1,1083 @@
+// MI355X (gfx950) kernels and C ABI of the ppiclF coupling (include/picl_gpu.h).
+// Included at the end of rflu_gpu.hip: one translation unit, so the particle
+// stage shares the region's handle, stream, graph and device helpers.
+//
+// Layout in HBM:
+//   particles (original ppiclF order, SoA): position, velocity, the RK
+//     register y1 and ydot for those six components, ydotc (vx, vy, vz, T) of
+//     the previous stage (the projection map) and of this stage, constants
+//     (rho_p, d_p, V_p, SPL), element id;
+//   plist: particle indices grouped by element, ascending within an element
+//     (a stable counting sort rebuilt every stage).  The particle kernel walks
+//     this list, so neighbouring lanes share their element's neighbourhood in
+//     L2; the deposit walks it to sum each element's particles in ppiclF's
+//     order;
+//   cells (device numbering): piclVF, rho_g, grad rho_g (raw and final, 4
+//     doubles per cell), the 20 interpolation inputs (160 B per cell), the 5
+//     projected fields (SoA), 1/V_hex, ppiclF centroid (4 per cell), the 8
+//     nodes (24 per cell) and bounding box for point location;
+//   two uniform bin grids: element bounding boxes (location) and element
+//     centroids (interpolation neighbours).
+//
+// Integer results (element ids, neighbour sets and order, summation order of
+// the deposit) follow the oracle's rules exactly.  The drag law calls pow,
+// exp and log, whose device versions may differ from glibc by an ulp, so the
+// particle state is compared to the oracle with a tolerance.
+
+#define PICL_NFLD 20
+#define F32(x) ((double)(x##f))
+
+namespace {
+
+struct PiclPhys {
+  int rmuFlag, amFlag, pgFlag, feedbackFlag;
+  double rmuRef, tref, suth, rcpPart;
+};
+
+struct PiclDev {
+  int nP = 0, nC = 0;
+  double *px = nullptr, *py = nullptr, *pz = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *y1 = nullptr;   // [6][nP]
+  double *yd = nullptr;   // [6][nP]
+  double *ycP = nullptr;  // [4][nP] ydotc of the previous stage
+  double *ycN = nullptr;  // [4][nP] ydotc of this stage
+  double *pc = nullptr;   // [4][nP] rho_p, d_p, V_p, SPL
+  int *elem = nullptr;
+  int *plist = nullptr, *plistN = nullptr, *cnt = nullptr, *off = nullptr, *bsum = nullptr, *nAct = nullptr;
+  double *fld = nullptr;   // [nC][20]
+  double *gRho = nullptr;  // [nC][4]
+  double *proj = nullptr;  // [5][nC]
+  double *rexp = nullptr, *cen = nullptr, *X = nullptr, *bbox = nullptr;
+  int *refId = nullptr, *inv = nullptr;
+  int lbN[3] = {1, 1, 1};
+  double lbLo[3] = {0, 0, 0}, lbH[3] = {1, 1, 1};
+  int *lbOff = nullptr, *lbMem = nullptr;
+  int cbN[3] = {1, 1, 1};
+  double cbLo[3] = {0, 0, 0}, cbH[3] = {1, 1, 1}, R[3] = {0, 0, 0};
+  int *cbOff = nullptr, *cbMem = nullptr;
+  double d2Max[3] = {0, 0, 0};
+  double *dbg = nullptr;  // [20][nP] interpolated values (debug)
+};
+
+// ---------------------------------------------------------------------------
+// Point location: the lowest-numbered element whose trilinear map holds the
+// point within |r|,|s|,|t| <= 1 + 1e-10 (Newton from the element centre);
+// the stand-in for gslib findpts.  Same operations as the oracle.
+// ---------------------------------------------------------------------------
+__device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3]) {
+  double r = 0.0, s = 0.0, t = 0.0;
+  for (int it = 0; it < 25; it++) {
+    double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
+    double b[2] = {0.5 * (1.0 - s), 0.5 * (1.0 + s)};
+    double c[2] = {0.5 * (1.0 - t), 0.5 * (1.0 + t)};
+    const double da[2] = {-0.5, 0.5};
+    double x[3] = {0.0, 0.0, 0.0};
+    double J[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const int i = n & 1, j = (n >> 1) & 1, k = n >> 2;
+      double w = a[i] * b[j] * c[k];
+      double wr = da[i] * b[j] * c[k], ws = a[i] * da[j] * c[k], wt = a[i] * b[j] * da[k];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        double v = X[3 * n + d];
+        x[d] = x[d] + w * v;
+        J[d][0] = J[d][0] + wr * v;
+        J[d][1] = J[d][1] + ws * v;
+        J[d][2] = J[d][2] + wt * v;
+      }
+    }
+    double f0 = xp[0] - x[0], f1 = xp[1] - x[1], f2 = xp[2] - x[2];
+    double m00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double m01 = J[1][0] * J[2][2] - J[1][2] * J[2][0];
+    double m02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * m00 - J[0][1] * m01 + J[0][2] * m02;
+    if (!(fabs(det) > 0.0)) return false;
+    double dr = (f0 * m00 - J[0][1] * (f1 * J[2][2] - J[1][2] * f2) + J[0][2] * (f1 * J[2][1] - J[1][1] * f2)) / det;
+    double ds = (J[0][0] * (f1 * J[2][2] - J[1][2] * f2) - f0 * m01 + J[0][2] * (J[1][0] * f2 - f1 * J[2][0])) / det;
+    double dt = (J[0][0] * (J[1][1] * f2 - f1 * J[2][1]) - J[0][1] * (J[1][0] * f2 - f1 * J[2][0]) + f0 * m02) / det;
+    r = r + dr; s = s + ds; t = t + dt;
+    if (fabs(dr) + fabs(ds) + fabs(dt) < 1.0e-14) break;
+  }
+  const double lim = 1.0 + 1.0e-10;
+  return fabs(r) <= lim && fabs(s) <= lim && fabs(t) <= lim;
+}
+
+__device__ int picl_locate(const PiclDev &P, const double xp[3]) {
+  if (!(isfinite(xp[0]) && isfinite(xp[1]) && isfinite(xp[2]))) return -1;
+  int b[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    double t = (xp[l] - P.lbLo[l]) / P.lbH[l];
+    t = fmin(fmax(t, 0.0), (double)(P.lbN[l] - 1));
+    b[l] = (int)t;
+  }
+  int bin = b[0] + P.lbN[0] * (b[1] + P.lbN[1] * b[2]);
+  for (int q = P.lbOff[bin]; q < P.lbOff[bin + 1]; q++) {
+    int e = P.lbMem[q];
+    const double *bb = P.bbox + 8 * (size_t)e;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double tol = 1.0e-9 * (bb[4 + d] - bb[d]);
+      if (xp[d] < bb[d] - tol || xp[d] > bb[4 + d] + tol) in = false;
+    }
+    if (in && trilinear_inside(P.X + 24 * (size_t)e, xp)) return e;
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Gradient of rho_g: WENO-XYZ (modflu/RFLU_ModWENO.F90:1402-1588) over the
+// cell and its stencil in stencil order, then the positivity limiter
+// (modflu/RFLU_ModLimiters.F90:793-921) over the cell's faces.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_rhog(Dev d, PiclDev P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
+  int sb = d.csOff[i], se = d.csOff[i + 1];
+  double g[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double term = d.gRaw[4 * (size_t)i + k];
+    double sum = 1.0 / (term * term + 1.0E-15);
+    double loc = sum * term;
+    for (int s = sb; s < se; s++) {
+      term = d.gRaw[4 * (size_t)(mem[s] & CS_IDX) + k];
+      double w = 1.0 / (term * term + 1.0E-15);
+      sum = sum + w;
+      loc = loc + w * term;
+    }
+    g[k] = loc / sum;
+  }
+  int N = d.nC;
+  double varc = d.rhog[i];
+  double xc = d.cofg[i], yc = d.cofg[N + i], zc = d.cofg[2 * N + i];
+  bool zero = false;
+  for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+    int slot = d.cfIdx[e] >> 1;
+    double dx = d.fcx[slot] - xc, dy = d.fcy[slot] - yc, dz = d.fcz[slot] - zc;
+    double varf = varc + g[0] * dx + g[1] * dy + g[2] * dz;
+    if (varf <= 0.0) zero = true;
+  }
+  if (zero) g[0] = g[1] = g[2] = 0.0;
+  double2 *o = reinterpret_cast<double2 *>(P.gRho + 4 * (size_t)i);
+  o[0] = make_double2(g[0], g[1]);
+  o[1] = make_double2(g[2], 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// PICL_TEMP_Runge field assembly (rocpicl/PICL_TEMP_Runge.F90:712-842): one
+// value per element, already averaged over its 8 identical node copies the way
+// LocalInterp does (sum of eight, divided by 8, ppiclf_solve.f:2363-2373).
+// The cv the reference reads here has been through CellGradientsMP's and
+// SourceTermsMP's round trips; they are replayed in registers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sum8(double v) {
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) s = s + v;
+  return s;
+}
+__device__ __forceinline__ double phip_of(double vfP, double vol) {
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) s = s + (0.125 * vfP) * vol;
+  s = s / vol;
+  if (s > F32(0.62)) s = F32(0.62);
+  return s;
+}
+
+__global__ void __launch_bounds__(BLK) k_picl_fields(Dev d, Gas G, PiclDev P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double r[8];
+  load_rec(d.rec, i, r);
+  double vol = d.vol[i], phi = d.phi[i];
+  double vfg = 1.0 - phi;
+  double rho = r[0];
+  double rP = rho / vfg * vfg;                    // CellGradientsMP rho_g round trip
+  double ir = 1.0 / rP;                           // SourceTermsMP cons->prim ...
+  double uB = ir * (rho * r[1]), vB = ir * (rho * r[2]), wB = ir * (rho * r[3]);
+  double cvB[4] = {rP, rP * uB, rP * vB, rP * wB};  // ... and prim->cons
+  const double *g = d.grad + 15 * (size_t)i;
+  double gg[15];
+#pragma unroll
+  for (int q = 0; q < 15; q++) gg[q] = g[q];
+  double rh[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) rh[k] = d.rhs[k * N + i];
+  double ug[3] = {cvB[1] / cvB[0], cvB[2] / cvB[0], cvB[3] / cvB[0]};
+  double sdr[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    double dot = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) dot = dot + ug[k] * gg[3 * (1 + m) + k];
+    sdr[m] = (-(rh[1 + m] / vol) + ug[m] * rh[0] / vol) / cvB[0] + dot;
+  }
+  double T = r[4] / (rho / (1.0 - phi) * G.gcD);  // dv from RFLU_SetDependentVars
+  double f[PICL_NFLD];
+  f[0] = cvB[0] / (1.0 - phi);
+  f[1] = cvB[1] / cvB[0];
+  f[2] = cvB[2] / cvB[0];
+  f[3] = cvB[3] / cvB[0];
+  f[4] = gg[12]; f[5] = gg[13]; f[6] = gg[14];
+  f[7] = sqrt(G.gD * G.gcD * T);
+  f[8] = T;
+  f[9] = phip_of(P.proj[i], vol);
+  f[10] = sdr[0]; f[11] = sdr[1]; f[12] = sdr[2];
+  f[13] = -(rh[0] / vol);
+  f[14] = gg[0]; f[15] = gg[1]; f[16] = gg[2];
+  const double *q = P.gRho + 4 * (size_t)i;
+  f[17] = q[0]; f[18] = q[1]; f[19] = q[2];
+  double2 *o = reinterpret_cast<double2 *>(P.fld + PICL_NFLD * (size_t)i);
+#pragma unroll
+  for (int k = 0; k < PICL_NFLD / 2; k++) o[k] = make_double2(sum8(f[2 * k]) / 8, sum8(f[2 * k + 1]) / 8);
+}
+
+// ---------------------------------------------------------------------------
+// Forces: ppiclf_user_QS_Parmar (libpicl/user_files/ppiclf_user_QuasiSteady.f:22-122);
+// real*4 literals widened as the reference's compiler does.
+// ---------------------------------------------------------------------------
+__device__ double qs_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  double mp = fmax(rmachp, 0.01), re = fmax(rep, 0.1), rcd1;
+  if (re < F32(1E-14)) {
+    rcd1 = 1.0;
+  } else {
+    double rmacr = F32(0.6);
+    double rcd_mcr = (1.0 + F32(0.15) * pow(re, F32(0.684))) +
+                     (re / 24.0) * (F32(0.513) / (1.0 + 483.0 / pow(re, F32(0.669))));
+    if (mp <= rmacr) {
+      double rcd_std = (1.0 + F32(0.15) * pow(re, F32(0.687))) +
+                       (re / 24.0) * (F32(0.42) / (1.0 + 42500.0 / pow(re, F32(1.16))));
+      double rmach_rat = mp / rmacr;
+      rcd1 = rcd_std + (rcd_mcr - rcd_std) * rmach_rat;
+    } else {
+      double rcd_M1 = (1.0 + F32(0.118) * pow(re, F32(0.813))) +
+                      (re / 24.0) * F32(0.69) / (1.0 + 3550.0 / pow(re, F32(.793)));
+      double m2 = mp * mp, m3 = mp * mp * mp;
+      double lrep = log(re);
+      if (mp <= 1.0) {
+        double C1 = F32(6.48), C2 = F32(9.28), C3 = F32(12.21);
+        double f1M = -F32(1.884) + F32(8.422) * mp - F32(13.70) * m2 + F32(8.162) * m3;
+        double f2M = -F32(2.228) + F32(10.35) * mp - F32(16.96) * m2 + F32(9.840) * m3;
+        double f3M = F32(4.362) - F32(16.91) * mp + F32(19.84) * m2 - F32(6.296) * m3;
+        double factor = f1M * (lrep - C2) * (lrep - C3) / ((C1 - C2) * (C1 - C3)) +
+                        f2M * (lrep - C1) * (lrep - C3) / ((C2 - C1) * (C2 - C3)) +
+                        f3M * (lrep - C1) * (lrep - C2) / ((C3 - C1) * (C3 - C2));
+        rcd1 = rcd_mcr + (rcd_M1 - rcd_mcr) * factor;
+      } else {
+        double rcd_M2 = (1.0 + F32(0.107) * pow(re, F32(0.867))) +
+                        (re / 24.0) * F32(0.646) / (1.0 + 861.0 / pow(re, F32(.634)));
+        if (mp < 1.75) {
+          double C1 = F32(6.48), C2 = F32(8.93), C3 = F32(12.21);
+          double f1M = -F32(2.963) + F32(4.392) * mp - F32(1.169) * m2 - F32(0.027) * m3 -
+                       F32(0.233) * exp((1.0 - mp) / F32(0.011));
+          double f2M = -F32(6.617) + F32(12.11) * mp - F32(6.501) * m2 + F32(1.182) * m3 -
+                       F32(0.174) * exp((1.0 - mp) / F32(0.010));
+          double f3M = -F32(5.866) + F32(11.57) * mp - F32(6.665) * m2 + F32(1.312) * m3 -
+                       F32(0.350) * exp((1.0 - mp) / F32(0.012));
+          double factor = f1M * (lrep - C2) * (lrep - C3) / ((C1 - C2) * (C1 - C3)) +
+                          f2M * (lrep - C1) * (lrep - C3) / ((C2 - C1) * (C2 - C3)) +
+                          f3M * (lrep - C1) * (lrep - C2) / ((C3 - C1) * (C3 - C2));
+          rcd1 = rcd_M1 + (rcd_M2 - rcd_M1) * factor;
+        } else {
+          rcd1 = rcd_M2;
+        }
+      }
+    }
+  }
+  double phi_corr = 1.0 + F32(5.94) * fmin(rphip, 0.5);
+  double beta = rcd1 * 3.0 * rpi * rmu * dp;
+  return beta * phi_corr;
+}
+
+// ---------------------------------------------------------------------------
+// One particle per lane, in element-grouped order: locate, IDW-27
+// interpolation (ppiclf_solve.f:2234-2409), ppiclf_user_SetYdot
+// (libpicl/user_files/ppiclf_user.f:39-1042 with QS Parmar, AM Parmar
+// ppiclf_user_AddedMass.f:16-124, pressure gradient; Euler flow) and the
+// IntegrateRK3s_Rocflu update (ppiclf_solve.f:1845-1936, 2774-2793).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool key_less(double a, int ra, double b, int rb) {
+  return a < b || (a == b && ra < rb);
+}
+
+__global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  const int nP = P.nP;
+  int p = P.plist[t];
+  double xp[3] = {P.px[p], P.py[p], P.pz[p]};
+  int e = picl_locate(P, xp);
+  P.elem[p] = e;
+  if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
+#pragma unroll
+    for (int k = 0; k < 4; k++) P.ycP[k * (size_t)nP + p] = 0.0;
+    return;
+  }
+  atomicAdd(&P.cnt[e], 1);
+
+  // ---- the 27 nearest element centroids, ordered by (d2, element number)
+  double kd[27];
+  int kr[27];
+#pragma unroll
+  for (int k = 0; k < 27; k++) { kd[k] = F32(1E20); kr[k] = -1; }
+  int nadd = 0;
+  // bins reached within the per-axis radius; the outer bins also hold every
+  // centroid beyond the grid (bin indices are clamped when binning)
+  int blo[3], bhi[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    double top = (double)(P.cbN[l] - 1);
+    double a = (xp[l] - P.R[l] - P.cbLo[l]) / P.cbH[l], b = (xp[l] + P.R[l] - P.cbLo[l]) / P.cbH[l];
+    a = fmin(fmax(a, 0.0), top);
+    b = fmin(fmax(b, 0.0), top);
+    blo[l] = (int)a;
+    bhi[l] = (int)b;
+  }
+  {
+    for (int bz = blo[2]; bz <= bhi[2]; bz++)
+      for (int by = blo[1]; by <= bhi[1]; by++)
+        for (int bx = blo[0]; bx <= bhi[0]; bx++) {
+          int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
+          for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
+            int c = P.cbMem[q];
+            const double2 *cp = reinterpret_cast<const double2 *>(P.cen + 4 * (size_t)c);
+            double2 c01 = cp[0], c2_ = cp[1];
+            double t0 = c01.x - xp[0], t1 = c01.y - xp[1], t2 = c2_.x - xp[2];
+            double l0 = t0 * t0, l1 = t1 * t1, l2 = t2 * t2;
+            if (l0 > P.d2Max[0] || l1 > P.d2Max[1] || l2 > P.d2Max[2]) continue;
+            double d2i = 0.0;
+            d2i = d2i + l0;
+            d2i = d2i + l1;
+            d2i = d2i + l2;
+            int rc = P.refId[c];
+            if (!key_less(d2i, rc, kd[26], kr[26]) || !(d2i < F32(1E20))) {
+              if (d2i < F32(1E20)) nadd++;
+              continue;
+            }
+            nadd++;
+            // insertion into the sorted registers (constant indices after unrolling)
+            double cd = d2i;
+            int cr = rc;
+#pragma unroll
+            for (int j = 26; j >= 0; j--) {
+              bool before = (j == 0) ? false : key_less(cd, cr, kd[j - 1], kr[j - 1]);
+              if (j == 26 || key_less(cd, cr, kd[j], kr[j])) {
+                if (before) {
+                  kd[j] = kd[j - 1];
+                  kr[j] = kr[j - 1];
+                } else {
+                  kd[j] = cd;
+                  kr[j] = cr;
+                }
+              }
+            }
+          }
+        }
+  }
+  int nn = nadd < 27 ? nadd : 27;
+  if (nn < 1) {  // ppiclf_solve.f:2331-2341 stops the run here
+    atomicOr(d.err, E_INTERP);
+    return;
+  }
+  double v[PICL_NFLD];
+#pragma unroll
+  for (int f = 0; f < PICL_NFLD; f++) v[f] = 0.0;
+  double wsum = 0.0;
+  const double eps = F32(1.0e-12);
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    if (k < nn) {
+      double w = 1.0 / (sqrt(kd[k]) + eps);
+      const double2 *fp = reinterpret_cast<const double2 *>(P.fld + PICL_NFLD * (size_t)P.inv[kr[k]]);
+#pragma unroll
+      for (int f = 0; f < PICL_NFLD / 2; f++) {
+        double2 a = fp[f];
+        v[2 * f] = v[2 * f] + w * a.x;
+        v[2 * f + 1] = v[2 * f + 1] + w * a.y;
+      }
+      wsum = wsum + w;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < PICL_NFLD; f++) v[f] = v[f] / wsum;
+  if (P.dbg) {
+#pragma unroll
+    for (int f = 0; f < PICL_NFLD; f++) P.dbg[f * (size_t)nP + p] = v[f];
+  }
+  // slots: 0 rhof 1-3 u 4-6 dp/dx 7 c 8 T 9 phi 10-12 Du/Dt 13 rhsR 14-16 grad(rho phi) 17-19 grad rho_g
+
+  // ---- SetYdot
+  double yv[3] = {P.vx[p], P.vy[p], P.vz[p]};
+  double rhop = P.pc[p], dpp = P.pc[(size_t)nP + p], volp = P.pc[2 * (size_t)nP + p], spl = P.pc[3 * (size_t)nP + p];
+  double rmu;
+  if (Ph.rmuFlag == 0) {
+    rmu = Ph.rmuRef;
+  } else {
+    double temp = v[8];
+    rmu = Ph.rmuRef * sqrt(temp / Ph.tref) * (1.0 + Ph.suth / Ph.tref) / (1.0 + Ph.suth / temp);
+  }
+  double rmass = volp * rhop;
+  double vxr = v[1] - yv[0], vyr = v[2] - yv[1], vzr = v[3] - yv[2];
+  double vmag = sqrt(vxr * vxr + vyr * vyr + vzr * vzr);
+  double rhof = v[0];
+  double rep = vmag * dpp * rhof / rmu;
+  double rphip = v[9];
+  double rmachp = vmag / v[7];
+  double ydn[6];
+  bool skip = vmag <= 1.0e-3;  // keeps the previous ydot; ydotc stays 0
+  double yc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (!skip) {
+    rep = fmax(0.1, rep);
+    rphip = fmin(rphip, 0.62);
+    double rphif = 1.0 - rphip;
+    double famx = 0.0, famy = 0.0, famz = 0.0, rmass_add = 0.0;
+    double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
+    double beta = qs_parmar(rmachp, rphip, rep, rmu, dpp);
+    double fqsx = beta * vxr, fqsy = beta * vyr, fqsz = beta * vzr;
+    fqsx = fqsx + 0.0;
+    fqsy = fqsy + 0.0;
+    fqsz = fqsz + 0.0;
+    if (Ph.amFlag == 1) {
+      double rcd_am;
+      if (rmachp > F32(0.6)) {
+        rcd_am = 0x1.5104d6p+1;  // 1.0 + 1.8*((0.6)**2) + 7.6*((0.6)**4) folded in single precision
+      } else {
+        double m2 = rmachp * rmachp;
+        rcd_am = 1.0 + F32(1.8) * m2 + F32(7.6) * (m2 * m2);
+      }
+      rcd_am = rcd_am * 0.5;
+      rcd_am = rcd_am * (1.0 + F32(0.68) * (rphip * rphip));
+      rmass_add = rhof * volp * rcd_am;
+      double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
+      SDrho = SDrho / rphif;
+      double vgradrho = vxr * v[17] + vyr * v[18] + vzr * v[19];
+      famx = rcd_am * volp * (vxr * SDrho + rhof * v[10] + v[1] * vgradrho);
+      famy = rcd_am * volp * (vyr * SDrho + rhof * v[11] + v[2] * vgradrho);
+      famz = rcd_am * volp * (vzr * SDrho + rhof * v[12] + v[3] * vgradrho);
+    }
+    if (Ph.pgFlag == 1) {
+      fdpdx = -volp * v[4];
+      fdpdy = -volp * v[5];
+      fdpdz = -volp * v[6];
+      fdpdx = fdpdx + 0.0;
+      fdpdy = fdpdy + 0.0;
+      fdpdz = fdpdz + 0.0;
+    }
+    ydn[0] = yv[0];
+    ydn[1] = yv[1];
+    ydn[2] = yv[2];
+    ydn[3] = (fqsx + famx + fdpdx + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
+    ydn[4] = (fqsy + famy + fdpdy + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
+    ydn[5] = (fqsz + famz + fdpdz + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
+    if (Ph.feedbackFlag == 1) {
+      yc[0] = spl * (ydn[3] * rmass - 0.0);
+      yc[1] = spl * (ydn[4] * rmass - 0.0);
+      yc[2] = spl * (ydn[5] * rmass - 0.0);
+      yc[3] = spl * ((fqsx + 0.0 + 0.0) * yv[0] + (fqsy + 0.0 + 0.0) * yv[1] + (fqsz + 0.0 + 0.0) * yv[2] +
+                     famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) P.yd[k * (size_t)nP + p] = ydn[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) ydn[k] = P.yd[k * (size_t)nP + p];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) P.ycN[k * (size_t)nP + p] = yc[k];
+
+  // ---- RK3 (Rocflu coefficients), y = -c1*y1 + y + c3*ydot, y1 = ydot
+  double dt = d.scal[S_DTMIN];
+  double k0, k2;
+  if (istage == 1) { k0 = 0.0; k2 = dt * 8.0 / 15.0; }
+  else if (istage == 2) { k0 = dt * 17.0 / 60.0; k2 = dt * 5.0 / 12.0; }
+  else { k0 = dt * 5.0 / 12.0; k2 = dt * 3.0 / 4.0; }
+  double *yy[6] = {P.px, P.py, P.pz, P.vx, P.vy, P.vz};
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    double y1 = istage == 1 ? 0.0 : P.y1[k * (size_t)nP + p];
+    double y = yy[k][p];
+    yy[k][p] = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
+    P.y1[k * (size_t)nP + p] = ydn[k];
+  }
+}
+
+// Locate only (the set-up's InitSolve: no fields, no forces).
+__global__ void __launch_bounds__(BLK) k_locate_init(PiclDev P) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  int p = P.plist[t];
+  double xp[3] = {P.px[p], P.py[p], P.pz[p]};
+  int e = picl_locate(P, xp);
+  P.elem[p] = e;
+  if (e >= 0) atomicAdd(&P.cnt[e], 1);
+}
+
+__global__ void __launch_bounds__(BLK) k_locate_pts(PiclDev P, int n, const double *xyz, int *out) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= n) return;
+  double xp[3] = {xyz[3 * (size_t)t], xyz[3 * (size_t)t + 1], xyz[3 * (size_t)t + 2]};
+  int e = picl_locate(P, xp);
+  out[t] = e < 0 ? -1 : P.refId[e];
+}
+
+__global__ void __launch_bounds__(BLK) k_zero_int(int *a, int n) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of the per-element counts (cnt[0..n], cnt[n] = 0) into off;
+// the counts are cleared for use as fill cursors.  4096 entries per block.
+// ---------------------------------------------------------------------------
+#define SCAN_T 1024
+#define SCAN_V 4
+__global__ void __launch_bounds__(SCAN_T) k_scan1(const int *cnt, int n, int *bsum) {
+  __shared__ int sm[SCAN_T / 64];
+  size_t base = (size_t)blockIdx.x * SCAN_T * SCAN_V + (size_t)threadIdx.x * SCAN_V;
+  int s = 0;
+  for (int k = 0; k < SCAN_V; k++)
+    if (base + k < (size_t)n) s += cnt[base + k];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int q = 0; q < SCAN_T / 64; q++) tot += sm[q];
+    bsum[blockIdx.x] = tot;
+  }
+}
+__global__ void __launch_bounds__(SCAN_T) k_scan2(int *bsum, int nb) {
+  __shared__ int sm[SCAN_T];
+  int carry = 0;
+  for (int c0 = 0; c0 < nb; c0 += SCAN_T) {
+    int i = c0 + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_T; o <<= 1) {
+      int a = threadIdx.x >= o ? sm[threadIdx.x - o] : 0;
+      __syncthreads();
+      sm[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + sm[threadIdx.x] - v;
+    int tot = sm[SCAN_T - 1];
+    __syncthreads();
+    carry += tot;
+  }
+}
+__global__ void __launch_bounds__(SCAN_T) k_scan3(int *cnt, int n, const int *bsum, int *off) {
+  __shared__ int sm[SCAN_T];
+  size_t base = (size_t)blockIdx.x * SCAN_T * SCAN_V + (size_t)threadIdx.x * SCAN_V;
+  int v[SCAN_V], s = 0;
+  for (int k = 0; k < SCAN_V; k++) {
+    v[k] = base + k < (size_t)n ? cnt[base + k] : 0;
+    s += v[k];
+  }
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < SCAN_T; o <<= 1) {
+    int a = threadIdx.x >= o ? sm[threadIdx.x - o] : 0;
+    __syncthreads();
+    sm[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int run = bsum[blockIdx.x] + sm[threadIdx.x] - s;
+  for (int k = 0; k < SCAN_V; k++)
+    if (base + k < (size_t)n) {
+      off[base + k] = run;
+      run += v[k];
+      cnt[base + k] = 0;
+    }
+}
+
+// Particles into their element's slots (arbitrary order within an element).
+__global__ void __launch_bounds__(BLK) k_fill(PiclDev P) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  int p = P.plist[t];
+  int e = P.elem[p];
+  if (e < 0) return;
+  int slot = atomicAdd(&P.cnt[e], 1);
+  P.plistN[P.off[e] + slot] = p;
+}
+
+// ---------------------------------------------------------------------------
+// ProjectParticleGrid, single-element box mode (ppiclf_solve.f:3285-3447):
+// each element sums map * (1/V_hex) of its particles in ascending particle
+// index; the map (ppiclf_user_MapProjPart.f:37-66) is built from the previous
+// stage's ydotc (ppiclf_comm.f:1190-1217 runs before SetYdot).  The sorted
+// segment becomes the next stage's particle order, and this stage's ydotc
+// becomes the next map.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_deposit(PiclDev P) {
+  int e = xcd_block() * BLK + threadIdx.x;
+  if (e >= P.nC) return;
+  const int nP = P.nP;
+  int b = P.off[e], n = P.off[e + 1] - b;
+  int *L = P.plistN + b;
+  for (int a = 1; a < n; a++) {  // insertion sort: segments hold tens of particles
+    int x = L[a], j = a - 1;
+    while (j >= 0 && L[j] > x) { L[j + 1] = L[j]; j--; }
+    L[j + 1] = x;
+  }
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double rexp = P.rexp[e];
+  for (int q = 0; q < n; q++) {
+    int p = L[q];
+    double spl = P.pc[3 * (size_t)nP + p];
+    double map[5];
+    map[0] = P.pc[2 * (size_t)nP + p] * spl;
+#pragma unroll
+    for (int k = 0; k < 4; k++) map[1 + k] = P.ycP[k * (size_t)nP + p] * spl;
+#pragma unroll
+    for (int k = 0; k < 5; k++) acc[k] = acc[k] + (map[k] * 1.0) * rexp;
+#pragma unroll
+    for (int k = 0; k < 4; k++) P.ycP[k * (size_t)nP + p] = P.ycN[k * (size_t)nP + p];
+    P.plist[b + q] = p;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) P.proj[k * (size_t)P.nC + e] = 0.0 + acc[k];
+  if (e == P.nC - 1) P.nAct[0] = P.off[P.nC];
+}
+
+// Rocflu's read-back (PICL_TEMP_Runge.F90:926-1038, 1217-1234): feedback into
+// rhs, and piclVF from the projected volume.
+__global__ void __launch_bounds__(BLK) k_feedback(Dev d, PiclDev P, int feedback) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double vol = d.vol[i];
+  if (feedback) {
+#pragma unroll
+    for (int k = 1; k < 5; k++) {
+      double jf = sum8(P.proj[k * (size_t)N + i]) * 0.125 * vol;
+      d.rhs[k * N + i] = d.rhs[k * N + i] + jf;
+    }
+  }
+  d.phi[i] = phip_of(P.proj[i], vol);
+}
+
+// PICL_TEMP_InitSolver.F90:789-835: piclVF from the first projection and
+// cv scaled to the bulk state.
+__global__ void __launch_bounds__(BLK) k_picl_init_cells(Dev d, PiclDev P) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double phi = phip_of(P.proj[i], d.vol[i]);
+  d.phi[i] = phi;
+  double vFrac = 1.0 - phi;
+#pragma unroll
+  for (int k = 0; k < 5; k++) d.cv[k * N + i] = vFrac * d.cv[k * N + i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+const int NODE2V[8] = {0, 3, 4, 7, 1, 2, 5, 6};  // ppiclF node n = i+2j+4k <- hex2v 1,4,5,8,2,3,6,7
+
+// Single-element box volume (ppiclf_solve.f:3285-3394), same operations as the oracle.
+double hex_volume(const double *X) {
+  double centroid[3];
+  for (int ix = 0; ix < 3; ix++) centroid[ix] = 0.0;
+  for (int ix = 0; ix < 3; ix++)
+    for (int n = 0; n < 8; n++) centroid[ix] = centroid[ix] + X[3 * n + ix];
+  for (int ix = 0; ix < 3; ix++) centroid[ix] = centroid[ix] / 8.0;
+  double evol = 0.0;
+  for (int ix = 0; ix < 3; ix++) {
+    int ia = (3 - (ix + 1) > 1 ? 3 - (ix + 1) : 1) - 1;
+    int ib = (5 - (ix + 1) < 3 ? 5 - (ix + 1) : 3) - 1;
+    for (int iface = 0; iface < 2; iface++) {
+      double face[2][2][3];
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+          int a[3];
+          a[ix] = iface; a[ia] = i; a[ib] = j;
+          int n = a[0] + 2 * a[1] + 4 * a[2];
+          for (int dd = 0; dd < 3; dd++) face[i][j][dd] = X[3 * n + dd];
+        }
+      double v1[3], v2[3], v3[3], cross[3];
+      for (int dd = 0; dd < 3; dd++) {
+        v1[dd] = face[0][1][dd] - face[1][0][dd];
+        v2[dd] = centroid[dd] - face[1][0][dd];
+      }
+      cross[0] = v1[1] * v2[2] - v1[2] * v2[1];
+      cross[1] = v1[2] * v2[0] - v1[0] * v2[2];
+      cross[2] = v1[0] * v2[1] - v1[1] * v2[0];
+      for (int inode = 0; inode < 2; inode++) {
+        for (int dd = 0; dd < 3; dd++) v3[dd] = face[inode][inode][dd] - face[1][0][dd];
+        double voltet = 0.0;
+        for (int dd = 0; dd < 3; dd++) voltet = voltet + v3[dd] * cross[dd];
+        evol = evol + fabs(voltet);
+      }
+    }
+  }
+  return evol / 6.0;
+}
+
+}  // namespace
+
+struct PiclState {
+  PiclDev P;
+  PiclPhys phys;
+  std::vector<double> y0;  // the 12 components as given (constant ones are returned as is)
+  bool debug = false;
+};
+
+namespace {
+
+void picl_free(PiclState *s) {
+  if (!s) return;
+  PiclDev &P = s->P;
+  void *ptrs[] = {P.px, P.y1, P.yd, P.ycP, P.ycN, P.pc, P.elem, P.plist, P.plistN, P.cnt, P.off, P.bsum, P.nAct,
+                  P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
+                  P.cbMem, P.dbg};
+  for (void *q : ptrs) if (q) (void)hipFree(q);
+  delete s;
+}
+
+int scan_blocks(int n) { return (n + SCAN_T * SCAN_V - 1) / (SCAN_T * SCAN_V); }
+
+// Counting sort of the located particles by element, then the deposit.
+void picl_sort_and_deposit(rflu_handle *h) {
+  PiclDev &P = h->picl->P;
+  int n = P.nC + 1, nb = scan_blocks(n);
+  {
+    ProfScope ps(h, "k_scan");
+    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(SCAN_T), 0, h->stream, P.cnt, n, P.bsum);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(SCAN_T), 0, h->stream, P.bsum, nb);
+    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(SCAN_T), 0, h->stream, P.cnt, n, P.bsum, P.off);
+  }
+  {
+    ProfScope ps(h, "k_fill");
+    hipLaunchKernelGGL(k_fill, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
+  }
+  ProfScope ps(h, "k_deposit");
+  hipLaunchKernelGGL(k_deposit, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
+}
+
+int launch_rhog(rflu_handle *h) {
+  ProfScope ps(h, "k_rhog");
+  hipLaunchKernelGGL(k_rhog, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->picl->P);
+  return 0;
+}
+
+// PICL_TEMP_Runge for one stage (after SourceTermsMP, before RKUpdateMP).
+int picl_stage(rflu_handle *h, int istage) {
+  PiclState *s = h->picl;
+  PiclDev &P = s->P;
+  {
+    ProfScope ps(h, "k_picl_fields");
+    hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P);
+  }
+  hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nC + 1)), dim3(BLK), 0, h->stream, P.cnt, P.nC + 1);
+  {
+    ProfScope ps(h, "k_particles");
+    hipLaunchKernelGGL(k_particles, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys, istage);
+  }
+  picl_sort_and_deposit(h);
+  ProfScope ps(h, "k_feedback");
+  hipLaunchKernelGGL(k_feedback, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P, s->phys.feedbackFlag);
+  return 0;
+}
+
+// Uniform bin grid over boxes [lo, hi] (element bounding boxes or points):
+// members are appended in the order given.
+void build_bins(const std::vector<double> &blo, const std::vector<double> &bhi, const std::vector<int> &order,
+                const double lo[3], const double H[3], const int nb[3], std::vector<int> &off,
+                std::vector<int> &mem) {
+  size_t nbin = (size_t)nb[0] * nb[1] * nb[2];
+  std::vector<std::vector<int>> lists(nbin);
+  for (int e : order) {
+    int a[3], b[3];
+    for (int l = 0; l < 3; l++) {
+      double ta = (blo[3 * (size_t)e + l] - lo[l]) / H[l], tb = (bhi[3 * (size_t)e + l] - lo[l]) / H[l];
+      ta = std::min(std::max(ta, 0.0), (double)(nb[l] - 1));
+      tb = std::min(std::max(tb, 0.0), (double)(nb[l] - 1));
+      a[l] = (int)ta;
+      b[l] = (int)tb;
+    }
+    for (int z = a[2]; z <= b[2]; z++)
+      for (int y = a[1]; y <= b[1]; y++)
+        for (int x = a[0]; x <= b[0]; x++) lists[x + (size_t)nb[0] * (y + (size_t)nb[1] * z)].push_back(e);
+  }
+  off.assign(nbin + 1, 0);
+  for (size_t q = 0; q < nbin; q++) off[q + 1] = off[q] + (int)lists[q].size();
+  mem.clear();
+  mem.reserve(off[nbin]);
+  for (auto &l : lists) mem.insert(mem.end(), l.begin(), l.end());
+  if (mem.empty()) mem.push_back(0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double *y, const double *pconst) {
+  if (!h || !pp || nPart < 0 || (nPart > 0 && (!y || !pconst))) return RFLU_GPU_EINVAL;
+  if (h->picl) return set_err(h, RFLU_GPU_EINVAL, "particles are already attached to this region");
+  const rflu::MeshHost &m = h->mesh;
+  if (m.nvc != 8) return set_err(h, RFLU_GPU_EINVAL, "PICL needs an all-hex grid (PICL_TEMP_InitSolver.F90:562-597)");
+  if (h->prm.spaceOrder != 2) return set_err(h, RFLU_GPU_EINVAL, "PICL needs spaceOrder 2 (its fields use gradients)");
+  if (h->prm.patchOrder != 1) return set_err(h, RFLU_GPU_EINVAL, "PICL is built for patchOrder 1");
+  if (pp->qsFlag != 1) return set_err(h, RFLU_GPU_EINVAL, "only QS Parmar (qsFlag 1) is built");
+  if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
+  if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
+  const int N = h->d.nC, nP = nPart;
+  PiclState *s = new PiclState();
+  PiclDev &P = s->P;
+  P.nP = nP;
+  P.nC = N;
+  s->phys = PiclPhys{pp->rmuFlag, pp->amFlag, pp->pgFlag, pp->feedbackFlag, pp->rmuRef, pp->tref, pp->suth,
+                     pp->rcpPart};
+  s->y0.assign(y, y + 12 * (size_t)nP);
+  // element geometry in device numbering
+  std::vector<double> X(24 * (size_t)N), bbox(8 * (size_t)N, 0.0), cen(4 * (size_t)N, 0.0), rexp(N);
+  std::vector<double> blo(3 * (size_t)N), bhi(3 * (size_t)N);
+  std::vector<int> refId(N), inv(N);
+  double lastExt2[3] = {0, 0, 0}, maxExt[3] = {0, 0, 0}, glo[3] = {1e300, 1e300, 1e300}, ghi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < N; i++) {
+    int o = h->perm[i];
+    refId[i] = o;
+    inv[o] = i;
+    double *Xe = X.data() + 24 * (size_t)i;
+    for (int n = 0; n < 8; n++)
+      for (int dd = 0; dd < 3; dd++) Xe[3 * n + dd] = h->xyz[3 * (size_t)h->c2v[8 * (size_t)o + NODE2V[n]] + dd];
+    for (int l = 0; l < 3; l++) {
+      double c = 0.0, mx = -1.0e10, mn = 1.0e10;
+      for (int n = 0; n < 8; n++) {
+        c = c + Xe[3 * n + l];
+        if (Xe[3 * n + l] > mx) mx = Xe[3 * n + l];
+        if (Xe[3 * n + l] < mn) mn = Xe[3 * n + l];
+      }
+      cen[4 * (size_t)i + l] = c / 8;
+      bbox[8 * (size_t)i + l] = mn;
+      bbox[8 * (size_t)i + 4 + l] = mx;
+      double tol = 1.0e-9 * (mx - mn);
+      blo[3 * (size_t)i + l] = mn - 2.0 * tol;
+      bhi[3 * (size_t)i + l] = mx + 2.0 * tol;
+      maxExt[l] = std::max(maxExt[l], mx - mn);
+      glo[l] = std::min(glo[l], mn);
+      ghi[l] = std::max(ghi[l], mx);
+      if (o == N - 1) {  // LocalInterp keeps the last element's extent (ppiclf_solve.f:2263-2289)
+        double d2i = (mx - mn) * (mx - mn), d2m = 0.0;
+        if (d2i > d2m) d2m = d2i;
+        lastExt2[l] = d2m;
+      }
+    }
+    rexp[i] = 1.0 / hex_volume(Xe);
+  }
+  for (int l = 0; l < 3; l++) P.d2Max[l] = lastExt2[l] * (1.5 * 1.5);
+  // location bins: one maximal element extent per bin, members in reference order
+  std::vector<int> refOrder(N);
+  for (int o = 0; o < N; o++) refOrder[o] = inv[o];
+  for (int l = 0; l < 3; l++) {
+    P.lbLo[l] = glo[l];
+    P.lbH[l] = maxExt[l] > 0 ? maxExt[l] : 1.0;
+    P.lbN[l] = std::max(1, std::min(4096, (int)std::floor((ghi[l] - glo[l]) / P.lbH[l]) + 1));
+  }
+  std::vector<int> lbOff, lbMem, cbOff, cbMem;
+  build_bins(blo, bhi, refOrder, P.lbLo, P.lbH, P.lbN, lbOff, lbMem);
+  // centroid bins: width = the interpolation reach per axis
+  std::vector<double> cpts(3 * (size_t)N);
+  double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < N; i++)
+    for (int l = 0; l < 3; l++) {
+      cpts[3 * (size_t)i + l] = cen[4 * (size_t)i + l];
+      clo[l] = std::min(clo[l], cen[4 * (size_t)i + l]);
+      chi[l] = std::max(chi[l], cen[4 * (size_t)i + l]);
+    }
+  for (int l = 0; l < 3; l++) {
+    double r = std::sqrt(P.d2Max[l]);
+    P.R[l] = r * (1.0 + 1.0e-9) + 1.0e-300;
+    P.cbLo[l] = clo[l];
+    P.cbH[l] = r > 0 ? r : (maxExt[l] > 0 ? maxExt[l] : 1.0);
+    P.cbN[l] = std::max(1, std::min(4096, (int)std::floor((chi[l] - clo[l]) / P.cbH[l]) + 1));
+  }
+  std::vector<int> idOrder(N);
+  for (int i = 0; i < N; i++) idOrder[i] = i;
+  build_bins(cpts, cpts, idOrder, P.cbLo, P.cbH, P.cbN, cbOff, cbMem);
+  // particles, SoA
+  std::vector<double> pos(6 * (size_t)nP), pcs(4 * (size_t)nP);
+  for (int p = 0; p < nP; p++) {
+    for (int k = 0; k < 6; k++) pos[k * (size_t)nP + p] = y[12 * (size_t)p + k];
+    for (int k = 0; k < 4; k++) pcs[k * (size_t)nP + p] = pconst[4 * (size_t)p + k];
+  }
+  std::vector<int> ident(nP);
+  for (int p = 0; p < nP; p++) ident[p] = p;
+  int rc = 0;
+  rc |= dalloc(h, &P.px, 6 * (size_t)nP, pos.data());
+  rc |= dalloc(h, &P.y1, 6 * (size_t)nP);
+  rc |= dalloc(h, &P.yd, 6 * (size_t)nP);
+  rc |= dalloc(h, &P.ycP, 4 * (size_t)nP);
+  rc |= dalloc(h, &P.ycN, 4 * (size_t)nP);
+  rc |= dalloc(h, &P.pc, 4 * (size_t)nP, pcs.data());
+  rc |= dalloc(h, &P.elem, (size_t)nP);
+  rc |= dalloc(h, &P.plist, (size_t)nP, ident.data());
+  rc |= dalloc(h, &P.plistN, (size_t)nP);
+  rc |= dalloc(h, &P.cnt, (size_t)N + 1);
+  rc |= dalloc(h, &P.off, (size_t)N + 1);
+  rc |= dalloc(h, &P.bsum, (size_t)scan_blocks(N + 1));
+  rc |= dalloc(h, &P.nAct, 1, &nP);
+  rc |= dalloc(h, &P.fld, PICL_NFLD * (size_t)N);
+  rc |= dalloc(h, &P.gRho, 4 * (size_t)N);
+  rc |= dalloc(h, &P.proj, 5 * (size_t)N);
+  rc |= dalloc(h, &P.rexp, (size_t)N, rexp.data());
+  rc |= dalloc(h, &P.cen, 4 * (size_t)N, cen.data());
+  rc |= dalloc(h, &P.X, 24 * (size_t)N, X.data());
+  rc |= dalloc(h, &P.bbox, 8 * (size_t)N, bbox.data());
+  rc |= dalloc(h, &P.refId, (size_t)N, refId.data());
+  rc |= dalloc(h, &P.inv, (size_t)N, inv.data());
+  rc |= dalloc(h, &P.lbOff, lbOff.size(), lbOff.data());
+  rc |= dalloc(h, &P.lbMem, lbMem.size(), lbMem.data());
+  rc |= dalloc(h, &P.cbOff, cbOff.size(), cbOff.data());
+  rc |= dalloc(h, &P.cbMem, cbMem.size(), cbMem.data());
+  rc |= dalloc(h, &h->d.phi, (size_t)N);
+  rc |= dalloc(h, &h->d.rhog, (size_t)N);
+  rc |= dalloc(h, &h->d.gRaw, 4 * (size_t)N);
+  if (rc) { std::string msg = h->err; picl_free(s); return set_err(h, RFLU_GPU_EHIP, msg); }
+  P.py = P.px + nP; P.pz = P.px + 2 * (size_t)nP;
+  P.vx = P.px + 3 * (size_t)nP; P.vy = P.px + 4 * (size_t)nP; P.vz = P.px + 5 * (size_t)nP;
+  HIPCHK(h, hipMemsetAsync(P.y1, 0, 6 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.yd, 0, 6 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.ycP, 0, 4 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.ycN, 0, 4 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d.gRaw, 0, 4 * (size_t)N * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.gRho, 0, 4 * (size_t)N * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.fld, 0, PICL_NFLD * (size_t)N * sizeof(double), h->stream));
+  h->picl = s;
+  if (h->graph) { hipGraphExecDestroy(h->graph); h->graph = nullptr; }  // the step changes
+  // ppiclf_comm_InitOverlapMesh -> InitSolve: locate and project (ydotc = 0)
+  hipLaunchKernelGGL(k_zero_int, dim3(nblk(N + 1)), dim3(BLK), 0, h->stream, P.cnt, N + 1);
+  hipLaunchKernelGGL(k_locate_init, dim3(nblk(nP)), dim3(BLK), 0, h->stream, P);
+  picl_sort_and_deposit(h);
+  hipLaunchKernelGGL(k_picl_init_cells, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P);
+  hipLaunchKernelGGL(k_records, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int picl_gpu_runge(rflu_handle *h, int istage) {
+  if (!h || istage < 1 || istage > 3) return RFLU_GPU_EINVAL;
+  if (!h->picl) return set_err(h, RFLU_GPU_EINVAL, "no particles attached (picl_gpu_init)");
+  picl_stage(h, istage);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int picl_gpu_sizes(const rflu_handle *h, int *nPart, int *nLost) {
+  if (!h || !h->picl) return RFLU_GPU_EINVAL;
+  const PiclDev &P = h->picl->P;
+  int act = 0;
+  if (hipMemcpy(&act, P.nAct, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return RFLU_GPU_EHIP;
+  if (nPart) *nPart = P.nP;
+  if (nLost) *nLost = P.nP - act;
+  return 0;
+}
+
+int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, double *ydotc, int *elem) {
+  if (!h || !h->picl) return RFLU_GPU_EINVAL;
+  PiclState *s = h->picl;
+  PiclDev &P = s->P;
+  size_t nP = P.nP;
+  std::vector<double> a(6 * nP);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  auto fill = [&](double *dst, const double *src, int ncomp, int first, bool keepConst) -> int {
+    if (!dst) return 0;
+    HIPCHK(h, hipMemcpy(a.data(), src, ncomp * nP * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < nP; p++) {
+      for (int j = 0; j < 12; j++) dst[12 * p + j] = keepConst ? s->y0[12 * p + j] : 0.0;
+      for (int k = 0; k < ncomp; k++) dst[12 * p + first + k] = a[k * nP + p];
+    }
+    return 0;
+  };
+  if (int e = fill(y, P.px, 6, 0, true)) return e;
+  if (int e = fill(y1, P.y1, 6, 0, false)) return e;
+  if (int e = fill(ydot, P.yd, 6, 0, false)) return e;
+  if (int e = fill(ydotc, P.ycP, 4, 3, false)) return e;
+  if (elem) {
+    std::vector<int> ev(nP);
+    HIPCHK(h, hipMemcpy(ev.data(), P.elem, nP * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < nP; p++) elem[p] = ev[p] < 0 ? -1 : h->perm[ev[p]];
+  }
+  return 0;
+}
+
+int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *proFld, double *fields) {
+  if (!h || !h->picl) return RFLU_GPU_EINVAL;
+  PiclDev &P = h->picl->P;
+  size_t N = P.nC;
+  if (phi)
+    if (int e = from_device_cells(h, phi, h->d.phi, 1, false)) return e;
+  if (gradRhog) {
+    std::vector<double> g4(4 * N);
+    if (int e = from_device_cells(h, g4.data(), P.gRho, 4, true)) return e;
+    for (size_t i = 0; i < N; i++)
+      for (int k = 0; k < 3; k++) gradRhog[3 * i + k] = g4[4 * i + k];
+  }
+  if (proFld) {
+    std::vector<double> pf(5 * N);
+    if (int e = from_device_cells(h, pf.data(), P.proj, 5, false)) return e;
+    for (size_t i = 0; i < N; i++)
+      for (int k = 0; k < 5; k++) proFld[5 * i + k] = pf[k * N + i];
+  }
+  if (fields)
+    if (int e = from_device_cells(h, fields, P.fld, PICL_NFLD, true)) return e;
+  return 0;
+}
+
+int picl_gpu_set_debug(rflu_handle *h, int on) {
+  if (!h || !h->picl) return RFLU_GPU_EINVAL;
+  PiclDev &P = h->picl->P;
+  if (on && !P.dbg) {
+    if (dalloc(h, &P.dbg, PICL_NFLD * (size_t)P.nP)) return RFLU_GPU_EHIP;
+    HIPCHK(h, hipMemset(P.dbg, 0, PICL_NFLD * (size_t)P.nP * sizeof(double)));
+  } else if (!on && P.dbg) {
+    (void)hipFree(P.dbg);
+    P.dbg = nullptr;
+  }
+  if (h->graph) { hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+  return 0;
+}
+
+int picl_gpu_get_interp(rflu_handle *h, double *rprop20) {
+  if (!h || !h->picl || !rprop20) return RFLU_GPU_EINVAL;
+  PiclDev &P = h->picl->P;
+  if (!P.dbg) return set_err(h, RFLU_GPU_EINVAL, "interpolated values are kept only with picl_gpu_set_debug(h, 1)");
+  size_t nP = P.nP;
+  std::vector<double> a(PICL_NFLD * nP);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(a.data(), P.dbg, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t p = 0; p < nP; p++)
+    for (int f = 0; f < PICL_NFLD; f++) rprop20[PICL_NFLD * p + f] = a[f * nP + p];
+  return 0;
+}
+
+int picl_gpu_locate(rflu_handle *h, int n, const double *xyz, int *elem) {
+  if (!h || !h->picl || n < 0 || (n > 0 && (!xyz || !elem))) return RFLU_GPU_EINVAL;
+  if (n == 0) return 0;
+  double *dx = nullptr;
+  int *de = nullptr;
+  if (dalloc(h, &dx, 3 * (size_t)n, xyz) || dalloc(h, &de, (size_t)n)) {
+    if (dx) (void)hipFree(dx);
+    return RFLU_GPU_EHIP;
+  }
+  hipLaunchKernelGGL(k_locate_pts, dim3(nblk(n)), dim3(BLK), 0, h->stream, h->picl->P, n, dx, de);
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = hipMemcpyAsync(elem, de, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+  hipError_t e3 = hipStreamSynchronize(h->stream);
+  (void)hipFree(dx);
+  (void)hipFree(de);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return set_err(h, RFLU_GPU_EHIP, "picl_gpu_locate failed");
+  return 0;
+}
+
+}  // extern "C"

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/rflu_gpu.h"
+#include "../../include/picl_gpu.h"
 #include "rflu_mesh.hpp"
 
 #define BLK 256
@@ -57,20 +58,27 @@ struct Dev {
   double *cv = nullptr, *rhsSum = nullptr, *rhs = nullptr, *rec = nullptr, *grad = nullptr;
   double *flux = nullptr, *srad = nullptr, *dtCell = nullptr, *dvOut = nullptr, *partial = nullptr, *scal = nullptr, *cofg = nullptr;
   int *err = nullptr;
+  // ppiclF coupling (null without particles): piclVF, rho_g = rho/(1-phi)
+  // and its raw LSQ gradient (4 doubles per cell, the 4th unused)
+  double *phi = nullptr, *rhog = nullptr, *gRaw = nullptr;
 };
 
 // scal[] slots
 enum { S_DTLOCAL = 0, S_DTMIN = 1, S_TIME = 2, S_TSR = 3 };
 // err bits
-enum { E_NAN = 1, E_NEG = 2, E_POSFIX = 4, E_BOUNDFIX = 8 };
+enum { E_NAN = 1, E_NEG = 2, E_POSFIX = 4, E_BOUNDFIX = 8, E_INTERP = 16 };
 // boundary kinds on device: 0 slip wall, 1..3 outflow (1 + bcOpt)
 }  // namespace
 
+struct PiclState;
 struct rflu_handle {
+  PiclState *picl = nullptr;  // particles attached by picl_gpu_init
   rflu_params prm;
   Gas gas;
   Dev d;
   rflu::MeshHost mesh;
+  std::vector<double> xyz;  // the grid as given (the particle set-up reads the hex nodes)
+  std::vector<int> c2v;
   hipStream_t stream = nullptr;
   std::string err;
   hipGraphExec_t graph = nullptr;
@@ -111,15 +119,19 @@ __device__ __forceinline__ void store_rec(double *__restrict__ rec, int i, const
   p[3] = make_double2(r[6], r[7]);
 }
 
-// libflu/RFLU_SetDependentVars.F90:256-287 (TCPERF, ksg = 0)
+// libflu/RFLU_SetDependentVars.F90:256-287 (TCPERF, ksg = 0).  With
+// particles the gas-phase density rho/(1-phi) replaces rho in p, T, c
+// (:276-281); Eo and |u|^2 still use rho.
 struct DV { double p, T, c; };
-__device__ __forceinline__ DV dep_vars(const Gas &G, double rho, double m1, double m2, double m3, double E) {
+__device__ __forceinline__ DV dep_vars(const Gas &G, double rho, double m1, double m2, double m3, double E,
+                                       const double *phi = nullptr, int i = 0) {
   double ir = 1.0 / rho;
   double Eo = ir * E;
   double Vm2 = ir * ir * (m1 * m1 + m2 * m2 + m3 * m3);
+  double r = phi ? rho / (1.0 - phi[i]) : rho;
   DV d;
-  d.p = (G.gD - 1.0) * rho * (Eo - 0.5 * Vm2 - 0.0);
-  d.T = d.p / (rho * G.gcD);
+  d.p = (G.gD - 1.0) * r * (Eo - 0.5 * Vm2 - 0.0);
+  d.T = d.p / (r * G.gcD);
   d.c = sqrt(G.gD * G.gcD * d.T);
   return d;
 }
@@ -141,10 +153,10 @@ __device__ __forceinline__ void make_rec(int order, double rho, double m1, doubl
   }
 }
 
-// Speed of sound of a cell from its record.
-__device__ __forceinline__ double rec_c(const Gas &G, int order, const double r[8]) {
+// Speed of sound of a cell from its record (and its phi with particles).
+__device__ __forceinline__ double rec_c(const Gas &G, int order, const double r[8], double phi = 0.0) {
   if (order == 1) return r[7];
-  double T = r[4] / (r[0] * G.gcD);
+  double T = r[4] / (r[0] / (1.0 - phi) * G.gcD);
   return sqrt(G.gD * G.gcD * T);
 }
 // Velocity the reference's dt loop uses: cv(m)*(1/cv(rho)) == cons2prim u.
@@ -178,10 +190,11 @@ __global__ void __launch_bounds__(BLK) k_records(Dev d, Gas G, int order) {
   if (i >= d.nC) return;
   int N = d.nC;
   double rho = d.cv[i], m1 = d.cv[N + i], m2 = d.cv[2 * N + i], m3 = d.cv[3 * N + i], E = d.cv[4 * N + i];
-  DV dv = dep_vars(G, rho, m1, m2, m3, E);
+  DV dv = dep_vars(G, rho, m1, m2, m3, E, d.phi, i);
   double r[8];
   make_rec(order, rho, m1, m2, m3, E, dv, d.cofg[i], d.cofg[N + i], d.cofg[2 * N + i], r);
   store_rec(d.rec, i, r);
+  if (d.rhog) d.rhog[i] = rho / (1.0 - d.phi[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,14 +221,15 @@ __device__ double minimum_time_step(const DtPar &P, double dtLocal, double t, do
 // interior faces, :264-285 for boundary faces; static grid, fs = 0).  The
 // expression is symmetric in the two cells, as in the reference.
 __device__ __forceinline__ double face_srad(const Gas &G, int order, const double L[8], const double *R,
-                                            double nx, double ny, double nz, double nm) {
+                                            double nx, double ny, double nz, double nm,
+                                            double phiL = 0.0, double phiR = 0.0) {
   double u1, v1, w1;
   rec_u(order, L, u1, v1, w1);
-  double a1 = rec_c(G, order, L);
+  double a1 = rec_c(G, order, L, phiL);
   if (R) {
     double u2, v2, w2;
     rec_u(order, R, u2, v2, w2);
-    double a2 = rec_c(G, order, R);
+    double a2 = rec_c(G, order, R, phiR);
     double vc = 0.5 * ((u1 + u2) * nx + (v1 + v2) * ny + (w1 + w2) * nz);
     double as = 0.5 * (a1 + a2);
     return (fabs(vc - 0.0) + as) * nm;
@@ -229,12 +243,15 @@ __global__ void __launch_bounds__(BLK) k_srad(Dev d, Gas G, int order) {
   if (s >= d.S) return;
   double L[8], R[8];
   if (s < d.nF) {
-    load_rec(d.rec, d.f2c[2 * s], L);
-    load_rec(d.rec, d.f2c[2 * s + 1], R);
-    d.srad[s] = face_srad(G, order, L, R, d.fnx[s], d.fny[s], d.fnz[s], d.fnm[s]);
+    int c1 = d.f2c[2 * s], c2 = d.f2c[2 * s + 1];
+    load_rec(d.rec, c1, L);
+    load_rec(d.rec, c2, R);
+    double p1 = d.phi ? d.phi[c1] : 0.0, p2 = d.phi ? d.phi[c2] : 0.0;
+    d.srad[s] = face_srad(G, order, L, R, d.fnx[s], d.fny[s], d.fnz[s], d.fnm[s], p1, p2);
   } else {
-    load_rec(d.rec, d.bf2c[s - d.nF], L);
-    d.srad[s] = face_srad(G, order, L, nullptr, d.fnx[s], d.fny[s], d.fnz[s], d.fnm[s]);
+    int c1 = d.bf2c[s - d.nF];
+    load_rec(d.rec, c1, L);
+    d.srad[s] = face_srad(G, order, L, nullptr, d.fnx[s], d.fny[s], d.fnz[s], d.fnm[s], d.phi ? d.phi[c1] : 0.0);
   }
 }
 
@@ -393,7 +410,10 @@ __global__ void __launch_bounds__(BLK) k_grad(Dev d, int venkat, int guard) {
 // fetched while the current one is processed.
 #define CS_FACE 0x80000000u
 #define CS_IDX 0x7fffffffu
-template <int MINW>
+// PICL = 1 also forms the LSQ gradient of rho_g = rho/(1-phi) over the same
+// stencil (CellGradientsMP.F90:375-378, before its WENO and positivity
+// passes in k_rhog).
+template <int MINW, int PICL>
 __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guard) {
   int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
@@ -417,11 +437,15 @@ __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guar
   unsigned jn = mem[sb];
   double rn[8];
   load_rec(d.rec, (int)(jn & CS_IDX), rn);
+  double gq0 = 0.0, gq1 = 0.0, gq2 = 0.0, qi = 0.0;
+  if (PICL) qi = d.rhog[i];
   for (int s = sb; s < se; s++) {
     double rj[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) rj[q] = rn[q];
     unsigned j = jn;
+    double qj = 0.0;
+    if (PICL) qj = d.rhog[j & CS_IDX];
     if (s + 1 < se) {
       jn = mem[s + 1];
       load_rec(d.rec, (int)(jn & CS_IDX), rn);
@@ -441,6 +465,12 @@ __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guar
       g[3 * k] = g[3 * k] + wtx * dVar;
       g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
       g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    }
+    if (PICL) {
+      double dq = qj - qi;
+      gq0 = gq0 + wtx * dq;
+      gq1 = gq1 + wty * dq;
+      gq2 = gq2 + wtz * dq;
     }
     if (j & CS_FACE) {
 #pragma unroll
@@ -482,6 +512,11 @@ __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guar
   double *o = d.grad + 15 * (size_t)i;
 #pragma unroll
   for (int q = 0; q < 15; q++) o[q] = g[q];
+  if (PICL) {
+    double2 *gr = reinterpret_cast<double2 *>(d.gRaw + 4 * (size_t)i);
+    gr[0] = make_double2(gq0, gq1);
+    gr[1] = make_double2(gq2, 0.0);
+  }
 }
 
 // Cooperative variant 3: a workgroup of 256 lanes takes 16 cells.
@@ -850,18 +885,21 @@ __device__ __forceinline__ void hllc_flux(const Gas &G, double nx, double ny, do
 // Face state reconstructed at (xc,yc,zc) from an order-2 record and gradient;
 // u is taken through the reference's prim->cons->prim round trip
 // (RFLU_ModConvertCv.F90:554-592 then RFLU_ModAUSMFlux.F90:2735-2783).
+// rhoP is cv(1) as the flux loop reads it: rho itself, or with particles
+// (rho/(1-phi))*(1-phi) after CellGradientsMP's rho_g round trip
+// (CellGradientsMP.F90:371-390), while rho*u was formed from rho before it.
 struct FaceState { double r, u, v, w, p, ir; };
 __device__ __forceinline__ FaceState reconstruct(const double rc[8], const double *__restrict__ gp,
-                                                 double xc, double yc, double zc) {
+                                                 double xc, double yc, double zc, double rhoP) {
   FaceState s;
   double rho = rc[0];
-  double ir = 1.0 / rho;
+  double ir = 1.0 / rhoP;
   double u = (rho * rc[1]) * ir, v = (rho * rc[2]) * ir, w = (rho * rc[3]) * ir, p = rc[4];
   double dx = xc - rc[5], dy = yc - rc[6], dz = zc - rc[7];
   double g[15];
 #pragma unroll
   for (int q = 0; q < 15; q++) g[q] = gp[q];
-  s.r = rho + g[0] * dx + g[1] * dy + g[2] * dz;
+  s.r = rhoP + g[0] * dx + g[1] * dy + g[2] * dz;
   s.u = u + g[3] * dx + g[4] * dy + g[5] * dz;
   s.v = v + g[6] * dx + g[7] * dy + g[8] * dz;
   s.w = w + g[9] * dx + g[10] * dy + g[11] * dz;
@@ -911,7 +949,15 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
     double L[8], R[8];
     load_rec(d.rec, c1, L);
     load_rec(d.rec, c2, R);
-    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, R, nx, ny, nz, nm);
+    double phL = 0.0, phR = 0.0, rpL = L[0], rpR = R[0];
+    if (d.phi) {
+      phL = d.phi[c1];
+      phR = d.phi[c2];
+      double vl = 1.0 - phL, vr = 1.0 - phR;
+      rpL = L[0] / vl * vl;
+      rpR = R[0] / vr * vr;
+    }
+    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, R, nx, ny, nz, nm, phL, phR);
     if (P.order == 1) {
       double rl = L[0], irl = 1.0 / rl, ul = L[1] * irl, vl = L[2] * irl, wl = L[3] * irl;
       double pl = L[5], tl = L[6], al = L[7];
@@ -933,8 +979,8 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
       }
     } else {
       double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
-      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, xc, yc, zc);
-      FaceState B = reconstruct(R, d.grad + 15 * (size_t)c2, xc, yc, zc);
+      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, xc, yc, zc, rpL);
+      FaceState B = reconstruct(R, d.grad + 15 * (size_t)c2, xc, yc, zc, rpR);
       if (P.discr == RFLU_DISCR_AUSM) {
         double tl = A.p / (A.r * G.gc), al = sqrt(G.g * G.gc * tl);
         double Hl = G.cp * tl + 0.5 * (A.u * A.u + A.v * A.v + A.w * A.w) + ksg;
@@ -963,17 +1009,24 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
     int c1 = d.bf2c[b], kind = d.bkind[b];
     double L[8];
     load_rec(d.rec, c1, L);
-    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, nullptr, nx, ny, nz, nm);
+    double phL = d.phi ? d.phi[c1] : 0.0;
+    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, nullptr, nx, ny, nz, nm, phL);
     double rl, rul, rvl, rwl, rel, pl;
     if (P.patchOrder == 1) {
       if (P.order == 1) {
         rl = L[0]; rul = L[1]; rvl = L[2]; rwl = L[3]; rel = L[4]; pl = L[5];
       } else {  // cv after CellGradientsMP's prim->cons conversion
         rl = L[0]; rul = L[0] * L[1]; rvl = L[0] * L[2]; rwl = L[0] * L[3]; pl = L[4];
-        rel = L[0] * eo_dgpuvw(L[0], G.gD, L[4], L[1], L[2], L[3]);
+        double rE = L[0];
+        if (d.phi) {  // energy from rho/(1-phi) (RFLU_ModConvertCv.F90:584-586), then the rho_g round trip
+          double vl = 1.0 - phL;
+          rE = L[0] / vl;
+          rl = L[0] / vl * vl;
+        }
+        rel = L[0] * eo_dgpuvw(rE, G.gD, L[4], L[1], L[2], L[3]);
       }
     } else {
-      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, d.fcx[s], d.fcy[s], d.fcz[s]);
+      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, d.fcx[s], d.fcy[s], d.fcz[s], L[0]);
       rl = A.r; pl = A.p;
       rul = rl * A.u; rvl = rl * A.v; rwl = rl * A.w;
       rel = (kind != 0) ? rl * eo_dgpuvw(rl, G.gD, pl, A.u, A.v, A.w) : 0.0;
@@ -1026,11 +1079,17 @@ __device__ __forceinline__ void gather_rhs(const Dev &d, int i, double rhs[5]) {
   }
 }
 
-// libfloflu/SourceTerms.F90:127-136 on the cv that SourceTermsMP sees
-__device__ __forceinline__ void gravity(const Gas &G, int order, const double r[8], double vol, double rhs[5]) {
+// libfloflu/SourceTerms.F90:127-136 on the cv that SourceTermsMP sees (with
+// particles, cv(1) carries CellGradientsMP's rho_g round trip)
+__device__ __forceinline__ void gravity(const Gas &G, int order, const double r[8], double vol, double rhs[5],
+                                        const double *phi = nullptr, int i = 0) {
   double rho = r[0], m1, m2, m3;
   if (order == 1) { m1 = r[1]; m2 = r[2]; m3 = r[3]; }
   else { m1 = rho * r[1]; m2 = rho * r[2]; m3 = rho * r[3]; }
+  if (phi) {
+    double vf = 1.0 - phi[i];
+    rho = rho / vf * vf;
+  }
   double rhoVol = vol * rho;
   rhs[1] = rhs[1] - G.accel[0] * rhoVol;
   rhs[2] = rhs[2] - G.accel[1] * rhoVol;
@@ -1076,7 +1135,7 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
   }
   // RFLU_SetDependentVars, then AfterUpdateMP: RFLU_CheckValidity,
   // RFLU_EnforceBounds, RFLU_CheckPositivity (libflu/*.F90, TCPERF branches)
-  DV dv = dep_vars(G, cv[0], cv[1], cv[2], cv[3], cv[4]);
+  DV dv = dep_vars(G, cv[0], cv[1], cv[2], cv[3], cv[4], d.phi, i);
   {
     double rho = cv[0], rrho = 1.0 / rho;
     double u = rrho * cv[1], v = rrho * cv[2], w = rrho * cv[3];
@@ -1110,6 +1169,7 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
   double r[8];
   make_rec(P.order, cv[0], cv[1], cv[2], cv[3], cv[4], dv, d.cofg[i], d.cofg[N + i], d.cofg[2 * N + i], r);
   store_rec(d.rec, i, r);
+  if (d.rhog) d.rhog[i] = cv[0] / (1.0 - d.phi[i]);
 }
 
 // ConvectiveFluxesMP / SourceTermsMP as separate steps (rhs kept in HBM).
@@ -1130,7 +1190,7 @@ __global__ void __launch_bounds__(BLK) k_source(Dev d, Gas G, int order) {
 #pragma unroll
   for (int k = 0; k < 5; k++) rhs[k] = d.rhs[k * N + i];
   load_rec(d.rec, i, r);
-  gravity(G, order, r, d.vol[i], rhs);
+  gravity(G, order, r, d.vol[i], rhs, d.phi, i);
 #pragma unroll
   for (int k = 0; k < 5; k++) d.rhs[k * N + i] = rhs[k];
 }
@@ -1151,7 +1211,7 @@ __global__ void __launch_bounds__(BLK) k_dv_out(Dev d, Gas G, int order, double 
   if (order == 1) {
     dv[i] = r[5]; dv[N + i] = r[6]; dv[2 * N + i] = r[7];
   } else {
-    double T = r[4] / (r[0] * G.gcD);
+    double T = r[4] / (r[0] / (1.0 - (d.phi ? d.phi[i] : 0.0)) * G.gcD);
     dv[i] = r[4]; dv[N + i] = T; dv[2 * N + i] = sqrt(G.gD * G.gcD * T);
   }
 }
@@ -1254,14 +1314,16 @@ int launch_grad(rflu_handle *h) {
   if (h->prm.spaceOrder < 2) return 0;
   ProfScope ps(h, "k_grad");
   int venkat = h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, guard = h->prm.limiterGuard ? 1 : 0;
-  if (h->gradVariant == 2)
-    hipLaunchKernelGGL(k_grad2<1>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  if (h->picl)
+    hipLaunchKernelGGL((k_grad2<1, 1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+  else if (h->gradVariant == 2)
+    hipLaunchKernelGGL((k_grad2<1, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 24)
-    hipLaunchKernelGGL(k_grad2<4>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<4, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 25)
-    hipLaunchKernelGGL(k_grad2<5>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<5, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 26)
-    hipLaunchKernelGGL(k_grad2<6>, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<6, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 3)
     hipLaunchKernelGGL(k_grad3, dim3((h->d.nC + G3C - 1) / G3C), dim3(256), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 0)
@@ -1291,16 +1353,39 @@ int launch_update(rflu_handle *h, int istage, int fused) {
   return 0;
 }
 
-// One RFLU_TimeStepping iteration.  The stage-1 flux kernel reads exactly the
-// state RFLU_TimeStepInviscid sees, so it also writes the face spectral radii;
-// k_dtmin then forms dt and dtMin before the stage-1 update needs it.
-int enqueue_step(rflu_handle *h) {
-  for (int st = 1; st <= 3; st++) {
+int picl_stage(rflu_handle *h, int istage);  // picl_gpu.inl
+int launch_rhog(rflu_handle *h);
+void picl_free(PiclState *s);
+
+// One RK stage.  The stage-1 flux kernel reads exactly the state
+// RFLU_TimeStepInviscid sees, so with withDt it also writes the face spectral
+// radii and k_dtmin forms dt and dtMin before the update needs it.  With
+// particles the stage follows libfloflu/RungeKuttaMP.F90:219-280 with
+// PICL_TEMP_Runge between SourceTermsMP and RKUpdateMP.
+void enqueue_stage(rflu_handle *h, int st, bool withDt) {
+  if (h->picl) {
     launch_grad(h);
-    launch_flux(h, st == 1);
-    if (st == 1) launch_dtmin(h, 1, 0);
-    launch_update(h, st, 1);
+    launch_rhog(h);
+    launch_flux(h, withDt);
+    if (withDt) launch_dtmin(h, 1, 0);
+    {
+      ProfScope ps(h, "k_gather");
+      hipLaunchKernelGGL(k_gather, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
+    }
+    if (h->gas.accelOn) hipLaunchKernelGGL(k_source, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
+    picl_stage(h, st);
+    launch_update(h, st, 0);
+    return;
   }
+  launch_grad(h);
+  launch_flux(h, withDt);
+  if (withDt) launch_dtmin(h, 1, 0);
+  launch_update(h, st, 1);
+}
+
+// One RFLU_TimeStepping iteration.
+int enqueue_step(rflu_handle *h) {
+  for (int st = 1; st <= 3; st++) enqueue_stage(h, st, st == 1);
   hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, h->stream, h->d);
   return 0;
 }
@@ -1311,6 +1396,7 @@ int check_err(rflu_handle *h) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (e & E_NAN) return set_err(h, RFLU_GPU_ENAN, "RFLU_CheckValidity: invalid (NaN) variables detected");
   if (e & E_NEG) return set_err(h, RFLU_GPU_ENEGATIVE, "RFLU_CheckPositivity: negative positive-definite variables");
+  if (e & E_INTERP) return set_err(h, RFLU_GPU_EINVAL, "ppiclf LocalInterp: a particle has no element centroid within reach");
   return 0;
 }
 
@@ -1353,6 +1439,8 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   std::string e = rflu::build_mesh(in, prm->spaceOrder > 1, h->mesh);
   if (!e.empty()) { delete h; return set_err(nullptr, RFLU_GPU_EMESH, "mesh: " + e); }
   const rflu::MeshHost &m = h->mesh;
+  h->xyz.assign(md->xyz, md->xyz + 3 * (size_t)md->nVert);
+  h->c2v.assign(md->c2v, md->c2v + (size_t)md->nvc * md->nCells);
   Gas &G = h->gas;
   G.cp = prm->refCp; G.g = prm->refGamma; G.gc = G.cp - G.cp / G.g;
   double mol = 8314.3 / G.gc;          // RFLU_SetGasVars: gv(MOL)
@@ -1509,6 +1597,8 @@ int rflu_gpu_destroy(rflu_handle *h) {
                   d.csMem, d.rm, d.grad, d.cv, d.rhsSum, d.rhs, d.rec, d.flux, d.srad, d.dtCell, d.dvOut, d.partial, d.scal,
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);
+  for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw}) if (p) hipFree(p);
+  picl_free(h->picl);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (auto e : h->evPool) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1635,6 +1725,7 @@ int rflu_gpu_rk_init(rflu_handle *h, int istage) {
 int rflu_gpu_cell_gradients(rflu_handle *h) {
   if (!h) return RFLU_GPU_EINVAL;
   launch_grad(h);
+  if (h->picl) launch_rhog(h);
   HIPCHK(h, hipGetLastError());
   return 0;
 }
@@ -1668,9 +1759,7 @@ int rflu_gpu_rk_update(rflu_handle *h, int istage) {
 
 int rflu_gpu_rk_stage(rflu_handle *h, int istage) {
   if (!h || istage < 1 || istage > 3) return RFLU_GPU_EINVAL;
-  launch_grad(h);
-  launch_flux(h, 0);
-  launch_update(h, istage, 1);
+  enqueue_stage(h, istage, false);
   HIPCHK(h, hipGetLastError());
   return check_err(h);
 }
@@ -1757,3 +1846,5 @@ int rflu_gpu_kernel_time(rflu_handle *h, const char *name, double *ms, int *n) {
 }
 
 }  // extern "C"
+
+#include "picl_gpu.inl"

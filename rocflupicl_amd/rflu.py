@@ -22,6 +22,12 @@ DEFAULTS = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patch
                 accel=(0.0, 0.0, 0.0), limiterGuard=0)
 
 
+PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
+                     amFlag=1, pgFlag=1, feedbackFlag=1)
+# rprop index (1-based, PPICLF_USER.h) of each interpolated field slot
+PICL_FIELD_RPROP = [2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39]
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -164,6 +170,59 @@ class Region:
 
     def RFLU_TimeStepping(self, nSteps, use_graph=True):
         self._chk(self._L.rflu_gpu_step(self.h, int(nSteps), 1 if use_graph else 0))
+
+    # ---- ppiclF coupling (include/picl_gpu.h) ---------------------------
+    def picl_init(self, y, pconst, **picl_params):
+        """PICL_TEMP_InitSolver: attach particles.  y (n,12) in ppiclF order,
+        pconst (n,4) = (rho_p, d_p, V_p, SPL); the state must hold the
+        pure-gas cv (it is scaled by 1 - piclVF)."""
+        d = dict(PICL_DEFAULTS)
+        unknown = set(picl_params) - set(d)
+        if unknown:
+            raise TypeError(f"unknown PICL parameters {sorted(unknown)}")
+        d.update(picl_params)
+        pp = _lib.PiclParams(**d)
+        y = np.ascontiguousarray(y, np.float64)
+        pc = np.ascontiguousarray(pconst, np.float64)
+        assert y.ndim == 2 and y.shape[1] == 12 and pc.shape == (y.shape[0], 4)
+        self._chk(self._L.picl_gpu_init(self.h, C.byref(pp), y.shape[0], _p(y), _p(pc)))
+        self.nPart = y.shape[0]
+
+    def PICL_TEMP_Runge(self, istage):
+        """The coupling stage: call after SourceTermsMP, before RKUpdateMP."""
+        self._chk(self._L.picl_gpu_runge(self.h, istage))
+
+    def picl_sizes(self):
+        n, lost = C.c_int(), C.c_int()
+        self._chk(self._L.picl_gpu_sizes(self.h, C.byref(n), C.byref(lost)))
+        return n.value, lost.value
+
+    def picl_particles(self):
+        n = self.nPart
+        y, y1, yd, yc = (np.zeros((n, 12)) for _ in range(4))
+        elem = np.zeros(n, np.int32)
+        self._chk(self._L.picl_gpu_get_particles(self.h, _p(y), _p(y1), _p(yd), _p(yc), _p(elem)))
+        return dict(y=y, y1=y1, ydot=yd, ydotc=yc, elem=elem)
+
+    def picl_cells(self):
+        N = self.nCells
+        phi, g, pf, fld = np.zeros(N), np.zeros((N, 3)), np.zeros((N, 5)), np.zeros((N, 20))
+        self._chk(self._L.picl_gpu_get_cells(self.h, _p(phi), _p(g), _p(pf), _p(fld)))
+        return dict(phi=phi, gradRhog=g, proFld=pf, fields=fld)
+
+    def picl_set_debug(self, on):
+        self._chk(self._L.picl_gpu_set_debug(self.h, 1 if on else 0))
+
+    def picl_interp(self):
+        a = np.zeros((self.nPart, 20))
+        self._chk(self._L.picl_gpu_get_interp(self.h, _p(a)))
+        return a
+
+    def picl_locate(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros(xyz.shape[0], np.int32)
+        self._chk(self._L.picl_gpu_locate(self.h, xyz.shape[0], _p(xyz), _p(out)))
+        return out
 
     def time(self):
         t, dt = C.c_double(), C.c_double()

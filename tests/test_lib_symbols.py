@@ -29,17 +29,19 @@ def built_lib():
     return _lib.LIB_PATH
 
 
-def test_header_declarations_match_loader_list():
+def test_header_declarations_match_loader_lists():
     assert declared("rflu_gpu.h") == sorted(_lib.RFLU_SYMBOLS)
+    assert declared("picl_gpu.h") == sorted(_lib.PICL_SYMBOLS)
 
 
-def test_library_exports_every_declared_symbol(built_lib):
+@pytest.mark.parametrize("header", ["rflu_gpu.h", "picl_gpu.h"])
+def test_library_exports_every_declared_symbol(header, built_lib):
     exported = _lib.exported_symbols()
-    missing = [s for s in declared("rflu_gpu.h") if s not in exported]
+    missing = [s for s in declared(header) if s not in exported]
     assert not missing, missing
 
 
 def test_library_opens_without_a_gpu(built_lib):
     L = ctypes.CDLL(built_lib)
-    for s in declared("rflu_gpu.h"):
+    for s in declared("rflu_gpu.h") + declared("picl_gpu.h"):
         assert hasattr(L, s)
