@@ -3,15 +3,16 @@
 // stage shares the region's handle, stream, graph and device helpers.
 //
 // Layout in HBM:
-//   particles (original ppiclF order, SoA): position, velocity, the RK
-//     register y1 and ydot for those six components, ydotc (vx, vy, vz, T) of
-//     the previous stage (the projection map) and of this stage, constants
-//     (rho_p, d_p, V_p, SPL), element id;
-//   plist: particle indices grouped by element, ascending within an element
-//     (a stable counting sort rebuilt every stage).  The particle kernel walks
-//     this list, so neighbouring lanes share their element's neighbourhood in
-//     L2; the deposit walks it to sum each element's particles in ppiclF's
-//     order;
+//   particles, SoA in "slot" order: grouped by element, ascending original
+//     (ppiclF) index within an element, lost particles last.  Rows: position,
+//     velocity, the RK register y1 and ydot of those six components, ydotc
+//     (vx, vy, vz, T) of the previous stage (the projection map); constants
+//     (rho_p, d_p, V_p, SPL); original index; element.  Two copies: each
+//     stage's counting sort by element writes the particles in their new
+//     order into the other copy (a device-side parity flag selects the
+//     current one, so a captured graph stays valid).  Lanes of a wave thus
+//     read contiguous particle data and share their elements' neighbourhood,
+//     and the deposit sums each element's particles in ppiclF's order;
 //   cells (device numbering): piclVF, rho_g, grad rho_g (raw and final, 4
 //     doubles per cell), the 20 interpolation inputs (160 B per cell), the 5
 //     projected fields (SoA), 1/V_hex, ppiclF centroid (4 per cell), the 8
@@ -34,16 +35,24 @@ struct PiclPhys {
   double rmuRef, tref, suth, rcpPart;
 };
 
+// Rows of PBuf::st
+enum { ROW_X = 0, ROW_Y1 = 6, ROW_YD = 12, ROW_YCP = 18, NROW = 22 };
+
+struct PBuf {
+  double *st = nullptr;  // [NROW][nP]
+  double *pc = nullptr;  // [4][nP] rho_p, d_p, V_p, SPL
+  int *pid = nullptr;    // original (ppiclF) index
+  int *elem = nullptr;   // element (device cell) of the current stage, -1 lost
+};
+
 struct PiclDev {
   int nP = 0, nC = 0;
-  double *px = nullptr, *py = nullptr, *pz = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
-  double *y1 = nullptr;   // [6][nP]
-  double *yd = nullptr;   // [6][nP]
-  double *ycP = nullptr;  // [4][nP] ydotc of the previous stage
-  double *ycN = nullptr;  // [4][nP] ydotc of this stage
-  double *pc = nullptr;   // [4][nP] rho_p, d_p, V_p, SPL
-  int *elem = nullptr;
-  int *plist = nullptr, *plistN = nullptr, *cnt = nullptr, *off = nullptr, *bsum = nullptr, *nAct = nullptr;
+  PBuf buf[2];
+  int *par = nullptr;     // buf[par[0]] is current
+  double *ycN = nullptr;  // [4][nP] ydotc of this stage, by current slot
+  int *src = nullptr;     // new position -> current slot (active particles)
+  int *dest = nullptr;    // current slot -> new position
+  int *cnt = nullptr, *off = nullptr, *bsum = nullptr, *nAct = nullptr, *lostCur = nullptr;
   double *fld = nullptr;   // [nC][20]
   double *gRho = nullptr;  // [nC][4]
   double *proj = nullptr;  // [5][nC]
@@ -136,21 +145,34 @@ __global__ void __launch_bounds__(BLK) k_rhog(Dev d, PiclDev P) {
   int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
   const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
+  const double2 *gr = reinterpret_cast<const double2 *>(d.gRaw);
   int sb = d.csOff[i], se = d.csOff[i + 1];
-  double g[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    double term = d.gRaw[4 * (size_t)i + k];
-    double sum = 1.0 / (term * term + 1.0E-15);
-    double loc = sum * term;
-    for (int s = sb; s < se; s++) {
-      term = d.gRaw[4 * (size_t)(mem[s] & CS_IDX) + k];
-      double w = 1.0 / (term * term + 1.0E-15);
-      sum = sum + w;
-      loc = loc + w * term;
-    }
-    g[k] = loc / sum;
+  // one pass over the stencil, the next member's gradient in flight while
+  // the current one is weighted; each component keeps its own sequence
+  // (the cell first, then the members in stencil order)
+  double2 a = gr[2 * (size_t)i], b = gr[2 * (size_t)i + 1];
+  double g0 = a.x, g1 = a.y, g2 = b.x;
+  double s0 = 1.0 / (g0 * g0 + 1.0E-15), s1 = 1.0 / (g1 * g1 + 1.0E-15), s2 = 1.0 / (g2 * g2 + 1.0E-15);
+  double l0 = s0 * g0, l1 = s1 * g1, l2 = s2 * g2;
+  double2 na = make_double2(0.0, 0.0), nb = make_double2(0.0, 0.0);
+  if (sb < se) {
+    unsigned j = mem[sb] & CS_IDX;
+    na = gr[2 * (size_t)j];
+    nb = gr[2 * (size_t)j + 1];
   }
+  unsigned jn = (sb + 1 < se) ? (mem[sb + 1] & CS_IDX) : 0u;
+  for (int s = sb; s < se; s++) {
+    double t0 = na.x, t1 = na.y, t2 = nb.x;
+    if (s + 1 < se) {
+      na = gr[2 * (size_t)jn];
+      nb = gr[2 * (size_t)jn + 1];
+      if (s + 2 < se) jn = mem[s + 2] & CS_IDX;
+    }
+    double w0 = 1.0 / (t0 * t0 + 1.0E-15), w1 = 1.0 / (t1 * t1 + 1.0E-15), w2 = 1.0 / (t2 * t2 + 1.0E-15);
+    s0 = s0 + w0; s1 = s1 + w1; s2 = s2 + w2;
+    l0 = l0 + w0 * t0; l1 = l1 + w1 * t1; l2 = l2 + w2 * t2;
+  }
+  double g[3] = {l0 / s0, l1 / s1, l2 / s2};
   int N = d.nC;
   double varc = d.rhog[i];
   double xc = d.cofg[i], yc = d.cofg[N + i], zc = d.cofg[2 * N + i];
@@ -307,17 +329,20 @@ __device__ __forceinline__ bool key_less(double a, int ra, double b, int rb) {
   return a < b || (a == b && ra < rb);
 }
 
+__device__ __forceinline__ PBuf cur_buf(const PiclDev &P) { return P.par[0] ? P.buf[1] : P.buf[0]; }
+
 __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
-  const int nP = P.nP;
-  int p = P.plist[t];
-  double xp[3] = {P.px[p], P.py[p], P.pz[p]};
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  double *st = C.st;
+  double xp[3] = {st[t], st[nP + t], st[2 * nP + t]};
   int e = picl_locate(P, xp);
-  P.elem[p] = e;
+  C.elem[t] = e;
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
-    for (int k = 0; k < 4; k++) P.ycP[k * (size_t)nP + p] = 0.0;
+    for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = 0.0;
     return;
   }
   atomicAdd(&P.cnt[e], 1);
@@ -408,14 +433,15 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
 #pragma unroll
   for (int f = 0; f < PICL_NFLD; f++) v[f] = v[f] / wsum;
   if (P.dbg) {
+    int pid = C.pid[t];
 #pragma unroll
-    for (int f = 0; f < PICL_NFLD; f++) P.dbg[f * (size_t)nP + p] = v[f];
+    for (int f = 0; f < PICL_NFLD; f++) P.dbg[f * nP + pid] = v[f];
   }
   // slots: 0 rhof 1-3 u 4-6 dp/dx 7 c 8 T 9 phi 10-12 Du/Dt 13 rhsR 14-16 grad(rho phi) 17-19 grad rho_g
 
   // ---- SetYdot
-  double yv[3] = {P.vx[p], P.vy[p], P.vz[p]};
-  double rhop = P.pc[p], dpp = P.pc[(size_t)nP + p], volp = P.pc[2 * (size_t)nP + p], spl = P.pc[3 * (size_t)nP + p];
+  double yv[3] = {st[3 * nP + t], st[4 * nP + t], st[5 * nP + t]};
+  double rhop = C.pc[t], dpp = C.pc[nP + t], volp = C.pc[2 * nP + t], spl = C.pc[3 * nP + t];
   double rmu;
   if (Ph.rmuFlag == 0) {
     rmu = Ph.rmuRef;
@@ -484,13 +510,13 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
                      famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + 0.0);
     }
 #pragma unroll
-    for (int k = 0; k < 6; k++) P.yd[k * (size_t)nP + p] = ydn[k];
+    for (int k = 0; k < 6; k++) st[(ROW_YD + k) * nP + t] = ydn[k];
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; k++) ydn[k] = P.yd[k * (size_t)nP + p];
+    for (int k = 0; k < 6; k++) ydn[k] = st[(ROW_YD + k) * nP + t];
   }
 #pragma unroll
-  for (int k = 0; k < 4; k++) P.ycN[k * (size_t)nP + p] = yc[k];
+  for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = yc[k];
 
   // ---- RK3 (Rocflu coefficients), y = -c1*y1 + y + c3*ydot, y1 = ydot
   double dt = d.scal[S_DTMIN];
@@ -498,13 +524,12 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
   if (istage == 1) { k0 = 0.0; k2 = dt * 8.0 / 15.0; }
   else if (istage == 2) { k0 = dt * 17.0 / 60.0; k2 = dt * 5.0 / 12.0; }
   else { k0 = dt * 5.0 / 12.0; k2 = dt * 3.0 / 4.0; }
-  double *yy[6] = {P.px, P.py, P.pz, P.vx, P.vy, P.vz};
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    double y1 = istage == 1 ? 0.0 : P.y1[k * (size_t)nP + p];
-    double y = yy[k][p];
-    yy[k][p] = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
-    P.y1[k * (size_t)nP + p] = ydn[k];
+    double y1 = istage == 1 ? 0.0 : st[(ROW_Y1 + k) * nP + t];
+    double y = st[(ROW_X + k) * nP + t];
+    st[(ROW_X + k) * nP + t] = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
+    st[(ROW_Y1 + k) * nP + t] = ydn[k];
   }
 }
 
@@ -512,11 +537,16 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
 __global__ void __launch_bounds__(BLK) k_locate_init(PiclDev P) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
-  int p = P.plist[t];
-  double xp[3] = {P.px[p], P.py[p], P.pz[p]};
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  double xp[3] = {C.st[t], C.st[nP + t], C.st[2 * nP + t]};
   int e = picl_locate(P, xp);
-  P.elem[p] = e;
+  C.elem[t] = e;
   if (e >= 0) atomicAdd(&P.cnt[e], 1);
+  else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = 0.0;
+  }
 }
 
 __global__ void __launch_bounds__(BLK) k_locate_pts(PiclDev P, int n, const double *xyz, int *out) {
@@ -598,54 +628,102 @@ __global__ void __launch_bounds__(SCAN_T) k_scan3(int *cnt, int n, const int *bs
     }
 }
 
-// Particles into their element's slots (arbitrary order within an element).
+// Particles into their element's new positions (arbitrary order within an
+// element; k_deposit sorts each segment); particles that left every element
+// go after the active ones.
 __global__ void __launch_bounds__(BLK) k_fill(PiclDev P) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
-  int p = P.plist[t];
-  int e = P.elem[p];
-  if (e < 0) return;
+  PBuf C = cur_buf(P);
+  int e = C.elem[t];
+  if (e < 0) {
+    P.dest[t] = P.off[P.nC] + atomicAdd(P.lostCur, 1);
+    return;
+  }
   int slot = atomicAdd(&P.cnt[e], 1);
-  P.plistN[P.off[e] + slot] = p;
+  P.src[P.off[e] + slot] = t;
 }
 
 // ---------------------------------------------------------------------------
 // ProjectParticleGrid, single-element box mode (ppiclf_solve.f:3285-3447):
 // each element sums map * (1/V_hex) of its particles in ascending particle
 // index; the map (ppiclf_user_MapProjPart.f:37-66) is built from the previous
-// stage's ydotc (ppiclf_comm.f:1190-1217 runs before SetYdot).  The sorted
-// segment becomes the next stage's particle order, and this stage's ydotc
-// becomes the next map.
+// stage's ydotc (ppiclf_comm.f:1190-1217 runs before SetYdot).  The segment,
+// sorted by original index, fixes the particles' positions in the next order.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_deposit(PiclDev P) {
-  int e = xcd_block() * BLK + threadIdx.x;
+#define DEP_T 128
+#define DEP_CAP 64
+__global__ void __launch_bounds__(DEP_T) k_deposit(PiclDev P) {
+  __shared__ long long sk[DEP_T * DEP_CAP];
+  int e = blockIdx.x * DEP_T + threadIdx.x;
   if (e >= P.nC) return;
-  const int nP = P.nP;
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
   int b = P.off[e], n = P.off[e + 1] - b;
-  int *L = P.plistN + b;
-  for (int a = 1; a < n; a++) {  // insertion sort: segments hold tens of particles
-    int x = L[a], j = a - 1;
-    while (j >= 0 && L[j] > x) { L[j + 1] = L[j]; j--; }
-    L[j + 1] = x;
+  int *L = P.src + b;
+  // sort the segment by original index: key = (pid << 32) | slot
+  if (n <= DEP_CAP) {
+    long long *K = sk + (size_t)threadIdx.x * DEP_CAP;
+    for (int q = 0; q < n; q++) {
+      int t = L[q];
+      K[q] = ((long long)C.pid[t] << 32) | (unsigned)t;
+    }
+    for (int a = 1; a < n; a++) {
+      long long x = K[a];
+      int j = a - 1;
+      while (j >= 0 && K[j] > x) { K[j + 1] = K[j]; j--; }
+      K[j + 1] = x;
+    }
+    for (int q = 0; q < n; q++) L[q] = (int)(unsigned)(K[q] & 0xffffffffll);
+  } else {
+    for (int a = 1; a < n; a++) {
+      int x = L[a], px = C.pid[x], j = a - 1;
+      while (j >= 0 && C.pid[L[j]] > px) { L[j + 1] = L[j]; j--; }
+      L[j + 1] = x;
+    }
   }
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double rexp = P.rexp[e];
   for (int q = 0; q < n; q++) {
-    int p = L[q];
-    double spl = P.pc[3 * (size_t)nP + p];
+    int t = L[q];
+    double spl = C.pc[3 * nP + t];
     double map[5];
-    map[0] = P.pc[2 * (size_t)nP + p] * spl;
+    map[0] = C.pc[2 * nP + t] * spl;
 #pragma unroll
-    for (int k = 0; k < 4; k++) map[1 + k] = P.ycP[k * (size_t)nP + p] * spl;
+    for (int k = 0; k < 4; k++) map[1 + k] = C.st[(ROW_YCP + k) * nP + t] * spl;
 #pragma unroll
     for (int k = 0; k < 5; k++) acc[k] = acc[k] + (map[k] * 1.0) * rexp;
-#pragma unroll
-    for (int k = 0; k < 4; k++) P.ycP[k * (size_t)nP + p] = P.ycN[k * (size_t)nP + p];
-    P.plist[b + q] = p;
+    P.dest[t] = b + q;
   }
 #pragma unroll
   for (int k = 0; k < 5; k++) P.proj[k * (size_t)P.nC + e] = 0.0 + acc[k];
-  if (e == P.nC - 1) P.nAct[0] = P.off[P.nC];
+}
+
+// Every particle to its new position in the other copy; this stage's ydotc
+// becomes the next map.  Particles lost earlier keep their positions.
+__global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nP) return;
+  const size_t nP = P.nP;
+  int par = P.par[0];
+  PBuf C = par ? P.buf[1] : P.buf[0], X = par ? P.buf[0] : P.buf[1];
+  int nA = P.nAct[0];
+  int pos = t < nA ? P.dest[t] : t;
+#pragma unroll
+  for (int r = 0; r < ROW_YCP; r++) X.st[r * nP + pos] = C.st[r * nP + t];
+#pragma unroll
+  for (int k = 0; k < 4; k++) X.st[(ROW_YCP + k) * nP + pos] = t < nA ? P.ycN[k * nP + t] : C.st[(ROW_YCP + k) * nP + t];
+#pragma unroll
+  for (int k = 0; k < 4; k++) X.pc[k * nP + pos] = C.pc[k * nP + t];
+  X.pid[pos] = C.pid[t];
+  X.elem[pos] = C.elem[t];
+}
+
+__global__ void k_flip(PiclDev P) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    P.par[0] = 1 - P.par[0];
+    P.nAct[0] = P.off[P.nC];
+  }
 }
 
 // Rocflu's read-back (PICL_TEMP_Runge.F90:926-1038, 1217-1234): feedback into
@@ -736,7 +814,8 @@ namespace {
 void picl_free(PiclState *s) {
   if (!s) return;
   PiclDev &P = s->P;
-  void *ptrs[] = {P.px, P.y1, P.yd, P.ycP, P.ycN, P.pc, P.elem, P.plist, P.plistN, P.cnt, P.off, P.bsum, P.nAct,
+  void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
+                  P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
                   P.cbMem, P.dbg};
   for (void *q : ptrs) if (q) (void)hipFree(q);
@@ -759,8 +838,13 @@ void picl_sort_and_deposit(rflu_handle *h) {
     ProfScope ps(h, "k_fill");
     hipLaunchKernelGGL(k_fill, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
   }
-  ProfScope ps(h, "k_deposit");
-  hipLaunchKernelGGL(k_deposit, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
+  {
+    ProfScope ps(h, "k_deposit");
+    hipLaunchKernelGGL(k_deposit, dim3((P.nC + DEP_T - 1) / DEP_T), dim3(DEP_T), 0, h->stream, P);
+  }
+  ProfScope ps(h, "k_permute");
+  hipLaunchKernelGGL(k_permute, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
+  hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, h->stream, P);
 }
 
 int launch_rhog(rflu_handle *h) {
@@ -778,6 +862,7 @@ int picl_stage(rflu_handle *h, int istage) {
     hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P);
   }
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nC + 1)), dim3(BLK), 0, h->stream, P.cnt, P.nC + 1);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lostCur, 1);
   {
     ProfScope ps(h, "k_particles");
     hipLaunchKernelGGL(k_particles, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys, istage);
@@ -904,24 +989,27 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   std::vector<int> idOrder(N);
   for (int i = 0; i < N; i++) idOrder[i] = i;
   build_bins(cpts, cpts, idOrder, P.cbLo, P.cbH, P.cbN, cbOff, cbMem);
-  // particles, SoA
-  std::vector<double> pos(6 * (size_t)nP), pcs(4 * (size_t)nP);
+  // particles, SoA rows in the given order (the first sort groups them)
+  std::vector<double> st(NROW * (size_t)nP, 0.0), pcs(4 * (size_t)nP);
   for (int p = 0; p < nP; p++) {
-    for (int k = 0; k < 6; k++) pos[k * (size_t)nP + p] = y[12 * (size_t)p + k];
+    for (int k = 0; k < 6; k++) st[(ROW_X + k) * (size_t)nP + p] = y[12 * (size_t)p + k];
     for (int k = 0; k < 4; k++) pcs[k * (size_t)nP + p] = pconst[4 * (size_t)p + k];
   }
-  std::vector<int> ident(nP);
+  std::vector<int> ident(nP), minus1(nP, -1);
   for (int p = 0; p < nP; p++) ident[p] = p;
+  int zero = 0;
   int rc = 0;
-  rc |= dalloc(h, &P.px, 6 * (size_t)nP, pos.data());
-  rc |= dalloc(h, &P.y1, 6 * (size_t)nP);
-  rc |= dalloc(h, &P.yd, 6 * (size_t)nP);
-  rc |= dalloc(h, &P.ycP, 4 * (size_t)nP);
+  for (int b = 0; b < 2; b++) {
+    rc |= dalloc(h, &P.buf[b].st, NROW * (size_t)nP, b == 0 ? st.data() : nullptr);
+    rc |= dalloc(h, &P.buf[b].pc, 4 * (size_t)nP, b == 0 ? pcs.data() : nullptr);
+    rc |= dalloc(h, &P.buf[b].pid, (size_t)nP, b == 0 ? ident.data() : nullptr);
+    rc |= dalloc(h, &P.buf[b].elem, (size_t)nP, minus1.data());
+  }
+  rc |= dalloc(h, &P.par, 1, &zero);
   rc |= dalloc(h, &P.ycN, 4 * (size_t)nP);
-  rc |= dalloc(h, &P.pc, 4 * (size_t)nP, pcs.data());
-  rc |= dalloc(h, &P.elem, (size_t)nP);
-  rc |= dalloc(h, &P.plist, (size_t)nP, ident.data());
-  rc |= dalloc(h, &P.plistN, (size_t)nP);
+  rc |= dalloc(h, &P.src, (size_t)nP);
+  rc |= dalloc(h, &P.dest, (size_t)nP);
+  rc |= dalloc(h, &P.lostCur, 1, &zero);
   rc |= dalloc(h, &P.cnt, (size_t)N + 1);
   rc |= dalloc(h, &P.off, (size_t)N + 1);
   rc |= dalloc(h, &P.bsum, (size_t)scan_blocks(N + 1));
@@ -943,12 +1031,12 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   rc |= dalloc(h, &h->d.rhog, (size_t)N);
   rc |= dalloc(h, &h->d.gRaw, 4 * (size_t)N);
   if (rc) { std::string msg = h->err; picl_free(s); return set_err(h, RFLU_GPU_EHIP, msg); }
-  P.py = P.px + nP; P.pz = P.px + 2 * (size_t)nP;
-  P.vx = P.px + 3 * (size_t)nP; P.vy = P.px + 4 * (size_t)nP; P.vz = P.px + 5 * (size_t)nP;
-  HIPCHK(h, hipMemsetAsync(P.y1, 0, 6 * (size_t)nP * sizeof(double), h->stream));
-  HIPCHK(h, hipMemsetAsync(P.yd, 0, 6 * (size_t)nP * sizeof(double), h->stream));
-  HIPCHK(h, hipMemsetAsync(P.ycP, 0, 4 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.buf[1].st, 0, NROW * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.buf[1].pc, 0, 4 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.buf[1].pid, 0, (size_t)nP * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(P.ycN, 0, 4 * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.src, 0, (size_t)nP * sizeof(int), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.dest, 0, (size_t)nP * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(h->d.gRaw, 0, 4 * (size_t)N * sizeof(double), h->stream));
   HIPCHK(h, hipMemsetAsync(P.gRho, 0, 4 * (size_t)N * sizeof(double), h->stream));
   HIPCHK(h, hipMemsetAsync(P.fld, 0, PICL_NFLD * (size_t)N * sizeof(double), h->stream));
@@ -988,26 +1076,29 @@ int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, 
   PiclState *s = h->picl;
   PiclDev &P = s->P;
   size_t nP = P.nP;
-  std::vector<double> a(6 * nP);
+  int par = 0;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  auto fill = [&](double *dst, const double *src, int ncomp, int first, bool keepConst) -> int {
-    if (!dst) return 0;
-    HIPCHK(h, hipMemcpy(a.data(), src, ncomp * nP * sizeof(double), hipMemcpyDeviceToHost));
-    for (size_t p = 0; p < nP; p++) {
+  HIPCHK(h, hipMemcpy(&par, P.par, sizeof(int), hipMemcpyDeviceToHost));
+  const PBuf &C = P.buf[par];
+  std::vector<double> st(NROW * nP);
+  std::vector<int> pid(nP), ev(nP);
+  HIPCHK(h, hipMemcpy(st.data(), C.st, st.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(pid.data(), C.pid, nP * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(ev.data(), C.elem, nP * sizeof(int), hipMemcpyDeviceToHost));
+  auto fill = [&](double *dst, int row, int ncomp, int first, bool keepConst) {
+    if (!dst) return;
+    for (size_t t = 0; t < nP; t++) {
+      size_t p = pid[t];
       for (int j = 0; j < 12; j++) dst[12 * p + j] = keepConst ? s->y0[12 * p + j] : 0.0;
-      for (int k = 0; k < ncomp; k++) dst[12 * p + first + k] = a[k * nP + p];
+      for (int k = 0; k < ncomp; k++) dst[12 * p + first + k] = st[(row + k) * nP + t];
     }
-    return 0;
   };
-  if (int e = fill(y, P.px, 6, 0, true)) return e;
-  if (int e = fill(y1, P.y1, 6, 0, false)) return e;
-  if (int e = fill(ydot, P.yd, 6, 0, false)) return e;
-  if (int e = fill(ydotc, P.ycP, 4, 3, false)) return e;
-  if (elem) {
-    std::vector<int> ev(nP);
-    HIPCHK(h, hipMemcpy(ev.data(), P.elem, nP * sizeof(int), hipMemcpyDeviceToHost));
-    for (size_t p = 0; p < nP; p++) elem[p] = ev[p] < 0 ? -1 : h->perm[ev[p]];
-  }
+  fill(y, ROW_X, 6, 0, true);
+  fill(y1, ROW_Y1, 6, 0, false);
+  fill(ydot, ROW_YD, 6, 0, false);
+  fill(ydotc, ROW_YCP, 4, 3, false);
+  if (elem)
+    for (size_t t = 0; t < nP; t++) elem[pid[t]] = ev[t] < 0 ? -1 : h->perm[ev[t]];
   return 0;
 }
 
