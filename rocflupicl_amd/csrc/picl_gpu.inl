@@ -63,7 +63,8 @@ struct PiclDev {
   int *lbOff = nullptr, *lbMem = nullptr;
   int cbN[3] = {1, 1, 1};
   double cbLo[3] = {0, 0, 0}, cbH[3] = {1, 1, 1}, R[3] = {0, 0, 0};
-  int *cbOff = nullptr, *cbMem = nullptr;
+  int *cbOff = nullptr;
+  double *cbRec = nullptr;  // per bin member: cx, cy, cz, (reference id << 32 | device id)
   double d2Max[3] = {0, 0, 0};
   double *dbg = nullptr;  // [20][nP] interpolated values (debug)
 };
@@ -73,7 +74,7 @@ struct PiclDev {
 // point within |r|,|s|,|t| <= 1 + 1e-10 (Newton from the element centre);
 // the stand-in for gslib findpts.  Same operations as the oracle.
 // ---------------------------------------------------------------------------
-__device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3]) {
+__device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3], double *maxrst = nullptr) {
   double r = 0.0, s = 0.0, t = 0.0;
   for (int it = 0; it < 25; it++) {
     double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
@@ -109,11 +110,20 @@ __device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3
     if (fabs(dr) + fabs(ds) + fabs(dt) < 1.0e-14) break;
   }
   const double lim = 1.0 + 1.0e-10;
+  if (maxrst) *maxrst = fmax(fabs(r), fmax(fabs(s), fabs(t)));
   return fabs(r) <= lim && fabs(s) <= lim && fabs(t) <= lim;
 }
 
-__device__ int picl_locate(const PiclDev &P, const double xp[3]) {
+// hint: an element that probably holds the point (the particle's element of
+// the previous stage).  A point well inside it (|r|,|s|,|t| <= 1 - 1e-8) lies
+// in no other element, not even within the 1e-10 tolerance, so the rule's
+// answer is the hint itself; otherwise the full search decides.
+__device__ int picl_locate(const PiclDev &P, const double xp[3], int hint = -1) {
   if (!(isfinite(xp[0]) && isfinite(xp[1]) && isfinite(xp[2]))) return -1;
+  if (hint >= 0) {
+    double m = 2.0;
+    if (trilinear_inside(P.X + 24 * (size_t)hint, xp, &m) && m <= 1.0 - 1.0e-8) return hint;
+  }
   int b[3];
 #pragma unroll
   for (int l = 0; l < 3; l++) {
@@ -338,7 +348,7 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
   PBuf C = cur_buf(P);
   double *st = C.st;
   double xp[3] = {st[t], st[nP + t], st[2 * nP + t]};
-  int e = picl_locate(P, xp);
+  int e = picl_locate(P, xp, C.elem[t]);
   C.elem[t] = e;
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
@@ -371,38 +381,32 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
         for (int bx = blo[0]; bx <= bhi[0]; bx++) {
           int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
           for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
-            int c = P.cbMem[q];
-            const double2 *cp = reinterpret_cast<const double2 *>(P.cen + 4 * (size_t)c);
-            double2 c01 = cp[0], c2_ = cp[1];
-            double t0 = c01.x - xp[0], t1 = c01.y - xp[1], t2 = c2_.x - xp[2];
+            const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)q);
+            double2 c01 = cp[0], c23 = cp[1];
+            double t0 = c01.x - xp[0], t1 = c01.y - xp[1], t2 = c23.x - xp[2];
             double l0 = t0 * t0, l1 = t1 * t1, l2 = t2 * t2;
             if (l0 > P.d2Max[0] || l1 > P.d2Max[1] || l2 > P.d2Max[2]) continue;
             double d2i = 0.0;
             d2i = d2i + l0;
             d2i = d2i + l1;
             d2i = d2i + l2;
-            int rc = P.refId[c];
-            if (!key_less(d2i, rc, kd[26], kr[26]) || !(d2i < F32(1E20))) {
-              if (d2i < F32(1E20)) nadd++;
-              continue;
-            }
+            if (!(d2i < F32(1E20))) continue;
             nadd++;
-            // insertion into the sorted registers (constant indices after unrolling)
-            double cd = d2i;
-            int cr = rc;
+            int rc = (int)(__double_as_longlong(c23.y) >> 32);
+            if (!key_less(d2i, rc, kd[26], kr[26])) continue;
+            // insertion into the sorted registers, branch-free (constant indices)
+            bool lt[27];
 #pragma unroll
-            for (int j = 26; j >= 0; j--) {
-              bool before = (j == 0) ? false : key_less(cd, cr, kd[j - 1], kr[j - 1]);
-              if (j == 26 || key_less(cd, cr, kd[j], kr[j])) {
-                if (before) {
-                  kd[j] = kd[j - 1];
-                  kr[j] = kr[j - 1];
-                } else {
-                  kd[j] = cd;
-                  kr[j] = cr;
-                }
-              }
+            for (int j = 0; j < 27; j++) lt[j] = key_less(d2i, rc, kd[j], kr[j]);
+#pragma unroll
+            for (int j = 26; j >= 1; j--) {
+              double nd = lt[j - 1] ? kd[j - 1] : d2i;
+              int nr = lt[j - 1] ? kr[j - 1] : rc;
+              kd[j] = lt[j] ? nd : kd[j];
+              kr[j] = lt[j] ? nr : kr[j];
             }
+            kd[0] = lt[0] ? d2i : kd[0];
+            kr[0] = lt[0] ? rc : kr[0];
           }
         }
   }
@@ -817,7 +821,7 @@ void picl_free(PiclState *s) {
   void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
-                  P.cbMem, P.dbg};
+                  P.cbRec, P.dbg};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1026,7 +1030,14 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   rc |= dalloc(h, &P.lbOff, lbOff.size(), lbOff.data());
   rc |= dalloc(h, &P.lbMem, lbMem.size(), lbMem.data());
   rc |= dalloc(h, &P.cbOff, cbOff.size(), cbOff.data());
-  rc |= dalloc(h, &P.cbMem, cbMem.size(), cbMem.data());
+  std::vector<double> cbRec(4 * cbMem.size());
+  for (size_t q = 0; q < cbMem.size(); q++) {
+    int c = cbMem[q];
+    for (int l = 0; l < 3; l++) cbRec[4 * q + l] = cen[4 * (size_t)c + l];
+    long long key = ((long long)refId[c] << 32) | (unsigned)c;
+    memcpy(&cbRec[4 * q + 3], &key, sizeof key);
+  }
+  rc |= dalloc(h, &P.cbRec, cbRec.size(), cbRec.data());
   rc |= dalloc(h, &h->d.phi, (size_t)N);
   rc |= dalloc(h, &h->d.rhog, (size_t)N);
   rc |= dalloc(h, &h->d.gRaw, 4 * (size_t)N);
