@@ -222,9 +222,10 @@ def test_sod_workload_matches_oracle_100_steps(oracle_mod):
     assert same_bits(gcv, o.cv()), first_diff(gcv, o.cv())
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2", "3"])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4"])
 def test_gradient_kernel_variants_match_oracle(variant, oracle_mod, monkeypatch):
-    """Both gradient kernels (one thread per cell; cooperative LDS) are
+    """Every gradient kernel (one thread per cell, plain or trimmed; the
+    cooperative LDS ones; LDS tiles of staged stencil records) is
     bit-identical to the oracle."""
     monkeypatch.setenv("ROCFLUPICL_GRAD_VARIANT", variant)
     mesh = MESHES["tet"]
