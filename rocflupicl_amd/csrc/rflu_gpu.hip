@@ -562,18 +562,61 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard)
   for (int k = 0; k < 5; k++) { vmin[k] = ri[k]; vmax[k] = ri[k]; }
   const double *S = reinterpret_cast<const double *>(sRec);
   int sb = d.csOff[i], se = d.csOff[i + 1];
-  unsigned jn = d.sSlot[sb];
-  double rn[8];
-  load_rec(S, (int)(jn & 0x7fffu), rn);
-  for (int s = sb; s < se; s++) {
-    double rj[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) rj[q] = rn[q];
-    unsigned j = jn;
-    if (s + 1 < se) {
-      jn = d.sSlot[s + 1];
-      load_rec(S, (int)(jn & 0x7fffu), rn);
+  // two members per iteration: their weights (sqrt, division) are computed
+  // independently, then added in stencil order
+  int s = sb;
+  for (; s + 1 < se; s += 2) {
+    unsigned ja = d.sSlot[s], jb = d.sSlot[s + 1];
+    double ra[8], rb[8];
+    load_rec(S, (int)(ja & 0x7fffu), ra);
+    load_rec(S, (int)(jb & 0x7fffu), rb);
+    double wa[3], wb[3];
+    {
+      double dx = ra[5] - ri[5], dy = ra[6] - ri[6], dz = ra[7] - ri[7];
+      double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      dx = term * dx; dy = term * dy; dz = term * dz;
+      double term1 = c1111 * (dx);
+      double term2 = c2222 * (dy + c12 * dx);
+      double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+      wa[0] = term * (term1 + c12 * term2 + c13 * term3);
+      wa[1] = term * (term2 + c23 * term3);
+      wa[2] = term * (term3);
     }
+    {
+      double dx = rb[5] - ri[5], dy = rb[6] - ri[6], dz = rb[7] - ri[7];
+      double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      dx = term * dx; dy = term * dy; dz = term * dz;
+      double term1 = c1111 * (dx);
+      double term2 = c2222 * (dy + c12 * dx);
+      double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+      wb[0] = term * (term1 + c12 * term2 + c13 * term3);
+      wb[1] = term * (term2 + c23 * term3);
+      wb[2] = term * (term3);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      double dVar = ra[k] - ri[k];
+      g[3 * k] = g[3 * k] + wa[0] * dVar;
+      g[3 * k + 1] = g[3 * k + 1] + wa[1] * dVar;
+      g[3 * k + 2] = g[3 * k + 2] + wa[2] * dVar;
+      dVar = rb[k] - ri[k];
+      g[3 * k] = g[3 * k] + wb[0] * dVar;
+      g[3 * k + 1] = g[3 * k + 1] + wb[1] * dVar;
+      g[3 * k + 2] = g[3 * k + 2] + wb[2] * dVar;
+    }
+    if (ja & 0x8000u) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], ra[k]); vmax[k] = fmax(vmax[k], ra[k]); }
+    }
+    if (jb & 0x8000u) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rb[k]); vmax[k] = fmax(vmax[k], rb[k]); }
+    }
+  }
+  if (s < se) {
+    unsigned j = d.sSlot[s];
+    double rj[8];
+    load_rec(S, (int)(j & 0x7fffu), rj);
     double dx = rj[5] - ri[5], dy = rj[6] - ri[6], dz = rj[7] - ri[7];
     double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
     dx = term * dx; dy = term * dy; dz = term * dz;
