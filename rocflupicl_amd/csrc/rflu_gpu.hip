@@ -530,6 +530,9 @@ __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guar
 // every cell its 256 cells' stencils touch (Morton-contiguous cells share most
 // of their members: ~3.3 records per cell instead of ~40 gathers), then each
 // lane runs k_grad2's member loop on LDS.  Same operations, same order.
+#ifndef GU
+#define GU 4
+#endif
 __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard) {
   extern __shared__ double2 sRec[];  // [nU][4]: one 64-byte record per slot
   int tile = xcd_block();
@@ -562,58 +565,45 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard)
   for (int k = 0; k < 5; k++) { vmin[k] = ri[k]; vmax[k] = ri[k]; }
   const double *S = reinterpret_cast<const double *>(sRec);
   int sb = d.csOff[i], se = d.csOff[i + 1];
-  // two members per iteration: their weights (sqrt, division) are computed
+  // GU members per iteration: their weights (sqrt, division) are computed
   // independently, then added in stencil order
   int s = sb;
-  for (; s + 1 < se; s += 2) {
-    unsigned ja = d.sSlot[s], jb = d.sSlot[s + 1];
-    double ra[8], rb[8];
-    load_rec(S, (int)(ja & 0x7fffu), ra);
-    load_rec(S, (int)(jb & 0x7fffu), rb);
-    double wa[3], wb[3];
-    {
-      double dx = ra[5] - ri[5], dy = ra[6] - ri[6], dz = ra[7] - ri[7];
+  for (; s + GU - 1 < se; s += GU) {
+    unsigned jj[GU];
+    double rr[GU][8], ww[GU][3];
+#pragma unroll
+    for (int u = 0; u < GU; u++) {
+      jj[u] = d.sSlot[s + u];
+      load_rec(S, (int)(jj[u] & 0x7fffu), rr[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < GU; u++) {
+      double dx = rr[u][5] - ri[5], dy = rr[u][6] - ri[6], dz = rr[u][7] - ri[7];
       double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
       dx = term * dx; dy = term * dy; dz = term * dz;
       double term1 = c1111 * (dx);
       double term2 = c2222 * (dy + c12 * dx);
       double term3 = c3333 * (dz + c23 * dy + c13 * dx);
-      wa[0] = term * (term1 + c12 * term2 + c13 * term3);
-      wa[1] = term * (term2 + c23 * term3);
-      wa[2] = term * (term3);
-    }
-    {
-      double dx = rb[5] - ri[5], dy = rb[6] - ri[6], dz = rb[7] - ri[7];
-      double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-      dx = term * dx; dy = term * dy; dz = term * dz;
-      double term1 = c1111 * (dx);
-      double term2 = c2222 * (dy + c12 * dx);
-      double term3 = c3333 * (dz + c23 * dy + c13 * dx);
-      wb[0] = term * (term1 + c12 * term2 + c13 * term3);
-      wb[1] = term * (term2 + c23 * term3);
-      wb[2] = term * (term3);
+      ww[u][0] = term * (term1 + c12 * term2 + c13 * term3);
+      ww[u][1] = term * (term2 + c23 * term3);
+      ww[u][2] = term * (term3);
     }
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-      double dVar = ra[k] - ri[k];
-      g[3 * k] = g[3 * k] + wa[0] * dVar;
-      g[3 * k + 1] = g[3 * k + 1] + wa[1] * dVar;
-      g[3 * k + 2] = g[3 * k + 2] + wa[2] * dVar;
-      dVar = rb[k] - ri[k];
-      g[3 * k] = g[3 * k] + wb[0] * dVar;
-      g[3 * k + 1] = g[3 * k + 1] + wb[1] * dVar;
-      g[3 * k + 2] = g[3 * k + 2] + wb[2] * dVar;
-    }
-    if (ja & 0x8000u) {
+    for (int u = 0; u < GU; u++) {
 #pragma unroll
-      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], ra[k]); vmax[k] = fmax(vmax[k], ra[k]); }
-    }
-    if (jb & 0x8000u) {
+      for (int k = 0; k < 5; k++) {
+        double dVar = rr[u][k] - ri[k];
+        g[3 * k] = g[3 * k] + ww[u][0] * dVar;
+        g[3 * k + 1] = g[3 * k + 1] + ww[u][1] * dVar;
+        g[3 * k + 2] = g[3 * k + 2] + ww[u][2] * dVar;
+      }
+      if (jj[u] & 0x8000u) {
 #pragma unroll
-      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rb[k]); vmax[k] = fmax(vmax[k], rb[k]); }
+        for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rr[u][k]); vmax[k] = fmax(vmax[k], rr[u][k]); }
+      }
     }
   }
-  if (s < se) {
+  for (; s < se; s++) {
     unsigned j = d.sSlot[s];
     double rj[8];
     load_rec(S, (int)(j & 0x7fffu), rj);
