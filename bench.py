@@ -207,7 +207,9 @@ def main():
     if kern:
         dom = max((k for k in kern if k in byt), key=lambda k: kern[k]["total_ms"])
         ach = byt[dom] / (kern[dom]["ms"] * 1e-3) / 1e9
-        traffic = profiled_traffic("k_grad2" if dom == "k_grad" else dom) if args.workload == "tets64_roe" else None
+        # the gradient routine runs as k_grad_tile (default) or k_grad2 (variants 1-3)
+        gname = "k_grad_tile" if os.environ.get("ROCFLUPICL_GRAD_VARIANT", "4") == "4" else "k_grad2"
+        traffic = profiled_traffic(gname if dom == "k_grad" else dom) if args.workload == "tets64_roe" else None
         roof = dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4),
                     traffic=None if traffic is None else int(traffic),

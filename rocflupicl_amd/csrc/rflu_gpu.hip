@@ -90,6 +90,7 @@ struct rflu_handle {
   hipGraphExec_t graph = nullptr;
   int gradVariant = 4;  // 4: LDS tiles (default when they fit), 2: one thread per cell, 0: plain, 1/3: cooperative
   size_t tileLds = 0;   // LDS bytes per gradient tile (0: tiles not built)
+  size_t ldsMax = 65536;
   std::vector<int> perm, inv;  // device cell i holds reference cell perm[i]
   std::vector<double> stage;   // host staging for (un)permuting cell arrays
   bool profiling = false;
@@ -533,17 +534,23 @@ __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guar
 #ifndef GU
 #define GU 4
 #endif
-__global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard) {
-  extern __shared__ double2 sRec[];  // [nU][4]: one 64-byte record per slot
+// PICL = 1 also stages rho_g = rho/(1-phi) per slot (after the records) and
+// forms its LSQ gradient (CellGradientsMP.F90:375-378), as k_grad2<.,1>.
+template <int PICL>
+__global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard, int nUmax) {
+  extern __shared__ double2 sRec[];  // [nU][4]: one 64-byte record per slot; then rho_g[nUmax]
+  double *sQ = reinterpret_cast<double *>(sRec + 4 * (size_t)nUmax);
   int tile = xcd_block();
   int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
   for (int u = threadIdx.x; u < nU; u += BLK) {
-    const double2 *g = reinterpret_cast<const double2 *>(d.rec + 8 * (size_t)d.tMem[u0 + u]);
+    int c0 = d.tMem[u0 + u];
+    const double2 *g = reinterpret_cast<const double2 *>(d.rec + 8 * (size_t)c0);
     double2 a = g[0], b = g[1], c = g[2], e = g[3];
     sRec[4 * u] = a;
     sRec[4 * u + 1] = b;
     sRec[4 * u + 2] = c;
     sRec[4 * u + 3] = e;
+    if (PICL) sQ[u] = d.rhog[c0];
   }
   __syncthreads();
   int i = tile * BLK + threadIdx.x;
@@ -567,6 +574,8 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard)
   int sb = d.csOff[i], se = d.csOff[i + 1];
   // GU members per iteration: their weights (sqrt, division) are computed
   // independently, then added in stencil order
+  double gq0 = 0.0, gq1 = 0.0, gq2 = 0.0, qi = 0.0;
+  if (PICL) qi = sQ[threadIdx.x];
   int s = sb;
   for (; s + GU - 1 < se; s += GU) {
     unsigned jj[GU];
@@ -597,6 +606,12 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard)
         g[3 * k + 1] = g[3 * k + 1] + ww[u][1] * dVar;
         g[3 * k + 2] = g[3 * k + 2] + ww[u][2] * dVar;
       }
+      if (PICL) {
+        double dq = sQ[jj[u] & 0x7fffu] - qi;
+        gq0 = gq0 + ww[u][0] * dq;
+        gq1 = gq1 + ww[u][1] * dq;
+        gq2 = gq2 + ww[u][2] * dq;
+      }
       if (jj[u] & 0x8000u) {
 #pragma unroll
         for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rr[u][k]); vmax[k] = fmax(vmax[k], rr[u][k]); }
@@ -622,6 +637,12 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard)
       g[3 * k] = g[3 * k] + wtx * dVar;
       g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
       g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    }
+    if (PICL) {
+      double dq = sQ[j & 0x7fffu] - qi;
+      gq0 = gq0 + wtx * dq;
+      gq1 = gq1 + wty * dq;
+      gq2 = gq2 + wtz * dq;
     }
     if (j & 0x8000u) {
 #pragma unroll
@@ -663,6 +684,11 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard)
   double *o = d.grad + 15 * (size_t)i;
 #pragma unroll
   for (int q = 0; q < 15; q++) o[q] = g[q];
+  if (PICL) {
+    double2 *gr = reinterpret_cast<double2 *>(d.gRaw + 4 * (size_t)i);
+    gr[0] = make_double2(gq0, gq1);
+    gr[1] = make_double2(gq2, 0.0);
+  }
 }
 
 // Cooperative variant 3: a workgroup of 256 lanes takes 16 cells.
@@ -1460,10 +1486,14 @@ int launch_grad(rflu_handle *h) {
   if (h->prm.spaceOrder < 2) return 0;
   ProfScope ps(h, "k_grad");
   int venkat = h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, guard = h->prm.limiterGuard ? 1 : 0;
-  if (h->picl)
+  int nUmax = (int)(h->tileLds / 64);
+  if (h->picl && h->gradVariant == 4 && h->tileLds && h->tileLds + 8 * (size_t)nUmax <= h->ldsMax)
+    hipLaunchKernelGGL((k_grad_tile<1>), dim3(h->d.nBlk), dim3(BLK), h->tileLds + 8 * (size_t)nUmax, h->stream,
+                       h->d, venkat, guard, nUmax);
+  else if (h->picl)
     hipLaunchKernelGGL((k_grad2<1, 1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 4 && h->tileLds)
-    hipLaunchKernelGGL(k_grad_tile, dim3(h->d.nBlk), dim3(BLK), h->tileLds, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad_tile<0>), dim3(h->d.nBlk), dim3(BLK), h->tileLds, h->stream, h->d, venkat, guard, nUmax);
   else if (h->gradVariant == 2 || h->gradVariant == 4)
     hipLaunchKernelGGL((k_grad2<1, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
   else if (h->gradVariant == 24)
@@ -1734,6 +1764,7 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
           sSlot[e] = (unsigned short)(slot | ((v & 0x80000000u) ? 0x8000 : 0));
         }
     }
+    h->ldsMax = ldsMax;
     if (ok && (size_t)nUmax * 64 <= ldsMax) h->tileLds = (size_t)nUmax * 64;
   }
   double *faceSoa = nullptr;
