@@ -9,9 +9,11 @@
  * SourceTermsMP and RKUpdateMP; picl_gpu_runge is that call on its own, for
  * hosts that drive the stage routine by routine.
  *
- * Physics restated (rocpicl/, libpicl/): QS Parmar drag, AM Parmar added mass
- * and the pressure-gradient force on an Euler flow, Sutherland or fixed
- * viscosity, IDW-27 interpolation, single-element box projection, two-way
+ * Physics restated (rocpicl/, libpicl/): the quasi-steady drag laws of
+ * ppiclf_user_QuasiSteady.f (Parmar, Osnes, modified Parmar, Gidaspow) plus
+ * Schiller-Naumann, AM Parmar added mass and the pressure-gradient force on
+ * an Euler flow, Sutherland or fixed viscosity, IDW-27 or trilinear
+ * interpolation, single-element box or Gaussian-filter projection, two-way
  * feedback.  Requirements: all-hex grid, spaceOrder 2, patchOrder 1.
  *
  * Particle arrays use ppiclF's layout: y(PPICLF_LRS=12, n) -> y[12*p + j]
@@ -35,10 +37,24 @@ typedef struct picl_params {
   double tref;       /* Sutherland reference temperature */
   double suth;       /* Sutherland constant */
   double rcpPart;    /* particle heat capacity */
-  int qsFlag;        /* 1: QS Parmar (the only one built) */
+  int qsFlag;        /* ppiclf_user.f:473-485: 1 Parmar, 2 Osnes, 3 modified
+                        Parmar, 4 Gidaspow; 5 Schiller-Naumann (rocinteract's
+                        INRT_DRAG_MODEL_SN law in the same slot, not a
+                        ppiclF choice) */
   int amFlag;        /* 0 off, 1 AM Parmar */
   int pgFlag;        /* 0 off, 1 pressure-gradient force */
   int feedbackFlag;  /* 0 one-way, 1 two-way */
+  int interpMode;    /* 0: IDW over the 27 nearest element centroids
+                        (ppiclf_solve_LocalInterp, ppiclf_solve.f:2234-2409,
+                        the reference's choice); 1: trilinear evaluation of
+                        the element's nodal field at the particle's (r,s,t)
+                        (pfgslib_findpts_eval_local, ppiclf_solve.f:2198-2208) */
+  int projMode;      /* 0: single-element box (InitBoxFilter(filt,0,1), the
+                        reference's choice); 1: Gaussian filter
+                        (ppiclf_solve_InitGaussianFilter, ppiclf_solve.f:1520-1576,
+                        ProjectParticleGrid :3103-3246, no wall mirror) */
+  double filterWidth;  /* projMode 1: filter width, the Gaussian's full width at half maximum */
+  double filterAlpha;  /* projMode 1: cut-off; node weights below alpha are dropped (0 < alpha < 1) */
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):
@@ -67,6 +83,10 @@ int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, 
  * interpolation inputs of the last stage, in rprop order 2,6,7,8,10,11,12,9,
  * 24,5,13,14,15,16,17,18,19,37,38,39 (PPICLF_USER.h). */
 int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *proFld, double *fields);
+/* Projected values at the 8 nodes of every element (ppiclf_pro_fld via
+ * GetProFldIJKEF): 40 per cell, (PHIP, FX, FY, FZ, E) x ppiclF node
+ * n = i + 2j + 4k, field-major. */
+int picl_gpu_get_nodes(rflu_handle *h, double *proNode);
 /* Values LocalInterp produced at the last stage, 20 per particle in the
  * order above (kept only while picl_gpu_set_debug(h, 1) is on). */
 int picl_gpu_get_interp(rflu_handle *h, double *rprop20);

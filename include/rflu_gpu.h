@@ -86,6 +86,12 @@ typedef struct rflu_params {
    * gradient when denom lies in (-1e-12, 0).  1: add 1e-12 with the sign of
    * denom (a deliberate deviation, off by default). */
   int limiterGuard;
+  /* 0 (reference): FP64 throughout.  1: gradients (LSQ + limiter) and
+   * interior second-order face fluxes evaluated in FP32, with FP64 kept for
+   * the residual accumulation, the conserved-state RK update, p/T recovery,
+   * boundary patches and the dt reduction.  Needs spaceOrder 2, no
+   * particles.  No reference counterpart: an added precision mode. */
+  int fluxPrecision;
 } rflu_params;
 
 /* RFLU_InitFlowSolver grid/geometry/stencil set-up

@@ -28,11 +28,13 @@ class OrcParams(C.Structure):
 class OrcPiclParams(C.Structure):
     _fields_ = [("rmuFlag", C.c_int), ("rmuRef", C.c_double), ("tref", C.c_double), ("suth", C.c_double),
                 ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
-                ("feedbackFlag", C.c_int)]
+                ("feedbackFlag", C.c_int), ("interpMode", C.c_int), ("projMode", C.c_int),
+                ("filterWidth", C.c_double), ("filterAlpha", C.c_double)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
-                     amFlag=1, pgFlag=1, feedbackFlag=1)
+                     amFlag=1, pgFlag=1, feedbackFlag=1, interpMode=0, projMode=0, filterWidth=0.0,
+                     filterAlpha=1e-2)
 # rprop index (1-based, PPICLF_USER.h) of each interpolated field slot
 PICL_FIELD_RPROP = [2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39]
 
@@ -96,6 +98,9 @@ def lib():
         L.orc_picl_locate.argtypes = [P, I, P, P]
         L.orc_picl_neighbors.argtypes = [P, P, P, P]
         L.orc_picl_neighbors.restype = I
+        L.orc_picl_get_nodes.argtypes = [P, P]
+        L.orc_picl_qs_beta.argtypes = [I] + [C.c_double] * 7
+        L.orc_picl_qs_beta.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -251,6 +256,12 @@ class Region:
         lib().orc_picl_get_cells(self.h, _p(phi), _p(g), _p(pf), _p(fld))
         return dict(phi=phi, gradRhog=g, proFld=pf, fields=fld)
 
+    def picl_nodes(self):
+        """Projected values at the 8 nodes of every element, (nCells, 5, 8)."""
+        a = np.zeros((self.nCells, 5, 8))
+        lib().orc_picl_get_nodes(self.h, _p(a))
+        return a
+
     def picl_nlost(self):
         return lib().orc_picl_nlost(self.h)
 
@@ -294,3 +305,11 @@ def flux_point(discr, n, L, R, **params):
     R = np.ascontiguousarray(R, np.float64)
     lib().orc_flux_point(C.byref(prm), discr, _p(n), _p(L), _p(R), _p(out))
     return out
+
+
+def qs_beta(qsFlag, rmachp, rphip, rep, rmu, dp, rhof=1.0, vmag=1.0):
+    """Quasi-steady coefficient beta of model qsFlag (1 Parmar, 2 Osnes,
+    3 modified Parmar, 4 Gidaspow, 5 Schiller-Naumann) from SetYdot's
+    capped inputs."""
+    return lib().orc_picl_qs_beta(int(qsFlag), float(rmachp), float(rphip), float(rep), float(rmu), float(dp),
+                                  float(rhof), float(vmag))

@@ -53,7 +53,8 @@ struct orc_picl {
   double *bbox;    /* min xyz, max xyz */
   double d2Max[3]; /* (1.5 * extent of the last element)^2 per axis */
   double *rexp;    /* 1 / (12-tet hex volume) */
-  double *proFld;  /* NPRO per element, the value of all 8 nodes */
+  double *proFld;  /* 8 * NPRO per element: field jj at node n in [8 * jj + n] */
+  double gRdum, gMult, gD2chk; /* Gaussian filter: 1/(-2 sigma^2), normalisation, cut-off^2 */
   double *fld;     /* ORC_PICL_NFLD interpolation inputs per element */
 };
 
@@ -117,7 +118,7 @@ static int setup_mesh(orc_region *r, struct orc_picl *p) {
   p->centeri = malloc(sizeof(double) * 3 * (size_t)ne);
   p->bbox = malloc(sizeof(double) * 6 * (size_t)ne);
   p->rexp = malloc(sizeof(double) * (size_t)ne);
-  p->proFld = calloc(NPRO * (size_t)ne, sizeof(double));
+  p->proFld = calloc(8 * NPRO * (size_t)ne, sizeof(double));
   p->fld = calloc(ORC_PICL_NFLD * (size_t)ne, sizeof(double));
   for (int e = 0; e < ne; e++) {
     double *X = p->xm1b + 24 * (size_t)e;
@@ -157,7 +158,7 @@ static int setup_mesh(orc_region *r, struct orc_picl *p) {
 /* lowest-numbered element whose trilinear map holds the point within        */
 /* |r|,|s|,|t| <= 1 + 1e-10 (Newton from the element centre).                */
 /* ------------------------------------------------------------------------ */
-static int trilinear_inside(const double *X, const double *xp) {
+static int trilinear_inside(const double *X, const double *xp, double *rst) {
   double r = 0.0, s = 0.0, t = 0.0;
   for (int it = 0; it < 25; it++) {
     double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
@@ -189,6 +190,7 @@ static int trilinear_inside(const double *X, const double *xp) {
     r = r + dr; s = s + ds; t = t + dt;
     if (fabs(dr) + fabs(ds) + fabs(dt) < 1.0e-14) break;
   }
+  if (rst) { rst[0] = r; rst[1] = s; rst[2] = t; }
   const double lim = 1.0 + 1.0e-10;
   return fabs(r) <= lim && fabs(s) <= lim && fabs(t) <= lim;
 }
@@ -201,7 +203,7 @@ static int locate(const struct orc_picl *p, const double *xp) {
       double tol = 1.0e-9 * (bb[3 + d] - bb[d]);
       if (xp[d] < bb[d] - tol || xp[d] > bb[3 + d] + tol) in = 0;
     }
-    if (in && trilinear_inside(p->xm1b + 24 * (size_t)e, xp)) return e;
+    if (in && trilinear_inside(p->xm1b + 24 * (size_t)e, xp, NULL)) return e;
   }
   return -1;
 }
@@ -263,6 +265,30 @@ static void local_interp(struct orc_picl *p, int ip) {
   }
 }
 
+/* Trilinear mode: gslib findpts_eval_local, the evaluation ppiclF made before
+ * LocalInterp (ppiclf_solve.f:2198-2208), on the element's 2x2x2 nodal field,
+ * which PICL_TEMP_Runge fills with the cell value on every node.  Lagrange
+ * weights on the nodes -1, 1: h0 = (1-r)/2, h1 = (1+r)/2; contracted along r,
+ * then s, then t.  (r,s,t) is the location rule's Newton solution, not
+ * gslib's. */
+static void trilinear_interp(struct orc_picl *p, int ip) {
+  const double *xp = p->y + LRS * (size_t)ip;
+  int e = p->elem[ip];
+  double rst[3];
+  trilinear_inside(p->xm1b + 24 * (size_t)e, xp, rst);
+  double h[3][2];
+  for (int d = 0; d < 3; d++) {
+    h[d][0] = 0.5 * (1.0 - rst[d]);
+    h[d][1] = 0.5 * (1.0 + rst[d]);
+  }
+  for (int f = 0; f < ORC_PICL_NFLD; f++) {
+    double v = p->fld[ORC_PICL_NFLD * (size_t)e + f];
+    double u = h[0][0] * v + h[0][1] * v; /* every (j,k) line holds the same values */
+    double w = h[1][0] * u + h[1][1] * u;
+    p->rprop[LRP * (size_t)ip + orc_picl_field_rprop[f] - 1] = h[2][0] * w + h[2][1] * w;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* InitSolve: locate, interpolate, project the map built from the current    */
 /* ydotc, zero ydotc.  ppiclf_solve.f:1954-1992; map:                        */
@@ -278,23 +304,65 @@ static void init_solve(struct orc_picl *p, int interp) {
   }
   if (interp)
     for (int ip = 0; ip < p->n; ip++)
-      if (!p->lost[ip]) local_interp(p, ip);
-  double *fldb = calloc(NPRO * (size_t)p->ne, sizeof(double));
+      if (!p->lost[ip]) {
+        if (p->pp.interpMode == 1)
+          trilinear_interp(p, ip);
+        else
+          local_interp(p, ip);
+      }
+  double *fldb = calloc(8 * NPRO * (size_t)p->ne, sizeof(double));
+  double *map = malloc(sizeof(double) * NPRO * (size_t)(p->n > 0 ? p->n : 1));
   for (int ip = 0; ip < p->n; ip++) {
-    if (p->lost[ip]) continue;
     const double *rp = p->rprop + LRP * (size_t)ip, *yc = p->ydotc + LRS * (size_t)ip;
-    double map[NPRO];
-    map[P_JPHIP] = rp[R_JVOLP] * rp[R_JSPL];
-    map[P_JFX] = yc[JVX] * rp[R_JSPL];
-    map[P_JFY] = yc[JVY] * rp[R_JSPL];
-    map[P_JFZ] = yc[JVZ] * rp[R_JSPL];
-    map[P_JE] = yc[JT] * rp[R_JSPL];
-    int e = p->elem[ip];
-    double rexp = p->rexp[e];
-    for (int jj = 0; jj < NPRO; jj++)
-      fldb[NPRO * (size_t)e + jj] = fldb[NPRO * (size_t)e + jj] + (map[jj] * 1.0) * rexp;
+    double *m = map + NPRO * (size_t)ip;
+    m[P_JPHIP] = rp[R_JVOLP] * rp[R_JSPL];
+    m[P_JFX] = yc[JVX] * rp[R_JSPL];
+    m[P_JFY] = yc[JVY] * rp[R_JSPL];
+    m[P_JFZ] = yc[JVZ] * rp[R_JSPL];
+    m[P_JE] = yc[JT] * rp[R_JSPL];
   }
-  for (size_t q = 0; q < NPRO * (size_t)p->ne; q++) p->proFld[q] = 0.0 + fldb[q];
+  if (p->pp.projMode == 1) {
+    /* Gaussian filter, ppiclf_solve.f:3103-3109, 3118-3130, 3196-3246 with
+     * iwallm = 0: every element node sums the particles within the cut-off.
+     * The sums are compensated (Neumaier), so that their value does not
+     * depend on the order of the terms (ppiclF's order over its bins and
+     * ghost particles is not restated); here: ascending particle index. */
+    for (int e = 0; e < p->ne; e++)
+      for (int n = 0; n < 8; n++) {
+        const double *xn = p->xm1b + 24 * (size_t)e + 3 * n;
+        double *nb = fldb + 8 * NPRO * (size_t)e;
+        double cmp[NPRO] = {0.0};
+        for (int ip = 0; ip < p->n; ip++) {
+          if (p->lost[ip]) continue;
+          const double *xp = p->y + LRS * (size_t)ip;
+          double rdist2 = (xn[0] - xp[0]) * (xn[0] - xp[0]) + (xn[1] - xp[1]) * (xn[1] - xp[1]);
+          rdist2 = rdist2 + (xn[2] - xp[2]) * (xn[2] - xp[2]);
+          if (rdist2 > p->gD2chk) continue;
+          double rexp = exp(rdist2 * p->gRdum);
+          for (int jj = 0; jj < NPRO; jj++) {
+            double x = (map[NPRO * (size_t)ip + jj] * p->gMult) * rexp, s = nb[8 * jj + n], t = s + x;
+            if (fabs(s) >= fabs(x))
+              cmp[jj] = cmp[jj] + ((s - t) + x);
+            else
+              cmp[jj] = cmp[jj] + ((x - t) + s);
+            nb[8 * jj + n] = t;
+          }
+        }
+        for (int jj = 0; jj < NPRO; jj++) nb[8 * jj + n] = nb[8 * jj + n] + cmp[jj];
+      }
+  } else {
+    for (int ip = 0; ip < p->n; ip++) {
+      if (p->lost[ip]) continue;
+      int e = p->elem[ip];
+      double rexp = p->rexp[e];
+      for (int jj = 0; jj < NPRO; jj++)
+        for (int n = 0; n < 8; n++)
+          fldb[8 * (NPRO * (size_t)e + jj) + n] =
+              fldb[8 * (NPRO * (size_t)e + jj) + n] + (map[NPRO * (size_t)ip + jj] * 1.0) * rexp;
+    }
+  }
+  for (size_t q = 0; q < 8 * NPRO * (size_t)p->ne; q++) p->proFld[q] = 0.0 + fldb[q];
+  free(map);
   free(fldb);
   memset(p->ydotc, 0, sizeof(double) * LRS * (size_t)p->n);
 }
@@ -303,9 +371,8 @@ static void init_solve(struct orc_picl *p, int interp) {
 /* Forces                                                                    */
 /* ------------------------------------------------------------------------ */
 /* ppiclf_user_QS_Parmar: libpicl/user_files/ppiclf_user_QuasiSteady.f:22-122 */
-static double qs_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
-  const double rpi = acos(-1.0);
-  double mp = fmax(rmachp, 0.01), re = fmax(rep, 0.1), rcd1;
+static double parmar_rcd1(double mp, double re) {
+  double rcd1;
   if (re < F32(1E-14)) {
     rcd1 = 1.0;
   } else {
@@ -353,9 +420,136 @@ static double qs_parmar(double rmachp, double rphip, double rep, double rmu, dou
              (re / 24.0) * F32(0.646) / (1.0 + 861.0 / pow(re, F32(.634)));
     }
   }
+  return rcd1;
+}
+
+static double qs_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = acos(-1.0);
+  double mp = fmax(rmachp, 0.01), re = fmax(rep, 0.1);
+  double rcd1 = parmar_rcd1(mp, re);
   double phi_corr = 1.0 + F32(5.94) * fmin(rphip, 0.5);
   double beta = rcd1 * 3.0 * rpi * rmu * dp;
   return beta * phi_corr;
+}
+
+/* Integer powers as gfortran expands them (x**2 = x*x, x**3 = x2*x,
+ * x**4 = x2*x2, x**5 = x2*x3, x**(-3) = 1/x**3). */
+static double ip2(double x) { return x * x; }
+static double ip3(double x) { return (x * x) * x; }
+static double ip4(double x) { double x2 = x * x; return x2 * x2; }
+static double ip5(double x) { double x2 = x * x; return x2 * (x2 * x); }
+
+/* Osnes et al. volume-fraction terms, ppiclf_user_QuasiSteady.f:208-218, 346-356 */
+static void osnes_b(double phi, double re, double mp, double *b1, double *b2, double *b3) {
+  double om = 1.0 - phi;
+  *b1 = F32(5.81) * phi / ip2(om) + F32(0.48) * pow(phi, 1.0 / 3.0) / ip3(om);
+  *b2 = ip2(om) * ip3(phi) * re * (F32(0.95) + F32(0.61) * ip3(phi) / (om * 2.0));
+  *b3 = fmin(sqrt(20.0 * mp), 1.0) * (F32(5.65) * phi - 22.0 * ip2(phi) + F32(23.4) * ip3(phi)) *
+        (1.0 + tanh((mp - (F32(0.65) - F32(0.24) * phi)) / F32(0.35)));
+}
+
+/* Parmar's rcd1 (Re and Mach corrections), shared by QS_Parmar and
+ * QS_ModifiedParmar: ppiclf_user_QuasiSteady.f:43-100, 151-205 */
+/* ppiclf_user_QS_ModifiedParmar: ppiclf_user_QuasiSteady.f:125-230 */
+static double qs_modified_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = acos(-1.0);
+  double mp = fmax(rmachp, 0.01), phi = fmax(rphip, 0.0001), re = fmax(rep, 0.1);
+  double rcd1 = parmar_rcd1(mp, re);
+  double b1, b2, b3;
+  osnes_b(phi, re, mp, &b1, &b2, &b3);
+  double cd = (24.0 / re) * rcd1;
+  cd = cd / (1.0 - phi) + b3 + (24.0 / re) * (1.0 - phi) * (b1 + b2);
+  return 3.0 * rpi * rmu * dp * (re / 24.0) * cd;
+}
+
+/* ppiclf_user_QS_Osnes (Loth et al. 2021 + Osnes et al. 2023):
+ * ppiclf_user_QuasiSteady.f:233-362 */
+static double qs_osnes(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = acos(-1.0), gamma = 1.4;
+  double mp = fmax(rmachp, 0.01), phi = fmax(rphip, 0.0001), re = fmax(rep, 0.1);
+  double cd_loth;
+  if (re <= 45.0) {
+    double Knp = sqrt(0.5 * rpi * gamma) * mp / re, fKn;
+    if (Knp > F32(0.01))
+      fKn = 1.0 / (1.0 + Knp * (2.514 + 0.8 * exp(-0.55 / Knp)));
+    else
+      fKn = 1.0 / (1.0 + Knp * (2.514 + 0.8 * exp(-0.55 / F32(0.01))));
+    double CD1 = (24.0 / re) * (1.0 + 0.15 * pow(re, 0.687)) * fKn;
+    double s = mp * sqrt(0.5 * gamma);
+    double sgby2 = sqrt(0.5 * gamma);
+    double JMt;
+    if (mp <= 1.0)
+      JMt = 2.26 * ip4(mp) + 0.14 * mp;
+    else
+      JMt = 1.6 * ip4(mp) + 0.25 * ip3(mp) + 0.11 * ip2(mp) + 0.44 * mp;
+    double CD2 = (1.0 + 2.0 * ip2(s)) * exp(-ip2(s)) * mp / (ip3(sgby2) * sqrt(rpi)) +
+                 (4.0 * ip4(s) + 4.0 * ip2(s) - 1.0) * erf(s) / (2.0 * ip4(sgby2)) +
+                 (2.0 * ip3(mp) / (3.0 * sgby2)) * sqrt(rpi);
+    CD2 = CD2 / (1.0 + (((CD2 / JMt) - 1.0) * sqrt(re / 45.0)));
+    cd_loth = CD1 / (1.0 + ip4(mp)) + CD2 / (1.0 + ip4(mp));
+  } else {
+    double CM, GM, HM;
+    if (mp < 1.5)
+      CM = 1.65 + 0.65 * tanh(4.0 * mp - 3.4);
+    else
+      CM = 2.18 - 0.12913149918318745 * tanh(0.9 * mp - 2.7);
+    if (mp < F32(0.8))
+      GM = 166.0 * ip3(mp) + 3.29 * ip2(mp) - 10.9 * mp + 20.0;
+    else
+      GM = 5.0 + 47.809331200000017 * (1.0 / ip3(mp));
+    if (mp < 1.0)
+      HM = 0.0239 * ip3(mp) + 0.212 * ip2(mp) - 0.074 * mp + 1.0;
+    else
+      HM = 0.93967777777777772 + 1.0 / (3.5 + ip5(mp));
+    cd_loth = (24.0 / re) * (1.0 + F32(0.15) * pow(re, F32(0.687))) * HM +
+              F32(0.42) * CM / (1.0 + 42500.0 / pow(re, F32(1.16) * CM) + GM / sqrt(re));
+  }
+  double b1, b2, b3;
+  osnes_b(phi, re, mp, &b1, &b2, &b3);
+  double cd = cd_loth / (1.0 - phi) + b3 + (24.0 / re) * (1.0 - phi) * (b1 + b2);
+  return 3.0 * rpi * rmu * dp * (re / 24.0) * cd;
+}
+
+/* ppiclf_user_QS_Gidaspow (Ergun / Wen-Yu, per particle):
+ * ppiclf_user_QuasiSteady.f:364-397 */
+static double qs_gidaspow(double rphip, double rphif, double rep, double rmu, double dp, double rhof,
+                          double vmag) {
+  const double rpi = acos(-1.0);
+  double phi = fmax(rphip, 0.0001), phif = fmax(rphif, 0.0001), re = fmax(rep, 0.1);
+  double phifRep = phif * re, cd, beta;
+  if (phifRep < 1000.0)
+    cd = 24.0 / phifRep * (1.0 + F32(0.15) * pow(phifRep, F32(0.687)));
+  else
+    cd = F32(0.44);
+  if (phif < F32(0.8))
+    beta = 150.0 * (ip2(phi) * rmu) / (phif * ip2(dp)) + 1.75 * (rhof * phi * vmag / dp);
+  else
+    beta = F32(0.75) * cd * phi * rhof * vmag / (dp * pow(phif, F32(1.65)));
+  return beta * (rpi * ip3(dp)) / (6.0 * phi);
+}
+
+/* Schiller-Naumann: rocinteract's INRT_DRAG_MODEL_SN correction
+ * psi = 1 + 0.15 Re^0.687 (rocinteract/INRT_CalcDrag.F90:189-191) in
+ * ppiclF's quasi-steady slot, beta = psi * 3 pi mu d_p (the form of
+ * ppiclf_user_QuasiSteady.f:113); no volume-fraction correction.  Not a
+ * ppiclF model: SetYdot rejects qs_flag 5. */
+static double qs_sn(double rep, double rmu, double dp) {
+  const double rpi = acos(-1.0);
+  double re = fmax(rep, 0.1);
+  double psi = 1.0 + 0.15 * pow(re, 0.687);
+  return psi * 3.0 * rpi * rmu * dp;
+}
+
+double orc_picl_qs_beta(int qsFlag, double rmachp, double rphip, double rep, double rmu, double dp,
+                        double rhof, double vmag) {
+  switch (qsFlag) {
+    case 1: return qs_parmar(rmachp, rphip, rep, rmu, dp);
+    case 2: return qs_osnes(rmachp, rphip, rep, rmu, dp);
+    case 3: return qs_modified_parmar(rmachp, rphip, rep, rmu, dp);
+    case 4: return qs_gidaspow(rphip, 1.0 - rphip, rep, rmu, dp, rhof, vmag);
+    case 5: return qs_sn(rep, rmu, dp);
+  }
+  return 0.0;
 }
 
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
@@ -394,7 +588,14 @@ static void set_ydot(struct orc_picl *p, int i) {
   double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
   const double fcx = 0.0, fcy = 0.0, fcz = 0.0, fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
   const double liftx = 0.0, lifty = 0.0, liftz = 0.0, qq = 0.0;
-  double beta = qs_parmar(rmachp, rphip, rep, rmu, dp);
+  double beta; /* ppiclf_user.f:473-485 */
+  switch (P->qsFlag) {
+    case 1: beta = qs_parmar(rmachp, rphip, rep, rmu, dp); break;
+    case 2: beta = qs_osnes(rmachp, rphip, rep, rmu, dp); break;
+    case 3: beta = qs_modified_parmar(rmachp, rphip, rep, rmu, dp); break;
+    case 4: beta = qs_gidaspow(rphip, rphif, rep, rmu, dp, rhof, vmag); break;
+    default: beta = qs_sn(rep, rmu, dp); break;
+  }
   double fqsx = beta * vx, fqsy = beta * vy, fqsz = beta * vz;
   fqsx = fqsx + 0.0; /* + fqs_fluct (zero) */
   fqsy = fqsy + 0.0;
@@ -483,10 +684,16 @@ static double sum8(double v) {
   for (int q = 0; q < 8; q++) s = s + v;
   return s;
 }
-/* Rocflu's read-back of a projected volume fraction, :826-842 and :1217-1234 */
-static double phip_of(double vfP, double vol) {
+/* Sum over the element's nodes in the read-back's lz, ly, lx loop order */
+static double sum_nodes(const double *v8) {
   double s = 0.0;
-  for (int q = 0; q < 8; q++) s = s + (0.125 * vfP) * vol;
+  for (int q = 0; q < 8; q++) s = s + v8[q];
+  return s;
+}
+/* Rocflu's read-back of a projected volume fraction, :826-842 and :1217-1234 */
+static double phip_of(const double *vfP8, double vol) {
+  double s = 0.0;
+  for (int q = 0; q < 8; q++) s = s + (0.125 * vfP8[q]) * vol;
   s = s / vol;
   if (s > F32(0.62)) s = F32(0.62);
   return s;
@@ -501,7 +708,7 @@ int orc__picl_runge(orc_region *r, int istage) {
     const double *cv = r->cv + 5 * (size_t)i, *dv = r->dv + 3 * (size_t)i;
     const double *rhs = r->rhs + 5 * (size_t)i, *g = r->grad + 15 * (size_t)i;
     double vol = r->vol[i];
-    double PhiP = phip_of(p->proFld[NPRO * (size_t)i + P_JPHIP], vol);
+    double PhiP = phip_of(p->proFld + 8 * (NPRO * (size_t)i + P_JPHIP), vol);
     double ug[3] = {cv[1] / cv[0], cv[2] / cv[0], cv[3] / cv[0]};
     double sdr[3];
     for (int m = 0; m < 3; m++) {
@@ -525,17 +732,20 @@ int orc__picl_runge(orc_region *r, int istage) {
     f[17] = r->gradRhog[3 * (size_t)i];
     f[18] = r->gradRhog[3 * (size_t)i + 1];
     f[19] = r->gradRhog[3 * (size_t)i + 2];
-    for (int q = 0; q < ORC_PICL_NFLD; q++) p->fld[ORC_PICL_NFLD * (size_t)i + q] = sum8(f[q]) / 8;
+    /* IDW reads LocalInterp's node average; the trilinear evaluation reads
+     * the nodes themselves */
+    for (int q = 0; q < ORC_PICL_NFLD; q++)
+      p->fld[ORC_PICL_NFLD * (size_t)i + q] = p->pp.interpMode == 1 ? f[q] : sum8(f[q]) / 8;
   }
   integrate_particle(p, istage, r->dtMin);
   if (p->pp.feedbackFlag == 1) { /* :926-1038 */
     for (int i = 0; i < N; i++) {
-      const double *pf = p->proFld + NPRO * (size_t)i;
+      const double *pf = p->proFld + 8 * NPRO * (size_t)i;
       double vol = r->vol[i];
-      double jfx = sum8(pf[P_JFX]) * 0.125 * vol;
-      double jfy = sum8(pf[P_JFY]) * 0.125 * vol;
-      double jfz = sum8(pf[P_JFZ]) * 0.125 * vol;
-      double jfe = sum8(pf[P_JE]) * 0.125 * vol;
+      double jfx = sum_nodes(pf + 8 * P_JFX) * 0.125 * vol;
+      double jfy = sum_nodes(pf + 8 * P_JFY) * 0.125 * vol;
+      double jfz = sum_nodes(pf + 8 * P_JFZ) * 0.125 * vol;
+      double jfe = sum_nodes(pf + 8 * P_JE) * 0.125 * vol;
       double *rhs = r->rhs + 5 * (size_t)i;
       rhs[1] = rhs[1] + jfx;
       rhs[2] = rhs[2] + jfy;
@@ -543,7 +753,7 @@ int orc__picl_runge(orc_region *r, int istage) {
       rhs[4] = rhs[4] + jfe;
     }
   }
-  for (int i = 0; i < N; i++) r->phi[i] = phip_of(p->proFld[NPRO * (size_t)i + P_JPHIP], r->vol[i]);
+  for (int i = 0; i < N; i++) r->phi[i] = phip_of(p->proFld + 8 * (NPRO * (size_t)i + P_JPHIP), r->vol[i]);
   return 0;
 }
 
@@ -551,8 +761,24 @@ int orc__picl_runge(orc_region *r, int istage) {
 int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const double *y,
                   const double *pconst) {
   if (r->nvc != 8 || r->p.spaceOrder != 2) return -1;
+  if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
+      pp->projMode > 1)
+    return -1;
+  if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0)) return -1;
   struct orc_picl *p = calloc(1, sizeof *p);
   p->pp = *pp;
+  if (pp->projMode == 1) {
+    /* ppiclf_solve_InitGaussianFilter (:1520-1576) and the constants of
+     * ProjectParticleGrid (:3103-3109) */
+    const double pi = 4.0 * atan(1.0);
+    double rsig = pp->filterWidth / (2.0 * sqrt(2.0 * log(2.0)));
+    double d2chk = rsig * sqrt(-2.0 * log(pp->filterAlpha));
+    double s2 = sqrt(2.0 * pi);
+    double multfci = 1.0 / ((s2 * s2) * (rsig * rsig));
+    p->gMult = pow(multfci, 1.5);
+    p->gRdum = 1.0 / (-2.0 * (rsig * rsig));
+    p->gD2chk = d2chk * d2chk;
+  }
   p->n = nPart;
   size_t n = nPart > 0 ? (size_t)nPart : 1;
   p->y = malloc(sizeof(double) * LRS * n);
@@ -577,7 +803,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   r->gradRhog = calloc(3 * (size_t)N, sizeof(double));
   init_solve(p, 0); /* ppiclf_comm_InitOverlapMesh -> InitSolve (no fields yet) */
   /* PICL_TEMP_InitSolver.F90:789-835 */
-  for (int i = 0; i < N; i++) r->phi[i] = phip_of(p->proFld[NPRO * (size_t)i + P_JPHIP], r->vol[i]);
+  for (int i = 0; i < N; i++) r->phi[i] = phip_of(p->proFld + 8 * (NPRO * (size_t)i + P_JPHIP), r->vol[i]);
   for (int i = 0; i < N; i++) {
     double vFrac = 1.0 - r->phi[i];
     for (int k = 0; k < 5; k++) r->cv[5 * (size_t)i + k] = vFrac * r->cv[5 * (size_t)i + k];
@@ -612,8 +838,13 @@ void orc_picl_get_cells(const orc_region *r, double *phi, double *gradRhog, doub
   int N = r->nCells;
   if (phi) memcpy(phi, r->phi, sizeof(double) * (size_t)N);
   if (gradRhog) memcpy(gradRhog, r->gradRhog, sizeof(double) * 3 * (size_t)N);
-  if (proFld) memcpy(proFld, r->picl->proFld, sizeof(double) * NPRO * (size_t)N);
+  if (proFld)
+    for (int i = 0; i < N; i++)
+      for (int jj = 0; jj < NPRO; jj++) proFld[NPRO * (size_t)i + jj] = r->picl->proFld[8 * (NPRO * (size_t)i + jj)];
   if (fields) memcpy(fields, r->picl->fld, sizeof(double) * ORC_PICL_NFLD * (size_t)N);
+}
+void orc_picl_get_nodes(const orc_region *r, double *proNode) {
+  memcpy(proNode, r->picl->proFld, sizeof(double) * 8 * NPRO * (size_t)r->nCells);
 }
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem) {
   for (int i = 0; i < n; i++) elem[i] = locate(r->picl, xyz + 3 * (size_t)i);

@@ -3,9 +3,11 @@
  *
  * Restates PICL_TEMP_InitSolver / PICL_TEMP_Runge (rocpicl/) and the ppiclF
  * routines they drive for one rank: element location, LocalInterp (IDW over
- * the 27 nearest element centroids), ppiclf_user_SetYdot with QS Parmar,
- * AM Parmar and the pressure-gradient force, IntegrateRK3s_Rocflu, and the
- * single-element box projection with its read-back into rhs and piclVF.
+ * the 27 nearest element centroids) or the trilinear findpts_eval_local,
+ * ppiclf_user_SetYdot with the QS laws of ppiclf_user_QuasiSteady.f, AM
+ * Parmar and the pressure-gradient force, IntegrateRK3s_Rocflu, and the
+ * single-element box or Gaussian-filter projection with its read-back into
+ * rhs and piclVF.
  * Only tests/ and bench.py's CPU-baseline leg use it, as the checker.
  *
  * Particle arrays use ppiclF's layout: y(PPICLF_LRS=12, n) -> y[12*p + j],
@@ -25,10 +27,15 @@ typedef struct orc_picl_params {
   double tref;       /* Sutherland reference temperature */
   double suth;       /* Sutherland constant */
   double rcpPart;    /* particle heat capacity (ppiclf_rcp_part) */
-  int qsFlag;        /* 1: QS Parmar (the model restated here) */
+  int qsFlag;        /* 1 Parmar, 2 Osnes, 3 modified Parmar, 4 Gidaspow,
+                        5 Schiller-Naumann (INRT_DRAG_MODEL_SN in ppiclF's slot) */
   int amFlag;        /* 0: off, 1: AM Parmar */
   int pgFlag;        /* 0: off, 1: pressure-gradient force */
   int feedbackFlag;  /* 0: one-way, 1: two-way coupling */
+  int interpMode;    /* 0: IDW-27 (LocalInterp); 1: trilinear (findpts_eval_local) */
+  int projMode;      /* 0: single-element box; 1: Gaussian filter */
+  double filterWidth;  /* Gaussian filter width (full width at half maximum) */
+  double filterAlpha;  /* Gaussian cut-off: weights below alpha are dropped */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index
@@ -52,6 +59,13 @@ void orc_picl_get_rprop(const orc_region *r, double *rprop);
  * (PHIP, FX, FY, FZ, E) (5), interpolation inputs of the last stage (20). */
 void orc_picl_get_cells(const orc_region *r, double *phi, double *gradRhog, double *proFld,
                         double *fields);
+/* Projected values at the 8 nodes of every element: 40 per cell, field-major
+ * ((PHIP, FX, FY, FZ, E) x ppiclF node n = i + 2j + 4k). */
+void orc_picl_get_nodes(const orc_region *r, double *proNode);
+/* The quasi-steady coefficient beta (force = beta * v_rel) of model qsFlag
+ * for one particle, from SetYdot's capped inputs. */
+double orc_picl_qs_beta(int qsFlag, double rmachp, double rphip, double rep, double rmu, double dp,
+                        double rhof, double vmag);
 /* Element containing each point (-1: none), by the oracle's location rule. */
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
 /* Elements LocalInterp sums for point xp, nearest first; returns how many. */

@@ -24,7 +24,7 @@ RFLU_SYMBOLS = [
 # Every symbol include/picl_gpu.h declares.
 PICL_SYMBOLS = [
     "picl_gpu_init", "picl_gpu_runge", "picl_gpu_sizes", "picl_gpu_get_particles", "picl_gpu_get_cells",
-    "picl_gpu_get_interp", "picl_gpu_set_debug", "picl_gpu_locate",
+    "picl_gpu_get_interp", "picl_gpu_set_debug", "picl_gpu_locate", "picl_gpu_get_nodes",
 ]
 
 ERRORS = {1: "EINVAL", 2: "ENODEV", 3: "EHIP", 4: "EMESH", 5: "ENAN", 6: "ENEGATIVE"}
@@ -42,13 +42,15 @@ class Params(C.Structure):
                 ("patchOrder", C.c_int), ("cfl", C.c_double), ("dtImposed", C.c_double),
                 ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
                 ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
-                ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int)]
+                ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int),
+                ("fluxPrecision", C.c_int)]
 
 
 class PiclParams(C.Structure):
     _fields_ = [("rmuFlag", C.c_int), ("rmuRef", C.c_double), ("tref", C.c_double), ("suth", C.c_double),
                 ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
-                ("feedbackFlag", C.c_int)]
+                ("feedbackFlag", C.c_int), ("interpMode", C.c_int), ("projMode", C.c_int),
+                ("filterWidth", C.c_double), ("filterAlpha", C.c_double)]
 
 
 _lib = None
@@ -94,6 +96,7 @@ def load():
     L.picl_gpu_get_particles.argtypes = [P, P, P, P, P, P]
     L.picl_gpu_get_cells.argtypes = [P, P, P, P, P]
     L.picl_gpu_get_interp.argtypes = [P, P]
+    L.picl_gpu_get_nodes.argtypes = [P, P]
     L.picl_gpu_set_debug.argtypes = [P, I]
     L.picl_gpu_locate.argtypes = [P, I, P, P]
     for n in RFLU_SYMBOLS + PICL_SYMBOLS:

@@ -31,7 +31,7 @@
 namespace {
 
 struct PiclPhys {
-  int rmuFlag, amFlag, pgFlag, feedbackFlag;
+  int rmuFlag, amFlag, pgFlag, feedbackFlag, qsFlag, interpMode;
   double rmuRef, tref, suth, rcpPart;
 };
 
@@ -67,6 +67,16 @@ struct PiclDev {
   double *cbRec = nullptr;  // per bin member: cx, cy, cz, (reference id << 32 | device id)
   double d2Max[3] = {0, 0, 0};
   double *dbg = nullptr;  // [20][nP] interpolated values (debug)
+  // Gaussian-filter projection (gauss = 1): node values per mesh vertex
+  int gauss = 0, nV = 0, nCB = 0;
+  double *vX = nullptr;     // [3][nV] vertex coordinates
+  int *ev = nullptr;        // [8][nC] vertex of ppiclF node n of each (device) element
+  int *ecb = nullptr;       // centroid bin of each element
+  int *binCnt = nullptr;    // particles per centroid bin (this stage)
+  double *gq = nullptr;     // [nP][8] by sorted position: x, y, z, map * multfci (5)
+  double *xo = nullptr;     // [3][nP] position at the start of the stage, by current slot
+  double *vproj = nullptr;  // [5][nV]
+  double gRdum = 0, gMult = 0, gD2chk = 0, gReach = 0, gReach2 = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -74,7 +84,8 @@ struct PiclDev {
 // point within |r|,|s|,|t| <= 1 + 1e-10 (Newton from the element centre);
 // the stand-in for gslib findpts.  Same operations as the oracle.
 // ---------------------------------------------------------------------------
-__device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3], double *maxrst = nullptr) {
+__device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3], double *maxrst = nullptr,
+                                 double *rst = nullptr) {
   double r = 0.0, s = 0.0, t = 0.0;
   for (int it = 0; it < 25; it++) {
     double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
@@ -111,6 +122,7 @@ __device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3
   }
   const double lim = 1.0 + 1.0e-10;
   if (maxrst) *maxrst = fmax(fabs(r), fmax(fabs(s), fabs(t)));
+  if (rst) { rst[0] = r; rst[1] = s; rst[2] = t; }
   return fabs(r) <= lim && fabs(s) <= lim && fabs(t) <= lim;
 }
 
@@ -212,16 +224,30 @@ __device__ __forceinline__ double sum8(double v) {
   for (int q = 0; q < 8; q++) s = s + v;
   return s;
 }
-__device__ __forceinline__ double phip_of(double vfP, double vol) {
+// Projected field k at ppiclF node n of (device) element i: the box deposit
+// gives all 8 nodes the element's value; the Gaussian filter one per vertex.
+__device__ __forceinline__ double node_val(const PiclDev &P, int k, int i, int n) {
+  return P.gauss ? P.vproj[(size_t)k * P.nV + P.ev[(size_t)n * P.nC + i]] : P.proj[(size_t)k * P.nC + i];
+}
+// Rocflu's read-backs loop over the nodes lz, ly, lx (PICL_TEMP_Runge.F90:923-1000, 1217-1234)
+__device__ __forceinline__ double phip_of(const PiclDev &P, int i, double vol) {
   double s = 0.0;
 #pragma unroll
-  for (int q = 0; q < 8; q++) s = s + (0.125 * vfP) * vol;
+  for (int q = 0; q < 8; q++) s = s + (0.125 * node_val(P, 0, i, q)) * vol;
   s = s / vol;
   if (s > F32(0.62)) s = F32(0.62);
   return s;
 }
+__device__ __forceinline__ double sum_nodes(const PiclDev &P, int k, int i) {
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) s = s + node_val(P, k, i, q);
+  return s;
+}
 
-__global__ void __launch_bounds__(BLK) k_picl_fields(Dev d, Gas G, PiclDev P) {
+// tri = 1 (trilinear interpolation) keeps the node value itself: the
+// evaluation reads the nodes, not LocalInterp's average.
+__global__ void __launch_bounds__(BLK) k_picl_fields(Dev d, Gas G, PiclDev P, int tri) {
   int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
@@ -259,7 +285,7 @@ __global__ void __launch_bounds__(BLK) k_picl_fields(Dev d, Gas G, PiclDev P) {
   f[4] = gg[12]; f[5] = gg[13]; f[6] = gg[14];
   f[7] = sqrt(G.gD * G.gcD * T);
   f[8] = T;
-  f[9] = phip_of(P.proj[i], vol);
+  f[9] = phip_of(P, i, vol);
   f[10] = sdr[0]; f[11] = sdr[1]; f[12] = sdr[2];
   f[13] = -(rh[0] / vol);
   f[14] = gg[0]; f[15] = gg[1]; f[16] = gg[2];
@@ -267,16 +293,16 @@ __global__ void __launch_bounds__(BLK) k_picl_fields(Dev d, Gas G, PiclDev P) {
   f[17] = q[0]; f[18] = q[1]; f[19] = q[2];
   double2 *o = reinterpret_cast<double2 *>(P.fld + PICL_NFLD * (size_t)i);
 #pragma unroll
-  for (int k = 0; k < PICL_NFLD / 2; k++) o[k] = make_double2(sum8(f[2 * k]) / 8, sum8(f[2 * k + 1]) / 8);
+  for (int k = 0; k < PICL_NFLD / 2; k++)
+    o[k] = tri ? make_double2(f[2 * k], f[2 * k + 1]) : make_double2(sum8(f[2 * k]) / 8, sum8(f[2 * k + 1]) / 8);
 }
 
 // ---------------------------------------------------------------------------
 // Forces: ppiclf_user_QS_Parmar (libpicl/user_files/ppiclf_user_QuasiSteady.f:22-122);
 // real*4 literals widened as the reference's compiler does.
 // ---------------------------------------------------------------------------
-__device__ double qs_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
-  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
-  double mp = fmax(rmachp, 0.01), re = fmax(rep, 0.1), rcd1;
+__device__ double parmar_rcd1(double mp, double re) {
+  double rcd1;
   if (re < F32(1E-14)) {
     rcd1 = 1.0;
   } else {
@@ -323,9 +349,121 @@ __device__ double qs_parmar(double rmachp, double rphip, double rep, double rmu,
       }
     }
   }
+  return rcd1;
+}
+
+__device__ double qs_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  double mp = fmax(rmachp, 0.01), re = fmax(rep, 0.1);
+  double rcd1 = parmar_rcd1(mp, re);
   double phi_corr = 1.0 + F32(5.94) * fmin(rphip, 0.5);
   double beta = rcd1 * 3.0 * rpi * rmu * dp;
   return beta * phi_corr;
+}
+
+// Integer powers as gfortran expands them (x**3 = x2*x, x**4 = x2*x2,
+// x**5 = x2*x3, x**(-3) = 1/x**3).
+__device__ __forceinline__ double ip2(double x) { return x * x; }
+__device__ __forceinline__ double ip3(double x) { return (x * x) * x; }
+__device__ __forceinline__ double ip4(double x) { double x2 = x * x; return x2 * x2; }
+__device__ __forceinline__ double ip5(double x) { double x2 = x * x; return x2 * (x2 * x); }
+
+// Osnes et al. volume-fraction terms (ppiclf_user_QuasiSteady.f:208-218, 346-356)
+__device__ void osnes_b(double phi, double re, double mp, double &b1, double &b2, double &b3) {
+  double om = 1.0 - phi;
+  b1 = F32(5.81) * phi / ip2(om) + F32(0.48) * pow(phi, 1.0 / 3.0) / ip3(om);
+  b2 = ip2(om) * ip3(phi) * re * (F32(0.95) + F32(0.61) * ip3(phi) / (om * 2.0));
+  b3 = fmin(sqrt(20.0 * mp), 1.0) * (F32(5.65) * phi - 22.0 * ip2(phi) + F32(23.4) * ip3(phi)) *
+       (1.0 + tanh((mp - (F32(0.65) - F32(0.24) * phi)) / F32(0.35)));
+}
+
+// ppiclf_user_QS_ModifiedParmar (ppiclf_user_QuasiSteady.f:125-230)
+__device__ double qs_modified_parmar(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  double mp = fmax(rmachp, 0.01), phi = fmax(rphip, 0.0001), re = fmax(rep, 0.1);
+  double rcd1 = parmar_rcd1(mp, re);
+  double b1, b2, b3;
+  osnes_b(phi, re, mp, b1, b2, b3);
+  double cd = (24.0 / re) * rcd1;
+  cd = cd / (1.0 - phi) + b3 + (24.0 / re) * (1.0 - phi) * (b1 + b2);
+  return 3.0 * rpi * rmu * dp * (re / 24.0) * cd;
+}
+
+// ppiclf_user_QS_Osnes, Loth et al. drag with Osnes et al. volume fraction
+// terms (ppiclf_user_QuasiSteady.f:233-362)
+__device__ double qs_osnes(double rmachp, double rphip, double rep, double rmu, double dp) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  const double gamma = 1.4;
+  double mp = fmax(rmachp, 0.01), phi = fmax(rphip, 0.0001), re = fmax(rep, 0.1);
+  double cd_loth;
+  if (re <= 45.0) {
+    double Knp = sqrt(0.5 * rpi * gamma) * mp / re, fKn;
+    if (Knp > F32(0.01))
+      fKn = 1.0 / (1.0 + Knp * (2.514 + 0.8 * exp(-0.55 / Knp)));
+    else
+      fKn = 1.0 / (1.0 + Knp * (2.514 + 0.8 * exp(-0.55 / F32(0.01))));
+    double CD1 = (24.0 / re) * (1.0 + 0.15 * pow(re, 0.687)) * fKn;
+    double sq = mp * sqrt(0.5 * gamma);
+    double sgby2 = sqrt(0.5 * gamma);
+    double JMt;
+    if (mp <= 1.0)
+      JMt = 2.26 * ip4(mp) + 0.14 * mp;
+    else
+      JMt = 1.6 * ip4(mp) + 0.25 * ip3(mp) + 0.11 * ip2(mp) + 0.44 * mp;
+    double CD2 = (1.0 + 2.0 * ip2(sq)) * exp(-ip2(sq)) * mp / (ip3(sgby2) * sqrt(rpi)) +
+                 (4.0 * ip4(sq) + 4.0 * ip2(sq) - 1.0) * erf(sq) / (2.0 * ip4(sgby2)) +
+                 (2.0 * ip3(mp) / (3.0 * sgby2)) * sqrt(rpi);
+    CD2 = CD2 / (1.0 + (((CD2 / JMt) - 1.0) * sqrt(re / 45.0)));
+    cd_loth = CD1 / (1.0 + ip4(mp)) + CD2 / (1.0 + ip4(mp));
+  } else {
+    double CM, GM, HM;
+    if (mp < 1.5)
+      CM = 1.65 + 0.65 * tanh(4.0 * mp - 3.4);
+    else
+      CM = 2.18 - 0.12913149918318745 * tanh(0.9 * mp - 2.7);
+    if (mp < F32(0.8))
+      GM = 166.0 * ip3(mp) + 3.29 * ip2(mp) - 10.9 * mp + 20.0;
+    else
+      GM = 5.0 + 47.809331200000017 * (1.0 / ip3(mp));
+    if (mp < 1.0)
+      HM = 0.0239 * ip3(mp) + 0.212 * ip2(mp) - 0.074 * mp + 1.0;
+    else
+      HM = 0.93967777777777772 + 1.0 / (3.5 + ip5(mp));
+    cd_loth = (24.0 / re) * (1.0 + F32(0.15) * pow(re, F32(0.687))) * HM +
+              F32(0.42) * CM / (1.0 + 42500.0 / pow(re, F32(1.16) * CM) + GM / sqrt(re));
+  }
+  double b1, b2, b3;
+  osnes_b(phi, re, mp, b1, b2, b3);
+  double cd = cd_loth / (1.0 - phi) + b3 + (24.0 / re) * (1.0 - phi) * (b1 + b2);
+  return 3.0 * rpi * rmu * dp * (re / 24.0) * cd;
+}
+
+// ppiclf_user_QS_Gidaspow, Ergun / Wen-Yu per particle (ppiclf_user_QuasiSteady.f:364-397)
+__device__ double qs_gidaspow(double rphip, double rphif, double rep, double rmu, double dp, double rhof,
+                              double vmag) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  double phi = fmax(rphip, 0.0001), phif = fmax(rphif, 0.0001), re = fmax(rep, 0.1);
+  double phifRep = phif * re, cd, beta;
+  if (phifRep < 1000.0)
+    cd = 24.0 / phifRep * (1.0 + F32(0.15) * pow(phifRep, F32(0.687)));
+  else
+    cd = F32(0.44);
+  if (phif < F32(0.8))
+    beta = 150.0 * (ip2(phi) * rmu) / (phif * ip2(dp)) + 1.75 * (rhof * phi * vmag / dp);
+  else
+    beta = F32(0.75) * cd * phi * rhof * vmag / (dp * pow(phif, F32(1.65)));
+  return beta * (rpi * ip3(dp)) / (6.0 * phi);
+}
+
+// Schiller-Naumann: rocinteract's INRT_DRAG_MODEL_SN correction
+// psi = 1 + 0.15 Re^0.687 (rocinteract/INRT_CalcDrag.F90:189-191) in ppiclF's
+// quasi-steady slot, beta = psi * 3 pi mu d_p (ppiclf_user_QuasiSteady.f:113),
+// no volume-fraction correction.
+__device__ double qs_sn(double rep, double rmu, double dp) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  double re = fmax(rep, 0.1);
+  double psi = 1.0 + 0.15 * pow(re, 0.687);
+  return psi * 3.0 * rpi * rmu * dp;
 }
 
 // ---------------------------------------------------------------------------
@@ -341,6 +479,9 @@ __device__ __forceinline__ bool key_less(double a, int ra, double b, int rb) {
 
 __device__ __forceinline__ PBuf cur_buf(const PiclDev &P) { return P.par[0] ? P.buf[1] : P.buf[0]; }
 
+// TRI: trilinear interpolation instead of IDW-27; QS: the quasi-steady law
+// (ppiclf_user.f:473-485 numbering, 5 = Schiller-Naumann)
+template <int TRI, int QS>
 __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
@@ -356,7 +497,33 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
     return;
   }
   atomicAdd(&P.cnt[e], 1);
-
+  if (P.gauss) {  // the Gaussian projection reads the position at the start of the stage
+#pragma unroll
+    for (int k = 0; k < 3; k++) P.xo[k * nP + t] = xp[k];
+  }
+  double v[PICL_NFLD];
+  if (TRI) {
+    // gslib findpts_eval_local on the element's 2x2x2 nodal field (every node
+    // holds the cell value): Lagrange weights (1-r)/2, (1+r)/2 on the nodes
+    // -1, 1, contracted along r, then s, then t
+    double rst[3];
+    trilinear_inside(P.X + 24 * (size_t)e, xp, nullptr, rst);
+    double h[3][2];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      h[l][0] = 0.5 * (1.0 - rst[l]);
+      h[l][1] = 0.5 * (1.0 + rst[l]);
+    }
+    const double2 *fp = reinterpret_cast<const double2 *>(P.fld + PICL_NFLD * (size_t)e);
+#pragma unroll
+    for (int f = 0; f < PICL_NFLD / 2; f++) {
+      double2 a = fp[f];
+      double ua = h[0][0] * a.x + h[0][1] * a.x, ub = h[0][0] * a.y + h[0][1] * a.y;
+      double wa = h[1][0] * ua + h[1][1] * ua, wb = h[1][0] * ub + h[1][1] * ub;
+      v[2 * f] = h[2][0] * wa + h[2][1] * wa;
+      v[2 * f + 1] = h[2][0] * wb + h[2][1] * wb;
+    }
+  } else {
   // ---- the 27 nearest element centroids, ordered by (d2, element number)
   double kd[27];
   int kr[27];
@@ -415,7 +582,6 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
     atomicOr(d.err, E_INTERP);
     return;
   }
-  double v[PICL_NFLD];
 #pragma unroll
   for (int f = 0; f < PICL_NFLD; f++) v[f] = 0.0;
   double wsum = 0.0;
@@ -436,6 +602,7 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
   }
 #pragma unroll
   for (int f = 0; f < PICL_NFLD; f++) v[f] = v[f] / wsum;
+  }
   if (P.dbg) {
     int pid = C.pid[t];
 #pragma unroll
@@ -469,7 +636,14 @@ __global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph
     double rphif = 1.0 - rphip;
     double famx = 0.0, famy = 0.0, famz = 0.0, rmass_add = 0.0;
     double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
-    double beta = qs_parmar(rmachp, rphip, rep, rmu, dpp);
+    double beta;  // ppiclf_user.f:473-485
+    switch (QS) {
+      case 1: beta = qs_parmar(rmachp, rphip, rep, rmu, dpp); break;
+      case 2: beta = qs_osnes(rmachp, rphip, rep, rmu, dpp); break;
+      case 3: beta = qs_modified_parmar(rmachp, rphip, rep, rmu, dpp); break;
+      case 4: beta = qs_gidaspow(rphip, rphif, rep, rmu, dpp, rhof, vmag); break;
+      default: beta = qs_sn(rep, rmu, dpp); break;
+    }
     double fqsx = beta * vxr, fqsy = beta * vyr, fqsz = beta * vzr;
     fqsx = fqsx + 0.0;
     fqsy = fqsy + 0.0;
@@ -657,7 +831,10 @@ __global__ void __launch_bounds__(BLK) k_fill(PiclDev P) {
 // ---------------------------------------------------------------------------
 #define DEP_T 128
 #define DEP_CAP 64
-__global__ void __launch_bounds__(DEP_T) k_deposit(PiclDev P) {
+// Gaussian mode (P.gauss): the sorted segments instead receive one record per
+// particle, position at the start of the stage and map * multfci, read by
+// k_gauss.  posXo = 0 reads the position from the particle rows (set-up).
+__global__ void __launch_bounds__(DEP_T) k_deposit(PiclDev P, int posXo) {
   __shared__ long long sk[DEP_T * DEP_CAP];
   int e = blockIdx.x * DEP_T + threadIdx.x;
   if (e >= P.nC) return;
@@ -698,9 +875,91 @@ __global__ void __launch_bounds__(DEP_T) k_deposit(PiclDev P) {
 #pragma unroll
     for (int k = 0; k < 5; k++) acc[k] = acc[k] + (map[k] * 1.0) * rexp;
     P.dest[t] = b + q;
+    if (P.gauss) {
+      const double *xs = posXo ? P.xo : C.st;
+      double2 *g = reinterpret_cast<double2 *>(P.gq + 8 * (size_t)(b + q));
+      g[0] = make_double2(xs[t], xs[nP + t]);
+      g[1] = make_double2(xs[2 * nP + t], map[0] * P.gMult);
+      g[2] = make_double2(map[1] * P.gMult, map[2] * P.gMult);
+      g[3] = make_double2(map[3] * P.gMult, map[4] * P.gMult);
+    }
   }
 #pragma unroll
   for (int k = 0; k < 5; k++) P.proj[k * (size_t)P.nC + e] = 0.0 + acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// ProjectParticleGrid with the Gaussian filter (ppiclf_solve.f:3103-3109,
+// 3196-3246, iwallm = 0): every mesh vertex sums map * multfci *
+// exp(-r^2 / (2 sigma^2)) over the particles within the cut-off.  One thread
+// per vertex gathers from the elements whose centroid lies within cut-off +
+// element radius (centroid bins; bins without particles are skipped), each
+// element's particles in ascending original index.  The sums are compensated
+// (Neumaier), which makes them independent of the gather order up to the
+// last bit in all but contrived cases; ppiclF's own order over its bins and
+// ghost particles is not restated.
+__device__ __forceinline__ void neumaier(double &s, double &c, double x) {
+  double t = s + x;
+  if (fabs(s) >= fabs(x))
+    c = c + ((s - t) + x);
+  else
+    c = c + ((x - t) + s);
+  s = t;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
+  int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= P.nC) return;
+  int n = P.off[e + 1] - P.off[e];
+  if (n) atomicAdd(&P.binCnt[P.ecb[e]], n);
+}
+
+__global__ void __launch_bounds__(BLK) k_gauss(PiclDev P) {
+  int v = blockIdx.x * BLK + threadIdx.x;
+  if (v >= P.nV) return;
+  const size_t nV = P.nV;
+  double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
+  int blo[3], bhi[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    double top = (double)(P.cbN[l] - 1);
+    double a = (xv[l] - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xv[l] + P.gReach - P.cbLo[l]) / P.cbH[l];
+    blo[l] = (int)fmin(fmax(a, 0.0), top);
+    bhi[l] = (int)fmin(fmax(b, 0.0), top);
+  }
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, cmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int bz = blo[2]; bz <= bhi[2]; bz++)
+    for (int by = blo[1]; by <= bhi[1]; by++)
+      for (int bx = blo[0]; bx <= bhi[0]; bx++) {
+        int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
+        if (P.binCnt[bin] == 0) continue;
+        for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
+          const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)q);
+          double2 c01 = cp[0], c23 = cp[1];
+          double t0 = c01.x - xv[0], t1 = c01.y - xv[1], t2 = c23.x - xv[2];
+          if (t0 * t0 + t1 * t1 + t2 * t2 > P.gReach2) continue;
+          int e = (int)(unsigned)(__double_as_longlong(c23.y) & 0xffffffffll);
+          int p1 = P.off[e + 1];
+          for (int pos = P.off[e]; pos < p1; pos++) {
+            const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
+            double2 g0 = g[0], g1 = g[1];
+            double dx = xv[0] - g0.x, dy = xv[1] - g0.y, dz = xv[2] - g1.x;
+            double rdist2 = dx * dx + dy * dy;
+            rdist2 = rdist2 + dz * dz;
+            if (rdist2 > P.gD2chk) continue;
+            double rexp = exp(rdist2 * P.gRdum);
+            double2 g2 = g[2], g3 = g[3];
+            neumaier(acc[0], cmp[0], g1.y * rexp);
+            neumaier(acc[1], cmp[1], g2.x * rexp);
+            neumaier(acc[2], cmp[2], g2.y * rexp);
+            neumaier(acc[3], cmp[3], g3.x * rexp);
+            neumaier(acc[4], cmp[4], g3.y * rexp);
+          }
+        }
+      }
+#pragma unroll
+  for (int k = 0; k < 5; k++) P.vproj[k * nV + v] = 0.0 + (acc[k] + cmp[k]);
 }
 
 // Every particle to its new position in the other copy; this stage's ydotc
@@ -740,11 +999,11 @@ __global__ void __launch_bounds__(BLK) k_feedback(Dev d, PiclDev P, int feedback
   if (feedback) {
 #pragma unroll
     for (int k = 1; k < 5; k++) {
-      double jf = sum8(P.proj[k * (size_t)N + i]) * 0.125 * vol;
+      double jf = sum_nodes(P, k, i) * 0.125 * vol;
       d.rhs[k * N + i] = d.rhs[k * N + i] + jf;
     }
   }
-  d.phi[i] = phip_of(P.proj[i], vol);
+  d.phi[i] = phip_of(P, i, vol);
 }
 
 // PICL_TEMP_InitSolver.F90:789-835: piclVF from the first projection and
@@ -753,7 +1012,7 @@ __global__ void __launch_bounds__(BLK) k_picl_init_cells(Dev d, PiclDev P) {
   int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
-  double phi = phip_of(P.proj[i], d.vol[i]);
+  double phi = phip_of(P, i, d.vol[i]);
   d.phi[i] = phi;
   double vFrac = 1.0 - phi;
 #pragma unroll
@@ -821,7 +1080,7 @@ void picl_free(PiclState *s) {
   void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
-                  P.cbRec, P.dbg};
+                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -829,7 +1088,9 @@ void picl_free(PiclState *s) {
 int scan_blocks(int n) { return (n + SCAN_T * SCAN_V - 1) / (SCAN_T * SCAN_V); }
 
 // Counting sort of the located particles by element, then the deposit.
-void picl_sort_and_deposit(rflu_handle *h) {
+// posXo: the Gaussian deposit reads the start-of-stage positions saved by
+// k_particles (0 at set-up, where the particle rows hold them).
+void picl_sort_and_deposit(rflu_handle *h, int posXo) {
   PiclDev &P = h->picl->P;
   int n = P.nC + 1, nb = scan_blocks(n);
   {
@@ -844,7 +1105,13 @@ void picl_sort_and_deposit(rflu_handle *h) {
   }
   {
     ProfScope ps(h, "k_deposit");
-    hipLaunchKernelGGL(k_deposit, dim3((P.nC + DEP_T - 1) / DEP_T), dim3(DEP_T), 0, h->stream, P);
+    hipLaunchKernelGGL(k_deposit, dim3((P.nC + DEP_T - 1) / DEP_T), dim3(DEP_T), 0, h->stream, P, posXo);
+  }
+  if (P.gauss) {
+    ProfScope ps(h, "k_gauss");
+    hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nCB)), dim3(BLK), 0, h->stream, P.binCnt, P.nCB);
+    hipLaunchKernelGGL(k_gcount, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
+    hipLaunchKernelGGL(k_gauss, dim3(nblk(P.nV)), dim3(BLK), 0, h->stream, P);
   }
   ProfScope ps(h, "k_permute");
   hipLaunchKernelGGL(k_permute, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
@@ -863,15 +1130,21 @@ int picl_stage(rflu_handle *h, int istage) {
   PiclDev &P = s->P;
   {
     ProfScope ps(h, "k_picl_fields");
-    hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P);
+    hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P,
+                       s->phys.interpMode);
   }
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nC + 1)), dim3(BLK), 0, h->stream, P.cnt, P.nC + 1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lostCur, 1);
   {
     ProfScope ps(h, "k_particles");
-    hipLaunchKernelGGL(k_particles, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys, istage);
+    using KP = void (*)(Dev, PiclDev, PiclPhys, int);
+    static const KP kp[2][5] = {
+        {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
+        {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
+    hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
+                       P, s->phys, istage);
   }
-  picl_sort_and_deposit(h);
+  picl_sort_and_deposit(h, 1);
   ProfScope ps(h, "k_feedback");
   hipLaunchKernelGGL(k_feedback, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P, s->phys.feedbackFlag);
   return 0;
@@ -916,7 +1189,14 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (m.nvc != 8) return set_err(h, RFLU_GPU_EINVAL, "PICL needs an all-hex grid (PICL_TEMP_InitSolver.F90:562-597)");
   if (h->prm.spaceOrder != 2) return set_err(h, RFLU_GPU_EINVAL, "PICL needs spaceOrder 2 (its fields use gradients)");
   if (h->prm.patchOrder != 1) return set_err(h, RFLU_GPU_EINVAL, "PICL is built for patchOrder 1");
-  if (pp->qsFlag != 1) return set_err(h, RFLU_GPU_EINVAL, "only QS Parmar (qsFlag 1) is built");
+  if (h->prm.fluxPrecision != 0) return set_err(h, RFLU_GPU_EINVAL, "PICL runs in FP64 (fluxPrecision 0)");
+  if (pp->qsFlag < 1 || pp->qsFlag > 5)
+    return set_err(h, RFLU_GPU_EINVAL, "qsFlag must be 1 (Parmar), 2 (Osnes), 3 (modified Parmar), 4 (Gidaspow) or 5 "
+                                       "(Schiller-Naumann)");
+  if (pp->interpMode < 0 || pp->interpMode > 1) return set_err(h, RFLU_GPU_EINVAL, "interpMode must be 0 or 1");
+  if (pp->projMode < 0 || pp->projMode > 1) return set_err(h, RFLU_GPU_EINVAL, "projMode must be 0 or 1");
+  if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0))
+    return set_err(h, RFLU_GPU_EINVAL, "the Gaussian filter needs filterWidth > 0 and 0 < filterAlpha < 1");
   if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
   if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
   const int N = h->d.nC, nP = nPart;
@@ -924,8 +1204,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   PiclDev &P = s->P;
   P.nP = nP;
   P.nC = N;
-  s->phys = PiclPhys{pp->rmuFlag, pp->amFlag, pp->pgFlag, pp->feedbackFlag, pp->rmuRef, pp->tref, pp->suth,
-                     pp->rcpPart};
+  s->phys = PiclPhys{pp->rmuFlag, pp->amFlag, pp->pgFlag, pp->feedbackFlag, pp->qsFlag, pp->interpMode,
+                     pp->rmuRef, pp->tref, pp->suth, pp->rcpPart};
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
   std::vector<double> X(24 * (size_t)N), bbox(8 * (size_t)N, 0.0), cen(4 * (size_t)N, 0.0), rexp(N);
@@ -1038,6 +1318,56 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     memcpy(&cbRec[4 * q + 3], &key, sizeof key);
   }
   rc |= dalloc(h, &P.cbRec, cbRec.size(), cbRec.data());
+  if (pp->projMode == 1) {
+    // ppiclf_solve_InitGaussianFilter (:1520-1576) and ProjectParticleGrid's
+    // constants (:3103-3109); same operations as the oracle
+    const double pi = 4.0 * atan(1.0);
+    double rsig = pp->filterWidth / (2.0 * sqrt(2.0 * log(2.0)));
+    double d2chk = rsig * sqrt(-2.0 * log(pp->filterAlpha));
+    double s2 = sqrt(2.0 * pi);
+    double multfci = 1.0 / ((s2 * s2) * (rsig * rsig));
+    P.gMult = pow(multfci, 1.5);
+    P.gRdum = 1.0 / (-2.0 * (rsig * rsig));
+    P.gD2chk = d2chk * d2chk;
+    // reach of an element's particles: cut-off + the largest centroid-to-node
+    // distance (a located point is a convex combination of the nodes)
+    double hmax = 0.0;
+    for (int i = 0; i < N; i++)
+      for (int n = 0; n < 8; n++) {
+        double r2 = 0.0;
+        for (int l = 0; l < 3; l++) {
+          double t = X[24 * (size_t)i + 3 * n + l] - cen[4 * (size_t)i + l];
+          r2 += t * t;
+        }
+        hmax = std::max(hmax, std::sqrt(r2));
+      }
+    P.gReach = (d2chk + hmax) * (1.0 + 1.0e-6);
+    P.gReach2 = P.gReach * P.gReach;
+    P.gauss = 1;
+    P.nV = (int)(h->xyz.size() / 3);
+    P.nCB = P.cbN[0] * P.cbN[1] * P.cbN[2];
+    std::vector<double> vX(3 * (size_t)P.nV);
+    for (int v = 0; v < P.nV; v++)
+      for (int l = 0; l < 3; l++) vX[l * (size_t)P.nV + v] = h->xyz[3 * (size_t)v + l];
+    std::vector<int> ev(8 * (size_t)N), ecb(N);
+    for (int i = 0; i < N; i++) {
+      for (int n = 0; n < 8; n++) ev[n * (size_t)N + i] = h->c2v[8 * (size_t)refId[i] + NODE2V[n]];
+      int b[3];
+      for (int l = 0; l < 3; l++) {
+        double t = (cen[4 * (size_t)i + l] - P.cbLo[l]) / P.cbH[l];
+        t = std::min(std::max(t, 0.0), (double)(P.cbN[l] - 1));
+        b[l] = (int)t;
+      }
+      ecb[i] = b[0] + P.cbN[0] * (b[1] + P.cbN[1] * b[2]);
+    }
+    rc |= dalloc(h, &P.vX, vX.size(), vX.data());
+    rc |= dalloc(h, &P.ev, ev.size(), ev.data());
+    rc |= dalloc(h, &P.ecb, ecb.size(), ecb.data());
+    rc |= dalloc(h, &P.binCnt, (size_t)P.nCB);
+    rc |= dalloc(h, &P.gq, 8 * (size_t)nP);
+    rc |= dalloc(h, &P.xo, 3 * (size_t)nP);
+    rc |= dalloc(h, &P.vproj, 5 * (size_t)P.nV);
+  }
   rc |= dalloc(h, &h->d.phi, (size_t)N);
   rc |= dalloc(h, &h->d.rhog, (size_t)N);
   rc |= dalloc(h, &h->d.gRaw, 4 * (size_t)N);
@@ -1056,7 +1386,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   // ppiclf_comm_InitOverlapMesh -> InitSolve: locate and project (ydotc = 0)
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(N + 1)), dim3(BLK), 0, h->stream, P.cnt, N + 1);
   hipLaunchKernelGGL(k_locate_init, dim3(nblk(nP)), dim3(BLK), 0, h->stream, P);
-  picl_sort_and_deposit(h);
+  picl_sort_and_deposit(h, 0);
   hipLaunchKernelGGL(k_picl_init_cells, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P);
   hipLaunchKernelGGL(k_records, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
   HIPCHK(h, hipGetLastError());
@@ -1126,13 +1456,41 @@ int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *pr
       for (int k = 0; k < 3; k++) gradRhog[3 * i + k] = g4[4 * i + k];
   }
   if (proFld) {
-    std::vector<double> pf(5 * N);
-    if (int e = from_device_cells(h, pf.data(), P.proj, 5, false)) return e;
-    for (size_t i = 0; i < N; i++)
-      for (int k = 0; k < 5; k++) proFld[5 * i + k] = pf[k * N + i];
+    if (P.gauss) {  // the value at node (1,1,1)
+      std::vector<double> nodes(40 * N);
+      if (int e = picl_gpu_get_nodes(h, nodes.data())) return e;
+      for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 5; k++) proFld[5 * i + k] = nodes[40 * i + 8 * k];
+    } else {
+      std::vector<double> pf(5 * N);
+      if (int e = from_device_cells(h, pf.data(), P.proj, 5, false)) return e;
+      for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 5; k++) proFld[5 * i + k] = pf[k * N + i];
+    }
   }
   if (fields)
     if (int e = from_device_cells(h, fields, P.fld, PICL_NFLD, true)) return e;
+  return 0;
+}
+
+int picl_gpu_get_nodes(rflu_handle *h, double *proNode) {
+  if (!h || !h->picl || !proNode) return RFLU_GPU_EINVAL;
+  PiclDev &P = h->picl->P;
+  size_t N = P.nC;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (P.gauss) {
+    std::vector<double> vp(5 * (size_t)P.nV);
+    HIPCHK(h, hipMemcpy(vp.data(), P.vproj, vp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t o = 0; o < N; o++)
+      for (int k = 0; k < 5; k++)
+        for (int n = 0; n < 8; n++) proNode[40 * o + 8 * k + n] = vp[k * (size_t)P.nV + h->c2v[8 * o + NODE2V[n]]];
+  } else {
+    std::vector<double> pf(5 * N);
+    if (int e = from_device_cells(h, pf.data(), P.proj, 5, false)) return e;
+    for (size_t o = 0; o < N; o++)
+      for (int k = 0; k < 5; k++)
+        for (int n = 0; n < 8; n++) proNode[40 * o + 8 * k + n] = pf[k * N + o];
+  }
   return 0;
 }
 

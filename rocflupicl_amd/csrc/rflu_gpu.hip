@@ -67,6 +67,8 @@ struct Dev {
   // 15 marking face neighbours
   int *tOff = nullptr, *tMem = nullptr;
   unsigned short *sSlot = nullptr;
+  // FP32 evaluation mode: limited gradients as 16 floats per cell (15 + pad)
+  float *gradF = nullptr;
 };
 
 // scal[] slots
@@ -691,6 +693,104 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
   }
 }
 
+// FP32 evaluation of the same gradient (fluxPrecision 1): the tile's records
+// are staged in LDS as FP32 (state, and position relative to the tile's first
+// cell, formed in FP64), so a member costs 32 bytes of LDS; LSQ weights, sums
+// and the limiter run in FP32, GU members per iteration.
+__global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int guard) {
+  extern __shared__ float4 sRecF[];  // [nU][2]: (rho, u, v, w), (p, x - x0, y - y0, z - z0)
+  int tile = xcd_block();
+  int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
+  const double *r0 = d.rec + 8 * (size_t)d.tMem[u0];
+  double x0 = r0[5], y0 = r0[6], z0 = r0[7];
+  for (int u = threadIdx.x; u < nU; u += BLK) {
+    double r[8];
+    load_rec(d.rec, d.tMem[u0 + u], r);
+    sRecF[2 * u] = make_float4((float)r[0], (float)r[1], (float)r[2], (float)r[3]);
+    sRecF[2 * u + 1] = make_float4((float)r[4], (float)(r[5] - x0), (float)(r[6] - y0), (float)(r[7] - z0));
+  }
+  __syncthreads();
+  int i = tile * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  float4 ia = sRecF[2 * threadIdx.x], ib = sRecF[2 * threadIdx.x + 1];
+  float ri[8] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+  float r11 = (float)d.rm[i], r12 = (float)d.rm[N + i], r22 = (float)d.rm[2 * N + i];
+  float r13 = (float)d.rm[3 * N + i], r23 = (float)d.rm[4 * N + i], r33 = (float)d.rm[5 * N + i];
+  float c11 = 1.0f / r11, c22 = 1.0f / r22, c33 = 1.0f / r33;
+  float c12 = -(c11 * r12);
+  float c13 = -(c11 * r13 + c12 * c22 * r23);
+  float c23 = -(c22 * r23);
+  float c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
+  float g[15], dmin[5], dmax[5];
+#pragma unroll
+  for (int q = 0; q < 15; q++) g[q] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; k++) { dmin[k] = 0.0f; dmax[k] = 0.0f; }
+  int s = d.csOff[i], se = d.csOff[i + 1];
+  auto member = [&](unsigned j) {
+    float4 a = sRecF[2 * (j & 0x7fffu)], b = sRecF[2 * (j & 0x7fffu) + 1];
+    float dx = b.y - ri[5], dy = b.z - ri[6], dz = b.w - ri[7];
+    float term = rsqrtf(dx * dx + dy * dy + dz * dz);
+    dx = term * dx; dy = term * dy; dz = term * dz;
+    float term1 = c1111 * dx;
+    float term2 = c2222 * (dy + c12 * dx);
+    float term3 = c3333 * (dz + c23 * dy + c13 * dx);
+    float wtx = term * (term1 + c12 * term2 + c13 * term3);
+    float wty = term * (term2 + c23 * term3);
+    float wtz = term * term3;
+    float dv[5] = {a.x - ri[0], a.y - ri[1], a.z - ri[2], a.w - ri[3], b.x - ri[4]};
+    bool face = (j & 0x8000u) != 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      g[3 * k] += wtx * dv[k];
+      g[3 * k + 1] += wty * dv[k];
+      g[3 * k + 2] += wtz * dv[k];
+      if (face) { dmin[k] = fminf(dmin[k], dv[k]); dmax[k] = fmaxf(dmax[k], dv[k]); }
+    }
+  };
+  for (; s + GU - 1 < se; s += GU) {
+    unsigned jj[GU];
+#pragma unroll
+    for (int u = 0; u < GU; u++) jj[u] = d.sSlot[s + u];
+#pragma unroll
+    for (int u = 0; u < GU; u++) member(jj[u]);
+  }
+  for (; s < se; s++) member(d.sSlot[s]);
+  if (venkat) {
+    const float TINYP = 1.0e-12f, TINYN = guard ? -1.0e-12f : 1.0e-12f;
+    float lim[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) lim[k] = 1.0f;
+    float epsq = (float)d.epsq[i];
+    int eb = d.cfOff[i], ee = eb + d.cfNInt[i];
+    for (int e = eb; e < ee; e++) {
+      float dx1 = (float)d.cfdx[e], dy1 = (float)d.cfdy[e], dz1 = (float)d.cfdz[e];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        float d21 = g[3 * k] * dx1 + g[3 * k + 1] * dy1 + g[3 * k + 2] * dz1;
+        if (d21 != 0.0f) {
+          float d1 = d21 > 0.0f ? dmax[k] : dmin[k];
+          float numer = (d1 * d1 + epsq) * d21 + 2.0f * d21 * d21 * d1;
+          float denom = d21 * (d1 * d1 + 2.0f * d21 * d21 + d1 * d21 + epsq);
+          lim[k] = fminf(__fdividef(numer, denom + (denom >= 0.0f ? TINYP : TINYN)), lim[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      g[3 * k] *= lim[k];
+      g[3 * k + 1] *= lim[k];
+      g[3 * k + 2] *= lim[k];
+    }
+  }
+  float4 *o = reinterpret_cast<float4 *>(d.gradF + 16 * (size_t)i);
+  o[0] = make_float4(g[0], g[1], g[2], g[3]);
+  o[1] = make_float4(g[4], g[5], g[6], g[7]);
+  o[2] = make_float4(g[8], g[9], g[10], g[11]);
+  o[3] = make_float4(g[12], g[13], g[14], 0.0f);
+}
+
 // Cooperative variant 3: a workgroup of 256 lanes takes 16 cells.
 //  A: 16 lanes per cell fetch that cell's stencil members in parallel and
 //     stage each member's weights and variable differences in LDS (SoA, so
@@ -1054,6 +1154,148 @@ __device__ __forceinline__ void hllc_flux(const Gas &G, double nx, double ny, do
   }
 }
 
+// FP32 forms of the three flux functions (fluxPrecision 1), same formulas.
+__device__ __forceinline__ void ausm_flux_f(float nx, float ny, float nz, float nm, float rl, float ul, float vl,
+                                            float wl, float pl, float Hl, float al, float rr, float ur, float vr,
+                                            float wr, float pr, float Hr, float ar, float f[5]) {
+  float ql = ul * nx + vl * ny + wl * nz;
+  float qr = ur * nx + vr * ny + wr * nz;
+  float af = 0.5f * (al + ar);
+  float ml = ql / af, mla = fabsf(ml);
+  float mr = qr / af, mra = fabsf(mr);
+  float mlp, wtl, mrm, wtr;
+  if (mla <= 1.0f) {
+    mlp = 0.25f * (ml + 1.0f) * (ml + 1.0f) + 0.125f * (ml * ml - 1.0f) * (ml * ml - 1.0f);
+    wtl = 0.25f * (ml + 1.0f) * (ml + 1.0f) * (2.0f - ml) + 0.1875f * ml * (ml * ml - 1.0f) * (ml * ml - 1.0f);
+  } else {
+    mlp = 0.5f * (ml + mla);
+    wtl = 0.5f * (1.0f + ml / mla);
+  }
+  if (mra <= 1.0f) {
+    mrm = -0.25f * (mr - 1.0f) * (mr - 1.0f) - 0.125f * (mr * mr - 1.0f) * (mr * mr - 1.0f);
+    wtr = 0.25f * (mr - 1.0f) * (mr - 1.0f) * (2.0f + mr) - 0.1875f * mr * (mr * mr - 1.0f) * (mr * mr - 1.0f);
+  } else {
+    mrm = 0.5f * (mr - mra);
+    wtr = 0.5f * (1.0f - mr / mra);
+  }
+  float mf = mlp + mrm, mfa = fabsf(mf);
+  float mfp = 0.5f * (mf + mfa), mfm = 0.5f * (mf - mfa);
+  float pf = wtl * pl + wtr * pr;
+  f[0] = (af * (mfp * rl + mfm * rr)) * nm;
+  f[1] = (af * (mfp * rl * ul + mfm * rr * ur) + pf * nx) * nm;
+  f[2] = (af * (mfp * rl * vl + mfm * rr * vr) + pf * ny) * nm;
+  f[3] = (af * (mfp * rl * wl + mfm * rr * wr) + pf * nz) * nm;
+  f[4] = (af * (mfp * rl * Hl + mfm * rr * Hr)) * nm;
+}
+
+__device__ __forceinline__ float hh_fix_f(float l, float dd) { return (l > dd) ? l : (l * l + dd * dd) / (2.0f * dd); }
+__device__ __forceinline__ void roe_flux_f(float g1, float epsentr, float dissFact, float nx, float ny, float nz,
+                                           float nm, float rl, float ul, float vl, float wl, float pl, float Hl,
+                                           float ql, float rr, float ur, float vr, float wr, float pr, float Hr,
+                                           float qr, float f[5]) {
+  float rh = sqrtf(rl * rr);
+  float wt = rl / (rl + rh);
+  float uh = wt * ul + (1.0f - wt) * ur;
+  float vh = wt * vl + (1.0f - wt) * vr;
+  float wh = wt * wl + (1.0f - wt) * wr;
+  float Hh = wt * Hl + (1.0f - wt) * Hr;
+  float qh = uh * nx + vh * ny + wh * nz;
+  float sh = 0.5f * (uh * uh + vh * vh + wh * wh);
+  float ah = sqrtf(g1 * (Hh - sh));
+  float l1 = fabsf(qh - ah), l2 = fabsf(qh), l5 = fabsf(qh + ah);
+  float efc = epsentr * l5;
+  l1 = hh_fix_f(l1, efc);
+  l2 = hh_fix_f(l2, efc);
+  l5 = hh_fix_f(l5, efc);
+  float dr = rr - rl, du = ur - ul, de = vr - vl, dw = wr - wl, dp = pr - pl;
+  float dq = du * nx + de * ny + dw * nz;
+  float dv1 = (dp - rh * ah * dq) / (2.0f * ah * ah);
+  float dv5 = (dp + rh * ah * dq) / (2.0f * ah * ah);
+  float dv2 = dr - dp / (ah * ah);
+  float t1 = l1 * dv1, t2 = l2 * dv2, t3 = l2 * rh, t5 = l5 * dv5;
+  float hd = 0.5f * dissFact;
+  f[0] = 0.5f * (ql * rl + qr * rr) * nm - hd * (t1 + t2 + t5) * nm;
+  f[1] = 0.5f * (ql * rl * ul + pl * nx + qr * rr * ur + pr * nx) * nm -
+         hd * (t1 * (uh - nx * ah) + t2 * uh + t3 * (du - nx * dq) + t5 * (uh + nx * ah)) * nm;
+  f[2] = 0.5f * (ql * rl * vl + pl * ny + qr * rr * vr + pr * ny) * nm -
+         hd * (t1 * (vh - ny * ah) + t2 * vh + t3 * (de - ny * dq) + t5 * (vh + ny * ah)) * nm;
+  f[3] = 0.5f * (ql * rl * wl + pl * nz + qr * rr * wr + pr * nz) * nm -
+         hd * (t1 * (wh - nz * ah) + t2 * wh + t3 * (dw - nz * dq) + t5 * (wh + nz * ah)) * nm;
+  f[4] = 0.5f * (ql * rl * Hl + qr * rr * Hr) * nm -
+         hd * (t1 * (Hh - qh * ah) + t2 * sh + t3 * (uh * du + vh * de + wh * dw - qh * dq) + t5 * (Hh + qh * ah)) * nm;
+}
+
+__device__ __forceinline__ void hllc_flux_f(float g1, float nx, float ny, float nz, float nm, float rl, float ul,
+                                            float vl, float wl, float pl, float el, float al, float ql, float Hl,
+                                            float rr, float ur, float vr, float wr, float pr, float er, float ar,
+                                            float qr, float Hr, float f[5]) {
+  float wt1 = sqrtf(rr / rl);
+  float wt2 = 1.0f / (1.0f + wt1);
+  float us = wt2 * (ul + wt1 * ur);
+  float vs = wt2 * (vl + wt1 * vr);
+  float ws = wt2 * (wl + wt1 * wr);
+  float qs = wt2 * (ql + wt1 * qr);
+  float Hs = wt2 * (Hl + wt1 * Hr);
+  float Vms = us * us + vs * vs + ws * ws;
+  float as = sqrtf(g1 * (Hs - 0.5f * Vms));
+  float sl = fminf(ql - al, qs - as);
+  float sr = fmaxf(qr + ar, qs + as);
+  if (sl > 0.0f) {
+    f[0] = ql * rl * nm;
+    f[1] = (ql * rl * ul + pl * nx) * nm;
+    f[2] = (ql * rl * vl + pl * ny) * nm;
+    f[3] = (ql * rl * wl + pl * nz) * nm;
+    f[4] = (ql * (rl * el + pl)) * nm;
+  } else if (sr < 0.0f) {
+    f[0] = qr * rr * nm;
+    f[1] = (qr * rr * ur + pr * nx) * nm;
+    f[2] = (qr * rr * vr + pr * ny) * nm;
+    f[3] = (qr * rr * wr + pr * nz) * nm;
+    f[4] = (qr * (rr * er + pr)) * nm;
+  } else {
+    float sm = (rr * qr * (sr - qr) - rl * ql * (sl - ql) - pr + pl) / (rr * (sr - qr) - rl * (sl - ql));
+    float ps = pl + rl * (ql - sl) * (ql - sm);
+    float rs, rus, rvs, rws, res, term;
+    if (sm > 0.0f) {
+      term = 1.0f / (sl - sm);
+      rs = term * (sl - ql) * rl;
+      rus = term * ((sl - ql) * rl * ul + (ps - pl) * nx);
+      rvs = term * ((sl - ql) * rl * vl + (ps - pl) * ny);
+      rws = term * ((sl - ql) * rl * wl + (ps - pl) * nz);
+      res = term * ((sl - ql) * rl * el + (ps * sm - pl * ql));
+    } else {
+      term = 1.0f / (sr - sm);
+      rs = term * (sr - qr) * rr;
+      rus = term * ((sr - qr) * rr * ur + (ps - pr) * nx);
+      rvs = term * ((sr - qr) * rr * vr + (ps - pr) * ny);
+      rws = term * ((sr - qr) * rr * wr + (ps - pr) * nz);
+      res = term * ((sr - qr) * rr * er + (ps * sm - pr * qr));
+    }
+    f[0] = sm * rs * nm;
+    f[1] = (sm * rus + ps * nx) * nm;
+    f[2] = (sm * rvs + ps * ny) * nm;
+    f[3] = (sm * rws + ps * nz) * nm;
+    f[4] = (sm * (res + ps)) * nm;
+  }
+}
+
+// FP32 face state: the cell value rounded once plus the FP32 gradient
+// correction; the face-minus-centroid vector is formed in FP64.
+struct FaceStateF { float r, u, v, w, p; };
+__device__ __forceinline__ FaceStateF reconstruct_f(const double rc[8], const float *__restrict__ gp, double xc,
+                                                    double yc, double zc) {
+  const float4 *g4 = reinterpret_cast<const float4 *>(gp);
+  float4 a = g4[0], b = g4[1], c = g4[2], e = g4[3];
+  float dx = (float)(xc - rc[5]), dy = (float)(yc - rc[6]), dz = (float)(zc - rc[7]);
+  FaceStateF s;
+  s.r = (float)rc[0] + (a.x * dx + a.y * dy + a.z * dz);
+  s.u = (float)rc[1] + (a.w * dx + b.x * dy + b.y * dz);
+  s.v = (float)rc[2] + (b.z * dx + b.w * dy + c.x * dz);
+  s.w = (float)rc[3] + (c.y * dx + c.z * dy + c.w * dz);
+  s.p = (float)rc[4] + (e.x * dx + e.y * dy + e.z * dz);
+  return s;
+}
+
 // Face state reconstructed at (xc,yc,zc) from an order-2 record and gradient;
 // u is taken through the reference's prim->cons->prim round trip
 // (RFLU_ModConvertCv.F90:554-592 then RFLU_ModAUSMFlux.F90:2735-2783).
@@ -1110,6 +1352,9 @@ __device__ __forceinline__ void bcond_outflow(const Gas &G, int opt, double pout
 
 struct FluxPar { int order, discr, patchOrder, doSrad; };
 
+// F32 = 1: interior second-order faces in FP32 from the FP32 gradients;
+// boundary faces stay FP64 (their gradients widened from FP32).
+template <int F32>
 __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
   int s = xcd_block() * BLK + threadIdx.x;
   if (s >= d.S) return;
@@ -1149,6 +1394,36 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
         double Hl = el + irl * pl, Hr = er + irr * pr;
         hllc_flux(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, el, al, ql, Hl, rr, ur, vr, wr, pr, er, ar, qr, Hr, f);
       }
+    } else if (F32) {
+      double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
+      FaceStateF A = reconstruct_f(L, d.gradF + 16 * (size_t)c1, xc, yc, zc);
+      FaceStateF B = reconstruct_f(R, d.gradF + 16 * (size_t)c2, xc, yc, zc);
+      float fx = (float)nx, fy = (float)ny, fz = (float)nz, fm = (float)nm;
+      float g = (float)G.g, gc = (float)G.gc, cp = (float)G.cp, ff[5];
+      if (P.discr == RFLU_DISCR_AUSM) {
+        float tl = A.p / (A.r * gc), al = sqrtf(g * gc * tl);
+        float Hl = cp * tl + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w);
+        float tr = B.p / (B.r * gc), ar = sqrtf(g * gc * tr);
+        float Hr = cp * tr + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w);
+        ausm_flux_f(fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, Hl, al, B.r, B.u, B.v, B.w, B.p, Hr, ar, ff);
+      } else if (P.discr == RFLU_DISCR_ROE) {
+        float Hl = cp * (A.p / (A.r * gc)) + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w);
+        float ql = A.u * fx + A.v * fy + A.w * fz;
+        float Hr = cp * (B.p / (B.r * gc)) + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w);
+        float qr = B.u * fx + B.v * fy + B.w * fz;
+        roe_flux_f(g - 1.0f, (float)G.epsentr, (float)G.dissFact, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, Hl, ql,
+                   B.r, B.u, B.v, B.w, B.p, Hr, qr, ff);
+      } else {
+        // H = e + p / rho with the cell's rho, as the FP64 path (FaceState::ir)
+        float el = A.p / ((g - 1.0f) * A.r) + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w), al = sqrtf(g * A.p / A.r);
+        float ql = A.u * fx + A.v * fy + A.w * fz, Hl = el + A.p * (float)(1.0 / L[0]);
+        float er = B.p / ((g - 1.0f) * B.r) + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w), ar = sqrtf(g * B.p / B.r);
+        float qr = B.u * fx + B.v * fy + B.w * fz, Hr = er + B.p * (float)(1.0 / R[0]);
+        hllc_flux_f(g - 1.0f, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, el, al, ql, Hl, B.r, B.u, B.v, B.w, B.p, er,
+                    ar, qr, Hr, ff);
+      }
+#pragma unroll
+      for (int k = 0; k < 5; k++) f[k] = (double)ff[k];
     } else {
       double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
       FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, xc, yc, zc, rpL);
@@ -1198,7 +1473,14 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
         rel = L[0] * eo_dgpuvw(rE, G.gD, L[4], L[1], L[2], L[3]);
       }
     } else {
-      FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, d.fcx[s], d.fcy[s], d.fcz[s], L[0]);
+      double gw[15];
+      const double *gp = d.grad + 15 * (size_t)c1;
+      if (F32) {
+#pragma unroll
+        for (int q = 0; q < 15; q++) gw[q] = (double)d.gradF[16 * (size_t)c1 + q];
+        gp = gw;
+      }
+      FaceState A = reconstruct(L, gp, d.fcx[s], d.fcy[s], d.fcz[s], L[0]);
       rl = A.r; pl = A.p;
       rul = rl * A.u; rvl = rl * A.v; rwl = rl * A.w;
       rel = (kind != 0) ? rl * eo_dgpuvw(rl, G.gD, pl, A.u, A.v, A.w) : 0.0;
@@ -1487,6 +1769,10 @@ int launch_grad(rflu_handle *h) {
   ProfScope ps(h, "k_grad");
   int venkat = h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, guard = h->prm.limiterGuard ? 1 : 0;
   int nUmax = (int)(h->tileLds / 64);
+  if (h->prm.fluxPrecision == 1) {
+    hipLaunchKernelGGL(k_grad_tile_f32, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 2, h->stream, h->d, venkat, guard);
+    return 0;
+  }
   if (h->picl && h->gradVariant == 4 && h->tileLds && h->tileLds + 8 * (size_t)nUmax <= h->ldsMax)
     hipLaunchKernelGGL((k_grad_tile<1>), dim3(h->d.nBlk), dim3(BLK), h->tileLds + 8 * (size_t)nUmax, h->stream,
                        h->d, venkat, guard, nUmax);
@@ -1514,7 +1800,10 @@ int launch_grad(rflu_handle *h) {
 int launch_flux(rflu_handle *h, int doSrad) {
   ProfScope ps(h, "k_flux");
   FluxPar P{h->prm.spaceOrder, h->prm.spaceDiscr, h->prm.patchOrder, doSrad};
-  hipLaunchKernelGGL(k_flux, dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, P);
+  if (h->prm.fluxPrecision == 1)
+    hipLaunchKernelGGL((k_flux<1>), dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, P);
+  else
+    hipLaunchKernelGGL((k_flux<0>), dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, P);
   return 0;
 }
 
@@ -1597,6 +1886,10 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
     return set_err(nullptr, RFLU_GPU_EINVAL, "second-order patches need spaceOrder 2");
   if (prm->reconst != RFLU_RECONST_NONE && prm->reconst != RFLU_RECONST_VENKAT)
     return set_err(nullptr, RFLU_GPU_EINVAL, "reconst must be 0 or 11 (Venkatakrishnan)");
+  if (prm->fluxPrecision < 0 || prm->fluxPrecision > 1)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "fluxPrecision must be 0 (FP64) or 1 (FP32 gradients and fluxes)");
+  if (prm->fluxPrecision == 1 && prm->spaceOrder != 2)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "fluxPrecision 1 needs spaceOrder 2");
   for (int p = 0; p < md->nPatches; p++) {
     if (md->bcType[p] != RFLU_BC_SLIPWALL && md->bcType[p] != RFLU_BC_OUTFLOW)
       return set_err(nullptr, RFLU_GPU_EINVAL, "only slip-wall and outflow patches are supported");
@@ -1785,6 +2078,10 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
     rc |= dalloc(h, &d.csMem, csMemD.size(), csMemD.data());
     rc |= dalloc(h, &d.rm, 6 * (size_t)N, rmSoa.data());
     rc |= dalloc(h, &d.grad, 15 * (size_t)N);
+    if (prm->fluxPrecision == 1) {
+      if (!h->tileLds) rc |= set_err(h, RFLU_GPU_EINVAL, "fluxPrecision 1 needs gradient tiles that fit in LDS");
+      else rc |= dalloc(h, &d.gradF, 16 * (size_t)N);
+    }
     if (h->tileLds) {
       rc |= dalloc(h, &d.tOff, tOff.size(), tOff.data());
       rc |= dalloc(h, &d.tMem, tMem.size(), tMem.data());
@@ -1821,7 +2118,8 @@ int rflu_gpu_destroy(rflu_handle *h) {
                   d.csMem, d.rm, d.grad, d.cv, d.rhsSum, d.rhs, d.rec, d.flux, d.srad, d.dtCell, d.dvOut, d.partial, d.scal,
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);
-  for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot})
+  for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot,
+                  (void *)d.gradF})
     if (p) hipFree(p);
   picl_free(h->picl);
   if (h->graph) hipGraphExecDestroy(h->graph);
@@ -1915,6 +2213,15 @@ int rflu_gpu_get_rhs(rflu_handle *h, double *rhs) {
 int rflu_gpu_get_grad(rflu_handle *h, double *grad) {
   if (!h || !grad) return RFLU_GPU_EINVAL;
   if (h->prm.spaceOrder < 2) return set_err(h, RFLU_GPU_EINVAL, "no gradients at spaceOrder 1");
+  if (h->prm.fluxPrecision == 1) {  // FP32 gradients, widened
+    size_t N = h->d.nC;
+    std::vector<float> gf(16 * N);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(gf.data(), h->d.gradF, gf.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; i++)
+      for (int k = 0; k < 15; k++) grad[15 * (size_t)h->perm[i] + k] = (double)gf[16 * i + k];
+    return 0;
+  }
   return from_device_cells(h, grad, h->d.grad, 15, true);
 }
 

@@ -19,11 +19,12 @@ RECONST_NONE, RECONST_VENKAT = 0, 11
 DEFAULTS = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1,
                 cfl=0.5, dtImposed=1.0e-5, dtMinLimit=1.0e-10, dTimeSystem=0.0,
                 refCp=1004.5, refGamma=1.4, epsentr=0.05, dissFact=1.0, accelOn=0,
-                accel=(0.0, 0.0, 0.0), limiterGuard=0)
+                accel=(0.0, 0.0, 0.0), limiterGuard=0, fluxPrecision=0)
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
-                     amFlag=1, pgFlag=1, feedbackFlag=1)
+                     amFlag=1, pgFlag=1, feedbackFlag=1, interpMode=0, projMode=0, filterWidth=0.0,
+                     filterAlpha=1e-2)
 # rprop index (1-based, PPICLF_USER.h) of each interpolated field slot
 PICL_FIELD_RPROP = [2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39]
 
@@ -209,6 +210,12 @@ class Region:
         phi, g, pf, fld = np.zeros(N), np.zeros((N, 3)), np.zeros((N, 5)), np.zeros((N, 20))
         self._chk(self._L.picl_gpu_get_cells(self.h, _p(phi), _p(g), _p(pf), _p(fld)))
         return dict(phi=phi, gradRhog=g, proFld=pf, fields=fld)
+
+    def picl_nodes(self):
+        """Projected values at the 8 nodes of every element, (nCells, 5, 8)."""
+        a = np.zeros((self.nCells, 5, 8))
+        self._chk(self._L.picl_gpu_get_nodes(self.h, _p(a)))
+        return a
 
     def picl_set_debug(self, on):
         self._chk(self._L.picl_gpu_set_debug(self.h, 1 if on else 0))

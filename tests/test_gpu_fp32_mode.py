@@ -1,0 +1,66 @@
+"""FP32 evaluation mode (fluxPrecision 1): gradients and interior face fluxes
+in FP32, residual accumulation, RK update, p/T recovery, patches and dt in
+FP64.  Its result is held to the FP64 path (which equals the oracle bit for
+bit, tests/test_gpu_rflu_parity.py) by the relative L2 error of the density,
+the momentum vector and the total energy fields, ||q32 - q64|| / ||q64||,
+after a number of steps.  Momentum is measured as a vector: its transverse
+components are zero but for the mesh jitter, and a component-wise ratio
+would divide by that noise (in FP64, perturbing the initial state by 1e-7
+already moves them by ~1e-5 of their own norm).  The tolerance, 1e-6, is the
+accuracy this mode is specified to keep.
+"""
+import numpy as np
+import pytest
+
+from rocflupicl_amd import configs, rflu
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def rel_l2(a, b):
+    """max over rho, the momentum vector and rho*E of ||a - b|| / ||b||"""
+    groups = ([0], [1, 2, 3], [4])
+    return max(np.linalg.norm(a[:, g] - b[:, g]) / np.linalg.norm(b[:, g]) for g in groups)
+
+
+def run(mesh, params, ic, steps, prec, graph=True):
+    r = rflu.Region(mesh, fluxPrecision=prec, **params)
+    r.set_state(ic(r.mesh_arrays()["cofg"]))
+    r.RFLU_TimeStepping(steps, use_graph=graph)
+    return r
+
+
+@pytest.mark.parametrize("discr", [1, 3, 4])
+def test_fp32_mode_tracks_fp64_on_tets(discr):
+    mesh, params, ic = configs.shock_tets(n=12, discr=discr, limiterGuard=1)
+    a = run(mesh, params, ic, 40, 0)
+    b = run(mesh, params, ic, 40, 1)
+    ta, tb = a.time()[0], b.time()[0]
+    err = rel_l2(b.get_state()[0], a.get_state()[0])
+    print(f"discr {discr}: rel L2 {err:.3e}, time {ta:.6e} vs {tb:.6e}")
+    assert abs(ta - tb) <= 1e-6 * ta
+    assert err <= TOL, err
+
+
+def test_fp32_mode_on_hexes():
+    mesh, params, ic = configs.shock_hex(nx=40, ny=8, nz=8, limiterGuard=1)
+    a = run(mesh, params, ic, 40, 0)
+    b = run(mesh, params, ic, 40, 1)
+    err = rel_l2(b.get_state()[0], a.get_state()[0])
+    print(f"hex AUSM+: rel L2 {err:.3e}")
+    assert err <= TOL, err
+
+
+def test_fp32_mode_graph_equals_eager():
+    mesh, params, ic = configs.shock_tets(n=8, discr=1, limiterGuard=1)
+    a = run(mesh, params, ic, 5, 1, graph=True)
+    b = run(mesh, params, ic, 5, 1, graph=False)
+    assert np.array_equal(a.get_state()[0], b.get_state()[0])
+
+
+def test_fp32_mode_rejects_first_order_and_particles():
+    mesh, params, ic = configs.sod()
+    with pytest.raises(Exception):
+        rflu.Region(mesh, fluxPrecision=1, **params)
