@@ -144,6 +144,34 @@ int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph);
 int rflu_gpu_get_time(rflu_handle *h, double *currentTime, double *dtMin);
 int rflu_gpu_sync(rflu_handle *h);
 
+/* Asynchronous stepping for hosts that drive several regions at once (no
+ * reference counterpart; the serving loop uses them).  rflu_gpu_step_async
+ * enqueues nSteps graph replays on the handle's stream and returns at once;
+ * rflu_gpu_query returns 1 when the stream is idle and 0 while work is
+ * pending; rflu_gpu_wait blocks, then reports AfterUpdateMP's checks like
+ * rflu_gpu_step.  rflu_gpu_device_bytes: device memory the handle holds;
+ * rflu_gpu_mem_info: free and total device memory. */
+int rflu_gpu_step_async(rflu_handle *h, int nSteps);
+int rflu_gpu_query(rflu_handle *h);
+int rflu_gpu_wait(rflu_handle *h);
+int rflu_gpu_device_bytes(const rflu_handle *h, size_t *bytes);
+int rflu_gpu_mem_info(size_t *freeBytes, size_t *totalBytes);
+
+/* Batched ensemble (no reference counterpart: Rocflu runs one region per
+ * rank).  nInst independent regions of one cell type under one rflu_params
+ * (gas, scheme, CFL) become one handle: the union of their meshes, each
+ * instance's cells contiguous on the device, so every kernel launch covers
+ * all instances; each instance keeps its own dt (RFLU_MinimumTimeStep over
+ * its own cells) and time.  Results equal those of each region run alone,
+ * bit for bit.  Step it with rflu_gpu_step / rflu_gpu_step_async; the
+ * routine-level dt calls reject batches.  States are per instance, in that
+ * instance's own cell numbering (SoA, 5 x nCells). */
+int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params *prm, rflu_handle **out);
+int rflu_batch_sizes(const rflu_handle *h, int inst, int *nCells, int *nInst);
+int rflu_batch_set_state(rflu_handle *h, int inst, const double *cv);
+int rflu_batch_get_state(rflu_handle *h, int inst, double *cv, double *dv);
+int rflu_batch_get_time(rflu_handle *h, int inst, double *currentTime, double *dtMin);
+
 /* Mean kernel duration (ms) per launch of the named kernel over the last
  * rflu_gpu_step call with profiling enabled; for measurement only. */
 int rflu_gpu_set_profiling(rflu_handle *h, int on);

@@ -19,6 +19,9 @@ RFLU_SYMBOLS = [
     "rflu_gpu_convective_fluxes", "rflu_gpu_source_terms", "rflu_gpu_integrate_source_terms",
     "rflu_gpu_rk_update", "rflu_gpu_rk_stage", "rflu_gpu_step", "rflu_gpu_get_time",
     "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time", "rflu_gpu_bandwidth_probe",
+    "rflu_gpu_step_async", "rflu_gpu_query", "rflu_gpu_wait", "rflu_gpu_device_bytes", "rflu_gpu_mem_info",
+    "rflu_batch_create", "rflu_batch_sizes", "rflu_batch_set_state", "rflu_batch_get_state",
+    "rflu_batch_get_time",
 ]
 
 # Every symbol include/picl_gpu.h declares.
@@ -90,6 +93,16 @@ def load():
     L.rflu_gpu_set_profiling.argtypes = [P, I]
     L.rflu_gpu_kernel_time.argtypes = [P, C.c_char_p, P, P]
     L.rflu_gpu_bandwidth_probe.argtypes = [C.c_size_t, I, P]
+    L.rflu_gpu_step_async.argtypes = [P, I]
+    L.rflu_gpu_query.argtypes = [P]
+    L.rflu_gpu_wait.argtypes = [P]
+    L.rflu_gpu_device_bytes.argtypes = [P, P]
+    L.rflu_gpu_mem_info.argtypes = [P, P]
+    L.rflu_batch_create.argtypes = [I, C.POINTER(MeshDesc), C.POINTER(Params), PP]
+    L.rflu_batch_sizes.argtypes = [P, I, P, P]
+    L.rflu_batch_set_state.argtypes = [P, I, P]
+    L.rflu_batch_get_state.argtypes = [P, I, P, P]
+    L.rflu_batch_get_time.argtypes = [P, I, P, P]
     L.picl_gpu_init.argtypes = [P, C.POINTER(PiclParams), I, P, P]
     L.picl_gpu_runge.argtypes = [P, I]
     L.picl_gpu_sizes.argtypes = [P, P, P]

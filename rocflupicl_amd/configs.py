@@ -43,12 +43,13 @@ def sod(nx=200, ny=4, nz=4, jitter=0.1, seed=1, discr=4, order=1, **kw):
     return mesh, params, ic
 
 
-def shock_tets(n=64, jitter=0.05, seed=2, discr=1, mach=1.5, order=2, **kw):
+def shock_tets(n=64, jitter=0.05, seed=2, discr=1, mach=1.5, order=2, gamma=1.4, **kw):
     mesh = mg.tet_box(n, n, n, 1.0, 1.0, 1.0, jitter=jitter, seed=seed)
     params = dict(spaceDiscr=discr, spaceOrder=order, reconst=11 if order == 2 else 0,
                   cfl=0.5, dtImposed=1.0e-7, dtMinLimit=0.0, epsentr=0.05, **AIR)
+    params.update(refGamma=gamma)
     params.update(kw)
-    rho1, p1, g = 1.2, 1.0e5, 1.4
+    rho1, p1, g = 1.2, 1.0e5, gamma
     rho2, u2, p2 = mg.normal_shock_state(mach, rho1, p1, g)
 
     def ic(cofg):
@@ -64,12 +65,13 @@ def shock_tets(n=64, jitter=0.05, seed=2, discr=1, mach=1.5, order=2, **kw):
 
 
 def shock_hex(nx=400, ny=80, nz=80, lx=0.2, ly=0.04, lz=0.04, mach=2.0, x_shock=0.02,
-              discr=4, order=2, jitter=0.0, seed=3, **kw):
+              discr=4, order=2, jitter=0.0, seed=3, gamma=1.4, **kw):
     mesh = mg.hex_box(nx, ny, nz, lx, ly, lz, jitter=jitter, seed=seed)
     params = dict(spaceDiscr=discr, spaceOrder=order, reconst=11 if order == 2 else 0,
                   cfl=0.5, dtImposed=1.0e-8, dtMinLimit=0.0, **AIR)
+    params.update(refGamma=gamma)
     params.update(kw)
-    rho1, p1, g = 1.2, 1.0e5, 1.4
+    rho1, p1, g = 1.2, 1.0e5, gamma
     rho2, u2, p2 = mg.normal_shock_state(mach, rho1, p1, g)
 
     def ic(cofg):
@@ -130,3 +132,29 @@ def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=N
     y[:, 6] = 300.0
     pconst = np.stack([np.full(npart, rho_p), dp, vp, np.ones(npart)], axis=1)
     return mesh, params, ic, (y, pconst), dict(PICL_AIR)
+
+
+def ensemble(n_inst=256, cells=(5_000, 500_000), mach=(1.2, 3.0), gammas=(1.4, 1.67), seed=7):
+    """The batched ensemble: n_inst independent shock tubes with cell counts
+    log-uniform in `cells`, half 5-tet boxes ([0,1]^3) and half 5:1:1 hex
+    boxes, shock Mach number uniform in `mach`, gamma drawn from `gammas`;
+    second-order AUSM+ with the sign-safe Venkatakrishnan guard.  Returns a
+    list of (key, mesh, params, ic) with key = (cell type, gamma): instances
+    that share a key can share one batch."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        nc = float(np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1]))))
+        m = float(rng.uniform(*mach))
+        g = float(gammas[rng.integers(len(gammas))])
+        sd = int(rng.integers(1 << 30))
+        if i % 2 == 0:
+            k = max(2, int(round((nc / 5.0) ** (1.0 / 3.0))))
+            mesh, params, ic = shock_tets(n=k, discr=4, mach=m, seed=sd, gamma=g, limiterGuard=1)
+            out.append((("tet", g), mesh, params, ic))
+        else:
+            k = max(2, int(round((nc / 5.0) ** (1.0 / 3.0))))
+            mesh, params, ic = shock_hex(nx=5 * k, ny=k, nz=k, lx=0.2, ly=0.04, lz=0.04, mach=m,
+                                         x_shock=0.02, jitter=0.05, seed=sd, gamma=g, limiterGuard=1)
+            out.append((("hex", g), mesh, params, ic))
+    return out

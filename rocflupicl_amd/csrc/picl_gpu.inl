@@ -1190,6 +1190,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (h->prm.spaceOrder != 2) return set_err(h, RFLU_GPU_EINVAL, "PICL needs spaceOrder 2 (its fields use gradients)");
   if (h->prm.patchOrder != 1) return set_err(h, RFLU_GPU_EINVAL, "PICL is built for patchOrder 1");
   if (h->prm.fluxPrecision != 0) return set_err(h, RFLU_GPU_EINVAL, "PICL runs in FP64 (fluxPrecision 0)");
+  if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "particles attach to a single region, not a batch");
   if (pp->qsFlag < 1 || pp->qsFlag > 5)
     return set_err(h, RFLU_GPU_EINVAL, "qsFlag must be 1 (Parmar), 2 (Osnes), 3 (modified Parmar), 4 (Gidaspow) or 5 "
                                        "(Schiller-Naumann)");
@@ -1406,6 +1407,7 @@ int picl_gpu_sizes(const rflu_handle *h, int *nPart, int *nLost) {
   if (!h || !h->picl) return RFLU_GPU_EINVAL;
   const PiclDev &P = h->picl->P;
   int act = 0;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return RFLU_GPU_EHIP;
   if (hipMemcpy(&act, P.nAct, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return RFLU_GPU_EHIP;
   if (nPart) *nPart = P.nP;
   if (nLost) *nLost = P.nP - act;
@@ -1499,7 +1501,7 @@ int picl_gpu_set_debug(rflu_handle *h, int on) {
   PiclDev &P = h->picl->P;
   if (on && !P.dbg) {
     if (dalloc(h, &P.dbg, PICL_NFLD * (size_t)P.nP)) return RFLU_GPU_EHIP;
-    HIPCHK(h, hipMemset(P.dbg, 0, PICL_NFLD * (size_t)P.nP * sizeof(double)));
+    HIPCHK(h, hipMemsetAsync(P.dbg, 0, PICL_NFLD * (size_t)P.nP * sizeof(double), h->stream));
   } else if (!on && P.dbg) {
     (void)hipFree(P.dbg);
     P.dbg = nullptr;

@@ -69,6 +69,12 @@ struct Dev {
   unsigned short *sSlot = nullptr;
   // FP32 evaluation mode: limited gradients as 16 floats per cell (15 + pad)
   float *gradF = nullptr;
+  // batched ensemble (nInst > 0): instance k owns device cells
+  // [iOff[k], iOff[k+1]); cInst gives each cell's instance; iScal holds each
+  // instance's scal[] slots (4 per instance)
+  int nInst = 0;
+  int *cInst = nullptr, *iOff = nullptr;
+  double *iScal = nullptr;
 };
 
 // scal[] slots
@@ -100,6 +106,8 @@ struct rflu_handle {
   std::map<std::string, Prof> prof;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> evPool;
+  size_t devBytes = 0;  // device memory allocated for this handle
+  std::vector<int> instCell, instFace, instBFace;  // batch: per-instance offsets in the union (reference numbering)
 };
 
 static std::string g_err;
@@ -195,9 +203,9 @@ __device__ __forceinline__ int xcd_block() {
 // ---------------------------------------------------------------------------
 // Records from cv (RFLU_SetVarsWrapper at start-up)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_records(Dev d, Gas G, int order) {
-  int i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= d.nC) return;
+__global__ void __launch_bounds__(BLK) k_records(Dev d, Gas G, int order, int i0 = 0, int n = -1) {
+  int i = i0 + blockIdx.x * BLK + threadIdx.x;
+  if (i >= (n < 0 ? d.nC : i0 + n)) return;
   int N = d.nC;
   double rho = d.cv[i], m1 = d.cv[N + i], m2 = d.cv[2 * N + i], m3 = d.cv[3 * N + i], E = d.cv[4 * N + i];
   DV dv = dep_vars(G, rho, m1, m2, m3, E, d.phi, i);
@@ -307,12 +315,42 @@ __global__ void __launch_bounds__(1024) k_dtfinal(Dev d, DtPar P, int nPart) {
   }
 }
 
+// Batched ensemble: each instance's MINVAL over its own cells (one workgroup
+// per instance over the per-cell dt of k_dtmin), then its
+// RFLU_MinimumTimeStep with its own time.
+__global__ void __launch_bounds__(1024) k_dt_batch(Dev d, DtPar P) {
+  __shared__ double sm[1024 / 64];
+  int k = blockIdx.x;
+  double m = __builtin_inf();
+  for (int i = d.iOff[k] + threadIdx.x; i < d.iOff[k + 1]; i += 1024) m = fmin(m, d.dtCell[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sm[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = sm[0];
+    for (int q = 1; q < 1024 / 64; q++) mm = fmin(mm, sm[q]);
+    double *sc = d.iScal + 4 * (size_t)k;
+    sc[S_DTLOCAL] = mm;
+    sc[S_DTMIN] = minimum_time_step(P, mm, sc[S_TIME], sc[S_TSR]);
+  }
+}
+
 __global__ void k_min_time_step(Dev d, DtPar P) {
   if (threadIdx.x == 0 && blockIdx.x == 0)
     d.scal[S_DTMIN] = minimum_time_step(P, d.scal[S_DTLOCAL], d.scal[S_TIME], d.scal[S_TSR]);
 }
 
 __global__ void k_advance_time(Dev d) {
+  if (d.nInst) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < d.nInst) {
+      double *sc = d.iScal + 4 * (size_t)k;
+      sc[S_TIME] = sc[S_TIME] + sc[S_DTMIN];
+      sc[S_TSR] = sc[S_TSR] + sc[S_DTMIN];
+    }
+    return;
+  }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     d.scal[S_TIME] = d.scal[S_TIME] + d.scal[S_DTMIN];
     d.scal[S_TSR] = d.scal[S_TSR] + d.scal[S_DTMIN];
@@ -694,27 +732,38 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
 }
 
 // FP32 evaluation of the same gradient (fluxPrecision 1): the tile's records
-// are staged in LDS as FP32 (state, and position relative to the tile's first
-// cell, formed in FP64), so a member costs 32 bytes of LDS; LSQ weights, sums
-// and the limiter run in FP32, GU members per iteration.
+// are staged in LDS with the state in FP32 and the centroid in FP64 (48 bytes
+// a member); centroid differences are taken in FP64 and rounded once, so a
+// cell's result does not depend on which tile it falls in.  LSQ weights,
+// sums and the limiter run in FP32, GU members per iteration.
+struct __align__(16) SlotF {
+  float4 s;          // rho, u, v, w
+  float p, pad;
+  double x;
+  double2 yz;
+};
 __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int guard) {
-  extern __shared__ float4 sRecF[];  // [nU][2]: (rho, u, v, w), (p, x - x0, y - y0, z - z0)
+  extern __shared__ SlotF sRecF[];
   int tile = xcd_block();
   int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
-  const double *r0 = d.rec + 8 * (size_t)d.tMem[u0];
-  double x0 = r0[5], y0 = r0[6], z0 = r0[7];
   for (int u = threadIdx.x; u < nU; u += BLK) {
     double r[8];
     load_rec(d.rec, d.tMem[u0 + u], r);
-    sRecF[2 * u] = make_float4((float)r[0], (float)r[1], (float)r[2], (float)r[3]);
-    sRecF[2 * u + 1] = make_float4((float)r[4], (float)(r[5] - x0), (float)(r[6] - y0), (float)(r[7] - z0));
+    SlotF t;
+    t.s = make_float4((float)r[0], (float)r[1], (float)r[2], (float)r[3]);
+    t.p = (float)r[4];
+    t.pad = 0.0f;
+    t.x = r[5];
+    t.yz = make_double2(r[6], r[7]);
+    sRecF[u] = t;
   }
   __syncthreads();
   int i = tile * BLK + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
-  float4 ia = sRecF[2 * threadIdx.x], ib = sRecF[2 * threadIdx.x + 1];
-  float ri[8] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+  const SlotF own = sRecF[threadIdx.x];
+  float ri[5] = {own.s.x, own.s.y, own.s.z, own.s.w, own.p};
+  const double xi = own.x, yi = own.yz.x, zi = own.yz.y;
   float r11 = (float)d.rm[i], r12 = (float)d.rm[N + i], r22 = (float)d.rm[2 * N + i];
   float r13 = (float)d.rm[3 * N + i], r23 = (float)d.rm[4 * N + i], r33 = (float)d.rm[5 * N + i];
   float c11 = 1.0f / r11, c22 = 1.0f / r22, c33 = 1.0f / r33;
@@ -729,8 +778,11 @@ __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int gu
   for (int k = 0; k < 5; k++) { dmin[k] = 0.0f; dmax[k] = 0.0f; }
   int s = d.csOff[i], se = d.csOff[i + 1];
   auto member = [&](unsigned j) {
-    float4 a = sRecF[2 * (j & 0x7fffu)], b = sRecF[2 * (j & 0x7fffu) + 1];
-    float dx = b.y - ri[5], dy = b.z - ri[6], dz = b.w - ri[7];
+    const SlotF &m = sRecF[j & 0x7fffu];
+    float4 a = m.s;
+    float bp = m.p;
+    double2 yz = m.yz;
+    float dx = (float)(m.x - xi), dy = (float)(yz.x - yi), dz = (float)(yz.y - zi);
     float term = rsqrtf(dx * dx + dy * dy + dz * dz);
     dx = term * dx; dy = term * dy; dz = term * dz;
     float term1 = c1111 * dx;
@@ -739,7 +791,7 @@ __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int gu
     float wtx = term * (term1 + c12 * term2 + c13 * term3);
     float wty = term * (term2 + c23 * term3);
     float wtz = term * term3;
-    float dv[5] = {a.x - ri[0], a.y - ri[1], a.z - ri[2], a.w - ri[3], b.x - ri[4]};
+    float dv[5] = {a.x - ri[0], a.y - ri[1], a.z - ri[2], a.w - ri[3], bp - ri[4]};
     bool face = (j & 0x8000u) != 0;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
@@ -1569,7 +1621,7 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
     for (int k = 0; k < 5; k++) rhs[k] = d.rhs[k * N + i];
   }
   // libfloflu/RkUpdateGeneric.F90:285-314 (Wray RK3, static grid)
-  double fac = P.ark * d.scal[S_DTMIN];
+  double fac = P.ark * (d.nInst ? d.iScal[4 * (size_t)d.cInst[i] + S_DTMIN] : d.scal[S_DTMIN]);
   double adtv = fac / vol;
   double cv[5];
 #pragma unroll
@@ -1656,9 +1708,9 @@ __global__ void __launch_bounds__(BLK) k_copy_probe(double2 *__restrict__ dst, c
 }
 
 // dv for output
-__global__ void __launch_bounds__(BLK) k_dv_out(Dev d, Gas G, int order, double *dv) {
-  int i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= d.nC) return;
+__global__ void __launch_bounds__(BLK) k_dv_out(Dev d, Gas G, int order, double *dv, int i0 = 0, int n = -1) {
+  int i = i0 + blockIdx.x * BLK + threadIdx.x;
+  if (i >= (n < 0 ? d.nC : i0 + n)) return;
   int N = d.nC;
   double r[8];
   load_rec(d.rec, i, r);
@@ -1713,7 +1765,11 @@ template <class T>
 int dalloc(rflu_handle *h, T **p, size_t n, const T *src = nullptr) {
   if (n == 0) n = 1;
   HIPCHK(h, hipMalloc((void **)p, n * sizeof(T)));
-  if (src) HIPCHK(h, hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  h->devBytes += n * sizeof(T);
+  if (src) {  // the copy runs on the null stream, which the handle's non-blocking stream does not wait for
+    HIPCHK(h, hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(h, hipStreamSynchronize(nullptr));
+  }
   return 0;
 }
 
@@ -1755,6 +1811,15 @@ DtPar dtpar(const rflu_handle *h, int finalize) {
 }
 
 int launch_dtmin(rflu_handle *h, int finalize, int storeCell) {
+  if (h->d.nInst) {
+    {
+      ProfScope ps(h, "k_dtmin");
+      hipLaunchKernelGGL(k_dtmin, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, 1);
+    }
+    ProfScope ps(h, "k_dtfinal");
+    hipLaunchKernelGGL(k_dt_batch, dim3(h->d.nInst), dim3(1024), 0, h->stream, h->d, dtpar(h, 1));
+    return 0;
+  }
   {
     ProfScope ps(h, "k_dtmin");
     hipLaunchKernelGGL(k_dtmin, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, storeCell);
@@ -1770,7 +1835,8 @@ int launch_grad(rflu_handle *h) {
   int venkat = h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, guard = h->prm.limiterGuard ? 1 : 0;
   int nUmax = (int)(h->tileLds / 64);
   if (h->prm.fluxPrecision == 1) {
-    hipLaunchKernelGGL(k_grad_tile_f32, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 2, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL(k_grad_tile_f32, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 64 * sizeof(SlotF), h->stream, h->d,
+                       venkat, guard);
     return 0;
   }
   if (h->picl && h->gradVariant == 4 && h->tileLds && h->tileLds + 8 * (size_t)nUmax <= h->ldsMax)
@@ -1853,7 +1919,8 @@ void enqueue_stage(rflu_handle *h, int st, bool withDt) {
 // One RFLU_TimeStepping iteration.
 int enqueue_step(rflu_handle *h) {
   for (int st = 1; st <= 3; st++) enqueue_stage(h, st, st == 1);
-  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, h->stream, h->d);
+  hipLaunchKernelGGL(k_advance_time, dim3(h->d.nInst ? (h->d.nInst + 255) / 256 : 1), dim3(h->d.nInst ? 256 : 64), 0,
+                     h->stream, h->d);
   return 0;
 }
 
@@ -1873,7 +1940,10 @@ extern "C" {
 
 const char *rflu_gpu_last_error(const rflu_handle *h) { return h ? h->err.c_str() : g_err.c_str(); }
 
-int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handle **out) {
+// groupOff (batched ensemble): the mesh is the disjoint union of instances,
+// instance k holding reference cells [groupOff[k], groupOff[k+1]).
+static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handle **out,
+                       const std::vector<int> *groupOff) {
   if (!md || !prm || !out) return set_err(nullptr, RFLU_GPU_EINVAL, "null argument");
   *out = nullptr;
   int ndev = 0;
@@ -1921,7 +1991,12 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   for (int q = 0; q < 3; q++) G.accel[q] = prm->accel[q];
   Dev &d = h->d;
   d.nC = m.nCells; d.nF = m.nFaces; d.nB = m.nBFaces; d.S = m.nFaces + m.nBFaces; d.nBlk = nblk(d.nC);
-  if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed"); }
+  // non-blocking: work of other handles (other requests in flight) and
+  // null-stream copies do not serialise against this handle's stream
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed");
+  }
   int rc = 0;
   // Device numbering: cells and faces are renumbered along a Morton curve of
   // their centroids so stencil and face neighbours sit close in HBM and L2.
@@ -1931,6 +2006,13 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   morton_order(m.cofg.data(), N, h->perm);
   if (getenv("ROCFLUPICL_NO_MORTON"))
     for (int i = 0; i < N; i++) h->perm[i] = i;
+  std::vector<int> grp;
+  if (groupOff) {  // instances contiguous on the device, Morton order within each
+    grp.assign(N, 0);
+    for (size_t k = 0; k + 1 < groupOff->size(); k++)
+      for (int o = (*groupOff)[k]; o < (*groupOff)[k + 1]; o++) grp[o] = (int)k;
+    std::stable_sort(h->perm.begin(), h->perm.end(), [&](int a, int b) { return grp[a] < grp[b]; });
+  }
   h->inv.assign(N, 0);
   for (int i = 0; i < N; i++) h->inv[h->perm[i]] = i;
   std::vector<int> fperm, bperm;
@@ -2098,16 +2180,126 @@ int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handl
   rc |= dalloc(h, &d.dvOut, 3 * (size_t)N);
   rc |= dalloc(h, &d.partial, (size_t)d.nBlk);
   rc |= dalloc(h, &d.scal, 8);
+  if (groupOff) {
+    int K = (int)groupOff->size() - 1;
+    std::vector<int> cI(N);
+    for (int i = 0; i < N; i++) cI[i] = grp[h->perm[i]];
+    std::vector<double> zero(4 * (size_t)K, 0.0);
+    d.nInst = K;
+    rc |= dalloc(h, &d.cInst, (size_t)N, cI.data());
+    rc |= dalloc(h, &d.iOff, groupOff->size(), groupOff->data());  // same counts, contiguous on the device
+    rc |= dalloc(h, &d.iScal, 4 * (size_t)K, zero.data());
+  }
   rc |= dalloc(h, &d.err, 1);
   if (rc) { std::string msg = h->err; rflu_gpu_destroy(h); return set_err(nullptr, RFLU_GPU_EHIP, msg); }
   d.cfdy = d.cfdx + cfIdxD.size(); d.cfdz = d.cfdx + 2 * cfIdxD.size();
   d.fnx = faceSoa; d.fny = faceSoa + S; d.fnz = faceSoa + 2 * (size_t)S; d.fnm = faceSoa + 3 * (size_t)S;
   d.fcx = faceSoa + 4 * (size_t)S; d.fcy = faceSoa + 5 * (size_t)S; d.fcz = faceSoa + 6 * (size_t)S;
-  hipMemset(d.scal, 0, 8 * sizeof(double));
-  hipMemset(d.err, 0, sizeof(int));
-  hipMemset(d.rhsSum, 0, 5 * (size_t)N * sizeof(double));
-  hipDeviceSynchronize();
+  (void)hipMemsetAsync(d.scal, 0, 8 * sizeof(double), h->stream);
+  (void)hipMemsetAsync(d.err, 0, sizeof(int), h->stream);
+  (void)hipMemsetAsync(d.rhsSum, 0, 5 * (size_t)N * sizeof(double), h->stream);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    rflu_gpu_destroy(h);
+    return set_err(nullptr, RFLU_GPU_EHIP, "set-up failed on the device");
+  }
   *out = h;
+  return 0;
+}
+
+int rflu_gpu_create(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handle **out) {
+  return create_impl(md, prm, out, nullptr);
+}
+
+int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params *prm, rflu_handle **out) {
+  if (nInst < 1 || !meshes || !prm || !out) return set_err(nullptr, RFLU_GPU_EINVAL, "null argument or nInst < 1");
+  *out = nullptr;
+  const int nvc = meshes[0].nvc, nvf = meshes[0].nvf;
+  std::vector<double> xyz;
+  std::vector<int> c2v, bcType, bcOpt, nbf, bf2v, cellOff{0};
+  std::vector<double> pout;
+  int nV = 0;
+  for (int k = 0; k < nInst; k++) {
+    const rflu_mesh_desc &m = meshes[k];
+    if (m.nvc != nvc || m.nvf != nvf)
+      return set_err(nullptr, RFLU_GPU_EINVAL, "a batch holds one cell type (all tets or all hexes)");
+    xyz.insert(xyz.end(), m.xyz, m.xyz + 3 * (size_t)m.nVert);
+    for (size_t q = 0; q < (size_t)m.nvc * m.nCells; q++) c2v.push_back(m.c2v[q] + nV);
+    size_t nb = 0;
+    for (int p = 0; p < m.nPatches; p++) {
+      bcType.push_back(m.bcType[p]);
+      bcOpt.push_back(m.bcOpt[p]);
+      pout.push_back(m.pout[p]);
+      nbf.push_back(m.nBFaces[p]);
+      nb += (size_t)m.nBFaces[p];
+    }
+    for (size_t q = 0; q < nb * (size_t)m.nvf; q++) bf2v.push_back(m.bf2v[q] + nV);
+    nV += m.nVert;
+    cellOff.push_back(cellOff.back() + m.nCells);
+  }
+  rflu_mesh_desc u{nV, xyz.data(), nvc, cellOff.back(), c2v.data(), (int)bcType.size(), bcType.data(),
+                   bcOpt.data(), pout.data(), nbf.data(), nvf, bf2v.data()};
+  if (int e = create_impl(&u, prm, out, &cellOff)) return e;
+  (*out)->instCell = cellOff;
+  return 0;
+}
+
+int rflu_batch_sizes(const rflu_handle *h, int inst, int *nCells, int *nInst) {
+  if (!h || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
+  if (nCells) *nCells = h->instCell[inst + 1] - h->instCell[inst];
+  if (nInst) *nInst = h->d.nInst;
+  return 0;
+}
+
+int rflu_batch_set_state(rflu_handle *h, int inst, const double *cv) {
+  if (!h || !cv || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
+  const int off = h->instCell[inst], nk = h->instCell[inst + 1] - off, N = h->d.nC;
+  std::vector<double> tmp(5 * (size_t)nk);
+  for (int i = 0; i < nk; i++) {
+    int o = h->perm[off + i] - off;
+    for (int v = 0; v < 5; v++) tmp[v * (size_t)nk + i] = cv[v * (size_t)nk + o];
+  }
+  for (int v = 0; v < 5; v++)
+    HIPCHK(h, hipMemcpyAsync(h->d.cv + v * (size_t)N + off, tmp.data() + v * (size_t)nk, nk * sizeof(double),
+                             hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_records, dim3(nblk(nk)), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder, off, nk);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int rflu_batch_get_state(rflu_handle *h, int inst, double *cv, double *dv) {
+  if (!h || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
+  const int off = h->instCell[inst], nk = h->instCell[inst + 1] - off, N = h->d.nC;
+  std::vector<double> tmp(5 * (size_t)nk);
+  auto fetch = [&](double *dst, const double *src, int nv) -> int {
+    for (int v = 0; v < nv; v++)
+      HIPCHK(h, hipMemcpyAsync(tmp.data() + v * (size_t)nk, src + v * (size_t)N + off, nk * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < nk; i++) {
+      int o = h->perm[off + i] - off;
+      for (int v = 0; v < nv; v++) dst[v * (size_t)nk + o] = tmp[v * (size_t)nk + i];
+    }
+    return 0;
+  };
+  if (cv)
+    if (int e = fetch(cv, h->d.cv, 5)) return e;
+  if (dv) {
+    hipLaunchKernelGGL(k_dv_out, dim3(nblk(nk)), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder,
+                       h->d.dvOut, off, nk);
+    HIPCHK(h, hipGetLastError());
+    if (int e = fetch(dv, h->d.dvOut, 3)) return e;
+  }
+  return 0;
+}
+
+int rflu_batch_get_time(rflu_handle *h, int inst, double *t, double *dtMin) {
+  if (!h || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
+  double sc[4];
+  HIPCHK(h, hipMemcpyAsync(sc, h->d.iScal + 4 * (size_t)inst, sizeof sc, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (t) *t = sc[S_TIME];
+  if (dtMin) *dtMin = sc[S_DTMIN];
   return 0;
 }
 
@@ -2119,7 +2311,7 @@ int rflu_gpu_destroy(rflu_handle *h) {
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);
   for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot,
-                  (void *)d.gradF})
+                  (void *)d.gradF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal})
     if (p) hipFree(p);
   picl_free(h->picl);
   if (h->graph) hipGraphExecDestroy(h->graph);
@@ -2227,6 +2419,7 @@ int rflu_gpu_get_grad(rflu_handle *h, double *grad) {
 
 int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLocal) {
   if (!h) return RFLU_GPU_EINVAL;
+  if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "a batch steps with rflu_gpu_step (per-instance dt)");
   {
     ProfScope ps(h, "k_srad");
     hipLaunchKernelGGL(k_srad, dim3(nblk(h->d.S)), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
@@ -2242,6 +2435,7 @@ int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLoc
 
 int rflu_gpu_minimum_time_step(rflu_handle *h, double *dtMin) {
   if (!h) return RFLU_GPU_EINVAL;
+  if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "a batch steps with rflu_gpu_step (per-instance dt)");
   hipLaunchKernelGGL(k_min_time_step, dim3(1), dim3(64), 0, h->stream, h->d, dtpar(h, 1));
   HIPCHK(h, hipGetLastError());
   if (dtMin) HIPCHK(h, hipMemcpyAsync(dtMin, h->d.scal + S_DTMIN, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -2296,17 +2490,56 @@ int rflu_gpu_rk_stage(rflu_handle *h, int istage) {
   return check_err(h);
 }
 
+static int capture_step(rflu_handle *h) {
+  if (h->graph) return 0;
+  hipGraph_t g;
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(h);
+  HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+  HIPCHK(h, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+  hipGraphDestroy(g);
+  return 0;
+}
+
+int rflu_gpu_step_async(rflu_handle *h, int nSteps) {
+  if (!h || nSteps < 0) return RFLU_GPU_EINVAL;
+  if (int e = capture_step(h)) return e;
+  for (int n = 0; n < nSteps; n++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+  return 0;
+}
+
+int rflu_gpu_query(rflu_handle *h) {
+  if (!h) return RFLU_GPU_EINVAL;
+  hipError_t e = hipStreamQuery(h->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  h->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+  return RFLU_GPU_EHIP;
+}
+
+int rflu_gpu_wait(rflu_handle *h) {
+  if (!h) return RFLU_GPU_EINVAL;
+  return check_err(h);
+}
+
+int rflu_gpu_device_bytes(const rflu_handle *h, size_t *bytes) {
+  if (!h || !bytes) return RFLU_GPU_EINVAL;
+  *bytes = h->devBytes;
+  return 0;
+}
+
+int rflu_gpu_mem_info(size_t *freeBytes, size_t *totalBytes) {
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return set_err(nullptr, RFLU_GPU_EHIP, "hipMemGetInfo failed");
+  if (freeBytes) *freeBytes = f;
+  if (totalBytes) *totalBytes = t;
+  return 0;
+}
+
 int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph) {
   if (!h || nSteps < 0) return RFLU_GPU_EINVAL;
   if (useGraph && !h->profiling) {
-    if (!h->graph) {
-      hipGraph_t g;
-      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(h);
-      HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-      HIPCHK(h, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
-      hipGraphDestroy(g);
-    }
+    if (int e = capture_step(h)) return e;
     for (int n = 0; n < nSteps; n++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
   } else {
     for (int n = 0; n < nSteps; n++) enqueue_step(h);

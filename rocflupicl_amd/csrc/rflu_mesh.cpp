@@ -4,6 +4,7 @@
 #include "rflu_mesh.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -308,87 +309,126 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
   }
   m.csOff.assign(N + 1, 0);
   m.csMem.clear();
-  m.csMem.reserve((size_t)N * 30);
   m.moms.assign(10 * (size_t)N, 0.0);
   m.stencilMinSvRatio = 1e300;
-  std::vector<int> cand;
-  std::vector<double> frac;
-  std::vector<std::array<double, 4>> amat;
+  // Cells are independent: chunks of cells run on host threads, each with
+  // its own member list, merged in cell order (the result does not depend
+  // on the number of threads).
+  struct Part {
+    std::vector<int> mem, cnt;
+    double minRatio = 1e300;
+    int maxKeep = 0;
+    std::string err;
+  };
+  int nThr = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (N < 20000) nThr = 1;
+  std::vector<Part> parts(nThr);
   const double iNVert = 1.0 / (double)in.nvc;
-  for (int i = 0; i < N; i++) {
-    const int *vi = in.c2v + (size_t)i * in.nvc;
-    cand.clear();
-    for (int q = 0; q < in.nvc; q++)
-      for (int t = v2cOff[vi[q]]; t < v2cOff[vi[q] + 1]; t++)
-        if (v2c[t] != i) cand.push_back(v2c[t]);
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-    int own[8];
-    std::copy(vi, vi + in.nvc, own);
-    std::sort(own, own + in.nvc);
-    frac.resize(cand.size());
-    for (size_t t = 0; t < cand.size(); t++) {
-      int oth[8];
-      std::copy(in.c2v + (size_t)cand[t] * in.nvc, in.c2v + (size_t)(cand[t] + 1) * in.nvc, oth);
-      std::sort(oth, oth + in.nvc);
-      int common = 0;
-      for (int a = 0, b = 0; a < in.nvc && b < in.nvc;) {
-        if (own[a] == oth[b]) { common++; a++; b++; }
-        else if (own[a] < oth[b]) a++;
-        else b++;
+  auto work = [&](int tid) {
+    Part &P = parts[tid];
+    const int c0 = (int)((long long)N * tid / nThr), c1 = (int)((long long)N * (tid + 1) / nThr);
+    P.cnt.reserve(c1 - c0);
+    P.mem.reserve((size_t)(c1 - c0) * 30);
+    std::vector<int> cand, mem;
+    std::vector<double> frac;
+    std::vector<std::array<double, 4>> amat;
+    for (int i = c0; i < c1; i++) {
+      const int *vi = in.c2v + (size_t)i * in.nvc;
+      cand.clear();
+      for (int q = 0; q < in.nvc; q++)
+        for (int t = v2cOff[vi[q]]; t < v2cOff[vi[q] + 1]; t++)
+          if (v2c[t] != i) cand.push_back(v2c[t]);
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      int own[8];
+      std::copy(vi, vi + in.nvc, own);
+      std::sort(own, own + in.nvc);
+      frac.resize(cand.size());
+      for (size_t t = 0; t < cand.size(); t++) {
+        int oth[8];
+        std::copy(in.c2v + (size_t)cand[t] * in.nvc, in.c2v + (size_t)(cand[t] + 1) * in.nvc, oth);
+        std::sort(oth, oth + in.nvc);
+        int common = 0;
+        for (int a = 0, b = 0; a < in.nvc && b < in.nvc;) {
+          if (own[a] == oth[b]) { common++; a++; b++; }
+          else if (own[a] < oth[b]) a++;
+          else b++;
+        }
+        frac[t] = (double)common * iNVert;
       }
-      frac[t] = (double)common * iNVert;
+      int n2 = (int)cand.size();
+      pair_quicksort(frac.data(), cand.data(), n2);
+      int keep = std::min(n2, SMAX);
+      mem.assign(cand.end() - keep, cand.end());
+      std::sort(mem.begin(), mem.end());
+      if (keep < SMIN) { P.err = "stencil smaller than 4 cells needs layer addition (not supported)"; return; }
+      const double *x0 = &m.cofg[3 * (size_t)i];
+      amat.resize(keep);
+      for (int q = 0; q < keep; q++) {
+        const double *xj = &m.cofg[3 * (size_t)mem[q]];
+        double dx = xj[0] - x0[0], dy = xj[1] - x0[1], dz = xj[2] - x0[2];
+        double term = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+        amat[q] = {term, term * dx, term * dy, term * dz};
+      }
+      double s0;
+      double ratio = min_sv_ratio(amat, s0);
+      P.minRatio = std::min(P.minRatio, ratio);
+      if (!(s0 > 100.0 * 2.220446049250313e-16) || !(ratio > 0.01)) {
+        P.err = "singular c2c stencil needs layer addition (not supported)";
+        return;
+      }
+      P.maxKeep = std::max(P.maxKeep, keep);
+      // RFLU_ComputeStencilMoments3D1 (RFLU_ModWeights.F90:1770-1894)
+      double xm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = 0; q < keep; q++) {
+        const double *xj = &m.cofg[3 * (size_t)mem[q]];
+        double dx = xj[0] - x0[0], dy = xj[1] - x0[1], dz = xj[2] - x0[2];
+        double wt = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+        dx = wt * dx; dy = wt * dy; dz = wt * dz;
+        xm[0] = xm[0] + dx * dx; xm[1] = xm[1] + dx * dy; xm[2] = xm[2] + dy * dy;
+        xm[3] = xm[3] + dx * dz; xm[4] = xm[4] + dy * dz; xm[5] = xm[5] + dz * dz;
+        xm[6] = xm[6] + wt * dx; xm[7] = xm[7] + wt * dy; xm[8] = xm[8] + wt * dz;
+        xm[9] = xm[9] + wt * wt;
+      }
+      double r11 = std::sqrt(xm[0]);
+      double ir11 = 1.0 / r11;
+      double r12 = ir11 * xm[1];
+      double r22 = std::sqrt(xm[2] - r12 * r12);
+      double ir22 = 1.0 / r22;
+      double r13 = ir11 * xm[3];
+      double r23 = ir22 * (xm[4] - r12 * r13);
+      double r33 = std::sqrt(xm[5] - (r13 * r13 + r23 * r23));
+      double ir33 = 1.0 / r33;
+      double r14 = ir11 * xm[6];
+      double r24 = ir22 * (xm[7] - r12 * r14);
+      double r34 = ir33 * (xm[8] - (r13 * r14 + r23 * r24));
+      double r44 = std::sqrt(xm[9] - (r14 * r14 + r24 * r24 + r34 * r34));
+      double *mo = &m.moms[10 * (size_t)i];
+      mo[0] = r11; mo[1] = r12; mo[2] = r22; mo[3] = r13; mo[4] = r23;
+      mo[5] = r33; mo[6] = r14; mo[7] = r24; mo[8] = r34; mo[9] = r44;
+      P.mem.insert(P.mem.end(), mem.begin(), mem.end());
+      P.cnt.push_back(keep);
     }
-    int n2 = (int)cand.size();
-    pair_quicksort(frac.data(), cand.data(), n2);
-    int keep = std::min(n2, SMAX);
-    std::vector<int> mem(cand.end() - keep, cand.end());
-    std::sort(mem.begin(), mem.end());
-    if (keep < SMIN) return "stencil smaller than 4 cells needs layer addition (not supported)";
-    const double *x0 = &m.cofg[3 * (size_t)i];
-    amat.resize(keep);
-    for (int q = 0; q < keep; q++) {
-      const double *xj = &m.cofg[3 * (size_t)mem[q]];
-      double dx = xj[0] - x0[0], dy = xj[1] - x0[1], dz = xj[2] - x0[2];
-      double term = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
-      amat[q] = {term, term * dx, term * dy, term * dz};
-    }
-    double s0;
-    double ratio = min_sv_ratio(amat, s0);
-    m.stencilMinSvRatio = std::min(m.stencilMinSvRatio, ratio);
-    if (!(s0 > 100.0 * 2.220446049250313e-16) || !(ratio > 0.01))
-      return "singular c2c stencil needs layer addition (not supported)";
-    m.stencilMax = std::max(m.stencilMax, keep);
-    // RFLU_ComputeStencilMoments3D1 (RFLU_ModWeights.F90:1770-1894)
-    double xm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int q = 0; q < keep; q++) {
-      const double *xj = &m.cofg[3 * (size_t)mem[q]];
-      double dx = xj[0] - x0[0], dy = xj[1] - x0[1], dz = xj[2] - x0[2];
-      double wt = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
-      dx = wt * dx; dy = wt * dy; dz = wt * dz;
-      xm[0] = xm[0] + dx * dx; xm[1] = xm[1] + dx * dy; xm[2] = xm[2] + dy * dy;
-      xm[3] = xm[3] + dx * dz; xm[4] = xm[4] + dy * dz; xm[5] = xm[5] + dz * dz;
-      xm[6] = xm[6] + wt * dx; xm[7] = xm[7] + wt * dy; xm[8] = xm[8] + wt * dz;
-      xm[9] = xm[9] + wt * wt;
-    }
-    double r11 = std::sqrt(xm[0]);
-    double ir11 = 1.0 / r11;
-    double r12 = ir11 * xm[1];
-    double r22 = std::sqrt(xm[2] - r12 * r12);
-    double ir22 = 1.0 / r22;
-    double r13 = ir11 * xm[3];
-    double r23 = ir22 * (xm[4] - r12 * r13);
-    double r33 = std::sqrt(xm[5] - (r13 * r13 + r23 * r23));
-    double ir33 = 1.0 / r33;
-    double r14 = ir11 * xm[6];
-    double r24 = ir22 * (xm[7] - r12 * r14);
-    double r34 = ir33 * (xm[8] - (r13 * r14 + r23 * r24));
-    double r44 = std::sqrt(xm[9] - (r14 * r14 + r24 * r24 + r34 * r34));
-    double *mo = &m.moms[10 * (size_t)i];
-    mo[0] = r11; mo[1] = r12; mo[2] = r22; mo[3] = r13; mo[4] = r23;
-    mo[5] = r33; mo[6] = r14; mo[7] = r24; mo[8] = r34; mo[9] = r44;
-    m.csMem.insert(m.csMem.end(), mem.begin(), mem.end());
-    m.csOff[i + 1] = (int)m.csMem.size();
+  };
+  if (nThr == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nThr; t++) th.emplace_back(work, t);
+    for (auto &t : th) t.join();
+  }
+  size_t total = 0;
+  for (auto &P : parts) {
+    if (!P.err.empty()) return P.err;
+    total += P.mem.size();
+  }
+  m.csMem.reserve(total);
+  int i = 0;
+  for (auto &P : parts) {
+    m.stencilMinSvRatio = std::min(m.stencilMinSvRatio, P.minRatio);
+    m.stencilMax = std::max(m.stencilMax, P.maxKeep);
+    m.csMem.insert(m.csMem.end(), P.mem.begin(), P.mem.end());
+    for (int c : P.cnt) { m.csOff[i + 1] = m.csOff[i] + c; i++; }
   }
   return "";
 }

@@ -33,6 +33,22 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def mesh_desc(mesh):
+    """rflu_mesh_desc of a mesh dict (xyz, c2v, patches) and the arrays it
+    points into (keep them alive while the descriptor is used)."""
+    xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
+    c2v = np.ascontiguousarray(mesh["c2v"], dtype=np.int32)
+    pts = mesh["patches"]
+    bt = np.array([p["bcType"] for p in pts], np.int32)
+    bo = np.array([p["bcOpt"] for p in pts], np.int32)
+    po = np.array([p["pout"] for p in pts], np.float64)
+    nb = np.array([p["bf2v"].shape[0] for p in pts], np.int32)
+    bf = np.ascontiguousarray(np.concatenate([p["bf2v"] for p in pts]), np.int32)
+    md = _lib.MeshDesc(xyz.shape[0], _p(xyz), c2v.shape[1], c2v.shape[0], _p(c2v), len(pts),
+                       _p(bt), _p(bo), _p(po), _p(nb), bf.shape[1], _p(bf))
+    return md, [xyz, c2v, bt, bo, po, nb, bf]
+
+
 def make_params(**kw):
     d = dict(DEFAULTS)
     unknown = set(kw) - set(d)
@@ -56,17 +72,7 @@ class Region:
         L = _lib.load()
         self._L = L
         self.params = make_params(**params)
-        xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
-        c2v = np.ascontiguousarray(mesh["c2v"], dtype=np.int32)
-        pts = mesh["patches"]
-        self._keep = [xyz, c2v]
-        bt = np.array([p["bcType"] for p in pts], np.int32)
-        bo = np.array([p["bcOpt"] for p in pts], np.int32)
-        po = np.array([p["pout"] for p in pts], np.float64)
-        nb = np.array([p["bf2v"].shape[0] for p in pts], np.int32)
-        bf = np.ascontiguousarray(np.concatenate([p["bf2v"] for p in pts]), np.int32)
-        md = _lib.MeshDesc(xyz.shape[0], _p(xyz), c2v.shape[1], c2v.shape[0], _p(c2v), len(pts),
-                           _p(bt), _p(bo), _p(po), _p(nb), bf.shape[1], _p(bf))
+        md, self._keep = mesh_desc(mesh)
         h = C.c_void_p()
         rc = L.rflu_gpu_create(C.byref(md), C.byref(self.params), C.byref(h))
         _lib.check(rc, None)
@@ -239,6 +245,26 @@ class Region:
     def sync(self):
         self._chk(self._L.rflu_gpu_sync(self.h))
 
+    # ---- asynchronous stepping (several regions in flight) --------------
+    def step_async(self, nSteps):
+        """Enqueue nSteps graph replays on this region's stream; returns at once."""
+        self._chk(self._L.rflu_gpu_step_async(self.h, int(nSteps)))
+
+    def idle(self):
+        rc = self._L.rflu_gpu_query(self.h)
+        if rc < 0:
+            self._chk(rc)
+        return rc == 1
+
+    def wait(self):
+        """Block until the stream drains; raise on AfterUpdateMP's checks."""
+        self._chk(self._L.rflu_gpu_wait(self.h))
+
+    def device_bytes(self):
+        b = C.c_size_t()
+        self._chk(self._L.rflu_gpu_device_bytes(self.h, C.byref(b)))
+        return b.value
+
     def set_profiling(self, on):
         self._chk(self._L.rflu_gpu_set_profiling(self.h, 1 if on else 0))
 
@@ -246,3 +272,68 @@ class Region:
         ms, n = C.c_double(), C.c_int()
         self._chk(self._L.rflu_gpu_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    L = _lib.load()
+    f, t = C.c_size_t(), C.c_size_t()
+    _lib.check(L.rflu_gpu_mem_info(C.byref(f), C.byref(t)), None)
+    return f.value, t.value
+
+
+class Batch(Region):
+    """Batched ensemble: independent regions of one cell type under one
+    parameter set in one handle (include/rflu_gpu.h rflu_batch_*).  Every
+    kernel launch covers all instances; each instance keeps its own dt and
+    time, and its result equals a Region run alone, bit for bit.  Step it
+    with RFLU_TimeStepping / step_async; states are per instance."""
+
+    def __init__(self, meshes, **params):
+        L = _lib.load()
+        self._L = L
+        self.params = make_params(**params)
+        descs, self._keep = [], []
+        for m in meshes:
+            md, keep = mesh_desc(m)
+            descs.append(md)
+            self._keep += keep
+        arr = (_lib.MeshDesc * len(descs))(*descs)
+        self._keep.append(arr)
+        h = C.c_void_p()
+        _lib.check(L.rflu_batch_create(len(descs), arr, C.byref(self.params), C.byref(h)), None)
+        self.h = h
+        n = [C.c_int() for _ in range(4)]
+        L.rflu_gpu_sizes(self.h, *[C.byref(x) for x in n])
+        self.nCells, self.nFaces, self.nBFaces, self.nStencil = (x.value for x in n)
+        self.nInst = len(descs)
+        self.instCells = []
+        for k in range(self.nInst):
+            nc = C.c_int()
+            self._chk(L.rflu_batch_sizes(self.h, k, C.byref(nc), None))
+            self.instCells.append(nc.value)
+        self.instOff = np.concatenate([[0], np.cumsum(self.instCells)]).astype(int)
+        self._cofg = None
+
+    def cofg(self, inst):
+        """Cell centroids of one instance, in its own cell numbering."""
+        if self._cofg is None:
+            self._cofg = self.mesh_arrays()["cofg"]
+        return self._cofg[self.instOff[inst]:self.instOff[inst + 1]]
+
+    def set_state(self, inst, cv):
+        cv = np.asarray(cv, dtype=np.float64)
+        assert cv.shape == (self.instCells[inst], 5)
+        soa = np.ascontiguousarray(cv.T)
+        self._chk(self._L.rflu_batch_set_state(self.h, int(inst), _p(soa)))
+
+    def get_state(self, inst):
+        n = self.instCells[inst]
+        cv, dv = np.zeros((5, n)), np.zeros((3, n))
+        self._chk(self._L.rflu_batch_get_state(self.h, int(inst), _p(cv), _p(dv)))
+        return cv.T.copy(), dv.T.copy()
+
+    def time(self, inst):
+        t, dt = C.c_double(), C.c_double()
+        self._chk(self._L.rflu_batch_get_time(self.h, int(inst), C.byref(t), C.byref(dt)))
+        return t.value, dt.value
