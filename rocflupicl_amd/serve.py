@@ -21,7 +21,7 @@ state, optionally a particle cloud.  The server
 
 Each request's result is independent of the mix it ran in: regions share no
 device state, and every kernel is deterministic, so a request gives the same
-bits alone and under load (tests/test_gpu_serve.py checks this).
+bits alone and under load (tests/test_gpu_batch_serve.py checks this).
 """
 import concurrent.futures as cf
 import dataclasses
