@@ -68,7 +68,9 @@ struct Dev {
   int *tOff = nullptr, *tMem = nullptr;
   unsigned short *sSlot = nullptr;
   // FP32 evaluation mode: limited gradients as 16 floats per cell (15 + pad)
-  float *gradF = nullptr;
+  // and the interior faces' fluxes as 5 floats per face (boundary faces keep
+  // their FP64 fluxes in flux[])
+  float *gradF = nullptr, *fluxF = nullptr;
   // batched ensemble (nInst > 0): instance k owns device cells
   // [iOff[k], iOff[k+1]); cInst gives each cell's instance; iScal holds each
   // instance's scal[] slots (4 per instance)
@@ -1474,8 +1476,10 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
         hllc_flux_f(g - 1.0f, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, el, al, ql, Hl, B.r, B.u, B.v, B.w, B.p, er,
                     ar, qr, Hr, ff);
       }
+      float *of = d.fluxF + 5 * (size_t)s;
 #pragma unroll
-      for (int k = 0; k < 5; k++) f[k] = (double)ff[k];
+      for (int k = 0; k < 5; k++) of[k] = ff[k];
+      return;
     } else {
       double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
       FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, xc, yc, zc, rpL);
@@ -1570,13 +1574,22 @@ struct UpdPar { int order, istage, fused, storeRhs; double ark, grk; };
 
 // rhs = -diss (diss = 0) + face fluxes in ascending slot order
 // (libfloflu/ConvectiveFluxes.F90:133-141 and the face/patch loops).
+// F32: interior faces' fluxes come from the FP32 array (FP32 evaluation mode)
+template <int F32>
 __device__ __forceinline__ void gather_rhs(const Dev &d, int i, double rhs[5]) {
 #pragma unroll
   for (int k = 0; k < 5; k++) rhs[k] = -0.0;
   for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
     int code = d.cfIdx[e];
-    const double *f = d.flux + 5 * (size_t)(code >> 1);
-    double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    int slot = code >> 1;
+    double f0, f1, f2, f3, f4;
+    if (F32 && slot < d.nF) {  // interior fluxes widened, summed in FP64
+      const float *f = d.fluxF + 5 * (size_t)slot;
+      f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4];
+    } else {
+      const double *f = d.flux + 5 * (size_t)slot;
+      f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4];
+    }
     if (code & 1) {
       rhs[0] = rhs[0] - f0; rhs[1] = rhs[1] - f1; rhs[2] = rhs[2] - f2; rhs[3] = rhs[3] - f3; rhs[4] = rhs[4] - f4;
     } else {
@@ -1603,6 +1616,7 @@ __device__ __forceinline__ void gravity(const Gas &G, int order, const double r[
   rhs[4] = rhs[4] - vol * (G.accel[0] * m1 + G.accel[1] * m2 + G.accel[2] * m3);
 }
 
+template <int F32>
 __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
   int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
@@ -1610,7 +1624,7 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
   double rhs[5];
   double vol = d.vol[i];
   if (P.fused) {
-    gather_rhs(d, i, rhs);
+    gather_rhs<F32>(d, i, rhs);
     if (G.accelOn) {
       double r[8];
       load_rec(d.rec, i, r);
@@ -1679,11 +1693,12 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
 }
 
 // ConvectiveFluxesMP / SourceTermsMP as separate steps (rhs kept in HBM).
+template <int F32>
 __global__ void __launch_bounds__(BLK) k_gather(Dev d) {
   int i = xcd_block() * BLK + threadIdx.x;
   if (i >= d.nC) return;
   double rhs[5];
-  gather_rhs(d, i, rhs);
+  gather_rhs<F32>(d, i, rhs);
 #pragma unroll
   for (int k = 0; k < 5; k++) d.rhs[k * d.nC + i] = rhs[k];
 }
@@ -1882,7 +1897,10 @@ UpdPar updpar(const rflu_handle *h, int istage, int fused) {
 
 int launch_update(rflu_handle *h, int istage, int fused) {
   ProfScope ps(h, "k_update");
-  hipLaunchKernelGGL(k_update, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, updpar(h, istage, fused));
+  if (h->prm.fluxPrecision == 1)
+    hipLaunchKernelGGL((k_update<1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, updpar(h, istage, fused));
+  else
+    hipLaunchKernelGGL((k_update<0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, updpar(h, istage, fused));
   return 0;
 }
 
@@ -1903,7 +1921,7 @@ void enqueue_stage(rflu_handle *h, int st, bool withDt) {
     if (withDt) launch_dtmin(h, 1, 0);
     {
       ProfScope ps(h, "k_gather");
-      hipLaunchKernelGGL(k_gather, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
+      hipLaunchKernelGGL((k_gather<0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
     }
     if (h->gas.accelOn) hipLaunchKernelGGL(k_source, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, h->prm.spaceOrder);
     picl_stage(h, st);
@@ -2162,7 +2180,10 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
     rc |= dalloc(h, &d.grad, 15 * (size_t)N);
     if (prm->fluxPrecision == 1) {
       if (!h->tileLds) rc |= set_err(h, RFLU_GPU_EINVAL, "fluxPrecision 1 needs gradient tiles that fit in LDS");
-      else rc |= dalloc(h, &d.gradF, 16 * (size_t)N);
+      else {
+        rc |= dalloc(h, &d.gradF, 16 * (size_t)N);
+        rc |= dalloc(h, &d.fluxF, 5 * (size_t)m.nFaces);
+      }
     }
     if (h->tileLds) {
       rc |= dalloc(h, &d.tOff, tOff.size(), tOff.data());
@@ -2311,7 +2332,7 @@ int rflu_gpu_destroy(rflu_handle *h) {
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);
   for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot,
-                  (void *)d.gradF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal})
+                  (void *)d.gradF, (void *)d.fluxF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal})
     if (p) hipFree(p);
   picl_free(h->picl);
   if (h->graph) hipGraphExecDestroy(h->graph);
@@ -2461,7 +2482,10 @@ int rflu_gpu_convective_fluxes(rflu_handle *h) {
   launch_flux(h, 0);
   {
     ProfScope ps(h, "k_gather");
-    hipLaunchKernelGGL(k_gather, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
+    if (h->prm.fluxPrecision == 1)
+      hipLaunchKernelGGL((k_gather<1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
+    else
+      hipLaunchKernelGGL((k_gather<0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
   }
   HIPCHK(h, hipGetLastError());
   return 0;
