@@ -76,6 +76,7 @@ struct PiclDev {
   double *gq = nullptr;     // [nP][8] by sorted position: x, y, z, map * multfci (5)
   double *xo = nullptr;     // [3][nP] position at the start of the stage, by current slot
   double *vproj = nullptr;  // [5][nV]
+  int *vperm = nullptr;     // vertices in Morton order (the gather's thread order)
   double gRdum = 0, gMult = 0, gD2chk = 0, gReach = 0, gReach2 = 0;
 };
 
@@ -193,6 +194,54 @@ __global__ void __launch_bounds__(BLK) k_rhog(Dev d, PiclDev P) {
     double w0 = 1.0 / (t0 * t0 + 1.0E-15), w1 = 1.0 / (t1 * t1 + 1.0E-15), w2 = 1.0 / (t2 * t2 + 1.0E-15);
     s0 = s0 + w0; s1 = s1 + w1; s2 = s2 + w2;
     l0 = l0 + w0 * t0; l1 = l1 + w1 * t1; l2 = l2 + w2 * t2;
+  }
+  double g[3] = {l0 / s0, l1 / s1, l2 / s2};
+  int N = d.nC;
+  double varc = d.rhog[i];
+  double xc = d.cofg[i], yc = d.cofg[N + i], zc = d.cofg[2 * N + i];
+  bool zero = false;
+  for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+    int slot = d.cfIdx[e] >> 1;
+    double dx = d.fcx[slot] - xc, dy = d.fcy[slot] - yc, dz = d.fcz[slot] - zc;
+    double varf = varc + g[0] * dx + g[1] * dy + g[2] * dz;
+    if (varf <= 0.0) zero = true;
+  }
+  if (zero) g[0] = g[1] = g[2] = 0.0;
+  double2 *o = reinterpret_cast<double2 *>(P.gRho + 4 * (size_t)i);
+  o[0] = make_double2(g[0], g[1]);
+  o[1] = make_double2(g[2], 0.0);
+}
+
+// The same WENO-XYZ + positivity limiter over the gradient tiles: the tile's
+// slots (its cells and their stencil members) carry w = 1/(g^2 + 1e-15) and
+// w*g per component, formed once per slot in LDS instead of once per
+// stencil appearance; each cell then adds them in stencil order.  Same
+// operations on the same values, same bits as k_rhog.
+__global__ void __launch_bounds__(BLK) k_rhog_tile(Dev d, PiclDev P) {
+  extern __shared__ double2 sW[];  // [nU][3]: (w0, w0*g0), (w1, w1*g1), (w2, w2*g2)
+  int tile = xcd_block();
+  int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
+  const double2 *gr = reinterpret_cast<const double2 *>(d.gRaw);
+  for (int u = threadIdx.x; u < nU; u += BLK) {
+    size_t c = d.tMem[u0 + u];
+    double2 a = gr[2 * c], b = gr[2 * c + 1];
+    double t0 = a.x, t1 = a.y, t2 = b.x;
+    double w0 = 1.0 / (t0 * t0 + 1.0E-15), w1 = 1.0 / (t1 * t1 + 1.0E-15), w2 = 1.0 / (t2 * t2 + 1.0E-15);
+    sW[3 * u] = make_double2(w0, w0 * t0);
+    sW[3 * u + 1] = make_double2(w1, w1 * t1);
+    sW[3 * u + 2] = make_double2(w2, w2 * t2);
+  }
+  __syncthreads();
+  int i = tile * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  double2 o0 = sW[3 * threadIdx.x], o1 = sW[3 * threadIdx.x + 1], o2 = sW[3 * threadIdx.x + 2];
+  double s0 = o0.x, s1 = o1.x, s2 = o2.x, l0 = o0.y, l1 = o1.y, l2 = o2.y;
+  int sb = d.csOff[i], se = d.csOff[i + 1];
+  for (int s = sb; s < se; s++) {
+    int u = d.sSlot[s] & 0x7fffu;
+    double2 a = sW[3 * u], b = sW[3 * u + 1], c = sW[3 * u + 2];
+    s0 = s0 + a.x; s1 = s1 + b.x; s2 = s2 + c.x;
+    l0 = l0 + a.y; l1 = l1 + b.y; l2 = l2 + c.y;
   }
   double g[3] = {l0 / s0, l1 / s1, l2 / s2};
   int N = d.nC;
@@ -482,7 +531,7 @@ __device__ __forceinline__ PBuf cur_buf(const PiclDev &P) { return P.par[0] ? P.
 // TRI: trilinear interpolation instead of IDW-27; QS: the quasi-steady law
 // (ppiclf_user.f:473-485 numbering, 5 = Schiller-Naumann)
 template <int TRI, int QS>
-__global__ void __launch_bounds__(BLK) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
+__global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
   const size_t nP = P.nP;
@@ -915,51 +964,79 @@ __global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
   if (n) atomicAdd(&P.binCnt[P.ecb[e]], n);
 }
 
+// GAUSS_L consecutive lanes share a vertex and split each candidate
+// element's particles; their compensated partial sums are combined pairwise
+// in lane order (error-free TwoSum), so the result does not depend on the
+// schedule.
+#define GAUSS_L 8
 __global__ void __launch_bounds__(BLK) k_gauss(PiclDev P) {
-  int v = blockIdx.x * BLK + threadIdx.x;
-  if (v >= P.nV) return;
+  // vertex order: each XCD walks a contiguous range, so the particles a
+  // workgroup reads stay in its XCD's L2
+  int t = xcd_block() * BLK + threadIdx.x;
+  int vt = t / GAUSS_L, sub = t % GAUSS_L;
+  bool live = vt < P.nV;
+  int v = live ? P.vperm[vt] : 0;
   const size_t nV = P.nV;
-  double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
-  int blo[3], bhi[3];
-#pragma unroll
-  for (int l = 0; l < 3; l++) {
-    double top = (double)(P.cbN[l] - 1);
-    double a = (xv[l] - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xv[l] + P.gReach - P.cbLo[l]) / P.cbH[l];
-    blo[l] = (int)fmin(fmax(a, 0.0), top);
-    bhi[l] = (int)fmin(fmax(b, 0.0), top);
-  }
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, cmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int bz = blo[2]; bz <= bhi[2]; bz++)
-    for (int by = blo[1]; by <= bhi[1]; by++)
-      for (int bx = blo[0]; bx <= bhi[0]; bx++) {
-        int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
-        if (P.binCnt[bin] == 0) continue;
-        for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
-          const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)q);
-          double2 c01 = cp[0], c23 = cp[1];
-          double t0 = c01.x - xv[0], t1 = c01.y - xv[1], t2 = c23.x - xv[2];
-          if (t0 * t0 + t1 * t1 + t2 * t2 > P.gReach2) continue;
-          int e = (int)(unsigned)(__double_as_longlong(c23.y) & 0xffffffffll);
-          int p1 = P.off[e + 1];
-          for (int pos = P.off[e]; pos < p1; pos++) {
-            const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
-            double2 g0 = g[0], g1 = g[1];
-            double dx = xv[0] - g0.x, dy = xv[1] - g0.y, dz = xv[2] - g1.x;
-            double rdist2 = dx * dx + dy * dy;
-            rdist2 = rdist2 + dz * dz;
-            if (rdist2 > P.gD2chk) continue;
-            double rexp = exp(rdist2 * P.gRdum);
-            double2 g2 = g[2], g3 = g[3];
-            neumaier(acc[0], cmp[0], g1.y * rexp);
-            neumaier(acc[1], cmp[1], g2.x * rexp);
-            neumaier(acc[2], cmp[2], g2.y * rexp);
-            neumaier(acc[3], cmp[3], g3.x * rexp);
-            neumaier(acc[4], cmp[4], g3.y * rexp);
+  if (live) {
+    double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
+    int blo[3], bhi[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      double top = (double)(P.cbN[l] - 1);
+      double a = (xv[l] - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xv[l] + P.gReach - P.cbLo[l]) / P.cbH[l];
+      blo[l] = (int)fmin(fmax(a, 0.0), top);
+      bhi[l] = (int)fmin(fmax(b, 0.0), top);
+    }
+    for (int bz = blo[2]; bz <= bhi[2]; bz++)
+      for (int by = blo[1]; by <= bhi[1]; by++)
+        for (int bx = blo[0]; bx <= bhi[0]; bx++) {
+          int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
+          if (P.binCnt[bin] == 0) continue;
+          for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
+            const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)q);
+            double2 c01 = cp[0], c23 = cp[1];
+            double t0 = c01.x - xv[0], t1 = c01.y - xv[1], t2 = c23.x - xv[2];
+            if (t0 * t0 + t1 * t1 + t2 * t2 > P.gReach2) continue;
+            int e = (int)(unsigned)(__double_as_longlong(c23.y) & 0xffffffffll);
+            int p1 = P.off[e + 1];
+            for (int pos = P.off[e] + sub; pos < p1; pos += GAUSS_L) {
+              const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
+              double2 g0 = g[0], g1 = g[1];
+              double dx = xv[0] - g0.x, dy = xv[1] - g0.y, dz = xv[2] - g1.x;
+              double rdist2 = dx * dx + dy * dy;
+              rdist2 = rdist2 + dz * dz;
+              if (rdist2 > P.gD2chk) continue;
+              double rexp = exp(rdist2 * P.gRdum);
+              double2 g2 = g[2], g3 = g[3];
+              neumaier(acc[0], cmp[0], g1.y * rexp);
+              neumaier(acc[1], cmp[1], g2.x * rexp);
+              neumaier(acc[2], cmp[2], g2.y * rexp);
+              neumaier(acc[3], cmp[3], g3.x * rexp);
+              neumaier(acc[4], cmp[4], g3.y * rexp);
+            }
           }
         }
-      }
+  }
+  // combine the lanes' (sum, compensation) pairs, lower lane first
 #pragma unroll
-  for (int k = 0; k < 5; k++) P.vproj[k * nV + v] = 0.0 + (acc[k] + cmp[k]);
+  for (int off = 1; off < GAUSS_L; off <<= 1) {
+    bool upper = (sub & off) != 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      double so = __shfl_xor(acc[k], off, 64), co = __shfl_xor(cmp[k], off, 64);
+      double a = upper ? so : acc[k], b = upper ? acc[k] : so;
+      double ca = upper ? co : cmp[k], cb = upper ? cmp[k] : co;
+      double s2 = a + b, bb = s2 - a;
+      double e = (a - (s2 - bb)) + (b - bb);
+      acc[k] = s2;
+      cmp[k] = (ca + cb) + e;
+    }
+  }
+  if (live && sub == 0) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) P.vproj[k * nV + v] = 0.0 + (acc[k] + cmp[k]);
+  }
 }
 
 // Every particle to its new position in the other copy; this stage's ydotc
@@ -1080,7 +1157,7 @@ void picl_free(PiclState *s) {
   void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
-                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj};
+                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1111,7 +1188,7 @@ void picl_sort_and_deposit(rflu_handle *h, int posXo) {
     ProfScope ps(h, "k_gauss");
     hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nCB)), dim3(BLK), 0, h->stream, P.binCnt, P.nCB);
     hipLaunchKernelGGL(k_gcount, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
-    hipLaunchKernelGGL(k_gauss, dim3(nblk(P.nV)), dim3(BLK), 0, h->stream, P);
+    hipLaunchKernelGGL(k_gauss, dim3(nblk(P.nV * GAUSS_L)), dim3(BLK), 0, h->stream, P);
   }
   ProfScope ps(h, "k_permute");
   hipLaunchKernelGGL(k_permute, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
@@ -1120,7 +1197,11 @@ void picl_sort_and_deposit(rflu_handle *h, int posXo) {
 
 int launch_rhog(rflu_handle *h) {
   ProfScope ps(h, "k_rhog");
-  hipLaunchKernelGGL(k_rhog, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->picl->P);
+  size_t lds = h->tileLds / 64 * 48;
+  if (h->gradVariant == 4 && h->tileLds && lds <= h->ldsMax)
+    hipLaunchKernelGGL(k_rhog_tile, dim3(h->d.nBlk), dim3(BLK), lds, h->stream, h->d, h->picl->P);
+  else
+    hipLaunchKernelGGL(k_rhog, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->picl->P);
   return 0;
 }
 
@@ -1368,6 +1449,9 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.gq, 8 * (size_t)nP);
     rc |= dalloc(h, &P.xo, 3 * (size_t)nP);
     rc |= dalloc(h, &P.vproj, 5 * (size_t)P.nV);
+    std::vector<int> vperm(P.nV);
+    for (int v = 0; v < P.nV; v++) vperm[v] = v;
+    rc |= dalloc(h, &P.vperm, vperm.size(), vperm.data());
   }
   rc |= dalloc(h, &h->d.phi, (size_t)N);
   rc |= dalloc(h, &h->d.rhog, (size_t)N);
