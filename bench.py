@@ -176,6 +176,7 @@ def run_batch(args):
         b = rflu.Batch([it[0] for it in items], **dict(items[0][1], fluxPrecision=prec))
         for k, (mesh, params, ic) in enumerate(items):
             b.set_state(k, ic(b.cofg(k)))
+        b.autotune(reps=3)
         batches.append((key, b, items))
     cells = sum(b.nCells for _, b, _ in batches)
     setup_s = time.perf_counter() - t0

@@ -172,6 +172,12 @@ int rflu_batch_set_state(rflu_handle *h, int inst, const double *cv);
 int rflu_batch_get_state(rflu_handle *h, int inst, double *cv, double *dv);
 int rflu_batch_get_time(rflu_handle *h, int inst, double *currentTime, double *dtMin);
 
+/* Autotuner (no reference counterpart): times every gradient kernel that
+ * gives the same bits on this handle's own mesh (reps launches each) and
+ * keeps the fastest for later stages; the captured step graph is rebuilt.
+ * Reports the choice and its time per launch. */
+int rflu_gpu_autotune(rflu_handle *h, int reps, int *gradVariant, double *msPerLaunch);
+
 /* Mean kernel duration (ms) per launch of the named kernel over the last
  * rflu_gpu_step call with profiling enabled; for measurement only. */
 int rflu_gpu_set_profiling(rflu_handle *h, int on);

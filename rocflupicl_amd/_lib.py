@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librflu_gpu.so")
+# ROCFLUPICL_LIB selects another build of the same library (e.g. a tuning
+# variant); the default is the in-tree build.
+LIB_PATH = os.environ.get("ROCFLUPICL_LIB", os.path.join(HERE, "librflu_gpu.so"))
 
 # Every symbol include/rflu_gpu.h declares (checked by the CPU test suite).
 RFLU_SYMBOLS = [
@@ -21,7 +23,7 @@ RFLU_SYMBOLS = [
     "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time", "rflu_gpu_bandwidth_probe",
     "rflu_gpu_step_async", "rflu_gpu_query", "rflu_gpu_wait", "rflu_gpu_device_bytes", "rflu_gpu_mem_info",
     "rflu_batch_create", "rflu_batch_sizes", "rflu_batch_set_state", "rflu_batch_get_state",
-    "rflu_batch_get_time",
+    "rflu_batch_get_time", "rflu_gpu_autotune",
 ]
 
 # Every symbol include/picl_gpu.h declares.
@@ -103,6 +105,7 @@ def load():
     L.rflu_batch_set_state.argtypes = [P, I, P]
     L.rflu_batch_get_state.argtypes = [P, I, P, P]
     L.rflu_batch_get_time.argtypes = [P, I, P, P]
+    L.rflu_gpu_autotune.argtypes = [P, I, P, P]
     L.picl_gpu_init.argtypes = [P, C.POINTER(PiclParams), I, P, P]
     L.picl_gpu_runge.argtypes = [P, I]
     L.picl_gpu_sizes.argtypes = [P, P, P]

@@ -2618,6 +2618,48 @@ int rflu_gpu_bandwidth_probe(size_t bytes, int reps, double *gbps) {
   return 0;
 }
 
+int rflu_gpu_autotune(rflu_handle *h, int reps, int *variant, double *msPerLaunch) {
+  if (!h || reps < 1) return RFLU_GPU_EINVAL;
+  if (variant) *variant = h->gradVariant;
+  if (msPerLaunch) *msPerLaunch = 0.0;
+  if (h->prm.spaceOrder < 2 || h->prm.fluxPrecision == 1) return 0;  // a single gradient kernel
+  // gradient kernels that give the same bits: LDS tiles (4), one thread per
+  // cell at 1, 4, 5, 6 waves per SIMD (2, 24, 25, 26), cooperative (3);
+  // with particles the tile kernel (4) or the per-cell one (2)
+  std::vector<int> cand;
+  if (h->tileLds) cand.push_back(4);
+  cand.push_back(2);
+  if (!h->picl) { cand.push_back(24); cand.push_back(25); cand.push_back(26); cand.push_back(3); }
+  bool prof = h->profiling;
+  h->profiling = false;
+  hipEvent_t a, b;
+  HIPCHK(h, hipEventCreate(&a));
+  HIPCHK(h, hipEventCreate(&b));
+  int best = h->gradVariant;
+  float bestMs = 1e30f;
+  for (int c : cand) {
+    h->gradVariant = c;
+    launch_grad(h);  // warm-up (and first-touch of the code object)
+    HIPCHK(h, hipEventRecord(a, h->stream));
+    for (int r = 0; r < reps; r++) launch_grad(h);
+    HIPCHK(h, hipEventRecord(b, h->stream));
+    HIPCHK(h, hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(h, hipEventElapsedTime(&ms, a, b));
+    ms /= (float)reps;
+    if (ms < bestMs) { bestMs = ms; best = c; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  HIPCHK(h, hipGetLastError());
+  h->gradVariant = best;
+  h->profiling = prof;
+  if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }  // re-captured with the choice
+  if (variant) *variant = best;
+  if (msPerLaunch) *msPerLaunch = bestMs;
+  return 0;
+}
+
 int rflu_gpu_set_profiling(rflu_handle *h, int on) {
   if (!h) return RFLU_GPU_EINVAL;
   h->profiling = on != 0;

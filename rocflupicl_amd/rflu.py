@@ -260,6 +260,13 @@ class Region:
         """Block until the stream drains; raise on AfterUpdateMP's checks."""
         self._chk(self._L.rflu_gpu_wait(self.h))
 
+    def autotune(self, reps=5):
+        """Pick the fastest of the bit-identical gradient kernels on this
+        mesh; returns (variant, ms per launch)."""
+        v, ms = C.c_int(), C.c_double()
+        self._chk(self._L.rflu_gpu_autotune(self.h, int(reps), C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
     def device_bytes(self):
         b = C.c_size_t()
         self._chk(self._L.rflu_gpu_device_bytes(self.h, C.byref(b)))

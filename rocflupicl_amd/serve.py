@@ -10,7 +10,8 @@ state, optionally a particle cloud.  The server
   ``bytes_per_particle`` x particles);
 * sets admitted requests up on builder threads (the host mesh pipeline and
   the upload run in C++ with the GIL released), so set-up overlaps with the
-  device work of the requests already running;
+  device work of the requests already running, and lets the autotuner pick
+  the gradient kernel for each request's mesh (``tune``);
 * advances every request in flight by ``chunk`` steps at a time with
   asynchronous graph replays on its own non-blocking stream, so the work of
   different requests overlaps on the device; a request whose stream is busy
@@ -62,7 +63,7 @@ class Result:
 
 class Server:
     def __init__(self, max_active=8, chunk=10, builders=4, reserve_frac=0.05, bytes_per_cell=3000,
-                 bytes_per_particle=800, keep_state=False, on_done=None):
+                 bytes_per_particle=800, keep_state=False, on_done=None, tune=True):
         self.max_active = max_active
         self.chunk = chunk
         self.builders = builders
@@ -71,18 +72,20 @@ class Server:
         self.bytes_per_particle = bytes_per_particle
         self.keep_state = keep_state
         self.on_done = on_done
+        self.tune = tune
 
     def _estimate(self, req):
         return self.bytes_per_cell * req.cells + self.bytes_per_particle * req.particles
 
-    @staticmethod
-    def _build(req):
+    def _build(self, req):
         mesh, params, ic, parts, picl = req.make()
         reg = rflu.Region(mesh, **params)
         reg.set_state(ic(reg.mesh_arrays()["cofg"]))
         if parts is not None:
             y, pc = parts
             reg.picl_init(y, pc, **(picl or {}))
+        if self.tune:
+            reg.autotune(reps=2)
         return reg
 
     def run(self, requests):

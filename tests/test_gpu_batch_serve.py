@@ -94,3 +94,15 @@ def test_async_stepping_matches_blocking_steps():
     assert r.device_bytes() > 0
     free, total = rflu.mem_info()
     assert 0 < free <= total
+
+
+def test_autotune_keeps_the_bits():
+    mesh, params, ic = configs.shock_tets(n=10, discr=1, limiterGuard=1)
+    a = alone(mesh, params, ic, 6)
+    r = rflu.Region(mesh, **params)
+    r.set_state(ic(r.mesh_arrays()["cofg"]))
+    r.RFLU_TimeStepping(2, use_graph=True)
+    v, ms = r.autotune(reps=3)
+    assert v in (2, 3, 4, 24, 25, 26) and ms > 0.0
+    r.RFLU_TimeStepping(4, use_graph=True)
+    assert same_bits(r.get_state()[0], a.get_state()[0])
