@@ -142,6 +142,9 @@ int rflu_gpu_rk_stage(rflu_handle *h, int istage);
  * time advance); useGraph = 1 replays one captured hipGraph per step. */
 int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph);
 int rflu_gpu_get_time(rflu_handle *h, double *currentTime, double *dtMin);
+/* global%currentTime as a restart sets it from the flow file
+ * (modflu/RFLU_ModReadWriteFlow.F90:1219-1224); time-since-restart is reset. */
+int rflu_gpu_set_time(rflu_handle *h, double currentTime);
 int rflu_gpu_sync(rflu_handle *h);
 
 /* Asynchronous stepping for hosts that drive several regions at once (no

@@ -23,7 +23,7 @@ RFLU_SYMBOLS = [
     "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time", "rflu_gpu_bandwidth_probe",
     "rflu_gpu_step_async", "rflu_gpu_query", "rflu_gpu_wait", "rflu_gpu_device_bytes", "rflu_gpu_mem_info",
     "rflu_batch_create", "rflu_batch_sizes", "rflu_batch_set_state", "rflu_batch_get_state",
-    "rflu_batch_get_time", "rflu_gpu_autotune",
+    "rflu_batch_get_time", "rflu_gpu_autotune", "rflu_gpu_set_time",
 ]
 
 # Every symbol include/picl_gpu.h declares.
@@ -106,6 +106,7 @@ def load():
     L.rflu_batch_get_state.argtypes = [P, I, P, P]
     L.rflu_batch_get_time.argtypes = [P, I, P, P]
     L.rflu_gpu_autotune.argtypes = [P, I, P, P]
+    L.rflu_gpu_set_time.argtypes = [P, C.c_double]
     L.picl_gpu_init.argtypes = [P, C.POINTER(PiclParams), I, P, P]
     L.picl_gpu_runge.argtypes = [P, I]
     L.picl_gpu_sizes.argtypes = [P, P, P]

@@ -2574,6 +2574,18 @@ int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph) {
   return rc;
 }
 
+int rflu_gpu_set_time(rflu_handle *h, double t) {
+  if (!h) return RFLU_GPU_EINVAL;
+  if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "set the time of a batch per instance (not supported)");
+  double s[8];
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(s, h->d.scal, sizeof s, hipMemcpyDeviceToHost));
+  s[S_TIME] = t;
+  s[S_TSR] = 0.0;
+  HIPCHK(h, hipMemcpy(h->d.scal, s, sizeof s, hipMemcpyHostToDevice));
+  return 0;
+}
+
 int rflu_gpu_get_time(rflu_handle *h, double *t, double *dtMin) {
   if (!h) return RFLU_GPU_EINVAL;
   double s[8];

@@ -1,0 +1,216 @@
+"""Rocflu flow and restart-info files (modflu/RFLU_ModReadWriteFlow.F90,
+libflu/RFLU_WriteRestartInfo.F90), so that GPU runs can start from a
+reference solution file and write files the reference (and its
+post-processors) read back.
+
+* ASCII flow file ``<case>.mixt.cva_<region:05d>_<time:1PE11.5>``
+  (RFLU_WriteFlowASCII, :2187-2367; read by RFLU_ReadFlowASCII, :1078-1301):
+  section lines ``# ROCFLU flow file (v2)``, ``# Precision and range``
+  (2I8), ``# Initial residual`` (E23.16), ``# Physical time`` (E23.16),
+  ``# Dimensions`` (2I16: cells, variables), then per conserved variable
+  ``# <I2>`` and its values five per line in E23.16, and ``# End``.
+* Binary flow file ``<case>.mixt.cv_...`` (RFLU_WriteFlowBinary,
+  :2392-2559): the same sections as Fortran unformatted sequential records
+  (4-byte length markers around each record; section strings are
+  CHARACTER(80), CHRLEN in modfloflu/ModDataTypes.F90:73; integers 4 bytes).
+  Little-endian, as the reference's FORMAT_BINARY and FORMAT_BINARY_L write
+  on x86-64.
+* Restart info ``<case>.rin``: one line per written solution, the physical
+  time, list-directed (RFLU_WriteRestartInfo.F90), appended.
+
+The E23.16 text form keeps 16 significant digits, so an ASCII round trip may
+change the last bit of a value; the binary form is exact.  Cells are in the
+region's (reference) numbering; cv is (nCells, nCv) with the conserved
+variables rho, rho*u, rho*v, rho*w, rho*E.
+"""
+import os
+import struct
+
+import numpy as np
+
+CHRLEN = 80
+PRECISION, RANGE = 15, 307   # PRECISION(1.0_RFREAL), RANGE(1.0_RFREAL) for real*8
+
+
+def file_name(case, ext, region, time, directory=""):
+    """BuildFileNameUnsteady (libfloflu/BuildFileNameUnsteady.F90):
+    dir + case + ext + '_' + I5.5 + '_' + 1PE11.5."""
+    return os.path.join(directory, f"{case}{ext}_{region:05d}_{fortran_1pe(time, 11, 5)}")
+
+
+def fortran_1pe(x, w, d):
+    """Fortran 1PEw.d: one digit before the point, d after, two-digit exponent."""
+    s = f"{x:.{d}E}"
+    mant, exp = s.split("E")
+    e = int(exp)
+    out = f"{mant}E{'+' if e >= 0 else '-'}{abs(e):02d}"
+    return out.rjust(w)
+
+
+def fortran_e23_16(x):
+    """Fortran E23.16: 0.dddddddddddddddd E+xx (a 3-digit exponent drops the
+    'E', as gfortran prints it)."""
+    x = float(x)
+    if x == 0.0:
+        return " 0.0000000000000000E+00"
+    s = f"{x:.15e}"                        # d.ddddddddddddddde+XX, 16 significant digits
+    mant, exp = s.split("e")
+    sign = "-" if mant.startswith("-") else ""
+    digits = mant.lstrip("-").replace(".", "")
+    e = int(exp) + 1
+    es = f"E{'+' if e >= 0 else '-'}{abs(e):02d}" if abs(e) < 100 else f"{'+' if e >= 0 else '-'}{abs(e):03d}"
+    return f"{sign}0.{digits}{es}".rjust(23)
+
+
+def _parse_e(tok):
+    tok = tok.strip()
+    if "E" not in tok.upper():  # 0.1234+100 form
+        for k in range(len(tok) - 1, 0, -1):
+            if tok[k] in "+-":
+                tok = tok[:k] + "E" + tok[k:]
+                break
+    return float(tok)
+
+
+def write_flow_ascii(path, cv, time, res_init=0.0):
+    cv = np.asarray(cv, np.float64)
+    n, nv = cv.shape
+    lines = ["# ROCFLU flow file (v2)", "# Precision and range", f"{PRECISION:8d}{RANGE:8d}",
+             "# Initial residual", fortran_e23_16(res_init), "# Physical time", fortran_e23_16(time),
+             "# Dimensions", f"{n:16d}{nv:16d}"]
+    for k in range(nv):
+        lines.append(f"# {k + 1:2d}")
+        col = cv[:, k]
+        for a in range(0, n, 5):
+            lines.append("".join(fortran_e23_16(x) for x in col[a:a + 5]))
+    lines.append("# End")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_flow_ascii(path):
+    """RFLU_ReadFlowASCII: returns dict(time, res_init, cv (nCells, nCv))."""
+    with open(path) as f:
+        lines = [ln.rstrip("\n") for ln in f]
+    it = iter(lines)
+    out = {}
+    head = next(it)
+    if not head.startswith("# ROCFLU flow file"):
+        raise ValueError(f"{path}: not a Rocflu flow file")
+    cv = None
+    for ln in it:
+        s = ln.strip()
+        if s == "# Precision and range":
+            prec, rng = (int(t) for t in next(it).split())
+            if prec < PRECISION or rng < RANGE:
+                raise ValueError(f"{path}: precision/range {prec}/{rng} below real*8")
+        elif s == "# Initial residual":
+            out["res_init"] = _parse_e(next(it))
+        elif s == "# Physical time":
+            out["time"] = _parse_e(next(it))
+        elif s == "# Dimensions":
+            n, nv = (int(t) for t in next(it).split())
+            cv = np.zeros((n, nv))
+        elif s == "# End":
+            break
+        elif s.startswith("#"):
+            k = int(s[1:]) - 1
+            vals = []
+            while len(vals) < n:
+                row = next(it)
+                vals.extend(_parse_e(row[a:a + 23]) for a in range(0, len(row), 23) if row[a:a + 23].strip())
+            cv[:, k] = vals[:n]
+    out["cv"] = cv
+    return out
+
+
+def _rec(f, payload):
+    f.write(struct.pack("<i", len(payload)))
+    f.write(payload)
+    f.write(struct.pack("<i", len(payload)))
+
+
+def _str(s):
+    return s.ljust(CHRLEN).encode("ascii")
+
+
+def write_flow_binary(path, cv, time, res_init=0.0):
+    cv = np.asarray(cv, np.float64)
+    n, nv = cv.shape
+    with open(path, "wb") as f:
+        _rec(f, _str("# ROCFLU flow file (v2)"))
+        _rec(f, _str("# Precision and range"))
+        _rec(f, struct.pack("<ii", PRECISION, RANGE))
+        _rec(f, _str("# Initial residual"))
+        _rec(f, struct.pack("<d", res_init))
+        _rec(f, _str("# Physical time"))
+        _rec(f, struct.pack("<d", time))
+        _rec(f, _str("# Dimensions"))
+        _rec(f, struct.pack("<ii", n, nv))
+        for k in range(nv):
+            _rec(f, _str(f"# {k + 1:2d}"))
+            _rec(f, np.ascontiguousarray(cv[:, k], "<f8").tobytes())
+        _rec(f, _str("# End"))
+
+
+def read_flow_binary(path):
+    """RFLU_ReadFlowBinary: returns dict(time, res_init, cv (nCells, nCv))."""
+    data = open(path, "rb").read()
+    pos, recs = 0, []
+    while pos < len(data):
+        (ln,) = struct.unpack_from("<i", data, pos)
+        recs.append(data[pos + 4:pos + 4 + ln])
+        (ln2,) = struct.unpack_from("<i", data, pos + 4 + ln)
+        if ln2 != ln:
+            raise ValueError(f"{path}: broken record markers at byte {pos}")
+        pos += 8 + ln
+    it = iter(recs)
+    if not next(it).decode("ascii").startswith("# ROCFLU flow file"):
+        raise ValueError(f"{path}: not a Rocflu flow file")
+    out, cv, n = {}, None, 0
+    for r in it:
+        s = r.decode("ascii", errors="replace").strip()
+        if s == "# Precision and range":
+            next(it)
+        elif s == "# Initial residual":
+            (out["res_init"],) = struct.unpack("<d", next(it))
+        elif s == "# Physical time":
+            (out["time"],) = struct.unpack("<d", next(it))
+        elif s == "# Dimensions":
+            n, nv = struct.unpack("<ii", next(it))
+            cv = np.zeros((n, nv))
+        elif s == "# End":
+            break
+        elif s.startswith("#"):
+            k = int(s[1:]) - 1
+            cv[:, k] = np.frombuffer(next(it), "<f8", count=n)
+    out["cv"] = cv
+    return out
+
+
+def fortran_list_real8(x):
+    """gfortran list-directed REAL(8): F form with 17 significant digits for
+    0.1 <= |x| < 1e16, else E form with a three-digit exponent."""
+    x = float(x)
+    ax = abs(x)
+    if ax == 0.0:
+        return "   0.0000000000000000     "
+    if 0.1 <= ax < 1e16:
+        ip = len(str(int(ax))) if ax >= 1 else 0
+        return f"   {x:.{max(17 - ip, 0)}f}"
+    s = f"{x:.16e}"
+    mant, exp = s.split("e")
+    e = int(exp)
+    return f"   {mant}E{'+' if e >= 0 else '-'}{abs(e):03d}"
+
+
+def append_restart_info(case, time, directory=""):
+    """RFLU_WriteRestartInfo: append the physical time to <case>.rin."""
+    with open(os.path.join(directory, f"{case}.rin"), "a") as f:
+        f.write(fortran_list_real8(time) + "\n")
+
+
+def read_restart_info(case, directory=""):
+    """Times listed in <case>.rin (the last one is where a run resumes)."""
+    with open(os.path.join(directory, f"{case}.rin")) as f:
+        return [float(t) for t in f.read().split()]

@@ -242,6 +242,30 @@ class Region:
         self._chk(self._L.rflu_gpu_get_time(self.h, C.byref(t), C.byref(dt)))
         return t.value, dt.value
 
+    def set_time(self, t):
+        self._chk(self._L.rflu_gpu_set_time(self.h, float(t)))
+
+    # ---- flow files (RFLU_ModReadWriteFlow) ------------------------------
+    def write_flow(self, case, directory="", region=1, binary=True):
+        """RFLU_WriteFlowBinary / RFLU_WriteFlowASCII of the current state and
+        RFLU_WriteRestartInfo; returns the file name."""
+        from . import flowio
+        t = self.time()[0]
+        name = flowio.file_name(case, ".mixt.cv" if binary else ".mixt.cva", region, t, directory)
+        (flowio.write_flow_binary if binary else flowio.write_flow_ascii)(name, self.get_state()[0], t)
+        flowio.append_restart_info(case, t, directory)
+        return name
+
+    def read_flow(self, path):
+        """RFLU_ReadFlowBinary / ASCII: state and physical time from a flow file."""
+        from . import flowio
+        with open(path, "rb") as f:
+            binary = f.read(4) == (80).to_bytes(4, "little")
+        d = (flowio.read_flow_binary if binary else flowio.read_flow_ascii)(path)
+        self.set_state(d["cv"])
+        self.set_time(d["time"])
+        return d["time"]
+
     def sync(self):
         self._chk(self._L.rflu_gpu_sync(self.h))
 

@@ -1,0 +1,52 @@
+"""Rocflu flow-file and restart-info formats (rocflupicl_amd/flowio.py) on
+the CPU: Fortran edit descriptors against hand-worked strings, the section
+layout of RFLU_WriteFlowASCII / RFLU_WriteFlowBinary, and round trips."""
+import struct
+
+import numpy as np
+
+from rocflupicl_amd import flowio
+
+
+def test_fortran_edit_descriptors():
+    assert flowio.fortran_e23_16(1.0) == " 0.1000000000000000E+01"
+    assert flowio.fortran_e23_16(-2.5e-7) == "-0.2500000000000000E-06"
+    assert flowio.fortran_e23_16(123456.789) == " 0.1234567890000000E+06"
+    assert flowio.fortran_e23_16(0.0) == " 0.0000000000000000E+00"
+    assert flowio.fortran_e23_16(1e-300) == " 0.1000000000000000-299"
+    assert all(len(flowio.fortran_e23_16(x)) == 23 for x in (1.0, -1e200, 3e-5))
+    assert flowio.file_name("rectshktb", ".mixt.cva", 1, 1.5e-3) == "rectshktb.mixt.cva_00001_1.50000E-03"
+
+
+def test_ascii_layout_and_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    cv = rng.standard_normal((7, 5)) * np.array([1.0, 300.0, 300.0, 300.0, 2.5e5])
+    p = tmp_path / "case.mixt.cva_00001_1.00000E-03"
+    flowio.write_flow_ascii(p, cv, 1.0e-3, res_init=0.0)
+    lines = p.read_text().splitlines()
+    assert lines[:9] == ["# ROCFLU flow file (v2)", "# Precision and range", "      15     307",
+                         "# Initial residual", " 0.0000000000000000E+00", "# Physical time",
+                         " 0.1000000000000000E-02", "# Dimensions", "               7               5"]
+    assert lines[9] == "#  1" and len(lines[10]) == 5 * 23 and len(lines[11]) == 2 * 23
+    assert lines[-1] == "# End"
+    d = flowio.read_flow_ascii(p)
+    assert d["time"] == 1.0e-3
+    assert np.allclose(d["cv"], cv, rtol=6e-16, atol=0)
+
+
+def test_binary_records_and_exact_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    cv = rng.standard_normal((11, 5))
+    p = tmp_path / "case.mixt.cv_00001_2.00000E-03"
+    flowio.write_flow_binary(p, cv, 2.0e-3, res_init=0.5)
+    raw = p.read_bytes()
+    assert struct.unpack_from("<i", raw, 0)[0] == 80 and raw[4:27] == b"# ROCFLU flow file (v2)"
+    d = flowio.read_flow_binary(p)
+    assert d["time"] == 2.0e-3 and d["res_init"] == 0.5
+    assert np.array_equal(d["cv"], cv)
+
+
+def test_restart_info_appends(tmp_path):
+    flowio.append_restart_info("case", 1.0e-3, tmp_path)
+    flowio.append_restart_info("case", 0.25, tmp_path)
+    assert flowio.read_restart_info("case", tmp_path) == [1.0e-3, 0.25]
