@@ -214,3 +214,91 @@ def read_restart_info(case, directory=""):
     """Times listed in <case>.rin (the last one is where a run resumes)."""
     with open(os.path.join(directory, f"{case}.rin")) as f:
         return [float(t) for t in f.read().split()]
+
+
+# ---------------------------------------------------------------------------
+# ppiclF particle files (libpicl/ppiclF/source/ppiclf_io.f:1362-1758)
+# ---------------------------------------------------------------------------
+LRS, LRP = 12, 48  # PPICLF_LRS, PPICLF_LRP (libpicl/user_files/PPICLF_USER.h)
+
+
+def fortran_e14_7(x):
+    """Fortran E14.7."""
+    x = float(x)
+    if x == 0.0:
+        return " 0.0000000E+00"
+    m = f"{x:.6e}"
+    mant, exp = m.split("e")
+    sign = "-" if mant.startswith("-") else ""
+    digits = mant.lstrip("-").replace(".", "")
+    e = int(exp) + 1
+    return f"{sign}0.{digits}E{'+' if e >= 0 else '-'}{abs(e):02d}".rjust(14)
+
+
+def write_particle_vtu(path, y, rprop=None, tags=None, time=0.0, cycle=0):
+    """ppiclf_io_WriteParticleVTU: an UnstructuredGrid of points with
+    Float32 arrays Position, y01-y12, rprop01-rprop48 and tag01-tag03 in
+    raw appended data (each block a 4-byte byte count, then the values).
+    y: (n, 12); rprop: (n, 48) or None (zeros); tags: (n, 3) or None."""
+    y = np.asarray(y, np.float64)
+    n = y.shape[0]
+    rprop = np.zeros((n, LRP)) if rprop is None else np.asarray(rprop, np.float64)
+    tags = np.zeros((n, 3)) if tags is None else np.asarray(tags, np.float64)
+    arrays = [("Position", 3, y[:, :3])]
+    arrays += [(f"y{j + 1:02d}", 1, y[:, j]) for j in range(LRS)]
+    arrays += [(f"rprop{j + 1:02d}", 1, rprop[:, j]) for j in range(LRP)]
+    arrays += [(f"tag{j + 1:02d}", 1, tags[:, j]) for j in range(3)]
+    head = ['<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian">',
+            ' <UnstructuredGrid>', '  <FieldData>',
+            '   <DataArray type="Float32" Name="TIME" NumberOfTuples="1" format="ascii"> '
+            f'{fortran_e14_7(time)} </DataArray> ',
+            '   <DataArray type="Int32" Name="CYCLE" NumberOfTuples="1" format="ascii"> '
+            f'{int(cycle)} </DataArray> ',
+            '  </FieldData>', f'  <Piece NumberOfPoints="{n}" NumberOfCells="0"> ']
+    off = 0
+
+    def da(name, nc):
+        return (f'    <DataArray type="Float32" Name="{name}" NumberOfComponents="{nc}" '
+                f'format="append" offset="{off}"/>')
+    head.append('   <Points>')
+    head.append(da("Position", 3))
+    off += 3 * 4 * n + 4
+    head.append('   </Points>')
+    head.append('   <PointData>')
+    for name, nc, _ in arrays[1:]:
+        head.append(da(name, nc))
+        off += nc * 4 * n + 4
+    head += ['   </PointData> ', '   <Cells> ',
+             '    <DataArray type="Int32" Name="connectivity" format="ascii"/> ',
+             '    <DataArray type="Int32" Name="offsets" format="ascii"/> ',
+             '    <DataArray type="Int32" Name="types" format="ascii"/> ',
+             '   </Cells> ', '  </Piece> ', ' </UnstructuredGrid> ']
+    with open(path, "wb") as f:
+        f.write(("\n".join(head) + "\n").encode("ascii"))
+        f.write(b' <AppendedData encoding="raw">_')
+        for name, nc, a in arrays:
+            blob = np.ascontiguousarray(a, "<f4").tobytes()
+            f.write(struct.pack("<i", len(blob)))
+            f.write(blob)
+        f.write(b"</AppendedData>\n</VTKFile>\n")
+
+
+def read_particle_vtu(path):
+    """ppiclf_io_ReadParticleVTU's counterpart: returns dict(time, cycle,
+    y (n, 12), rprop (n, 48), tags (n, 3)) as float64 (the file is Float32)."""
+    import re
+    data = open(path, "rb").read()
+    cut = data.index(b'<AppendedData encoding="raw">_') + len(b'<AppendedData encoding="raw">_')
+    head = data[:cut].decode("ascii")
+    n = int(re.search(r'NumberOfPoints="(\d+)"', head).group(1))
+    time = float(re.search(r'Name="TIME"[^>]*>\s*([^<\s]+)', head).group(1))
+    cycle = int(re.search(r'Name="CYCLE"[^>]*>\s*([^<\s]+)', head).group(1))
+    arrs = {}
+    for name, nc, off in re.findall(r'Name="([^"]+)" NumberOfComponents="(\d+)" format="append" offset="(\d+)"', head):
+        p = cut + int(off)
+        (nb,) = struct.unpack_from("<i", data, p)
+        arrs[name] = np.frombuffer(data, "<f4", count=nb // 4, offset=p + 4).astype(np.float64).reshape(n, int(nc))
+    y = np.concatenate([arrs[f"y{j + 1:02d}"] for j in range(LRS)], axis=1)
+    rprop = np.concatenate([arrs[f"rprop{j + 1:02d}"] for j in range(LRP)], axis=1)
+    tags = np.concatenate([arrs[f"tag{j + 1:02d}"] for j in range(3)], axis=1)
+    return dict(time=time, cycle=cycle, y=y, rprop=rprop, tags=tags, position=arrs["Position"])

@@ -194,6 +194,20 @@ class Region:
         assert y.ndim == 2 and y.shape[1] == 12 and pc.shape == (y.shape[0], 4)
         self._chk(self._L.picl_gpu_init(self.h, C.byref(pp), y.shape[0], _p(y), _p(pc)))
         self.nPart = y.shape[0]
+        self._pconst = pc.copy()
+
+    def write_particles(self, path, cycle=0):
+        """ppiclf_io_WriteParticleVTU of the cloud: y, the constant rprop
+        entries (rho_p, d_p, V_p, SPL at PPICLF_R_JRHOP, JDP, JVOLP, JSPL) and
+        tags (rank 0, particle number, element number + 1; 0 when lost)."""
+        from . import flowio
+        p = self.picl_particles()
+        n = self.nPart
+        rprop = np.zeros((n, flowio.LRP))
+        for col, j in zip(range(4), (0, 2, 3, 21)):
+            rprop[:, j] = self._pconst[:, col]
+        tags = np.stack([np.zeros(n), np.arange(1, n + 1), p["elem"] + 1.0], axis=1)
+        flowio.write_particle_vtu(path, p["y"], rprop, tags, time=self.time()[0], cycle=cycle)
 
     def PICL_TEMP_Runge(self, istage):
         """The coupling stage: call after SourceTermsMP, before RKUpdateMP."""

@@ -50,3 +50,26 @@ def test_restart_info_appends(tmp_path):
     flowio.append_restart_info("case", 1.0e-3, tmp_path)
     flowio.append_restart_info("case", 0.25, tmp_path)
     assert flowio.read_restart_info("case", tmp_path) == [1.0e-3, 0.25]
+
+
+def test_particle_vtu_layout_and_round_trip(tmp_path):
+    rng = np.random.default_rng(5)
+    n = 9
+    y = rng.standard_normal((n, 12))
+    rprop = rng.standard_normal((n, 48))
+    tags = np.stack([np.zeros(n), np.arange(1, n + 1), np.arange(1, n + 1)], axis=1)
+    p = tmp_path / "par00001.vtu"
+    flowio.write_particle_vtu(p, y, rprop, tags, time=2.5e-4, cycle=7)
+    raw = p.read_bytes()
+    assert raw.startswith(b'<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian">\n')
+    assert b'Name="TIME" NumberOfTuples="1" format="ascii">  0.2500000E-03 </DataArray>' in raw
+    # the 67 arrays follow each other: offset = previous + 4 * components * n + 4
+    assert b'Name="y01" NumberOfComponents="1" format="append" offset="112"/>' in raw
+    assert raw.endswith(b"</AppendedData>\n</VTKFile>\n")
+    d = flowio.read_particle_vtu(p)
+    assert d["time"] == np.float64(np.float32(2.5e-4)) or abs(d["time"] - 2.5e-4) < 1e-10
+    assert d["cycle"] == 7
+    assert np.array_equal(d["y"], y.astype(np.float32).astype(np.float64))
+    assert np.array_equal(d["rprop"], rprop.astype(np.float32).astype(np.float64))
+    assert np.array_equal(d["position"], y[:, :3].astype(np.float32).astype(np.float64))
+    assert np.array_equal(d["tags"], tags)

@@ -36,3 +36,17 @@ def test_resume_from_flow_file(tmp_path, binary):
         err = (np.abs(cc - ca) / np.abs(ca).max(axis=0)).max()
         assert err < 1e-13, err
     assert (tmp_path / "shock.rin").exists()
+
+
+def test_particle_file_of_a_coupled_run(tmp_path):
+    from rocflupicl_amd import flowio
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=300, jitter=0.05)
+    r = region(mesh, params, ic)
+    r.picl_init(y, pc, **picl)
+    r.RFLU_TimeStepping(3)
+    r.write_particles(tmp_path / "par00001.vtu", cycle=3)
+    d = flowio.read_particle_vtu(tmp_path / "par00001.vtu")
+    p = r.picl_particles()
+    assert d["cycle"] == 3 and d["y"].shape == (300, 12)
+    assert np.array_equal(d["y"], p["y"].astype(np.float32).astype(np.float64))
+    assert np.array_equal(d["tags"][:, 2], p["elem"] + 1.0)
