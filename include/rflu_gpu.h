@@ -37,6 +37,7 @@ extern "C" {
 #define RFLU_DISCR_AUSM 4
 /* reconst values (modfloflu/ModParameters.F90:682) */
 #define RFLU_RECONST_NONE 0
+#define RFLU_RECONST_WENO_XYZ 2 /* WENO-XYZ average + positivity (the deck default) */
 #define RFLU_RECONST_VENKAT 11
 /* boundary kinds on the path (rocflu/RFLU_CentralFirstPatch.F90:575,790) */
 #define RFLU_BC_SLIPWALL 1

@@ -741,33 +741,35 @@ static void gradient_var1(orc_region *r, double *out) {
  * per component, the average of the cell's own and its stencil members'
  * gradients weighted by 1/(g^2 + 1e-15), accumulated self first, then the
  * members in stencil order; all cells read the unmodified input. */
-static void weno_xyz(orc_region *r, double *grad) {
+static void weno_xyz_n(orc_region *r, double *grad, int stride) {
   int N = r->nCells;
-  double *eno = malloc(sizeof(double) * 3 * (size_t)N);
+  double *eno = malloc(sizeof(double) * (size_t)stride * (size_t)N);
   for (int icg = 0; icg < N; icg++) {
-    for (int d = 0; d < 3; d++) {
-      double term = grad[3 * (size_t)icg + d];
+    for (int d = 0; d < stride; d++) {
+      double term = grad[(size_t)stride * icg + d];
       double sum = 1.0 / (term * term + 1.0E-15);
       double loc = sum * term;
       for (int s = r->csOff[icg]; s < r->csOff[icg + 1]; s++) {
-        term = grad[3 * (size_t)r->csMem[s] + d];
+        term = grad[(size_t)stride * r->csMem[s] + d];
         double w = 1.0 / (term * term + 1.0E-15);
         sum = sum + w;
         loc = loc + w * term;
       }
-      eno[3 * (size_t)icg + d] = loc / sum;
+      eno[(size_t)stride * icg + d] = loc / sum;
     }
   }
-  memcpy(grad, eno, sizeof(double) * 3 * (size_t)N);
+  memcpy(grad, eno, sizeof(double) * (size_t)stride * (size_t)N);
   free(eno);
 }
+static void weno_xyz(orc_region *r, double *grad) { weno_xyz_n(r, grad, 3); }
 
 /* RFLU_LimitGradCellsSimple, one VAR_INFO_POS variable
  * (modflu/RFLU_ModLimiters.F90:793-921): the gradient is zeroed when the
  * extrapolation to any face centroid of the cell (interior or boundary) is
  * <= 0.  The face loop exits at the first such face and only then changes
  * the gradient, so the outcome does not depend on the face order. */
-static void limit_simple(orc_region *r, double *grad) {
+/* grad: the variable's 3 components at grad[stride * cell]; var: its index in cv */
+static void limit_simple_var(orc_region *r, double *grad, int stride, int var) {
   int N = r->nCells;
   char *zero = calloc((size_t)N, 1);
   for (int f = 0; f < r->nFaces + r->nBFaces; f++) {
@@ -775,16 +777,17 @@ static void limit_simple(orc_region *r, double *grad) {
     const double *fc = f < r->nFaces ? r->fc + 3 * (size_t)f : r->bfc + 3 * (size_t)(f - r->nFaces);
     for (int s = 0; s < nc; s++) {
       int c = f < r->nFaces ? r->f2c[2 * f + s] : r->bf2c[f - r->nFaces];
-      const double *x = r->cofg + 3 * (size_t)c, *g = grad + 3 * (size_t)c;
+      const double *x = r->cofg + 3 * (size_t)c, *g = grad + (size_t)stride * c;
       double dx = fc[0] - x[0], dy = fc[1] - x[1], dz = fc[2] - x[2];
-      double varf = r->cv[5 * (size_t)c] + g[0] * dx + g[1] * dy + g[2] * dz;
+      double varf = r->cv[5 * (size_t)c + var] + g[0] * dx + g[1] * dy + g[2] * dz;
       if (varf <= 0.0) zero[c] = 1;
     }
   }
   for (int i = 0; i < N; i++)
-    if (zero[i]) grad[3 * (size_t)i] = grad[3 * (size_t)i + 1] = grad[3 * (size_t)i + 2] = 0.0;
+    if (zero[i]) grad[(size_t)stride * i] = grad[(size_t)stride * i + 1] = grad[(size_t)stride * i + 2] = 0.0;
   free(zero);
 }
+static void limit_simple(orc_region *r, double *grad) { limit_simple_var(r, grad, 3, 0); }
 
 /* CellGradientsMP: libfloflu/CellGradientsMP.F90:158-253, and the PICL
  * block :355-390 (gradient of rho_g = rho/(1-phi), WENO-XYZ, positivity
@@ -795,6 +798,13 @@ static void cell_gradients(orc_region *r) {
     cons2prim(r);
     gradients(r);
     if (r->p.reconst == ORC_RECONST_VENKAT) limiter_venkat(r);
+    if (r->p.reconst == ORC_RECONST_WENO_XYZ) {
+      /* :210-215: RFLU_WENOGradCellsXYZWrapper, then RFLU_LimitGradCellsSimple
+       * on the VAR_INFO_POS variables of the DUVWP state, rho and p */
+      weno_xyz_n(r, r->grad, 15);
+      limit_simple_var(r, r->grad, 15, 0);
+      limit_simple_var(r, r->grad + 12, 15, 4);
+    }
     prim2cons(r);
   }
   if (r->phi) {

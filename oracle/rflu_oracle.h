@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 enum { ORC_DISCR_ROE = 1, ORC_DISCR_HLLC = 3, ORC_DISCR_AUSM = 4 };
-enum { ORC_RECONST_NONE = 0, ORC_RECONST_VENKAT = 11 };
+enum { ORC_RECONST_NONE = 0, ORC_RECONST_WENO_XYZ = 2, ORC_RECONST_VENKAT = 11 };
 enum { ORC_BC_SLIPWALL = 1, ORC_BC_OUTFLOW = 2 };
 enum { ORC_BCOPT_SUPERSONIC = 0, ORC_BCOPT_SUBSONIC = 1, ORC_BCOPT_MIXED = 2 };
 

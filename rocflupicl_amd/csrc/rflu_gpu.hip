@@ -71,6 +71,8 @@ struct Dev {
   // and the interior faces' fluxes as 5 floats per face (boundary faces keep
   // their FP64 fluxes in flux[])
   float *gradF = nullptr, *fluxF = nullptr;
+  // WENO-XYZ reconstruction (reconst 2): the unlimited LSQ gradients
+  double *gradRaw = nullptr;
   // batched ensemble (nInst > 0): instance k owns device cells
   // [iOff[k], iOff[k+1]); cInst gives each cell's instance; iScal holds each
   // instance's scal[] slots (4 per instance)
@@ -843,6 +845,93 @@ __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int gu
   o[1] = make_float4(g[4], g[5], g[6], g[7]);
   o[2] = make_float4(g[8], g[9], g[10], g[11]);
   o[3] = make_float4(g[12], g[13], g[14], 0.0f);
+}
+
+// RFLU_LimitGradCellsSimple on rho and p of the DUVWP record, then the store.
+__device__ __forceinline__ void weno_positivity_store(const Dev &d, int i, double g[15]) {
+  double r[8];
+  load_rec(d.rec, i, r);  // rho u v w p x y z
+  double xc = r[5], yc = r[6], zc = r[7];
+  bool z0 = false, z4 = false;
+  for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+    int slot = d.cfIdx[e] >> 1;
+    double dx = d.fcx[slot] - xc, dy = d.fcy[slot] - yc, dz = d.fcz[slot] - zc;
+    if (r[0] + g[0] * dx + g[1] * dy + g[2] * dz <= 0.0) z0 = true;
+    if (r[4] + g[12] * dx + g[13] * dy + g[14] * dz <= 0.0) z4 = true;
+  }
+  if (z0) g[0] = g[1] = g[2] = 0.0;
+  if (z4) g[12] = g[13] = g[14] = 0.0;
+  double *o = d.grad + 15 * (size_t)i;
+#pragma unroll
+  for (int q = 0; q < 15; q++) o[q] = g[q];
+}
+
+// RECONST_WENO_XYZ (libfloflu/CellGradientsMP.F90:208-215): the WENO-XYZ
+// average of the unlimited LSQ gradients over each cell and its stencil
+// (RFLU_WENOGradCellsXYZFast_3D, modflu/RFLU_ModWENO.F90:1402-1588: per
+// component, weights 1/(g^2 + 1e-15), the cell first, then the members in
+// stencil order), then RFLU_LimitGradCellsSimple on rho and p
+// (modflu/RFLU_ModLimiters.F90:793-921: the gradient is zeroed when the
+// extrapolation to any face centroid of the cell is <= 0).  Over the
+// gradient tiles: for each variable the tile's slots carry w and w*g per
+// component in LDS, formed once per slot.
+__global__ void __launch_bounds__(BLK) k_weno_tile(Dev d) {
+  extern __shared__ double2 sW[];  // [nU][3]
+  int tile = xcd_block();
+  int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
+  int i = tile * BLK + threadIdx.x;
+  bool live = i < d.nC;
+  int sb = live ? d.csOff[i] : 0, se = live ? d.csOff[i + 1] : 0;
+  double g[15];
+  for (int k = 0; k < 5; k++) {
+    if (k) __syncthreads();  // the previous variable's slots are consumed
+    for (int u = threadIdx.x; u < nU; u += BLK) {
+      const double *gr = d.gradRaw + 15 * (size_t)d.tMem[u0 + u] + 3 * k;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        double t = gr[q];
+        double w = 1.0 / (t * t + 1.0E-15);
+        sW[3 * u + q] = make_double2(w, w * t);
+      }
+    }
+    __syncthreads();
+    if (live) {
+      double2 o0 = sW[3 * threadIdx.x], o1 = sW[3 * threadIdx.x + 1], o2 = sW[3 * threadIdx.x + 2];
+      double s0 = o0.x, s1 = o1.x, s2 = o2.x, l0 = o0.y, l1 = o1.y, l2 = o2.y;
+      for (int s = sb; s < se; s++) {
+        int u = d.sSlot[s] & 0x7fffu;
+        double2 a = sW[3 * u], b = sW[3 * u + 1], c = sW[3 * u + 2];
+        s0 = s0 + a.x; s1 = s1 + b.x; s2 = s2 + c.x;
+        l0 = l0 + a.y; l1 = l1 + b.y; l2 = l2 + c.y;
+      }
+      g[3 * k] = l0 / s0;
+      g[3 * k + 1] = l1 / s1;
+      g[3 * k + 2] = l2 / s2;
+    }
+  }
+  if (live) weno_positivity_store(d, i, g);
+}
+
+// The same average straight from global memory (meshes whose tiles do not
+// fit in LDS).
+__global__ void __launch_bounds__(BLK) k_weno(Dev d) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
+  int sb = d.csOff[i], se = d.csOff[i + 1];
+  double g[15];
+  for (int c = 0; c < 15; c++) {
+    double t = d.gradRaw[15 * (size_t)i + c];
+    double sum = 1.0 / (t * t + 1.0E-15), loc = sum * t;
+    for (int s = sb; s < se; s++) {
+      t = d.gradRaw[15 * (size_t)(mem[s] & CS_IDX) + c];
+      double w = 1.0 / (t * t + 1.0E-15);
+      sum = sum + w;
+      loc = loc + w * t;
+    }
+    g[c] = loc / sum;
+  }
+  weno_positivity_store(d, i, g);
 }
 
 // Cooperative variant 3: a workgroup of 256 lanes takes 16 cells.
@@ -1844,37 +1933,58 @@ int launch_dtmin(rflu_handle *h, int finalize, int storeCell) {
   return 0;
 }
 
+int launch_grad_impl(rflu_handle *h, const Dev &dv, int venkat, int guard);
+
 int launch_grad(rflu_handle *h) {
   if (h->prm.spaceOrder < 2) return 0;
+  if (h->prm.reconst == RFLU_RECONST_WENO_XYZ) {
+    // unlimited LSQ gradients into gradRaw, then WENO-XYZ + positivity into grad
+    Dev dv = h->d;
+    dv.grad = h->d.gradRaw;
+    {
+      ProfScope ps(h, "k_grad");
+      launch_grad_impl(h, dv, 0, 0);
+    }
+    ProfScope ps(h, "k_weno");
+    if (h->gradVariant == 4 && h->tileLds && h->tileLds / 64 * 48 <= h->ldsMax)
+      hipLaunchKernelGGL(k_weno_tile, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 64 * 48, h->stream, h->d);
+    else
+      hipLaunchKernelGGL(k_weno, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
+    return 0;
+  }
   ProfScope ps(h, "k_grad");
   int venkat = h->prm.reconst == RFLU_RECONST_VENKAT ? 1 : 0, guard = h->prm.limiterGuard ? 1 : 0;
+  return launch_grad_impl(h, h->d, venkat, guard);
+}
+
+int launch_grad_impl(rflu_handle *h, const Dev &dv, int venkat, int guard) {
   int nUmax = (int)(h->tileLds / 64);
   if (h->prm.fluxPrecision == 1) {
-    hipLaunchKernelGGL(k_grad_tile_f32, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 64 * sizeof(SlotF), h->stream, h->d,
+    hipLaunchKernelGGL(k_grad_tile_f32, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 64 * sizeof(SlotF), h->stream, dv,
                        venkat, guard);
     return 0;
   }
   if (h->picl && h->gradVariant == 4 && h->tileLds && h->tileLds + 8 * (size_t)nUmax <= h->ldsMax)
     hipLaunchKernelGGL((k_grad_tile<1>), dim3(h->d.nBlk), dim3(BLK), h->tileLds + 8 * (size_t)nUmax, h->stream,
-                       h->d, venkat, guard, nUmax);
+                       dv, venkat, guard, nUmax);
   else if (h->picl)
-    hipLaunchKernelGGL((k_grad2<1, 1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<1, 1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 4 && h->tileLds)
-    hipLaunchKernelGGL((k_grad_tile<0>), dim3(h->d.nBlk), dim3(BLK), h->tileLds, h->stream, h->d, venkat, guard, nUmax);
+    hipLaunchKernelGGL((k_grad_tile<0>), dim3(h->d.nBlk), dim3(BLK), h->tileLds, h->stream, dv, venkat, guard, nUmax);
   else if (h->gradVariant == 2 || h->gradVariant == 4)
-    hipLaunchKernelGGL((k_grad2<1, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<1, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 24)
-    hipLaunchKernelGGL((k_grad2<4, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<4, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 25)
-    hipLaunchKernelGGL((k_grad2<5, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<5, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 26)
-    hipLaunchKernelGGL((k_grad2<6, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL((k_grad2<6, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 3)
-    hipLaunchKernelGGL(k_grad3, dim3((h->d.nC + G3C - 1) / G3C), dim3(256), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL(k_grad3, dim3((h->d.nC + G3C - 1) / G3C), dim3(256), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 0)
-    hipLaunchKernelGGL(k_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL(k_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else
-    hipLaunchKernelGGL(k_grad_lds, dim3((h->d.nC + GCPB - 1) / GCPB), dim3(BLK), 0, h->stream, h->d, venkat, guard);
+    hipLaunchKernelGGL(k_grad_lds, dim3((h->d.nC + GCPB - 1) / GCPB), dim3(BLK), 0, h->stream, dv, venkat, guard);
   return 0;
 }
 
@@ -1972,8 +2082,11 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
     return set_err(nullptr, RFLU_GPU_EINVAL, "spaceDiscr must be 1 (Roe), 3 (HLLC) or 4 (AUSM+)");
   if (prm->patchOrder == 2 && prm->spaceOrder != 2)
     return set_err(nullptr, RFLU_GPU_EINVAL, "second-order patches need spaceOrder 2");
-  if (prm->reconst != RFLU_RECONST_NONE && prm->reconst != RFLU_RECONST_VENKAT)
-    return set_err(nullptr, RFLU_GPU_EINVAL, "reconst must be 0 or 11 (Venkatakrishnan)");
+  if (prm->reconst != RFLU_RECONST_NONE && prm->reconst != RFLU_RECONST_WENO_XYZ &&
+      prm->reconst != RFLU_RECONST_VENKAT)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "reconst must be 0, 2 (WENO-XYZ) or 11 (Venkatakrishnan)");
+  if (prm->fluxPrecision == 1 && prm->reconst == RFLU_RECONST_WENO_XYZ)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "fluxPrecision 1 supports reconst 0 and 11 only");
   if (prm->fluxPrecision < 0 || prm->fluxPrecision > 1)
     return set_err(nullptr, RFLU_GPU_EINVAL, "fluxPrecision must be 0 (FP64) or 1 (FP32 gradients and fluxes)");
   if (prm->fluxPrecision == 1 && prm->spaceOrder != 2)
@@ -2178,6 +2291,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
     rc |= dalloc(h, &d.csMem, csMemD.size(), csMemD.data());
     rc |= dalloc(h, &d.rm, 6 * (size_t)N, rmSoa.data());
     rc |= dalloc(h, &d.grad, 15 * (size_t)N);
+    if (prm->reconst == RFLU_RECONST_WENO_XYZ) rc |= dalloc(h, &d.gradRaw, 15 * (size_t)N);
     if (prm->fluxPrecision == 1) {
       if (!h->tileLds) rc |= set_err(h, RFLU_GPU_EINVAL, "fluxPrecision 1 needs gradient tiles that fit in LDS");
       else {

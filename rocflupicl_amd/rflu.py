@@ -14,7 +14,7 @@ from . import _lib
 from .meshgen import BC_OUTFLOW, BC_SLIPWALL  # noqa: F401  (re-exported)
 
 DISCR_ROE, DISCR_HLLC, DISCR_AUSM = 1, 3, 4
-RECONST_NONE, RECONST_VENKAT = 0, 11
+RECONST_NONE, RECONST_WENO_XYZ, RECONST_VENKAT = 0, 2, 11
 
 DEFAULTS = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1,
                 cfl=0.5, dtImposed=1.0e-5, dtMinLimit=1.0e-10, dTimeSystem=0.0,

@@ -231,3 +231,27 @@ def test_gaussian_graph_replays_equal_eager_steps(oracle_mod):
         e.RFLU_TimeStepping(2, use_graph=False)
         assert same_bits(d.get_state()[0], e.get_state()[0])
         assert same_bits(d.picl_nodes(), e.picl_nodes())
+
+
+def test_coupled_steps_with_weno_reconstruction(oracle_mod):
+    """The gas side with WENO-XYZ reconstruction (the deck default) under the
+    particle coupling."""
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=500, jitter=0.05, seed=4,
+                                                      reconst=2)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(6, use_graph=True)
+    o.step(6)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(op["y"][:, 3]).max() > 0.0
+    for k in ("y", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err

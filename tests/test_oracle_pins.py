@@ -347,3 +347,100 @@ def test_sod_first_order_ausm_approaches_exact_solution(oracle_mod):
     l1 = np.sum(np.abs(rho - ex) * vol) / vol.sum()
     assert l1 < 0.02, l1
     assert np.isfinite(rho).all() and rho.min() > 0.1
+
+
+# ---- WENO-XYZ reconstruction (reconst 2) ----
+def test_weno_xyz_exact_for_linear_fields(oracle_mod):
+    """Every stencil member of a linear field carries the same LSQ gradient,
+    so the weighted average returns it (and rho, p stay positive)."""
+    mesh = mg.tet_box(4, 4, 4, 1.0, 1.0, 1.0, jitter=0.1, seed=8)
+    o = oracle_mod.Region(mesh, spaceOrder=2, reconst=2, refCp=CP, refGamma=GAMMA)
+    vol, cofg = o.cells()
+    a = np.array([1.2, 0.1, -0.2, 0.05, 10.0, 3.0, 2.0, -1.0, -5.0, 2.0, 1.0e5, 2.0e3, -1.0e3, 5.0e2])
+    P = linear_state(cofg, a)
+    o.set_cv(mg.prim_to_cons(*P.T, GAMMA).T.copy())
+    o.residual()
+    g = o.grad()
+    expect = np.array([[a[1], a[2], a[3]], [a[5], -a[6], a[7]], [-a[5], a[6], 0.0], [0.0, 0.0, a[7]],
+                       [a[11], a[12], -a[13]]])
+    for k in range(5):
+        scale = np.abs(expect[k]).max()
+        assert np.abs(g[:, k, :] - expect[k]).max() <= 1e-9 * scale, k
+
+
+def _weno_pair(oracle_mod, P, mesh):
+    kw = dict(spaceOrder=2, refCp=CP, refGamma=GAMMA)
+    a0 = oracle_mod.Region(mesh, reconst=0, **kw)
+    a2 = oracle_mod.Region(mesh, reconst=2, **kw)
+    cv = mg.prim_to_cons(*P.T, GAMMA).T.copy()
+    for o in (a0, a2):
+        o.set_cv(cv)
+        o.residual()
+    return a0, a2
+
+
+def test_weno_xyz_is_a_stencil_average_that_prefers_the_smoothest(oracle_mod):
+    mesh = mg.tet_box(5, 4, 4, 1.0, 0.8, 0.8, jitter=0.05, seed=12)
+    o = oracle_mod.Region(mesh, spaceOrder=2, reconst=0, refCp=CP, refGamma=GAMMA)
+    vol, cofg = o.cells()
+    n = len(vol)
+    rng = np.random.default_rng(3)
+    x = cofg[:, 0]
+    P = np.stack([1.2 + 0.1 * rng.random(n),
+                  np.where(x < 0.5, 0.0, 100.0),           # u: a step, flat on either side
+                  20 * rng.standard_normal(n), 20 * rng.standard_normal(n),
+                  1e5 * (1 + 0.05 * rng.random(n))], axis=1)
+    a0, a2 = _weno_pair(oracle_mod, P, mesh)
+    g0, g2 = a0.grad(), a2.grad()
+    off, mem, _ = a0.stencils()
+    mem = mem & 0x7fffffff
+    flat = np.all(g0[:, 1, :] == 0.0, axis=1)
+    assert flat.any() and (~flat).any()
+    hits = 0
+    for i in range(n):
+        members = np.concatenate([[i], mem[off[i]:off[i + 1]]])
+        lo, hi = g0[members].min(axis=0), g0[members].max(axis=0)
+        span = np.maximum(hi - lo, 1e-300)
+        # a weighted average of the cell's and its members' raw gradients
+        assert np.all(g2[i] >= lo - 1e-12 * span) and np.all(g2[i] <= hi + 1e-12 * span), i
+        # a member with a zero u gradient outweighs the steep ones (weight 1e15)
+        if np.abs(g0[i, 1]).max() > 1.0 and flat[members].any():
+            hits += 1
+            assert np.abs(g2[i, 1]).max() <= 1e-12 * np.abs(g0[i, 1]).max(), i
+    assert hits > 0
+
+
+def test_weno_xyz_positivity_zeroes_rho_and_p_gradients(oracle_mod):
+    """RFLU_LimitGradCellsSimple: the rho (p) gradient of a cell is zeroed
+    exactly when its extrapolation to one of the cell's face centroids is
+    <= 0.  v carries rho's profile and w carries p's (scaled), so their
+    unlimited WENO gradients give the extrapolations; rho and p fall to zero
+    at different walls so a swapped variable index fails."""
+    mesh = mg.tet_box(5, 4, 4, 1.0, 0.8, 0.8, jitter=0.05, seed=12)
+    o = oracle_mod.Region(mesh, spaceOrder=2, reconst=0, refCp=CP, refGamma=GAMMA)
+    vol, cofg = o.cells()
+    n = len(vol)
+    rng = np.random.default_rng(9)
+    # ramps whose extrapolations cross zero at the x = 0 (rho) and y = 0 (p)
+    # walls while every cell value stays positive
+    rho = 2.0 * (cofg[:, 0] - 0.03) * (1 + 0.01 * rng.random(n))
+    p = 1e5 * (cofg[:, 1] - 0.03) * (1 + 0.01 * rng.random(n))
+    assert rho.min() > 0 and p.min() > 0
+    P = np.stack([rho, 10.0 * rng.standard_normal(n), 100.0 * rho, 1e-3 * p, p], axis=1)
+    a0, a2 = _weno_pair(oracle_mod, P, mesh)
+    g2 = a2.grad()
+    f2c, _, _, fc = a2.faces()
+    bf2c, _, bfc = a2.bfaces()
+    cells = np.concatenate([f2c[:, 0], f2c[:, 1], bf2c])
+    cents = np.concatenate([fc, fc, bfc])
+    d = cents - cofg[cells]
+    for var, mirror, scale, val in ((0, 2, 100.0, rho), (4, 3, 1e-3, p)):
+        ext = val[cells] + (g2[cells, mirror, :] / scale * d).sum(axis=1)
+        worst = np.full(n, np.inf)
+        np.minimum.at(worst, cells, ext / np.abs(val).max())
+        zeroed = np.all(g2[:, var, :] == 0.0, axis=1)
+        assert np.all(zeroed[worst < -1e-9]), var
+        assert not np.any(zeroed[worst > 1e-9]), var
+        assert zeroed.sum() >= 1 and (worst < -1e-9).sum() >= 1, var
+        # the mirrored component itself is not limited
+        assert not np.all(g2[zeroed, mirror, :] == 0.0)
