@@ -35,6 +35,7 @@ extern "C" {
 #define RFLU_DISCR_ROE 1
 #define RFLU_DISCR_HLLC 3
 #define RFLU_DISCR_AUSM 4
+#define RFLU_DISCR_AUSMUP 5 /* AUSM+up (modflu/RFLU_ModAUSMPlusUpFlux.F90) */
 /* reconst values (modfloflu/ModParameters.F90:682) */
 #define RFLU_RECONST_NONE 0
 #define RFLU_RECONST_WENO_XYZ 2 /* WENO-XYZ average + positivity (the deck default) */
@@ -93,6 +94,10 @@ typedef struct rflu_params {
    * boundary patches and the dt reduction.  Needs spaceOrder 2, no
    * particles.  No reference counterpart: an added precision mode. */
   int fluxPrecision;
+  /* AUSM+up constants (NUMERICS SIGMA, KP, KU, MINF; the reference's
+   * defaults are 1, 0.25, 0.75, 0.1, libfloflu/ReadNumericsSection.F90:236-275).
+   * The TCP flux function sets f_a = 1, so mInf has no effect there. */
+  double sigma, kp, ku, mInf;
 } rflu_params;
 
 /* RFLU_InitFlowSolver grid/geometry/stencil set-up

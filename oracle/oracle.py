@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-DISCR_ROE, DISCR_HLLC, DISCR_AUSM = 1, 3, 4
+DISCR_ROE, DISCR_HLLC, DISCR_AUSM, DISCR_AUSMUP = 1, 3, 4, 5
 RECONST_NONE, RECONST_VENKAT = 0, 11
 
 
@@ -22,7 +22,8 @@ class OrcParams(C.Structure):
                 ("patchOrder", C.c_int), ("cfl", C.c_double), ("dtImposed", C.c_double),
                 ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
                 ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
-                ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int)]
+                ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int),
+                ("sigma", C.c_double), ("kp", C.c_double), ("ku", C.c_double), ("mInf", C.c_double)]
 
 
 class OrcPiclParams(C.Structure):
@@ -113,7 +114,8 @@ def make_params(**kw):
     p = OrcParams()
     d = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1, cfl=0.5,
              dtImposed=1e-5, dtMinLimit=1e-10, dTimeSystem=0.0, refCp=1004.5, refGamma=1.4,
-             epsentr=0.05, dissFact=1.0, accelOn=0, accel=(0.0, 0.0, 0.0), limiterGuard=0)
+             epsentr=0.05, dissFact=1.0, accelOn=0, accel=(0.0, 0.0, 0.0), limiterGuard=0,
+             sigma=1.0, kp=0.25, ku=0.75, mInf=0.1)
     d.update(kw)
     for k, v in d.items():
         if k == "accel":

@@ -967,6 +967,55 @@ static void hllc_flux(const orc_params *p, double nx, double ny, double nz, doub
   }
 }
 
+/* RFLU_AUSMPlusUp_FluxFunction2 (modflu/RFLU_ModAUSMPlusUpFlux.F90:469-588),
+ * the function both TCP drivers call.  It overwrites the Mach-number
+ * function fa with 1 (:519), so mInf and the M_o terms have no effect and
+ * alpha = 3/16; vFracF scales the convected part (the face's gas volume
+ * fraction with particles, else 1). */
+static void ausmup_flux_function(const orc_params *p, double nx, double ny, double nz, double nm, double fs,
+                                 double rl, double ul, double vl, double wl, double pl, double Hl, double al,
+                                 double rr, double ur, double vr, double wr, double pr, double Hr, double ar,
+                                 double vFracF, double *flx) {
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double qr = ur * nx + vr * ny + wr * nz - fs;
+  double af = 0.5 * (al + ar);
+  double ml = ql / af, mla = fabs(ml);
+  double mr = qr / af, mra = fabs(mr);
+  double mbarsq = 0.5 * (ml * ml + mr * mr);
+  double fa = 1.0;
+  double alpha = 0.1875 * (5.0 * fa * fa - 4.0);
+  double mlp, wtl, mrm, wtr;
+  if (mla <= 1.0) {
+    mlp = 0.25 * (ml + 1.0) * (ml + 1.0) + 0.125 * (ml * ml - 1.0) * (ml * ml - 1.0);
+    wtl = 0.25 * (ml + 1.0) * (ml + 1.0) * (2.0 - ml) + alpha * ml * (ml * ml - 1.0) * (ml * ml - 1.0);
+  } else {
+    mlp = 0.5 * (ml + mla);
+    wtl = 0.5 * (1.0 + ml / mla);
+  }
+  if (mra <= 1.0) {
+    mrm = -0.25 * (mr - 1.0) * (mr - 1.0) - 0.125 * (mr * mr - 1.0) * (mr * mr - 1.0);
+    wtr = 0.25 * (mr - 1.0) * (mr - 1.0) * (2.0 + mr) - alpha * mr * (mr * mr - 1.0) * (mr * mr - 1.0);
+  } else {
+    mrm = 0.5 * (mr - mra);
+    wtr = 0.5 * (1.0 - mr / mra);
+  }
+  double rf = 0.5 * (rl + rr);
+  double sigmabar = 1.0 - p->sigma * mbarsq;
+  /* the pressure diffusion term M_p and the velocity diffusion term p_u */
+  double mp = -(p->kp * fmax(sigmabar, 0.0) * (pr - pl)) / (fa * rf * af * af);
+  double pu = -(p->ku * wtl * wtr * fa * af * (rl + rr) * (qr - ql));
+  double mf = mlp + mrm + mp;
+  double mfa = fabs(mf);
+  double mfp = 0.5 * (mf + mfa), mfm = 0.5 * (mf - mfa);
+  double pf = wtl * pl + wtr * pr + pu;
+  double av = af * vFracF;
+  flx[0] = (av * (mfp * rl + mfm * rr)) * nm;
+  flx[1] = (av * (mfp * rl * ul + mfm * rr * ur) + pf * nx) * nm;
+  flx[2] = (av * (mfp * rl * vl + mfm * rr * vr) + pf * ny) * nm;
+  flx[3] = (av * (mfp * rl * wl + mfm * rr * wr) + pf * nz) * nm;
+  flx[4] = (av * (mfp * rl * Hl + mfm * rr * Hr) + pf * fs) * nm;
+}
+
 /* Point fluxes for closed-form tests.  L/R = (rho,u,v,w,p); H, e and a are
  * derived with the flux path's perfect-gas relations (refCp, refGamma). */
 void orc_flux_point(const orc_params *p, int discr, const double *n, const double *L, const double *R,
@@ -978,6 +1027,9 @@ void orc_flux_point(const orc_params *p, int discr, const double *n, const doubl
   if (discr == ORC_DISCR_AUSM) {
     ausm_flux_function(n[0], n[1], n[2], n[3], fs, L[0], L[1], L[2], L[3], L[4], Hl, C_GRT(g, gc, tl),
                        R[0], R[1], R[2], R[3], R[4], Hr, C_GRT(g, gc, tr), flx);
+  } else if (discr == ORC_DISCR_AUSMUP) {
+    ausmup_flux_function(p, n[0], n[1], n[2], n[3], fs, L[0], L[1], L[2], L[3], L[4], Hl, C_GRT(g, gc, tl),
+                         R[0], R[1], R[2], R[3], R[4], Hr, C_GRT(g, gc, tr), 1.0, flx);
   } else if (discr == ORC_DISCR_ROE) {
     roe_flux(p, n[0], n[1], n[2], n[3], fs, L[0], L[1], L[2], L[3], L[4], Hl, ql,
              R[0], R[1], R[2], R[3], R[4], Hr, qr, flx);
@@ -1030,6 +1082,12 @@ static void interior_fluxes(orc_region *r) {
         double Hl = q1[4] * irl + pl / rl;
         double Hr = q2[4] * irr + pr / rr;
         ausm_flux_function(nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, flx);
+      } else if (p->spaceDiscr == ORC_DISCR_AUSMUP) {
+        /* RFLU_AUSMPlusUp_ComputeFlux1 (RFLU_ModAUSMPlusUpFlux.F90:894-1366) */
+        double Hl = q1[4] * irl + pl / rl;
+        double Hr = q2[4] * irr + pr / rr;
+        ausmup_flux_function(p, nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, 1.0,
+                             flx);
       } else if (p->spaceDiscr == ORC_DISCR_ROE) {
         /* RFLU_ModRoeFlux.F90:1723 (first order TCP) */
         double Hl = Ho_CpTUVW(cp, tl, ul, vl, wl, ksg);
@@ -1049,7 +1107,22 @@ static void interior_fluxes(orc_region *r) {
       double rl, ul, vl, wl, pl, rr, ur, vr, wr, pr;
       reconstruct(r, c1, xc, yc, zc, &rl, &ul, &vl, &wl, &pl);
       reconstruct(r, c2, xc, yc, zc, &rr, &ur, &vr, &wr, &pr);
-      if (p->spaceDiscr == ORC_DISCR_AUSM) {
+      if (p->spaceDiscr == ORC_DISCR_AUSMUP) {
+        /* RFLU_AUSMPlusUp_ComputeFlux2_TCP (RFLU_ModAUSMPlusUpFlux.F90:3255-3632):
+         * with particles the reconstructed density is divided by the
+         * cell's gas fraction before T, a and H, and the face's gas fraction
+         * (the mean of the two cells') scales the convected flux */
+        double vFracF = 1.0;
+        if (r->phi) {
+          rl = rl / (1.0 - r->phi[c1]);
+          rr = rr / (1.0 - r->phi[c2]);
+          vFracF = 0.5 * ((1.0 - r->phi[c1]) + (1.0 - r->phi[c2]));
+        }
+        double tl = T_DPR(rl, pl, gc), al = C_GRT(g, gc, tl), Hl = Ho_CpTUVW(cp, tl, ul, vl, wl, ksg);
+        double tr = T_DPR(rr, pr, gc), ar = C_GRT(g, gc, tr), Hr = Ho_CpTUVW(cp, tr, ur, vr, wr, ksg);
+        ausmup_flux_function(p, nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, vFracF,
+                             flx);
+      } else if (p->spaceDiscr == ORC_DISCR_AUSM) {
         /* RFLU_ModAUSMFlux.F90:2784-2843 */
         double tl = T_DPR(rl, pl, gc), al = C_GRT(g, gc, tl), Hl = Ho_CpTUVW(cp, tl, ul, vl, wl, ksg);
         double tr = T_DPR(rr, pr, gc), ar = C_GRT(g, gc, tr), Hr = Ho_CpTUVW(cp, tr, ur, vr, wr, ksg);

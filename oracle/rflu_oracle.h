@@ -18,15 +18,15 @@
 extern "C" {
 #endif
 
-enum { ORC_DISCR_ROE = 1, ORC_DISCR_HLLC = 3, ORC_DISCR_AUSM = 4 };
+enum { ORC_DISCR_ROE = 1, ORC_DISCR_HLLC = 3, ORC_DISCR_AUSM = 4, ORC_DISCR_AUSMUP = 5 };
 enum { ORC_RECONST_NONE = 0, ORC_RECONST_WENO_XYZ = 2, ORC_RECONST_VENKAT = 11 };
 enum { ORC_BC_SLIPWALL = 1, ORC_BC_OUTFLOW = 2 };
 enum { ORC_BCOPT_SUPERSONIC = 0, ORC_BCOPT_SUBSONIC = 1, ORC_BCOPT_MIXED = 2 };
 
 typedef struct orc_params {
-  int spaceDiscr;     /* 1 Roe, 3 HLLC, 4 AUSM+ */
+  int spaceDiscr;     /* 1 Roe, 3 HLLC, 4 AUSM+, 5 AUSM+up */
   int spaceOrder;     /* 1 or 2 */
-  int reconst;        /* 0 none, 11 Venkatakrishnan */
+  int reconst;        /* 0 none, 2 WENO-XYZ, 11 Venkatakrishnan */
   int patchOrder;     /* 1 or 2: order of all boundary patches (ORDERBF) */
   double cfl;
   double dtImposed;
@@ -39,6 +39,9 @@ typedef struct orc_params {
   int accelOn;
   double accel[3];
   int limiterGuard;   /* 0: reference denom+TINY; 1: TINY with the sign of denom */
+  /* AUSM+up constants (NUMERICS SIGMA, KP, KU, MINF; defaults 1, 0.25, 0.75,
+   * 0.1 in libfloflu/ReadNumericsSection.F90:236-275) */
+  double sigma, kp, ku, mInf;
 } orc_params;
 
 typedef struct orc_region orc_region;

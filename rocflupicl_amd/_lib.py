@@ -48,7 +48,8 @@ class Params(C.Structure):
                 ("dtMinLimit", C.c_double), ("dTimeSystem", C.c_double), ("refCp", C.c_double),
                 ("refGamma", C.c_double), ("epsentr", C.c_double), ("dissFact", C.c_double),
                 ("accelOn", C.c_int), ("accel", C.c_double * 3), ("limiterGuard", C.c_int),
-                ("fluxPrecision", C.c_int)]
+                ("fluxPrecision", C.c_int), ("sigma", C.c_double), ("kp", C.c_double), ("ku", C.c_double),
+                ("mInf", C.c_double)]
 
 
 class PiclParams(C.Structure):

@@ -43,6 +43,7 @@ struct Gas {
   double cp, g, gc;   // flux path: refCp, refGamma, R = cp - cp/g (MixtPerf_R_CpG)
   double gD, gcD;     // dependent-variable path: R = 8314.3/(8314.3/R), g = cp/(cp-R)
   double epsentr, dissFact;
+  double sigma, kp, ku;  // AUSM+up
   double accel[3];
   int accelOn;
 };
@@ -1202,6 +1203,51 @@ __device__ __forceinline__ void ausm_flux(double nx, double ny, double nz, doubl
   f[4] = (af * (mfp * rl * Hl + mfm * rr * Hr) + pf * fs) * nm;
 }
 
+// RFLU_AUSMPlusUp_FluxFunction2 (modflu/RFLU_ModAUSMPlusUpFlux.F90:469-588),
+// called by both TCP drivers: f_a is overwritten with 1 there, so alpha =
+// 3/16 and M_inf has no effect.  vFracF scales the convected part.
+__device__ __forceinline__ void ausmup_flux(const Gas &G, double nx, double ny, double nz, double nm,
+                                            double rl, double ul, double vl, double wl, double pl, double Hl,
+                                            double al, double rr, double ur, double vr, double wr, double pr,
+                                            double Hr, double ar, double vFracF, double f[5]) {
+  const double fs = 0.0, fa = 1.0;
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double qr = ur * nx + vr * ny + wr * nz - fs;
+  double af = 0.5 * (al + ar);
+  double ml = ql / af, mla = fabs(ml);
+  double mr = qr / af, mra = fabs(mr);
+  double mbarsq = 0.5 * (ml * ml + mr * mr);
+  const double alpha = 0.1875;  // 0.1875 * (5 fa^2 - 4)
+  double mlp, wtl, mrm, wtr;
+  if (mla <= 1.0) {
+    mlp = 0.25 * (ml + 1.0) * (ml + 1.0) + 0.125 * (ml * ml - 1.0) * (ml * ml - 1.0);
+    wtl = 0.25 * (ml + 1.0) * (ml + 1.0) * (2.0 - ml) + alpha * ml * (ml * ml - 1.0) * (ml * ml - 1.0);
+  } else {
+    mlp = 0.5 * (ml + mla);
+    wtl = 0.5 * (1.0 + ml / mla);
+  }
+  if (mra <= 1.0) {
+    mrm = -0.25 * (mr - 1.0) * (mr - 1.0) - 0.125 * (mr * mr - 1.0) * (mr * mr - 1.0);
+    wtr = 0.25 * (mr - 1.0) * (mr - 1.0) * (2.0 + mr) - alpha * mr * (mr * mr - 1.0) * (mr * mr - 1.0);
+  } else {
+    mrm = 0.5 * (mr - mra);
+    wtr = 0.5 * (1.0 - mr / mra);
+  }
+  double rf = 0.5 * (rl + rr);
+  double sigmabar = 1.0 - G.sigma * mbarsq;
+  double mp = -(G.kp * fmax(sigmabar, 0.0) * (pr - pl)) / (fa * rf * af * af);  // pressure diffusion
+  double pu = -(G.ku * wtl * wtr * fa * af * (rl + rr) * (qr - ql));            // velocity diffusion
+  double mf = mlp + mrm + mp, mfa = fabs(mf);
+  double mfp = 0.5 * (mf + mfa), mfm = 0.5 * (mf - mfa);
+  double pf = wtl * pl + wtr * pr + pu;
+  double av = af * vFracF;
+  f[0] = (av * (mfp * rl + mfm * rr)) * nm;
+  f[1] = (av * (mfp * rl * ul + mfm * rr * ur) + pf * nx) * nm;
+  f[2] = (av * (mfp * rl * vl + mfm * rr * vr) + pf * ny) * nm;
+  f[3] = (av * (mfp * rl * wl + mfm * rr * wr) + pf * nz) * nm;
+  f[4] = (av * (mfp * rl * Hl + mfm * rr * Hr) + pf * fs) * nm;
+}
+
 // modflu/RFLU_ModRoeFlux.F90:2783-2855 (+ RFLU_ModEntropyFixes.F90:95-106)
 __device__ __forceinline__ double hh_fix(double l, double dd) { return (l > dd) ? l : (l * l + dd * dd) / (2.0 * dd); }
 __device__ __forceinline__ void roe_flux(const Gas &G, double nx, double ny, double nz, double nm,
@@ -1526,6 +1572,9 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
       if (P.discr == RFLU_DISCR_AUSM) {
         double Hl = L[4] * irl + pl / rl, Hr = R[4] * irr + pr / rr;
         ausm_flux(nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, f);
+      } else if (P.discr == RFLU_DISCR_AUSMUP) {  // RFLU_AUSMPlusUp_ComputeFlux1
+        double Hl = L[4] * irl + pl / rl, Hr = R[4] * irr + pr / rr;
+        ausmup_flux(G, nx, ny, nz, nm, rl, ul, vl, wl, pl, Hl, al, rr, ur, vr, wr, pr, Hr, ar, 1.0, f);
       } else if (P.discr == RFLU_DISCR_ROE) {
         double Hl = G.cp * tl + 0.5 * (ul * ul + vl * vl + wl * wl) + ksg;
         double Hr = G.cp * tr + 0.5 * (ur * ur + vr * vr + wr * wr) + ksg;
@@ -1573,7 +1622,22 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
       double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
       FaceState A = reconstruct(L, d.grad + 15 * (size_t)c1, xc, yc, zc, rpL);
       FaceState B = reconstruct(R, d.grad + 15 * (size_t)c2, xc, yc, zc, rpR);
-      if (P.discr == RFLU_DISCR_AUSM) {
+      if (P.discr == RFLU_DISCR_AUSMUP) {
+        // RFLU_AUSMPlusUp_ComputeFlux2_TCP: with particles the reconstructed
+        // density is divided by the cell's gas fraction before T, a and H, and
+        // the face's mean gas fraction scales the convected flux
+        double rl = A.r, rr = B.r, vFracF = 1.0;
+        if (d.phi) {
+          rl = rl / (1.0 - phL);
+          rr = rr / (1.0 - phR);
+          vFracF = 0.5 * ((1.0 - phL) + (1.0 - phR));
+        }
+        double tl = A.p / (rl * G.gc), al = sqrt(G.g * G.gc * tl);
+        double Hl = G.cp * tl + 0.5 * (A.u * A.u + A.v * A.v + A.w * A.w) + ksg;
+        double tr = B.p / (rr * G.gc), ar = sqrt(G.g * G.gc * tr);
+        double Hr = G.cp * tr + 0.5 * (B.u * B.u + B.v * B.v + B.w * B.w) + ksg;
+        ausmup_flux(G, nx, ny, nz, nm, rl, A.u, A.v, A.w, A.p, Hl, al, rr, B.u, B.v, B.w, B.p, Hr, ar, vFracF, f);
+      } else if (P.discr == RFLU_DISCR_AUSM) {
         double tl = A.p / (A.r * G.gc), al = sqrt(G.g * G.gc * tl);
         double Hl = G.cp * tl + 0.5 * (A.u * A.u + A.v * A.v + A.w * A.w) + ksg;
         double tr = B.p / (B.r * G.gc), ar = sqrt(G.g * G.gc * tr);
@@ -2078,8 +2142,11 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, RFLU_GPU_ENODEV, "no HIP device available (the library has no CPU path)");
   if (prm->spaceOrder != 1 && prm->spaceOrder != 2) return set_err(nullptr, RFLU_GPU_EINVAL, "spaceOrder must be 1 or 2");
-  if (prm->spaceDiscr != RFLU_DISCR_ROE && prm->spaceDiscr != RFLU_DISCR_HLLC && prm->spaceDiscr != RFLU_DISCR_AUSM)
-    return set_err(nullptr, RFLU_GPU_EINVAL, "spaceDiscr must be 1 (Roe), 3 (HLLC) or 4 (AUSM+)");
+  if (prm->spaceDiscr != RFLU_DISCR_ROE && prm->spaceDiscr != RFLU_DISCR_HLLC && prm->spaceDiscr != RFLU_DISCR_AUSM &&
+      prm->spaceDiscr != RFLU_DISCR_AUSMUP)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "spaceDiscr must be 1 (Roe), 3 (HLLC), 4 (AUSM+) or 5 (AUSM+up)");
+  if (prm->fluxPrecision == 1 && prm->spaceDiscr == RFLU_DISCR_AUSMUP)
+    return set_err(nullptr, RFLU_GPU_EINVAL, "fluxPrecision 1 supports spaceDiscr 1, 3 and 4 only");
   if (prm->patchOrder == 2 && prm->spaceOrder != 2)
     return set_err(nullptr, RFLU_GPU_EINVAL, "second-order patches need spaceOrder 2");
   if (prm->reconst != RFLU_RECONST_NONE && prm->reconst != RFLU_RECONST_WENO_XYZ &&
@@ -2119,6 +2186,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   G.gcD = 8314.3 / mol;                // MixtPerf_R_M
   G.gD = G.cp / (G.cp - G.gcD);        // MixtPerf_G_CpR
   G.epsentr = prm->epsentr; G.dissFact = prm->dissFact; G.accelOn = prm->accelOn;
+  G.sigma = prm->sigma; G.kp = prm->kp; G.ku = prm->ku;
   for (int q = 0; q < 3; q++) G.accel[q] = prm->accel[q];
   Dev &d = h->d;
   d.nC = m.nCells; d.nF = m.nFaces; d.nB = m.nBFaces; d.S = m.nFaces + m.nBFaces; d.nBlk = nblk(d.nC);

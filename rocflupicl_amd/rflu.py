@@ -13,13 +13,14 @@ import numpy as np
 from . import _lib
 from .meshgen import BC_OUTFLOW, BC_SLIPWALL  # noqa: F401  (re-exported)
 
-DISCR_ROE, DISCR_HLLC, DISCR_AUSM = 1, 3, 4
+DISCR_ROE, DISCR_HLLC, DISCR_AUSM, DISCR_AUSMUP = 1, 3, 4, 5
 RECONST_NONE, RECONST_WENO_XYZ, RECONST_VENKAT = 0, 2, 11
 
 DEFAULTS = dict(spaceDiscr=DISCR_AUSM, spaceOrder=1, reconst=RECONST_NONE, patchOrder=1,
                 cfl=0.5, dtImposed=1.0e-5, dtMinLimit=1.0e-10, dTimeSystem=0.0,
                 refCp=1004.5, refGamma=1.4, epsentr=0.05, dissFact=1.0, accelOn=0,
-                accel=(0.0, 0.0, 0.0), limiterGuard=0, fluxPrecision=0)
+                accel=(0.0, 0.0, 0.0), limiterGuard=0, fluxPrecision=0, sigma=1.0, kp=0.25, ku=0.75,
+                mInf=0.1)
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,

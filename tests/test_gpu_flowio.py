@@ -26,7 +26,10 @@ def test_resume_from_flow_file(tmp_path, binary):
     name = b.write_flow("shock", str(tmp_path), binary=binary)
     c = rflu.Region(mesh, **params)
     t = c.read_flow(name)
-    assert t == b.time()[0]
+    if binary:
+        assert t == b.time()[0]
+    else:  # E23.16 keeps 16 significant digits
+        assert t == pytest.approx(b.time()[0], rel=1e-15)
     c.RFLU_TimeStepping(4)
     ca, cc = a.get_state()[0], c.get_state()[0]
     if binary:

@@ -233,11 +233,14 @@ def test_gaussian_graph_replays_equal_eager_steps(oracle_mod):
         assert same_bits(d.picl_nodes(), e.picl_nodes())
 
 
-def test_coupled_steps_with_weno_reconstruction(oracle_mod):
+@pytest.mark.parametrize("discr", [4, 5])
+def test_coupled_steps_with_weno_reconstruction(oracle_mod, discr):
     """The gas side with WENO-XYZ reconstruction (the deck default) under the
-    particle coupling."""
+    particle coupling, with AUSM+ and with AUSM+up (whose particle branch
+    divides the face density by the gas fraction and scales the convected
+    flux by the face's gas fraction)."""
     mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=500, jitter=0.05, seed=4,
-                                                      reconst=2)
+                                                      reconst=2, discr=discr)
     g = rflu.Region(mesh, **params)
     o = oracle_mod.Region(mesh, **params)
     cv = ic(g.mesh_arrays()["cofg"])

@@ -81,7 +81,7 @@ def _shock_ic(cofg, gamma=1.4, x0=None):
     return mg.prim_to_cons(rho, u, v, w, p, gamma).T.copy()
 
 
-CASES = [(k, d, o) for k in ("hex", "tet") for d in (1, 3, 4) for o in (1, 2)]
+CASES = [(k, d, o) for k in ("hex", "tet") for d in (1, 3, 4, 5) for o in (1, 2)]
 
 
 @pytest.mark.parametrize("kind,discr,order", CASES)
@@ -110,7 +110,8 @@ def test_timestep_and_residual_match_oracle(kind, discr, order, oracle_mod):
 
 
 @pytest.mark.parametrize("kind,discr,order", [("hex", 4, 1), ("hex", 1, 2), ("tet", 1, 2),
-                                              ("tet", 3, 2), ("tet", 4, 2), ("hex", 3, 1)])
+                                              ("tet", 3, 2), ("tet", 4, 2), ("hex", 3, 1),
+                                              ("hex", 5, 1), ("tet", 5, 2)])
 def test_steps_match_oracle(kind, discr, order, oracle_mod):
     mesh = MESHES[kind]
     prm = dict(spaceDiscr=discr, spaceOrder=order, reconst=11 if order == 2 else 0,
@@ -241,7 +242,7 @@ def test_gradient_kernel_variants_match_oracle(variant, oracle_mod, monkeypatch)
 
 
 # ---- WENO-XYZ reconstruction (reconst 2, the shipped deck's default) ----
-@pytest.mark.parametrize("kind,discr", [("hex", 1), ("tet", 1), ("tet", 3), ("tet", 4)])
+@pytest.mark.parametrize("kind,discr", [("hex", 1), ("tet", 1), ("tet", 3), ("tet", 4), ("tet", 5), ("hex", 5)])
 @pytest.mark.parametrize("variant", ["4", "2"])
 def test_weno_xyz_matches_oracle(kind, discr, variant, oracle_mod, monkeypatch):
     """LSQ gradients, the WENO-XYZ stencil average and the rho/p positivity
@@ -294,6 +295,29 @@ def test_weno_positivity_limiter_matches_oracle(variant, oracle_mod, monkeypatch
     assert np.all(gg[:, 0, :] == 0.0, axis=1).any() and np.all(gg[:, 4, :] == 0.0, axis=1).any()
 
 
-def test_weno_rejected_in_fp32_mode():
+def test_weno_and_ausmup_rejected_in_fp32_mode():
     with pytest.raises(Exception):
         rflu.Region(MESHES["tet"], spaceDiscr=1, spaceOrder=2, reconst=2, fluxPrecision=1)
+    with pytest.raises(Exception):
+        rflu.Region(MESHES["tet"], spaceDiscr=5, spaceOrder=2, reconst=11, fluxPrecision=1)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_ausmup_constants_reach_the_flux(order, oracle_mod):
+    """Non-default SIGMA, KP, KU (AUSM+up's low-Mach terms) on a shock with
+    a subsonic tail, bit-identical over 10 steps; the default run differs."""
+    prm = dict(spaceDiscr=5, spaceOrder=order, reconst=11 if order == 2 else 0, refCp=1004.64, refGamma=1.4,
+               dtImposed=2e-8, cfl=0.8)
+    mesh = MESHES["tet"]
+    res = []
+    for kw in (dict(sigma=0.6, kp=0.5, ku=0.3), {}):
+        g, o, cofg = _pair(mesh, **prm, **kw)
+        cv = _shock_ic(cofg)
+        g.set_state(cv)
+        o.set_cv(cv)
+        g.RFLU_TimeStepping(10)
+        o.step(10)
+        gcv, _ = g.get_state()
+        assert same_bits(gcv, o.cv()), (kw, first_diff(gcv, o.cv()))
+        res.append(gcv)
+    assert not np.array_equal(res[0], res[1])

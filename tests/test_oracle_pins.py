@@ -36,7 +36,7 @@ def rand_normal(rng):
     return np.array([*(n / np.linalg.norm(n)), rng.uniform(0.1, 3.0)])
 
 
-@pytest.mark.parametrize("discr", [1, 3, 4])
+@pytest.mark.parametrize("discr", [1, 3, 4, 5])
 @pytest.mark.parametrize("mach", [0.0, 0.3, 0.95, 1.7, 3.0])
 def test_flux_consistency_F_U_U_is_euler_flux(discr, mach, oracle_mod):
     rng = np.random.default_rng(100 + discr + int(10 * mach))
@@ -49,7 +49,7 @@ def test_flux_consistency_F_U_U_is_euler_flux(discr, mach, oracle_mod):
         assert np.abs(F - E).max() <= 1e-13 * scale, (F, E)
 
 
-@pytest.mark.parametrize("discr", [1, 3, 4])
+@pytest.mark.parametrize("discr", [1, 3, 4, 5])
 def test_supersonic_flux_is_fully_upwind(discr, oracle_mod):
     """Both states supersonic along +n: every scheme returns F(U_L)."""
     rng = np.random.default_rng(7)
@@ -72,7 +72,7 @@ def test_flux_distinguishes_left_and_right(oracle_mod):
     Roe is tested without the entropy fix: its threshold eps*|q+a| is not
     symmetric under the swap, which is the reference's own formulation."""
     rng = np.random.default_rng(3)
-    for discr in (1, 3, 4):
+    for discr in (1, 3, 4, 5):
         for _ in range(10):
             L, R = rand_state(rng, 0.6), rand_state(rng, 0.4)
             n = rand_normal(rng)
@@ -444,3 +444,45 @@ def test_weno_xyz_positivity_zeroes_rho_and_p_gradients(oracle_mod):
         assert zeroed.sum() >= 1 and (worst < -1e-9).sum() >= 1, var
         # the mirrored component itself is not limited
         assert not np.all(g2[zeroed, mirror, :] == 0.0)
+
+
+# ---- AUSM+up (DISCR 5): Liou's low-Mach diffusion terms ----
+def test_ausmup_pressure_diffusion_drives_mass_flux_at_rest(oracle_mod):
+    """At rest with a pressure jump, AUSM+ moves no mass; AUSM+up's M_p =
+    -K_p max(1 - sigma Mbar^2, 0) (p_R - p_L) / (rho_1/2 a^2) gives the mass
+    flux K_p (p_L - p_R) / a (equal densities), and the pressure flux is the
+    mean pressure."""
+    n = np.array([1.0, 0.0, 0.0, 2.0])
+    for kp in (0.25, 0.6):
+        L = np.array([1.2, 0.0, 0.0, 0.0, 1.1e5])
+        R = np.array([1.2, 0.0, 0.0, 0.0, 0.9e5])
+        gc = CP - CP / GAMMA
+        a = 0.5 * (np.sqrt(GAMMA * gc * L[4] / (L[0] * gc)) + np.sqrt(GAMMA * gc * R[4] / (R[0] * gc)))
+        F = oracle_mod.flux_point(5, n, L, R, refCp=CP, refGamma=GAMMA, kp=kp)
+        assert F[0] == pytest.approx(kp * (L[4] - R[4]) / a * n[3], rel=1e-12)
+        assert F[1] == pytest.approx(0.5 * (L[4] + R[4]) * n[3], rel=1e-12)
+        assert oracle_mod.flux_point(4, n, L, R, refCp=CP, refGamma=GAMMA)[0] == 0.0
+        # sigma Mbar^2 >= 1 switches the term off (and p_u vanishes for u_L = u_R)
+        Ls, Rs = L.copy(), R.copy()
+        Ls[1] = Rs[1] = 0.9 * a
+        Fs = oracle_mod.flux_point(5, n, Ls, Rs, refCp=CP, refGamma=GAMMA, kp=kp, sigma=1.0 / 0.8)
+        F4 = oracle_mod.flux_point(4, n, Ls, Rs, refCp=CP, refGamma=GAMMA)
+        assert np.allclose(Fs, F4, rtol=1e-13, atol=0)
+
+
+def test_ausmup_velocity_diffusion_is_ku_rho_a_du(oracle_mod):
+    """p_u = -K_u P+ P- (rho_L + rho_R) a (u_R - u_L) -> K_u rho a delta for a
+    weak compression u_L = -u_R = delta (P+ P- -> 1/4); it enters the
+    pressure flux only, so the K_u difference of the normal momentum flux
+    is p_u |S|."""
+    n = np.array([1.0, 0.0, 0.0, 1.5])
+    rho, p = 1.2, 1.0e5
+    a = np.sqrt(GAMMA * p / rho)
+    d = 1e-4 * a
+    L = np.array([rho, d, 0.0, 0.0, p])
+    R = np.array([rho, -d, 0.0, 0.0, p])
+    F0 = oracle_mod.flux_point(5, n, L, R, refCp=CP, refGamma=GAMMA, ku=0.0)
+    for ku in (0.75, 0.3):
+        F = oracle_mod.flux_point(5, n, L, R, refCp=CP, refGamma=GAMMA, ku=ku)
+        assert F[0] == F0[0]
+        assert (F[1] - F0[1]) / n[3] == pytest.approx(ku * rho * a * d, rel=1e-3)
