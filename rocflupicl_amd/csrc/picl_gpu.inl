@@ -217,12 +217,13 @@ __global__ void __launch_bounds__(BLK) k_rhog(Dev d, PiclDev P) {
 // w*g per component, formed once per slot in LDS instead of once per
 // stencil appearance; each cell then adds them in stencil order.  Same
 // operations on the same values, same bits as k_rhog.
-__global__ void __launch_bounds__(BLK) k_rhog_tile(Dev d, PiclDev P) {
+template <int TB>
+__global__ void __launch_bounds__(TB) k_rhog_tile(Dev d, PiclDev P) {
   extern __shared__ double2 sW[];  // [nU][3]: (w0, w0*g0), (w1, w1*g1), (w2, w2*g2)
   int tile = xcd_block();
   int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
   const double2 *gr = reinterpret_cast<const double2 *>(d.gRaw);
-  for (int u = threadIdx.x; u < nU; u += BLK) {
+  for (int u = threadIdx.x; u < nU; u += TB) {
     size_t c = d.tMem[u0 + u];
     double2 a = gr[2 * c], b = gr[2 * c + 1];
     double t0 = a.x, t1 = a.y, t2 = b.x;
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(BLK) k_rhog_tile(Dev d, PiclDev P) {
     sW[3 * u + 2] = make_double2(w2, w2 * t2);
   }
   __syncthreads();
-  int i = tile * BLK + threadIdx.x;
+  int i = tile * TB + threadIdx.x;
   if (i >= d.nC) return;
   double2 o0 = sW[3 * threadIdx.x], o1 = sW[3 * threadIdx.x + 1], o2 = sW[3 * threadIdx.x + 2];
   double s0 = o0.x, s1 = o1.x, s2 = o2.x, l0 = o0.y, l1 = o1.y, l2 = o2.y;
@@ -1199,7 +1200,7 @@ int launch_rhog(rflu_handle *h) {
   ProfScope ps(h, "k_rhog");
   size_t lds = h->tileLds / 64 * 48;
   if (h->gradVariant == 4 && h->tileLds && lds <= h->ldsMax)
-    hipLaunchKernelGGL(k_rhog_tile, dim3(h->d.nBlk), dim3(BLK), lds, h->stream, h->d, h->picl->P);
+    TILE_LAUNCH(k_rhog_tile, lds, h->d, h->picl->P);
   else
     hipLaunchKernelGGL(k_rhog, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->picl->P);
   return 0;

@@ -36,6 +36,7 @@
 #include "rflu_mesh.hpp"
 
 #define BLK 256
+#define TBIG 768  // cells per gradient tile when the tile records fit one workgroup's LDS
 
 namespace {
 
@@ -68,6 +69,7 @@ struct Dev {
   // 15 marking face neighbours
   int *tOff = nullptr, *tMem = nullptr;
   unsigned short *sSlot = nullptr;
+  unsigned short *cfSlot = nullptr;  // per interior cell-face entry: the LDS slot of the face neighbour
   // FP32 evaluation mode: limited gradients as 16 floats per cell (15 + pad)
   // and the interior faces' fluxes as 5 floats per face (boundary faces keep
   // their FP64 fluxes in flux[])
@@ -103,6 +105,8 @@ struct rflu_handle {
   hipGraphExec_t graph = nullptr;
   int gradVariant = 4;  // 4: LDS tiles (default when they fit), 2: one thread per cell, 0: plain, 1/3: cooperative
   size_t tileLds = 0;   // LDS bytes per gradient tile (0: tiles not built)
+  int tileCells = BLK;  // cells per gradient tile (= threads per tile workgroup)
+  int nTiles = 0;
   size_t ldsMax = 65536;
   std::vector<int> perm, inv;  // device cell i holds reference cell perm[i]
   std::vector<double> stage;   // host staging for (un)permuting cell arrays
@@ -579,30 +583,41 @@ __global__ void __launch_bounds__(BLK, MINW) k_grad2(Dev d, int venkat, int guar
 #ifndef GU
 #define GU 4
 #endif
+// LDS records of the gradient tiles, one 64-byte record per slot.  (Rotating
+// each slot's 16-byte chunks by (u >> 2) & 3 cut the LDS bank conflicts of
+// the member reads 2.7x but added 12 % VALU instructions to a VALU-bound
+// loop: measured 2 % slower.)
+__device__ __forceinline__ void lds_rec_store(double2 *s, int u, double2 a, double2 b, double2 c, double2 e) {
+  s[4 * u] = a;
+  s[4 * u + 1] = b;
+  s[4 * u + 2] = c;
+  s[4 * u + 3] = e;
+}
+__device__ __forceinline__ void lds_rec_load(const double2 *s, int u, double q[8]) {
+  double2 a = s[4 * u], b = s[4 * u + 1], c = s[4 * u + 2], e = s[4 * u + 3];
+  q[0] = a.x; q[1] = a.y; q[2] = b.x; q[3] = b.y; q[4] = c.x; q[5] = c.y; q[6] = e.x; q[7] = e.y;
+}
+
 // PICL = 1 also stages rho_g = rho/(1-phi) per slot (after the records) and
 // forms its LSQ gradient (CellGradientsMP.F90:375-378), as k_grad2<.,1>.
-template <int PICL>
-__global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard, int nUmax) {
+template <int PICL, int TB>
+__device__ __forceinline__ void grad_tile_body(Dev d, int venkat, int guard, int nUmax) {
   extern __shared__ double2 sRec[];  // [nU][4]: one 64-byte record per slot; then rho_g[nUmax]
   double *sQ = reinterpret_cast<double *>(sRec + 4 * (size_t)nUmax);
   int tile = xcd_block();
   int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
-  for (int u = threadIdx.x; u < nU; u += BLK) {
+  for (int u = threadIdx.x; u < nU; u += TB) {
     int c0 = d.tMem[u0 + u];
     const double2 *g = reinterpret_cast<const double2 *>(d.rec + 8 * (size_t)c0);
-    double2 a = g[0], b = g[1], c = g[2], e = g[3];
-    sRec[4 * u] = a;
-    sRec[4 * u + 1] = b;
-    sRec[4 * u + 2] = c;
-    sRec[4 * u + 3] = e;
+    lds_rec_store(sRec, u, g[0], g[1], g[2], g[3]);
     if (PICL) sQ[u] = d.rhog[c0];
   }
   __syncthreads();
-  int i = tile * BLK + threadIdx.x;
+  int i = tile * TB + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
   double ri[8];
-  load_rec(reinterpret_cast<const double *>(sRec), threadIdx.x, ri);
+  lds_rec_load(sRec, threadIdx.x, ri);
   double r11 = d.rm[i], r12 = d.rm[N + i], r22 = d.rm[2 * N + i];
   double r13 = d.rm[3 * N + i], r23 = d.rm[4 * N + i], r33 = d.rm[5 * N + i];
   double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
@@ -615,7 +630,6 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
   for (int q = 0; q < 15; q++) g[q] = 0.0;
 #pragma unroll
   for (int k = 0; k < 5; k++) { vmin[k] = ri[k]; vmax[k] = ri[k]; }
-  const double *S = reinterpret_cast<const double *>(sRec);
   int sb = d.csOff[i], se = d.csOff[i + 1];
   // GU members per iteration: their weights (sqrt, division) are computed
   // independently, then added in stencil order
@@ -628,7 +642,7 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
 #pragma unroll
     for (int u = 0; u < GU; u++) {
       jj[u] = d.sSlot[s + u];
-      load_rec(S, (int)(jj[u] & 0x7fffu), rr[u]);
+      lds_rec_load(sRec, (int)(jj[u] & 0x7fffu), rr[u]);
     }
 #pragma unroll
     for (int u = 0; u < GU; u++) {
@@ -657,16 +671,12 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
         gq1 = gq1 + ww[u][1] * dq;
         gq2 = gq2 + ww[u][2] * dq;
       }
-      if (jj[u] & 0x8000u) {
-#pragma unroll
-        for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rr[u][k]); vmax[k] = fmax(vmax[k], rr[u][k]); }
-      }
     }
   }
   for (; s < se; s++) {
     unsigned j = d.sSlot[s];
     double rj[8];
-    load_rec(S, (int)(j & 0x7fffu), rj);
+    lds_rec_load(sRec, (int)(j & 0x7fffu), rj);
     double dx = rj[5] - ri[5], dy = rj[6] - ri[6], dz = rj[7] - ri[7];
     double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
     dx = term * dx; dy = term * dy; dz = term * dz;
@@ -689,10 +699,6 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
       gq1 = gq1 + wty * dq;
       gq2 = gq2 + wtz * dq;
     }
-    if (j & 0x8000u) {
-#pragma unroll
-      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rj[k]); vmax[k] = fmax(vmax[k], rj[k]); }
-    }
   }
   if (venkat) {
     const double TINYP = 1.0e-12, TINYN = guard ? -1.0e-12 : 1.0e-12;
@@ -701,6 +707,14 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
     for (int k = 0; k < 5; k++) lim[k] = 1.0;
     double epsq = d.epsq[i];
     int eb = d.cfOff[i], ee = eb + d.cfNInt[i];
+    // min/max over the face neighbours (RFLU_ModLimiters.F90:428-438; order-free),
+    // their records read from the tile's LDS slots
+    for (int e = eb; e < ee; e++) {
+      double rn[8];
+      lds_rec_load(sRec, (int)d.cfSlot[e], rn);
+#pragma unroll
+      for (int k = 0; k < 5; k++) { vmin[k] = fmin(vmin[k], rn[k]); vmax[k] = fmax(vmax[k], rn[k]); }
+    }
     for (int e = eb; e < ee; e++) {
       double dx1 = d.cfdx[e], dy1 = d.cfdy[e], dz1 = d.cfdz[e];
 #pragma unroll
@@ -735,6 +749,14 @@ __global__ void __launch_bounds__(BLK) k_grad_tile(Dev d, int venkat, int guard,
     gr[1] = make_double2(gq2, 0.0);
   }
 }
+template <int TB>
+__global__ void __launch_bounds__(TB) k_grad_tile_gas(Dev d, int venkat, int guard, int nUmax) {
+  grad_tile_body<0, TB>(d, venkat, guard, nUmax);
+}
+template <int TB>
+__global__ void __launch_bounds__(TB) k_grad_tile_picl(Dev d, int venkat, int guard, int nUmax) {
+  grad_tile_body<1, TB>(d, venkat, guard, nUmax);
+}
 
 // FP32 evaluation of the same gradient (fluxPrecision 1): the tile's records
 // are staged in LDS with the state in FP32 and the centroid in FP64 (48 bytes
@@ -747,11 +769,12 @@ struct __align__(16) SlotF {
   double x;
   double2 yz;
 };
-__global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int guard) {
+template <int TB>
+__global__ void __launch_bounds__(TB) k_grad_tile_f32(Dev d, int venkat, int guard) {
   extern __shared__ SlotF sRecF[];
   int tile = xcd_block();
   int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
-  for (int u = threadIdx.x; u < nU; u += BLK) {
+  for (int u = threadIdx.x; u < nU; u += TB) {
     double r[8];
     load_rec(d.rec, d.tMem[u0 + u], r);
     SlotF t;
@@ -763,7 +786,7 @@ __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int gu
     sRecF[u] = t;
   }
   __syncthreads();
-  int i = tile * BLK + threadIdx.x;
+  int i = tile * TB + threadIdx.x;
   if (i >= d.nC) return;
   int N = d.nC;
   const SlotF own = sRecF[threadIdx.x];
@@ -797,13 +820,11 @@ __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int gu
     float wty = term * (term2 + c23 * term3);
     float wtz = term * term3;
     float dv[5] = {a.x - ri[0], a.y - ri[1], a.z - ri[2], a.w - ri[3], bp - ri[4]};
-    bool face = (j & 0x8000u) != 0;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       g[3 * k] += wtx * dv[k];
       g[3 * k + 1] += wty * dv[k];
       g[3 * k + 2] += wtz * dv[k];
-      if (face) { dmin[k] = fminf(dmin[k], dv[k]); dmax[k] = fmaxf(dmax[k], dv[k]); }
     }
   };
   for (; s + GU - 1 < se; s += GU) {
@@ -821,6 +842,12 @@ __global__ void __launch_bounds__(BLK) k_grad_tile_f32(Dev d, int venkat, int gu
     for (int k = 0; k < 5; k++) lim[k] = 1.0f;
     float epsq = (float)d.epsq[i];
     int eb = d.cfOff[i], ee = eb + d.cfNInt[i];
+    for (int e = eb; e < ee; e++) {  // face neighbours' differences, from the tile's LDS slots
+      const SlotF &m = sRecF[d.cfSlot[e]];
+      float dv[5] = {m.s.x - ri[0], m.s.y - ri[1], m.s.z - ri[2], m.s.w - ri[3], m.p - ri[4]};
+#pragma unroll
+      for (int k = 0; k < 5; k++) { dmin[k] = fminf(dmin[k], dv[k]); dmax[k] = fmaxf(dmax[k], dv[k]); }
+    }
     for (int e = eb; e < ee; e++) {
       float dx1 = (float)d.cfdx[e], dy1 = (float)d.cfdy[e], dz1 = (float)d.cfdz[e];
 #pragma unroll
@@ -876,17 +903,18 @@ __device__ __forceinline__ void weno_positivity_store(const Dev &d, int i, doubl
 // extrapolation to any face centroid of the cell is <= 0).  Over the
 // gradient tiles: for each variable the tile's slots carry w and w*g per
 // component in LDS, formed once per slot.
-__global__ void __launch_bounds__(BLK) k_weno_tile(Dev d) {
+template <int TB>
+__global__ void __launch_bounds__(TB) k_weno_tile(Dev d) {
   extern __shared__ double2 sW[];  // [nU][3]
   int tile = xcd_block();
   int u0 = d.tOff[tile], nU = d.tOff[tile + 1] - u0;
-  int i = tile * BLK + threadIdx.x;
+  int i = tile * TB + threadIdx.x;
   bool live = i < d.nC;
   int sb = live ? d.csOff[i] : 0, se = live ? d.csOff[i + 1] : 0;
   double g[15];
   for (int k = 0; k < 5; k++) {
     if (k) __syncthreads();  // the previous variable's slots are consumed
-    for (int u = threadIdx.x; u < nU; u += BLK) {
+    for (int u = threadIdx.x; u < nU; u += TB) {
       const double *gr = d.gradRaw + 15 * (size_t)d.tMem[u0 + u] + 3 * k;
 #pragma unroll
       for (int q = 0; q < 3; q++) {
@@ -1924,6 +1952,61 @@ void morton_order(const double *xyz, int n, std::vector<int> &order) {
   for (int i = 0; i < n; i++) order[i] = key[i].second;
 }
 
+// Gradient tiles of T consecutive (Morton-ordered) cells: per tile the sorted
+// union of its cells' stencil members (its own cells first, as slots 0..T-1),
+// each stencil entry rewritten as a 16-bit LDS slot (bit 15: the member is
+// also a face neighbour), and each interior cell-face entry's neighbour as a
+// slot.  Fails when a tile has more than 0x7fff slots or a face neighbour is
+// not a stencil member.
+bool build_tiles(int T, int N, const std::vector<int> &csOff, const std::vector<int> &csMem,
+                 const std::vector<int> &cfOff, const std::vector<int> &cfNInt, const std::vector<int> &cfNb,
+                 std::vector<int> &tOff, std::vector<int> &tMem, std::vector<unsigned short> &sSlot,
+                 std::vector<unsigned short> &cfSlot, int &nUmax) {
+  int nT = (N + T - 1) / T;
+  nUmax = 0;
+  tOff.assign(nT + 1, 0);
+  tMem.clear();
+  sSlot.assign(csMem.size(), 0);
+  cfSlot.assign(cfNb.size(), 0);
+  std::vector<int> halo;
+  auto slot_of = [&](int j, int c0, int c1) -> int {
+    if (j >= c0 && j < c1) return j - c0;
+    auto it = std::lower_bound(halo.begin(), halo.end(), j);
+    if (it == halo.end() || *it != j) return -1;
+    return (c1 - c0) + (int)(it - halo.begin());
+  };
+  for (int t = 0; t < nT; t++) {
+    int c0 = t * T, c1 = std::min(N, c0 + T);
+    halo.clear();
+    for (int i = c0; i < c1; i++)
+      for (int e = csOff[i]; e < csOff[i + 1]; e++) {
+        int j = csMem[e] & 0x7fffffff;
+        if (j < c0 || j >= c1) halo.push_back(j);
+      }
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+    int nU = (c1 - c0) + (int)halo.size();
+    if (nU > 0x7fff) return false;
+    nUmax = std::max(nUmax, nU);
+    for (int i = c0; i < c1; i++) tMem.push_back(i);
+    tMem.insert(tMem.end(), halo.begin(), halo.end());
+    tOff[t + 1] = (int)tMem.size();
+    for (int i = c0; i < c1; i++) {
+      for (int e = csOff[i]; e < csOff[i + 1]; e++) {
+        unsigned v = (unsigned)csMem[e];
+        int slot = slot_of((int)(v & 0x7fffffffu), c0, c1);
+        sSlot[e] = (unsigned short)(slot | ((v & 0x80000000u) ? 0x8000 : 0));
+      }
+      for (int e = cfOff[i]; e < cfOff[i] + cfNInt[i]; e++) {  // face neighbours, for the limiter's min/max
+        int slot = slot_of(cfNb[e], c0, c1);
+        if (slot < 0) return false;
+        cfSlot[e] = (unsigned short)slot;
+      }
+    }
+  }
+  return true;
+}
+
 int set_err(rflu_handle *h, int code, const std::string &m) {
   if (h) h->err = m; else g_err = m;
   return code;
@@ -1999,6 +2082,16 @@ int launch_dtmin(rflu_handle *h, int finalize, int storeCell) {
 
 int launch_grad_impl(rflu_handle *h, const Dev &dv, int venkat, int guard);
 
+// Launch a tile kernel template K<TB> over the gradient tiles, one workgroup
+// of tileCells threads per tile.
+#define TILE_LAUNCH(K, lds, ...)                                                                          \
+  do {                                                                                                    \
+    if (h->tileCells == TBIG)                                                                             \
+      hipLaunchKernelGGL((K<TBIG>), dim3(h->nTiles), dim3(TBIG), (lds), h->stream, __VA_ARGS__);          \
+    else                                                                                                  \
+      hipLaunchKernelGGL((K<BLK>), dim3(h->nTiles), dim3(BLK), (lds), h->stream, __VA_ARGS__);            \
+  } while (0)
+
 int launch_grad(rflu_handle *h) {
   if (h->prm.spaceOrder < 2) return 0;
   if (h->prm.reconst == RFLU_RECONST_WENO_XYZ) {
@@ -2011,7 +2104,7 @@ int launch_grad(rflu_handle *h) {
     }
     ProfScope ps(h, "k_weno");
     if (h->gradVariant == 4 && h->tileLds && h->tileLds / 64 * 48 <= h->ldsMax)
-      hipLaunchKernelGGL(k_weno_tile, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 64 * 48, h->stream, h->d);
+      TILE_LAUNCH(k_weno_tile, h->tileLds / 64 * 48, h->d);
     else
       hipLaunchKernelGGL(k_weno, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d);
     return 0;
@@ -2024,17 +2117,15 @@ int launch_grad(rflu_handle *h) {
 int launch_grad_impl(rflu_handle *h, const Dev &dv, int venkat, int guard) {
   int nUmax = (int)(h->tileLds / 64);
   if (h->prm.fluxPrecision == 1) {
-    hipLaunchKernelGGL(k_grad_tile_f32, dim3(h->d.nBlk), dim3(BLK), h->tileLds / 64 * sizeof(SlotF), h->stream, dv,
-                       venkat, guard);
+    TILE_LAUNCH(k_grad_tile_f32, h->tileLds / 64 * sizeof(SlotF), dv, venkat, guard);
     return 0;
   }
   if (h->picl && h->gradVariant == 4 && h->tileLds && h->tileLds + 8 * (size_t)nUmax <= h->ldsMax)
-    hipLaunchKernelGGL((k_grad_tile<1>), dim3(h->d.nBlk), dim3(BLK), h->tileLds + 8 * (size_t)nUmax, h->stream,
-                       dv, venkat, guard, nUmax);
+    TILE_LAUNCH(k_grad_tile_picl, h->tileLds + 8 * (size_t)nUmax, dv, venkat, guard, nUmax);
   else if (h->picl)
     hipLaunchKernelGGL((k_grad2<1, 1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 4 && h->tileLds)
-    hipLaunchKernelGGL((k_grad_tile<0>), dim3(h->d.nBlk), dim3(BLK), h->tileLds, h->stream, dv, venkat, guard, nUmax);
+    TILE_LAUNCH(k_grad_tile_gas, h->tileLds, dv, venkat, guard, nUmax);
   else if (h->gradVariant == 2 || h->gradVariant == 4)
     hipLaunchKernelGGL((k_grad2<1, 0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, dv, venkat, guard);
   else if (h->gradVariant == 24)
@@ -2262,7 +2353,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
     if (prm->spaceOrder > 1)
       for (int k = 0; k < 6; k++) rmSoa[k * (size_t)N + i] = m.moms[10 * (size_t)o + k];
   }
-  std::vector<int> csOffD, csMemD, cfNIntD(N, 0);
+  std::vector<int> csOffD, csMemD, cfNIntD(N, 0), cfNbD(cfIdxD.size(), -1);
   std::vector<double> cfd(3 * cfIdxD.size(), 0.0);
   for (int i = 0; i < N; i++) {
     int o = h->perm[i];
@@ -2270,6 +2361,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
       int slot = m.cfIdx[e] >> 1;
       if (slot >= F) break;
       cfNIntD[i]++;
+      cfNbD[t] = h->inv[m.f2c[2 * (size_t)slot + ((m.cfIdx[e] & 1) ? 0 : 1)]];
       for (int k = 0; k < 3; k++)  // cofg(cell) - fc(face), RFLU_ModLimiters.F90:443-448
         cfd[k * cfIdxD.size() + t] = m.cofg[3 * (size_t)o + k] - m.fc[3 * (size_t)slot + k];
     }
@@ -2300,46 +2392,31 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
       }
     }
   }
-  // gradient tiles (see k_grad_tile)
+  // gradient tiles (see k_grad_tile): BLK consecutive cells per tile;
+  // ROCFLUPICL_TILE=768 asks for TBIG-cell tiles (one 12-wave workgroup per
+  // CU when its records fit; measured slower on the tet meshes: a CU then
+  // idles while its only workgroup stages records)
   std::vector<int> tOff, tMem;
-  std::vector<unsigned short> sSlot;
+  std::vector<unsigned short> sSlot, cfSlotD;
   if (prm->spaceOrder > 1) {
     int dev = 0;
     hipDeviceProp_t prop;
     size_t ldsMax = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
                         ? prop.sharedMemPerBlock : 65536;
-    int nT = d.nBlk, nUmax = 0;
-    tOff.assign(nT + 1, 0);
-    sSlot.resize(csMemD.size());
-    std::vector<int> halo;
-    bool ok = true;
-    for (int t = 0; t < nT && ok; t++) {
-      int c0 = t * BLK, c1 = std::min(N, c0 + BLK);
-      halo.clear();
-      for (int i = c0; i < c1; i++)
-        for (int e = csOffD[i]; e < csOffD[i + 1]; e++) {
-          int j = csMemD[e] & 0x7fffffff;
-          if (j < c0 || j >= c1) halo.push_back(j);
-        }
-      std::sort(halo.begin(), halo.end());
-      halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
-      int nU = (c1 - c0) + (int)halo.size();
-      if (nU > 0x7fff) { ok = false; break; }
-      nUmax = std::max(nUmax, nU);
-      for (int i = c0; i < c1; i++) tMem.push_back(i);
-      tMem.insert(tMem.end(), halo.begin(), halo.end());
-      tOff[t + 1] = (int)tMem.size();
-      for (int i = c0; i < c1; i++)
-        for (int e = csOffD[i]; e < csOffD[i + 1]; e++) {
-          unsigned v = (unsigned)csMemD[e];
-          int j = (int)(v & 0x7fffffffu), slot;
-          if (j >= c0 && j < c1) slot = j - c0;
-          else slot = (c1 - c0) + (int)(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
-          sSlot[e] = (unsigned short)(slot | ((v & 0x80000000u) ? 0x8000 : 0));
-        }
-    }
     h->ldsMax = ldsMax;
-    if (ok && (size_t)nUmax * 64 <= ldsMax) h->tileLds = (size_t)nUmax * 64;
+    const char *ts = getenv("ROCFLUPICL_TILE");
+    int want = ts ? atoi(ts) : BLK;
+    for (int T : {TBIG, BLK}) {
+      if (T == TBIG && want != TBIG) continue;
+      int nUmax = 0;
+      if (build_tiles(T, N, csOffD, csMemD, cfOffD, cfNIntD, cfNbD, tOff, tMem, sSlot, cfSlotD, nUmax) &&
+          (size_t)nUmax * 64 <= ldsMax) {
+        h->tileLds = (size_t)nUmax * 64;
+        h->tileCells = T;
+        h->nTiles = (N + T - 1) / T;
+        break;
+      }
+    }
   }
   double *faceSoa = nullptr;
   rc |= dalloc(h, &d.f2c, 2 * (size_t)d.nF, f2cD.data());
@@ -2371,6 +2448,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
       rc |= dalloc(h, &d.tOff, tOff.size(), tOff.data());
       rc |= dalloc(h, &d.tMem, tMem.size(), tMem.data());
       rc |= dalloc(h, &d.sSlot, sSlot.size(), sSlot.data());
+      rc |= dalloc(h, &d.cfSlot, cfSlotD.size(), cfSlotD.data());
     }
   }
   rc |= dalloc(h, &d.cv, 5 * (size_t)N);
@@ -2514,7 +2592,8 @@ int rflu_gpu_destroy(rflu_handle *h) {
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);
   for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot,
-                  (void *)d.gradF, (void *)d.fluxF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal})
+                  (void *)d.gradF, (void *)d.fluxF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal,
+                  (void *)d.gradRaw, (void *)d.cfSlot})
     if (p) hipFree(p);
   picl_free(h->picl);
   if (h->graph) hipGraphExecDestroy(h->graph);
