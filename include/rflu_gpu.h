@@ -41,8 +41,25 @@ extern "C" {
 #define RFLU_RECONST_WENO_XYZ 2 /* WENO-XYZ average + positivity (the deck default) */
 #define RFLU_RECONST_VENKAT 11
 /* boundary kinds on the path (rocflu/RFLU_CentralFirstPatch.F90:575,790) */
+/* Patch kinds: BC_SLIPWALL, BC_OUTFLOW, BC_NOSLIPWALL_HFLUX/_TEMP (the same
+ * inviscid flux), BC_FARFIELD, BC_INJECTION, BC_INFLOW_VELTEMP and
+ * BC_INFLOW_TOTANG of modfloflu/ModParameters.F90:397-434 (first- and
+ * second-order patch fluxes: rocflu/RFLU_CentralFirstPatch.F90:230-1134,
+ * RFLU_CentralSecondPatch.F90:241-1351). */
 #define RFLU_BC_SLIPWALL 1
 #define RFLU_BC_OUTFLOW 2
+#define RFLU_BC_NOSLIPWALL 3
+#define RFLU_BC_FARFIELD 4
+#define RFLU_BC_INJECTION 5
+#define RFLU_BC_INFLOW_VELTEMP 6
+#define RFLU_BC_INFLOW_TOTANG 7
+/* Values per patch in rflu_mesh_desc.bcVals (RFLU_BCVALS each, uniform over
+ * the patch; the reference's pPatch%mixt%vals with distrib = 0):
+ *   FARFIELD        mach, angle of attack, slip angle (radians), p, T
+ *   INJECTION       mass flux per area (> 0: burning surface), T
+ *   INFLOW_VELTEMP  u, v, w, T, p (p unless bcOpt is subsonic)
+ *   INFLOW_TOTANG   p_tot, T_tot, beta_h, beta_v (radians), mach, fixed flag */
+#define RFLU_BCVALS 6
 #define RFLU_BCOPT_SUPERSONIC 0
 #define RFLU_BCOPT_SUBSONIC 1
 #define RFLU_BCOPT_MIXED 2
@@ -64,6 +81,7 @@ typedef struct rflu_mesh_desc {
   const int *nBFaces;     /* faces per patch */
   int nvf;                /* vertices per boundary face (3 or 4) */
   const int *bf2v;        /* nvf per face, patches concatenated */
+  const double *bcVals;   /* RFLU_BCVALS per patch (NULL: all zero) */
 } rflu_mesh_desc;
 
 /* The input-deck values the path reads (NUMERICS, TIMESTEP, REFERENCE,

@@ -87,6 +87,7 @@ def lib():
         L.orc_rk_stage.argtypes = [P, I]
         L.orc_rk_stage.restype = I
         L.orc_set_dtmin.argtypes = [P, D]
+        L.orc_set_patch_vals.argtypes = [P, I, P]
         L.orc_flux_point.argtypes = [C.POINTER(OrcParams), I, P, P, P, P]
         L.orc_picl_init.argtypes = [P, C.POINTER(OrcPiclParams), I, P, P]
         L.orc_picl_init.restype = I
@@ -146,6 +147,11 @@ class Region:
                               _p(bt), _p(bo), _p(po), _p(nb), nvf, _p(bf), C.byref(self.params))
         if not self.h:
             raise RuntimeError("oracle: " + L.orc_last_error().decode())
+        for ip, p in enumerate(pts):
+            if "vals" in p:
+                v = np.zeros(6)
+                v[:len(p["vals"])] = p["vals"]
+                L.orc_set_patch_vals(self.h, ip, _p(v))
         self.nCells = c2v.shape[0]
         self.nFaces = L.orc_nfaces(self.h)
         self.nBFaces = L.orc_nbfaces(self.h)

@@ -1182,52 +1182,261 @@ static void bcond_outflow_perf(int bcOpt, double pout, double sxn, double syn, d
   }
 }
 
-/* Patches: rocflu/RFLU_CentralFirstPatch.F90:575-909 and
- * rocflu/RFLU_CentralSecondPatch.F90:638-1035 (outflow, slip wall) */
+/* libfloflu/MixtPerf_*.F90 relations the patch rind states use */
+static inline double D_PRT(double P, double R, double T) { return P / (R * T); }
+static inline double P_DRT(double D, double R, double T) { return D * R * T; }
+static inline double Eo_DGPVm(double D, double G, double P, double Vm, double K) {
+  return P / (D * (G - 1.0)) + 0.5 * Vm * Vm + K;
+}
+static inline double C2_GRT(double G, double R, double T) { return G * R * T; }
+static inline double C_Co2GUVW(double Co2, double G, double U, double V, double W) {
+  return sqrt(Co2 - 0.5 * (G - 1.0) * (U * U + V * V + W * W));
+}
+static inline double T_CGR(double C, double G, double R) { return C * C / (G * R); }
+static inline double P_GPoTTo(double G, double Po, double T, double To) { return Po * pow(T / To, G / (G - 1.0)); }
+static inline double Vm_C2Co2G(double C2, double Co2, double G) {
+  return Co2 > C2 ? sqrt(2.0 / (G - 1.0) * (Co2 - C2)) : 0.0;
+}
+static inline double P_GMaPo(double G, double Ma, double Po) {
+  return Po / pow(1.0 + 0.5 * (G - 1.0) * Ma * Ma, G / (G - 1.0));
+}
+static inline double T_GMaTo(double G, double Ma, double To) { return To / (1.0 + 0.5 * (G - 1.0) * Ma * Ma); }
+
+/* RFLU_SetRindStateFarfieldPerf without the lift correction
+ * (modflu/RFLU_ModRindStates.F90:249-421); pl is recomputed from rel. */
+static void rind_farfield(double cp, double mm, double nx, double ny, double nz, double machInf, double pInf,
+                          double tInf, double alphaInf, double betaInf, double rl, double rul, double rvl, double rwl,
+                          double rel, double *o) {
+  const double ksg = 0.0;
+  double rGas = R_M(mm), gGas = G_CpR(cp, rGas);
+  double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+  double ql = ul * nx + vl * ny + wl * nz;
+  double el = rel / rl, sl2 = ul * ul + vl * vl + wl * wl;
+  double pl = P_DEoGVm2(rl, el, gGas, sl2, ksg);
+  double ri = D_PRT(pInf, rGas, tInf), pi = pInf;
+  double qi = machInf * C_GRT(gGas, rGas, tInf);
+  double ui = qi * cos(alphaInf) * cos(betaInf);
+  double vi = qi * sin(alphaInf) * cos(betaInf);
+  double wi = qi * sin(betaInf);
+  if (machInf < 1.0) {
+    double al = C_DGP(rl, gGas, pl), pb, rb, ub, vb, wb;
+    if (ql < 0.0) {
+      pb = 0.5 * (pi + pl - rl * al * ((ui - ul) * nx + (vi - vl) * ny + (wi - wl) * nz));
+      rb = ri - (pi - pb) / (al * al);
+      ub = ui - nx * (pi - pb) / (rl * al);
+      vb = vi - ny * (pi - pb) / (rl * al);
+      wb = wi - nz * (pi - pb) / (rl * al);
+    } else {
+      pb = pi;
+      rb = rl - (pl - pi) / (al * al);
+      ub = ul + nx * (pl - pi) / (rl * al);
+      vb = vl + ny * (pl - pi) / (rl * al);
+      wb = wl + nz * (pl - pi) / (rl * al);
+    }
+    o[0] = rb; o[1] = rb * ub; o[2] = rb * vb; o[3] = rb * wb;
+    o[4] = rb * Eo_DGPUVW(rb, gGas, pb, ub, vb, wb, ksg);
+    o[5] = pb;
+  } else if (ql < 0.0) {
+    o[0] = ri; o[1] = ri * ui; o[2] = ri * vi; o[3] = ri * wi;
+    o[4] = ri * Eo_DGPVm(ri, gGas, pi, qi, ksg);
+    o[5] = pi;
+  } else {
+    o[0] = rl; o[1] = rul; o[2] = rvl; o[3] = rwl; o[4] = rel; o[5] = pl;
+  }
+}
+
+/* RFLU_SetRindStateSlipWallPerf (modflu/RFLU_ModRindStates.F90:539-600):
+ * the wall pressure from the one-dimensional wave towards the wall */
+static double rind_slipwall_p(double cp, double mm, double nx, double ny, double nz, double rl, double rul,
+                              double rvl, double rwl, double fs, double pl) {
+  double rGas = R_M(mm), gGas = G_CpR(cp, rGas);
+  double irl = 1.0 / rl;
+  double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
+  double ql = ul * nx + vl * ny + wl * nz - fs;
+  double al = C_DGP(rl, gGas, pl);
+  if (ql < 0.0) {
+    double term = 1.0 + 0.5 * (gGas - 1.0) * ql / al;
+    return pl * pow(term, 2.0 * gGas / (gGas - 1.0));
+  }
+  double term = (gGas + 1.0) / 4.0;
+  return pl + term * rl * ql * (ql + sqrt(al * al + term * term * ql * ql) / term);
+}
+
+/* libfloflu/BcondInflowPerf.F90: total pressure, total temperature and flow
+ * angles.  Where the reference leaves the state undefined (a subsonic-only
+ * patch seeing |q| >= c) the stagnation state at rest is used. */
+static void bcond_inflow_perf(int opt, int fixed, double ptot, double ttot, double betah, double betav,
+                              double mach, double sxn, double syn, double szn, double cpgas, double mm, double rl,
+                              double rul, double rvl, double rwl, double *o) {
+  const double ksg = 0.0;
+  double rgas = R_M(mm), g = G_CpR(cpgas, rgas);
+  double rr, ur, vr, wr, pr, rer;
+  int supersonicState = 0;
+  if (opt == ORC_BCOPT_SUBSONIC || opt == ORC_BCOPT_MIXED) {
+    double gm1 = g - 1.0;
+    double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+    double a02 = C2_GRT(g, rgas, ttot);
+    double al = C_Co2GUVW(a02, g, ul, vl, wl);
+    double ql = ul * sxn + vl * syn + wl * szn;
+    if (fabs(ql) < al) {
+      double sl = sqrt(ul * ul + vl * vl + wl * wl), eta;
+      if (!fixed) eta = sl > 1.0E-6 ? ql / sl : -1.0;
+      else eta = -1.0;
+      double Rm = al - 0.5 * gm1 * ql;
+      double disc = 0.5 * gm1 * (eta * eta) * (a02 / (Rm * Rm) * (1.0 + 0.5 * gm1 * (eta * eta)) - 1.0);
+      double ar, tr, sr;
+      if (disc < 0.0) {
+        ar = sqrt(a02);
+        tr = ttot;
+        pr = ptot;
+        sr = 0.0;
+      } else {
+        ar = Rm / (1.0 + 0.5 * gm1 * eta * eta) * (1.0 + sqrt(disc));
+        tr = T_CGR(ar, g, rgas);
+        pr = P_GPoTTo(g, ptot, tr, ttot);
+        sr = Vm_C2Co2G(ar * ar, a02, g);
+      }
+      (void)ar;
+      rr = D_PRT(pr, rgas, tr);
+      ur = sr * cos(betah) * cos(betav);
+      vr = sr * sin(betah);
+      wr = sr * cos(betah) * sin(betav);
+      rer = rr * Eo_DGPVm(rr, g, pr, sr, ksg);
+    } else if (opt == ORC_BCOPT_MIXED) {
+      supersonicState = 1;
+    } else {  /* undefined in the reference: stagnation state */
+      pr = ptot;
+      rr = D_PRT(pr, rgas, ttot);
+      ur = vr = wr = 0.0;
+      rer = rr * Eo_DGPUVW(rr, g, pr, ur, vr, wr, ksg);
+    }
+  } else {
+    supersonicState = 1;
+  }
+  if (supersonicState) {
+    pr = P_GMaPo(g, mach, ptot);
+    double tr = T_GMaTo(g, mach, ttot);
+    rr = D_PRT(pr, rgas, tr);
+    double ar = C_GRT(g, rgas, tr);
+    ur = mach * ar * cos(betah) * cos(betav);
+    vr = mach * ar * sin(betah);
+    wr = mach * ar * cos(betah) * sin(betav);
+    rer = rr * Eo_DGPUVW(rr, g, pr, ur, vr, wr, ksg);
+  }
+  o[0] = rr; o[1] = rr * ur; o[2] = rr * vr; o[3] = rr * wr; o[4] = rer; o[5] = pr;
+}
+
+/* the central average of the interior and rind fluxes the inflow, outflow
+ * and farfield patches share (RFLU_CentralFirstPatch.F90:700-712) */
+static void central_patch_flux(double nx, double ny, double nz, double nm, double fs, double rl, double rul,
+                               double rvl, double rwl, double rel, double pl, const double *b, double *flx) {
+  double rr = b[0], rur = b[1], rvr = b[2], rwr = b[3], rer = b[4], pr = b[5];
+  double ql = (rul * nx + rvl * ny + rwl * nz) / rl - fs;
+  double qr = (rur * nx + rvr * ny + rwr * nz) / rr - fs;
+  flx[0] = 0.5 * (ql * rl + qr * rr) * nm;
+  flx[1] = 0.5 * (ql * rul + pl * nx + qr * rur + pr * nx) * nm;
+  flx[2] = 0.5 * (ql * rvl + pl * ny + qr * rvr + pr * ny) * nm;
+  flx[3] = 0.5 * (ql * rwl + pl * nz + qr * rwr + pr * nz) * nm;
+  flx[4] = 0.5 * (ql * (rel + pl) + pl * fs + qr * (rer + pr) + pr * fs) * nm;
+}
+
+/* Patches: rocflu/RFLU_CentralFirstPatch.F90:230-1134 and
+ * rocflu/RFLU_CentralSecondPatch.F90:241-1351 (inflow, outflow, slip and
+ * no-slip walls, farfield, injection) */
 static void patch_fluxes(orc_region *r) {
   const double fs = 0.0, ksg = 0.0;
+  double cp = r->cpGv, mm = r->molGv, rgas = R_M(mm), g = G_CpR(cp, rgas);
   for (int ip = 0; ip < r->nPatches; ip++) {
+    int type = r->pBcType[ip], opt = r->pBcOpt[ip];
+    const double *pv = r->pVals + ORC_BCVALS * (size_t)ip;
     for (int b = r->pOff[ip]; b < r->pOff[ip + 1]; b++) {
       int c1 = r->bf2c[b];
       double nx = r->bfn[4 * b], ny = r->bfn[4 * b + 1], nz = r->bfn[4 * b + 2], nm = r->bfn[4 * b + 3];
       const double *cv = r->cv + 5 * (size_t)c1;
       double *rh = r->rhs + 5 * (size_t)c1;
-      double rl, rul, rvl, rwl, rel, pl;
+      double rl, rul, rvl, rwl, rel, pl, ul, vl, wl;
       if (r->p.patchOrder == 1) {
         rl = cv[0]; rul = cv[1]; rvl = cv[2]; rwl = cv[3]; rel = cv[4];
         pl = r->dv[3 * (size_t)c1];
+        ul = rul / rl; vl = rvl / rl; wl = rwl / rl;
       } else {
-        double ul, vl, wl;
         reconstruct(r, c1, r->bfc[3 * b], r->bfc[3 * b + 1], r->bfc[3 * b + 2], &rl, &ul, &vl, &wl, &pl);
         rul = rl * ul; rvl = rl * vl; rwl = rl * wl;
-        if (r->pBcType[ip] == ORC_BC_OUTFLOW) {
-          double rgas = R_M(r->molGv), g = G_CpR(r->cpGv, rgas);
+        if (type == ORC_BC_OUTFLOW || type == ORC_BC_FARFIELD || type == ORC_BC_INFLOW_VELTEMP ||
+            type == ORC_BC_INFLOW_TOTANG)
           rel = rl * Eo_DGPUVW(rl, g, pl, ul, vl, wl, ksg);
-        } else {
+        else
           rel = 0.0;
+      }
+      double flx[5], bb[6];
+      switch (type) {
+        case ORC_BC_SLIPWALL:
+        case ORC_BC_NOSLIPWALL:  /* Euler: the pressure force only, no mass term */
+          rh[1] = rh[1] + pl * nx * nm;
+          rh[2] = rh[2] + pl * ny * nm;
+          rh[3] = rh[3] + pl * nz * nm;
+          rh[4] = rh[4] + pl * fs * nm;
+          break;
+        case ORC_BC_OUTFLOW: {
+          double pr = (opt != ORC_BCOPT_SUPERSONIC) ? r->pPout[ip] : pl;
+          bcond_outflow_perf(opt, pr, nx, ny, nz, cp, mm, rl, rul, rvl, rwl, rel, pl, bb);
+          bb[5] = pr;
+          central_patch_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, bb, flx);
+          for (int k = 0; k < 5; k++) rh[k] = rh[k] + flx[k];
+          break;
+        }
+        case ORC_BC_FARFIELD:
+          rind_farfield(cp, mm, nx, ny, nz, pv[0], pv[3], pv[4], pv[1], pv[2], rl, rul, rvl, rwl, rel, bb);
+          central_patch_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, bb, flx);
+          for (int k = 0; k < 5; k++) rh[k] = rh[k] + flx[k];
+          break;
+        case ORC_BC_INFLOW_VELTEMP: {
+          double ur = pv[0], vr = pv[1], wr = pv[2], tr = pv[3], rr, pr;
+          if (opt != ORC_BCOPT_SUBSONIC) {
+            pr = pv[4];
+            rr = D_PRT(pr, rgas, tr);
+          } else {
+            rr = rl;
+            pr = P_DRT(rr, rgas, tr);
+          }
+          bb[0] = rr; bb[4] = rr * Eo_DGPUVW(rr, g, pr, ur, vr, wr, ksg);
+          bb[1] = rr * ur; bb[2] = rr * vr; bb[3] = rr * wr; bb[5] = pr;
+          central_patch_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, bb, flx);
+          for (int k = 0; k < 5; k++) rh[k] = rh[k] + flx[k];
+          break;
+        }
+        case ORC_BC_INFLOW_TOTANG:
+          bcond_inflow_perf(opt, pv[5] != 0.0, pv[0], pv[1], pv[2], pv[3], opt != ORC_BCOPT_SUBSONIC ? pv[4] : 0.0,
+                            nx, ny, nz, cp, mm, rl, rul, rvl, rwl, bb);
+          central_patch_flux(nx, ny, nz, nm, fs, rl, rul, rvl, rwl, rel, pl, bb, flx);
+          for (int k = 0; k < 5; k++) rh[k] = rh[k] + flx[k];
+          break;
+        case ORC_BC_INJECTION: {
+          double minj = pv[0];
+          if (minj > 0.0) {  /* RFLU_SetRindStateInjectPerf (RFLU_ModRindStates.F90:460-505) */
+            double tinj = pv[1];
+            double ri = D_PRT(pl, rgas, tinj);
+            double qi = -minj / ri + fs;
+            double ui = qi * nx, vi = qi * ny, wi = qi * nz;
+            double Hi = Ho_CpTUVW(cp, tinj, ui, vi, wi, 0.0);
+            flx[0] = -minj * nm;
+            flx[1] = (-minj * ui + pl * nx) * nm;
+            flx[2] = (-minj * vi + pl * ny) * nm;
+            flx[3] = (-minj * wi + pl * nz) * nm;
+            flx[4] = (-minj * Hi + pl * fs) * nm;
+          } else {
+            double pw = rind_slipwall_p(cp, mm, nx, ny, nz, rl, rul, rvl, rwl, fs, pl);
+            flx[0] = 0.0;
+            flx[1] = pw * nx * nm;
+            flx[2] = pw * ny * nm;
+            flx[3] = pw * nz * nm;
+            flx[4] = pw * fs * nm;
+          }
+          for (int k = 0; k < 5; k++) rh[k] = rh[k] + flx[k];
+          break;
         }
       }
-      if (r->pBcType[ip] == ORC_BC_SLIPWALL) {
-        rh[1] = rh[1] + pl * nx * nm;
-        rh[2] = rh[2] + pl * ny * nm;
-        rh[3] = rh[3] + pl * nz * nm;
-        rh[4] = rh[4] + pl * fs * nm;
-      } else {
-        int opt = r->pBcOpt[ip];
-        double pr = (opt != ORC_BCOPT_SUPERSONIC) ? r->pPout[ip] : pl;
-        double bb[5];
-        bcond_outflow_perf(opt, pr, nx, ny, nz, r->cpGv, r->molGv, rl, rul, rvl, rwl, rel, pl, bb);
-        double rr = bb[0], rur = bb[1], rvr = bb[2], rwr = bb[3], rer = bb[4];
-        double ql = (rul * nx + rvl * ny + rwl * nz) / rl - fs;
-        double qr = (rur * nx + rvr * ny + rwr * nz) / rr - fs;
-        double flx[5];
-        flx[0] = 0.5 * (ql * rl + qr * rr) * nm;
-        flx[1] = 0.5 * (ql * rul + pl * nx + qr * rur + pr * nx) * nm;
-        flx[2] = 0.5 * (ql * rvl + pl * ny + qr * rvr + pr * ny) * nm;
-        flx[3] = 0.5 * (ql * rwl + pl * nz + qr * rwr + pr * nz) * nm;
-        flx[4] = 0.5 * (ql * (rel + pl) + pl * fs + qr * (rer + pr) + pr * fs) * nm;
-        for (int k = 0; k < 5; k++) rh[k] = rh[k] + flx[k];
-      }
+      (void)ul; (void)vl; (void)wl;
     }
   }
 }
@@ -1404,6 +1613,7 @@ orc_region *orc_create(int nVert, const double *xyz, int nvc, int nCells, const 
   memcpy(r->c2v, c2v, sizeof(int) * (size_t)nvc * nCells);
   r->pBcType = malloc(sizeof(int) * nPatches); r->pBcOpt = malloc(sizeof(int) * nPatches);
   r->pPout = malloc(sizeof(double) * nPatches); r->pNBF = malloc(sizeof(int) * nPatches);
+  r->pVals = calloc((size_t)ORC_BCVALS * (nPatches > 0 ? nPatches : 1), sizeof(double));
   r->pOff = malloc(sizeof(int) * (nPatches + 1));
   r->pOff[0] = 0;
   for (int p = 0; p < nPatches; p++) {
@@ -1437,10 +1647,14 @@ void orc_destroy(orc_region *r) {
   void *ptrs[] = {r->xyz, r->c2v, r->f2c, r->f2v, r->fn, r->fc, r->pBcType, r->pBcOpt, r->pNBF, r->pOff,
                   r->pPout, r->bf2v, r->bf2c, r->bfn, r->bfc, r->vol, r->cofg, r->csOff, r->csMem, r->moms,
                   r->cv, r->cvOld, r->dv, r->rhs, r->rhsSum, r->diss, r->grad, r->lim, r->dt,
-                  r->phi, r->gradRhog};
+                  r->phi, r->gradRhog, r->pVals};
   for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
   orc__picl_free(r->picl);
   free(r);
+}
+
+void orc_set_patch_vals(orc_region *r, int ip, const double *vals) {
+  if (ip >= 0 && ip < r->nPatches) memcpy(r->pVals + ORC_BCVALS * (size_t)ip, vals, sizeof(double) * ORC_BCVALS);
 }
 
 int orc_nfaces(const orc_region *r) { return r->nFaces; }

@@ -20,7 +20,16 @@ extern "C" {
 
 enum { ORC_DISCR_ROE = 1, ORC_DISCR_HLLC = 3, ORC_DISCR_AUSM = 4, ORC_DISCR_AUSMUP = 5 };
 enum { ORC_RECONST_NONE = 0, ORC_RECONST_WENO_XYZ = 2, ORC_RECONST_VENKAT = 11 };
-enum { ORC_BC_SLIPWALL = 1, ORC_BC_OUTFLOW = 2 };
+/* patch kinds (the reference's BC_SLIPWALL, BC_OUTFLOW, BC_NOSLIPWALL_HFLUX/
+ * _TEMP, BC_FARFIELD, BC_INJECTION, BC_INFLOW_VELTEMP, BC_INFLOW_TOTANG) */
+enum { ORC_BC_SLIPWALL = 1, ORC_BC_OUTFLOW = 2, ORC_BC_NOSLIPWALL = 3, ORC_BC_FARFIELD = 4, ORC_BC_INJECTION = 5,
+       ORC_BC_INFLOW_VELTEMP = 6, ORC_BC_INFLOW_TOTANG = 7 };
+/* values per patch (uniform over the patch):
+ *   FARFIELD        mach, angle of attack, slip angle (radians), p, T
+ *   INJECTION       mass flux (> 0: burning), T
+ *   INFLOW_VELTEMP  u, v, w, T, p (p used unless bcOpt is subsonic)
+ *   INFLOW_TOTANG   p_tot, T_tot, beta_h, beta_v (radians), mach, fixed (0/1) */
+enum { ORC_BCVALS = 6 };
 enum { ORC_BCOPT_SUPERSONIC = 0, ORC_BCOPT_SUBSONIC = 1, ORC_BCOPT_MIXED = 2 };
 
 typedef struct orc_params {
@@ -56,6 +65,7 @@ orc_region *orc_create(int nVert, const double *xyz, int nvc, int nCells,
                        int nvf, const int *bf2v, const orc_params *prm);
 void orc_destroy(orc_region *r);
 const char *orc_last_error(void);
+void orc_set_patch_vals(orc_region *r, int ip, const double *vals);
 
 int orc_nfaces(const orc_region *r);
 int orc_nbfaces(const orc_region *r);

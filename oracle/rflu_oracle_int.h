@@ -20,6 +20,7 @@ struct orc_region {
   int nPatches, nvf, nBFaces;
   int *pBcType, *pBcOpt, *pNBF, *pOff;
   double *pPout;
+  double *pVals;  /* ORC_BCVALS per patch */
   int *bf2v, *bf2c;
   double *bfn, *bfc;
   /* cells */

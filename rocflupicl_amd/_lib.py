@@ -39,7 +39,7 @@ class MeshDesc(C.Structure):
     _fields_ = [("nVert", C.c_int), ("xyz", C.c_void_p), ("nvc", C.c_int), ("nCells", C.c_int),
                 ("c2v", C.c_void_p), ("nPatches", C.c_int), ("bcType", C.c_void_p),
                 ("bcOpt", C.c_void_p), ("pout", C.c_void_p), ("nBFaces", C.c_void_p),
-                ("nvf", C.c_int), ("bf2v", C.c_void_p)]
+                ("nvf", C.c_int), ("bf2v", C.c_void_p), ("bcVals", C.c_void_p)]
 
 
 class Params(C.Structure):

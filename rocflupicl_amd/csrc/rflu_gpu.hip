@@ -52,10 +52,13 @@ struct Gas {
 struct Dev {
   int nC = 0, nF = 0, nB = 0, S = 0, nBlk = 0;
   int *f2c = nullptr, *bf2c = nullptr, *bkind = nullptr, *cfOff = nullptr, *cfIdx = nullptr;
+  int *bkopt = nullptr;  // bcOpt of each boundary face's patch
   int *csOff = nullptr, *csMem = nullptr, *cfNInt = nullptr;
   double *cfdx = nullptr, *cfdy = nullptr, *cfdz = nullptr;
   double *fnx = nullptr, *fny = nullptr, *fnz = nullptr, *fnm = nullptr;
   double *fcx = nullptr, *fcy = nullptr, *fcz = nullptr, *bpout = nullptr;
+  int *bpatch = nullptr;   // patch of each boundary face
+  double *pcon = nullptr;  // PCON derived constants per patch (inflow, farfield, injection)
   double *vol = nullptr, *epsq = nullptr, *rm = nullptr;
   double *cv = nullptr, *rhsSum = nullptr, *rhs = nullptr, *rec = nullptr, *grad = nullptr;
   double *flux = nullptr, *srad = nullptr, *dtCell = nullptr, *dvOut = nullptr, *partial = nullptr, *scal = nullptr, *cofg = nullptr;
@@ -1569,6 +1572,102 @@ __device__ __forceinline__ void bcond_outflow(const Gas &G, int opt, double pout
 
 struct FluxPar { int order, discr, patchOrder, doSrad; };
 
+// Boundary-face kinds (d.bkind): 0 slip wall, 1 + bcOpt outflow, then
+enum { BK_NOSLIP = 4, BK_FARFIELD = 5, BK_INJECTION = 6, BK_VELTEMP = 7, BK_TOTANG = 8 };
+#define PCON 24
+// Patch constants (host-computed with the reference's expressions, so the
+// angle functions and the patch-constant powers match the reference bit for bit):
+//   farfield: mach, p_inf, rho_inf, q_inf, u_inf, v_inf, w_inf
+//   injection: mass flux, T
+//   velocity/temperature inflow: u, v, w, T, p
+//   total-pressure inflow: p_tot, T_tot, cos bh, sin bh, cos bv, sin bv, mach,
+//     fixed, supersonic state (rho, rho u, rho v, rho w, rho E, p) at 8..13,
+//     stagnation state (rho, rho E, p) at 14..16, c_0^2 at 17
+enum { PC_TT_SUP = 8, PC_TT_STAG = 14, PC_TT_A02 = 17 };
+
+// RFLU_SetRindStateFarfieldPerf without the lift correction
+// (modflu/RFLU_ModRindStates.F90:249-421): b = (rho, rho u, rho v, rho w, rho E, p)
+__device__ __forceinline__ void rind_farfield(const Gas &G, const double *pc, double nx, double ny, double nz,
+                                              double rl, double rul, double rvl, double rwl, double rel, double b[6]) {
+  const double g = G.gD;
+  double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+  double ql = ul * nx + vl * ny + wl * nz;
+  double el = rel / rl, sl2 = ul * ul + vl * vl + wl * wl;
+  double pl = (g - 1.0) * rl * (el - 0.5 * sl2 - 0.0);
+  double machInf = pc[0], pi = pc[1], ri = pc[2], qi = pc[3], ui = pc[4], vi = pc[5], wi = pc[6];
+  if (machInf < 1.0) {
+    double al = sqrt(g * pl / rl), pb, rb, ub, vb, wb;
+    if (ql < 0.0) {
+      pb = 0.5 * (pi + pl - rl * al * ((ui - ul) * nx + (vi - vl) * ny + (wi - wl) * nz));
+      rb = ri - (pi - pb) / (al * al);
+      ub = ui - nx * (pi - pb) / (rl * al);
+      vb = vi - ny * (pi - pb) / (rl * al);
+      wb = wi - nz * (pi - pb) / (rl * al);
+    } else {
+      pb = pi;
+      rb = rl - (pl - pi) / (al * al);
+      ub = ul + nx * (pl - pi) / (rl * al);
+      vb = vl + ny * (pl - pi) / (rl * al);
+      wb = wl + nz * (pl - pi) / (rl * al);
+    }
+    b[0] = rb; b[1] = rb * ub; b[2] = rb * vb; b[3] = rb * wb;
+    b[4] = rb * eo_dgpuvw(rb, g, pb, ub, vb, wb);
+    b[5] = pb;
+  } else if (ql < 0.0) {
+    b[0] = ri; b[1] = ri * ui; b[2] = ri * vi; b[3] = ri * wi;
+    b[4] = ri * (pi / (ri * (g - 1.0)) + 0.5 * qi * qi + 0.0);  // MixtPerf_Eo_DGPVm
+    b[5] = pi;
+  } else {
+    b[0] = rl; b[1] = rul; b[2] = rvl; b[3] = rwl; b[4] = rel; b[5] = pl;
+  }
+}
+
+// libfloflu/BcondInflowPerf.F90 (total pressure, total temperature, angles);
+// a subsonic-only patch seeing |q| >= c, which the reference leaves
+// undefined, gets the stagnation state at rest.
+__device__ __forceinline__ void rind_totang(const Gas &G, int opt, const double *pc, double nx, double ny, double nz,
+                                            double rl, double rul, double rvl, double rwl, double b[6]) {
+  const double g = G.gD, rgas = G.gcD;
+  if (opt == RFLU_BCOPT_SUBSONIC || opt == RFLU_BCOPT_MIXED) {
+    double gm1 = g - 1.0;
+    double ul = rul / rl, vl = rvl / rl, wl = rwl / rl;
+    double a02 = pc[PC_TT_A02];
+    double al = sqrt(a02 - 0.5 * (g - 1.0) * (ul * ul + vl * vl + wl * wl));  // MixtPerf_C_Co2GUVW
+    double ql = ul * nx + vl * ny + wl * nz;
+    if (fabs(ql) < al) {
+      double sl = sqrt(ul * ul + vl * vl + wl * wl), eta;
+      if (pc[7] == 0.0) eta = sl > 1.0E-6 ? ql / sl : -1.0;
+      else eta = -1.0;
+      double Rm = al - 0.5 * gm1 * ql;
+      double disc = 0.5 * gm1 * (eta * eta) * (a02 / (Rm * Rm) * (1.0 + 0.5 * gm1 * (eta * eta)) - 1.0);
+      double tr, pr, sr;
+      if (disc < 0.0) {
+        tr = pc[1];
+        pr = pc[0];
+        sr = 0.0;
+      } else {
+        double ar = Rm / (1.0 + 0.5 * gm1 * eta * eta) * (1.0 + sqrt(disc));
+        tr = ar * ar / (g * rgas);                           // MixtPerf_T_CGR
+        pr = pc[0] * pow(tr / pc[1], g / (g - 1.0));         // MixtPerf_P_GPoTTo
+        double c2 = ar * ar;                                 // MixtPerf_Vm_C2Co2G
+        sr = a02 > c2 ? sqrt(2.0 / (g - 1.0) * (a02 - c2)) : 0.0;
+      }
+      double rr = pr / (rgas * tr);
+      double ur = sr * pc[2] * pc[4], vr = sr * pc[3], wr = sr * pc[2] * pc[5];
+      b[0] = rr; b[1] = rr * ur; b[2] = rr * vr; b[3] = rr * wr;
+      b[4] = rr * (pr / (rr * (g - 1.0)) + 0.5 * sr * sr + 0.0);  // MixtPerf_Eo_DGPVm
+      b[5] = pr;
+      return;
+    }
+    if (opt != RFLU_BCOPT_MIXED) {
+      b[0] = pc[PC_TT_STAG]; b[1] = 0.0; b[2] = 0.0; b[3] = 0.0; b[4] = pc[PC_TT_STAG + 1]; b[5] = pc[PC_TT_STAG + 2];
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) b[k] = pc[PC_TT_SUP + k];
+}
+
 // F32 = 1: interior second-order faces in FP32 from the FP32 gradients;
 // boundary faces stay FP64 (their gradients widened from FP32).
 template <int F32>
@@ -1720,20 +1819,75 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
       FaceState A = reconstruct(L, gp, d.fcx[s], d.fcy[s], d.fcz[s], L[0]);
       rl = A.r; pl = A.p;
       rul = rl * A.u; rvl = rl * A.v; rwl = rl * A.w;
-      rel = (kind != 0) ? rl * eo_dgpuvw(rl, G.gD, pl, A.u, A.v, A.w) : 0.0;
+      bool needE = kind != 0 && kind != BK_NOSLIP && kind != BK_INJECTION;
+      rel = needE ? rl * eo_dgpuvw(rl, G.gD, pl, A.u, A.v, A.w) : 0.0;
     }
-    if (kind == 0) {
-      f[0] = -0.0;  // slip wall adds nothing to the continuity residual
+    if (kind == 0 || kind == BK_NOSLIP) {  // slip / no-slip wall: the pressure force only
+      f[0] = -0.0;  // nothing added to the continuity residual
       f[1] = pl * nx * nm;
       f[2] = pl * ny * nm;
       f[3] = pl * nz * nm;
       f[4] = pl * fs * nm;
-    } else {
-      int opt = kind - 1;
-      double pr = (opt != RFLU_BCOPT_SUPERSONIC) ? d.bpout[b] : pl;
-      double bb[5];
-      bcond_outflow(G, opt, pr, nx, ny, nz, rl, rul, rvl, rwl, rel, pl, bb);
-      double rr = bb[0], rur = bb[1], rvr = bb[2], rwr = bb[3], rer = bb[4];
+    } else if (kind == BK_INJECTION) {  // RFLU_CentralFirstPatch.F90:1049-1133
+      const double *pc = d.pcon + PCON * (size_t)d.bpatch[b];
+      double minj = pc[0];
+      if (minj > 0.0) {  // RFLU_SetRindStateInjectPerf (RFLU_ModRindStates.F90:460-505)
+        double tinj = pc[1];
+        double ri = pl / (G.gcD * tinj);
+        double qi = -minj / ri + fs;
+        double ui = qi * nx, vi = qi * ny, wi = qi * nz;
+        double Hi = G.cp * tinj + 0.5 * (ui * ui + vi * vi + wi * wi) + 0.0;
+        f[0] = -minj * nm;
+        f[1] = (-minj * ui + pl * nx) * nm;
+        f[2] = (-minj * vi + pl * ny) * nm;
+        f[3] = (-minj * wi + pl * nz) * nm;
+        f[4] = (-minj * Hi + pl * fs) * nm;
+      } else {  // RFLU_SetRindStateSlipWallPerf (RFLU_ModRindStates.F90:539-600)
+        const double g = G.gD;
+        double irl = 1.0 / rl;
+        double ul = irl * rul, vl = irl * rvl, wl = irl * rwl;
+        double ql = ul * nx + vl * ny + wl * nz - fs;
+        double al = sqrt(g * pl / rl), pw;
+        if (ql < 0.0) {
+          double term = 1.0 + 0.5 * (g - 1.0) * ql / al;
+          pw = pl * pow(term, 2.0 * g / (g - 1.0));
+        } else {
+          double term = (g + 1.0) / 4.0;
+          pw = pl + term * rl * ql * (ql + sqrt(al * al + term * term * ql * ql) / term);
+        }
+        f[0] = 0.0;
+        f[1] = pw * nx * nm;
+        f[2] = pw * ny * nm;
+        f[3] = pw * nz * nm;
+        f[4] = pw * fs * nm;
+      }
+    } else {  // outflow, farfield and inflow: the central average with a rind state
+      double bb[6];
+      if (kind <= 3) {
+        int opt = kind - 1;
+        double pr = (opt != RFLU_BCOPT_SUPERSONIC) ? d.bpout[b] : pl;
+        bcond_outflow(G, opt, pr, nx, ny, nz, rl, rul, rvl, rwl, rel, pl, bb);
+        bb[5] = pr;
+      } else {
+        const double *pc = d.pcon + PCON * (size_t)d.bpatch[b];
+        if (kind == BK_FARFIELD) {
+          rind_farfield(G, pc, nx, ny, nz, rl, rul, rvl, rwl, rel, bb);
+        } else if (kind == BK_VELTEMP) {  // RFLU_CentralFirstPatch.F90:377-574
+          double ur = pc[0], vr = pc[1], wr = pc[2], tr = pc[3], rr, pr;
+          if (d.bkopt[b] != RFLU_BCOPT_SUBSONIC) {
+            pr = pc[4];
+            rr = pr / (G.gcD * tr);
+          } else {
+            rr = rl;
+            pr = rr * G.gcD * tr;
+          }
+          bb[0] = rr; bb[4] = rr * eo_dgpuvw(rr, G.gD, pr, ur, vr, wr);
+          bb[1] = rr * ur; bb[2] = rr * vr; bb[3] = rr * wr; bb[5] = pr;
+        } else {
+          rind_totang(G, d.bkopt[b], pc, nx, ny, nz, rl, rul, rvl, rwl, bb);
+        }
+      }
+      double rr = bb[0], rur = bb[1], rvr = bb[2], rwr = bb[3], rer = bb[4], pr = bb[5];
       double ql = (rul * nx + rvl * ny + rwl * nz) / rl - fs;
       double qr = (rur * nx + rvr * ny + rwr * nz) / rr - fs;
       f[0] = 0.5 * (ql * rl + qr * rr) * nm;
@@ -2225,6 +2379,51 @@ const char *rflu_gpu_last_error(const rflu_handle *h) { return h ? h->err.c_str(
 
 // groupOff (batched ensemble): the mesh is the disjoint union of instances,
 // instance k holding reference cells [groupOff[k], groupOff[k+1]).
+// Per-patch constants of the inflow, farfield and injection patches, with the
+// reference's expressions (libfloflu/MixtPerf_*.F90, BcondInflowPerf.F90,
+// RFLU_ModRindStates.F90); the gas is the patch gas (R = 8314.3/M).
+static void patch_constants(const rflu_mesh_desc *md, const Gas &G, double *pcon) {
+  const double g = G.gD, rgas = G.gcD;
+  for (int p = 0; p < md->nPatches; p++) {
+    const double *v = md->bcVals ? md->bcVals + RFLU_BCVALS * (size_t)p : nullptr;
+    double *c = pcon + PCON * (size_t)p;
+    switch (md->bcType[p]) {
+      case RFLU_BC_FARFIELD: {
+        double mach = v[0], aoa = v[1], aos = v[2], pInf = v[3], tInf = v[4];
+        double qi = mach * sqrt(g * rgas * tInf);
+        c[0] = mach; c[1] = pInf; c[2] = pInf / (rgas * tInf); c[3] = qi;
+        c[4] = qi * cos(aoa) * cos(aos);
+        c[5] = qi * sin(aoa) * cos(aos);
+        c[6] = qi * sin(aos);
+        break;
+      }
+      case RFLU_BC_INJECTION: c[0] = v[0]; c[1] = v[1]; break;
+      case RFLU_BC_INFLOW_VELTEMP: for (int k = 0; k < 5; k++) c[k] = v[k]; break;
+      case RFLU_BC_INFLOW_TOTANG: {
+        double ptot = v[0], ttot = v[1], bh = v[2], bv = v[3];
+        double mach = md->bcOpt[p] != RFLU_BCOPT_SUBSONIC ? v[4] : 0.0;
+        c[0] = ptot; c[1] = ttot; c[2] = cos(bh); c[3] = sin(bh); c[4] = cos(bv); c[5] = sin(bv);
+        c[6] = mach; c[7] = v[5] != 0.0 ? 1.0 : 0.0;
+        // supersonic / mixed state: MixtPerf_P_GMaPo, _T_GMaTo, _D_PRT, _C_GRT
+        double pr = ptot / pow(1.0 + 0.5 * (g - 1.0) * mach * mach, g / (g - 1.0));
+        double tr = ttot / (1.0 + 0.5 * (g - 1.0) * mach * mach);
+        double rr = pr / (rgas * tr), ar = sqrt(g * rgas * tr);
+        double ur = mach * ar * cos(bh) * cos(bv), vr = mach * ar * sin(bh), wr = mach * ar * cos(bh) * sin(bv);
+        c[PC_TT_SUP] = rr; c[PC_TT_SUP + 1] = rr * ur; c[PC_TT_SUP + 2] = rr * vr; c[PC_TT_SUP + 3] = rr * wr;
+        c[PC_TT_SUP + 4] = rr * (pr / (rr * (g - 1.0)) + 0.5 * (ur * ur + vr * vr + wr * wr) + 0.0);
+        c[PC_TT_SUP + 5] = pr;
+        double rs = ptot / (rgas * ttot);  // stagnation state at rest
+        c[PC_TT_STAG] = rs;
+        c[PC_TT_STAG + 1] = rs * (ptot / (rs * (g - 1.0)) + 0.5 * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0) + 0.0);
+        c[PC_TT_STAG + 2] = ptot;
+        c[PC_TT_A02] = g * rgas * ttot;  // MixtPerf_C2_GRT
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
 static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_handle **out,
                        const std::vector<int> *groupOff) {
   if (!md || !prm || !out) return set_err(nullptr, RFLU_GPU_EINVAL, "null argument");
@@ -2250,10 +2449,14 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   if (prm->fluxPrecision == 1 && prm->spaceOrder != 2)
     return set_err(nullptr, RFLU_GPU_EINVAL, "fluxPrecision 1 needs spaceOrder 2");
   for (int p = 0; p < md->nPatches; p++) {
-    if (md->bcType[p] != RFLU_BC_SLIPWALL && md->bcType[p] != RFLU_BC_OUTFLOW)
-      return set_err(nullptr, RFLU_GPU_EINVAL, "only slip-wall and outflow patches are supported");
-    if (md->bcType[p] == RFLU_BC_OUTFLOW && (md->bcOpt[p] < 0 || md->bcOpt[p] > 2))
-      return set_err(nullptr, RFLU_GPU_EINVAL, "outflow bcOpt must be 0, 1 or 2");
+    int t = md->bcType[p];
+    if (t < RFLU_BC_SLIPWALL || t > RFLU_BC_INFLOW_TOTANG)
+      return set_err(nullptr, RFLU_GPU_EINVAL, "bcType must be one of RFLU_BC_* (1..7)");
+    if ((t == RFLU_BC_OUTFLOW || t == RFLU_BC_INFLOW_VELTEMP || t == RFLU_BC_INFLOW_TOTANG) &&
+        (md->bcOpt[p] < 0 || md->bcOpt[p] > 2))
+      return set_err(nullptr, RFLU_GPU_EINVAL, "outflow/inflow bcOpt must be 0, 1 or 2");
+    if (t >= RFLU_BC_FARFIELD && !md->bcVals)
+      return set_err(nullptr, RFLU_GPU_EINVAL, "farfield, injection and inflow patches need bcVals");
   }
   rflu_handle *h = new rflu_handle();
   h->prm = *prm;
@@ -2317,7 +2520,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   slotNew2Old.insert(slotNew2Old.end(), bperm.begin(), bperm.end());
   std::vector<int> slotOld2New(S);
   for (int q = 0; q < S; q++) slotOld2New[slotNew2Old[q]] = q;
-  std::vector<int> f2cD(2 * (size_t)F), bf2cD(m.nBFaces), bkind(m.nBFaces);
+  std::vector<int> f2cD(2 * (size_t)F), bf2cD(m.nBFaces), bkind(m.nBFaces), bkopt(m.nBFaces), bpatch(m.nBFaces);
   std::vector<double> bpout(m.nBFaces);
   for (int q = 0; q < F; q++) {
     int o = slotNew2Old[q];
@@ -2328,9 +2531,15 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
     int o = slotNew2Old[F + b] - F;
     int p = m.bPatch[o];
     bf2cD[b] = h->inv[m.bf2c[o]];
-    bkind[b] = md->bcType[p] == RFLU_BC_SLIPWALL ? 0 : 1 + md->bcOpt[p];
+    static const int kindOf[8] = {0, 0, 0, BK_NOSLIP, BK_FARFIELD, BK_INJECTION, BK_VELTEMP, BK_TOTANG};
+    int t = md->bcType[p];
+    bkind[b] = t == RFLU_BC_SLIPWALL ? 0 : t == RFLU_BC_OUTFLOW ? 1 + md->bcOpt[p] : kindOf[t];
+    bkopt[b] = md->bcOpt[p];
+    bpatch[b] = p;
     bpout[b] = md->pout[p];
   }
+  std::vector<double> pcon(PCON * (size_t)std::max(md->nPatches, 1), 0.0);
+  patch_constants(md, G, pcon.data());
   std::vector<double> soa(7 * (size_t)S);
   for (int q = 0; q < S; q++) {
     int o = slotNew2Old[q];
@@ -2422,6 +2631,9 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   rc |= dalloc(h, &d.f2c, 2 * (size_t)d.nF, f2cD.data());
   rc |= dalloc(h, &d.bf2c, (size_t)d.nB, bf2cD.data());
   rc |= dalloc(h, &d.bkind, (size_t)d.nB, bkind.data());
+  rc |= dalloc(h, &d.bkopt, (size_t)d.nB, bkopt.data());
+  rc |= dalloc(h, &d.bpatch, (size_t)d.nB, bpatch.data());
+  rc |= dalloc(h, &d.pcon, pcon.size(), pcon.data());
   rc |= dalloc(h, &d.bpout, (size_t)d.nB, bpout.data());
   rc |= dalloc(h, &faceSoa, 7 * (size_t)S, soa.data());
   rc |= dalloc(h, &d.cfOff, (size_t)N + 1, cfOffD.data());
@@ -2497,7 +2709,7 @@ int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params
   const int nvc = meshes[0].nvc, nvf = meshes[0].nvf;
   std::vector<double> xyz;
   std::vector<int> c2v, bcType, bcOpt, nbf, bf2v, cellOff{0};
-  std::vector<double> pout;
+  std::vector<double> pout, bcVals;
   int nV = 0;
   for (int k = 0; k < nInst; k++) {
     const rflu_mesh_desc &m = meshes[k];
@@ -2510,6 +2722,7 @@ int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params
       bcType.push_back(m.bcType[p]);
       bcOpt.push_back(m.bcOpt[p]);
       pout.push_back(m.pout[p]);
+      for (int k = 0; k < RFLU_BCVALS; k++) bcVals.push_back(m.bcVals ? m.bcVals[RFLU_BCVALS * (size_t)p + k] : 0.0);
       nbf.push_back(m.nBFaces[p]);
       nb += (size_t)m.nBFaces[p];
     }
@@ -2518,7 +2731,7 @@ int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params
     cellOff.push_back(cellOff.back() + m.nCells);
   }
   rflu_mesh_desc u{nV, xyz.data(), nvc, cellOff.back(), c2v.data(), (int)bcType.size(), bcType.data(),
-                   bcOpt.data(), pout.data(), nbf.data(), nvf, bf2v.data()};
+                   bcOpt.data(), pout.data(), nbf.data(), nvf, bf2v.data(), bcVals.data()};
   if (int e = create_impl(&u, prm, out, &cellOff)) return e;
   (*out)->instCell = cellOff;
   return 0;
@@ -2587,7 +2800,7 @@ int rflu_batch_get_time(rflu_handle *h, int inst, double *t, double *dtMin) {
 int rflu_gpu_destroy(rflu_handle *h) {
   if (!h) return 0;
   Dev &d = h->d;
-  void *ptrs[] = {d.f2c, d.bf2c, d.bkind, d.bpout, d.fnx, d.cfOff, d.cfIdx, d.cfNInt, d.cfdx, d.vol, d.epsq, d.cofg, d.csOff,
+  void *ptrs[] = {d.f2c, d.bf2c, d.bkind, d.bkopt, d.bpatch, d.pcon, d.bpout, d.fnx, d.cfOff, d.cfIdx, d.cfNInt, d.cfdx, d.vol, d.epsq, d.cofg, d.csOff,
                   d.csMem, d.rm, d.grad, d.cv, d.rhsSum, d.rhs, d.rec, d.flux, d.srad, d.dtCell, d.dvOut, d.partial, d.scal,
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);

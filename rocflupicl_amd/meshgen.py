@@ -24,6 +24,22 @@ import numpy as np
 # integers are this library's own enumeration, mirrored in include/rflu_gpu.h).
 BC_SLIPWALL = 1
 BC_OUTFLOW = 2
+BC_NOSLIPWALL = 3
+BC_FARFIELD = 4
+BC_INJECTION = 5
+BC_INFLOW_VELTEMP = 6
+BC_INFLOW_TOTANG = 7
+BCVALS = 6  # values per patch; their meaning per kind is documented in include/rflu_gpu.h
+
+
+def _patch(bc, bf2v):
+    """Patch dict from a (bcType, bcOpt, pout[, vals]) tuple."""
+    t, o, p = bc[:3]
+    vals = np.zeros(BCVALS)
+    if len(bc) > 3:
+        v = np.asarray(bc[3], np.float64)
+        vals[:v.size] = v
+    return dict(bcType=t, bcOpt=o, pout=p, vals=vals, bf2v=bf2v.astype(np.int64))
 BCOPT_SUPERSONIC = 0
 BCOPT_SUBSONIC = 1
 BCOPT_MIXED = 2
@@ -98,7 +114,7 @@ def _box_patches(c2v, i, j, k, nx, ny, nz, cell_kind, bc):
     """Boundary faces of the six box sides, one patch per side.
 
     Patch order: x-, x+, y-, y+, z-, z+.  Within a patch the faces follow the
-    cell order.  ``bc`` maps side index -> (bcType, bcOpt, pout).
+    cell order.  ``bc`` maps side index -> (bcType, bcOpt, pout[, vals]).
     """
     if cell_kind == 8:
         # hex local faces on each box side: x- -> face 5 (0,4,7,3), x+ -> 2,
@@ -109,8 +125,7 @@ def _box_patches(c2v, i, j, k, nx, ny, nz, cell_kind, bc):
         for s in range(6):
             cells = np.nonzero(sel[s])[0]
             bf2v = c2v[cells][:, F2V_HEX[side_face[s]]]
-            t, o, p = bc[s]
-            patches.append(dict(bcType=t, bcOpt=o, pout=p, bf2v=bf2v.astype(np.int64)))
+            patches.append(_patch(bc[s], bf2v))
         return patches
     raise ValueError("tet patches are built by tet_box")
 
@@ -147,8 +162,7 @@ def tet_box(nx, ny, nz, lx, ly, lz, jitter=0.0, seed=0, bc=None):
         on = onside[s][fv].all(axis=2)  # (nt,4)
         t_idx, f_idx = np.nonzero(on)
         bf2v = fv[t_idx, f_idx]
-        tt, o, p = bc[s]
-        patches.append(dict(bcType=tt, bcOpt=o, pout=p, bf2v=bf2v.astype(np.int64)))
+        patches.append(_patch(bc[s], bf2v))
     return dict(xyz=xyz, c2v=tets.astype(np.int64), nvc=4, patches=patches,
                 dims=(nx, ny, nz), lengths=(lx, ly, lz))
 

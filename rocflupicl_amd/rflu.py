@@ -45,9 +45,13 @@ def mesh_desc(mesh):
     po = np.array([p["pout"] for p in pts], np.float64)
     nb = np.array([p["bf2v"].shape[0] for p in pts], np.int32)
     bf = np.ascontiguousarray(np.concatenate([p["bf2v"] for p in pts]), np.int32)
+    bv = np.zeros((len(pts), 6))
+    for ip, p in enumerate(pts):
+        v = np.asarray(p.get("vals", ()), np.float64)
+        bv[ip, :v.size] = v
     md = _lib.MeshDesc(xyz.shape[0], _p(xyz), c2v.shape[1], c2v.shape[0], _p(c2v), len(pts),
-                       _p(bt), _p(bo), _p(po), _p(nb), bf.shape[1], _p(bf))
-    return md, [xyz, c2v, bt, bo, po, nb, bf]
+                       _p(bt), _p(bo), _p(po), _p(nb), bf.shape[1], _p(bf), _p(bv))
+    return md, [xyz, c2v, bt, bo, po, nb, bf, bv]
 
 
 def make_params(**kw):
