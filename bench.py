@@ -59,18 +59,42 @@ def picl_bytes(nP, nCp):
     return {"k_particles": 256.0 * nP, "k_picl_fields": 208.0 * nCp, "k_deposit": 80.0 * nCp}
 
 
-def profiled_traffic(kernel):
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4  # wave-level VALU instructions/s: 1024 SIMDs, one per 4 cycles
+
+
+def _profile_rows(fp32, ncols):
+    name = "rflu_tets64_roe_fp32_kernels.md" if fp32 else "rflu_tets64_roe_kernels.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return []
+    rows = []
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == ncols:
+            rows.append(cells)
+    return rows
+
+
+def profiled_traffic(kernel, fp32=False):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
     calibration recorded there), or None when the summary lacks the kernel."""
-    path = os.path.join(ROOT, "profiles", "rflu_tets64_roe_kernels.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].startswith(kernel):
+    for cells in _profile_rows(fp32, 5):
+        if cells[0].startswith(kernel):
             try:
                 return (float(cells[2]) + float(cells[4])) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
+def profiled_valu(kernel, fp32=False):
+    """Wave-level VALU instructions per launch of `kernel` (SQ_INSTS_VALU)
+    from the committed counter summary, or None."""
+    for cells in _profile_rows(fp32, 8):
+        if cells[0].startswith(kernel):
+            try:
+                return float(cells[2])
             except ValueError:
                 return None
     return None
@@ -329,13 +353,25 @@ def main():
     if kern:
         dom = max((k for k in kern if k in byt), key=lambda k: kern[k]["total_ms"])
         ach = byt[dom] / (kern[dom]["ms"] * 1e-3) / 1e9
-        # the gradient routine runs as k_grad_tile (default) or k_grad2 (variants 1-3)
-        gname = "k_grad_tile" if os.environ.get("ROCFLUPICL_GRAD_VARIANT", "4") == "4" else "k_grad2"
-        traffic = profiled_traffic(gname if dom == "k_grad" else dom) if args.workload == "tets64_roe" else None
+        # the gradient routine runs as the tile kernel (default) or k_grad2 (variants 1-3)
+        if os.environ.get("ROCFLUPICL_GRAD_VARIANT", "4") == "4":
+            gname = "k_grad_tile_f32" if args.fp32 else "k_grad_tile_gas"
+        else:
+            gname = "k_grad2"
+        pname = gname if dom == "k_grad" else dom
+        traffic = profiled_traffic(pname, args.fp32) if args.workload == "tets64_roe" else None
         roof = dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4),
                     traffic=None if traffic is None else int(traffic),
                     algorithmic_bytes=int(byt[dom]), ms_per_launch=round(kern[dom]["ms"], 4))
+        # the gradient kernel is bound by VALU issue, not HBM: its instruction
+        # rate from the committed SQ_INSTS_VALU count over the live launch time
+        vi = profiled_valu(pname, args.fp32) if args.workload == "tets64_roe" else None
+        if vi:
+            rate = vi / (kern[dom]["ms"] * 1e-3)
+            roof["valu"] = dict(bound="VALU issue", achieved=round(rate / 1e9, 1), peak=VALU_ISSUE_PEAK / 1e9,
+                                unit="G wave-instructions/s", frac=round(rate / VALU_ISSUE_PEAK, 4),
+                                instructions_per_launch=int(vi))
     step_gbs = byt["step"] / (ms_step * 1e-3) / 1e9
     copy_gbs = None
     if args.copy_probe_mib > 0:
