@@ -71,16 +71,22 @@ typedef struct rflu_handle rflu_handle;
 typedef struct rflu_mesh_desc {
   int nVert;
   const double *xyz;      /* 3*nVert, (x,y,z) per node */
-  int nvc;                /* 4: all tets, 8: all hexes */
+  /* 4: all tets.  8: hexes, or mixed cells: each row holds a tet (4), pyramid
+   * (5), prism (6) or hex (8) in Rocflu's local vertex order, padded with -1
+   * (tet2v/pyr2v/pri2v/hex2v); rows in Rocflu's global order (tets, hexes,
+   * prisms, pyramids) reproduce its face numbering. */
+  int nvc;
   int nCells;
-  const int *c2v;         /* nvc*nCells, 0-based, Rocflu local vertex order */
+  const int *c2v;         /* nvc*nCells, 0-based */
   int nPatches;
   const int *bcType;      /* RFLU_BC_* per patch */
   const int *bcOpt;       /* RFLU_BCOPT_* per patch (outflow) */
   const double *pout;     /* outflow static pressure per patch */
   const int *nBFaces;     /* faces per patch */
   int nvf;                /* vertices per boundary face (3 or 4) */
-  const int *bf2v;        /* nvf per face, patches concatenated */
+  const int *bf2v;        /* nvf per face, patches concatenated; with nvf 4 a
+                             triangle is padded with -1 (bTri2v before bQuad2v
+                             within a patch, as Rocflu orders them) */
   const double *bcVals;   /* RFLU_BCVALS per patch (NULL: all zero) */
 } rflu_mesh_desc;
 

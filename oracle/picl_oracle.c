@@ -761,6 +761,8 @@ int orc__picl_runge(orc_region *r, int istage) {
 int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const double *y,
                   const double *pconst) {
   if (r->nvc != 8 || r->p.spaceOrder != 2) return -1;
+  for (size_t q = 0; q < (size_t)8 * r->nCells; q++)
+    if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1)
     return -1;

@@ -23,9 +23,30 @@ const char *orc_last_error(void) { return g_err; }
 #define FAIL(...) do { snprintf(g_err, sizeof g_err, __VA_ARGS__); goto fail; } while (0)
 
 /* modflu/RFLU_ModGrid.F90:92-96 (0-based) */
-static const int F2V_TET[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+static const int F2V_TET[4][4] = {{0, 1, 2, -1}, {1, 3, 2, -1}, {0, 2, 3, -1}, {0, 3, 1, -1}};
+/* prisms and pyramids: f2vPri, f2vPyr of modflu/RFLU_ModGrid.F90:97-102 */
+static const int F2V_PRI[5][4] = {{0, 2, 1, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {0, 3, 5, 2}, {3, 4, 5, -1}};
+static const int F2V_PYR[5][4] = {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 4, 3, -1}};
 static const int F2V_HEX[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
                                   {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+
+/* Cells: rows of nvc entries; with nvc = 8 a row may hold a tetrahedron (4),
+ * pyramid (5), prism (6) or hexahedron (8) with -1 padding.  Returns the
+ * vertex count and the local face table (-1 marks a triangle's 4th vertex). */
+static int cell_nv(const orc_region *r, int i) {
+  if (r->nvc == 4) return 4;
+  int n = 0;
+  for (int q = 0; q < 8; q++) n += r->c2v[(size_t)i * 8 + q] >= 0;
+  return n;
+}
+static const int (*cell_faces(int nv, int *nlf))[4] {
+  switch (nv) {
+    case 4: *nlf = 4; return F2V_TET;
+    case 5: *nlf = 5; return F2V_PYR;
+    case 6: *nlf = 5; return F2V_PRI;
+    default: *nlf = 6; return F2V_HEX;
+  }
+}
 
 /* struct orc_region: rflu_oracle_int.h */
 
@@ -105,8 +126,7 @@ static void sort_small(int *v, int n) {
 }
 
 static int build_faces(orc_region *r) {
-  int nlf = (r->nvc == 4) ? 4 : 6, nfv = (r->nvc == 4) ? 3 : 4;
-  int maxF = r->nCells * nlf;
+  int maxF = r->nCells * 6;
   int *c1 = malloc(sizeof(int) * maxF), *c2 = malloc(sizeof(int) * maxF);
   int *lf = malloc(sizeof(int) * maxF), *k3 = malloc(sizeof(int) * 3 * maxF);
   size_t tsz = 1;
@@ -115,10 +135,11 @@ static int build_faces(orc_region *r) {
   int nF = 0, ok = 1;
   for (size_t t = 0; t < tsz; t++) tab[t] = -1;
   for (int icl = 0; icl < r->nCells; icl++) {
+    int nlf;
+    const int(*tbl)[4] = cell_faces(cell_nv(r, icl), &nlf);
     for (int ifl = 0; ifl < nlf; ifl++) {
-      int fv[4];
-      for (int q = 0; q < nfv; q++)
-        fv[q] = r->c2v[(size_t)icl * r->nvc + (nfv == 3 ? F2V_TET[ifl][q] : F2V_HEX[ifl][q])];
+      int fv[4], nfv = tbl[ifl][3] < 0 ? 3 : 4;
+      for (int q = 0; q < nfv; q++) fv[q] = r->c2v[(size_t)icl * r->nvc + tbl[ifl][q]];
       sort_small(fv, nfv);
       size_t h = key3(fv) & (tsz - 1);
       for (;;) {
@@ -152,20 +173,18 @@ static int build_faces(orc_region *r) {
     old2int[f] = fi;
     r->f2c[2 * fi] = c1[f];
     r->f2c[2 * fi + 1] = c2[f];
-    for (int q = 0; q < 4; q++) {
-      if (q < nfv)
-        r->f2v[4 * fi + q] = r->c2v[(size_t)c1[f] * r->nvc + (nfv == 3 ? F2V_TET[lf[f]][q] : F2V_HEX[lf[f]][q])];
-      else
-        r->f2v[4 * fi + q] = -1;
-    }
+    int nlf;
+    const int(*tbl)[4] = cell_faces(cell_nv(r, c1[f]), &nlf);
+    for (int q = 0; q < 4; q++)
+      r->f2v[4 * fi + q] = tbl[lf[f]][q] >= 0 ? r->c2v[(size_t)c1[f] * r->nvc + tbl[lf[f]][q]] : -1;
     fi++;
   }
   /* boundary faces: RFLU_UnHashBFace -> bf2c = c1 of the matching face */
   r->bf2c = malloc(sizeof(int) * (r->nBFaces > 0 ? r->nBFaces : 1));
   for (int b = 0; b < r->nBFaces; b++) {
-    int fv[4];
-    for (int q = 0; q < r->nvf; q++) fv[q] = r->bf2v[(size_t)b * r->nvf + q];
-    sort_small(fv, r->nvf);
+    int fv[4], nfv = (r->nvf == 4 && r->bf2v[(size_t)b * 4 + 3] < 0) ? 3 : r->nvf;
+    for (int q = 0; q < nfv; q++) fv[q] = r->bf2v[(size_t)b * r->nvf + q];
+    sort_small(fv, nfv);
     size_t h = key3(fv) & (tsz - 1);
     int found = -1;
     for (;;) {
@@ -228,21 +247,18 @@ static void build_geometry(orc_region *r) {
     r->vol[i] = 0.0;
     r->cofg[3 * i] = r->cofg[3 * i + 1] = r->cofg[3 * i + 2] = 0.0;
     const int *v = r->c2v + (size_t)i * r->nvc;
+    int nv = cell_nv(r, i);
+    /* RFLU_ComputeApproxCentroids: term * (sum in vertex order), term = 1/nv */
+    double term = nv == 4 ? 1.0 / 4.0 : nv == 5 ? 1.0 / 5.0 : nv == 6 ? 1.0 / 6.0 : 1.0 / 8.0;
     for (int d = 0; d < 3; d++) {
-      if (r->nvc == 4) {
-        double term = 1.0 / 4.0;
-        app[3 * i + d] = term * (r->xyz[3 * v[0] + d] + r->xyz[3 * v[1] + d] + r->xyz[3 * v[2] + d] + r->xyz[3 * v[3] + d]);
-      } else {
-        double term = 1.0 / 8.0;
-        app[3 * i + d] = term * (r->xyz[3 * v[0] + d] + r->xyz[3 * v[1] + d] + r->xyz[3 * v[2] + d] + r->xyz[3 * v[3] + d] +
-                                 r->xyz[3 * v[4] + d] + r->xyz[3 * v[5] + d] + r->xyz[3 * v[6] + d] + r->xyz[3 * v[7] + d]);
-      }
+      double s = r->xyz[3 * v[0] + d];
+      for (int q = 1; q < nv; q++) s = s + r->xyz[3 * v[q] + d];
+      app[3 * i + d] = term * s;
     }
   }
-  int nfv = (r->nvc == 4) ? 3 : 4;
   for (int f = 0; f < r->nFaces; f++) {
     double fv[3], fc[3];
-    face_geom(r, r->f2v + 4 * f, nfv, fv, fc);
+    face_geom(r, r->f2v + 4 * f, r->f2v[4 * f + 3] < 0 ? 3 : 4, fv, fc);
     double fVecM = sqrt(fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2]);
     double term = 1.0 / fVecM;
     r->fn[4 * f] = fv[0] * term;
@@ -266,7 +282,8 @@ static void build_geometry(orc_region *r) {
   }
   for (int b = 0; b < r->nBFaces; b++) {
     double fv[3], fc[3];
-    face_geom(r, r->bf2v + (size_t)b * r->nvf, r->nvf, fv, fc);
+    const int *bv = r->bf2v + (size_t)b * r->nvf;
+    face_geom(r, bv, (r->nvf == 4 && bv[3] < 0) ? 3 : r->nvf, fv, fc);
     double fVecM = sqrt(fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2]);
     double term = 1.0 / fVecM;
     r->bfn[4 * b] = fv[0] * term;
@@ -362,13 +379,13 @@ static int build_stencils(orc_region *r) {
   const int SMAX = 40, SMIN = 4;
   int N = r->nCells, nvc = r->nvc;
   int *v2cCnt = calloc(r->nVert + 1, sizeof(int));
-  for (size_t q = 0; q < (size_t)N * nvc; q++) v2cCnt[r->c2v[q] + 1]++;
+  for (size_t q = 0; q < (size_t)N * nvc; q++) if (r->c2v[q] >= 0) v2cCnt[r->c2v[q] + 1]++;
   for (int v = 0; v < r->nVert; v++) v2cCnt[v + 1] += v2cCnt[v];
   int *v2c = malloc(sizeof(int) * (size_t)N * nvc), *fill = calloc(r->nVert, sizeof(int));
   for (int i = 0; i < N; i++)
     for (int q = 0; q < nvc; q++) {
       int v = r->c2v[(size_t)i * nvc + q];
-      v2c[v2cCnt[v] + fill[v]++] = i;
+      if (v >= 0) v2c[v2cCnt[v] + fill[v]++] = i;
     }
   free(fill);
   r->csOff = malloc(sizeof(int) * (N + 1));
@@ -379,10 +396,10 @@ static int build_stencils(orc_region *r) {
   double (*a)[4] = malloc(sizeof(double) * 4 * SMAX);
   double minRatio = 1e300;
   int ok = 1, tot = 0;
-  double iNVert = 1.0 / (double)nvc;
   for (int icg = 0; icg < N; icg++) {
-    int n = 0;
-    for (int q = 0; q < nvc; q++) {
+    int n = 0, nv = cell_nv(r, icg);
+    double iNVert = 1.0 / (double)nv;  /* RFLU_BuildC2CStencilBasic: 1/nVert of the cell itself */
+    for (int q = 0; q < nv; q++) {
       int v = r->c2v[(size_t)icg * nvc + q];
       for (int t = v2cCnt[v]; t < v2cCnt[v + 1]; t++) {
         if (n >= cap) { cap *= 2; tmp = realloc(tmp, sizeof(int) * cap); frac = realloc(frac, sizeof(double) * cap); }
@@ -397,14 +414,14 @@ static int build_stencils(orc_region *r) {
     for (int t = 0; t < n2; t++) if (tmp[t] != icg) tmp[m++] = tmp[t];
     n2 = m;
     int vt[8];
-    for (int q = 0; q < nvc; q++) vt[q] = r->c2v[(size_t)icg * nvc + q];
-    sort_small(vt, nvc);
+    for (int q = 0; q < nv; q++) vt[q] = r->c2v[(size_t)icg * nvc + q];
+    sort_small(vt, nv);
     for (int t = 0; t < n2; t++) {
-      int v2[8], nc = 0;
-      for (int q = 0; q < nvc; q++) v2[q] = r->c2v[(size_t)tmp[t] * nvc + q];
-      sort_small(v2, nvc);
+      int v2[8], nc = 0, nv2 = cell_nv(r, tmp[t]);
+      for (int q = 0; q < nv2; q++) v2[q] = r->c2v[(size_t)tmp[t] * nvc + q];
+      sort_small(v2, nv2);
       int ia = 0, ib = 0;
-      while (ia < nvc && ib < nvc) {
+      while (ia < nv && ib < nv2) {
         if (vt[ia] == v2[ib]) { nc++; ia++; ib++; }
         else if (vt[ia] < v2[ib]) ia++;
         else ib++;
@@ -1604,7 +1621,7 @@ orc_region *orc_create(int nVert, const double *xyz, int nvc, int nCells, const 
                        const int *pNBF, int nvf, const int *bf2v, const orc_params *prm) {
   orc_region *r = calloc(1, sizeof *r);
   g_err[0] = 0;
-  if (nvc != 4 && nvc != 8) FAIL("nvc must be 4 or 8");
+  if (nvc != 4 && nvc != 8) FAIL("nvc must be 4 (tets) or 8 (hexes, or mixed cells padded with -1)");
   r->p = *prm;
   r->nVert = nVert; r->nvc = nvc; r->nCells = nCells; r->nPatches = nPatches; r->nvf = nvf;
   r->xyz = malloc(sizeof(double) * 3 * nVert);

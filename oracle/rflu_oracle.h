@@ -56,7 +56,9 @@ typedef struct orc_params {
 typedef struct orc_region orc_region;
 
 /* Build the region: face list, geometry, stencils (order 2), moments.
- * c2v is nCells x nvc (nvc 4 = tets, 8 = hexes), 0-based.
+ * c2v is nCells x nvc, 0-based: nvc 4 = tets; nvc 8 = hexes, or mixed cells
+ * (tet 4, pyramid 5, prism 6, hex 8 vertices) padded with -1; boundary
+ * triangles among quads (nvf 4) are padded with -1 too.
  * Patches: patch p has pNBF[p] faces with nvf vertices each, concatenated in
  * bf2v.  Returns NULL on error (message via orc_last_error). */
 orc_region *orc_create(int nVert, const double *xyz, int nvc, int nCells,

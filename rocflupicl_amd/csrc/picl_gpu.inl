@@ -1268,7 +1268,9 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (!h || !pp || nPart < 0 || (nPart > 0 && (!y || !pconst))) return RFLU_GPU_EINVAL;
   if (h->picl) return set_err(h, RFLU_GPU_EINVAL, "particles are already attached to this region");
   const rflu::MeshHost &m = h->mesh;
-  if (m.nvc != 8) return set_err(h, RFLU_GPU_EINVAL, "PICL needs an all-hex grid (PICL_TEMP_InitSolver.F90:562-597)");
+  bool allHex = m.nvc == 8;
+  for (size_t q = 0; allHex && q < h->c2v.size(); q++) allHex = h->c2v[q] >= 0;
+  if (!allHex) return set_err(h, RFLU_GPU_EINVAL, "PICL needs an all-hex grid (PICL_TEMP_InitSolver.F90:562-597)");
   if (h->prm.spaceOrder != 2) return set_err(h, RFLU_GPU_EINVAL, "PICL needs spaceOrder 2 (its fields use gradients)");
   if (h->prm.patchOrder != 1) return set_err(h, RFLU_GPU_EINVAL, "PICL is built for patchOrder 1");
   if (h->prm.fluxPrecision != 0) return set_err(h, RFLU_GPU_EINVAL, "PICL runs in FP64 (fluxPrecision 0)");

@@ -2716,7 +2716,7 @@ int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params
     if (m.nvc != nvc || m.nvf != nvf)
       return set_err(nullptr, RFLU_GPU_EINVAL, "a batch holds one cell type (all tets or all hexes)");
     xyz.insert(xyz.end(), m.xyz, m.xyz + 3 * (size_t)m.nVert);
-    for (size_t q = 0; q < (size_t)m.nvc * m.nCells; q++) c2v.push_back(m.c2v[q] + nV);
+    for (size_t q = 0; q < (size_t)m.nvc * m.nCells; q++) c2v.push_back(m.c2v[q] < 0 ? -1 : m.c2v[q] + nV);
     size_t nb = 0;
     for (int p = 0; p < m.nPatches; p++) {
       bcType.push_back(m.bcType[p]);
@@ -2726,7 +2726,7 @@ int rflu_batch_create(int nInst, const rflu_mesh_desc *meshes, const rflu_params
       nbf.push_back(m.nBFaces[p]);
       nb += (size_t)m.nBFaces[p];
     }
-    for (size_t q = 0; q < nb * (size_t)m.nvf; q++) bf2v.push_back(m.bf2v[q] + nV);
+    for (size_t q = 0; q < nb * (size_t)m.nvf; q++) bf2v.push_back(m.bf2v[q] < 0 ? -1 : m.bf2v[q] + nV);
     nV += m.nVert;
     cellOff.push_back(cellOff.back() + m.nCells);
   }

@@ -14,14 +14,34 @@ namespace rflu {
 
 namespace {
 
-// Local face tables: modflu/RFLU_ModGrid.F90:92-96, 0-based.
-const int kTetFace[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+// Local face tables: modflu/RFLU_ModGrid.F90:92-102, 0-based (-1: the 4th
+// vertex of a triangle).
+const int kTetFace[4][4] = {{0, 1, 2, -1}, {1, 3, 2, -1}, {0, 2, 3, -1}, {0, 3, 1, -1}};
 const int kHexFace[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
                             {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+const int kPriFace[5][4] = {{0, 2, 1, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {0, 3, 5, 2}, {3, 4, 5, -1}};
+const int kPyrFace[5][4] = {{0, 3, 2, 1}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 4, 3, -1}};
+
+// Cells are rows of nvc vertex ids; with nvc = 8 a row holds a tetrahedron
+// (4), pyramid (5), prism (6) or hexahedron (8), padded with -1.
+int row_nv(const MeshInput &in, int i) {
+  if (in.nvc == 4) return 4;
+  int n = 0;
+  for (int q = 0; q < 8; q++) n += in.c2v[(size_t)i * 8 + q] >= 0;
+  return n;
+}
+const int (*faces_of(int nv, int &nlf))[4] {
+  switch (nv) {
+    case 4: nlf = 4; return kTetFace;
+    case 5: nlf = 5; return kPyrFace;
+    case 6: nlf = 5; return kPriFace;
+    default: nlf = 6; return kHexFace;
+  }
+}
 
 struct FaceKey {
   int k0, k1, k2;  // three smallest vertex ids, ascending
-  int rank;        // insertion rank = cell*nlf + localFace
+  int rank;        // insertion rank = cell*6 + localFace
   bool operator<(const FaceKey &o) const {
     if (k0 != o.k0) return k0 < o.k0;
     if (k1 != o.k1) return k1 < o.k1;
@@ -149,27 +169,43 @@ double min_sv_ratio(std::vector<std::array<double, 4>> &a, double &s0) {
 }  // namespace
 
 std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
-  if (in.nvc != 4 && in.nvc != 8) return "nvc must be 4 (tets) or 8 (hexes)";
-  const int nlf = in.nvc == 4 ? 4 : 6, nfv = in.nvc == 4 ? 3 : 4;
-  if (in.nvf != nfv) return "boundary faces must have as many vertices as the cells' faces";
+  if (in.nvc != 4 && in.nvc != 8) return "nvc must be 4 (tets) or 8 (hexes, or mixed cells padded with -1)";
+  if (in.nvf != 3 && in.nvf != 4) return "nvf must be 3 or 4";
+  if (in.nvc == 4 && in.nvf != 3) return "tet meshes have triangular boundary faces (nvf 3)";
   const int N = in.nCells;
   m.nCells = N;
   m.nvc = in.nvc;
-  auto lface = [&](int icl, int ifl, int q) {
-    return in.c2v[(size_t)icl * in.nvc + (nfv == 3 ? kTetFace[ifl][q] : kHexFace[ifl][q])];
+  std::vector<unsigned char> cnv(N);
+  for (int i = 0; i < N; i++) {
+    int nv = row_nv(in, i);
+    if (nv != 4 && nv != 5 && nv != 6 && nv != 8) return "a cell row must hold 4, 5, 6 or 8 vertices";
+    for (int q = 0; q < nv; q++)
+      if (in.c2v[(size_t)i * in.nvc + q] < 0) return "a cell row's vertices must precede its -1 padding";
+    cnv[i] = (unsigned char)nv;
+  }
+  auto lface = [&](int icl, int ifl, int q) {  // -1: the 4th vertex of a triangle
+    int nlf;
+    int lv = faces_of(cnv[icl], nlf)[ifl][q];
+    return lv < 0 ? -1 : in.c2v[(size_t)icl * in.nvc + lv];
   };
+  auto bface_nv = [&](int b) { return (in.nvf == 4 && in.bf2v[(size_t)b * 4 + 3] < 0) ? 3 : in.nvf; };
 
   // ---- face list: sort all (cell, local face) keys; a face's number is the
   // rank of its first occurrence (RFLU_ModFaceList.F90:888-1015 numbers faces
-  // by first insertion); duplicates pair into interior faces.
-  std::vector<FaceKey> keys((size_t)N * nlf);
-  for (int icl = 0; icl < N; icl++)
+  // by first insertion, cells in their global order, which Rocflu assigns by
+  // type: tets, hexes, prisms, pyramids); duplicates pair into interior faces.
+  std::vector<FaceKey> keys;
+  keys.reserve((size_t)N * 6);
+  for (int icl = 0; icl < N; icl++) {
+    int nlf;
+    faces_of(cnv[icl], nlf);
     for (int ifl = 0; ifl < nlf; ifl++) {
-      int v[4];
+      int v[4], nfv = lface(icl, ifl, 3) < 0 ? 3 : 4;
       for (int q = 0; q < nfv; q++) v[q] = lface(icl, ifl, q);
       std::sort(v, v + nfv);
-      keys[(size_t)icl * nlf + ifl] = FaceKey{v[0], v[1], v[2], icl * nlf + ifl};
+      keys.push_back(FaceKey{v[0], v[1], v[2], icl * 6 + ifl});
     }
+  }
   std::sort(keys.begin(), keys.end());
   struct Grp { int first, second; };
   std::vector<Grp> interior;
@@ -188,10 +224,10 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
   m.f2c.resize(2 * (size_t)F);
   m.f2v.assign(4 * (size_t)F, -1);
   for (int f = 0; f < F; f++) {
-    int c1 = interior[f].first / nlf, l1 = interior[f].first % nlf;
+    int c1 = interior[f].first / 6, l1 = interior[f].first % 6;
     m.f2c[2 * f] = c1;
-    m.f2c[2 * f + 1] = interior[f].second / nlf;
-    for (int q = 0; q < nfv; q++) m.f2v[4 * f + q] = lface(c1, l1, q);
+    m.f2c[2 * f + 1] = interior[f].second / 6;
+    for (int q = 0; q < 4; q++) m.f2v[4 * f + q] = lface(c1, l1, q);
   }
   // boundary faces -> owning cell
   int nB = 0;
@@ -208,14 +244,14 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
     int b = 0;
     for (int p = 0; p < in.nPatches; p++)
       for (int k = 0; k < in.nBFaces[p]; k++, b++) {
-        int v[4];
-        for (int q = 0; q < nfv; q++) v[q] = in.bf2v[(size_t)b * nfv + q];
+        int v[4], nfv = bface_nv(b);
+        for (int q = 0; q < nfv; q++) v[q] = in.bf2v[(size_t)b * in.nvf + q];
         std::sort(v, v + nfv);
         FaceKey probe{v[0], v[1], v[2], -1};
         auto it = std::lower_bound(open.begin(), open.end(), probe);
         if (it == open.end() || it->k0 != v[0] || it->k1 != v[1] || it->k2 != v[2])
           return "boundary face not found among open cell faces";
-        m.bf2c[b] = it->rank / nlf;
+        m.bf2c[b] = it->rank / 6;
         m.bPatch[b] = p;
       }
   }
@@ -243,7 +279,8 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
   m.fc.resize(3 * (size_t)S);
   std::vector<double> fvec(3 * (size_t)S);
   for (int s = 0; s < S; s++) {
-    const int *v = s < F ? &m.f2v[4 * (size_t)s] : in.bf2v + (size_t)(s - F) * nfv;
+    const int *v = s < F ? &m.f2v[4 * (size_t)s] : in.bf2v + (size_t)(s - F) * in.nvf;
+    int nfv = s < F ? (v[3] < 0 ? 3 : 4) : bface_nv(s - F);
     double fv[3], fcen[3];
     face_vec_cen(in.xyz, v, nfv, fv, fcen);
     double mag = std::sqrt(fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2]);
@@ -259,11 +296,14 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
   const double THRD = 1.0 / 3.0;
   for (int i = 0; i < N; i++) {
     const int *cv = in.c2v + (size_t)i * in.nvc;
+    const int nv = cnv[i];
+    // RFLU_ComputeApproxCentroids (RFLU_ModGeometry.F90): term * (sum in vertex order)
+    const double term0 = nv == 4 ? 1.0 / 4.0 : nv == 5 ? 1.0 / 5.0 : nv == 6 ? 1.0 / 6.0 : 1.0 / 8.0;
     double app[3];
     for (int d = 0; d < 3; d++) {
       double s = in.xyz[3 * (size_t)cv[0] + d];
-      for (int q = 1; q < in.nvc; q++) s = s + in.xyz[3 * (size_t)cv[q] + d];
-      app[d] = (in.nvc == 4 ? 1.0 / 4.0 : 1.0 / 8.0) * s;
+      for (int q = 1; q < nv; q++) s = s + in.xyz[3 * (size_t)cv[q] + d];
+      app[d] = term0 * s;
     }
     double vol = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     for (int e = m.cfOff[i]; e < m.cfOff[i + 1]; e++) {
@@ -300,12 +340,13 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
   // neighbours ranked by shared-vertex fraction, at most 40, then ascending.
   const int SMAX = 40, SMIN = 4;
   std::vector<int> v2cOff(in.nVert + 1, 0), v2c((size_t)N * in.nvc);
-  for (size_t q = 0; q < (size_t)N * in.nvc; q++) v2cOff[in.c2v[q] + 1]++;
+  for (int i = 0; i < N; i++)
+    for (int q = 0; q < cnv[i]; q++) v2cOff[in.c2v[(size_t)i * in.nvc + q] + 1]++;
   for (int v = 0; v < in.nVert; v++) v2cOff[v + 1] += v2cOff[v];
   {
     std::vector<int> fill(v2cOff.begin(), v2cOff.end() - 1);
     for (int i = 0; i < N; i++)
-      for (int q = 0; q < in.nvc; q++) v2c[fill[in.c2v[(size_t)i * in.nvc + q]]++] = i;
+      for (int q = 0; q < cnv[i]; q++) v2c[fill[in.c2v[(size_t)i * in.nvc + q]]++] = i;
   }
   m.csOff.assign(N + 1, 0);
   m.csMem.clear();
@@ -323,7 +364,6 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
   int nThr = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
   if (N < 20000) nThr = 1;
   std::vector<Part> parts(nThr);
-  const double iNVert = 1.0 / (double)in.nvc;
   auto work = [&](int tid) {
     Part &P = parts[tid];
     const int c0 = (int)((long long)N * tid / nThr), c1 = (int)((long long)N * (tid + 1) / nThr);
@@ -334,22 +374,24 @@ std::string build_mesh(const MeshInput &in, bool needStencils, MeshHost &m) {
     std::vector<std::array<double, 4>> amat;
     for (int i = c0; i < c1; i++) {
       const int *vi = in.c2v + (size_t)i * in.nvc;
+      const int nvi = cnv[i];
+      const double iNVert = 1.0 / (double)nvi;  // 1/nVert of the cell itself (RFLU_BuildC2CStencilBasic)
       cand.clear();
-      for (int q = 0; q < in.nvc; q++)
+      for (int q = 0; q < nvi; q++)
         for (int t = v2cOff[vi[q]]; t < v2cOff[vi[q] + 1]; t++)
           if (v2c[t] != i) cand.push_back(v2c[t]);
       std::sort(cand.begin(), cand.end());
       cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
       int own[8];
-      std::copy(vi, vi + in.nvc, own);
-      std::sort(own, own + in.nvc);
+      std::copy(vi, vi + nvi, own);
+      std::sort(own, own + nvi);
       frac.resize(cand.size());
       for (size_t t = 0; t < cand.size(); t++) {
-        int oth[8];
-        std::copy(in.c2v + (size_t)cand[t] * in.nvc, in.c2v + (size_t)(cand[t] + 1) * in.nvc, oth);
-        std::sort(oth, oth + in.nvc);
+        int oth[8], nvo = cnv[cand[t]];
+        std::copy(in.c2v + (size_t)cand[t] * in.nvc, in.c2v + (size_t)cand[t] * in.nvc + nvo, oth);
+        std::sort(oth, oth + nvo);
         int common = 0;
-        for (int a = 0, b = 0; a < in.nvc && b < in.nvc;) {
+        for (int a = 0, b = 0; a < nvi && b < nvo;) {
           if (own[a] == oth[b]) { common++; a++; b++; }
           else if (own[a] < oth[b]) a++;
           else b++;

@@ -16,7 +16,7 @@ namespace rflu {
 struct MeshInput {
   int nVert = 0;
   const double *xyz = nullptr;  // 3*nVert
-  int nvc = 0;                  // 4 (tets) or 8 (hexes)
+  int nvc = 0;                  // 4 (tets) or 8 (hexes, or mixed cells padded with -1)
   int nCells = 0;
   const int *c2v = nullptr;     // nvc*nCells, 0-based
   int nPatches = 0;

@@ -188,3 +188,124 @@ def prim_to_cons(rho, u, v, w, p, gamma):
     """Conserved (rho, rho u, rho v, rho w, rho E) of a perfect gas."""
     E = p / (rho * (gamma - 1.0)) + 0.5 * (u * u + v * v + w * w)
     return np.stack([rho, rho * u, rho * v, rho * w, rho * E], axis=0)
+
+
+# ---- mixed meshes (tetrahedra, hexahedra, prisms, pyramids) ----
+# Local face tables of all four cell kinds (modflu/RFLU_ModGrid.F90:92-102),
+# 0-based, -1 for a triangle's 4th vertex.
+F2V_BY_NV = {
+    4: [[0, 1, 2, -1], [1, 3, 2, -1], [0, 2, 3, -1], [0, 3, 1, -1]],
+    5: [[0, 3, 2, 1], [0, 1, 4, -1], [1, 2, 4, -1], [2, 3, 4, -1], [0, 4, 3, -1]],
+    6: [[0, 2, 1, -1], [0, 1, 4, 3], [1, 2, 5, 4], [0, 3, 5, 2], [3, 4, 5, -1]],
+    8: [list(r) for r in F2V_HEX],
+}
+
+
+def _pad8(cells):
+    out = np.full((len(cells), 8), -1, np.int64)
+    for q, c in enumerate(cells):
+        out[q, :len(c)] = c
+    return out
+
+
+def _mixed_patches(c2v, lat, dims, bc):
+    """Boundary faces of a mixed box mesh, one patch per side (x-, x+, y-,
+    y+, z-, z+); within a patch the triangles come first, then the
+    quadrilaterals (Rocflu's bTri2v / bQuad2v order), each in cell order.
+    Vertex order is the owning cell's local face order (outward normals);
+    triangles are padded with -1 to 4 entries."""
+    nx, ny, nz = dims
+    count = {}
+    faces = []
+    for icl, row in enumerate(c2v):
+        nv = int((row >= 0).sum())
+        for f in F2V_BY_NV[nv]:
+            v = [int(row[q]) for q in f if q >= 0]
+            key = tuple(sorted(v))
+            count[key] = count.get(key, 0) + 1
+            faces.append((key, v))
+    sides = [[[], []] for _ in range(6)]
+    for key, v in faces:
+        if count[key] != 1:
+            continue
+        L = lat[v]  # lattice (i, j, k) of each vertex; -1 for cell-centre vertices
+        if (L < 0).any():
+            raise ValueError("boundary face through an interior vertex")
+        for s, (ax, val) in enumerate([(0, 0), (0, nx), (1, 0), (1, ny), (2, 0), (2, nz)]):
+            if (L[:, ax] == val).all():
+                sides[s][len(v) == 4].append(v + [-1] * (4 - len(v)))
+                break
+        else:
+            raise ValueError("open face not on the box surface")
+    patches = []
+    for s in range(6):
+        bf = sides[s][0] + sides[s][1]
+        patches.append(_patch(bc[s], np.array(bf, np.int64).reshape(-1, 4)))
+    return patches
+
+
+def mixed_box(nx, ny, nz, lx, ly, lz, layout="tet_prism", split=None, jitter=0.0, seed=0, bc=None):
+    """A box of mixed cells on one node lattice, conforming everywhere.
+
+    layout "tet_prism": hexes with i < split are cut into 5 tetrahedra (as in
+    tet_box), the others into 2 prisms along x whose triangles carry the
+    diagonal the tetrahedra put on the x = split plane.
+    layout "prism_hex_pyr": for i < split[0] prisms along z, for
+    split[0] <= i < split[1] hexahedra, beyond that each hex is cut into 6
+    pyramids around a vertex added at its centre.
+
+    Cells are ordered by kind as Rocflu numbers them (tets, hexes, prisms,
+    pyramids); rows are padded with -1 to 8 vertices (nvc = 8), boundary
+    faces to 4 (nvf = 4)."""
+    xyz = _node_grid(nx, ny, nz, lx, ly, lz, jitter, seed)
+    hc2v, i, j, k = _hex_c2v(nx, ny, nz)
+    nvx, nvy = nx + 1, ny + 1
+    vi = np.arange(xyz.shape[0])
+    lat = np.stack([vi % nvx, (vi // nvx) % nvy, vi // (nvx * nvy)], axis=1)
+    tets, hexs, pris, pyrs = [], [], [], []
+    extra = []
+    if layout == "tet_prism":
+        ix = nx // 2 if split is None else split
+        par = (i + j + k) & 1
+        for h in range(hc2v.shape[0]):
+            c = hc2v[h]
+            if i[h] < ix:
+                loc = _TETS_EVEN if par[h] == 0 else _TETS_ODD
+                tets.extend(c[loc])
+            else:
+                # diagonal of the x = ix face of hex (ix-1, j, k) in its tet split
+                anti = ((ix - 1 + j[h] + k[h]) & 1) == 0
+                # yz square corners A (j,k), B (j+1,k), C (j+1,k+1), D (j,k+1) on the hex's x- and x+ sides
+                lo = {"A": c[0], "B": c[3], "C": c[7], "D": c[4]}
+                hi = {"A": c[1], "B": c[2], "C": c[6], "D": c[5]}
+                tris = [("A", "B", "D"), ("B", "C", "D")] if anti else [("A", "B", "C"), ("A", "C", "D")]
+                for t in tris:
+                    pris.append([lo[t[0]], lo[t[1]], lo[t[2]], hi[t[0]], hi[t[1]], hi[t[2]]])
+        tets = _orient_tets(np.array(tets, np.int64), xyz).tolist() if tets else []
+    elif layout == "prism_hex_pyr":
+        i1, i2 = (nx // 3, 2 * nx // 3) if split is None else split
+        for h in range(hc2v.shape[0]):
+            c = hc2v[h]
+            if i[h] < i1:
+                # prisms along z: the xy square (0,1,2,3) cut along 0-2, counter-clockwise from +z
+                pris.append([c[0], c[1], c[2], c[4], c[5], c[6]])
+                pris.append([c[0], c[2], c[3], c[4], c[6], c[7]])
+            elif i[h] < i2:
+                hexs.append(list(c))
+            else:
+                ctr = xyz.shape[0] + len(extra)
+                extra.append(xyz[c].mean(axis=0))
+                for f in F2V_HEX:  # base (a, d, c, b) so that face (1, 4, 3, 2) is the hex face (a, b, c, d)
+                    a, b, cc, d = c[f]
+                    pyrs.append([a, d, cc, b, ctr])
+    else:
+        raise ValueError(layout)
+    if extra:
+        xyz = np.ascontiguousarray(np.concatenate([xyz, np.array(extra)]))
+        lat = np.concatenate([lat, np.full((len(extra), 3), -1)])
+    c2v = _pad8(list(tets) + hexs + pris + pyrs)
+    if bc is None:
+        bc = [(BC_OUTFLOW, BCOPT_SUPERSONIC, 0.0)] * 2 + [(BC_SLIPWALL, 0, 0.0)] * 4
+    patches = _mixed_patches(c2v, lat, (nx, ny, nz), bc)
+    return dict(xyz=xyz, c2v=c2v, nvc=8, patches=patches, dims=(nx, ny, nz), lengths=(lx, ly, lz),
+                kinds=dict(tet=len(tets), hex=len(hexs), prism=len(pris), pyramid=len(pyrs)))

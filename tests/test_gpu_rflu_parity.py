@@ -371,3 +371,53 @@ def test_patch_kinds_match_oracle(case, order, porder, oracle_mod):
     else:
         assert same_bits(g.get_rhs(), rhs_o), "rhs: " + first_diff(g.get_rhs(), rhs_o)
         assert same_bits(gcv, o2.cv()), "cv: " + first_diff(gcv, o2.cv())
+
+
+# ---- mixed meshes: tetrahedra + prisms, and prisms + hexahedra + pyramids ----
+MIXED = {lay: mg.mixed_box(6, 4, 4, 0.6, 0.4, 0.4, layout=lay, jitter=0.1, seed=7)
+         for lay in ("tet_prism", "prism_hex_pyr")}
+
+
+@pytest.mark.parametrize("layout", sorted(MIXED))
+def test_mixed_mesh_products_match_oracle(layout, oracle_mod):
+    mesh = MIXED[layout]
+    g = rflu.Region(mesh, spaceOrder=2, reconst=11, spaceDiscr=1)
+    o = oracle_mod.Region(mesh, spaceOrder=2, reconst=11, spaceDiscr=1)
+    m = g.mesh_arrays()
+    f2c, f2v, fn, fc = o.faces()
+    bf2c, bfn, bfc = o.bfaces()
+    vol, cofg = o.cells()
+    assert np.array_equal(m["f2c"], f2c)
+    assert np.array_equal(m["bf2c"], bf2c)
+    for a, b, n in ((m["fn"], fn, "fn"), (m["fc"], fc, "fc"), (m["bfn"], bfn, "bfn"),
+                    (m["bfc"], bfc, "bfc"), (m["vol"], vol, "vol"), (m["cofg"], cofg, "cofg")):
+        assert same_bits(a, b), f"{n}: {first_diff(a, b)}"
+    off, mem, moms = g.stencils()
+    ooff, omem, omoms = o.stencils()
+    assert np.array_equal(off, ooff) and np.array_equal(mem, omem)
+    assert same_bits(moms, omoms), first_diff(moms, omoms)
+
+
+@pytest.mark.parametrize("layout", sorted(MIXED))
+@pytest.mark.parametrize("discr,order,reconst", [(4, 1, 0), (1, 2, 11), (3, 2, 11), (5, 2, 2)])
+def test_mixed_mesh_steps_match_oracle(layout, discr, order, reconst, oracle_mod):
+    mesh = MIXED[layout]
+    prm = dict(spaceDiscr=discr, spaceOrder=order, reconst=reconst, refCp=1004.64, refGamma=1.4, dtImposed=2e-8,
+               cfl=0.8)
+    g, o, cofg = _pair(mesh, **prm)
+    cv = _shock_ic(cofg)
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.RKInitMP(1)
+    g.CellGradientsMP()
+    g.ConvectiveFluxesMP()
+    g.SourceTermsMP()
+    rhs_o = o.residual()
+    assert same_bits(g.get_rhs(), rhs_o), "rhs: " + first_diff(g.get_rhs(), rhs_o)
+    g2, o2, _ = _pair(mesh, **prm)
+    g2.set_state(cv)
+    o2.set_cv(cv)
+    g2.RFLU_TimeStepping(8, use_graph=True)
+    o2.step(8)
+    gcv = g2.get_state()[0]
+    assert same_bits(gcv, o2.cv()), "cv: " + first_diff(gcv, o2.cv())
