@@ -55,6 +55,10 @@ typedef struct picl_params {
                         ProjectParticleGrid :3103-3246, no wall mirror) */
   double filterWidth;  /* projMode 1: filter width, the Gaussian's full width at half maximum */
   double filterAlpha;  /* projMode 1: cut-off; node weights below alpha are dropped (0 < alpha < 1) */
+  /* HEATTRANSFER (ppiclf_user_HeatTransfer.f): 0 off; Nusselt law 1 Stokes,
+   * 2 Ranz-Marshall, 3 Gunn, 4 Fox.  The particle temperature y(JT) then
+   * evolves and the heat flux joins the energy feedback. */
+  int heatTransfer;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

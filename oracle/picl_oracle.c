@@ -552,10 +552,36 @@ double orc_picl_qs_beta(int qsFlag, double rmachp, double rphip, double rep, dou
   return 0.0;
 }
 
+/* ppiclf_user_HT_driver (libpicl/user_files/ppiclf_user_HeatTransfer.f:17-222):
+ * Q = pi kappa d (T_gas - T_p) Nu with the Stokes, Ranz-Marshall, Gunn or Fox
+ * Nusselt law; ppiclF fixes Pr = 0.70 and cp_gas = 1004.64
+ * (ppiclf_user.f:177-178, 310). */
+static double heat_transfer(int flag, double rkappa, double dp, double tgas, double tpart, double rep,
+                            double rphif, double rmachp) {
+  const double rpi = 3.141592653589793116, rpr = 0.70, OneThird = 1.0 / 3.0;
+  double Q_conv = rpi * rkappa * dp * (tgas - tpart);
+  double Nuss = 0.0;
+  switch (flag) {
+    case 1: Nuss = 2.0; break;
+    case 2: Nuss = 2.0 + 0.6 * pow(rep, 0.5) * pow(rpr, OneThird); break;
+    case 3: {
+      double vg = rphif;
+      Nuss = (7.0 - 10.0 * vg + 5.0 * vg * vg) * (1.0 + 0.7 * pow(rep, 0.2) * pow(rpr, OneThird)) +
+             (1.33 - 2.4 * vg + 1.2 * vg * vg) * pow(rep, 0.7) * pow(rpr, OneThird);
+      break;
+    }
+    case 4:
+      Nuss = 2.0 * exp(-rmachp) / (1.0 + 17.0 * rmachp / rep) +
+             0.495 * pow(rpr, OneThird) * pow(rep, 0.55) * ((1.0 + 0.5 * exp(-17.0 * rmachp / rep)) / 1.5);
+      break;
+  }
+  return 0.0 + Q_conv * Nuss;
+}
+
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
- * particle with QS Parmar, AM Parmar (ppiclf_user_AddedMass.f:16-124), the
- * pressure-gradient force, Euler flow (no viscous stress term), no
- * collisions, lift, torque, heat transfer or burning. */
+ * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124)
+ * and pressure-gradient forces and heat transfer, Euler flow (no viscous
+ * stress term), no collisions, lift, torque or burning. */
 static void set_ydot(struct orc_picl *p, int i) {
   const orc_picl_params *P = &p->pp;
   double *y = p->y + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i, *yc = p->ydotc + LRS * (size_t)i;
@@ -587,7 +613,7 @@ static void set_ydot(struct orc_picl *p, int i) {
   double famx = 0.0, famy = 0.0, famz = 0.0, rmass_add = 0.0;
   double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
   const double fcx = 0.0, fcy = 0.0, fcz = 0.0, fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
-  const double liftx = 0.0, lifty = 0.0, liftz = 0.0, qq = 0.0;
+  const double liftx = 0.0, lifty = 0.0, liftz = 0.0;
   double beta; /* ppiclf_user.f:473-485 */
   switch (P->qsFlag) {
     case 1: beta = qs_parmar(rmachp, rphip, rep, rmu, dp); break;
@@ -630,6 +656,11 @@ static void set_ydot(struct orc_picl *p, int i) {
     fdpdz = fdpdz + 0.0;
   }
   double rmass_therm = rmass * rcp_part;
+  double qq = 0.0;
+  if (P->heatTransfer >= 1) {
+    double rkappa = 1004.64 * rmu / 0.70; /* rcp_fluid*rmu/rpr (ppiclf_user.f:310) */
+    qq = heat_transfer(P->heatTransfer, rkappa, dp, rp[R_JT], y[JT], rep, rphif, rmachp);
+  }
   double rmass_omega = rmass * dp * dp / 10.0;
   const double taux = 0.0, tauy = 0.0, tauz = 0.0;
   yd[JX] = y[JVX];
@@ -764,7 +795,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   for (size_t q = 0; q < (size_t)8 * r->nCells; q++)
     if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
-      pp->projMode > 1)
+      pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4)
     return -1;
   if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0)) return -1;
   struct orc_picl *p = calloc(1, sizeof *p);

@@ -36,6 +36,7 @@ typedef struct orc_picl_params {
   int projMode;      /* 0: single-element box; 1: Gaussian filter */
   double filterWidth;  /* Gaussian filter width (full width at half maximum) */
   double filterAlpha;  /* Gaussian cut-off: weights below alpha are dropped */
+  int heatTransfer;  /* 0: off; Nusselt law 1 Stokes, 2 Ranz-Marshall, 3 Gunn, 4 Fox */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index

@@ -33,10 +33,13 @@ namespace {
 struct PiclPhys {
   int rmuFlag, amFlag, pgFlag, feedbackFlag, qsFlag, interpMode;
   double rmuRef, tref, suth, rcpPart;
+  int heatTransfer;
 };
 
 // Rows of PBuf::st
-enum { ROW_X = 0, ROW_Y1 = 6, ROW_YD = 12, ROW_YCP = 18, NROW = 22 };
+// particle state rows: x, y, z, u, v, w (and their y1, ydot), the feedback
+// map, then the temperature's y, y1 and ydot, touched only with heat transfer
+enum { ROW_X = 0, ROW_Y1 = 6, ROW_YD = 12, ROW_YCP = 18, ROW_T = 22, ROW_TY1 = 23, ROW_TYD = 24, NROW = 25 };
 
 struct PBuf {
   double *st = nullptr;  // [NROW][nP]
@@ -47,6 +50,7 @@ struct PBuf {
 
 struct PiclDev {
   int nP = 0, nC = 0;
+  int ht = 0;  // heat transfer on: the temperature rows move with the particles
   PBuf buf[2];
   int *par = nullptr;     // buf[par[0]] is current
   double *ycN = nullptr;  // [4][nP] ydotc of this stage, by current slot
@@ -531,6 +535,29 @@ __device__ __forceinline__ PBuf cur_buf(const PiclDev &P) { return P.par[0] ? P.
 
 // TRI: trilinear interpolation instead of IDW-27; QS: the quasi-steady law
 // (ppiclf_user.f:473-485 numbering, 5 = Schiller-Naumann)
+// ppiclf_user_HT_driver (libpicl/user_files/ppiclf_user_HeatTransfer.f:17-222):
+// Q = pi kappa d (T_gas - T_p) Nu, Nusselt law 1 Stokes, 2 Ranz-Marshall,
+// 3 Gunn, 4 Fox; ppiclF fixes Pr = 0.70 (ppiclf_user.f:177).
+__device__ double heat_transfer(int flag, double rkappa, double dp, double tgas, double tpart, double rep,
+                                double rphif, double rmachp) {
+  const double rpi = 3.141592653589793116, rpr = 0.70, OneThird = 1.0 / 3.0;
+  double Q_conv = rpi * rkappa * dp * (tgas - tpart);
+  double Nuss = 0.0;
+  if (flag == 1) {
+    Nuss = 2.0;
+  } else if (flag == 2) {
+    Nuss = 2.0 + 0.6 * pow(rep, 0.5) * pow(rpr, OneThird);
+  } else if (flag == 3) {
+    double vg = rphif;
+    Nuss = (7.0 - 10.0 * vg + 5.0 * vg * vg) * (1.0 + 0.7 * pow(rep, 0.2) * pow(rpr, OneThird)) +
+           (1.33 - 2.4 * vg + 1.2 * vg * vg) * pow(rep, 0.7) * pow(rpr, OneThird);
+  } else {
+    Nuss = 2.0 * exp(-rmachp) / (1.0 + 17.0 * rmachp / rep) +
+           0.495 * pow(rpr, OneThird) * pow(rep, 0.55) * ((1.0 + 0.5 * exp(-17.0 * rmachp / rep)) / 1.5);
+  }
+  return 0.0 + Q_conv * Nuss;
+}
+
 template <int TRI, int QS>
 __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
   int t = blockIdx.x * BLK + threadIdx.x;
@@ -677,7 +704,7 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   double rep = vmag * dpp * rhof / rmu;
   double rphip = v[9];
   double rmachp = vmag / v[7];
-  double ydn[6];
+  double ydn[6], ydT = 0.0;
   bool skip = vmag <= 1.0e-3;  // keeps the previous ydot; ydotc stays 0
   double yc[4] = {0.0, 0.0, 0.0, 0.0};
   if (!skip) {
@@ -730,18 +757,26 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
     ydn[3] = (fqsx + famx + fdpdx + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
     ydn[4] = (fqsy + famy + fdpdy + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
     ydn[5] = (fqsz + famz + fdpdz + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
+    double qq = 0.0;
+    if (Ph.heatTransfer) {
+      double rkappa = 1004.64 * rmu / 0.70;  // rcp_fluid*rmu/rpr (ppiclf_user.f:310)
+      qq = heat_transfer(Ph.heatTransfer, rkappa, dpp, v[8], st[ROW_T * nP + t], rep, rphif, rmachp);
+      ydT = qq / (rmass * Ph.rcpPart);
+      st[ROW_TYD * nP + t] = ydT;
+    }
     if (Ph.feedbackFlag == 1) {
       yc[0] = spl * (ydn[3] * rmass - 0.0);
       yc[1] = spl * (ydn[4] * rmass - 0.0);
       yc[2] = spl * (ydn[5] * rmass - 0.0);
       yc[3] = spl * ((fqsx + 0.0 + 0.0) * yv[0] + (fqsy + 0.0 + 0.0) * yv[1] + (fqsz + 0.0 + 0.0) * yv[2] +
-                     famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + 0.0);
+                     famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + qq);
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) st[(ROW_YD + k) * nP + t] = ydn[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 6; k++) ydn[k] = st[(ROW_YD + k) * nP + t];
+    if (Ph.heatTransfer) ydT = st[ROW_TYD * nP + t];
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = yc[k];
@@ -758,6 +793,12 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
     double y = st[(ROW_X + k) * nP + t];
     st[(ROW_X + k) * nP + t] = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
     st[(ROW_Y1 + k) * nP + t] = ydn[k];
+  }
+  if (Ph.heatTransfer) {
+    double y1 = istage == 1 ? 0.0 : st[ROW_TY1 * nP + t];
+    double y = st[ROW_T * nP + t];
+    st[ROW_T * nP + t] = -(k0 * y1) + 1.0 * y + k2 * ydT;
+    st[ROW_TY1 * nP + t] = ydT;
   }
 }
 
@@ -1056,6 +1097,8 @@ __global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
   for (int k = 0; k < 4; k++) X.st[(ROW_YCP + k) * nP + pos] = t < nA ? P.ycN[k * nP + t] : C.st[(ROW_YCP + k) * nP + t];
 #pragma unroll
   for (int k = 0; k < 4; k++) X.pc[k * nP + pos] = C.pc[k * nP + t];
+  if (P.ht)
+    for (int r = ROW_T; r < NROW; r++) X.st[r * nP + pos] = C.st[r * nP + t];
   X.pid[pos] = C.pid[t];
   X.elem[pos] = C.elem[t];
 }
@@ -1275,6 +1318,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (h->prm.patchOrder != 1) return set_err(h, RFLU_GPU_EINVAL, "PICL is built for patchOrder 1");
   if (h->prm.fluxPrecision != 0) return set_err(h, RFLU_GPU_EINVAL, "PICL runs in FP64 (fluxPrecision 0)");
   if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "particles attach to a single region, not a batch");
+  if (pp->heatTransfer < 0 || pp->heatTransfer > 4)
+    return set_err(h, RFLU_GPU_EINVAL, "heatTransfer must be 0 (off), 1 (Stokes), 2 (Ranz-Marshall), 3 (Gunn) or 4 (Fox)");
   if (pp->qsFlag < 1 || pp->qsFlag > 5)
     return set_err(h, RFLU_GPU_EINVAL, "qsFlag must be 1 (Parmar), 2 (Osnes), 3 (modified Parmar), 4 (Gidaspow) or 5 "
                                        "(Schiller-Naumann)");
@@ -1290,7 +1335,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   P.nP = nP;
   P.nC = N;
   s->phys = PiclPhys{pp->rmuFlag, pp->amFlag, pp->pgFlag, pp->feedbackFlag, pp->qsFlag, pp->interpMode,
-                     pp->rmuRef, pp->tref, pp->suth, pp->rcpPart};
+                     pp->rmuRef, pp->tref, pp->suth, pp->rcpPart, pp->heatTransfer};
+  s->P.ht = pp->heatTransfer != 0 ? 1 : 0;
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
   std::vector<double> X(24 * (size_t)N), bbox(8 * (size_t)N, 0.0), cen(4 * (size_t)N, 0.0), rexp(N);
@@ -1362,6 +1408,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   std::vector<double> st(NROW * (size_t)nP, 0.0), pcs(4 * (size_t)nP);
   for (int p = 0; p < nP; p++) {
     for (int k = 0; k < 6; k++) st[(ROW_X + k) * (size_t)nP + p] = y[12 * (size_t)p + k];
+    st[ROW_T * (size_t)nP + p] = y[12 * (size_t)p + 6];
     for (int k = 0; k < 4; k++) pcs[k * (size_t)nP + p] = pconst[4 * (size_t)p + k];
   }
   std::vector<int> ident(nP), minus1(nP, -1);
@@ -1527,6 +1574,14 @@ int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, 
   fill(y1, ROW_Y1, 6, 0, false);
   fill(ydot, ROW_YD, 6, 0, false);
   fill(ydotc, ROW_YCP, 4, 3, false);
+  if (P.ht) {  // the particle temperature evolves
+    for (size_t t = 0; t < nP; t++) {
+      size_t p = pid[t];
+      if (y) y[12 * p + 6] = st[ROW_T * nP + t];
+      if (y1) y1[12 * p + 6] = st[ROW_TY1 * nP + t];
+      if (ydot) ydot[12 * p + 6] = st[ROW_TYD * nP + t];
+    }
+  }
   if (elem)
     for (size_t t = 0; t < nP; t++) elem[pid[t]] = ev[t] < 0 ? -1 : h->perm[ev[t]];
   return 0;

@@ -258,3 +258,32 @@ def test_coupled_steps_with_weno_reconstruction(oracle_mod, discr):
         assert ok, (k, err)
     ok, err = close(g.get_state()[0], o.cv())
     assert ok, err
+
+
+@pytest.mark.parametrize("ht", [1, 2, 3, 4])
+def test_coupled_steps_with_heat_transfer(oracle_mod, ht):
+    """HEATTRANSFER 1-4: the particle temperature evolves and the heat flux
+    joins the energy feedback (a dilute curtain in the post-shock stream)."""
+    n, h = 12, 0.0005
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=n, ny=4, nz=4, npart=500, jitter=0.05, seed=4, phi=0.01,
+                                                      x_shock=0.9 * n * h, x_curtain=0.3 * n * h)
+    y = y.copy()
+    y[:, 6] = 250.0
+    picl.update(heatTransfer=ht)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(6, use_graph=True)
+    o.step(6)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(op["y"][:, 6] - 250.0).max() > 0.0  # the temperature moved
+    for k in ("y", "y1", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err

@@ -140,3 +140,59 @@ def test_trilinear_evaluation_returns_the_element_value(oracle_mod):
     ref = fields[elem]
     scale = np.maximum(np.abs(ref), 1e-300)
     assert (np.abs(rp - ref) / scale).max() <= 8 * np.finfo(float).eps
+
+
+# ---- heat transfer (HEATTRANSFER 1-4, ppiclf_user_HeatTransfer.f) ----
+def _ht_stage(oracle_mod, ht, tp=250.0):
+    """One coupled RK stage of a dilute curtain (phi 1 %) sitting in the
+    post-shock stream; returns (particles, rprop, picl params) of that stage."""
+    n, h = 12, 0.0005
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=n, ny=4, nz=4, npart=60, jitter=0.05, seed=4, phi=0.01,
+                                                      x_shock=0.9 * n * h, x_curtain=0.3 * n * h)
+    y = y.copy()
+    y[:, 6] = tp
+    picl.update(rmuFlag=0, heatTransfer=ht)
+    o = oracle_mod.Region(mesh, **params)
+    o.set_cv(ic(o.cells()[1]))
+    o.picl_init(y, pc, **picl)
+    o.rk_stage(1, 1e-8)
+    return o.picl_particles(), o.picl_rprop(), picl
+
+
+@pytest.mark.parametrize("ht", [1, 2, 3, 4])
+def test_heat_transfer_nusselt_laws(oracle_mod, ht):
+    p, rp, picl = _ht_stage(oracle_mod, ht)
+    y0 = p["y"][:, 6] - (8.0 / 15.0) * 1e-8 * p["ydot"][:, 6]  # stage-1 input temperature
+    rhop, rhof, dp, volp, phip = rp[:, 0], rp[:, 1], rp[:, 2], rp[:, 3], rp[:, 4]
+    u = rp[:, 5:8]
+    cs, tg = rp[:, 8], rp[:, 23]
+    vrel = np.linalg.norm(u - (p["y"][:, 3:6] - (8.0 / 15.0) * 1e-8 * p["ydot"][:, 3:6]), axis=1)
+    mu = picl["rmuRef"]
+    re = np.maximum(0.1, vrel * dp * rhof / mu)
+    ma = vrel / cs
+    vg = 1.0 - np.minimum(phip, 0.62)
+    kappa = 1004.64 * mu / 0.70
+    moving = vrel > 1e-3
+    assert moving.sum() >= 10
+    nu = p["ydot"][:, 6] * rhop * volp * picl["rcpPart"] / (math.pi * kappa * dp * (tg - y0))
+    pr3 = 0.7 ** (1.0 / 3.0)
+    expect = {1: np.full_like(re, 2.0),                                          # conduction, Nu = 2
+              2: 2.0 + 0.6 * np.sqrt(re) * pr3,                                  # Ranz-Marshall
+              3: (7 - 10 * vg + 5 * vg ** 2) * (1 + 0.7 * re ** 0.2 * pr3)
+              + (1.33 - 2.4 * vg + 1.2 * vg ** 2) * re ** 0.7 * pr3,            # Gunn
+              4: 2 * np.exp(-ma) / (1 + 17 * ma / re)
+              + 0.495 * pr3 * re ** 0.55 * (1 + 0.5 * np.exp(-17 * ma / re)) / 1.5}[ht]  # Fox
+    assert np.allclose(nu[moving], expect[moving], rtol=1e-9, atol=0)
+    # a hotter particle gives heat to the gas, a colder one takes it
+    assert np.all(np.sign(p["ydot"][moving, 6]) == np.sign(tg - y0)[moving])
+
+
+def test_heat_transfer_feeds_the_gas_energy_back(oracle_mod):
+    on, _, picl = _ht_stage(oracle_mod, 2)
+    off, rp, _ = _ht_stage(oracle_mod, 0)
+    rmass_therm = rp[:, 0] * rp[:, 3] * picl["rcpPart"]
+    spl = rp[:, 21]
+    d = on["ydotc"][:, 6] - off["ydotc"][:, 6]
+    q = spl * rmass_therm * on["ydot"][:, 6]
+    assert np.allclose(d, q, rtol=1e-9, atol=1e-12 * np.abs(q).max())
+    assert np.array_equal(on["ydot"][:, 3:6], off["ydot"][:, 3:6])  # momentum untouched
