@@ -59,6 +59,15 @@ typedef struct picl_params {
    * 2 Ranz-Marshall, 3 Gunn, 4 Fox.  The particle temperature y(JT) then
    * evolves and the heat flux joins the energy feedback. */
   int heatTransfer;
+  /* PERIODICX/Y/Z (global%piclPeriodicXFlag..., PICL_TEMP_InitSolver.F90:
+   * 234-262, 715-734): 1 makes the axis periodic for the particles over the
+   * mesh's vertex minimum to maximum.  After every RK stage a particle beyond
+   * one bound re-enters as far inside the other
+   * (ppiclf_solve_InvokeLinearPeriodic, ppiclf_solve.f:1319-1353), and the
+   * Gaussian projection also deposits the particles' images shifted by one
+   * period (the ghost particles of ppiclf_comm_CreateGhost,
+   * ppiclf_comm.f:1115-1530). */
+  int periodicX, periodicY, periodicZ;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

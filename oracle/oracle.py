@@ -30,7 +30,8 @@ class OrcPiclParams(C.Structure):
     _fields_ = [("rmuFlag", C.c_int), ("rmuRef", C.c_double), ("tref", C.c_double), ("suth", C.c_double),
                 ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
                 ("feedbackFlag", C.c_int), ("interpMode", C.c_int), ("projMode", C.c_int),
-                ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int)]
+                ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
+                ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,

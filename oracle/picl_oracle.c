@@ -55,6 +55,8 @@ struct orc_picl {
   double *rexp;    /* 1 / (12-tet hex volume) */
   double *proFld;  /* 8 * NPRO per element: field jj at node n in [8 * jj + n] */
   double gRdum, gMult, gD2chk; /* Gaussian filter: 1/(-2 sigma^2), normalisation, cut-off^2 */
+  int per[3];                  /* linear periodicity per axis */
+  double perLo[3], perHi[3];   /* ppiclf_xdrange: the mesh's vertex bounds */
   double *fld;     /* ORC_PICL_NFLD interpolation inputs per element */
 };
 
@@ -324,6 +326,9 @@ static void init_solve(struct orc_picl *p, int interp) {
   if (p->pp.projMode == 1) {
     /* Gaussian filter, ppiclf_solve.f:3103-3109, 3118-3130, 3196-3246 with
      * iwallm = 0: every element node sums the particles within the cut-off.
+     * On a periodic axis the particles' images shifted by +-(hi - lo) count
+     * too: ppiclf_comm_CreateGhost (ppiclf_comm.f:1115-1530) makes them as
+     * ghost particles, shifted by ppiclf_comm_CheckPeriodicBC (:2136-2206).
      * The sums are compensated (Neumaier), so that their value does not
      * depend on the order of the terms (ppiclF's order over its bins and
      * ghost particles is not restated); here: ascending particle index. */
@@ -334,18 +339,28 @@ static void init_solve(struct orc_picl *p, int interp) {
         double cmp[NPRO] = {0.0};
         for (int ip = 0; ip < p->n; ip++) {
           if (p->lost[ip]) continue;
-          const double *xp = p->y + LRS * (size_t)ip;
-          double rdist2 = (xn[0] - xp[0]) * (xn[0] - xp[0]) + (xn[1] - xp[1]) * (xn[1] - xp[1]);
-          rdist2 = rdist2 + (xn[2] - xp[2]) * (xn[2] - xp[2]);
-          if (rdist2 > p->gD2chk) continue;
-          double rexp = exp(rdist2 * p->gRdum);
-          for (int jj = 0; jj < NPRO; jj++) {
-            double x = (map[NPRO * (size_t)ip + jj] * p->gMult) * rexp, s = nb[8 * jj + n], t = s + x;
-            if (fabs(s) >= fabs(x))
-              cmp[jj] = cmp[jj] + ((s - t) + x);
-            else
-              cmp[jj] = cmp[jj] + ((x - t) + s);
-            nb[8 * jj + n] = t;
+          const double *y = p->y + LRS * (size_t)ip;
+          for (int img = 0; img < 27; img++) {
+            double xp[3];
+            int skip = 0;
+            for (int d = 0, q = img; d < 3; d++, q /= 3) {
+              int sd = q % 3 - 1;
+              if (sd != 0 && !p->per[d]) skip = 1;
+              xp[d] = sd == 0 ? y[d] : y[d] + (double)sd * (p->perHi[d] - p->perLo[d]);
+            }
+            if (skip) continue;
+            double rdist2 = (xn[0] - xp[0]) * (xn[0] - xp[0]) + (xn[1] - xp[1]) * (xn[1] - xp[1]);
+            rdist2 = rdist2 + (xn[2] - xp[2]) * (xn[2] - xp[2]);
+            if (rdist2 > p->gD2chk) continue;
+            double rexp = exp(rdist2 * p->gRdum);
+            for (int jj = 0; jj < NPRO; jj++) {
+              double x = (map[NPRO * (size_t)ip + jj] * p->gMult) * rexp, s = nb[8 * jj + n], t = s + x;
+              if (fabs(s) >= fabs(x))
+                cmp[jj] = cmp[jj] + ((s - t) + x);
+              else
+                cmp[jj] = cmp[jj] + ((x - t) + s);
+              nb[8 * jj + n] = t;
+            }
           }
         }
         for (int jj = 0; jj < NPRO; jj++) nb[8 * jj + n] = nb[8 * jj + n] + cmp[jj];
@@ -688,6 +703,21 @@ static void set_ydot(struct orc_picl *p, int i) {
   }
 }
 
+/* ppiclf_solve_InvokeLinearPeriodic (ppiclf_solve.f:1319-1353), reached
+ * from RemoveParticle (ppiclf.f:13140-13144) after every stage's update
+ * (ppiclf_solve.f:1933): each axis on its own, a particle beyond one bound
+ * re-enters the same distance inside the other.  A non-periodic axis keeps
+ * ppiclF's +-1e20 range, so nothing happens there. */
+static void linear_periodic(const struct orc_picl *p, double *y) {
+  for (int d = 0; d < 3; d++) {
+    if (!p->per[d]) continue;
+    if (y[d] < p->perLo[d])
+      y[d] = p->perHi[d] - fabs(p->perLo[d] - y[d]);
+    else if (y[d] > p->perHi[d])
+      y[d] = p->perLo[d] + fabs(y[d] - p->perHi[d]);
+  }
+}
+
 /* ppiclf_solve_IntegrateRK3s_Rocflu + SetRK3Coeff (imethod 2):
  * ppiclf_solve.f:1845-1936, 2774-2793 */
 static void integrate_particle(struct orc_picl *p, int istage, double dt) {
@@ -704,6 +734,7 @@ static void integrate_particle(struct orc_picl *p, int istage, double dt) {
     double *y = p->y + LRS * (size_t)i, *y1 = p->y1 + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i;
     for (int j = 0; j < LRS; j++) y[j] = -(k[0] * y1[j]) + k[1] * y[j] + k[2] * yd[j];
     for (int j = 0; j < LRS; j++) y1[j] = yd[j];
+    linear_periodic(p, y);
   }
 }
 
@@ -797,9 +828,26 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4)
     return -1;
+  const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};
+  double vlo[3] = {1e300, 1e300, 1e300}, vhi[3] = {-1e300, -1e300, -1e300};
+  for (int v = 0; v < r->nVert; v++)
+    for (int d = 0; d < 3; d++) {
+      double x = r->xyz[3 * (size_t)v + d];
+      if (x < vlo[d]) vlo[d] = x;
+      if (x > vhi[d]) vhi[d] = x;
+    }
+  for (int d = 0; d < 3; d++) /* ppiclf_solve_InitPeriodicX/Y/Z: xl < xr */
+    if (perFlag[d] < 0 || perFlag[d] > 1 || (perFlag[d] && !(vlo[d] < vhi[d]))) return -1;
   if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0)) return -1;
   struct orc_picl *p = calloc(1, sizeof *p);
   p->pp = *pp;
+  /* PICL_TEMP_InitSolver.F90:234-262: the range of a periodic axis is the
+   * mesh's vertex minimum and maximum on it */
+  for (int d = 0; d < 3; d++) {
+    p->per[d] = perFlag[d];
+    p->perLo[d] = vlo[d];
+    p->perHi[d] = vhi[d];
+  }
   if (pp->projMode == 1) {
     /* ppiclf_solve_InitGaussianFilter (:1520-1576) and the constants of
      * ProjectParticleGrid (:3103-3109) */

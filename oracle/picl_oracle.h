@@ -37,6 +37,8 @@ typedef struct orc_picl_params {
   double filterWidth;  /* Gaussian filter width (full width at half maximum) */
   double filterAlpha;  /* Gaussian cut-off: weights below alpha are dropped */
   int heatTransfer;  /* 0: off; Nusselt law 1 Stokes, 2 Ranz-Marshall, 3 Gunn, 4 Fox */
+  int periodicX, periodicY, periodicZ; /* PERIODICX/Y/Z: particles wrap across the mesh's
+                                          vertex bounds on that axis */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index

@@ -34,6 +34,8 @@ struct PiclPhys {
   int rmuFlag, amFlag, pgFlag, feedbackFlag, qsFlag, interpMode;
   double rmuRef, tref, suth, rcpPart;
   int heatTransfer;
+  int per[3];                 // linear periodicity per axis
+  double perLo[3], perHi[3];  // ppiclf_xdrange: the mesh's vertex bounds
 };
 
 // Rows of PBuf::st
@@ -82,6 +84,10 @@ struct PiclDev {
   double *vproj = nullptr;  // [5][nV]
   int *vperm = nullptr;     // vertices in Morton order (the gather's thread order)
   double gRdum = 0, gMult = 0, gD2chk = 0, gReach = 0, gReach2 = 0;
+  // periodic images of the Gaussian deposit: shift hi - lo per periodic
+  // axis (0 elsewhere); cbTop = the largest centroid bin coordinate
+  double gShift[3] = {0, 0, 0}, cbTop[3] = {0, 0, 0};
+  int gImg = 0;  // any periodic axis
 };
 
 // ---------------------------------------------------------------------------
@@ -791,7 +797,16 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   for (int k = 0; k < 6; k++) {
     double y1 = istage == 1 ? 0.0 : st[(ROW_Y1 + k) * nP + t];
     double y = st[(ROW_X + k) * nP + t];
-    st[(ROW_X + k) * nP + t] = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
+    double yn = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
+    // ppiclf_solve_InvokeLinearPeriodic (ppiclf_solve.f:1319-1353) after the
+    // update (:1933): beyond one bound, re-enter as far inside the other
+    if (k < 3 && Ph.per[k]) {
+      if (yn < Ph.perLo[k])
+        yn = Ph.perHi[k] - fabs(Ph.perLo[k] - yn);
+      else if (yn > Ph.perHi[k])
+        yn = Ph.perLo[k] + fabs(yn - Ph.perHi[k]);
+    }
+    st[(ROW_X + k) * nP + t] = yn;
     st[(ROW_Y1 + k) * nP + t] = ydn[k];
   }
   if (Ph.heatTransfer) {
@@ -1020,16 +1035,28 @@ __global__ void __launch_bounds__(BLK) k_gauss(PiclDev P) {
   int v = live ? P.vperm[vt] : 0;
   const size_t nV = P.nV;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, cmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (live) {
-    double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
+  // image im: shift (im % 3 - 1, im / 3 % 3 - 1, im / 9 - 1) periods; a
+  // particle image x_p + s counts at the vertex when x_p lies within reach
+  // of xv - s
+  const int nImg = P.gImg ? 27 : 1;
+  for (int im = 0; im < nImg && live; im++) {
+    double sh[3];
+    bool skip = false;
     int blo[3], bhi[3];
+    double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
 #pragma unroll
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0, r = im; l < 3; l++, r /= 3) {
+      int sd = P.gImg ? r % 3 - 1 : 0;
+      if (sd != 0 && P.gShift[l] == 0.0) skip = true;
+      sh[l] = sd == 0 ? 0.0 : (double)sd * P.gShift[l];
       double top = (double)(P.cbN[l] - 1);
-      double a = (xv[l] - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xv[l] + P.gReach - P.cbLo[l]) / P.cbH[l];
+      double xs = xv[l] - sh[l];
+      double a = (xs - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xs + P.gReach - P.cbLo[l]) / P.cbH[l];
+      if (b < 0.0 || a > P.cbTop[l]) skip = true;  // no centroid within reach
       blo[l] = (int)fmin(fmax(a, 0.0), top);
       bhi[l] = (int)fmin(fmax(b, 0.0), top);
     }
+    if (skip) continue;
     for (int bz = blo[2]; bz <= bhi[2]; bz++)
       for (int by = blo[1]; by <= bhi[1]; by++)
         for (int bx = blo[0]; bx <= bhi[0]; bx++) {
@@ -1038,14 +1065,17 @@ __global__ void __launch_bounds__(BLK) k_gauss(PiclDev P) {
           for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
             const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)q);
             double2 c01 = cp[0], c23 = cp[1];
-            double t0 = c01.x - xv[0], t1 = c01.y - xv[1], t2 = c23.x - xv[2];
+            double t0 = c01.x - (xv[0] - sh[0]), t1 = c01.y - (xv[1] - sh[1]), t2 = c23.x - (xv[2] - sh[2]);
             if (t0 * t0 + t1 * t1 + t2 * t2 > P.gReach2) continue;
             int e = (int)(unsigned)(__double_as_longlong(c23.y) & 0xffffffffll);
             int p1 = P.off[e + 1];
             for (int pos = P.off[e] + sub; pos < p1; pos += GAUSS_L) {
               const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
               double2 g0 = g[0], g1 = g[1];
-              double dx = xv[0] - g0.x, dy = xv[1] - g0.y, dz = xv[2] - g1.x;
+              // the image's coordinates as ppiclF's ghost holds them, x_p + s
+              double dx = xv[0] - (sh[0] == 0.0 ? g0.x : g0.x + sh[0]);
+              double dy = xv[1] - (sh[1] == 0.0 ? g0.y : g0.y + sh[1]);
+              double dz = xv[2] - (sh[2] == 0.0 ? g1.x : g1.x + sh[2]);
               double rdist2 = dx * dx + dy * dy;
               rdist2 = rdist2 + dz * dz;
               if (rdist2 > P.gD2chk) continue;
@@ -1329,6 +1359,21 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     return set_err(h, RFLU_GPU_EINVAL, "the Gaussian filter needs filterWidth > 0 and 0 < filterAlpha < 1");
   if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
   if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
+  // PICL_TEMP_InitSolver.F90:234-262: a periodic axis spans the mesh's
+  // vertex minimum to maximum (ppiclf_solve_InitPeriodicX/Y/Z need xl < xr)
+  const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};
+  double vlo[3] = {1e300, 1e300, 1e300}, vhi[3] = {-1e300, -1e300, -1e300};
+  for (size_t v = 0; v < h->xyz.size() / 3; v++)
+    for (int l = 0; l < 3; l++) {
+      vlo[l] = std::min(vlo[l], h->xyz[3 * v + l]);
+      vhi[l] = std::max(vhi[l], h->xyz[3 * v + l]);
+    }
+  for (int l = 0; l < 3; l++) {
+    if (perFlag[l] < 0 || perFlag[l] > 1)
+      return set_err(h, RFLU_GPU_EINVAL, "periodicX/Y/Z must be 0 or 1");
+    if (perFlag[l] && !(vlo[l] < vhi[l]))
+      return set_err(h, RFLU_GPU_EINVAL, "a periodic axis needs a mesh of positive extent along it");
+  }
   const int N = h->d.nC, nP = nPart;
   PiclState *s = new PiclState();
   PiclDev &P = s->P;
@@ -1336,6 +1381,11 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   P.nC = N;
   s->phys = PiclPhys{pp->rmuFlag, pp->amFlag, pp->pgFlag, pp->feedbackFlag, pp->qsFlag, pp->interpMode,
                      pp->rmuRef, pp->tref, pp->suth, pp->rcpPart, pp->heatTransfer};
+  for (int l = 0; l < 3; l++) {
+    s->phys.per[l] = perFlag[l];
+    s->phys.perLo[l] = vlo[l];
+    s->phys.perHi[l] = vhi[l];
+  }
   s->P.ht = pp->heatTransfer != 0 ? 1 : 0;
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
@@ -1400,6 +1450,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     P.cbLo[l] = clo[l];
     P.cbH[l] = r > 0 ? r : (maxExt[l] > 0 ? maxExt[l] : 1.0);
     P.cbN[l] = std::max(1, std::min(4096, (int)std::floor((chi[l] - clo[l]) / P.cbH[l]) + 1));
+    P.cbTop[l] = (chi[l] - clo[l]) / P.cbH[l];
   }
   std::vector<int> idOrder(N);
   for (int i = 0; i < N; i++) idOrder[i] = i;
@@ -1476,6 +1527,12 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     P.gReach = (d2chk + hmax) * (1.0 + 1.0e-6);
     P.gReach2 = P.gReach * P.gReach;
     P.gauss = 1;
+    // ghost particles across periodic bounds (ppiclf_comm.f:1171-1185,
+    // CheckPeriodicBC :2136-2206): images shifted by +-(hi - lo)
+    for (int l = 0; l < 3; l++) {
+      P.gShift[l] = perFlag[l] ? vhi[l] - vlo[l] : 0.0;
+      if (perFlag[l]) P.gImg = 1;
+    }
     P.nV = (int)(h->xyz.size() / 3);
     P.nCB = P.cbN[0] * P.cbN[1] * P.cbN[2];
     std::vector<double> vX(3 * (size_t)P.nV);

@@ -287,3 +287,45 @@ def test_coupled_steps_with_heat_transfer(oracle_mod, ht):
         assert ok, (k, err)
     ok, err = close(g.get_state()[0], o.cv())
     assert ok, err
+
+
+@pytest.mark.parametrize("proj", [0, 1])
+def test_periodic_particles_match_oracle(oracle_mod, proj):
+    """PERIODICY and PERIODICZ: a quarter of the particles start within 1e-5
+    of a y or z face moving outward at 300 m/s, leave during the run and
+    re-enter at the opposite face; with the Gaussian filter their images
+    across the faces deposit too."""
+    n, h = 12, 0.0005
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=n, ny=4, nz=4, npart=400, jitter=0.05, seed=4)
+    rng = np.random.default_rng(11)
+    y = y.copy()
+    y[:, 4] = rng.uniform(-300.0, 300.0, len(y))
+    y[:, 5] = rng.uniform(-300.0, 300.0, len(y))
+    edge = np.arange(len(y)) % 4 == 0
+    for c in (4, 5):  # fast enough to cross within the run
+        y[edge, c] = np.sign(y[edge, c]) * rng.uniform(200.0, 300.0, edge.sum())
+    hi_y = edge & (y[:, 4] > 0)
+    y[edge, 1] = np.where(y[edge, 4] > 0, 4 * h - 1e-5 * rng.random(edge.sum()), 1e-5 * rng.random(edge.sum()))
+    y[edge, 2] = np.where(y[edge, 5] > 0, 4 * h - 1e-5 * rng.random(edge.sum()), 1e-5 * rng.random(edge.sum()))
+    picl.update(periodicY=1, periodicZ=1, projMode=proj, filterWidth=2 * h, filterAlpha=1e-2)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(6, use_graph=True)
+    o.step(6)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.all(gp["elem"] >= 0)
+    assert np.all(gp["y"][hi_y, 1] < 2 * h)  # left through y = 4h, back in near y = 0
+    for k in ("y", "y1", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    if proj == 1:
+        ok, err = close(g.picl_nodes().reshape(-1, 40), o.picl_nodes().reshape(-1, 40))
+        assert ok, err
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err
