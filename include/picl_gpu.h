@@ -68,6 +68,11 @@ typedef struct picl_params {
    * period (the ghost particles of ppiclf_comm_CreateGhost,
    * ppiclf_comm.f:1115-1530). */
   int periodicX, periodicY, periodicZ;
+  /* ViscousUnsteady_flag (global%piclViscousUnsteady,
+   * PICL_TEMP_InitSolver.F90:205): 1 adds the viscous-unsteady force of
+   * ppiclf_user_ViscousUnsteady.f, a history kernel over the last 5 steps'
+   * d(rho u)/dt of gas and particle (PPICLF_VU = 5). */
+  int viscousUnsteady;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

@@ -31,7 +31,8 @@ class OrcPiclParams(C.Structure):
                 ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
                 ("feedbackFlag", C.c_int), ("interpMode", C.c_int), ("projMode", C.c_int),
                 ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
-                ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int)]
+                ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
+                ("viscousUnsteady", C.c_int)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
@@ -104,6 +105,9 @@ def lib():
         L.orc_picl_get_nodes.argtypes = [P, P]
         L.orc_picl_qs_beta.argtypes = [I] + [C.c_double] * 7
         L.orc_picl_qs_beta.restype = C.c_double
+        L.orc_picl_vu_force.argtypes = [I, P, P, P] + [C.c_double] * 6 + [P]
+        L.orc_picl_vu_state.argtypes = [P, P, P]
+        L.orc_picl_vu_state.restype = I
         _LIB = L
     return _LIB
 
@@ -254,6 +258,14 @@ class Region:
         lib().orc_picl_get_particles(self.h, _p(y), _p(y1), _p(yd), _p(yc), _p(elem))
         return dict(y=y, y1=y1, ydot=yd, ydotc=yc, elem=elem)
 
+    def picl_vu_state(self):
+        """Viscous-unsteady bookkeeping: (nTimeBH, timeBH[5], mixt[n, 3, 5],
+        plag[n, 3, 5])."""
+        t = np.zeros(5)
+        h = np.zeros((self.nPart, 2, 3, 5))
+        n = lib().orc_picl_vu_state(self.h, _p(t), _p(h))
+        return n, t, h[:, 0], h[:, 1]
+
     def picl_rprop(self):
         a = np.zeros((self.nPart, 48))
         lib().orc_picl_get_rprop(self.h, _p(a))
@@ -313,6 +325,19 @@ def flux_point(discr, n, L, R, **params):
     L = np.ascontiguousarray(L, np.float64)
     R = np.ascontiguousarray(R, np.float64)
     lib().orc_flux_point(C.byref(prm), discr, _p(n), _p(L), _p(R), _p(out))
+    return out
+
+
+def vu_force(nTimeBH, timeBH, mixt, plag, dt, rmu, rhof, dp, vmag, rphip):
+    """Viscous-unsteady force of one particle from its history: timeBH (5),
+    mixt and plag (3 x 5, entry k of component c at [c, k])."""
+    t = np.ascontiguousarray(timeBH, dtype=np.float64)
+    m = np.ascontiguousarray(mixt, dtype=np.float64)
+    q = np.ascontiguousarray(plag, dtype=np.float64)
+    assert t.shape == (5,) and m.shape == (3, 5) and q.shape == (3, 5)
+    out = np.zeros(3)
+    lib().orc_picl_vu_force(int(nTimeBH), _p(t), _p(m), _p(q), float(dt), float(rmu), float(rhof), float(dp),
+                            float(vmag), float(rphip), _p(out))
     return out
 
 

@@ -55,6 +55,12 @@ struct orc_picl {
   double *rexp;    /* 1 / (12-tet hex volume) */
   double *proFld;  /* 8 * NPRO per element: field jj at node n in [8 * jj + n] */
   double gRdum, gMult, gD2chk; /* Gaussian filter: 1/(-2 sigma^2), normalisation, cut-off^2 */
+  /* viscous-unsteady history (ppiclF's rprop3, drudtMixt/drudtPlag): per
+   * particle VU_N values of d(rho u)/dt of the gas at [c * VU_N + k] and of
+   * the particle at [3 * VU_N + c * VU_N + k], k = 0 the newest */
+  double *hist;
+  int nTimeBH;               /* ppiclf_nTimeBH */
+  double timeBH[5];          /* ppiclf_timeBH: age of each history entry */
   int per[3];                  /* linear periodicity per axis */
   double perLo[3], perHi[3];   /* ppiclf_xdrange: the mesh's vertex bounds */
   double *fld;     /* ORC_PICL_NFLD interpolation inputs per element */
@@ -63,7 +69,7 @@ struct orc_picl {
 void orc__picl_free(struct orc_picl *p) {
   if (!p) return;
   void *ptrs[] = {p->y, p->y1, p->ydot, p->ydotc, p->rprop, p->elem, p->lost, p->xm1b,
-                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld};
+                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld, p->hist};
   for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
   free(p);
 }
@@ -593,11 +599,89 @@ static double heat_transfer(int flag, double rkappa, double dp, double tgas, dou
   return 0.0 + Q_conv * Nuss;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Viscous-unsteady force: ppiclf_user_ViscousUnsteady.f (PPICLF_VU = 5      */
+/* history entries, PPICLF_USER.h:12)                                        */
+/* ------------------------------------------------------------------------ */
+#define VU_N 5
+
+/* ppiclf_user_VU_Rocflu (:19-90): the history kernel summed over entries
+ * 2..nTimeBH-1, half weight on the oldest one; entry 1 (this stage's) is
+ * not used.  (A+B)**(-2) as gfortran expands it, 1/((A+B)*(A+B)). */
+static void vu_sum(int nTimeBH, const double *timeBH, const double *mixt, const double *plag, double dt, double rmu,
+                   double rhof, double dp, double vmag, double rphip, double *fvu) {
+  const double rpi = acos(-1.0);
+  double reyL = dp * vmag * rhof / rmu; /* ppiclf_user.f:365-366 */
+  double rnu = rmu / rhof;
+  double fH = 0.75 + 0.105 * reyL;
+  double factor = 3.0 * rpi * rnu * dp * dt * (1.0 + F32(2.28) * fmin(rphip, 0.5));
+  fvu[0] = fvu[1] = fvu[2] = 0.0;
+  if (nTimeBH > 1) {
+    for (int iT = 2; iT <= nTimeBH; iT++) {
+      double time = timeBH[iT - 1];
+      double A = pow(4.0 * rpi * time * rnu / ip2(dp), 0.25);
+      double B = pow(0.5 * rpi * ip3(vmag) * ip2(time) / (0.5 * dp * rnu * ip3(fH)), 0.5);
+      double kernelVU = (iT < nTimeBH ? factor : 0.5 * factor) * (1.0 / ((A + B) * (A + B)));
+      for (int c = 0; c < 3; c++) fvu[c] = fvu[c] + kernelVU * (mixt[c * VU_N + iT - 1] - plag[c * VU_N + iT - 1]);
+    }
+  }
+}
+
+static void vu_force(const struct orc_picl *p, int i, double dt, double rmu, double rhof, double dp, double vmag,
+                     double rphip, double *fvu) {
+  const double *mixt = p->hist + 6 * VU_N * (size_t)i, *plag = mixt + 3 * VU_N;
+  vu_sum(p->nTimeBH, p->timeBH, mixt, plag, dt, rmu, rhof, dp, vmag, rphip, fvu);
+}
+
+void orc_picl_vu_force(int nTimeBH, const double *timeBH, const double *mixt, const double *plag, double dt,
+                       double rmu, double rhof, double dp, double vmag, double rphip, double *fvu) {
+  if (nTimeBH < 1 || nTimeBH > VU_N) { fvu[0] = fvu[1] = fvu[2] = 0.0; return; }
+  vu_sum(nTimeBH, timeBH, mixt, plag, dt, rmu, rhof, dp, vmag, rphip, fvu);
+}
+
+int orc_picl_vu_state(const orc_region *r, double *timeBH, double *hist) {
+  const struct orc_picl *p = r->picl;
+  if (!p) return -1;
+  if (timeBH) memcpy(timeBH, p->timeBH, sizeof p->timeBH);
+  if (hist) memcpy(hist, p->hist, sizeof(double) * 6 * VU_N * (size_t)p->n);
+  return p->nTimeBH;
+}
+
+/* ppiclf_user_UpdatePlag (:146-206): this stage's d(rho u)/dt of the gas
+ * (material derivative of rho_g u_g along the particle) and of the particle,
+ * into entry 1 */
+static void vu_update(struct orc_picl *p, int i, const double *rp, const double *y, const double *yd, double vx,
+                      double vy, double vz, double rhof, double rphif) {
+  double *mixt = p->hist + 6 * VU_N * (size_t)i, *plag = mixt + 3 * VU_N;
+  double SDrho = rp[R_JRHSR] + y[JVX] * rp[R_JPGCX] + y[JVY] * rp[R_JPGCY] + y[JVZ] * rp[R_JPGCZ];
+  SDrho = SDrho / rphif;
+  double vgradrho = vx * rp[R_JRHOGX] + vy * rp[R_JRHOGY] + vz * rp[R_JRHOGZ];
+  const double u[3] = {rp[R_JUX], rp[R_JUY], rp[R_JUZ]};
+  const double sdr[3] = {rp[R_JSDRX], rp[R_JSDRY], rp[R_JSDRZ]};
+  for (int c = 0; c < 3; c++) {
+    mixt[c * VU_N] = u[c] * (SDrho + vgradrho) + rhof * sdr[c];
+    plag[c * VU_N] = y[JVX + c] * SDrho + rhof * yd[JVX + c];
+  }
+}
+
+/* ppiclf_user_ShiftUnsteadyData (:93-143), after the third stage's SetYdot
+ * (ppiclf_user.f:1030-1036): entries move one older, the count grows to
+ * VU_N, and each entry's age is its younger neighbour's plus dt */
+static void vu_shift(struct orc_picl *p, double dt) {
+  for (int i = 0; i < p->n; i++) {
+    double *h = p->hist + 6 * VU_N * (size_t)i;
+    for (int c = 0; c < 6; c++)
+      for (int k = VU_N - 1; k >= 1; k--) h[c * VU_N + k] = h[c * VU_N + k - 1];
+  }
+  if (p->nTimeBH < VU_N) p->nTimeBH = p->nTimeBH + 1;
+  for (int iT = p->nTimeBH; iT >= 2; iT--) p->timeBH[iT - 1] = p->timeBH[iT - 2] + dt;
+}
+
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
- * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124)
- * and pressure-gradient forces and heat transfer, Euler flow (no viscous
- * stress term), no collisions, lift, torque or burning. */
-static void set_ydot(struct orc_picl *p, int i) {
+ * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124),
+ * pressure-gradient and viscous-unsteady forces and heat transfer, Euler flow
+ * (no viscous stress term), no collisions, lift, torque or burning. */
+static void set_ydot(struct orc_picl *p, int i, double dt) {
   const orc_picl_params *P = &p->pp;
   double *y = p->y + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i, *yc = p->ydotc + LRS * (size_t)i;
   const double *rp = p->rprop + LRP * (size_t)i;
@@ -627,7 +711,8 @@ static void set_ydot(struct orc_picl *p, int i) {
   rphif = 1.0 - rphip;
   double famx = 0.0, famy = 0.0, famz = 0.0, rmass_add = 0.0;
   double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
-  const double fcx = 0.0, fcy = 0.0, fcz = 0.0, fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
+  const double fcx = 0.0, fcy = 0.0, fcz = 0.0;
+  double fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
   const double liftx = 0.0, lifty = 0.0, liftz = 0.0;
   double beta; /* ppiclf_user.f:473-485 */
   switch (P->qsFlag) {
@@ -670,6 +755,13 @@ static void set_ydot(struct orc_picl *p, int i) {
     fdpdy = fdpdy + 0.0;
     fdpdz = fdpdz + 0.0;
   }
+  if (P->viscousUnsteady == 1) { /* Step 7, ppiclf_user.f:605-607 */
+    double fvu[3];
+    vu_force(p, i, dt, rmu, rhof, dp, vmag, rphip, fvu);
+    fvux = fvu[0];
+    fvuy = fvu[1];
+    fvuz = fvu[2];
+  }
   double rmass_therm = rmass * rcp_part;
   double qq = 0.0;
   if (P->heatTransfer >= 1) {
@@ -690,6 +782,7 @@ static void set_ydot(struct orc_picl *p, int i) {
   yd[9] = tauz / rmass_omega;
   yd[10] = 0.0;
   yd[11] = 0.0;
+  if (P->viscousUnsteady >= 1) vu_update(p, i, rp, y, yd, vx, vy, vz, rhof, rphif); /* :669-672 */
   if (P->feedbackFlag == 1) {
     const double tauxh = 0.0, tauyh = 0.0, tauzh = 0.0;
     yc[JVX] = rp[R_JSPL] * (yd[JVX] * rmass - fcx);
@@ -726,7 +819,8 @@ static void integrate_particle(struct orc_picl *p, int istage, double dt) {
                     {dt * 5.0 / 12.0, 1.0, dt * 3.0 / 4.0}};
   init_solve(p, 1);
   for (int i = 0; i < p->n; i++)
-    if (!p->lost[i]) set_ydot(p, i);
+    if (!p->lost[i]) set_ydot(p, i, dt);
+  if (p->pp.viscousUnsteady == 1 && istage == 3) vu_shift(p, dt);
   if (istage == 1) memset(p->y1, 0, sizeof(double) * LRS * (size_t)p->n);
   const double *k = c[istage - 1];
   for (int i = 0; i < p->n; i++) {
@@ -826,7 +920,8 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   for (size_t q = 0; q < (size_t)8 * r->nCells; q++)
     if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
-      pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4)
+      pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4 || pp->viscousUnsteady < 0 ||
+      pp->viscousUnsteady > 1)
     return -1;
   const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};
   double vlo[3] = {1e300, 1e300, 1e300}, vhi[3] = {-1e300, -1e300, -1e300};
@@ -870,6 +965,8 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   p->rprop = calloc(LRP * n, sizeof(double));
   p->elem = malloc(sizeof(int) * n);
   p->lost = calloc(n, 1);
+  p->hist = calloc(6 * VU_N * n, sizeof(double)); /* ppiclf_solve.f:432-434 */
+  p->nTimeBH = 1;                                  /* PICL_TEMP_InitSolver.F90:290 */
   for (int i = 0; i < nPart; i++) {
     double *rp = p->rprop + LRP * (size_t)i;
     rp[R_JRHOP] = pconst[4 * (size_t)i];

@@ -39,6 +39,7 @@ typedef struct orc_picl_params {
   int heatTransfer;  /* 0: off; Nusselt law 1 Stokes, 2 Ranz-Marshall, 3 Gunn, 4 Fox */
   int periodicX, periodicY, periodicZ; /* PERIODICX/Y/Z: particles wrap across the mesh's
                                           vertex bounds on that axis */
+  int viscousUnsteady; /* 0: off; 1: viscous-unsteady force with its history kernel */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index
@@ -69,6 +70,14 @@ void orc_picl_get_nodes(const orc_region *r, double *proNode);
  * for one particle, from SetYdot's capped inputs. */
 double orc_picl_qs_beta(int qsFlag, double rmachp, double rphip, double rep, double rmu, double dp,
                         double rhof, double vmag);
+/* The viscous-unsteady force of one particle from its history: entries
+ * 1..nTimeBH of the ages timeBH and of d(rho u)/dt of the gas (mixt) and the
+ * particle (plag), 5 per component ([c * 5 + k]); SetYdot's capped inputs. */
+void orc_picl_vu_force(int nTimeBH, const double *timeBH, const double *mixt, const double *plag, double dt,
+                       double rmu, double rhof, double dp, double vmag, double rphip, double *fvu);
+/* Viscous-unsteady bookkeeping: returns nTimeBH, fills the 5 ages and the
+ * particles' histories (30 per particle: gas then particle, [c * 5 + k]). */
+int orc_picl_vu_state(const orc_region *r, double *timeBH, double *hist);
 /* Element containing each point (-1: none), by the oracle's location rule. */
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
 /* Elements LocalInterp sums for point xp, nearest first; returns how many. */

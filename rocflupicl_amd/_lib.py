@@ -57,7 +57,8 @@ class PiclParams(C.Structure):
                 ("rcpPart", C.c_double), ("qsFlag", C.c_int), ("amFlag", C.c_int), ("pgFlag", C.c_int),
                 ("feedbackFlag", C.c_int), ("interpMode", C.c_int), ("projMode", C.c_int),
                 ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
-                ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int)]
+                ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
+                ("viscousUnsteady", C.c_int)]
 
 
 _lib = None

@@ -36,15 +36,21 @@ struct PiclPhys {
   int heatTransfer;
   int per[3];                 // linear periodicity per axis
   double perLo[3], perHi[3];  // ppiclf_xdrange: the mesh's vertex bounds
+  int vu;                     // viscous-unsteady force
 };
 
 // Rows of PBuf::st
 // particle state rows: x, y, z, u, v, w (and their y1, ydot), the feedback
-// map, then the temperature's y, y1 and ydot, touched only with heat transfer
-enum { ROW_X = 0, ROW_Y1 = 6, ROW_YD = 12, ROW_YCP = 18, ROW_T = 22, ROW_TY1 = 23, ROW_TYD = 24, NROW = 25 };
+// map, then the temperature's y, y1 and ydot, touched only with heat
+// transfer.  With the viscous-unsteady force VU_N entries of d(rho u)/dt of
+// the gas (ROW_VU + c * VU_N + k) and of the particle (ROW_VU + 3 * VU_N +
+// c * VU_N + k), k = 0 the newest, follow (ppiclF's rprop3).
+#define VU_N 5
+enum { ROW_X = 0, ROW_Y1 = 6, ROW_YD = 12, ROW_YCP = 18, ROW_T = 22, ROW_TY1 = 23, ROW_TYD = 24, NROW = 25,
+       ROW_VU = 25, NROW_VU = ROW_VU + 6 * VU_N };
 
 struct PBuf {
-  double *st = nullptr;  // [NROW][nP]
+  double *st = nullptr;  // [nRow][nP]
   double *pc = nullptr;  // [4][nP] rho_p, d_p, V_p, SPL
   int *pid = nullptr;    // original (ppiclF) index
   int *elem = nullptr;   // element (device cell) of the current stage, -1 lost
@@ -53,6 +59,9 @@ struct PBuf {
 struct PiclDev {
   int nP = 0, nC = 0;
   int ht = 0;  // heat transfer on: the temperature rows move with the particles
+  int vu = 0;  // viscous-unsteady force on: the history rows move with the particles
+  int nRow = NROW;
+  double *vuT = nullptr;  // [VU_N] ppiclf_timeBH, then ppiclf_nTimeBH
   PBuf buf[2];
   int *par = nullptr;     // buf[par[0]] is current
   double *ycN = nullptr;  // [4][nP] ydotc of this stage, by current slot
@@ -564,6 +573,44 @@ __device__ double heat_transfer(int flag, double rkappa, double dp, double tgas,
   return 0.0 + Q_conv * Nuss;
 }
 
+// ppiclf_user_VU_Rocflu (ppiclf_user_ViscousUnsteady.f:19-90): the history
+// kernel (Mei and Adrian's, with ppiclF's volume-fraction factor) summed over
+// entries 2..nTimeBH-1, half weight on the oldest; entry 1 (this stage's)
+// is not used.  (A+B)**(-2) as gfortran expands it, 1/((A+B)*(A+B)).
+__device__ void vu_force(const double *st, size_t nP, int t, const double *vuT, double dt, double rmu, double rhof,
+                         double dp, double vmag, double rphip, double fvu[3]) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  const int nT = (int)vuT[VU_N];
+  if (nT <= 1) return;
+  double reyL = dp * vmag * rhof / rmu;  // ppiclf_user.f:365-366
+  double rnu = rmu / rhof;
+  double fH = 0.75 + 0.105 * reyL;
+  double factor = 3.0 * rpi * rnu * dp * dt * (1.0 + F32(2.28) * fmin(rphip, 0.5));
+  double fH3 = (fH * fH) * fH, v3 = (vmag * vmag) * vmag;
+  for (int iT = 2; iT <= nT; iT++) {
+    double time = vuT[iT - 1];
+    double A = pow(4.0 * rpi * time * rnu / (dp * dp), 0.25);
+    double B = pow(0.5 * rpi * v3 * (time * time) / (0.5 * dp * rnu * fH3), 0.5);
+    double kernelVU = (iT < nT ? factor : 0.5 * factor) * (1.0 / ((A + B) * (A + B)));
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      fvu[c] = fvu[c] + kernelVU * (st[(ROW_VU + c * VU_N + iT - 1) * nP + t] -
+                                    st[(ROW_VU + 3 * VU_N + c * VU_N + iT - 1) * nP + t]);
+  }
+}
+
+// ppiclf_user_ShiftUnsteadyData's global part (:133-140): the entry count
+// grows to VU_N and each entry's age is its younger neighbour's plus dt
+__global__ void k_vu_time(Dev d, PiclDev P) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double *T = P.vuT;
+  int n = (int)T[VU_N];
+  if (n < VU_N) n = n + 1;
+  double dt = d.scal[S_DTMIN];
+  for (int iT = n; iT >= 2; iT--) T[iT - 1] = T[iT - 2] + dt;
+  T[VU_N] = (double)n;
+}
+
 template <int TRI, int QS>
 __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
   int t = blockIdx.x * BLK + threadIdx.x;
@@ -757,12 +804,24 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       fdpdy = fdpdy + 0.0;
       fdpdz = fdpdz + 0.0;
     }
+    double fvu[3] = {0.0, 0.0, 0.0};
+    if (Ph.vu) vu_force(st, nP, t, P.vuT, d.scal[S_DTMIN], rmu, rhof, dpp, vmag, rphip, fvu);
     ydn[0] = yv[0];
     ydn[1] = yv[1];
     ydn[2] = yv[2];
-    ydn[3] = (fqsx + famx + fdpdx + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
-    ydn[4] = (fqsy + famy + fdpdy + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
-    ydn[5] = (fqsz + famz + fdpdz + 0.0 + 0.0 + 0.0) / (rmass + rmass_add);
+    ydn[3] = (fqsx + famx + fdpdx + fvu[0] + 0.0 + 0.0) / (rmass + rmass_add);
+    ydn[4] = (fqsy + famy + fdpdy + fvu[1] + 0.0 + 0.0) / (rmass + rmass_add);
+    ydn[5] = (fqsz + famz + fdpdz + fvu[2] + 0.0 + 0.0) / (rmass + rmass_add);
+    if (Ph.vu) {  // ppiclf_user_UpdatePlag (ppiclf_user_ViscousUnsteady.f:146-206)
+      double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
+      SDrho = SDrho / rphif;
+      double vgradrho = vxr * v[17] + vyr * v[18] + vzr * v[19];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        st[(ROW_VU + c * VU_N) * nP + t] = v[1 + c] * (SDrho + vgradrho) + rhof * v[10 + c];
+        st[(ROW_VU + 3 * VU_N + c * VU_N) * nP + t] = yv[c] * SDrho + rhof * ydn[3 + c];
+      }
+    }
     double qq = 0.0;
     if (Ph.heatTransfer) {
       double rkappa = 1004.64 * rmu / 0.70;  // rcp_fluid*rmu/rpr (ppiclf_user.f:310)
@@ -774,7 +833,7 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       yc[0] = spl * (ydn[3] * rmass - 0.0);
       yc[1] = spl * (ydn[4] * rmass - 0.0);
       yc[2] = spl * (ydn[5] * rmass - 0.0);
-      yc[3] = spl * ((fqsx + 0.0 + 0.0) * yv[0] + (fqsy + 0.0 + 0.0) * yv[1] + (fqsz + 0.0 + 0.0) * yv[2] +
+      yc[3] = spl * ((fqsx + fvu[0] + 0.0) * yv[0] + (fqsy + fvu[1] + 0.0) * yv[1] + (fqsz + fvu[2] + 0.0) * yv[2] +
                      famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + qq);
     }
 #pragma unroll
@@ -786,6 +845,15 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = yc[k];
+  // ppiclf_user_ShiftUnsteadyData after the third stage's SetYdot
+  // (ppiclf_user.f:1030-1036): every entry moves one older
+  if (Ph.vu && istage == 3) {
+    for (int c = 0; c < 6; c++) {
+      double *hrow = st + (size_t)(ROW_VU + c * VU_N) * nP + t;
+#pragma unroll
+      for (int k = VU_N - 1; k >= 1; k--) hrow[k * nP] = hrow[(k - 1) * nP];
+    }
+  }
 
   // ---- RK3 (Rocflu coefficients), y = -c1*y1 + y + c3*ydot, y1 = ydot
   double dt = d.scal[S_DTMIN];
@@ -1129,6 +1197,8 @@ __global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
   for (int k = 0; k < 4; k++) X.pc[k * nP + pos] = C.pc[k * nP + t];
   if (P.ht)
     for (int r = ROW_T; r < NROW; r++) X.st[r * nP + pos] = C.st[r * nP + t];
+  if (P.vu)
+    for (int r = ROW_VU; r < NROW_VU; r++) X.st[r * nP + pos] = C.st[r * nP + t];
   X.pid[pos] = C.pid[t];
   X.elem[pos] = C.elem[t];
 }
@@ -1231,7 +1301,7 @@ void picl_free(PiclState *s) {
   void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
-                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm};
+                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1299,6 +1369,7 @@ int picl_stage(rflu_handle *h, int istage) {
     hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
                        P, s->phys, istage);
   }
+  if (s->phys.vu && istage == 3) hipLaunchKernelGGL(k_vu_time, dim3(1), dim3(64), 0, h->stream, h->d, P);
   picl_sort_and_deposit(h, 1);
   ProfScope ps(h, "k_feedback");
   hipLaunchKernelGGL(k_feedback, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P, s->phys.feedbackFlag);
@@ -1358,6 +1429,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0))
     return set_err(h, RFLU_GPU_EINVAL, "the Gaussian filter needs filterWidth > 0 and 0 < filterAlpha < 1");
   if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
+  if (pp->viscousUnsteady < 0 || pp->viscousUnsteady > 1)
+    return set_err(h, RFLU_GPU_EINVAL, "viscousUnsteady must be 0 or 1");
   if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
   // PICL_TEMP_InitSolver.F90:234-262: a periodic axis spans the mesh's
   // vertex minimum to maximum (ppiclf_solve_InitPeriodicX/Y/Z need xl < xr)
@@ -1387,6 +1460,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     s->phys.perHi[l] = vhi[l];
   }
   s->P.ht = pp->heatTransfer != 0 ? 1 : 0;
+  s->phys.vu = s->P.vu = pp->viscousUnsteady;
+  s->P.nRow = s->P.vu ? NROW_VU : NROW;
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
   std::vector<double> X(24 * (size_t)N), bbox(8 * (size_t)N, 0.0), cen(4 * (size_t)N, 0.0), rexp(N);
@@ -1456,7 +1531,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   for (int i = 0; i < N; i++) idOrder[i] = i;
   build_bins(cpts, cpts, idOrder, P.cbLo, P.cbH, P.cbN, cbOff, cbMem);
   // particles, SoA rows in the given order (the first sort groups them)
-  std::vector<double> st(NROW * (size_t)nP, 0.0), pcs(4 * (size_t)nP);
+  std::vector<double> st(P.nRow * (size_t)nP, 0.0), pcs(4 * (size_t)nP);  // VU history starts at 0 (ppiclf_solve.f:432-434)
   for (int p = 0; p < nP; p++) {
     for (int k = 0; k < 6; k++) st[(ROW_X + k) * (size_t)nP + p] = y[12 * (size_t)p + k];
     st[ROW_T * (size_t)nP + p] = y[12 * (size_t)p + 6];
@@ -1467,12 +1542,17 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   int zero = 0;
   int rc = 0;
   for (int b = 0; b < 2; b++) {
-    rc |= dalloc(h, &P.buf[b].st, NROW * (size_t)nP, b == 0 ? st.data() : nullptr);
+    rc |= dalloc(h, &P.buf[b].st, P.nRow * (size_t)nP, b == 0 ? st.data() : nullptr);
     rc |= dalloc(h, &P.buf[b].pc, 4 * (size_t)nP, b == 0 ? pcs.data() : nullptr);
     rc |= dalloc(h, &P.buf[b].pid, (size_t)nP, b == 0 ? ident.data() : nullptr);
     rc |= dalloc(h, &P.buf[b].elem, (size_t)nP, minus1.data());
   }
   rc |= dalloc(h, &P.par, 1, &zero);
+  if (P.vu) {  // ppiclf_nTimeBH = 1 (PICL_TEMP_InitSolver.F90:290), ages 0
+    std::vector<double> t0(VU_N + 1, 0.0);
+    t0[VU_N] = 1.0;
+    rc |= dalloc(h, &P.vuT, t0.size(), t0.data());
+  }
   rc |= dalloc(h, &P.ycN, 4 * (size_t)nP);
   rc |= dalloc(h, &P.src, (size_t)nP);
   rc |= dalloc(h, &P.dest, (size_t)nP);
@@ -1564,7 +1644,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   rc |= dalloc(h, &h->d.rhog, (size_t)N);
   rc |= dalloc(h, &h->d.gRaw, 4 * (size_t)N);
   if (rc) { std::string msg = h->err; picl_free(s); return set_err(h, RFLU_GPU_EHIP, msg); }
-  HIPCHK(h, hipMemsetAsync(P.buf[1].st, 0, NROW * (size_t)nP * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(P.buf[1].st, 0, P.nRow * (size_t)nP * sizeof(double), h->stream));
   HIPCHK(h, hipMemsetAsync(P.buf[1].pc, 0, 4 * (size_t)nP * sizeof(double), h->stream));
   HIPCHK(h, hipMemsetAsync(P.buf[1].pid, 0, (size_t)nP * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(P.ycN, 0, 4 * (size_t)nP * sizeof(double), h->stream));

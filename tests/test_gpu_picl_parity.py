@@ -329,3 +329,32 @@ def test_periodic_particles_match_oracle(oracle_mod, proj):
         assert ok, err
     ok, err = close(g.get_state()[0], o.cv())
     assert ok, err
+
+
+@pytest.mark.parametrize("dilute", [True, False])
+def test_coupled_steps_with_viscous_unsteady_force(oracle_mod, dilute):
+    """ViscousUnsteady_flag 1: the history kernel over the last five steps'
+    d(rho u)/dt of gas and particle, kept per particle and moved with it
+    through the per-stage sort; eight steps fill the window and shift it.
+    Graph replays on the GPU."""
+    n, h = 12, 0.0005
+    kw = dict(phi=0.01, x_shock=0.9 * n * h, x_curtain=0.3 * n * h) if dilute else {}
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=n, ny=4, nz=4, npart=500, jitter=0.05, seed=4, **kw)
+    picl.update(viscousUnsteady=1)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(8, use_graph=True)
+    o.step(8)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(op["ydot"][:, 3]).max() > 0.0
+    for k in ("y", "y1", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err
