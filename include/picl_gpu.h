@@ -11,10 +11,11 @@
  *
  * Physics restated (rocpicl/, libpicl/): the quasi-steady drag laws of
  * ppiclf_user_QuasiSteady.f (Parmar, Osnes, modified Parmar, Gidaspow) plus
- * Schiller-Naumann, AM Parmar added mass and the pressure-gradient force on
- * an Euler flow, Sutherland or fixed viscosity, IDW-27 or trilinear
- * interpolation, single-element box or Gaussian-filter projection, two-way
- * feedback.  Requirements: all-hex grid, spaceOrder 2, patchOrder 1.
+ * Schiller-Naumann, AM Parmar added mass, the pressure-gradient and
+ * viscous-unsteady forces, normal collisions and heat transfer on an Euler
+ * flow, Sutherland or fixed viscosity, IDW-27 or trilinear interpolation,
+ * single-element box or Gaussian-filter projection, linear periodicity,
+ * two-way feedback.  Requirements: all-hex grid, spaceOrder 2, patchOrder 1.
  *
  * Particle arrays use ppiclF's layout: y(PPICLF_LRS=12, n) -> y[12*p + j]
  * (x, y, z, vx, vy, vz, T, ox, oy, oz, metal, oxide).  Element ids are Rocflu
@@ -73,6 +74,15 @@ typedef struct picl_params {
    * ppiclf_user_ViscousUnsteady.f, a history kernel over the last 5 steps'
    * d(rho u)/dt of gas and particle (PPICLF_VU = 5). */
   int viscousUnsteady;
+  /* COLLISIONS (collisional_flag, PICL_TEMP_InitSolver.F90:204): 0 off; 1 the
+   * normal soft-sphere force between overlapping particles
+   * (ppiclf_user_EvalNearestNeighbor.f:118-233, Cundall and Strack), with
+   * periodic images on periodic axes.  2-4 (tangential force, torques, lift)
+   * are rejected.  KSP and EREST (global%piclKsp, piclERest, :222-223):
+   * the spring stiffness is capped at rmass pi^2 / (ksp dt)^2, erest in (0, 1]
+   * sets the damping. */
+  int collisionalFlag;
+  double ksp, erest;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

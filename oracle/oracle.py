@@ -5,6 +5,7 @@ leg, as the checker.  Arrays are returned in the reference's Fortran layout
 (AoS per cell: cv[N,5], dv[N,3], grad[N,5,3], fn[F,4], fc[F,3], f2c[F,2]).
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -32,7 +33,8 @@ class OrcPiclParams(C.Structure):
                 ("feedbackFlag", C.c_int), ("interpMode", C.c_int), ("projMode", C.c_int),
                 ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
                 ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
-                ("viscousUnsteady", C.c_int)]
+                ("viscousUnsteady", C.c_int), ("collisionalFlag", C.c_int), ("ksp", C.c_double),
+                ("erest", C.c_double)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
@@ -108,6 +110,8 @@ def lib():
         L.orc_picl_vu_force.argtypes = [I, P, P, P] + [C.c_double] * 6 + [P]
         L.orc_picl_vu_state.argtypes = [P, P, P]
         L.orc_picl_vu_state.restype = I
+        D = C.c_double
+        L.orc_picl_collision_force.argtypes = [P, P, D, D, D, P, P, D, D, D, D, D, D, P]
         _LIB = L
     return _LIB
 
@@ -338,6 +342,17 @@ def vu_force(nTimeBH, timeBH, mixt, plag, dt, rmu, rhof, dp, vmag, rphip):
     out = np.zeros(3)
     lib().orc_picl_vu_force(int(nTimeBH), _p(t), _p(m), _p(q), float(dt), float(rmu), float(rhof), float(dp),
                             float(vmag), float(rphip), _p(out))
+    return out
+
+
+def collision_force(xi, vi, dpi, xj, vj, dpj, dt, ksp, erest, rhop=2500.0):
+    """Collision force of particle j on particle i (COLLISIONS 1); spheres
+    of density rhop, volume pi d^3 / 6."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (xi, vi, xj, vj)]
+    out = np.zeros(3)
+    vol = lambda d: math.pi / 6.0 * d ** 3
+    lib().orc_picl_collision_force(_p(a[0]), _p(a[1]), float(dpi), rhop, vol(dpi), _p(a[2]), _p(a[3]), float(dpj),
+                                   rhop, vol(dpj), float(dt), float(ksp), float(erest), _p(out))
     return out
 
 

@@ -677,10 +677,107 @@ static void vu_shift(struct orc_picl *p, double dt) {
   for (int iT = p->nTimeBH; iT >= 2; iT--) p->timeBH[iT - 1] = p->timeBH[iT - 2] + dt;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Collisions (COLLISIONS 1): ppiclf_user_EvalNearestNeighbor.f:118-233      */
+/* ------------------------------------------------------------------------ */
+/* The normal soft-sphere force of neighbour j on particle i when they
+ * overlap (Cundall and Strack): spring min(rmass_i pi^2 / (ksp dt)^2, Hertz
+ * stiffness of the overlap), damping from the restitution coefficient;
+ * tangential part zero (Ftmin = 0).  Added into fc as ppiclF adds it into
+ * ydotc(JVX..JVZ, i). */
+static void collision_pair(const double *yi, const double *rpi_, const double *yj, const double *rpj, double rmass,
+                           double dt, double ksp, double erest, double *fc) {
+  const double rpi = acos(-1.0), pi2 = rpi * rpi;
+  double rthresh = 0.5 * (rpi_[R_JDP] + rpj[R_JDP]);
+  double rxdiff = yj[JX] - yi[JX], rydiff = yj[JY] - yi[JY], rzdiff = yj[JZ] - yi[JZ];
+  double rdiff = ip2(rxdiff) + ip2(rydiff) + ip2(rzdiff);
+  rdiff = sqrt(rdiff);
+  const double eps = 0.0;
+  if (!(rdiff < rthresh + eps)) return;
+  double ksp1 = rmass * rpi * rpi / ip2(ksp * dt);
+  const double E1 = 1.0e9, E2 = 1.0e9, nu1 = 0.35, nu2 = 0.35;
+  double Estar = (1.0 - nu1 * nu1) / E1 + (1.0 - nu2 * nu2) / E2;
+  Estar = 1.0 / Estar;
+  double r1 = 0.5 * rpi_[R_JDP], r2 = 0.5 * rpj[R_JDP];
+  double Rstar = r1 * r2 / (r1 + r2);
+  double ksp2 = (4.0 / 3.0) * Estar * sqrt(Rstar);
+  ksp2 = ksp2 * sqrt(fabs(rdiff - rthresh));
+  double ksp_min = fmin(ksp1, ksp2);
+  double rm1 = rpi_[R_JRHOP] * rpi_[R_JVOLP], rm2 = rpj[R_JRHOP] * rpj[R_JVOLP];
+  double rmult = (rm1 * rm2) / (rm1 + rm2);
+  double le = log(erest);
+  double eta_n = -2.0 * sqrt(ksp_min) * le / sqrt(ip2(le) + pi2) * sqrt(rmult);
+  double rbot = 1.0 / rdiff;
+  double rn[3] = {rxdiff * rbot, rydiff * rbot, rzdiff * rbot};
+  double u12[3] = {yi[JVX] - yj[JVX], yi[JVY] - yj[JVY], yi[JVZ] - yj[JVZ]};
+  double rv12_mag = u12[0] * rn[0] + u12[1] * rn[1] + u12[2] * rn[2];
+  double rdelta12 = rthresh - rdiff;
+  double rksp_max = ksp_min * rdelta12;
+  double rv12_mage = rv12_mag * eta_n;
+  double rnmag = -rksp_max - rv12_mage;
+  double ut[3];
+  for (int c = 0; c < 3; c++) ut[c] = u12[c] - rv12_mag * rn[c];
+  double ut_mag = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+  ut_mag = fmax(ut_mag, 1.0e-8);
+  const double Ftmin = 0.0;
+  for (int c = 0; c < 3; c++) fc[c] = fc[c] + rnmag * rn[c] + Ftmin * (ut[c] / ut_mag);
+}
+
+void orc_picl_collision_force(const double *xi, const double *vi, double dpi, double rhopi, double volpi,
+                              const double *xj, const double *vj, double dpj, double rhopj, double volpj, double dt,
+                              double ksp, double erest, double *fc) {
+  double yi[LRS] = {0}, yj[LRS] = {0}, ri[LRP] = {0}, rj[LRP] = {0};
+  for (int c = 0; c < 3; c++) {
+    yi[JX + c] = xi[c];
+    yi[JVX + c] = vi[c];
+    yj[JX + c] = xj[c];
+    yj[JVX + c] = vj[c];
+  }
+  ri[R_JDP] = dpi;
+  ri[R_JRHOP] = rhopi;
+  ri[R_JVOLP] = volpi;
+  rj[R_JDP] = dpj;
+  rj[R_JRHOP] = rhopj;
+  rj[R_JVOLP] = volpj;
+  fc[0] = fc[1] = fc[2] = 0.0;
+  collision_pair(yi, ri, yj, rj, volpi * rhopi, dt, ksp, erest, fc);
+}
+
+/* ppiclf_solve_NearestNeighbor (ppiclf_solve.f:470-560) for the collision
+ * force: the other particles in ascending index, then the ghost particles,
+ * here the periodic images, by particle and then image. */
+static void collision_sum(const struct orc_picl *p, int i, double rmass, double dt, double *fc) {
+  const double *yi = p->y + LRS * (size_t)i, *rpi_ = p->rprop + LRP * (size_t)i;
+  const orc_picl_params *P = &p->pp;
+  fc[0] = fc[1] = fc[2] = 0.0;
+  for (int j = 0; j < p->n; j++) {
+    if (j == i || p->lost[j]) continue;
+    collision_pair(yi, rpi_, p->y + LRS * (size_t)j, p->rprop + LRP * (size_t)j, rmass, dt, P->ksp, P->erest, fc);
+  }
+  if (!(p->per[0] || p->per[1] || p->per[2])) return;
+  for (int j = 0; j < p->n; j++) {
+    if (p->lost[j]) continue;
+    for (int img = 0; img < 27; img++) {
+      if (img == 13) continue; /* the particle itself */
+      double yj[LRS];
+      memcpy(yj, p->y + LRS * (size_t)j, sizeof yj);
+      int skip = 0;
+      for (int d = 0, q = img; d < 3; d++, q /= 3) {
+        int sd = q % 3 - 1;
+        if (sd != 0 && !p->per[d]) skip = 1;
+        if (sd != 0) yj[d] = yj[d] + (double)sd * (p->perHi[d] - p->perLo[d]);
+      }
+      if (skip) continue;
+      collision_pair(yi, rpi_, yj, p->rprop + LRP * (size_t)j, rmass, dt, P->ksp, P->erest, fc);
+    }
+  }
+}
+
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
  * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124),
- * pressure-gradient and viscous-unsteady forces and heat transfer, Euler flow
- * (no viscous stress term), no collisions, lift, torque or burning. */
+ * pressure-gradient, viscous-unsteady and normal collision forces and heat
+ * transfer, Euler flow (no viscous stress term), no tangential collision
+ * force, lift, torque or burning. */
 static void set_ydot(struct orc_picl *p, int i, double dt) {
   const orc_picl_params *P = &p->pp;
   double *y = p->y + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i, *yc = p->ydotc + LRS * (size_t)i;
@@ -711,8 +808,15 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
   rphif = 1.0 - rphip;
   double famx = 0.0, famy = 0.0, famz = 0.0, rmass_add = 0.0;
   double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
-  const double fcx = 0.0, fcy = 0.0, fcz = 0.0;
+  double fcx = 0.0, fcy = 0.0, fcz = 0.0;
   double fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
+  if (P->collisionalFlag >= 1) { /* Step 1b and Step 6, ppiclf_user.f:411-470, 587-600 */
+    double fc[3];
+    collision_sum(p, i, rmass, dt, fc);
+    fcx = fc[0];
+    fcy = fc[1];
+    fcz = fc[2];
+  }
   const double liftx = 0.0, lifty = 0.0, liftz = 0.0;
   double beta; /* ppiclf_user.f:473-485 */
   switch (P->qsFlag) {
@@ -791,8 +895,11 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
     yc[JT] = rp[R_JSPL] * ((fqsx + fvux + liftx) * y[JVX] + (fqsy + fvuy + lifty) * y[JVY] +
                            (fqsz + fvuz + liftz) * y[JVZ] + famx * rp[R_JUX] + famy * rp[R_JUY] +
                            famz * rp[R_JUZ] + tauxh * y[7] + tauyh * y[8] + tauzh * y[9] + qq);
-  } else {
-    yc[JVX] = yc[JVY] = yc[JVZ] = yc[JT] = 0.0;
+  } else { /* ydotc keeps what the collisions added into it */
+    yc[JVX] = fcx;
+    yc[JVY] = fcy;
+    yc[JVZ] = fcz;
+    yc[JT] = 0.0;
   }
 }
 
@@ -921,8 +1028,9 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
     if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4 || pp->viscousUnsteady < 0 ||
-      pp->viscousUnsteady > 1)
+      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 1)
     return -1;
+  if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0)) return -1;
   const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};
   double vlo[3] = {1e300, 1e300, 1e300}, vhi[3] = {-1e300, -1e300, -1e300};
   for (int v = 0; v < r->nVert; v++)

@@ -40,6 +40,9 @@ typedef struct orc_picl_params {
   int periodicX, periodicY, periodicZ; /* PERIODICX/Y/Z: particles wrap across the mesh's
                                           vertex bounds on that axis */
   int viscousUnsteady; /* 0: off; 1: viscous-unsteady force with its history kernel */
+  int collisionalFlag; /* 0: off; 1: normal soft-sphere collision force (Cundall-Strack) */
+  double ksp;          /* KSP: collision time = ksp * dt sets the spring stiffness cap */
+  double erest;        /* ERESTITUTION: restitution coefficient of the damping */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index
@@ -78,6 +81,11 @@ void orc_picl_vu_force(int nTimeBH, const double *timeBH, const double *mixt, co
 /* Viscous-unsteady bookkeeping: returns nTimeBH, fills the 5 ages and the
  * particles' histories (30 per particle: gas then particle, [c * 5 + k]). */
 int orc_picl_vu_state(const orc_region *r, double *timeBH, double *hist);
+/* The collision force of particle j on particle i (COLLISIONS 1) for the
+ * given positions, velocities, diameters, densities and volumes. */
+void orc_picl_collision_force(const double *xi, const double *vi, double dpi, double rhopi, double volpi,
+                              const double *xj, const double *vj, double dpj, double rhopj, double volpj, double dt,
+                              double ksp, double erest, double *fc);
 /* Element containing each point (-1: none), by the oracle's location rule. */
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
 /* Elements LocalInterp sums for point xp, nearest first; returns how many. */

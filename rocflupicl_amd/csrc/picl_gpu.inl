@@ -37,6 +37,8 @@ struct PiclPhys {
   int per[3];                 // linear periodicity per axis
   double perLo[3], perHi[3];  // ppiclf_xdrange: the mesh's vertex bounds
   int vu;                     // viscous-unsteady force
+  int col;                    // normal collision force
+  double ksp, erest;          // collision time / dt, restitution coefficient
 };
 
 // Rows of PBuf::st
@@ -97,6 +99,13 @@ struct PiclDev {
   // axis (0 elsewhere); cbTop = the largest centroid bin coordinate
   double gShift[3] = {0, 0, 0}, cbTop[3] = {0, 0, 0};
   int gImg = 0;  // any periodic axis
+  // collisions: a hash of cubes of side colH >= the largest diameter (one
+  // stage's located particles), bucket lists by colOff / colMem
+  int col = 0, colNb = 0;
+  double colLo[3] = {0, 0, 0}, colHi[3] = {0, 0, 0}, colH = 1, colR = 0;
+  int *colCnt = nullptr, *colOff = nullptr, *colFill = nullptr, *colMem = nullptr, *colB = nullptr,
+      *colBsum = nullptr;
+  double *fc = nullptr;  // [3][nP] collision force by current slot
 };
 
 // ---------------------------------------------------------------------------
@@ -573,6 +582,179 @@ __device__ double heat_transfer(int flag, double rkappa, double dp, double tgas,
   return 0.0 + Q_conv * Nuss;
 }
 
+// ---------------------------------------------------------------------------
+// Collisions (COLLISIONS 1).  The search: every located particle goes into
+// the hash bucket of its cube (side colH >= the largest diameter); a particle
+// scans the buckets of the cubes within colR of itself, and of its images
+// one period away on periodic axes, and keeps the neighbours that overlap it.
+// Their forces are summed in ppiclf_solve_NearestNeighbor's order on one
+// rank (ppiclf_solve.f:470-560): the other particles by ascending index,
+// then the ghost (periodic image) particles by index and image.
+// ---------------------------------------------------------------------------
+#define COL_CAP 32
+__device__ __forceinline__ unsigned col_bucket(const PiclDev &P, int ix, int iy, int iz) {
+  unsigned hsh = (unsigned)ix * 73856093u ^ (unsigned)iy * 19349663u ^ (unsigned)iz * 83492791u;
+  return hsh & (unsigned)(P.colNb - 1);
+}
+
+// The stage's location (k_particles repeats it and gets the same element)
+// and the bucket counts
+__global__ void __launch_bounds__(BLK) k_col_count(PiclDev P) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  double xp[3] = {C.st[t], C.st[nP + t], C.st[2 * nP + t]};
+  int e = picl_locate(P, xp, C.elem[t]);
+  C.elem[t] = e;
+  if (e < 0) { P.colB[t] = -1; return; }
+  int c[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) c[l] = (int)floor((xp[l] - P.colLo[l]) / P.colH);
+  unsigned b = col_bucket(P, c[0], c[1], c[2]);
+  P.colB[t] = (int)b;
+  atomicAdd(&P.colCnt[b], 1);
+}
+
+__global__ void __launch_bounds__(BLK) k_col_fill(PiclDev P) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  int b = P.colB[t];
+  if (b < 0) return;
+  P.colMem[P.colOff[b] + atomicAdd(&P.colFill[b], 1)] = t;
+}
+
+// ppiclf_user_EvalNearestNeighbor, collisional part (:118-233), for
+// collisional_flag 1: the normal soft-sphere force of j on i (Cundall and
+// Strack) with spring min(rmass_i pi^2 / (ksp dt)^2, the overlap's Hertz
+// stiffness) and damping from the restitution coefficient; Ftmin = 0.
+// Same operations as the oracle.
+__device__ void col_pair(const double xi[3], const double vi[3], double dpi, double rmi, const double xj[3],
+                         const double vj[3], double dpj, double rmj, double rmass, double dt, double ksp,
+                         double erest, double fc[3]) {
+  const double rpi = 3.141592653589793116, pi2 = rpi * rpi;
+  double rthresh = 0.5 * (dpi + dpj);
+  double rxdiff = xj[0] - xi[0], rydiff = xj[1] - xi[1], rzdiff = xj[2] - xi[2];
+  double rdiff = rxdiff * rxdiff + rydiff * rydiff + rzdiff * rzdiff;
+  rdiff = sqrt(rdiff);
+  double ksp1 = rmass * rpi * rpi / ((ksp * dt) * (ksp * dt));
+  const double E1 = 1.0e9, E2 = 1.0e9, nu1 = 0.35, nu2 = 0.35;
+  double Estar = (1.0 - nu1 * nu1) / E1 + (1.0 - nu2 * nu2) / E2;
+  Estar = 1.0 / Estar;
+  double r1 = 0.5 * dpi, r2 = 0.5 * dpj;
+  double Rstar = r1 * r2 / (r1 + r2);
+  double ksp2 = (4.0 / 3.0) * Estar * sqrt(Rstar);
+  ksp2 = ksp2 * sqrt(fabs(rdiff - rthresh));
+  double ksp_min = fmin(ksp1, ksp2);
+  double rmult = (rmi * rmj) / (rmi + rmj);
+  double le = log(erest);
+  double eta_n = -2.0 * sqrt(ksp_min) * le / sqrt(le * le + pi2) * sqrt(rmult);
+  double rbot = 1.0 / rdiff;
+  double rn[3] = {rxdiff * rbot, rydiff * rbot, rzdiff * rbot};
+  double u12[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+  double rv12_mag = u12[0] * rn[0] + u12[1] * rn[1] + u12[2] * rn[2];
+  double rdelta12 = rthresh - rdiff;
+  double rksp_max = ksp_min * rdelta12;
+  double rv12_mage = rv12_mag * eta_n;
+  double rnmag = -rksp_max - rv12_mage;
+  double ut[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) ut[c] = u12[c] - rv12_mag * rn[c];
+  double ut_mag = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+  ut_mag = fmax(ut_mag, 1.0e-8);
+  const double Ftmin = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) fc[c] = fc[c] + rnmag * rn[c] + Ftmin * (ut[c] / ut_mag);
+}
+
+__global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  const double *st = C.st;
+  double fc[3] = {0.0, 0.0, 0.0};
+  if (C.elem[t] >= 0) {
+    double xi[3] = {st[t], st[nP + t], st[2 * nP + t]};
+    double dpi = C.pc[nP + t];
+    // overlapping neighbours: key = (ghost << 62) | (index << 5) | image
+    long long key[COL_CAP];
+    int slot[COL_CAP];
+    int nk = 0;
+    bool over = false;
+    const bool per = Ph.per[0] || Ph.per[1] || Ph.per[2];
+    for (int im = 0; im < 27; im++) {
+      if (im != 13 && !per) continue;
+      double sh[3];
+      bool skip = false;
+      int lo[3], hi[3];
+#pragma unroll
+      for (int l = 0, r = im; l < 3; l++, r /= 3) {
+        int sd = r % 3 - 1;
+        if (sd != 0 && !Ph.per[l]) skip = true;
+        sh[l] = sd == 0 ? 0.0 : (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
+        double q = xi[l] - sh[l];
+        if (q + P.colR < P.colLo[l] - P.colR || q - P.colR > P.colHi[l] + P.colR) skip = true;
+        lo[l] = (int)floor((q - P.colR - P.colLo[l]) / P.colH);
+        hi[l] = (int)floor((q + P.colR - P.colLo[l]) / P.colH);
+        if (hi[l] - lo[l] > 2) { skip = true; over = true; }  // cannot happen: 2 colR < 2 colH
+      }
+      if (skip) continue;
+      unsigned seen[27];
+      int ns = 0;
+      for (int cz = lo[2]; cz <= hi[2]; cz++)
+        for (int cy = lo[1]; cy <= hi[1]; cy++)
+          for (int cx = lo[0]; cx <= hi[0]; cx++) {
+            unsigned b = col_bucket(P, cx, cy, cz);
+            bool dup = false;
+            for (int q = 0; q < ns; q++) dup = dup || seen[q] == b;
+            if (dup) continue;
+            seen[ns++] = b;
+            for (int q = P.colOff[b]; q < P.colOff[b + 1]; q++) {
+              int sj = P.colMem[q];
+              if (im == 13 && sj == t) continue;
+              double xj[3];
+#pragma unroll
+              for (int l = 0; l < 3; l++) xj[l] = sh[l] == 0.0 ? st[l * nP + sj] : st[l * nP + sj] + sh[l];
+              double rx = xj[0] - xi[0], ry = xj[1] - xi[1], rz = xj[2] - xi[2];
+              double rdiff = sqrt(rx * rx + ry * ry + rz * rz);
+              if (!(rdiff < 0.5 * (dpi + C.pc[nP + sj]) + 0.0)) continue;
+              if (nk == COL_CAP) { over = true; continue; }
+              key[nk] = ((long long)(im != 13) << 62) | ((long long)C.pid[sj] << 5) | im;
+              slot[nk] = sj;
+              nk++;
+            }
+          }
+    }
+    if (over) atomicOr(d.err, E_CONTACT);
+    for (int a = 1; a < nk; a++) {  // ascending key
+      long long x = key[a];
+      int y = slot[a], j = a - 1;
+      while (j >= 0 && key[j] > x) { key[j + 1] = key[j]; slot[j + 1] = slot[j]; j--; }
+      key[j + 1] = x;
+      slot[j + 1] = y;
+    }
+    double vi[3] = {st[3 * nP + t], st[4 * nP + t], st[5 * nP + t]};
+    double rmi = C.pc[t] * C.pc[2 * nP + t];
+    double rmass = C.pc[2 * nP + t] * C.pc[t];  // volp * rhop (ppiclf_user.f:313)
+    double dt = d.scal[S_DTMIN];
+    for (int a = 0; a < nk; a++) {
+      int sj = slot[a], im = (int)(key[a] & 31);
+      double xj[3], vj[3];
+#pragma unroll
+      for (int l = 0, r = im; l < 3; l++, r /= 3) {
+        int sd = r % 3 - 1;
+        double x = st[l * nP + sj];
+        xj[l] = sd == 0 ? x : x + (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
+        vj[l] = st[(3 + l) * nP + sj];
+      }
+      col_pair(xi, vi, dpi, rmi, xj, vj, C.pc[nP + sj], C.pc[sj] * C.pc[2 * nP + sj], rmass, dt, Ph.ksp, Ph.erest, fc);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) P.fc[c * nP + t] = fc[c];
+}
+
 // ppiclf_user_VU_Rocflu (ppiclf_user_ViscousUnsteady.f:19-90): the history
 // kernel (Mei and Adrian's, with ppiclF's volume-fraction factor) summed over
 // entries 2..nTimeBH-1, half weight on the oldest; entry 1 (this stage's)
@@ -804,14 +986,19 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       fdpdy = fdpdy + 0.0;
       fdpdz = fdpdz + 0.0;
     }
+    double fcv[3] = {0.0, 0.0, 0.0};  // Step 6: the collision force (ppiclf_user.f:587-600)
+    if (Ph.col) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) fcv[c] = P.fc[c * nP + t];
+    }
     double fvu[3] = {0.0, 0.0, 0.0};
     if (Ph.vu) vu_force(st, nP, t, P.vuT, d.scal[S_DTMIN], rmu, rhof, dpp, vmag, rphip, fvu);
     ydn[0] = yv[0];
     ydn[1] = yv[1];
     ydn[2] = yv[2];
-    ydn[3] = (fqsx + famx + fdpdx + fvu[0] + 0.0 + 0.0) / (rmass + rmass_add);
-    ydn[4] = (fqsy + famy + fdpdy + fvu[1] + 0.0 + 0.0) / (rmass + rmass_add);
-    ydn[5] = (fqsz + famz + fdpdz + fvu[2] + 0.0 + 0.0) / (rmass + rmass_add);
+    ydn[3] = (fqsx + famx + fdpdx + fvu[0] + 0.0 + fcv[0]) / (rmass + rmass_add);
+    ydn[4] = (fqsy + famy + fdpdy + fvu[1] + 0.0 + fcv[1]) / (rmass + rmass_add);
+    ydn[5] = (fqsz + famz + fdpdz + fvu[2] + 0.0 + fcv[2]) / (rmass + rmass_add);
     if (Ph.vu) {  // ppiclf_user_UpdatePlag (ppiclf_user_ViscousUnsteady.f:146-206)
       double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
       SDrho = SDrho / rphif;
@@ -830,11 +1017,14 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       st[ROW_TYD * nP + t] = ydT;
     }
     if (Ph.feedbackFlag == 1) {
-      yc[0] = spl * (ydn[3] * rmass - 0.0);
-      yc[1] = spl * (ydn[4] * rmass - 0.0);
-      yc[2] = spl * (ydn[5] * rmass - 0.0);
+      yc[0] = spl * (ydn[3] * rmass - fcv[0]);
+      yc[1] = spl * (ydn[4] * rmass - fcv[1]);
+      yc[2] = spl * (ydn[5] * rmass - fcv[2]);
       yc[3] = spl * ((fqsx + fvu[0] + 0.0) * yv[0] + (fqsy + fvu[1] + 0.0) * yv[1] + (fqsz + fvu[2] + 0.0) * yv[2] +
                      famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + qq);
+    } else {  // ydotc keeps what the collisions added into it
+#pragma unroll
+      for (int c = 0; c < 3; c++) yc[c] = fcv[c];
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) st[(ROW_YD + k) * nP + t] = ydn[k];
@@ -1301,7 +1491,8 @@ void picl_free(PiclState *s) {
   void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
-                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT};
+                  P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
+                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1360,6 +1551,18 @@ int picl_stage(rflu_handle *h, int istage) {
   }
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nC + 1)), dim3(BLK), 0, h->stream, P.cnt, P.nC + 1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lostCur, 1);
+  if (P.col) {  // the stage's neighbour search for the collision force
+    ProfScope ps(h, "k_contacts");
+    int n = P.colNb + 1, nb = scan_blocks(n);
+    hipLaunchKernelGGL(k_zero_int, dim3(nblk(n)), dim3(BLK), 0, h->stream, P.colCnt, n);
+    hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.colNb)), dim3(BLK), 0, h->stream, P.colFill, P.colNb);
+    hipLaunchKernelGGL(k_col_count, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
+    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(SCAN_T), 0, h->stream, P.colCnt, n, P.colBsum);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(SCAN_T), 0, h->stream, P.colBsum, nb);
+    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(SCAN_T), 0, h->stream, P.colCnt, n, P.colBsum, P.colOff);
+    hipLaunchKernelGGL(k_col_fill, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
+    hipLaunchKernelGGL(k_contacts, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
+  }
   {
     ProfScope ps(h, "k_particles");
     using KP = void (*)(Dev, PiclDev, PiclPhys, int);
@@ -1431,6 +1634,12 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
   if (pp->viscousUnsteady < 0 || pp->viscousUnsteady > 1)
     return set_err(h, RFLU_GPU_EINVAL, "viscousUnsteady must be 0 or 1");
+  if (pp->collisionalFlag < 0 || pp->collisionalFlag > 1)
+    return set_err(h, RFLU_GPU_EINVAL,
+                   "collisionalFlag must be 0 (off) or 1 (normal force); the tangential force, torques and lift "
+                   "(2-4) are not built");
+  if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0))
+    return set_err(h, RFLU_GPU_EINVAL, "collisions need ksp > 0 and 0 < erest <= 1");
   if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
   // PICL_TEMP_InitSolver.F90:234-262: a periodic axis spans the mesh's
   // vertex minimum to maximum (ppiclf_solve_InitPeriodicX/Y/Z need xl < xr)
@@ -1461,6 +1670,9 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   }
   s->P.ht = pp->heatTransfer != 0 ? 1 : 0;
   s->phys.vu = s->P.vu = pp->viscousUnsteady;
+  s->phys.col = s->P.col = pp->collisionalFlag;
+  s->phys.ksp = pp->ksp;
+  s->phys.erest = pp->erest;
   s->P.nRow = s->P.vu ? NROW_VU : NROW;
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
@@ -1548,6 +1760,23 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.buf[b].elem, (size_t)nP, minus1.data());
   }
   rc |= dalloc(h, &P.par, 1, &zero);
+  if (P.col) {  // hash of cubes of side just above the largest diameter, >= 2 buckets per particle
+    double dmax = 0.0;
+    for (int p = 0; p < nP; p++) dmax = std::max(dmax, pconst[4 * (size_t)p + 1]);
+    if (!(dmax > 0.0)) { picl_free(s); return set_err(h, RFLU_GPU_EINVAL, "collisions need positive diameters"); }
+    P.colR = dmax * (1.0 + 1.0e-9);
+    P.colH = dmax * (1.0 + 1.0e-8);
+    for (int l = 0; l < 3; l++) { P.colLo[l] = vlo[l]; P.colHi[l] = vhi[l]; }
+    P.colNb = 1024;
+    while (P.colNb < 2 * nP && P.colNb < (1 << 28)) P.colNb <<= 1;
+    rc |= dalloc(h, &P.colCnt, (size_t)P.colNb + 1);
+    rc |= dalloc(h, &P.colOff, (size_t)P.colNb + 1);
+    rc |= dalloc(h, &P.colFill, (size_t)P.colNb);
+    rc |= dalloc(h, &P.colMem, (size_t)nP);
+    rc |= dalloc(h, &P.colB, (size_t)nP);
+    rc |= dalloc(h, &P.colBsum, (size_t)scan_blocks(P.colNb + 1));
+    rc |= dalloc(h, &P.fc, 3 * (size_t)nP);
+  }
   if (P.vu) {  // ppiclf_nTimeBH = 1 (PICL_TEMP_InitSolver.F90:290), ages 0
     std::vector<double> t0(VU_N + 1, 0.0);
     t0[VU_N] = 1.0;

@@ -358,3 +358,32 @@ def test_coupled_steps_with_viscous_unsteady_force(oracle_mod, dilute):
         assert ok, (k, err)
     ok, err = close(g.get_state()[0], o.cv())
     assert ok, err
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_coupled_steps_with_collisions(oracle_mod, periodic):
+    """COLLISIONS 1: a curtain of randomly placed particles at phi 0.3, so
+    that many overlap (about 2 contacts each) and feel the soft-sphere
+    force; with periodic y and z also across those faces.  The GPU's hash
+    search must find exactly the oracle's contacts and sum them in the same
+    order."""
+    n = 12
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=n, ny=4, nz=4, npart=2000, jitter=0.05, seed=4, phi=0.3)
+    picl.update(collisionalFlag=1, ksp=10.0, erest=0.7, periodicY=periodic, periodicZ=periodic)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(4, use_graph=True)
+    o.step(4)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(op["ydot"][:, 3]).max() > 1e6  # collision-sized accelerations
+    for k in ("y", "y1", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err

@@ -1,0 +1,139 @@
+"""Pins of the oracle's particle collisions (COLLISIONS 1,
+ppiclf_user_EvalNearestNeighbor.f:118-233).  CPU only.
+
+What contact mechanics fixes, not a retyped formula:
+* at small overlap the spring is Hertz's: two spheres pressed together by an
+  overlap delta repel with (4/3) E* sqrt(R*) delta^(3/2), E* the contact
+  modulus of two like spheres (E / (2 (1 - nu^2)), E = 1e9, nu = 0.35),
+  R* = r1 r2 / (r1 + r2);
+* the stiffness cap is the linear spring whose half oscillation
+  pi sqrt(m / k) lasts ksp time steps;
+* the dashpot makes a linear-spring contact rebound with the restitution
+  coefficient erest (integrating m x'' = -k x - eta x' through the contact);
+* equal spheres feel opposite forces; the force lies along the line of
+  centres; none without overlap;
+* in a stage the collisions change the particles' accelerations but not the
+  force fed back to the gas (they are internal to the particle phase), and
+  two particles overlapping across a periodic face collide only when the
+  axis is periodic.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from rocflupicl_amd import configs
+
+E, NU, RHOP = 1.0e9, 0.35, 2500.0
+D = 1.0e-4
+DT = 1.0e-8
+
+
+def mass(d):
+    return RHOP * math.pi / 6.0 * d ** 3
+
+
+def test_hertz_regime(oracle_mod):
+    for d1, d2, delta in ((D, D, 1e-7), (D, 0.6 * D, 3e-7), (1.4 * D, D, 2e-8)):
+        r = 0.5 * (d1 + d2) - delta
+        f = oracle_mod.collision_force([0, 0, 0], [0, 0, 0], d1, [r, 0, 0], [0, 0, 0], d2, DT, 10.0, 0.7)
+        estar = E / (2.0 * (1.0 - NU * NU))
+        rstar = (0.5 * d1) * (0.5 * d2) / (0.5 * d1 + 0.5 * d2)
+        hertz = 4.0 / 3.0 * estar * math.sqrt(rstar) * delta ** 1.5
+        assert f[0] == pytest.approx(-hertz, rel=1e-9)  # pushes i away from j (j lies at +x)
+        assert f[1] == 0.0 and f[2] == 0.0
+
+
+def linear_spring(oracle_mod, ksp, erest, delta, vn):
+    """Force at overlap delta and normal approach speed vn (i moving towards j)."""
+    r = D - delta
+    return oracle_mod.collision_force([0, 0, 0], [vn, 0, 0], D, [r, 0, 0], [0, 0, 0], D, DT, ksp, erest)[0]
+
+
+def test_stiffness_cap_is_a_contact_of_ksp_steps(oracle_mod):
+    ksp = 1.0e4  # long contact: the cap binds
+    for delta in (1e-7, 5e-7, 2e-6):
+        k = -linear_spring(oracle_mod, ksp, 0.7, delta, 0.0) / delta
+        t_half = math.pi * math.sqrt(mass(D) / k)
+        assert t_half == pytest.approx(ksp * DT, rel=1e-9)
+
+
+@pytest.mark.parametrize("erest", [0.3, 0.7, 0.95])
+def test_dashpot_gives_the_restitution_coefficient(oracle_mod, erest):
+    ksp, delta = 1.0e4, 1e-6
+    f0 = linear_spring(oracle_mod, ksp, erest, delta, 0.0)
+    f1 = linear_spring(oracle_mod, ksp, erest, delta, 1.0)
+    k = -f0 / delta
+    eta = -(f1 - f0)  # per unit approach speed
+    m = mass(D)
+    meff = m / 2.0
+    # the two-body contact in the relative coordinate: meff x'' = -k x - eta' x',
+    # where each particle's force is the pair force, so the relative motion
+    # sees 2 k / m = k / meff and 2 eta / m = eta / meff
+    v0, n = 1.0, 200000
+    period = math.pi / math.sqrt(k / meff)
+    h = 4.0 * period / n
+    x, v = 0.0, v0
+    for _ in range(n):  # velocity Verlet until the overlap ends
+        a = (-k * x - eta * v) / meff
+        vh = v + 0.5 * h * a
+        x = x + h * vh
+        a = (-k * x - eta * vh) / meff
+        v = vh + 0.5 * h * a
+        if x < 0.0:
+            break
+    assert -v / v0 == pytest.approx(erest, rel=2e-3)
+
+
+def test_pair_symmetry_and_reach(oracle_mod):
+    xi, xj = np.array([0.1, 0.2, 0.3]), np.array([0.1 + 0.6 * D, 0.2 + 0.5 * D, 0.3 - 0.4 * D])
+    vi, vj = np.array([3.0, -1.0, 2.0]), np.array([-2.0, 0.5, 1.0])
+    fi = oracle_mod.collision_force(xi, vi, D, xj, vj, D, DT, 10.0, 0.7)
+    fj = oracle_mod.collision_force(xj, vj, D, xi, vi, D, DT, 10.0, 0.7)
+    assert np.allclose(fi, -fj, rtol=1e-12, atol=0)
+    n = (xj - xi) / np.linalg.norm(xj - xi)
+    assert np.allclose(np.cross(fi, n), 0.0, atol=1e-12 * np.abs(fi).max())
+    far = xi + 1.0001 * D * n
+    assert np.all(oracle_mod.collision_force(xi, vi, D, far, vj, D, DT, 10.0, 0.7) == 0.0)
+
+
+def stage(oracle_mod, y, pc, **pp):
+    n, h = 8, 0.0005
+    mesh, params, ic, _, picl = configs.curtain(nx=n, ny=4, nz=4, npart=4, x_shock=0.9 * n * h)
+    picl.update(amFlag=0, **pp)
+    o = oracle_mod.Region(mesh, **params)
+    o.set_cv(ic(o.cells()[1]))
+    o.picl_init(y, pc, **picl)
+    o.rk_stage(1, DT)
+    return o.picl_particles()
+
+
+def pair_state(x1, x2):
+    y = np.zeros((2, 12))
+    y[0, :3], y[1, :3] = x1, x2
+    y[:, 6] = 300.0
+    pc = np.array([[RHOP, D, math.pi / 6 * D ** 3, 1.0]] * 2)
+    return y, pc
+
+
+def test_collisions_stay_inside_the_particle_phase(oracle_mod):
+    y, pc = pair_state([1e-3, 1e-3, 1e-3], [1e-3 + 0.9 * D, 1e-3, 1e-3])
+    on = stage(oracle_mod, y, pc, collisionalFlag=1, ksp=10.0, erest=0.7)
+    off = stage(oracle_mod, y, pc, collisionalFlag=0)
+    m = mass(D)
+    fc = (on["ydot"][:, 3:6] - off["ydot"][:, 3:6]) * m
+    assert np.abs(fc[:, 0]).max() > 0.0
+    assert np.allclose(fc[0], -fc[1], rtol=1e-9, atol=0)
+    assert fc[0, 0] < 0.0  # the left particle is pushed left
+    assert np.allclose(on["ydotc"][:, :4], off["ydotc"][:, :4], rtol=0, atol=1e-9 * np.abs(fc).max())
+
+
+def test_contacts_across_a_periodic_face(oracle_mod):
+    ly = 4 * 0.0005
+    y, pc = pair_state([1e-3, 0.2 * D, 1e-3], [1e-3, ly - 0.5 * D, 1e-3])  # 0.7 D apart through y = 0
+    per = stage(oracle_mod, y, pc, collisionalFlag=1, ksp=10.0, erest=0.7, periodicY=1)
+    flat = stage(oracle_mod, y, pc, collisionalFlag=1, ksp=10.0, erest=0.7, periodicY=0)
+    off = stage(oracle_mod, y, pc, collisionalFlag=0, periodicY=1)
+    dy = (per["ydot"][:, 4] - off["ydot"][:, 4]) * mass(D)
+    assert dy[0] > 0.0 and dy[1] < 0.0  # pushed apart across the face
+    assert np.array_equal(flat["ydot"], stage(oracle_mod, y, pc, collisionalFlag=0)["ydot"])
