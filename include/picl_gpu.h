@@ -77,8 +77,10 @@ typedef struct picl_params {
   /* COLLISIONS (collisional_flag, PICL_TEMP_InitSolver.F90:204): 0 off; 1 the
    * normal soft-sphere force between overlapping particles
    * (ppiclf_user_EvalNearestNeighbor.f:118-233, Cundall and Strack), with
-   * periodic images on periodic axes.  2-4 (tangential force, torques, lift)
-   * are rejected.  KSP and EREST (global%piclKsp, piclERest, :222-223):
+   * periodic images on periodic axes; 2 adds the tangential force (Coulomb
+   * friction or dashpot, the smaller), the contact and rolling-resistance
+   * torques and the angular velocity y(JOX..JOZ) they drive.  3-4
+   * (hydrodynamic torque, lift) are rejected.  KSP and EREST (global%piclKsp, piclERest, :222-223):
    * the spring stiffness is capped at rmass pi^2 / (ksp dt)^2, erest in (0, 1]
    * sets the damping. */
   int collisionalFlag;

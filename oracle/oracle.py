@@ -112,6 +112,7 @@ def lib():
         L.orc_picl_vu_state.restype = I
         D = C.c_double
         L.orc_picl_collision_force.argtypes = [P, P, D, D, D, P, P, D, D, D, D, D, D, P]
+        L.orc_picl_collision_force2.argtypes = [P, P, D, D, D, D, D, D, D, D, D, P, P]
         _LIB = L
     return _LIB
 
@@ -354,6 +355,19 @@ def collision_force(xi, vi, dpi, xj, vj, dpj, dt, ksp, erest, rhop=2500.0):
     lib().orc_picl_collision_force(_p(a[0]), _p(a[1]), float(dpi), rhop, vol(dpi), _p(a[2]), _p(a[3]), float(dpj),
                                    rhop, vol(dpj), float(dt), float(ksp), float(erest), _p(out))
     return out
+
+
+def collision_force2(xi, vi, wi, dpi, xj, vj, wj, dpj, dt, ksp, erest, rhop=2500.0):
+    """COLLISIONS 2: (force, torque) of particle j on particle i, angular
+    velocities wi, wj; spheres of density rhop."""
+    yi, yj = np.zeros(12), np.zeros(12)
+    yi[:3], yi[3:6], yi[7:10] = xi, vi, wi
+    yj[:3], yj[3:6], yj[7:10] = xj, vj, wj
+    f, t = np.zeros(3), np.zeros(3)
+    vol = lambda d: math.pi / 6.0 * d ** 3
+    lib().orc_picl_collision_force2(_p(yi), _p(yj), float(dpi), rhop, vol(dpi), float(dpj), rhop, vol(dpj),
+                                    float(dt), float(ksp), float(erest), _p(f), _p(t))
+    return f, t
 
 
 def qs_beta(qsFlag, rmachp, rphip, rep, rmu, dp, rhof=1.0, vmag=1.0):

@@ -685,8 +685,8 @@ static void vu_shift(struct orc_picl *p, double dt) {
  * stiffness of the overlap), damping from the restitution coefficient;
  * tangential part zero (Ftmin = 0).  Added into fc as ppiclF adds it into
  * ydotc(JVX..JVZ, i). */
-static void collision_pair(const double *yi, const double *rpi_, const double *yj, const double *rpj, double rmass,
-                           double dt, double ksp, double erest, double *fc) {
+static void collision_pair(int flag, const double *yi, const double *rpi_, const double *yj, const double *rpj,
+                           double rmass, double dt, double ksp, double erest, double *fc, double *tq) {
   const double rpi = acos(-1.0), pi2 = rpi * rpi;
   double rthresh = 0.5 * (rpi_[R_JDP] + rpj[R_JDP]);
   double rxdiff = yj[JX] - yi[JX], rydiff = yj[JY] - yi[JY], rzdiff = yj[JZ] - yi[JZ];
@@ -710,17 +710,49 @@ static void collision_pair(const double *yi, const double *rpi_, const double *y
   double rbot = 1.0 / rdiff;
   double rn[3] = {rxdiff * rbot, rydiff * rbot, rzdiff * rbot};
   double u12[3] = {yi[JVX] - yj[JVX], yi[JVY] - yj[JVY], yi[JVZ] - yj[JVZ]};
+  const double *oi = yi + 7, *oj = yj + 7; /* JOX..JOZ */
+  if (flag >= 2) { /* contact-point velocity of the rotations */
+    double rad1 = 0.5 * rpi_[R_JDP], rad2 = 0.5 * rpj[R_JDP];
+    double A12[3];
+    for (int c = 0; c < 3; c++) A12[c] = rad1 * oi[c] + rad2 * oj[c];
+    u12[0] = u12[0] + (A12[1] * rn[2] - A12[2] * rn[1]);
+    u12[1] = u12[1] + (A12[2] * rn[0] - A12[0] * rn[2]);
+    u12[2] = u12[2] + (A12[0] * rn[1] - A12[1] * rn[0]);
+  }
   double rv12_mag = u12[0] * rn[0] + u12[1] * rn[1] + u12[2] * rn[2];
   double rdelta12 = rthresh - rdiff;
   double rksp_max = ksp_min * rdelta12;
   double rv12_mage = rv12_mag * eta_n;
   double rnmag = -rksp_max - rv12_mage;
+  double Fn_mag = fabs(rnmag);
   double ut[3];
   for (int c = 0; c < 3; c++) ut[c] = u12[c] - rv12_mag * rn[c];
   double ut_mag = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
   ut_mag = fmax(ut_mag, 1.0e-8);
-  const double Ftmin = 0.0;
-  for (int c = 0; c < 3; c++) fc[c] = fc[c] + rnmag * rn[c] + Ftmin * (ut[c] / ut_mag);
+  double rt[3] = {ut[0] / ut_mag, ut[1] / ut_mag, ut[2] / ut_mag};
+  double Ftmin = 0.0;
+  if (flag >= 2 && ut_mag > 0.0) { /* Coulomb friction or tangential dashpot, the smaller */
+    const double mu_c = 0.4;
+    double eta_t = eta_n;
+    Ftmin = -fmin(mu_c * Fn_mag, eta_t * ut_mag);
+  }
+  double tc[3] = {0.0, 0.0, 0.0}, tr[3] = {0.0, 0.0, 0.0};
+  if (flag >= 2) {
+    double Ft[3] = {Ftmin * rt[0], Ftmin * rt[1], Ftmin * rt[2]};
+    double rad1 = 0.5 * rpi_[R_JDP];
+    tc[0] = rad1 * (rn[1] * Ft[2] - rn[2] * Ft[1]);
+    tc[1] = rad1 * (rn[2] * Ft[0] - rn[0] * Ft[2]);
+    tc[2] = rad1 * (rn[0] * Ft[1] - rn[1] * Ft[0]);
+    const double thetar = F32(0.06); /* rolling resistance, a real*4 literal */
+    double dp1 = rpi_[R_JDP], dp2 = rpj[R_JDP];
+    double r12 = 0.5 * (dp1 * dp2) / (dp1 + dp2);
+    double omgr[3] = {oi[0] - oj[0], oi[1] - oj[1], oi[2] - oj[2]};
+    double omgr_mag = sqrt(omgr[0] * omgr[0] + omgr[1] * omgr[1] + omgr[2] * omgr[2]);
+    omgr_mag = fmax(omgr_mag, 1.0e-8);
+    for (int c = 0; c < 3; c++) tr[c] = -thetar * Fn_mag * r12 * omgr[c] / omgr_mag;
+  }
+  for (int c = 0; c < 3; c++) fc[c] = fc[c] + rnmag * rn[c] + Ftmin * rt[c];
+  for (int c = 0; c < 3; c++) tq[c] = tq[c] + tc[c] + tr[c];
 }
 
 void orc_picl_collision_force(const double *xi, const double *vi, double dpi, double rhopi, double volpi,
@@ -739,20 +771,39 @@ void orc_picl_collision_force(const double *xi, const double *vi, double dpi, do
   rj[R_JDP] = dpj;
   rj[R_JRHOP] = rhopj;
   rj[R_JVOLP] = volpj;
+  double tq[3] = {0.0, 0.0, 0.0};
   fc[0] = fc[1] = fc[2] = 0.0;
-  collision_pair(yi, ri, yj, rj, volpi * rhopi, dt, ksp, erest, fc);
+  collision_pair(1, yi, ri, yj, rj, volpi * rhopi, dt, ksp, erest, fc, tq);
+}
+
+void orc_picl_collision_force2(const double *yi12, const double *yj12, double dpi, double rhopi, double volpi,
+                               double dpj, double rhopj, double volpj, double dt, double ksp, double erest,
+                               double *fc, double *tq) {
+  double ri[LRP] = {0}, rj[LRP] = {0};
+  ri[R_JDP] = dpi;
+  ri[R_JRHOP] = rhopi;
+  ri[R_JVOLP] = volpi;
+  rj[R_JDP] = dpj;
+  rj[R_JRHOP] = rhopj;
+  rj[R_JVOLP] = volpj;
+  fc[0] = fc[1] = fc[2] = 0.0;
+  tq[0] = tq[1] = tq[2] = 0.0;
+  collision_pair(2, yi12, ri, yj12, rj, volpi * rhopi, dt, ksp, erest, fc, tq);
 }
 
 /* ppiclf_solve_NearestNeighbor (ppiclf_solve.f:470-560) for the collision
  * force: the other particles in ascending index, then the ghost particles,
  * here the periodic images, by particle and then image. */
-static void collision_sum(const struct orc_picl *p, int i, double rmass, double dt, double *fc) {
+static void collision_sum(const struct orc_picl *p, int i, double rmass, double dt, double *fc, double *tq) {
   const double *yi = p->y + LRS * (size_t)i, *rpi_ = p->rprop + LRP * (size_t)i;
   const orc_picl_params *P = &p->pp;
   fc[0] = fc[1] = fc[2] = 0.0;
+  tq[0] = tq[1] = tq[2] = 0.0;
+  const int flag = P->collisionalFlag;
   for (int j = 0; j < p->n; j++) {
     if (j == i || p->lost[j]) continue;
-    collision_pair(yi, rpi_, p->y + LRS * (size_t)j, p->rprop + LRP * (size_t)j, rmass, dt, P->ksp, P->erest, fc);
+    collision_pair(flag, yi, rpi_, p->y + LRS * (size_t)j, p->rprop + LRP * (size_t)j, rmass, dt, P->ksp, P->erest,
+                   fc, tq);
   }
   if (!(p->per[0] || p->per[1] || p->per[2])) return;
   for (int j = 0; j < p->n; j++) {
@@ -768,7 +819,7 @@ static void collision_sum(const struct orc_picl *p, int i, double rmass, double 
         if (sd != 0) yj[d] = yj[d] + (double)sd * (p->perHi[d] - p->perLo[d]);
       }
       if (skip) continue;
-      collision_pair(yi, rpi_, yj, p->rprop + LRP * (size_t)j, rmass, dt, P->ksp, P->erest, fc);
+      collision_pair(flag, yi, rpi_, yj, p->rprop + LRP * (size_t)j, rmass, dt, P->ksp, P->erest, fc, tq);
     }
   }
 }
@@ -810,9 +861,10 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
   double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
   double fcx = 0.0, fcy = 0.0, fcz = 0.0;
   double fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
+  double tqc[3] = {0.0, 0.0, 0.0}; /* collision torques (ydotc(JOX..JOZ)) */
   if (P->collisionalFlag >= 1) { /* Step 1b and Step 6, ppiclf_user.f:411-470, 587-600 */
     double fc[3];
-    collision_sum(p, i, rmass, dt, fc);
+    collision_sum(p, i, rmass, dt, fc, tqc);
     fcx = fc[0];
     fcy = fc[1];
     fcz = fc[2];
@@ -873,7 +925,12 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
     qq = heat_transfer(P->heatTransfer, rkappa, dp, rp[R_JT], y[JT], rep, rphif, rmachp);
   }
   double rmass_omega = rmass * dp * dp / 10.0;
-  const double taux = 0.0, tauy = 0.0, tauz = 0.0;
+  double taux = 0.0, tauy = 0.0, tauz = 0.0;
+  if (P->collisionalFlag >= 2) { /* Step 9a, :638-646; Torque_driver adds nothing below flag 3 */
+    taux = tqc[0] + 0.0 + 0.0;
+    tauy = tqc[1] + 0.0 + 0.0;
+    tauz = tqc[2] + 0.0 + 0.0;
+  }
   yd[JX] = y[JVX];
   yd[JY] = y[JVY];
   yd[JZ] = y[JVZ];
@@ -1028,7 +1085,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
     if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4 || pp->viscousUnsteady < 0 ||
-      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 1)
+      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 2)
     return -1;
   if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0)) return -1;
   const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};

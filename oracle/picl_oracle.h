@@ -40,7 +40,8 @@ typedef struct orc_picl_params {
   int periodicX, periodicY, periodicZ; /* PERIODICX/Y/Z: particles wrap across the mesh's
                                           vertex bounds on that axis */
   int viscousUnsteady; /* 0: off; 1: viscous-unsteady force with its history kernel */
-  int collisionalFlag; /* 0: off; 1: normal soft-sphere collision force (Cundall-Strack) */
+  int collisionalFlag; /* 0: off; 1: normal soft-sphere collision force (Cundall-Strack);
+                          2: + tangential force, contact and rolling torques */
   double ksp;          /* KSP: collision time = ksp * dt sets the spring stiffness cap */
   double erest;        /* ERESTITUTION: restitution coefficient of the damping */
 } orc_picl_params;
@@ -86,6 +87,11 @@ int orc_picl_vu_state(const orc_region *r, double *timeBH, double *hist);
 void orc_picl_collision_force(const double *xi, const double *vi, double dpi, double rhopi, double volpi,
                               const double *xj, const double *vj, double dpj, double rhopj, double volpj, double dt,
                               double ksp, double erest, double *fc);
+/* COLLISIONS 2: force and torque of particle j on particle i from their
+ * 12-component states (x, v, T, omega, ...). */
+void orc_picl_collision_force2(const double *yi12, const double *yj12, double dpi, double rhopi, double volpi,
+                               double dpj, double rhopj, double volpj, double dt, double ksp, double erest,
+                               double *fc, double *tq);
 /* Element containing each point (-1: none), by the oracle's location rule. */
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
 /* Elements LocalInterp sums for point xp, nearest first; returns how many. */

@@ -44,12 +44,13 @@ struct PiclPhys {
 // Rows of PBuf::st
 // particle state rows: x, y, z, u, v, w (and their y1, ydot), the feedback
 // map, then the temperature's y, y1 and ydot, touched only with heat
-// transfer.  With the viscous-unsteady force VU_N entries of d(rho u)/dt of
-// the gas (ROW_VU + c * VU_N + k) and of the particle (ROW_VU + 3 * VU_N +
-// c * VU_N + k), k = 0 the newest, follow (ppiclF's rprop3).
+// transfer.  With tangential collisions the angular velocity's y, y1 and
+// ydot follow (ROW_O..), with the viscous-unsteady force VU_N entries of
+// d(rho u)/dt of the gas (ROW_VU + c * VU_N + k) and of the particle
+// (ROW_VU + 3 * VU_N + c * VU_N + k), k = 0 the newest (ppiclF's rprop3).
 #define VU_N 5
 enum { ROW_X = 0, ROW_Y1 = 6, ROW_YD = 12, ROW_YCP = 18, ROW_T = 22, ROW_TY1 = 23, ROW_TYD = 24, NROW = 25,
-       ROW_VU = 25, NROW_VU = ROW_VU + 6 * VU_N };
+       ROW_O = 25, ROW_OY1 = 28, ROW_OYD = 31, ROW_VU = 34, NROW_VU = ROW_VU + 6 * VU_N };
 
 struct PBuf {
   double *st = nullptr;  // [nRow][nP]
@@ -62,6 +63,7 @@ struct PiclDev {
   int nP = 0, nC = 0;
   int ht = 0;  // heat transfer on: the temperature rows move with the particles
   int vu = 0;  // viscous-unsteady force on: the history rows move with the particles
+  int rot = 0;  // tangential collisions on: the angular-velocity rows move with the particles
   int nRow = NROW;
   double *vuT = nullptr;  // [VU_N] ppiclf_timeBH, then ppiclf_nTimeBH
   PBuf buf[2];
@@ -106,6 +108,7 @@ struct PiclDev {
   int *colCnt = nullptr, *colOff = nullptr, *colFill = nullptr, *colMem = nullptr, *colB = nullptr,
       *colBsum = nullptr;
   double *fc = nullptr;  // [3][nP] collision force by current slot
+  double *tq = nullptr;  // [3][nP] collision torque by current slot (tangential collisions)
 };
 
 // ---------------------------------------------------------------------------
@@ -629,9 +632,9 @@ __global__ void __launch_bounds__(BLK) k_col_fill(PiclDev P) {
 // Strack) with spring min(rmass_i pi^2 / (ksp dt)^2, the overlap's Hertz
 // stiffness) and damping from the restitution coefficient; Ftmin = 0.
 // Same operations as the oracle.
-__device__ void col_pair(const double xi[3], const double vi[3], double dpi, double rmi, const double xj[3],
-                         const double vj[3], double dpj, double rmj, double rmass, double dt, double ksp,
-                         double erest, double fc[3]) {
+__device__ void col_pair(int flag, const double xi[3], const double vi[3], const double oi[3], double dpi, double rmi,
+                         const double xj[3], const double vj[3], const double oj[3], double dpj, double rmj,
+                         double rmass, double dt, double ksp, double erest, double fc[3], double tq[3]) {
   const double rpi = 3.141592653589793116, pi2 = rpi * rpi;
   double rthresh = 0.5 * (dpi + dpj);
   double rxdiff = xj[0] - xi[0], rydiff = xj[1] - xi[1], rzdiff = xj[2] - xi[2];
@@ -652,19 +655,48 @@ __device__ void col_pair(const double xi[3], const double vi[3], double dpi, dou
   double rbot = 1.0 / rdiff;
   double rn[3] = {rxdiff * rbot, rydiff * rbot, rzdiff * rbot};
   double u12[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+  if (flag >= 2) {  // contact-point velocity of the rotations
+    double rad1 = 0.5 * dpi, rad2 = 0.5 * dpj;
+    double A12[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) A12[c] = rad1 * oi[c] + rad2 * oj[c];
+    u12[0] = u12[0] + (A12[1] * rn[2] - A12[2] * rn[1]);
+    u12[1] = u12[1] + (A12[2] * rn[0] - A12[0] * rn[2]);
+    u12[2] = u12[2] + (A12[0] * rn[1] - A12[1] * rn[0]);
+  }
   double rv12_mag = u12[0] * rn[0] + u12[1] * rn[1] + u12[2] * rn[2];
   double rdelta12 = rthresh - rdiff;
   double rksp_max = ksp_min * rdelta12;
   double rv12_mage = rv12_mag * eta_n;
   double rnmag = -rksp_max - rv12_mage;
+  double Fn_mag = fabs(rnmag);
   double ut[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) ut[c] = u12[c] - rv12_mag * rn[c];
   double ut_mag = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
   ut_mag = fmax(ut_mag, 1.0e-8);
-  const double Ftmin = 0.0;
+  double rt[3] = {ut[0] / ut_mag, ut[1] / ut_mag, ut[2] / ut_mag};
+  double Ftmin = 0.0;
+  if (flag >= 2 && ut_mag > 0.0) {  // Coulomb friction or tangential dashpot, the smaller
+    const double mu_c = 0.4;
+    double eta_t = eta_n;
+    Ftmin = -fmin(mu_c * Fn_mag, eta_t * ut_mag);
+  }
 #pragma unroll
-  for (int c = 0; c < 3; c++) fc[c] = fc[c] + rnmag * rn[c] + Ftmin * (ut[c] / ut_mag);
+  for (int c = 0; c < 3; c++) fc[c] = fc[c] + rnmag * rn[c] + Ftmin * rt[c];
+  if (flag >= 2) {  // contact torque and rolling resistance into ydotc(JOX..JOZ)
+    double Ft[3] = {Ftmin * rt[0], Ftmin * rt[1], Ftmin * rt[2]};
+    double rad1 = 0.5 * dpi;
+    double tc[3] = {rad1 * (rn[1] * Ft[2] - rn[2] * Ft[1]), rad1 * (rn[2] * Ft[0] - rn[0] * Ft[2]),
+                    rad1 * (rn[0] * Ft[1] - rn[1] * Ft[0])};
+    const double thetar = F32(0.06);  // a real*4 literal
+    double r12 = 0.5 * (dpi * dpj) / (dpi + dpj);
+    double omgr[3] = {oi[0] - oj[0], oi[1] - oj[1], oi[2] - oj[2]};
+    double omgr_mag = sqrt(omgr[0] * omgr[0] + omgr[1] * omgr[1] + omgr[2] * omgr[2]);
+    omgr_mag = fmax(omgr_mag, 1.0e-8);
+#pragma unroll
+    for (int c = 0; c < 3; c++) tq[c] = tq[c] + tc[c] + -thetar * Fn_mag * r12 * omgr[c] / omgr_mag;
+  }
 }
 
 __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph) {
@@ -673,7 +705,7 @@ __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph)
   const size_t nP = P.nP;
   PBuf C = cur_buf(P);
   const double *st = C.st;
-  double fc[3] = {0.0, 0.0, 0.0};
+  double fc[3] = {0.0, 0.0, 0.0}, tq[3] = {0.0, 0.0, 0.0};
   if (C.elem[t] >= 0) {
     double xi[3] = {st[t], st[nP + t], st[2 * nP + t]};
     double dpi = C.pc[nP + t];
@@ -735,24 +767,35 @@ __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph)
       slot[j + 1] = y;
     }
     double vi[3] = {st[3 * nP + t], st[4 * nP + t], st[5 * nP + t]};
+    double oi[3] = {0.0, 0.0, 0.0};
+    if (P.rot) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) oi[c] = st[(ROW_O + c) * nP + t];
+    }
     double rmi = C.pc[t] * C.pc[2 * nP + t];
     double rmass = C.pc[2 * nP + t] * C.pc[t];  // volp * rhop (ppiclf_user.f:313)
     double dt = d.scal[S_DTMIN];
     for (int a = 0; a < nk; a++) {
       int sj = slot[a], im = (int)(key[a] & 31);
-      double xj[3], vj[3];
+      double xj[3], vj[3], oj[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int l = 0, r = im; l < 3; l++, r /= 3) {
         int sd = r % 3 - 1;
         double x = st[l * nP + sj];
         xj[l] = sd == 0 ? x : x + (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
         vj[l] = st[(3 + l) * nP + sj];
+        if (P.rot) oj[l] = st[(ROW_O + l) * nP + sj];
       }
-      col_pair(xi, vi, dpi, rmi, xj, vj, C.pc[nP + sj], C.pc[sj] * C.pc[2 * nP + sj], rmass, dt, Ph.ksp, Ph.erest, fc);
+      col_pair(Ph.col, xi, vi, oi, dpi, rmi, xj, vj, oj, C.pc[nP + sj], C.pc[sj] * C.pc[2 * nP + sj], rmass, dt,
+               Ph.ksp, Ph.erest, fc, tq);
     }
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) P.fc[c * nP + t] = fc[c];
+  if (P.rot) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) P.tq[c * nP + t] = tq[c];
+  }
 }
 
 // ppiclf_user_VU_Rocflu (ppiclf_user_ViscousUnsteady.f:19-90): the history
@@ -939,7 +982,7 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   double rep = vmag * dpp * rhof / rmu;
   double rphip = v[9];
   double rmachp = vmag / v[7];
-  double ydn[6], ydT = 0.0;
+  double ydn[6], ydT = 0.0, ydO[3] = {0.0, 0.0, 0.0};
   bool skip = vmag <= 1.0e-3;  // keeps the previous ydot; ydotc stays 0
   double yc[4] = {0.0, 0.0, 0.0, 0.0};
   if (!skip) {
@@ -1016,6 +1059,14 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       ydT = qq / (rmass * Ph.rcpPart);
       st[ROW_TYD * nP + t] = ydT;
     }
+    if (P.rot) {  // Step 9a (:638-646): ydot(JOX..JOZ) = tau / (rmass dp^2 / 10)
+      double rmass_omega = rmass * dpp * dpp / 10.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        ydO[c] = (P.tq[c * nP + t] + 0.0 + 0.0) / rmass_omega;
+        st[(ROW_OYD + c) * nP + t] = ydO[c];
+      }
+    }
     if (Ph.feedbackFlag == 1) {
       yc[0] = spl * (ydn[3] * rmass - fcv[0]);
       yc[1] = spl * (ydn[4] * rmass - fcv[1]);
@@ -1032,6 +1083,10 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
 #pragma unroll
     for (int k = 0; k < 6; k++) ydn[k] = st[(ROW_YD + k) * nP + t];
     if (Ph.heatTransfer) ydT = st[ROW_TYD * nP + t];
+    if (P.rot) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) ydO[c] = st[(ROW_OYD + c) * nP + t];
+    }
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = yc[k];
@@ -1072,6 +1127,15 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
     double y = st[ROW_T * nP + t];
     st[ROW_T * nP + t] = -(k0 * y1) + 1.0 * y + k2 * ydT;
     st[ROW_TY1 * nP + t] = ydT;
+  }
+  if (P.rot) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double y1 = istage == 1 ? 0.0 : st[(ROW_OY1 + c) * nP + t];
+      double y = st[(ROW_O + c) * nP + t];
+      st[(ROW_O + c) * nP + t] = -(k0 * y1) + 1.0 * y + k2 * ydO[c];
+      st[(ROW_OY1 + c) * nP + t] = ydO[c];
+    }
   }
 }
 
@@ -1387,6 +1451,8 @@ __global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
   for (int k = 0; k < 4; k++) X.pc[k * nP + pos] = C.pc[k * nP + t];
   if (P.ht)
     for (int r = ROW_T; r < NROW; r++) X.st[r * nP + pos] = C.st[r * nP + t];
+  if (P.rot)
+    for (int r = ROW_O; r < ROW_VU; r++) X.st[r * nP + pos] = C.st[r * nP + t];
   if (P.vu)
     for (int r = ROW_VU; r < NROW_VU; r++) X.st[r * nP + pos] = C.st[r * nP + t];
   X.pid[pos] = C.pid[t];
@@ -1492,7 +1558,7 @@ void picl_free(PiclState *s) {
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
                   P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
-                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc};
+                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1634,10 +1700,10 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
   if (pp->viscousUnsteady < 0 || pp->viscousUnsteady > 1)
     return set_err(h, RFLU_GPU_EINVAL, "viscousUnsteady must be 0 or 1");
-  if (pp->collisionalFlag < 0 || pp->collisionalFlag > 1)
+  if (pp->collisionalFlag < 0 || pp->collisionalFlag > 2)
     return set_err(h, RFLU_GPU_EINVAL,
-                   "collisionalFlag must be 0 (off) or 1 (normal force); the tangential force, torques and lift "
-                   "(2-4) are not built");
+                   "collisionalFlag must be 0 (off), 1 (normal force) or 2 (+ tangential force and contact "
+                   "torques); the hydrodynamic torque and lift (3-4) are not built");
   if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0))
     return set_err(h, RFLU_GPU_EINVAL, "collisions need ksp > 0 and 0 < erest <= 1");
   if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
@@ -1671,9 +1737,10 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   s->P.ht = pp->heatTransfer != 0 ? 1 : 0;
   s->phys.vu = s->P.vu = pp->viscousUnsteady;
   s->phys.col = s->P.col = pp->collisionalFlag;
+  s->P.rot = pp->collisionalFlag >= 2 ? 1 : 0;
   s->phys.ksp = pp->ksp;
   s->phys.erest = pp->erest;
-  s->P.nRow = s->P.vu ? NROW_VU : NROW;
+  s->P.nRow = s->P.vu ? NROW_VU : (s->P.rot ? ROW_VU : NROW);
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
   std::vector<double> X(24 * (size_t)N), bbox(8 * (size_t)N, 0.0), cen(4 * (size_t)N, 0.0), rexp(N);
@@ -1747,6 +1814,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   for (int p = 0; p < nP; p++) {
     for (int k = 0; k < 6; k++) st[(ROW_X + k) * (size_t)nP + p] = y[12 * (size_t)p + k];
     st[ROW_T * (size_t)nP + p] = y[12 * (size_t)p + 6];
+    if (P.rot)
+      for (int k = 0; k < 3; k++) st[(ROW_O + k) * (size_t)nP + p] = y[12 * (size_t)p + 7 + k];
     for (int k = 0; k < 4; k++) pcs[k * (size_t)nP + p] = pconst[4 * (size_t)p + k];
   }
   std::vector<int> ident(nP), minus1(nP, -1);
@@ -1776,6 +1845,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.colB, (size_t)nP);
     rc |= dalloc(h, &P.colBsum, (size_t)scan_blocks(P.colNb + 1));
     rc |= dalloc(h, &P.fc, 3 * (size_t)nP);
+    if (P.rot) rc |= dalloc(h, &P.tq, 3 * (size_t)nP);
   }
   if (P.vu) {  // ppiclf_nTimeBH = 1 (PICL_TEMP_InitSolver.F90:290), ages 0
     std::vector<double> t0(VU_N + 1, 0.0);
@@ -1923,7 +1993,7 @@ int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, 
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(&par, P.par, sizeof(int), hipMemcpyDeviceToHost));
   const PBuf &C = P.buf[par];
-  std::vector<double> st(NROW * nP);
+  std::vector<double> st((size_t)(P.rot ? ROW_VU : NROW) * nP);
   std::vector<int> pid(nP), ev(nP);
   HIPCHK(h, hipMemcpy(st.data(), C.st, st.size() * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(pid.data(), C.pid, nP * sizeof(int), hipMemcpyDeviceToHost));
@@ -1946,6 +2016,16 @@ int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, 
       if (y) y[12 * p + 6] = st[ROW_T * nP + t];
       if (y1) y1[12 * p + 6] = st[ROW_TY1 * nP + t];
       if (ydot) ydot[12 * p + 6] = st[ROW_TYD * nP + t];
+    }
+  }
+  if (P.rot) {  // the angular velocity evolves
+    for (size_t t = 0; t < nP; t++) {
+      size_t p = pid[t];
+      for (int c = 0; c < 3; c++) {
+        if (y) y[12 * p + 7 + c] = st[(ROW_O + c) * nP + t];
+        if (y1) y1[12 * p + 7 + c] = st[(ROW_OY1 + c) * nP + t];
+        if (ydot) ydot[12 * p + 7 + c] = st[(ROW_OYD + c) * nP + t];
+      }
     }
   }
   if (elem)

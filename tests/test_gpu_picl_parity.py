@@ -387,3 +387,30 @@ def test_coupled_steps_with_collisions(oracle_mod, periodic):
         assert ok, (k, err)
     ok, err = close(g.get_state()[0], o.cv())
     assert ok, err
+
+
+def test_coupled_steps_with_tangential_collisions(oracle_mod):
+    """COLLISIONS 2: random spins and the dense curtain; the tangential
+    force, the contact and rolling torques and the angular velocity they
+    drive, with periodic y and z."""
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=2000, jitter=0.05, seed=4, phi=0.3)
+    y = y.copy()
+    y[:, 7:10] = np.random.default_rng(9).uniform(-3e4, 3e4, (len(y), 3))
+    picl.update(collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(4, use_graph=True)
+    o.step(4)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(op["ydot"][:, 7:10]).max() > 0.0  # the spins changed
+    for k in ("y", "y1", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err

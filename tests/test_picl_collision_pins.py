@@ -1,4 +1,4 @@
-"""Pins of the oracle's particle collisions (COLLISIONS 1,
+"""Pins of the oracle's particle collisions (COLLISIONS 1 and 2,
 ppiclf_user_EvalNearestNeighbor.f:118-233).  CPU only.
 
 What contact mechanics fixes, not a retyped formula:
@@ -15,7 +15,12 @@ What contact mechanics fixes, not a retyped formula:
 * in a stage the collisions change the particles' accelerations but not the
   force fed back to the gas (they are internal to the particle phase), and
   two particles overlapping across a periodic face collide only when the
-  axis is periodic.
+  axis is periodic;
+* with COLLISIONS 2 the tangential force obeys Coulomb's law at large slip
+  and is the normal dashpot at small slip, the contact torque is the lever
+  arm (d/2) n x Ft, rolling resistance is 0.06 |Fn| R*, friction spins a
+  sphere down, and the angular velocity changes at torque over a solid
+  sphere's moment of inertia (2/5) m r^2.
 """
 import math
 
@@ -137,3 +142,60 @@ def test_contacts_across_a_periodic_face(oracle_mod):
     dy = (per["ydot"][:, 4] - off["ydot"][:, 4]) * mass(D)
     assert dy[0] > 0.0 and dy[1] < 0.0  # pushed apart across the face
     assert np.array_equal(flat["ydot"], stage(oracle_mod, y, pc, collisionalFlag=0)["ydot"])
+
+
+# ---- COLLISIONS 2: tangential force, contact and rolling torques ----
+def pair2(oracle_mod, vt=0.0, wi=(0.0, 0.0, 0.0), wj=(0.0, 0.0, 0.0), vn=0.0, delta=1e-6, ksp=1.0e4, erest=0.7):
+    """i at the origin, j along +x overlapping by delta; i slides along +y at
+    vt and approaches at vn."""
+    return oracle_mod.collision_force2([0, 0, 0], [vn, vt, 0], wi, D, [D - delta, 0, 0], [0, 0, 0], wj, D, DT, ksp,
+                                       erest)
+
+
+def test_coulomb_friction_at_large_slip(oracle_mod):
+    f, t = pair2(oracle_mod, vt=50.0)
+    assert f[1] < 0.0  # opposes the slip
+    assert abs(f[1]) == pytest.approx(0.4 * abs(f[0]), rel=1e-12)
+    # lever arm: the contact point lies d/2 along the line of centres
+    lever = np.array([0.5 * D, 0.0, 0.0])
+    assert np.allclose(t, np.cross(lever, f - np.array([f[0], 0, 0])), rtol=1e-12, atol=0)
+
+
+def test_tangential_dashpot_at_small_slip_equals_the_normal_one(oracle_mod):
+    f0, _ = pair2(oracle_mod)
+    fn1, _ = pair2(oracle_mod, vn=1e-6)
+    eta_n = -(fn1[0] - f0[0]) / 1e-6
+    ft, _ = pair2(oracle_mod, vt=1e-6)
+    eta_t = -ft[1] / 1e-6
+    assert eta_t == pytest.approx(eta_n, rel=1e-6)
+
+
+def test_rolling_resistance_torque(oracle_mod):
+    d1, d2 = D, 0.6 * D
+    w = np.array([0.0, 0.0, 300.0])
+    # roll i on j without slip at the contact: the surface speeds cancel
+    f, t = oracle_mod.collision_force2([0, 0, 0], [0, 0, 0], w, d1, [0.5 * (d1 + d2) - 1e-7, 0, 0], [0, 0, 0],
+                                       -w * d1 / d2, d2, DT, 10.0, 0.7)
+    rstar = 0.5 * d1 * d2 / (d1 + d2)
+    tr = 0.06 * abs(f[0]) * rstar
+    assert t[2] < 0.0 and abs(t[2]) == pytest.approx(tr, rel=1e-7)  # opposes the relative spin, float32 0.06
+    assert abs(f[1]) < 1e-9 * abs(f[0])  # no slip, no friction
+
+
+def test_friction_spins_a_sphere_down(oracle_mod):
+    w = np.array([0.0, 0.0, 2.0e4])  # i spins about z, touching j on its +x side
+    f, t = pair2(oracle_mod, wi=w)
+    assert f[1] < 0.0  # i's contact point moves +y: friction pulls -y
+    assert t[2] < 0.0  # and the torque slows the spin
+
+
+def test_angular_velocity_follows_the_torque(oracle_mod):
+    y, pc = pair_state([1e-3, 1e-3, 1e-3], [1e-3 + 0.9 * D, 1e-3, 1e-3])
+    y[0, 4] = 5.0  # slide past each other
+    y[0, 9] = 1.0e3
+    on = stage(oracle_mod, y, pc, collisionalFlag=2, ksp=10.0, erest=0.7)
+    f, t = oracle_mod.collision_force2(y[0, :3], y[0, 3:6], y[0, 7:10], D, y[1, :3], y[1, 3:6], y[1, 7:10], D, DT,
+                                       10.0, 0.7)
+    inertia = 0.4 * mass(D) * (0.5 * D) ** 2  # a solid sphere's moment of inertia
+    assert np.abs(t).max() > 0.0
+    assert np.allclose(on["ydot"][0, 7:10] * inertia, t, rtol=1e-12, atol=1e-12 * np.abs(t).max())
