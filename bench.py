@@ -133,11 +133,23 @@ def build_workload(name):
         return (mesh, params, ic), dict(model="RFLU 2nd-order AUSM+ + ppiclF (trilinear, QS Schiller-Naumann + "
                                               "AM Parmar + PG, Gaussian projection 2dx), 400x80x80 hex, "
                                               "4M particles", n=None, particles=parts, picl=picl)
+    if name == "curtain_deck":
+        # the shock-particle-curtain with the particle physics of the AirShock
+        # input deck (input_files/AirShock/rectshktb.inp:100-116) as far as it
+        # is built: viscous-unsteady force, collisions with tangential force
+        # and torques (the deck's level 4 also adds hydrodynamic torque and
+        # lift), KSP 10, EREST 0.7, periodic y and z
+        mesh, params, ic, parts, picl = configs.curtain(limiterGuard=1)
+        picl.update(CURTAIN_DECK)
+        return (mesh, params, ic), dict(model="RFLU 2nd-order AUSM+ + ppiclF (IDW-27, QS/AM Parmar + PG + viscous "
+                                              "unsteady + collisions level 2, periodic y/z, box projection), "
+                                              "400x80x80 hex, 4M particles", n=None, particles=parts, picl=picl)
     raise SystemExit(f"unknown workload {name}")
 
 
 # h = 0.0005 is the curtain's spacing
 CURTAIN_SPEC = dict(interpMode=1, qsFlag=5, projMode=1, filterWidth=2 * 0.0005, filterAlpha=1e-2)
+CURTAIN_DECK = dict(viscousUnsteady=1, collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
 
 
 def cpu_baseline(name, budget_s=12.0):
@@ -148,6 +160,8 @@ def cpu_baseline(name, budget_s=12.0):
         mesh, params, ic, (y, pc), picl = configs.curtain(nx=40, ny=8, nz=8, npart=4000, limiterGuard=1)
         if name == "curtain_spec":
             picl.update(CURTAIN_SPEC)
+        if name == "curtain_deck":
+            picl.update(CURTAIN_DECK)
         o = orc.Region(mesh, **params)
         vol, cofg = o.cells()
         o.set_cv(ic(cofg))

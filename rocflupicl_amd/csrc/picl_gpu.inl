@@ -101,9 +101,10 @@ struct PiclDev {
   // axis (0 elsewhere); cbTop = the largest centroid bin coordinate
   double gShift[3] = {0, 0, 0}, cbTop[3] = {0, 0, 0};
   int gImg = 0;  // any periodic axis
-  // collisions: a hash of cubes of side colH >= the largest diameter (one
-  // stage's located particles), bucket lists by colOff / colMem
-  int col = 0, colNb = 0;
+  // collisions: a grid of cubes of side colH >= the largest diameter over
+  // the mesh's bounding box, x fastest, coordinates clamped into it (one
+  // stage's located particles); cell lists by colOff / colMem
+  int col = 0, colNb = 0, colN[3] = {1, 1, 1};
   double colLo[3] = {0, 0, 0}, colHi[3] = {0, 0, 0}, colH = 1, colR = 0;
   int *colCnt = nullptr, *colOff = nullptr, *colFill = nullptr, *colMem = nullptr, *colB = nullptr,
       *colBsum = nullptr;
@@ -586,18 +587,20 @@ __device__ double heat_transfer(int flag, double rkappa, double dp, double tgas,
 }
 
 // ---------------------------------------------------------------------------
-// Collisions (COLLISIONS 1).  The search: every located particle goes into
-// the hash bucket of its cube (side colH >= the largest diameter); a particle
-// scans the buckets of the cubes within colR of itself, and of its images
-// one period away on periodic axes, and keeps the neighbours that overlap it.
+// Collisions (COLLISIONS 1, 2).  The search: every located particle goes into
+// the grid cell of its cube (side colH >= the largest diameter; coordinates
+// clamped into the grid, so a particle a rounding outside the box lands in an
+// edge cell); a particle scans the cells within colR of itself, and of its
+// images one period away on periodic axes, with the same clamping, and keeps
+// the neighbours that overlap it.
 // Their forces are summed in ppiclf_solve_NearestNeighbor's order on one
 // rank (ppiclf_solve.f:470-560): the other particles by ascending index,
 // then the ghost (periodic image) particles by index and image.
 // ---------------------------------------------------------------------------
 #define COL_CAP 32
-__device__ __forceinline__ unsigned col_bucket(const PiclDev &P, int ix, int iy, int iz) {
-  unsigned hsh = (unsigned)ix * 73856093u ^ (unsigned)iy * 19349663u ^ (unsigned)iz * 83492791u;
-  return hsh & (unsigned)(P.colNb - 1);
+__device__ __forceinline__ int col_coord(const PiclDev &P, double x, int l) {
+  double c = floor((x - P.colLo[l]) / P.colH);
+  return (int)fmin(fmax(c, 0.0), (double)(P.colN[l] - 1));
 }
 
 // The stage's location (k_particles repeats it and gets the same element)
@@ -613,9 +616,9 @@ __global__ void __launch_bounds__(BLK) k_col_count(PiclDev P) {
   if (e < 0) { P.colB[t] = -1; return; }
   int c[3];
 #pragma unroll
-  for (int l = 0; l < 3; l++) c[l] = (int)floor((xp[l] - P.colLo[l]) / P.colH);
-  unsigned b = col_bucket(P, c[0], c[1], c[2]);
-  P.colB[t] = (int)b;
+  for (int l = 0; l < 3; l++) c[l] = col_coord(P, xp[l], l);
+  int b = c[0] + P.colN[0] * (c[1] + P.colN[1] * c[2]);
+  P.colB[t] = b;
   atomicAdd(&P.colCnt[b], 1);
 }
 
@@ -714,35 +717,39 @@ __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph)
     int slot[COL_CAP];
     int nk = 0;
     bool over = false;
-    const bool per = Ph.per[0] || Ph.per[1] || Ph.per[2];
-    for (int im = 0; im < 27; im++) {
-      if (im != 13 && !per) continue;
-      double sh[3];
-      bool skip = false;
-      int lo[3], hi[3];
+    // per axis, the shifts whose query box meets the grid (sd = 0 always;
+    // -1, +1 on a periodic axis near a face) with their cell ranges
+    int nsd[3], sdv[3][3], clo[3][3], chi[3][3];
+    double shv[3][3];
 #pragma unroll
-      for (int l = 0, r = im; l < 3; l++, r /= 3) {
-        int sd = r % 3 - 1;
-        if (sd != 0 && !Ph.per[l]) skip = true;
-        sh[l] = sd == 0 ? 0.0 : (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
-        double q = xi[l] - sh[l];
-        if (q + P.colR < P.colLo[l] - P.colR || q - P.colR > P.colHi[l] + P.colR) skip = true;
-        lo[l] = (int)floor((q - P.colR - P.colLo[l]) / P.colH);
-        hi[l] = (int)floor((q + P.colR - P.colLo[l]) / P.colH);
-        if (hi[l] - lo[l] > 2) { skip = true; over = true; }  // cannot happen: 2 colR < 2 colH
+    for (int l = 0; l < 3; l++) {
+      nsd[l] = 0;
+      for (int sd = -1; sd <= 1; sd++) {
+        if (sd != 0 && !Ph.per[l]) continue;
+        double shl = sd == 0 ? 0.0 : (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
+        double q = xi[l] - shl;
+        if (q + P.colR < P.colLo[l] - P.colR || q - P.colR > P.colHi[l] + P.colR) continue;
+        int a = col_coord(P, q - P.colR, l), b = col_coord(P, q + P.colR, l);
+        if (b - a > 2) { over = true; continue; }  // cannot happen: 2 colR < 2 colH
+        sdv[l][nsd[l]] = sd;
+        shv[l][nsd[l]] = shl;
+        clo[l][nsd[l]] = a;
+        chi[l][nsd[l]] = b;
+        nsd[l]++;
       }
-      if (skip) continue;
-      unsigned seen[27];
-      int ns = 0;
+    }
+    for (int iz = 0; iz < nsd[2]; iz++)
+    for (int iy = 0; iy < nsd[1]; iy++)
+    for (int ix = 0; ix < nsd[0]; ix++) {
+      // image index (sx + 1) + 3 (sy + 1) + 9 (sz + 1); 13 is the particle itself
+      int im = (sdv[0][ix] + 1) + 3 * (sdv[1][iy] + 1) + 9 * (sdv[2][iz] + 1);
+      double sh[3] = {shv[0][ix], shv[1][iy], shv[2][iz]};
+      int lo[3] = {clo[0][ix], clo[1][iy], clo[2][iz]}, hi[3] = {chi[0][ix], chi[1][iy], chi[2][iz]};
       for (int cz = lo[2]; cz <= hi[2]; cz++)
-        for (int cy = lo[1]; cy <= hi[1]; cy++)
-          for (int cx = lo[0]; cx <= hi[0]; cx++) {
-            unsigned b = col_bucket(P, cx, cy, cz);
-            bool dup = false;
-            for (int q = 0; q < ns; q++) dup = dup || seen[q] == b;
-            if (dup) continue;
-            seen[ns++] = b;
-            for (int q = P.colOff[b]; q < P.colOff[b + 1]; q++) {
+        for (int cy = lo[1]; cy <= hi[1]; cy++) {
+          int row = P.colN[0] * (cy + P.colN[1] * cz);
+          // the row's cells are contiguous: one span of the member list
+          for (int q = P.colOff[row + lo[0]]; q < P.colOff[row + hi[0] + 1]; q++) {
               int sj = P.colMem[q];
               if (im == 13 && sj == t) continue;
               double xj[3];
@@ -1829,15 +1836,26 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.buf[b].elem, (size_t)nP, minus1.data());
   }
   rc |= dalloc(h, &P.par, 1, &zero);
-  if (P.col) {  // hash of cubes of side just above the largest diameter, >= 2 buckets per particle
+  if (P.col) {  // grid of cubes of side just above the largest diameter over the bounding box
     double dmax = 0.0;
     for (int p = 0; p < nP; p++) dmax = std::max(dmax, pconst[4 * (size_t)p + 1]);
     if (!(dmax > 0.0)) { picl_free(s); return set_err(h, RFLU_GPU_EINVAL, "collisions need positive diameters"); }
     P.colR = dmax * (1.0 + 1.0e-9);
     P.colH = dmax * (1.0 + 1.0e-8);
     for (int l = 0; l < 3; l++) { P.colLo[l] = vlo[l]; P.colHi[l] = vhi[l]; }
-    P.colNb = 1024;
-    while (P.colNb < 2 * nP && P.colNb < (1 << 28)) P.colNb <<= 1;
+    // at most 2^25 cells and 8 per particle: coarser cubes find the same
+    // contacts among more candidates
+    const double cap = std::min((double)(1 << 25), 8.0 * nP + 4096.0);
+    for (;;) {
+      double cells = 1.0;
+      for (int l = 0; l < 3; l++) {
+        P.colN[l] = std::max(1, (int)std::ceil((vhi[l] - vlo[l]) / P.colH));
+        cells *= P.colN[l];
+      }
+      if (cells <= cap) break;
+      P.colH *= 1.25;
+    }
+    P.colNb = P.colN[0] * P.colN[1] * P.colN[2];
     rc |= dalloc(h, &P.colCnt, (size_t)P.colNb + 1);
     rc |= dalloc(h, &P.colOff, (size_t)P.colNb + 1);
     rc |= dalloc(h, &P.colFill, (size_t)P.colNb);
