@@ -105,7 +105,7 @@ struct PiclDev {
   // the mesh's bounding box, x fastest, coordinates clamped into it (one
   // stage's located particles); cell lists by colOff / colMem
   int col = 0, colNb = 0, colN[3] = {1, 1, 1};
-  double colLo[3] = {0, 0, 0}, colHi[3] = {0, 0, 0}, colH = 1, colR = 0;
+  double colLo[3] = {0, 0, 0}, colHi[3] = {0, 0, 0}, colH = 1, colR = 0, colDmax = 0;
   int *colCnt = nullptr, *colOff = nullptr, *colFill = nullptr, *colMem = nullptr, *colB = nullptr,
       *colBsum = nullptr;
   double *fc = nullptr;  // [3][nP] collision force by current slot
@@ -588,11 +588,12 @@ __device__ double heat_transfer(int flag, double rkappa, double dp, double tgas,
 
 // ---------------------------------------------------------------------------
 // Collisions (COLLISIONS 1, 2).  The search: every located particle goes into
-// the grid cell of its cube (side colH >= the largest diameter; coordinates
-// clamped into the grid, so a particle a rounding outside the box lands in an
-// edge cell); a particle scans the cells within colR of itself, and of its
-// images one period away on periodic axes, with the same clamping, and keeps
-// the neighbours that overlap it.
+// the grid cell of its cube (side colH, a third of the largest diameter or
+// more; coordinates clamped into the grid, so a particle a rounding outside
+// the box lands in an edge cell); a particle of diameter d scans the cells
+// within (d + dmax) / 2 of itself (no neighbour it touches lies further), and
+// of its images one period away on periodic axes, with the same clamping,
+// and keeps the neighbours that overlap it.
 // Their forces are summed in ppiclf_solve_NearestNeighbor's order on one
 // rank (ppiclf_solve.f:470-560): the other particles by ascending index,
 // then the ghost (periodic image) particles by index and image.
@@ -712,6 +713,7 @@ __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph)
   if (C.elem[t] >= 0) {
     double xi[3] = {st[t], st[nP + t], st[2 * nP + t]};
     double dpi = C.pc[nP + t];
+    const double Ri = 0.5 * (dpi + P.colDmax) * (1.0 + 1.0e-9);
     // overlapping neighbours: key = (ghost << 62) | (index << 5) | image
     long long key[COL_CAP];
     int slot[COL_CAP];
@@ -728,9 +730,8 @@ __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph)
         if (sd != 0 && !Ph.per[l]) continue;
         double shl = sd == 0 ? 0.0 : (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
         double q = xi[l] - shl;
-        if (q + P.colR < P.colLo[l] - P.colR || q - P.colR > P.colHi[l] + P.colR) continue;
-        int a = col_coord(P, q - P.colR, l), b = col_coord(P, q + P.colR, l);
-        if (b - a > 2) { over = true; continue; }  // cannot happen: 2 colR < 2 colH
+        if (q + Ri < P.colLo[l] - Ri || q - Ri > P.colHi[l] + Ri) continue;
+        int a = col_coord(P, q - Ri, l), b = col_coord(P, q + Ri, l);
         sdv[l][nsd[l]] = sd;
         shv[l][nsd[l]] = shl;
         clo[l][nsd[l]] = a;
@@ -1841,10 +1842,11 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     for (int p = 0; p < nP; p++) dmax = std::max(dmax, pconst[4 * (size_t)p + 1]);
     if (!(dmax > 0.0)) { picl_free(s); return set_err(h, RFLU_GPU_EINVAL, "collisions need positive diameters"); }
     P.colR = dmax * (1.0 + 1.0e-9);
-    P.colH = dmax * (1.0 + 1.0e-8);
+    P.colDmax = dmax;
+    P.colH = dmax / 3.0 * (1.0 + 1.0e-8);
     for (int l = 0; l < 3; l++) { P.colLo[l] = vlo[l]; P.colHi[l] = vhi[l]; }
     // at most 2^25 cells and 8 per particle: coarser cubes find the same
-    // contacts among more candidates
+    // contacts among more candidates (the span loops take any width)
     const double cap = std::min((double)(1 << 25), 8.0 * nP + 4096.0);
     for (;;) {
       double cells = 1.0;
