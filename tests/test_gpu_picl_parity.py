@@ -414,3 +414,19 @@ def test_coupled_steps_with_tangential_collisions(oracle_mod):
         assert ok, (k, err)
     ok, err = close(g.get_state()[0], o.cv())
     assert ok, err
+
+
+def test_collision_list_overflow_is_reported():
+    """Forty particles packed within a tenth of a diameter of one point each
+    overlap 39 neighbours, more than the 32 a particle's contact list holds:
+    the kernel finishes normally and the stage reports the overflow."""
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=40, seed=4)
+    y = y.copy()
+    rng = np.random.default_rng(2)
+    y[:, :3] = y[0, :3] + 0.1 * pc[:, 1:2].min() * rng.uniform(-1.0, 1.0, (40, 3))
+    picl.update(collisionalFlag=1, ksp=10.0, erest=0.7)
+    g = rflu.Region(mesh, **params)
+    g.set_state(ic(g.mesh_arrays()["cofg"]))
+    g.picl_init(y, pc, **picl)
+    with pytest.raises(Exception, match="32 neighbours"):
+        g.RFLU_TimeStepping(1, use_graph=False)
