@@ -135,21 +135,23 @@ def build_workload(name):
                                               "4M particles", n=None, particles=parts, picl=picl)
     if name == "curtain_deck":
         # the shock-particle-curtain with the particle physics of the AirShock
-        # input deck (input_files/AirShock/rectshktb.inp:100-116) as far as it
-        # is built: viscous-unsteady force, collisions with tangential force
-        # and torques (the deck's level 4 also adds hydrodynamic torque and
-        # lift), KSP 10, EREST 0.7, periodic y and z
+        # input deck (input_files/AirShock/rectshktb.inp:97-116) as far as it
+        # is built: Osnes quasi-steady drag, pressure gradient, viscous-unsteady
+        # force, collisions with tangential force and torques (the deck's
+        # level 4 also adds hydrodynamic torque and lift), KSP 10, EREST 0.7,
+        # periodic y and z; added mass Parmar (the deck's Briney model and
+        # its QS fluctuations are not built)
         mesh, params, ic, parts, picl = configs.curtain(limiterGuard=1)
         picl.update(CURTAIN_DECK)
-        return (mesh, params, ic), dict(model="RFLU 2nd-order AUSM+ + ppiclF (IDW-27, QS/AM Parmar + PG + viscous "
-                                              "unsteady + collisions level 2, periodic y/z, box projection), "
+        return (mesh, params, ic), dict(model="RFLU 2nd-order AUSM+ + ppiclF (IDW-27, QS Osnes, AM Parmar, PG, "
+                                              "viscous unsteady, collisions level 2, periodic y/z, box projection), "
                                               "400x80x80 hex, 4M particles", n=None, particles=parts, picl=picl)
     raise SystemExit(f"unknown workload {name}")
 
 
 # h = 0.0005 is the curtain's spacing
 CURTAIN_SPEC = dict(interpMode=1, qsFlag=5, projMode=1, filterWidth=2 * 0.0005, filterAlpha=1e-2)
-CURTAIN_DECK = dict(viscousUnsteady=1, collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
+CURTAIN_DECK = dict(qsFlag=2, viscousUnsteady=1, collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
 
 
 def cpu_baseline(name, budget_s=12.0):
