@@ -430,3 +430,17 @@ def test_collision_list_overflow_is_reported():
     g.picl_init(y, pc, **picl)
     with pytest.raises(Exception, match="32 neighbours"):
         g.RFLU_TimeStepping(1, use_graph=False)
+
+
+@pytest.mark.parametrize("bad", [dict(collisionalFlag=3), dict(collisionalFlag=4), dict(viscousUnsteady=2),
+                                 dict(collisionalFlag=1, ksp=0.0), dict(collisionalFlag=1, erest=0.0),
+                                 dict(collisionalFlag=1, erest=1.5), dict(periodicY=2), dict(heatTransfer=5)])
+def test_unbuilt_or_invalid_particle_physics_is_rejected(bad):
+    """Options outside what is built (the hydrodynamic torque and lift of
+    COLLISIONS 3-4) or outside their range fail at set-up with an error."""
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=8, ny=4, nz=4, npart=50, seed=4)
+    picl.update(bad)
+    g = rflu.Region(mesh, **params)
+    g.set_state(ic(g.mesh_arrays()["cofg"]))
+    with pytest.raises(Exception):
+        g.picl_init(y, pc, **picl)
