@@ -1560,6 +1560,7 @@ struct PiclState {
 namespace {
 
 void picl_free(PiclState *s) {
+  API_LOCK;
   if (!s) return;
   PiclDev &P = s->P;
   void *ptrs[] = {P.buf[0].st, P.buf[0].pc, P.buf[0].pid, P.buf[0].elem, P.buf[1].st, P.buf[1].pc, P.buf[1].pid,
@@ -1686,6 +1687,7 @@ void build_bins(const std::vector<double> &blo, const std::vector<double> &bhi, 
 extern "C" {
 
 int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double *y, const double *pconst) {
+  API_LOCK;
   if (!h || !pp || nPart < 0 || (nPart > 0 && (!y || !pconst))) return RFLU_GPU_EINVAL;
   if (h->picl) return set_err(h, RFLU_GPU_EINVAL, "particles are already attached to this region");
   const rflu::MeshHost &m = h->mesh;
@@ -1994,6 +1996,7 @@ int picl_gpu_runge(rflu_handle *h, int istage) {
 }
 
 int picl_gpu_sizes(const rflu_handle *h, int *nPart, int *nLost) {
+  API_LOCK;
   if (!h || !h->picl) return RFLU_GPU_EINVAL;
   const PiclDev &P = h->picl->P;
   int act = 0;
@@ -2005,6 +2008,7 @@ int picl_gpu_sizes(const rflu_handle *h, int *nPart, int *nLost) {
 }
 
 int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, double *ydotc, int *elem) {
+  API_LOCK;
   if (!h || !h->picl) return RFLU_GPU_EINVAL;
   PiclState *s = h->picl;
   PiclDev &P = s->P;
@@ -2054,6 +2058,7 @@ int picl_gpu_get_particles(rflu_handle *h, double *y, double *y1, double *ydot, 
 }
 
 int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *proFld, double *fields) {
+  API_LOCK;
   if (!h || !h->picl) return RFLU_GPU_EINVAL;
   PiclDev &P = h->picl->P;
   size_t N = P.nC;
@@ -2084,6 +2089,7 @@ int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *pr
 }
 
 int picl_gpu_get_nodes(rflu_handle *h, double *proNode) {
+  API_LOCK;
   if (!h || !h->picl || !proNode) return RFLU_GPU_EINVAL;
   PiclDev &P = h->picl->P;
   size_t N = P.nC;
@@ -2105,6 +2111,7 @@ int picl_gpu_get_nodes(rflu_handle *h, double *proNode) {
 }
 
 int picl_gpu_set_debug(rflu_handle *h, int on) {
+  API_LOCK;
   if (!h || !h->picl) return RFLU_GPU_EINVAL;
   PiclDev &P = h->picl->P;
   if (on && !P.dbg) {
@@ -2119,6 +2126,7 @@ int picl_gpu_set_debug(rflu_handle *h, int on) {
 }
 
 int picl_gpu_get_interp(rflu_handle *h, double *rprop20) {
+  API_LOCK;
   if (!h || !h->picl || !rprop20) return RFLU_GPU_EINVAL;
   PiclDev &P = h->picl->P;
   if (!P.dbg) return set_err(h, RFLU_GPU_EINVAL, "interpolated values are kept only with picl_gpu_set_debug(h, 1)");
@@ -2132,6 +2140,7 @@ int picl_gpu_get_interp(rflu_handle *h, double *rprop20) {
 }
 
 int picl_gpu_locate(rflu_handle *h, int n, const double *xyz, int *elem) {
+  API_LOCK;
   if (!h || !h->picl || n < 0 || (n > 0 && (!xyz || !elem))) return RFLU_GPU_EINVAL;
   if (n == 0) return 0;
   double *dx = nullptr;

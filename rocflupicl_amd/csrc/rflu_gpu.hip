@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -122,7 +123,19 @@ struct rflu_handle {
   std::vector<int> instCell, instFace, instBFace;  // batch: per-instance offsets in the union (reference numbering)
 };
 
-static std::string g_err;
+// Text of the last error of a call that had no handle (creation, memory
+// queries), per calling thread: concurrent set-ups never share it.
+static thread_local std::string g_err;
+
+// One process-wide lock orders the runtime calls that may not overlap a step
+// graph's capture on another thread: allocation and freeing, copies and
+// memsets, stream, event and graph creation or destruction, and every
+// synchronisation.  capture_step holds it from hipStreamBeginCapture to the
+// instantiation.  Kernel launches and graph replays do not take it, so the
+// hot path never waits on it.  Recursive: a locked entry point may call
+// another (picl_gpu_init allocates through dalloc).
+static std::recursive_mutex g_api;
+#define API_LOCK std::lock_guard<std::recursive_mutex> api_lock_(g_api)
 
 #define HIPCHK(h, x)                                                                \
   do {                                                                              \
@@ -2168,12 +2181,13 @@ int set_err(rflu_handle *h, int code, const std::string &m) {
 
 template <class T>
 int dalloc(rflu_handle *h, T **p, size_t n, const T *src = nullptr) {
+  API_LOCK;
   if (n == 0) n = 1;
   HIPCHK(h, hipMalloc((void **)p, n * sizeof(T)));
   h->devBytes += n * sizeof(T);
-  if (src) {  // the copy runs on the null stream, which the handle's non-blocking stream does not wait for
-    HIPCHK(h, hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
-    HIPCHK(h, hipStreamSynchronize(nullptr));
+  if (src) {  // on the handle's own stream: nothing here touches the null stream
+    HIPCHK(h, hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return 0;
 }
@@ -2182,6 +2196,7 @@ inline int nblk(int n) { return (n + BLK - 1) / BLK; }
 
 hipEvent_t ev_get(rflu_handle *h) {
   if (!h->evPool.empty()) { hipEvent_t e = h->evPool.back(); h->evPool.pop_back(); return e; }
+  API_LOCK;
   hipEvent_t e;
   hipEventCreate(&e);
   return e;
@@ -2198,6 +2213,7 @@ struct ProfScope {
 };
 
 void prof_collect(rflu_handle *h) {
+  API_LOCK;
   for (auto &p : h->pending) {
     float ms = 0.f;
     hipEventSynchronize(p.second.second);
@@ -2362,6 +2378,7 @@ int enqueue_step(rflu_handle *h) {
 }
 
 int check_err(rflu_handle *h) {
+  API_LOCK;
   int e = 0;
   HIPCHK(h, hipMemcpyAsync(&e, h->d.err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2463,6 +2480,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   h->prm = *prm;
   if (const char *gv = getenv("ROCFLUPICL_GRAD_VARIANT")) h->gradVariant = atoi(gv);
   if (getenv("ROCFLUPICL_NO_XCD_REMAP")) {
+    API_LOCK;
     int zero = 0;
     (void)hipMemcpyToSymbol(HIP_SYMBOL(g_xcdRemap), &zero, sizeof(int));
   }
@@ -2487,9 +2505,12 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   d.nC = m.nCells; d.nF = m.nFaces; d.nB = m.nBFaces; d.S = m.nFaces + m.nBFaces; d.nBlk = nblk(d.nC);
   // non-blocking: work of other handles (other requests in flight) and
   // null-stream copies do not serialise against this handle's stream
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete h;
-    return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed");
+  {
+    API_LOCK;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete h;
+      return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed");
+    }
   }
   int rc = 0;
   // Device numbering: cells and faces are renumbered along a Morton curve of
@@ -2689,6 +2710,7 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   d.cfdy = d.cfdx + cfIdxD.size(); d.cfdz = d.cfdx + 2 * cfIdxD.size();
   d.fnx = faceSoa; d.fny = faceSoa + S; d.fnz = faceSoa + 2 * (size_t)S; d.fnm = faceSoa + 3 * (size_t)S;
   d.fcx = faceSoa + 4 * (size_t)S; d.fcy = faceSoa + 5 * (size_t)S; d.fcz = faceSoa + 6 * (size_t)S;
+  API_LOCK;
   (void)hipMemsetAsync(d.scal, 0, 8 * sizeof(double), h->stream);
   (void)hipMemsetAsync(d.err, 0, sizeof(int), h->stream);
   (void)hipMemsetAsync(d.rhsSum, 0, 5 * (size_t)N * sizeof(double), h->stream);
@@ -2746,6 +2768,7 @@ int rflu_batch_sizes(const rflu_handle *h, int inst, int *nCells, int *nInst) {
 }
 
 int rflu_batch_set_state(rflu_handle *h, int inst, const double *cv) {
+  API_LOCK;
   if (!h || !cv || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
   const int off = h->instCell[inst], nk = h->instCell[inst + 1] - off, N = h->d.nC;
   std::vector<double> tmp(5 * (size_t)nk);
@@ -2763,6 +2786,7 @@ int rflu_batch_set_state(rflu_handle *h, int inst, const double *cv) {
 }
 
 int rflu_batch_get_state(rflu_handle *h, int inst, double *cv, double *dv) {
+  API_LOCK;
   if (!h || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
   const int off = h->instCell[inst], nk = h->instCell[inst + 1] - off, N = h->d.nC;
   std::vector<double> tmp(5 * (size_t)nk);
@@ -2789,6 +2813,7 @@ int rflu_batch_get_state(rflu_handle *h, int inst, double *cv, double *dv) {
 }
 
 int rflu_batch_get_time(rflu_handle *h, int inst, double *t, double *dtMin) {
+  API_LOCK;
   if (!h || h->d.nInst == 0 || inst < 0 || inst >= h->d.nInst) return RFLU_GPU_EINVAL;
   double sc[4];
   HIPCHK(h, hipMemcpyAsync(sc, h->d.iScal + 4 * (size_t)inst, sizeof sc, hipMemcpyDeviceToHost, h->stream));
@@ -2799,6 +2824,7 @@ int rflu_batch_get_time(rflu_handle *h, int inst, double *t, double *dtMin) {
 }
 
 int rflu_gpu_destroy(rflu_handle *h) {
+  API_LOCK;
   if (!h) return 0;
   Dev &d = h->d;
   void *ptrs[] = {d.f2c, d.bf2c, d.bkind, d.bkopt, d.bpatch, d.pcon, d.bpout, d.fnx, d.cfOff, d.cfIdx, d.cfNInt, d.cfdx, d.vol, d.epsq, d.cofg, d.csOff,
@@ -2873,6 +2899,7 @@ static int from_device_cells(rflu_handle *h, double *dst, const double *src, int
 }
 
 int rflu_gpu_set_state(rflu_handle *h, const double *cv) {
+  API_LOCK;
   if (!h || !cv) return RFLU_GPU_EINVAL;
   if (int e = to_device_cells(h, h->d.cv, cv, 5)) return e;
   HIPCHK(h, hipMemsetAsync(h->d.err, 0, sizeof(int), h->stream));
@@ -2883,6 +2910,7 @@ int rflu_gpu_set_state(rflu_handle *h, const double *cv) {
 }
 
 int rflu_gpu_get_state(rflu_handle *h, double *cv, double *dv) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   if (cv)
     if (int e = from_device_cells(h, cv, h->d.cv, 5, false)) return e;
@@ -2894,11 +2922,13 @@ int rflu_gpu_get_state(rflu_handle *h, double *cv, double *dv) {
 }
 
 int rflu_gpu_get_rhs(rflu_handle *h, double *rhs) {
+  API_LOCK;
   if (!h || !rhs) return RFLU_GPU_EINVAL;
   return from_device_cells(h, rhs, h->d.rhs, 5, false);
 }
 
 int rflu_gpu_get_grad(rflu_handle *h, double *grad) {
+  API_LOCK;
   if (!h || !grad) return RFLU_GPU_EINVAL;
   if (h->prm.spaceOrder < 2) return set_err(h, RFLU_GPU_EINVAL, "no gradients at spaceOrder 1");
   if (h->prm.fluxPrecision == 1) {  // FP32 gradients, widened
@@ -2914,6 +2944,7 @@ int rflu_gpu_get_grad(rflu_handle *h, double *grad) {
 }
 
 int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLocal) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "a batch steps with rflu_gpu_step (per-instance dt)");
   {
@@ -2930,6 +2961,7 @@ int rflu_gpu_time_step_inviscid(rflu_handle *h, double *dtCell, double *dtMinLoc
 }
 
 int rflu_gpu_minimum_time_step(rflu_handle *h, double *dtMin) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "a batch steps with rflu_gpu_step (per-instance dt)");
   hipLaunchKernelGGL(k_min_time_step, dim3(1), dim3(64), 0, h->stream, h->d, dtpar(h, 1));
@@ -2990,6 +3022,7 @@ int rflu_gpu_rk_stage(rflu_handle *h, int istage) {
 }
 
 static int capture_step(rflu_handle *h) {
+  API_LOCK;
   if (h->graph) return 0;
   hipGraph_t g;
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -3008,6 +3041,7 @@ int rflu_gpu_step_async(rflu_handle *h, int nSteps) {
 }
 
 int rflu_gpu_query(rflu_handle *h) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   hipError_t e = hipStreamQuery(h->stream);
   if (e == hipSuccess) return 1;
@@ -3028,6 +3062,7 @@ int rflu_gpu_device_bytes(const rflu_handle *h, size_t *bytes) {
 }
 
 int rflu_gpu_mem_info(size_t *freeBytes, size_t *totalBytes) {
+  API_LOCK;
   size_t f = 0, t = 0;
   if (hipMemGetInfo(&f, &t) != hipSuccess) return set_err(nullptr, RFLU_GPU_EHIP, "hipMemGetInfo failed");
   if (freeBytes) *freeBytes = f;
@@ -3050,6 +3085,7 @@ int rflu_gpu_step(rflu_handle *h, int nSteps, int useGraph) {
 }
 
 int rflu_gpu_set_time(rflu_handle *h, double t) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   if (h->d.nInst) return set_err(h, RFLU_GPU_EINVAL, "set the time of a batch per instance (not supported)");
   double s[8];
@@ -3062,6 +3098,7 @@ int rflu_gpu_set_time(rflu_handle *h, double t) {
 }
 
 int rflu_gpu_get_time(rflu_handle *h, double *t, double *dtMin) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   double s[8];
   HIPCHK(h, hipMemcpyAsync(s, h->d.scal, sizeof s, hipMemcpyDeviceToHost, h->stream));
@@ -3072,12 +3109,14 @@ int rflu_gpu_get_time(rflu_handle *h, double *t, double *dtMin) {
 }
 
 int rflu_gpu_sync(rflu_handle *h) {
+  API_LOCK;
   if (!h) return RFLU_GPU_EINVAL;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 
 int rflu_gpu_bandwidth_probe(size_t bytes, int reps, double *gbps) {
+  API_LOCK;
   if (bytes < 16 || reps < 1 || !gbps) return set_err(nullptr, RFLU_GPU_EINVAL, "bad probe arguments");
   size_t n = bytes / 16;
   double2 *a = nullptr, *b = nullptr;
@@ -3106,6 +3145,7 @@ int rflu_gpu_bandwidth_probe(size_t bytes, int reps, double *gbps) {
 }
 
 int rflu_gpu_autotune(rflu_handle *h, int reps, int *variant, double *msPerLaunch) {
+  API_LOCK;
   if (!h || reps < 1) return RFLU_GPU_EINVAL;
   if (variant) *variant = h->gradVariant;
   if (msPerLaunch) *msPerLaunch = 0.0;

@@ -18,7 +18,10 @@ state, optionally a particle cloud.  The server
   is skipped, not waited for;
 * on completion checks AfterUpdateMP's conditions, hands the final state to
   ``on_done`` and frees the request's device memory at once, which lets the
-  next request in.
+  next request in;
+* isolates failures: a request that cannot be set up, or whose steps fail
+  (a launch error, NaN or negative pressure), is closed and reported in its
+  own Result.error while the others keep running.
 
 Each request's result is independent of the mix it ran in: regions share no
 device state, and every kernel is deterministic, so a request gives the same
@@ -128,21 +131,22 @@ class Server:
                 progressed = False
                 for slot in list(active):
                     req, reg, left, started, need, adm = slot
-                    if not reg.idle():
-                        continue
-                    progressed = True
-                    if left > 0:
-                        n = min(self.chunk, left)
-                        reg.step_async(n)
-                        slot[2] = left - n
-                        continue
                     err, cv = None, None
                     try:
+                        if not reg.idle():
+                            continue
+                        progressed = True
+                        if left > 0:
+                            n = min(self.chunk, left)
+                            reg.step_async(n)
+                            slot[2] = left - n
+                            continue
                         reg.wait()
                         if self.keep_state or self.on_done:
                             cv = reg.get_state()[0]
-                    except Exception as e:
-                        err = str(e)
+                    except Exception as e:  # a request whose steps fail ends alone; the others go on
+                        progressed = True
+                        err, cv = str(e), None
                     done = time.perf_counter() - t0
                     res = Result(req.rid, req.cells, req.particles, req.steps, req.arrival, adm, started, done,
                                  cv if self.keep_state else None, err)
@@ -158,6 +162,9 @@ class Server:
             for slot in active:
                 slot[1].close()
             pool.shutdown(wait=True)
+            for fut in building:  # set-ups still in flight when a run is cut short
+                if fut.done() and fut.exception() is None:
+                    fut.result().close()
         wall = time.perf_counter() - t0
         return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)], wall
 
