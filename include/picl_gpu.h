@@ -125,6 +125,16 @@ int picl_gpu_set_debug(rflu_handle *h, int on);
  * rule (lowest-numbered element whose trilinear map contains the point). */
 int picl_gpu_locate(rflu_handle *h, int n, const double *xyz, int *elem);
 
+/* picl_gpu_locate without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The points
+ * and answers are staged in a static device array that each GPU holds until
+ * the process ends, set up by picl_gpu_init; n may not exceed
+ * PICL_LOCATE_STATIC_MAX (RFLU_GPU_EINVAL otherwise).  Calls that overlap on
+ * one GPU share that array and race. */
+#define PICL_LOCATE_STATIC_MAX 262144
+int picl_gpu_locate_static(rflu_handle *h, int n, const double *xyz, int *elem);
+
 #ifdef __cplusplus
 }
 #endif

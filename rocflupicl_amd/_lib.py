@@ -30,7 +30,11 @@ RFLU_SYMBOLS = [
 PICL_SYMBOLS = [
     "picl_gpu_init", "picl_gpu_runge", "picl_gpu_sizes", "picl_gpu_get_particles", "picl_gpu_get_cells",
     "picl_gpu_get_interp", "picl_gpu_set_debug", "picl_gpu_locate", "picl_gpu_get_nodes",
+    "picl_gpu_locate_static",
 ]
+
+# picl_gpu.h: PICL_LOCATE_STATIC_MAX, the largest n of picl_gpu_locate_static
+PICL_LOCATE_STATIC_MAX = 262144
 
 ERRORS = {1: "EINVAL", 2: "ENODEV", 3: "EHIP", 4: "EMESH", 5: "ENAN", 6: "ENEGATIVE"}
 
@@ -120,6 +124,7 @@ def load():
     L.picl_gpu_get_nodes.argtypes = [P, P]
     L.picl_gpu_set_debug.argtypes = [P, I]
     L.picl_gpu_locate.argtypes = [P, I, P, P]
+    L.picl_gpu_locate_static.argtypes = [P, I, P, P]
     for n in RFLU_SYMBOLS + PICL_SYMBOLS:
         if n != "rflu_gpu_last_error":
             getattr(L, n).restype = I

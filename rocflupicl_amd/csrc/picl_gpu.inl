@@ -1684,6 +1684,31 @@ void build_bins(const std::vector<double> &blo, const std::vector<double> &bhi, 
 
 }  // namespace
 
+// Static staging for picl_gpu_locate_static: PICL_LOCATE_STATIC_MAX points
+// (3 doubles each) and as many element ids.  Held by every device the library
+// runs on until the process ends.
+#define SCRATCH_ELEM_OFF ((size_t)PICL_LOCATE_STATIC_MAX * 3 * sizeof(double))
+#define SCRATCH_BYTES (SCRATCH_ELEM_OFF + (size_t)PICL_LOCATE_STATIC_MAX * sizeof(int))
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+#define SCRATCH_MAXDEV 64
+static unsigned char *g_scratchDev[SCRATCH_MAXDEV];  // each device's address of `scratch`
+
+// Resolve this device's copy of `scratch` once, with the device current.
+static int picl_scratch_init(rflu_handle *h) {
+  if (h->device < 0 || h->device >= SCRATCH_MAXDEV) return set_err(h, RFLU_GPU_EINVAL, "device ordinal out of range");
+  if (g_scratchDev[h->device]) return 0;
+  int cur = 0;
+  HIPCHK(h, hipGetDevice(&cur));
+  if (cur != h->device) HIPCHK(h, hipSetDevice(h->device));
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != h->device) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !ptr)
+    return set_err(h, RFLU_GPU_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratchDev[h->device] = static_cast<unsigned char *>(ptr);
+  return 0;
+}
+
 extern "C" {
 
 int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double *y, const double *pconst) {
@@ -1732,6 +1757,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     if (perFlag[l] && !(vlo[l] < vhi[l]))
       return set_err(h, RFLU_GPU_EINVAL, "a periodic axis needs a mesh of positive extent along it");
   }
+  if (int e = picl_scratch_init(h)) return e;  // the static staging of picl_gpu_locate_static
   const int N = h->d.nC, nP = nPart;
   PiclState *s = new PiclState();
   PiclDev &P = s->P;
@@ -2145,8 +2171,10 @@ int picl_gpu_locate(rflu_handle *h, int n, const double *xyz, int *elem) {
   if (n == 0) return 0;
   double *dx = nullptr;
   int *de = nullptr;
+  const size_t before = h->devBytes;
   if (dalloc(h, &dx, 3 * (size_t)n, xyz) || dalloc(h, &de, (size_t)n)) {
     if (dx) (void)hipFree(dx);
+    h->devBytes = before;
     return RFLU_GPU_EHIP;
   }
   hipLaunchKernelGGL(k_locate_pts, dim3(nblk(n)), dim3(BLK), 0, h->stream, h->picl->P, n, dx, de);
@@ -2155,7 +2183,34 @@ int picl_gpu_locate(rflu_handle *h, int n, const double *xyz, int *elem) {
   hipError_t e3 = hipStreamSynchronize(h->stream);
   (void)hipFree(dx);
   (void)hipFree(de);
+  h->devBytes = before;  // the staging buffers live for this call only
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return set_err(h, RFLU_GPU_EHIP, "picl_gpu_locate failed");
+  return 0;
+}
+
+// picl_gpu_locate without allocation: the points and the answers are staged
+// in `scratch` (points at byte 0, answers after PICL_LOCATE_STATIC_MAX points'
+// coordinates).  Each device's copy of the array is found once, by
+// hipGetSymbolAddress with that device current (picl_scratch_init, run by
+// picl_gpu_init); host code uses only that address, never the array's name.
+// Every call on a device shares its copy, so calls that overlap on one device
+// race; the API lock orders calls made through this library.
+int picl_gpu_locate_static(rflu_handle *h, int n, const double *xyz, int *elem) {
+  API_LOCK;
+  if (!h || !h->picl || n < 0 || (n > 0 && (!xyz || !elem))) return RFLU_GPU_EINVAL;
+  if (n > PICL_LOCATE_STATIC_MAX)
+    return set_err(h, RFLU_GPU_EINVAL, "picl_gpu_locate_static: n exceeds PICL_LOCATE_STATIC_MAX (" +
+                                           std::to_string(PICL_LOCATE_STATIC_MAX) + ") points");
+  if (n == 0) return 0;
+  unsigned char *base = g_scratchDev[h->device];
+  if (!base) return set_err(h, RFLU_GPU_EHIP, "picl_gpu_locate_static: the static scratch of this device is not set up");
+  double *dx = reinterpret_cast<double *>(base);
+  int *de = reinterpret_cast<int *>(base + SCRATCH_ELEM_OFF);
+  HIPCHK(h, hipMemcpyAsync(dx, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_locate_pts, dim3(nblk(n)), dim3(BLK), 0, h->stream, h->picl->P, n, (const double *)dx, de);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(elem, de, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 

@@ -105,6 +105,7 @@ struct rflu_handle {
   std::vector<double> xyz;  // the grid as given (the particle set-up reads the hex nodes)
   std::vector<int> c2v;
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at creation; the handle's memory and stream live there
   std::string err;
   hipGraphExec_t graph = nullptr;
   int gradVariant = 4;  // 4: LDS tiles (default when they fit), 2: one thread per cell, 0: plain, 1/3: cooperative
@@ -2507,7 +2508,8 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
   // null-stream copies do not serialise against this handle's stream
   {
     API_LOCK;
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
       delete h;
       return set_err(nullptr, RFLU_GPU_EHIP, "hipStreamCreate failed");
     }

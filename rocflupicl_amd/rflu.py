@@ -251,10 +251,14 @@ class Region:
         self._chk(self._L.picl_gpu_get_interp(self.h, _p(a)))
         return a
 
-    def picl_locate(self, xyz):
+    def picl_locate(self, xyz, static=False):
+        """Element of each point (-1: none).  static=True stages through the
+        library's static device array (picl_gpu_locate_static): no device
+        allocation, at most PICL_LOCATE_STATIC_MAX points."""
         xyz = np.ascontiguousarray(xyz, np.float64)
         out = np.zeros(xyz.shape[0], np.int32)
-        self._chk(self._L.picl_gpu_locate(self.h, xyz.shape[0], _p(xyz), _p(out)))
+        f = self._L.picl_gpu_locate_static if static else self._L.picl_gpu_locate
+        self._chk(f(self.h, xyz.shape[0], _p(xyz), _p(out)))
         return out
 
     def time(self):
