@@ -338,6 +338,17 @@ static void init_solve(struct orc_picl *p, int interp) {
      * The sums are compensated (Neumaier), so that their value does not
      * depend on the order of the terms (ppiclF's order over its bins and
      * ghost particles is not restated); here: ascending particle index. */
+    /* the images that exist: shift -1, 0, +1 periods on periodic axes,
+     * 0 elsewhere, in ascending image order (img = sx+1 + 3(sy+1) + 9(sz+1)) */
+    int nimg = 0, imgSd[27][3];
+    for (int img = 0; img < 27; img++) {
+      int skip = 0;
+      for (int d = 0, q = img; d < 3; d++, q /= 3) {
+        imgSd[nimg][d] = q % 3 - 1;
+        if (imgSd[nimg][d] != 0 && !p->per[d]) skip = 1;
+      }
+      if (!skip) nimg++;
+    }
     for (int e = 0; e < p->ne; e++)
       for (int n = 0; n < 8; n++) {
         const double *xn = p->xm1b + 24 * (size_t)e + 3 * n;
@@ -346,15 +357,12 @@ static void init_solve(struct orc_picl *p, int interp) {
         for (int ip = 0; ip < p->n; ip++) {
           if (p->lost[ip]) continue;
           const double *y = p->y + LRS * (size_t)ip;
-          for (int img = 0; img < 27; img++) {
+          for (int im = 0; im < nimg; im++) {
             double xp[3];
-            int skip = 0;
-            for (int d = 0, q = img; d < 3; d++, q /= 3) {
-              int sd = q % 3 - 1;
-              if (sd != 0 && !p->per[d]) skip = 1;
+            for (int d = 0; d < 3; d++) {
+              int sd = imgSd[im][d];
               xp[d] = sd == 0 ? y[d] : y[d] + (double)sd * (p->perHi[d] - p->perLo[d]);
             }
-            if (skip) continue;
             double rdist2 = (xn[0] - xp[0]) * (xn[0] - xp[0]) + (xn[1] - xp[1]) * (xn[1] - xp[1]);
             rdist2 = rdist2 + (xn[2] - xp[2]) * (xn[2] - xp[2]);
             if (rdist2 > p->gD2chk) continue;

@@ -14,7 +14,9 @@
   x = 0.02; the particle curtain lives in picl_configs.
 
 Each builder takes a scale argument so tests can run the same workload at
-sizes the CPU oracle finishes in seconds.
+sizes the CPU oracle finishes in seconds, and ``bc`` (one
+(bcType, bcOpt, pout) per box side x-, x+, y-, y+, z-, z+) to replace the
+default outflow ends and slip-wall sides, e.g. by a closed box.
 """
 import numpy as np
 
@@ -43,8 +45,8 @@ def sod(nx=200, ny=4, nz=4, jitter=0.1, seed=1, discr=4, order=1, **kw):
     return mesh, params, ic
 
 
-def shock_tets(n=64, jitter=0.05, seed=2, discr=1, mach=1.5, order=2, gamma=1.4, **kw):
-    mesh = mg.tet_box(n, n, n, 1.0, 1.0, 1.0, jitter=jitter, seed=seed)
+def shock_tets(n=64, jitter=0.05, seed=2, discr=1, mach=1.5, order=2, gamma=1.4, bc=None, **kw):
+    mesh = mg.tet_box(n, n, n, 1.0, 1.0, 1.0, jitter=jitter, seed=seed, bc=bc)
     params = dict(spaceDiscr=discr, spaceOrder=order, reconst=11 if order == 2 else 0,
                   cfl=0.5, dtImposed=1.0e-7, dtMinLimit=0.0, epsentr=0.05, **AIR)
     params.update(refGamma=gamma)
@@ -92,7 +94,7 @@ PICL_AIR = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840
 
 def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=None, x_curtain=None,
             phi=0.2, d_median=115e-6, sigma_ln=0.2, rho_p=2500.0, discr=4, jitter=0.0, seed=4,
-            nparts_cap=None, **kw):
+            nparts_cap=None, bc=None, **kw):
     """Shock-particle-curtain: hex box nx*ny*nz of spacing h, Mach `mach`
     shock at x_shock moving +x into quiescent air, and a slab of `npart`
     particles (lognormal diameters, uniform positions) whose thickness gives
@@ -102,7 +104,7 @@ def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=N
     (y[n,12], pconst[n,4] = (rho_p, d_p, V_p, SPL)).  ic(cofg) gives the
     pure-gas state; the PICL set-up scales it by (1 - phi).
     """
-    mesh = mg.hex_box(nx, ny, nz, nx * h, ny * h, nz * h, jitter=jitter, seed=seed)
+    mesh = mg.hex_box(nx, ny, nz, nx * h, ny * h, nz * h, jitter=jitter, seed=seed, bc=bc)
     params = dict(spaceDiscr=discr, spaceOrder=2, reconst=11, cfl=0.5, dtImposed=1.0e-8,
                   dtMinLimit=0.0, **AIR)
     params.update(kw)
