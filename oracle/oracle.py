@@ -113,8 +113,33 @@ def lib():
         D = C.c_double
         L.orc_picl_collision_force.argtypes = [P, P, D, D, D, P, P, D, D, D, D, D, D, P]
         L.orc_picl_collision_force2.argtypes = [P, P, D, D, D, D, D, D, D, D, D, P, P]
+        L.orc_test_rk3_linear.argtypes = [D, D, P]
+        L.orc_test_entropy_fix.argtypes = [D, D]
+        L.orc_test_entropy_fix.restype = D
+        L.orc_test_venkat_term.argtypes = [D, D, D, I]
+        L.orc_test_venkat_term.restype = D
         _LIB = L
     return _LIB
+
+
+def rk3_linear(lam, y0=1.0):
+    """The oracle's Wray RK3 update on y' = lam y with dt = 1: the values
+    after stages 1, 2 and 3."""
+    y = np.zeros(3)
+    lib().orc_test_rk3_linear(float(lam), float(y0), _p(y))
+    return y
+
+
+def entropy_fix(l, d):
+    """The oracle's Harten-Hyman entropy fix of eigenvalue magnitude l with
+    threshold d."""
+    return lib().orc_test_entropy_fix(float(l), float(d))
+
+
+def venkat_term(dm, d2, epsq, guard=0):
+    """The oracle's Venkatakrishnan term of one face: dm = neighbour extreme
+    minus the cell value, d2 = extrapolation to the face, epsq = eps^2."""
+    return lib().orc_test_venkat_term(float(dm), float(d2), float(epsq), int(guard))
 
 
 def _p(a):

@@ -645,12 +645,21 @@ static void gradients(orc_region *r) {
   }
 }
 
+/* One face's Venkatakrishnan term, modflu/RFLU_ModLimiters.F90:459-505:
+ * dm the neighbour extreme minus the cell value, d2 the gradient's
+ * extrapolation to the face centroid, epsq = (K vol^(1/3))^3; TINY carries
+ * tinyp for denom >= 0 and tinyn otherwise. */
+static inline double venkat_term(double dm, double d2, double epsq, double tinyp, double tinyn) {
+  double numer = (dm * dm + epsq) * d2 + 2.0 * d2 * d2 * dm;
+  double denom = d2 * (dm * dm + 2.0 * d2 * d2 + dm * d2 + epsq);
+  return numer / (denom + (denom >= 0.0 ? tinyp : tinyn));
+}
+
 /* Venkatakrishnan: modflu/RFLU_ModLimiters.F90:339-532, 707-760 */
 static void limiter_venkat(orc_region *r) {
   int N = r->nCells;
   const double THRD = 1.0 / 3.0, K = 1.0;
   const double TINYP = 1.0e-12, TINYN = r->p.limiterGuard ? -1.0e-12 : 1.0e-12;
-#define TINY (denom >= 0.0 ? TINYP : TINYN)
   double *vmin = malloc(sizeof(double) * 5 * N), *vmax = malloc(sizeof(double) * 5 * N);
   for (int i = 0; i < N; i++)
     for (int k = 0; k < 5; k++) {
@@ -682,33 +691,17 @@ static void limiter_venkat(orc_region *r) {
       const double *g1 = r->grad + 15 * (size_t)c1 + 3 * k, *g2 = r->grad + 15 * (size_t)c2 + 3 * k;
       double d21 = g1[0] * dx1 + g1[1] * dy1 + g1[2] * dz1;
       double d22 = g2[0] * dx2 + g2[1] * dy2 + g2[2] * dz2;
-      double numer, denom, term;
       double *l1 = &r->lim[5 * (size_t)c1 + k], *l2 = &r->lim[5 * (size_t)c2 + k];
-      if (d21 > 0.0) {
-        numer = (d1max1 * d1max1 + epsq1) * d21 + 2.0 * d21 * d21 * d1max1;
-        denom = d21 * (d1max1 * d1max1 + 2.0 * d21 * d21 + d1max1 * d21 + epsq1);
-        term = numer / (denom + TINY);
-        *l1 = fmin(term, *l1);
-      } else if (d21 < 0.0) {
-        numer = (d1min1 * d1min1 + epsq1) * d21 + 2.0 * d21 * d21 * d1min1;
-        denom = d21 * (d1min1 * d1min1 + 2.0 * d21 * d21 + d1min1 * d21 + epsq1);
-        term = numer / (denom + TINY);
-        *l1 = fmin(term, *l1);
-      }
-      if (d22 > 0.0) {
-        numer = (d1max2 * d1max2 + epsq2) * d22 + 2.0 * d22 * d22 * d1max2;
-        denom = d22 * (d1max2 * d1max2 + 2.0 * d22 * d22 + d1max2 * d22 + epsq2);
-        term = numer / (denom + TINY);
-        *l2 = fmin(term, *l2);
-      } else if (d22 < 0.0) {
-        numer = (d1min2 * d1min2 + epsq2) * d22 + 2.0 * d22 * d22 * d1min2;
-        denom = d22 * (d1min2 * d1min2 + 2.0 * d22 * d22 + d1min2 * d22 + epsq2);
-        term = numer / (denom + TINY);
-        *l2 = fmin(term, *l2);
-      }
+      if (d21 > 0.0)
+        *l1 = fmin(venkat_term(d1max1, d21, epsq1, TINYP, TINYN), *l1);
+      else if (d21 < 0.0)
+        *l1 = fmin(venkat_term(d1min1, d21, epsq1, TINYP, TINYN), *l1);
+      if (d22 > 0.0)
+        *l2 = fmin(venkat_term(d1max2, d22, epsq2, TINYP, TINYN), *l2);
+      else if (d22 < 0.0)
+        *l2 = fmin(venkat_term(d1min2, d22, epsq2, TINYP, TINYN), *l2);
     }
   }
-#undef TINY
   for (int i = 0; i < N; i++)
     for (int k = 0; k < 5; k++) {
       double l = r->lim[5 * (size_t)i + k];
@@ -1709,3 +1702,28 @@ void orc_get_rhs(const orc_region *r, double *rhs) { memcpy(rhs, r->rhs, sizeof(
 void orc_get_grad(const orc_region *r, double *g) { memcpy(g, r->grad, sizeof(double) * 15 * r->nCells); }
 double orc_time(const orc_region *r) { return r->currentTime; }
 double orc_last_dt(const orc_region *r) { return r->dtMin; }
+
+/* ------------------------------------------------------------------------ */
+/* Test hooks: the routines above applied to closed-form inputs.            */
+/* ------------------------------------------------------------------------ */
+/* rk_update on y' = lam y for one cell (RKInitMP copies cv to cvOld before
+ * every stage; the residual of dU/dt = lam U is rhs = -lam vol U), dt = 1;
+ * y[k] receives the value after stage k+1. */
+void orc_test_rk3_linear(double lam, double y0, double *y) {
+  orc_region r;
+  memset(&r, 0, sizeof r);
+  double vol = 0.75, cv[5], cvOld[5], rhs[5], rhsSum[5] = {0};
+  for (int k = 0; k < 5; k++) cv[k] = y0;
+  r.nCells = 1; r.vol = &vol; r.cv = cv; r.cvOld = cvOld; r.rhs = rhs; r.rhsSum = rhsSum; r.dtMin = 1.0;
+  for (int st = 1; st <= 3; st++) {
+    for (int k = 0; k < 5; k++) { cvOld[k] = cv[k]; rhs[k] = -lam * vol * cv[k]; }
+    rk_update(&r, st);
+    y[st - 1] = cv[0];
+  }
+}
+
+double orc_test_entropy_fix(double l, double d) { return entropy_fix_hh(l, d); }
+
+double orc_test_venkat_term(double dm, double d2, double epsq, int guard) {
+  return venkat_term(dm, d2, epsq, 1.0e-12, guard ? -1.0e-12 : 1.0e-12);
+}
