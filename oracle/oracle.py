@@ -113,6 +113,8 @@ def lib():
         D = C.c_double
         L.orc_picl_collision_force.argtypes = [P, P, D, D, D, P, P, D, D, D, D, D, D, P]
         L.orc_picl_collision_force2.argtypes = [P, P, D, D, D, D, D, D, D, D, D, P, P]
+        L.orc_set_threads.argtypes = [I]
+        L.orc_get_threads.restype = I
         L.orc_test_rk3_linear.argtypes = [D, D, P]
         L.orc_test_entropy_fix.argtypes = [D, D]
         L.orc_test_entropy_fix.restype = D
@@ -401,3 +403,12 @@ def qs_beta(qsFlag, rmachp, rphip, rep, rmu, dp, rhof=1.0, vmag=1.0):
     capped inputs."""
     return lib().orc_picl_qs_beta(int(qsFlag), float(rmachp), float(rphip), float(rep), float(rmu), float(dp),
                                   float(rhof), float(vmag))
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def threads():
+    return lib().orc_get_threads()

@@ -304,6 +304,7 @@ static void trilinear_interp(struct orc_picl *p, int ip) {
 /* ppiclf_solve.f:3285-3394 (sngl_elem); gather :3431-3447.                  */
 /* ------------------------------------------------------------------------ */
 static void init_solve(struct orc_picl *p, int interp) {
+#pragma omp parallel for schedule(dynamic, 64)
   for (int ip = 0; ip < p->n; ip++) {
     if (p->lost[ip]) continue;
     int e = locate(p, p->y + LRS * (size_t)ip);
@@ -311,6 +312,7 @@ static void init_solve(struct orc_picl *p, int interp) {
     if (e < 0) p->lost[ip] = 1;
   }
   if (interp)
+#pragma omp parallel for schedule(dynamic, 64)
     for (int ip = 0; ip < p->n; ip++)
       if (!p->lost[ip]) {
         if (p->pp.interpMode == 1)
@@ -349,6 +351,7 @@ static void init_solve(struct orc_picl *p, int interp) {
       }
       if (!skip) nimg++;
     }
+#pragma omp parallel for schedule(dynamic, 8)
     for (int e = 0; e < p->ne; e++)
       for (int n = 0; n < 8; n++) {
         const double *xn = p->xm1b + 24 * (size_t)e + 3 * n;
@@ -990,6 +993,7 @@ static void integrate_particle(struct orc_picl *p, int istage, double dt) {
                     {dt * 17.0 / 60.0, 1.0, dt * 5.0 / 12.0},
                     {dt * 5.0 / 12.0, 1.0, dt * 3.0 / 4.0}};
   init_solve(p, 1);
+#pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < p->n; i++)
     if (!p->lost[i]) set_ydot(p, i, dt);
   if (p->pp.viscousUnsteady == 1 && istage == 3) vu_shift(p, dt);

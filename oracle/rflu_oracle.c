@@ -17,6 +17,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static char g_err[512];
 const char *orc_last_error(void) { return g_err; }
@@ -609,6 +612,7 @@ static void prim2cons(orc_region *r) {
 /* LSQ gradients: modflu/RFLU_ModDifferentiationCells.F90:955-1174           */
 /* ------------------------------------------------------------------------ */
 static void gradients(orc_region *r) {
+#pragma omp parallel for schedule(static)
   for (int icg = 0; icg < r->nCells; icg++) {
     const double *m = r->moms + 10 * (size_t)icg;
     double r11 = m[0], r12 = m[1], r22 = m[2], r13 = m[3], r23 = m[4], r33 = m[5];
@@ -1726,4 +1730,22 @@ double orc_test_entropy_fix(double l, double d) { return entropy_fix_hh(l, d); }
 
 double orc_test_venkat_term(double dm, double d2, double epsq, int guard) {
   return venkat_term(dm, d2, epsq, 1.0e-12, guard ? -1.0e-12 : 1.0e-12);
+}
+
+/* Threads of the oracle's independent per-cell, per-element and per-particle
+ * loops (every such loop writes only its own entries, so the results do not
+ * depend on the thread count). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

@@ -92,10 +92,12 @@ struct PiclDev {
   int *ev = nullptr;        // [8][nC] vertex of ppiclF node n of each (device) element
   int *ecb = nullptr;       // centroid bin of each element
   int *binCnt = nullptr;    // particles per centroid bin (this stage)
+  int *gBox = nullptr;      // [6] lowest x, y, z and highest x, y, z centroid bin holding particles (this stage)
+  int *gvList = nullptr;    // vertices within reach of a bin holding particles (this stage), gvList[nV] = count
   double *gq = nullptr;     // [nP][8] by sorted position: x, y, z, map * multfci (5)
   double *xo = nullptr;     // [3][nP] position at the start of the stage, by current slot
   double *vproj = nullptr;  // [5][nV]
-  int *vperm = nullptr;     // vertices in Morton order (the gather's thread order)
+  int *vperm = nullptr;     // vertices in Morton order (the order of the deposit's vertex list)
   double gRdum = 0, gMult = 0, gD2chk = 0, gReach = 0, gReach2 = 0;
   // periodic images of the Gaussian deposit: shift hi - lo per periodic
   // axis (0 elsewhere); cbTop = the largest centroid bin coordinate
@@ -1344,86 +1346,270 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double x) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
-  int e = blockIdx.x * BLK + threadIdx.x;
-  if (e >= P.nC) return;
-  int n = P.off[e + 1] - P.off[e];
-  if (n) atomicAdd(&P.binCnt[P.ecb[e]], n);
+__device__ __forceinline__ int wave_max_int(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// GAUSS_L consecutive lanes share a vertex and split each candidate
-// element's particles; their compensated partial sums are combined pairwise
-// in lane order (error-free TwoSum), so the result does not depend on the
-// schedule.
-#define GAUSS_L 8
-__global__ void __launch_bounds__(BLK) k_gauss(PiclDev P) {
-  // vertex order: each XCD walks a contiguous range, so the particles a
-  // workgroup reads stay in its XCD's L2
-  int t = xcd_block() * BLK + threadIdx.x;
-  int vt = t / GAUSS_L, sub = t % GAUSS_L;
-  bool live = vt < P.nV;
-  int v = live ? P.vperm[vt] : 0;
+__global__ void k_gbox_reset(PiclDev P) {
+  int t = threadIdx.x;
+  if (t < 6) P.gBox[t] = t < 3 ? 0x7fffffff : -1;
+  if (t == 6) P.gvList[P.nV] = 0;
+}
+
+// The centroid-bin range a vertex's image reaches, clipped to the bins that
+// hold particles; false when it is empty.
+__device__ __forceinline__ bool gauss_bins(const PiclDev &P, const double xv[3], int im, double sh[3], int blo[3],
+                                           int bhi[3]) {
+  bool skip = false;
+#pragma unroll
+  for (int l = 0, r = im; l < 3; l++, r /= 3) {
+    int sd = P.gImg ? r % 3 - 1 : 0;
+    if (sd != 0 && P.gShift[l] == 0.0) skip = true;
+    sh[l] = sd == 0 ? 0.0 : (double)sd * P.gShift[l];
+    double top = (double)(P.cbN[l] - 1);
+    double xs = xv[l] - sh[l];
+    double a = (xs - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xs + P.gReach - P.cbLo[l]) / P.cbH[l];
+    if (b < 0.0 || a > P.cbTop[l]) skip = true;  // no centroid within reach
+    blo[l] = max((int)fmin(fmax(a, 0.0), top), P.gBox[l]);
+    bhi[l] = min((int)fmin(fmax(b, 0.0), top), P.gBox[3 + l]);
+    if (blo[l] > bhi[l]) skip = true;
+  }
+  return !skip;
+}
+
+// The vertices the deposit can reach this stage, appended to gvList (one
+// atomic per wave); every other vertex's values are zero.  The list's order
+// does not matter: each vertex's sum is computed on its own.
+__global__ void __launch_bounds__(BLK) k_gvsel(PiclDev P) {
+  const int t = blockIdx.x * BLK + threadIdx.x;
+  const int v = t < P.nV ? P.vperm[t] : 0;  // Morton order: the list's neighbours are close in space
+  bool sel = false;
   const size_t nV = P.nV;
+  if (t < P.nV) {
+    const double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
+    const int nImg = P.gImg ? 27 : 1;
+    for (int im = 0; im < nImg && !sel; im++) {
+      double sh[3];
+      int blo[3], bhi[3];
+      sel = gauss_bins(P, xv, im, sh, blo, bhi);
+    }
+    if (!sel) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) P.vproj[k * nV + v] = 0.0;
+    }
+  }
+  const unsigned long long m = __ballot(sel);
+  int base = 0;
+  if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(&P.gvList[P.nV], __popcll(m));
+  base = __shfl(base, 0, 64);
+  if (sel) P.gvList[base + lanes_below(m)] = v;
+}
+
+// Particles per centroid bin, and the box of bins that hold any (one atomic
+// per wave and bound).
+__global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
+  int e = blockIdx.x * BLK + threadIdx.x;
+  int n = e < P.nC ? P.off[e + 1] - P.off[e] : 0;
+  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+  if (n) {
+    int bin = P.ecb[e];
+    atomicAdd(&P.binCnt[bin], n);
+    int bx = bin % P.cbN[0], by = (bin / P.cbN[0]) % P.cbN[1], bz = bin / (P.cbN[0] * P.cbN[1]);
+    lo[0] = hi[0] = bx;
+    lo[1] = hi[1] = by;
+    lo[2] = hi[2] = bz;
+  }
+#pragma unroll
+  for (int l = 0; l < 3; l++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[l] = min(lo[l], __shfl_xor(lo[l], o, 64));
+      hi[l] = max(hi[l], __shfl_xor(hi[l], o, 64));
+    }
+  if ((threadIdx.x & 63) == 0 && hi[0] >= 0) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      atomicMin(&P.gBox[l], lo[l]);
+      atomicMax(&P.gBox[3 + l], hi[l]);
+    }
+  }
+}
+
+// One wave per vertex.  The wave scans the vertex's centroid bins 64 at a
+// time; elements within reach that hold particles are appended to a list in
+// LDS by ballot, in (bin, record) order, so the list's order is fixed.  The
+// 64 lanes then stride the list's particles flattened (lane l takes the l-th,
+// (l+64)-th, ... particle), so every loop of the wave is uniform.  Each lane
+// keeps a compensated sum; the lanes' sums are combined pairwise in lane
+// order at the end, so the result does not depend on the schedule.
+#define GW_CAP 256
+#define GW_WAVES (BLK / 64)
+#define GW_BLOCKS 8192  // a multiple of 8: 32 workgroups per CU
+
+// Compensated accumulation, branch-free: Knuth's TwoSum gives the exact
+// rounding error of s + x whatever their magnitudes (the same error the
+// Neumaier branch selects), in six additions.
+__device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
+  double t = s + x;
+  double z = t - s;
+  double e = (s - (t - z)) + (x - z);
+  c = c + e;
+  s = t;
+}
+
+// One batch of queued (particle slot, r^2) pairs within the cut-off: lane l
+// takes entry l (l < n).
+__device__ __forceinline__ void gauss_batch(const PiclDev &P, const int *Qp, const double *Qr, int n, int lane,
+                                            double acc[5], double cmp[5]) {
+  if (lane < n) {
+    const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)Qp[lane]);
+    double rexp = exp(Qr[lane] * P.gRdum);
+    double2 g1 = g[1], g2 = g[2], g3 = g[3];
+    two_sum_acc(acc[0], cmp[0], g1.y * rexp);
+    two_sum_acc(acc[1], cmp[1], g2.x * rexp);
+    two_sum_acc(acc[2], cmp[2], g2.y * rexp);
+    two_sum_acc(acc[3], cmp[3], g3.x * rexp);
+    two_sum_acc(acc[4], cmp[4], g3.y * rexp);
+  }
+}
+
+// The particles of the listed elements (E[j] = the element's first particle
+// slot in gq, C[j] = its particle count): each half-wave takes one element,
+// its 32 lanes its particles.  Those within the cut-off go to the queue (Qp,
+// Qr; *qn entries, fewer than 64 between calls) in lane order, and every 64
+// of them are summed as one batch.
+__device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, const int *C, int n, int lane,
+                                            const double xv[3], const double sh[3], int *Qp, double *Qr, int &qn,
+                                            double acc[5], double cmp[5]) {
+  const int half = lane >> 5, hl = lane & 31;
+  for (int k = 0; k < n; k += 2) {
+    const int j = k + half;
+    const int cnt = j < n ? C[j] : 0, first = j < n ? E[j] : 0;
+    const int maxc = max(cnt, __shfl_xor(cnt, 32, 64));
+    for (int pb = 0; pb < maxc; pb += 32) {
+      const int p = pb + hl;
+      bool take = false;
+      int pos = 0;
+      double rdist2 = 0.0;
+      if (p < cnt) {
+        pos = first + p;
+        const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
+        double2 g0 = g[0], g1 = g[1];
+        // the image's coordinates as ppiclF's ghost holds them, x_p + s
+        double dx = xv[0] - (sh[0] == 0.0 ? g0.x : g0.x + sh[0]);
+        double dy = xv[1] - (sh[1] == 0.0 ? g0.y : g0.y + sh[1]);
+        double dz = xv[2] - (sh[2] == 0.0 ? g1.x : g1.x + sh[2]);
+        rdist2 = dx * dx + dy * dy;
+        rdist2 = rdist2 + dz * dz;
+        take = rdist2 <= P.gD2chk;
+      }
+      const unsigned long long m = __ballot(take);
+      if (take) {
+        int at = qn + lanes_below(m);
+        Qp[at] = pos;
+        Qr[at] = rdist2;
+      }
+      qn += __popcll(m);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (qn >= 64) {
+        gauss_batch(P, Qp, Qr, 64, lane, acc, cmp);
+        // the remainder moves to the front
+        int rest = qn - 64, mp = lane < rest ? Qp[64 + lane] : 0;
+        double mr = lane < rest ? Qr[64 + lane] : 0.0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < rest) {
+          Qp[lane] = mp;
+          Qr[lane] = mr;
+        }
+        qn = rest;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+__device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, int *E, int *C, int *Qp, double *Qr) {
+  const size_t nV = P.nV;
+  const double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, cmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int qn = 0;  // queued pairs of the current image
   // image im: shift (im % 3 - 1, im / 3 % 3 - 1, im / 9 - 1) periods; a
   // particle image x_p + s counts at the vertex when x_p lies within reach
   // of xv - s
   const int nImg = P.gImg ? 27 : 1;
-  for (int im = 0; im < nImg && live; im++) {
+  for (int im = 0; im < nImg; im++) {
     double sh[3];
-    bool skip = false;
     int blo[3], bhi[3];
-    double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
-#pragma unroll
-    for (int l = 0, r = im; l < 3; l++, r /= 3) {
-      int sd = P.gImg ? r % 3 - 1 : 0;
-      if (sd != 0 && P.gShift[l] == 0.0) skip = true;
-      sh[l] = sd == 0 ? 0.0 : (double)sd * P.gShift[l];
-      double top = (double)(P.cbN[l] - 1);
-      double xs = xv[l] - sh[l];
-      double a = (xs - P.gReach - P.cbLo[l]) / P.cbH[l], b = (xs + P.gReach - P.cbLo[l]) / P.cbH[l];
-      if (b < 0.0 || a > P.cbTop[l]) skip = true;  // no centroid within reach
-      blo[l] = (int)fmin(fmax(a, 0.0), top);
-      bhi[l] = (int)fmin(fmax(b, 0.0), top);
-    }
-    if (skip) continue;
+    if (!gauss_bins(P, xv, im, sh, blo, bhi)) continue;
+    const int nbx = bhi[0] - blo[0] + 1, nby = bhi[1] - blo[1] + 1;
+    // lanes over (x, y) of one z layer: rows of nbx bins, rpr rows per round
+    // (chunks of 64 bins of a row when a row is longer)
+    const int xstep = nbx <= 64 ? nbx : 64, rpr = nbx <= 64 ? 64 / nbx : 1;
+    const int lx = nbx <= 64 ? lane % nbx : lane, lr = nbx <= 64 ? lane / nbx : 0;
+    int nList = 0;
     for (int bz = blo[2]; bz <= bhi[2]; bz++)
-      for (int by = blo[1]; by <= bhi[1]; by++)
-        for (int bx = blo[0]; bx <= bhi[0]; bx++) {
-          int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
-          if (P.binCnt[bin] == 0) continue;
-          for (int q = P.cbOff[bin]; q < P.cbOff[bin + 1]; q++) {
-            const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)q);
-            double2 c01 = cp[0], c23 = cp[1];
-            double t0 = c01.x - (xv[0] - sh[0]), t1 = c01.y - (xv[1] - sh[1]), t2 = c23.x - (xv[2] - sh[2]);
-            if (t0 * t0 + t1 * t1 + t2 * t2 > P.gReach2) continue;
+    for (int yb = 0; yb < nby; yb += rpr)
+    for (int xb = 0; xb < nbx; xb += xstep) {
+      int q0 = 0, nrec = 0;
+      int bx = blo[0] + xb + lx, by = blo[1] + yb + lr;
+      if (lr < rpr && bx <= bhi[0] && by <= bhi[1]) {
+        int bin = bx + P.cbN[0] * (by + P.cbN[1] * bz);
+        if (P.binCnt[bin]) {
+          q0 = P.cbOff[bin];
+          nrec = P.cbOff[bin + 1] - q0;
+        }
+      }
+      const int maxrec = wave_max_int(nrec);
+      for (int r = 0; r < maxrec; r++) {
+        bool take = false;
+        int first = 0, cnt = 0;
+        if (r < nrec) {
+          const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)(q0 + r));
+          double2 c01 = cp[0], c23 = cp[1];
+          double t0 = c01.x - (xv[0] - sh[0]), t1 = c01.y - (xv[1] - sh[1]), t2 = c23.x - (xv[2] - sh[2]);
+          if (t0 * t0 + t1 * t1 + t2 * t2 <= P.gReach2) {
             int e = (int)(unsigned)(__double_as_longlong(c23.y) & 0xffffffffll);
-            int p1 = P.off[e + 1];
-            for (int pos = P.off[e] + sub; pos < p1; pos += GAUSS_L) {
-              const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
-              double2 g0 = g[0], g1 = g[1];
-              // the image's coordinates as ppiclF's ghost holds them, x_p + s
-              double dx = xv[0] - (sh[0] == 0.0 ? g0.x : g0.x + sh[0]);
-              double dy = xv[1] - (sh[1] == 0.0 ? g0.y : g0.y + sh[1]);
-              double dz = xv[2] - (sh[2] == 0.0 ? g1.x : g1.x + sh[2]);
-              double rdist2 = dx * dx + dy * dy;
-              rdist2 = rdist2 + dz * dz;
-              if (rdist2 > P.gD2chk) continue;
-              double rexp = exp(rdist2 * P.gRdum);
-              double2 g2 = g[2], g3 = g[3];
-              neumaier(acc[0], cmp[0], g1.y * rexp);
-              neumaier(acc[1], cmp[1], g2.x * rexp);
-              neumaier(acc[2], cmp[2], g2.y * rexp);
-              neumaier(acc[3], cmp[3], g3.x * rexp);
-              neumaier(acc[4], cmp[4], g3.y * rexp);
-            }
+            first = P.off[e];
+            cnt = P.off[e + 1] - first;
+            take = cnt > 0;
           }
         }
+        const unsigned long long m = __ballot(take);
+        const int nm = __popcll(m);
+        if (nList + nm > GW_CAP) {
+          gauss_flush(P, E, C, nList, lane, xv, sh, Qp, Qr, qn, acc, cmp);
+          nList = 0;
+        }
+        if (take) {
+          int at = nList + lanes_below(m);
+          E[at] = first;
+          C[at] = cnt;
+        }
+        nList += nm;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
+    if (nList) gauss_flush(P, E, C, nList, lane, xv, sh, Qp, Qr, qn, acc, cmp);
+    gauss_batch(P, Qp, Qr, qn, lane, acc, cmp);  // what is left of this image's queue
+    qn = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
   // combine the lanes' (sum, compensation) pairs, lower lane first
 #pragma unroll
-  for (int off = 1; off < GAUSS_L; off <<= 1) {
-    bool upper = (sub & off) != 0;
+  for (int off = 1; off < 64; off <<= 1) {
+    bool upper = (lane & off) != 0;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       double so = __shfl_xor(acc[k], off, 64), co = __shfl_xor(cmp[k], off, 64);
@@ -1435,11 +1621,33 @@ __global__ void __launch_bounds__(BLK) k_gauss(PiclDev P) {
       cmp[k] = (ca + cb) + e;
     }
   }
-  if (live && sub == 0) {
+  if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < 5; k++) P.vproj[k * nV + v] = 0.0 + (acc[k] + cmp[k]);
   }
 }
+
+__global__ void __launch_bounds__(BLK) k_gauss_w(PiclDev P) {
+  __shared__ int sE[GW_WAVES][GW_CAP];
+  __shared__ int sC[GW_WAVES][GW_CAP];
+  __shared__ int sQp[GW_WAVES][128];
+  __shared__ double sQr[GW_WAVES][128];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int *E = sE[w], *C = sC[w], *Qp = sQp[w];
+  double *Qr = sQr[w];
+  const size_t nV = P.nV;
+  // the listed vertices are split into 8 contiguous ranges, one per XCD
+  // (hardware block b runs on XCD b % 8), walked by that XCD's waves
+  const int nList = P.gvList[P.nV];
+  const int xcd = blockIdx.x % 8, wavesX = (gridDim.x / 8) * GW_WAVES;
+  const int lo = (int)((long long)nList * xcd / 8), hi = (int)((long long)nList * (xcd + 1) / 8);
+  if (blockIdx.x >= (gridDim.x / 8) * 8) return;  // blocks beyond a multiple of 8 have no range
+  for (int vt = lo + (int)(blockIdx.x / 8) * GW_WAVES + w; vt < hi; vt += wavesX) {
+    const int v = P.gvList[vt];
+    gauss_vertex(P, v, lane, E, C, Qp, Qr);
+  }
+}
+
 
 // Every particle to its new position in the other copy; this stage's ydotc
 // becomes the next map.  Particles lost earlier keep their positions.
@@ -1567,7 +1775,7 @@ void picl_free(PiclState *s) {
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
                   P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
-                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq};
+                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq, P.gBox, P.gvList};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1597,8 +1805,10 @@ void picl_sort_and_deposit(rflu_handle *h, int posXo) {
   if (P.gauss) {
     ProfScope ps(h, "k_gauss");
     hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nCB)), dim3(BLK), 0, h->stream, P.binCnt, P.nCB);
+    hipLaunchKernelGGL(k_gbox_reset, dim3(1), dim3(64), 0, h->stream, P);
     hipLaunchKernelGGL(k_gcount, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
-    hipLaunchKernelGGL(k_gauss, dim3(nblk(P.nV * GAUSS_L)), dim3(BLK), 0, h->stream, P);
+    hipLaunchKernelGGL(k_gvsel, dim3(nblk(P.nV)), dim3(BLK), 0, h->stream, P);
+    hipLaunchKernelGGL(k_gauss_w, dim3(GW_BLOCKS), dim3(BLK), 0, h->stream, P);
   }
   ProfScope ps(h, "k_permute");
   hipLaunchKernelGGL(k_permute, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
@@ -1980,11 +2190,15 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.ev, ev.size(), ev.data());
     rc |= dalloc(h, &P.ecb, ecb.size(), ecb.data());
     rc |= dalloc(h, &P.binCnt, (size_t)P.nCB);
+    rc |= dalloc(h, &P.gBox, 6);
+    rc |= dalloc(h, &P.gvList, (size_t)P.nV + 1);
     rc |= dalloc(h, &P.gq, 8 * (size_t)nP);
     rc |= dalloc(h, &P.xo, 3 * (size_t)nP);
     rc |= dalloc(h, &P.vproj, 5 * (size_t)P.nV);
-    std::vector<int> vperm(P.nV);
-    for (int v = 0; v < P.nV; v++) vperm[v] = v;
+    // vertices along a Morton curve: the waves an XCD runs at once then work
+    // on one compact region and share its particles in that XCD's L2
+    std::vector<int> vperm;
+    morton_order(h->xyz.data(), P.nV, vperm);
     rc |= dalloc(h, &P.vperm, vperm.size(), vperm.data());
   }
   rc |= dalloc(h, &h->d.phi, (size_t)N);
