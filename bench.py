@@ -1,16 +1,22 @@
 """Benchmark of the MI355X Rocflu hot path (one GPU).
 
-Workload (default): second-order single-gas run on tets -- a 64^3 hex box
-split into 5 tets each (1,310,720 cells), nodes jittered +-5%, Mach 1.5
-shock at x = 0.25 into quiescent air, LSQ gradients + Venkatakrishnan
-limiter, Roe fluxes (ENTROPY 0.05), Wray RK3, FP64.  A step is one
-RFLU_TimeStepping iteration: dt evaluation + 3 RK stages over the whole mesh.
+Workload (default, ``curtain_spec``): the two-way coupled shock-particle
+curtain -- 400x80x80 hexes (2,560,000 cells), a Mach 2 shock into quiescent
+air, 4,000,000 particles (lognormal diameters, median 115 um, sigma_ln 0.2,
+rho_p 2500) uniform in a slab at 20% volume fraction; second-order AUSM+
+with LSQ gradients and the Venkatakrishnan limiter; trilinear interpolation,
+Schiller-Naumann quasi-steady drag + added mass + pressure gradient,
+Gaussian-filter projection of width 2 dx; Wray RK3, FP64.  A step is one
+RFLU_TimeStepping iteration: dt evaluation + 3 coupled RK stages.
 
-Prints one JSON line (see the driver contract): value = Mcell-updates/s,
-roofline = the dominant kernel's algorithmic HBM bytes per launch over its
-mean launch duration (HIP events on the library's stream, over the timed
-region), cpu_baseline = the CPU oracle (a single-threaded C restatement of
-the reference loops) on a bounded sample of the same workload.
+Prints one JSON line (see the driver contract): value = Mcell-updates/s;
+roofline = the dominant kernel (the one with the most time in the timed
+region) with its algorithmic bytes per launch over its mean launch
+duration (HIP events on the library's stream, over the timed region);
+cpu_baseline = the CPU oracle (a C restatement of the reference loops, its
+independent loops on all host cores, and on one core) on a bounded sample of
+the same workload; secondary = the second-order tet workload (64^3 x 5 tets,
+Roe) measured the same way.
 """
 import argparse
 import json
@@ -49,21 +55,44 @@ def algorithmic_bytes(nC, nF, nB, mbar, order):
     return b
 
 
-def picl_bytes(nP, nCp):
+def picl_bytes(nP, nCp, nVg=0):
     """Compulsory bytes per stage of the particle coupling (reference
     dataflow): per particle x, v, T and the RK register read and written,
     d_p, rho_p, V_p read, the element id read and written (256 B); per
     particle-occupied cell the interpolation inputs (cv 40, limited
     gradients 120, rhs 40, vol 8) and the 5 projected fields read and
-    written (288 B)."""
-    return {"k_particles": 256.0 * nP, "k_picl_fields": 208.0 * nCp, "k_deposit": 80.0 * nCp}
+    written (288 B).  With the Gaussian filter, k_gauss reads each
+    particle's position and 5 mapped values once (64 B) and each vertex that
+    receives a deposit its coordinates and writes its 5 values (64 B, nVg
+    vertices); those bytes are this kernel's share, not part of B_step."""
+    b = {"k_particles": 256.0 * nP, "k_picl_fields": 208.0 * nCp, "k_deposit": 80.0 * nCp}
+    if nVg:
+        b["k_gauss"] = 64.0 * nP + 64.0 * nVg
+    return b
+
+
+# ppiclF node n = i + 2j + 4k of a hex is its c2v entry NODE2V[n]
+NODE2V = np.array([0, 3, 4, 7, 1, 2, 5, 6])
+
+
+def deposit_vertices(reg, mesh):
+    """Vertices that receive a Gaussian deposit (nonzero PHIP)."""
+    nodes = reg.picl_nodes()  # (nCells, 5, 8)
+    cell, n = np.nonzero(nodes[:, 0, :])
+    return int(np.unique(np.asarray(mesh["c2v"])[cell, NODE2V[n]]).size)
 
 
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4  # wave-level VALU instructions/s: 1024 SIMDs, one per 4 cycles
 
 
-def _profile_rows(fp32, ncols):
-    name = "rflu_tets64_roe_fp32_kernels.md" if fp32 else "rflu_tets64_roe_kernels.md"
+PROFILES = {("tets64_roe", False): "rflu_tets64_roe_kernels.md", ("tets64_roe", True): "rflu_tets64_roe_fp32_kernels.md",
+            ("curtain_spec", False): "picl_curtain_spec_kernels.md"}
+
+
+def _profile_rows(workload, fp32, ncols):
+    name = PROFILES.get((workload, fp32))
+    if name is None:
+        return []
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return []
@@ -75,11 +104,11 @@ def _profile_rows(fp32, ncols):
     return rows
 
 
-def profiled_traffic(kernel, fp32=False):
+def profiled_traffic(kernel, workload, fp32=False):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
     calibration recorded there), or None when the summary lacks the kernel."""
-    for cells in _profile_rows(fp32, 5):
+    for cells in _profile_rows(workload, fp32, 5):
         if cells[0].startswith(kernel):
             try:
                 return (float(cells[2]) + float(cells[4])) * 1e6
@@ -88,10 +117,10 @@ def profiled_traffic(kernel, fp32=False):
     return None
 
 
-def profiled_valu(kernel, fp32=False):
+def profiled_valu(kernel, workload, fp32=False):
     """Wave-level VALU instructions per launch of `kernel` (SQ_INSTS_VALU)
     from the committed counter summary, or None."""
-    for cells in _profile_rows(fp32, 8):
+    for cells in _profile_rows(workload, fp32, 8):
         if cells[0].startswith(kernel):
             try:
                 return float(cells[2])
@@ -154,8 +183,19 @@ CURTAIN_SPEC = dict(interpMode=1, qsFlag=5, projMode=1, filterWidth=2 * 0.0005, 
 CURTAIN_DECK = dict(qsFlag=2, viscousUnsteady=1, collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
 
 
-def cpu_baseline(name, budget_s=12.0):
-    """Time the CPU oracle on a bounded sample of the same workload."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _oracle_sample(name):
+    """A bounded sample of the workload for the CPU oracle: (region,
+    particles or 0, description)."""
     from oracle import oracle as orc
     from rocflupicl_amd import configs
     if name.startswith("curtain"):
@@ -165,40 +205,53 @@ def cpu_baseline(name, budget_s=12.0):
         if name == "curtain_deck":
             picl.update(CURTAIN_DECK)
         o = orc.Region(mesh, **params)
-        vol, cofg = o.cells()
-        o.set_cv(ic(cofg))
+        o.set_cv(ic(o.cells()[1]))
         o.picl_init(y, pc, **picl)
-        o.step(1)
-        steps, t = 0, 0.0
-        t0 = time.perf_counter()
-        while t < budget_s:
-            o.step(1)
-            steps += 1
-            t = time.perf_counter() - t0
-        return dict(value=o.nCells * steps / t / 1e6, unit="Mcell-updates/s", cores=1, kind="port",
-                    particle_updates_per_s=4000 * steps / t,
-                    sample=f"{name} 40x8x8 hex (2,560 cells) + 4,000 particles, same physics, {steps} steps "
-                           f"in {t:.1f} s, single-threaded C oracle (it scans every element per particle, "
-                           "as the reference does)")
+        return o, 4000, f"{name} at 40x8x8 hexes (2,560 cells) + 4,000 particles, same physics"
     if name.startswith("tets64"):
-        n = 32
-        mesh, params, ic = configs.shock_tets(n=n, discr=1 if name.endswith("roe") else 3, limiterGuard=1)
-        sample = f"shock_tets n=32 (163,840 tets), same physics"
+        mesh, params, ic = configs.shock_tets(n=32, discr=1 if name.endswith("roe") else 3, limiterGuard=1)
+        desc = "shock_tets n=32 (163,840 tets), same physics"
     else:
         mesh, params, ic = configs.sod()
-        sample = "sod 200x4x4 (3,200 hexes), full workload"
+        desc = "sod 200x4x4 (3,200 hexes), full workload"
     o = orc.Region(mesh, **params)
-    vol, cofg = o.cells()
-    o.set_cv(ic(cofg))
-    o.step(1)
+    o.set_cv(ic(o.cells()[1]))
+    return o, 0, desc
+
+
+def _time_oracle(name, threads, budget_s):
+    from oracle import oracle as orc
+    orc.set_threads(threads)
+    o, nP, desc = _oracle_sample(name)
+    o.step(1)  # warm-up
     steps, t = 0, 0.0
     t0 = time.perf_counter()
     while t < budget_s:
         o.step(1)
         steps += 1
         t = time.perf_counter() - t0
-    return dict(value=o.nCells * steps / t / 1e6, unit="Mcell-updates/s", cores=1, kind="port",
-                sample=f"{sample}, {steps} steps in {t:.1f} s, single-threaded C oracle")
+    return o.nCells * steps / t / 1e6, nP * steps / t, steps, t, desc
+
+
+def cpu_baseline(name, budget_s=10.0):
+    """The CPU oracle timed on a bounded sample of the same workload, with its
+    independent loops on all host cores (OpenMP; threads = OMP_NUM_THREADS or
+    the host's CPU count) and on one core."""
+    from oracle import oracle as orc
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    v_all, p_all, st_all, t_all, desc = _time_oracle(name, cores, budget_s)
+    v_one, p_one, st_one, t_one, _ = _time_oracle(name, 1, budget_s)
+    orc.set_threads(cores)
+    out = dict(value=v_all, unit="Mcell-updates/s", cores=cores, kind="port",
+               sample=f"{desc}; {st_all} steps in {t_all:.1f} s on {cores} threads, {st_one} steps in "
+                      f"{t_one:.1f} s on 1 core; C oracle (OpenMP over its per-cell, per-element and "
+                      f"per-particle loops; the face scatters stay serial)",
+               single_core=dict(value=v_one, unit="Mcell-updates/s", cores=1),
+               cpu_model=cpu_model(), nproc=os.cpu_count())
+    if p_all:
+        out["particle_updates_per_s"] = p_all
+        out["single_core"]["particle_updates_per_s"] = p_one
+    return out
 
 
 def run_batch(args):
@@ -293,12 +346,112 @@ def run_serve(args):
     print(json.dumps(out))
 
 
+KERNELS = ("k_dtmin", "k_grad", "k_rhog", "k_flux", "k_gather", "k_update", "k_picl_fields", "k_particles",
+           "k_scan", "k_fill", "k_deposit", "k_gauss", "k_permute", "k_feedback")
+
+
+def measure(workload, args, fp32=False):
+    """Run one workload: warm-up, then the timed steps with per-kernel HIP
+    events on the library's stream.  Returns (result dict, region, inputs)."""
+    from rocflupicl_amd import rflu
+    (mesh, params, ic), info = build_workload(workload)
+    t0 = time.perf_counter()
+    reg = rflu.Region(mesh, **params, fluxPrecision=1 if fp32 else 0)
+    m = reg.mesh_arrays()
+    reg.set_state(ic(m["cofg"]))
+    nP = 0
+    if "particles" in info:
+        y, pc = info["particles"]
+        reg.picl_init(y, pc, **info["picl"])
+        nP = y.shape[0]
+    setup_s = time.perf_counter() - t0
+    nC, nF, nB = reg.nCells, reg.nFaces, reg.nBFaces
+    order = params["spaceOrder"]
+    mbar = reg.nStencil / nC if order == 2 else 0.0
+    byt = algorithmic_bytes(nC, nF, nB, mbar, order)
+
+    reg.RFLU_TimeStepping(args.warmup, use_graph=args.graph)
+    reg.sync()
+    reg.set_profiling(not args.graph)
+    t0 = time.perf_counter()
+    reg.RFLU_TimeStepping(args.steps, use_graph=args.graph)
+    reg.sync()
+    dt_s = time.perf_counter() - t0
+    reg.set_profiling(False)
+    ms_step = dt_s / args.steps * 1e3
+    value = nC * args.steps / dt_s / 1e6
+
+    nCp = nVg = 0
+    if nP:
+        nCp = int(np.count_nonzero(reg.picl_cells()["phi"]))
+        nVg = deposit_vertices(reg, mesh) if info["picl"].get("projMode") == 1 else 0
+        pb = picl_bytes(nP, nCp, nVg)
+        byt.update(pb)
+        # B_step of the reference dataflow: B_dt + 3 (B_stage + B_PICL), B_PICL = 256 Np + 288 Ncp
+        byt["step"] += 3.0 * (256.0 * nP + 288.0 * nCp)
+    print(f"# workload {workload}: {nC} cells, {nF} faces, {nB} bfaces, mbar {mbar:.2f}, {nP} particles, "
+          f"{nCp} occupied cells, {nVg} deposit vertices, setup {setup_s:.1f}s, B_step {byt['step']/1e9:.4f} GB",
+          file=sys.stderr, flush=True)
+
+    kern = {}
+    if not args.graph:
+        for k in KERNELS:
+            ms, n = reg.kernel_time(k)
+            if n:
+                kern[k] = dict(ms=ms, launches=n, total_ms=ms * n)
+    roof = None
+    if kern:
+        # the dominant kernel: the most time in the timed region
+        dom = max(kern, key=lambda k: kern[k]["total_ms"])
+        if dom == "k_grad":  # the gradient routine runs as the tile kernel (default) or k_grad2
+            if os.environ.get("ROCFLUPICL_GRAD_VARIANT", "4") != "4":
+                pname = "k_grad2"
+            elif nP:
+                pname = "k_grad_tile<1>"
+            else:
+                pname = "k_grad_tile_f32" if fp32 else "k_grad_tile_gas"
+        elif dom == "k_gauss":
+            pname = "k_gauss_w"
+        else:
+            pname = dom
+        traffic = profiled_traffic(pname, workload, fp32)
+        share = kern[dom]["total_ms"] / (dt_s * 1e3)
+        roof = dict(bound="hbm", kernel=dom, achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
+                    traffic=None if traffic is None else int(traffic), algorithmic_bytes=None,
+                    ms_per_launch=round(kern[dom]["ms"], 4), share_of_step=round(share, 4))
+        if dom in byt:
+            ach = byt[dom] / (kern[dom]["ms"] * 1e-3) / 1e9
+            roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=int(byt[dom]))
+        vi = profiled_valu(pname, workload, fp32)
+        if vi:
+            rate = vi / (kern[dom]["ms"] * 1e-3)
+            roof["valu"] = dict(bound="VALU issue", achieved=round(rate / 1e9, 1), peak=VALU_ISSUE_PEAK / 1e9,
+                                unit="G wave-instructions/s", frac=round(rate / VALU_ISSUE_PEAK, 4),
+                                instructions_per_launch=int(vi))
+    step_gbs = byt["step"] / (ms_step * 1e-3) / 1e9
+    t_end, dt_end = reg.time()
+    res = {
+        "value": round(value, 2), "ms_per_step": round(ms_step, 4),
+        "config": {"workload": workload, "model": info["model"], "cells": nC, "faces": nF, "bfaces": nB,
+                   "stencil_mean": round(mbar, 3), "particles": nP or None, "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"algorithmic_bytes_per_step": int(byt["step"]), "achieved_GBs": round(step_gbs, 1),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "kernels_ms": {k: round(v["ms"], 4) for k, v in kern.items()},
+        "particle_updates_per_s": round(nP * args.steps / dt_s, 1) if nP else None,
+        "sim_time": t_end,
+    }
+    return res, reg, (mesh, params, ic)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="tets64_roe")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="curtain_spec")
+    ap.add_argument("--secondary", default="tets64_roe",
+                    help="a second workload measured the same way and reported under 'secondary' ('none': skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time hipGraph replays (no per-kernel events)")
     ap.add_argument("--fp32", action="store_true",
@@ -324,71 +477,7 @@ def main():
     if args.workload == "serve_trace":
         return run_serve(args)
     from rocflupicl_amd import rflu
-    (mesh, params, ic), info = build_workload(args.workload)
-    t0 = time.perf_counter()
-    reg = rflu.Region(mesh, **params, fluxPrecision=1 if args.fp32 else 0)
-    m = reg.mesh_arrays()
-    reg.set_state(ic(m["cofg"]))
-    nP = 0
-    if "particles" in info:
-        y, pc = info["particles"]
-        reg.picl_init(y, pc, **info["picl"])
-        nP = y.shape[0]
-    setup_s = time.perf_counter() - t0
-    nC, nF, nB = reg.nCells, reg.nFaces, reg.nBFaces
-    order = params["spaceOrder"]
-    mbar = reg.nStencil / nC if order == 2 else 0.0
-    byt = algorithmic_bytes(nC, nF, nB, mbar, order)
-    if nP:
-        nCp = int(np.count_nonzero(reg.picl_cells()["phi"]))
-        pb = picl_bytes(nP, nCp)
-        byt.update(pb)
-        byt["step"] += 3.0 * sum(pb.values())
-    print(f"# workload {args.workload}: {nC} cells, {nF} faces, {nB} bfaces, mbar {mbar:.2f}, {nP} particles, "
-          f"setup {setup_s:.1f}s, B_step {byt['step']/1e9:.4f} GB", file=sys.stderr, flush=True)
-
-    reg.RFLU_TimeStepping(args.warmup, use_graph=args.graph)
-    reg.sync()
-    reg.set_profiling(not args.graph)
-    t0 = time.perf_counter()
-    reg.RFLU_TimeStepping(args.steps, use_graph=args.graph)
-    reg.sync()
-    dt_s = time.perf_counter() - t0
-    reg.set_profiling(False) if args.graph else None
-    ms_step = dt_s / args.steps * 1e3
-    value = nC * args.steps / dt_s / 1e6
-
-    kern = {}
-    if not args.graph:
-        for k in ("k_dtmin", "k_grad", "k_rhog", "k_flux", "k_gather", "k_update", "k_picl_fields",
-                  "k_particles", "k_scan", "k_fill", "k_deposit", "k_gauss", "k_permute", "k_feedback"):
-            ms, n = reg.kernel_time(k)
-            if n:
-                kern[k] = dict(ms=ms, launches=n, total_ms=ms * n)
-    roof = None
-    if kern:
-        dom = max((k for k in kern if k in byt), key=lambda k: kern[k]["total_ms"])
-        ach = byt[dom] / (kern[dom]["ms"] * 1e-3) / 1e9
-        # the gradient routine runs as the tile kernel (default) or k_grad2 (variants 1-3)
-        if os.environ.get("ROCFLUPICL_GRAD_VARIANT", "4") == "4":
-            gname = "k_grad_tile_f32" if args.fp32 else "k_grad_tile_gas"
-        else:
-            gname = "k_grad2"
-        pname = gname if dom == "k_grad" else dom
-        traffic = profiled_traffic(pname, args.fp32) if args.workload == "tets64_roe" else None
-        roof = dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=None if traffic is None else int(traffic),
-                    algorithmic_bytes=int(byt[dom]), ms_per_launch=round(kern[dom]["ms"], 4))
-        # the gradient kernel is bound by VALU issue, not HBM: its instruction
-        # rate from the committed SQ_INSTS_VALU count over the live launch time
-        vi = profiled_valu(pname, args.fp32) if args.workload == "tets64_roe" else None
-        if vi:
-            rate = vi / (kern[dom]["ms"] * 1e-3)
-            roof["valu"] = dict(bound="VALU issue", achieved=round(rate / 1e9, 1), peak=VALU_ISSUE_PEAK / 1e9,
-                                unit="G wave-instructions/s", frac=round(rate / VALU_ISSUE_PEAK, 4),
-                                instructions_per_launch=int(vi))
-    step_gbs = byt["step"] / (ms_step * 1e-3) / 1e9
+    res, reg, (mesh, params, ic) = measure(args.workload, args, args.fp32)
     copy_gbs = None
     if args.copy_probe_mib > 0:
         import ctypes
@@ -399,40 +488,44 @@ def main():
     fp32_err = None
     if args.fp32:
         ref = rflu.Region(mesh, **params)
-        ref.set_state(ic(m["cofg"]))
+        ref.set_state(ic(ref.mesh_arrays()["cofg"]))
         ref.RFLU_TimeStepping(args.warmup + args.steps, use_graph=True)
         a, b = reg.get_state()[0], ref.get_state()[0]
         # relative L2 error of rho, the momentum vector and rho*E (the largest)
         fp32_err = float(max(np.linalg.norm(a[:, g] - b[:, g]) / np.linalg.norm(b[:, g])
                              for g in ([0], [1, 2, 3], [4])))
         ref.close()
+    reg.close()
+    secondary = None
+    if args.secondary != "none" and args.secondary != args.workload:
+        sres, sreg, _ = measure(args.secondary, args, False)
+        sreg.close()
+        secondary = {k: sres[k] for k in ("value", "ms_per_step", "config", "roofline", "step_roofline", "kernels_ms")}
+        secondary["unit"] = "Mcell-updates/s"
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload)
-    t_end, dt_end = reg.time()
     out = {
         "metric": "Mcell-updates/s/GPU & % HBM roofline",
-        "value": round(value, 2),
+        "value": res["value"],
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-flux/f64-accumulate" if args.fp32 else "f64",
-        "data": "synthetic (seeded mesh jitter and shock initial state)",
-        "config": {"workload": args.workload, "model": info["model"], "cells": nC, "faces": nF,
-                   "bfaces": nB, "stencil_mean": round(mbar, 3), "parallelism": "single GPU"},
-        "roofline": roof,
-        "step_roofline": {"algorithmic_bytes_per_step": int(byt["step"]), "achieved_GBs": round(step_gbs, 1),
-                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
-        "kernels_ms": {k: round(v["ms"], 4) for k, v in kern.items()},
-        "particles": nP or None,
-        "particle_updates_per_s": round(nP * args.steps / dt_s, 1) if nP else None,
+        "data": "synthetic (seeded mesh, shock initial state and particle cloud)",
+        "config": res["config"],
+        "roofline": res["roofline"],
+        "step_roofline": res["step_roofline"],
+        "kernels_ms": res["kernels_ms"],
+        "particle_updates_per_s": res["particle_updates_per_s"],
         "copy_probe_GBs": copy_gbs,
         "fp32_rel_l2_vs_fp64": fp32_err,
         "cpu_baseline": cpu,
-        "sim_time": t_end,
+        "secondary": secondary,
+        "sim_time": res["sim_time"],
     }
     print(json.dumps(out))
 

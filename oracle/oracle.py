@@ -116,6 +116,7 @@ def lib():
         L.orc_set_threads.argtypes = [I]
         L.orc_get_threads.restype = I
         L.orc_test_rk3_linear.argtypes = [D, D, P]
+        L.orc_set_threads(int(os.environ.get("ROCFLUPICL_ORACLE_THREADS", "1")))
         L.orc_test_entropy_fix.argtypes = [D, D]
         L.orc_test_entropy_fix.restype = D
         L.orc_test_venkat_term.argtypes = [D, D, D, I]

@@ -680,6 +680,8 @@ static void limiter_venkat(orc_region *r) {
       vmax[5 * c2 + k] = fmax(vmax[5 * c2 + k], a);
     }
   }
+  double *tbuf = malloc(sizeof(double) * 10 * (size_t)(r->nFaces > 0 ? r->nFaces : 1));
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < r->nFaces; f++) {
     int c1 = r->f2c[2 * f], c2 = r->f2c[2 * f + 1];
     double xc = r->fc[3 * f], yc = r->fc[3 * f + 1], zc = r->fc[3 * f + 2];
@@ -695,17 +697,25 @@ static void limiter_venkat(orc_region *r) {
       const double *g1 = r->grad + 15 * (size_t)c1 + 3 * k, *g2 = r->grad + 15 * (size_t)c2 + 3 * k;
       double d21 = g1[0] * dx1 + g1[1] * dy1 + g1[2] * dz1;
       double d22 = g2[0] * dx2 + g2[1] * dy2 + g2[2] * dz2;
-      double *l1 = &r->lim[5 * (size_t)c1 + k], *l2 = &r->lim[5 * (size_t)c2 + k];
-      if (d21 > 0.0)
-        *l1 = fmin(venkat_term(d1max1, d21, epsq1, TINYP, TINYN), *l1);
-      else if (d21 < 0.0)
-        *l1 = fmin(venkat_term(d1min1, d21, epsq1, TINYP, TINYN), *l1);
-      if (d22 > 0.0)
-        *l2 = fmin(venkat_term(d1max2, d22, epsq2, TINYP, TINYN), *l2);
-      else if (d22 < 0.0)
-        *l2 = fmin(venkat_term(d1min2, d22, epsq2, TINYP, TINYN), *l2);
+      /* +inf stands for "no term" (d = 0): fmin(+inf, l) = l */
+      double *t = tbuf + 10 * (size_t)f;
+      t[k] = d21 > 0.0 ? venkat_term(d1max1, d21, epsq1, TINYP, TINYN)
+           : d21 < 0.0 ? venkat_term(d1min1, d21, epsq1, TINYP, TINYN) : INFINITY;
+      t[5 + k] = d22 > 0.0 ? venkat_term(d1max2, d22, epsq2, TINYP, TINYN)
+               : d22 < 0.0 ? venkat_term(d1min2, d22, epsq2, TINYP, TINYN) : INFINITY;
     }
   }
+  /* the per-face terms were formed in parallel; the minima in face order */
+  for (int f = 0; f < r->nFaces; f++) {
+    int c1 = r->f2c[2 * f], c2 = r->f2c[2 * f + 1];
+    const double *t = tbuf + 10 * (size_t)f;
+    for (int k = 0; k < 5; k++) {
+      double *l1 = &r->lim[5 * (size_t)c1 + k], *l2 = &r->lim[5 * (size_t)c2 + k];
+      *l1 = fmin(t[k], *l1);
+      *l2 = fmin(t[5 + k], *l2);
+    }
+  }
+  free(tbuf);
   for (int i = 0; i < N; i++)
     for (int k = 0; k < 5; k++) {
       double l = r->lim[5 * (size_t)i + k];
@@ -1080,6 +1090,9 @@ static void interior_fluxes(orc_region *r) {
   const orc_params *p = &r->p;
   const double fs = 0.0, ksg = 0.0;
   double cp = p->refCp, g = p->refGamma, gc = R_CpG(cp, g);
+  /* the face fluxes in parallel, then added into the cells in face order */
+  double *fbuf = malloc(sizeof(double) * 5 * (size_t)(r->nFaces > 0 ? r->nFaces : 1));
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < r->nFaces; f++) {
     int c1 = r->f2c[2 * f], c2 = r->f2c[2 * f + 1];
     double nx = r->fn[4 * f], ny = r->fn[4 * f + 1], nz = r->fn[4 * f + 2], nm = r->fn[4 * f + 3];
@@ -1159,8 +1172,10 @@ static void interior_fluxes(orc_region *r) {
         hllc_flux(p, nx, ny, nz, nm, fs, rl, ul, vl, wl, pl, el, al, ql, Hl, rr, ur, vr, wr, pr, er, ar, qr, Hr, flx);
       }
     }
-    add_rhs(r, c1, c2, flx);
+    memcpy(fbuf + 5 * (size_t)f, flx, sizeof flx);
   }
+  for (int f = 0; f < r->nFaces; f++) add_rhs(r, r->f2c[2 * f], r->f2c[2 * f + 1], fbuf + 5 * (size_t)f);
+  free(fbuf);
 }
 
 /* libfloflu/BcondOutflowPerf.F90:88-160 */

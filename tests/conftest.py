@@ -39,4 +39,8 @@ def pytest_collection_modifyitems(session, config, items):
 def oracle_mod():
     from oracle import oracle
     oracle.build()
+    # one process: the oracle's parallel loops may use the host's cores
+    # (a pytest-xdist worker keeps one thread, beside the other workers)
+    if "PYTEST_XDIST_WORKER" not in os.environ:
+        oracle.set_threads(min(16, os.cpu_count() or 1))
     return oracle
