@@ -377,7 +377,6 @@ def measure(workload, args, fp32=False):
     reg.RFLU_TimeStepping(args.steps, use_graph=args.graph)
     reg.sync()
     dt_s = time.perf_counter() - t0
-    reg.set_profiling(False)
     ms_step = dt_s / args.steps * 1e3
     value = nC * args.steps / dt_s / 1e6
 
@@ -399,6 +398,7 @@ def measure(workload, args, fp32=False):
             ms, n = reg.kernel_time(k)
             if n:
                 kern[k] = dict(ms=ms, launches=n, total_ms=ms * n)
+    reg.set_profiling(False)  # (clears the per-kernel times)
     roof = None
     if kern:
         # the dominant kernel: the most time in the timed region

@@ -1432,10 +1432,12 @@ __global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
       hi[l] = max(hi[l], __shfl_xor(hi[l], o, 64));
     }
   if ((threadIdx.x & 63) == 0 && hi[0] >= 0) {
+    // a plain read first: once the bounds are nearly final, few waves still
+    // tighten them, and the others issue no atomic on these six words
 #pragma unroll
     for (int l = 0; l < 3; l++) {
-      atomicMin(&P.gBox[l], lo[l]);
-      atomicMax(&P.gBox[3 + l], hi[l]);
+      if (lo[l] < __atomic_load_n(&P.gBox[l], __ATOMIC_RELAXED)) atomicMin(&P.gBox[l], lo[l]);
+      if (hi[l] > __atomic_load_n(&P.gBox[3 + l], __ATOMIC_RELAXED)) atomicMax(&P.gBox[3 + l], hi[l]);
     }
   }
 }
