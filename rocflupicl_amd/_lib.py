@@ -33,6 +33,15 @@ PICL_SYMBOLS = [
     "picl_gpu_locate_static",
 ]
 
+# Every symbol include/ppiclf_gpu.h declares (ppiclF's F77 entry points and
+# the set-up of their default instance).
+PPICLF_SYMBOLS = [
+    "ppiclf_gpu_bind", "ppiclf_gpu_reset", "ppiclf_gpu_set_abort", "ppiclf_gpu_last_error", "ppiclf_gpu_handle",
+    "ppiclf_solve_initialize_", "ppiclf_solve_initparticle_", "ppiclf_solve_initboxfilter_",
+    "ppiclf_solve_initgaussianfilter_", "ppiclf_solve_initneighborbin_", "ppiclf_comm_initoverlapmesh_",
+    "ppiclf_solve_interpfielduser_", "ppiclf_solve_integrateparticle_", "ppiclf_solve_getprofldijkef_",
+]
+
 # picl_gpu.h: PICL_LOCATE_STATIC_MAX, the largest n of picl_gpu_locate_static
 PICL_LOCATE_STATIC_MAX = 262144
 
@@ -128,6 +137,21 @@ def load():
     for n in RFLU_SYMBOLS + PICL_SYMBOLS:
         if n != "rflu_gpu_last_error":
             getattr(L, n).restype = I
+    # ppiclF's F77 entry points: every argument by reference, no result
+    nargs = {"ppiclf_solve_initialize_": 14, "ppiclf_solve_initparticle_": 8, "ppiclf_solve_initboxfilter_": 3,
+             "ppiclf_solve_initgaussianfilter_": 3, "ppiclf_solve_initneighborbin_": 1,
+             "ppiclf_comm_initoverlapmesh_": 7, "ppiclf_solve_interpfielduser_": 2,
+             "ppiclf_solve_integrateparticle_": 4, "ppiclf_solve_getprofldijkef_": 6}
+    for n, k in nargs.items():
+        getattr(L, n).argtypes = [P] * k
+        getattr(L, n).restype = None
+    L.ppiclf_gpu_bind.argtypes = [P, C.POINTER(PiclParams)]
+    L.ppiclf_gpu_bind.restype = I
+    L.ppiclf_gpu_reset.restype = I
+    L.ppiclf_gpu_set_abort.argtypes = [I]
+    L.ppiclf_gpu_set_abort.restype = None
+    L.ppiclf_gpu_last_error.restype = C.c_char_p
+    L.ppiclf_gpu_handle.restype = P
     _lib = L
     return L
 

@@ -26,6 +26,8 @@
 // particle state is compared to the oracle with a tolerance.
 
 #define PICL_NFLD 20
+#define PICL_LRS 12  // PPICLF_LRS: doubles per particle in y
+#define PICL_LRP 48  // PPICLF_LRP: doubles per particle in rprop
 #define F32(x) ((double)(x##f))
 
 namespace {
@@ -1828,14 +1830,12 @@ int launch_rhog(rflu_handle *h) {
 }
 
 // PICL_TEMP_Runge for one stage (after SourceTermsMP, before RKUpdateMP).
-int picl_stage(rflu_handle *h, int istage) {
+// ppiclf_solve_IntegrateParticle's work for RK stage istage, from the fields
+// in P.fld and dt in scal[S_DTMIN]: collisions' neighbour search, locate,
+// interpolate, forces, RK3, then the sort by element and the projection.
+void picl_particle_stage(rflu_handle *h, int istage) {
   PiclState *s = h->picl;
   PiclDev &P = s->P;
-  {
-    ProfScope ps(h, "k_picl_fields");
-    hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P,
-                       s->phys.interpMode);
-  }
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nC + 1)), dim3(BLK), 0, h->stream, P.cnt, P.nC + 1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lostCur, 1);
   if (P.col) {  // the stage's neighbour search for the collision force
@@ -1861,6 +1861,19 @@ int picl_stage(rflu_handle *h, int istage) {
   }
   if (s->phys.vu && istage == 3) hipLaunchKernelGGL(k_vu_time, dim3(1), dim3(64), 0, h->stream, h->d, P);
   picl_sort_and_deposit(h, 1);
+}
+
+// PICL_TEMP_Runge for RK stage istage: the fields from the gas state, the
+// particle stage, Rocflu's read-back into rhs and piclVF.
+int picl_stage(rflu_handle *h, int istage) {
+  PiclState *s = h->picl;
+  PiclDev &P = s->P;
+  {
+    ProfScope ps(h, "k_picl_fields");
+    hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P,
+                       s->phys.interpMode);
+  }
+  picl_particle_stage(h, istage);
   ProfScope ps(h, "k_feedback");
   hipLaunchKernelGGL(k_feedback, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P, s->phys.feedbackFlag);
   return 0;
@@ -2428,6 +2441,288 @@ int picl_gpu_locate_static(rflu_handle *h, int n, const double *xyz, int *elem) 
   HIPCHK(h, hipMemcpyAsync(elem, de, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ppiclF's F77 entry points (gfortran names: lowercase + '_', arguments by
+// reference) over one default instance, as ppiclF keeps one instance per
+// process in COMMON blocks.  The instance is bound to a Rocflu region
+// (ppiclf_gpu_bind); InitOverlapMesh attaches the particles to it
+// (picl_gpu_init), InterpFieldUser stages host fields, IntegrateParticle
+// runs one RK stage of the particles on the device from those fields, and
+// GetProFldIJKEF reads the projection back.  Errors stop the process with
+// ppiclF's message, as ppiclf_exittr does (ppiclf_op.f:288-318), unless the
+// host asks for them to be recorded instead.
+// ---------------------------------------------------------------------------
+namespace {
+struct F77State {
+  rflu_handle *h = nullptr;
+  picl_params pp{};
+  bool init = false, linit = false, lfilt = false, overlap = false;
+  int per[3] = {0, 0, 0};
+  int npart = 0;
+  std::vector<double> y, pc;
+  int proj = 0;
+  double width = 0.0, alpha = 0.0, nbw = 0.0;
+  int stage = 0;  // ppiclf_solve_IntegrateRK3s_Rocflu's SAVE stage counter
+  int icnt = 0;   // PPICLF_INT_ICNT
+  std::vector<double> fld;  // [PICL_NFLD][nCells], the host's cell numbering
+  std::vector<char> have;
+  std::vector<double> pro;  // [nCells][5][8] projection read back
+  bool proValid = false;
+  bool abortOnError = true;
+  std::string err;
+};
+F77State g_f77;
+
+// rprop index (PPICLF_USER.h) of each field the device interpolates, in its order
+const int F77_FIELD_RPROP[PICL_NFLD] = {2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39};
+
+void f77_fail(const std::string &m) {
+  g_f77.err = m;
+  if (g_f77.abortOnError) {
+    fprintf(stderr, "PPICLF: ERROR %s\n", m.c_str());
+    fflush(stderr);
+    std::exit(1);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int ppiclf_gpu_bind(rflu_handle *h, const picl_params *pp) {
+  API_LOCK;
+  if (!h || !pp) return RFLU_GPU_EINVAL;
+  if (h->picl) return set_err(h, RFLU_GPU_EINVAL, "ppiclf_gpu_bind: the region already carries particles");
+  bool abortOnError = g_f77.abortOnError;
+  g_f77 = F77State();
+  g_f77.abortOnError = abortOnError;
+  g_f77.h = h;
+  g_f77.pp = *pp;
+  return 0;
+}
+
+int ppiclf_gpu_reset(void) {
+  API_LOCK;
+  bool abortOnError = g_f77.abortOnError;
+  g_f77 = F77State();
+  g_f77.abortOnError = abortOnError;
+  return 0;
+}
+
+void ppiclf_gpu_set_abort(int on) {
+  API_LOCK;
+  g_f77.abortOnError = on != 0;
+}
+
+const char *ppiclf_gpu_last_error(void) { return g_f77.err.c_str(); }
+
+rflu_handle *ppiclf_gpu_handle(void) { return g_f77.h; }
+
+// ppiclf_solve.f:1-180
+void ppiclf_solve_initialize_(const int *xi1, const double *xpmin, const double *xpmax, const int *yi1,
+                              const double *ypmin, const double *ypmax, const int *zi1, const double *zpmin,
+                              const double *zpmax, const int *ai1, const double *apa, const double *apxa,
+                              const double *aprin, const double *aprout) {
+  API_LOCK;
+  (void)apa; (void)apxa; (void)aprin; (void)aprout;
+  if (!g_f77.h) return f77_fail("ppiclf_gpu_bind must come before ppiclf_solve_Initialize");
+  if (*ai1 == 1) return f77_fail("angular periodicity (ang_per_flag 1) is not built");
+  const int fl[3] = {*xi1, *yi1, *zi1};
+  const double lo[3] = {*xpmin, *ypmin, *zpmin}, hi[3] = {*xpmax, *ypmax, *zpmax};
+  double vlo[3] = {1e300, 1e300, 1e300}, vhi[3] = {-1e300, -1e300, -1e300};
+  const std::vector<double> &X = g_f77.h->xyz;
+  for (size_t v = 0; v < X.size() / 3; v++)
+    for (int l = 0; l < 3; l++) {
+      vlo[l] = std::min(vlo[l], X[3 * v + l]);
+      vhi[l] = std::max(vhi[l], X[3 * v + l]);
+    }
+  static const char *ax = "XYZ";
+  for (int l = 0; l < 3; l++) {
+    g_f77.per[l] = fl[l] == 1 ? 1 : 0;
+    if (fl[l] != 1) continue;
+    if (lo[l] >= hi[l]) return f77_fail(std::string("Periodic") + ax[l] + " must have min < max");
+    if (lo[l] != vlo[l] || hi[l] != vhi[l])
+      return f77_fail(std::string("Periodic") + ax[l] +
+                      " bounds must be the grid's vertex minimum and maximum (PICL_TEMP_InitSolver.F90:234-262)");
+  }
+  g_f77.init = true;
+}
+
+// ppiclf_solve.f:184-330
+void ppiclf_solve_initparticle_(const int *imethod, const int *ndim, const int *iendian, const int *npart,
+                                const double *y, const double *rprop, const double *filt2, const double *filt3) {
+  API_LOCK;
+  (void)iendian;
+  if (!g_f77.init) return f77_fail("ppiclf_solve_Initialize must come before InitParticle");
+  if (*imethod != 2) return f77_fail("imethod must be 2 (IntegrateRK3s_Rocflu)");
+  if (*ndim != 3) return f77_fail("ndim must be 3");
+  if (*npart < 1) return f77_fail("InitParticle needs particles");
+  const int n = *npart;
+  g_f77.npart = n;
+  g_f77.y.assign(y, y + (size_t)PICL_LRS * n);
+  g_f77.pc.resize(4 * (size_t)n);
+  for (int p = 0; p < n; p++) {  // (rho_p, d_p, V_p, SPL): rprop JRHOP, JDP, JVOLP, JSPL
+    const double *r = rprop + (size_t)PICL_LRP * p;
+    g_f77.pc[4 * (size_t)p] = r[0];
+    g_f77.pc[4 * (size_t)p + 1] = r[2];
+    g_f77.pc[4 * (size_t)p + 2] = r[3];
+    g_f77.pc[4 * (size_t)p + 3] = r[21];
+  }
+  (void)filt2; (void)filt3;
+  g_f77.linit = true;
+}
+
+// ppiclf_solve.f:1579-1640
+void ppiclf_solve_initboxfilter_(const double *filt, const int *iwallm, const int *sngl_elem) {
+  API_LOCK;
+  (void)filt;
+  if (!g_f77.linit) return f77_fail("InitParticle must be before InitFilter");
+  if (*iwallm != 0) return f77_fail("the wall mirror (iwallm 1) is not built");
+  if (*sngl_elem != 1) return f77_fail("the multi-element box filter (sngl_elem 0) is not built");
+  g_f77.proj = 0;
+  g_f77.lfilt = true;
+}
+
+// ppiclf_solve.f:1520-1576
+void ppiclf_solve_initgaussianfilter_(const double *filt, const double *alpha, const int *iwallm) {
+  API_LOCK;
+  if (!g_f77.linit) return f77_fail("InitParticle must be before InitFilter");
+  if (*iwallm != 0) return f77_fail("the wall mirror (iwallm 1) is not built");
+  g_f77.proj = 1;
+  g_f77.width = *filt;
+  g_f77.alpha = *alpha;
+  g_f77.lfilt = true;
+}
+
+// ppiclf_solve.f:339-380
+void ppiclf_solve_initneighborbin_(const double *rwidth) {
+  API_LOCK;
+  if (!g_f77.linit) return f77_fail("InitParticle must be before InitNeighborBin");
+  g_f77.nbw = *rwidth;  // the collision search sizes its own cubes
+}
+
+// ppiclf_comm.f:926-1000: the element nodes must be the bound region's hex
+// grid, node (i,j,k) = c2v entry NODE2V[(i-1) + 2(j-1) + 4(k-1)]
+// (PICL_TEMP_InitSolver.F90:538-597)
+void ppiclf_comm_initoverlapmesh_(const int *ncell, const int *lx1, const int *ly1, const int *lz1,
+                                  const double *xgrid, const double *ygrid, const double *zgrid) {
+  API_LOCK;
+  rflu_handle *h = g_f77.h;
+  if (!g_f77.linit) return f77_fail("InitParticle must be before InitOverlap");
+  if (!g_f77.lfilt) return f77_fail("a filter (InitBoxFilter or InitGaussianFilter) must be set before InitOverlap");
+  if (*lx1 != 2 || *ly1 != 2 || *lz1 != 2) return f77_fail("LX1, LY1, LZ1 must be 2 (PPICLF_LEX/LEY/LEZ)");
+  const int N = h->mesh.nCells;
+  if (*ncell != N) return f77_fail("InitOverlap's ncell must be the bound region's cell count");
+  if (h->c2v.size() != 8 * (size_t)N) return f77_fail("the bound region must be an all-hex grid");
+  for (int e = 0; e < N; e++)
+    for (int n = 0; n < 8; n++) {
+      const double *xv = &h->xyz[3 * (size_t)h->c2v[8 * (size_t)e + NODE2V[n]]];
+      size_t q = 8 * (size_t)e + n;
+      if (xgrid[q] != xv[0] || ygrid[q] != xv[1] || zgrid[q] != xv[2])
+        return f77_fail("InitOverlap's grid is not the bound region's grid (element " + std::to_string(e + 1) + ")");
+    }
+  picl_params p = g_f77.pp;
+  p.projMode = g_f77.proj;
+  p.filterWidth = g_f77.width;
+  p.filterAlpha = g_f77.alpha;
+  p.periodicX = g_f77.per[0];
+  p.periodicY = g_f77.per[1];
+  p.periodicZ = g_f77.per[2];
+  if (int rc = picl_gpu_init(h, &p, g_f77.npart, g_f77.y.data(), g_f77.pc.data()))
+    return f77_fail("InitOverlap: " + h->err + " (code " + std::to_string(rc) + ")");
+  g_f77.fld.assign((size_t)PICL_NFLD * N, 0.0);
+  g_f77.have.assign(PICL_NFLD, 0);
+  g_f77.overlap = true;
+  g_f77.stage = 0;
+  g_f77.icnt = 0;
+  g_f77.proValid = false;
+}
+
+// ppiclf_solve.f:2020-2058.  A field that no built model reads is accepted
+// and dropped.  With IDW interpolation the element value is LocalInterp's
+// mean of the 8 node values (ppiclf_solve.f:2363-2373); the trilinear
+// evaluation takes element-constant fields (PICL_TEMP_Runge fills all 8
+// nodes of a cell with its value).
+void ppiclf_solve_interpfielduser_(const int *jp, const double *infld) {
+  API_LOCK;
+  if (!g_f77.overlap) return f77_fail("InitOverlap must come before InterpFieldUser");
+  g_f77.icnt++;
+  if (g_f77.icnt > 30) return f77_fail("Interpolating too many fields (PPICLF_LRP_INT 30)");
+  if (*jp <= 0 || *jp > PICL_LRP) return f77_fail("Invalid particle array interp. location " + std::to_string(*jp));
+  int k = -1;
+  for (int q = 0; q < PICL_NFLD; q++)
+    if (F77_FIELD_RPROP[q] == *jp) k = q;
+  if (k < 0) return;
+  const int N = g_f77.h->mesh.nCells;
+  const bool tri = g_f77.h->picl->phys.interpMode == 1;
+  double *dst = g_f77.fld.data() + (size_t)k * N;
+  for (int c = 0; c < N; c++) {
+    const double *v = infld + 8 * (size_t)c;
+    if (tri) {
+      for (int n = 1; n < 8; n++)
+        if (v[n] != v[0])
+          return f77_fail("trilinear interpolation takes element-constant fields (rprop " + std::to_string(*jp) +
+                          ", element " + std::to_string(c + 1) + ")");
+      dst[c] = v[0];
+    } else {
+      double s = 0.0;
+      for (int n = 0; n < 8; n++) s = s + v[n];
+      dst[c] = s / 8;
+    }
+  }
+  g_f77.have[k] = 1;
+}
+
+// ppiclf_solve.f:1694-1745 with imethod 2: one RK stage of
+// IntegrateRK3s_Rocflu (the stage counter cycles 1, 2, 3 over calls)
+void ppiclf_solve_integrateparticle_(const int *istep, const int *iostep, const double *dt, const double *time) {
+  API_LOCK;
+  (void)istep; (void)iostep; (void)time;
+  if (!g_f77.overlap) return f77_fail("InitOverlap must come before IntegrateParticle");
+  rflu_handle *h = g_f77.h;
+  for (int k = 0; k < PICL_NFLD; k++)
+    if (!g_f77.have[k])
+      return f77_fail("IntegrateParticle: field rprop " + std::to_string(F77_FIELD_RPROP[k]) +
+                      " was not given by InterpFieldUser this stage");
+  const int N = h->mesh.nCells;
+  std::vector<double> dev((size_t)PICL_NFLD * N);
+  for (int i = 0; i < N; i++)
+    for (int k = 0; k < PICL_NFLD; k++) dev[(size_t)PICL_NFLD * i + k] = g_f77.fld[(size_t)k * N + h->perm[i]];
+  PiclDev &P = h->picl->P;
+  g_f77.stage = g_f77.stage % 3 + 1;
+  if (hipMemcpyAsync(P.fld, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(h->d.scal + S_DTMIN, dt, sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+    return f77_fail("IntegrateParticle: upload failed");
+  picl_particle_stage(h, g_f77.stage);
+  if (hipGetLastError() != hipSuccess) return f77_fail("IntegrateParticle: launch failed");
+  if (int rc = check_err(h)) return f77_fail("IntegrateParticle: " + h->err + " (code " + std::to_string(rc) + ")");
+  g_f77.icnt = 0;
+  std::fill(g_f77.have.begin(), g_f77.have.end(), 0);
+  g_f77.proValid = false;
+}
+
+// ppiclf_solve.f:3630-3647: ppiclf_pro_fld(i,j,k,e,m); the projected fields
+// built are PHIP, FX, FY, FZ, E (m = 1..5)
+void ppiclf_solve_getprofldijkef_(const int *i, const int *j, const int *k, const int *e, const int *m, double *fld) {
+  API_LOCK;
+  if (!g_f77.overlap) return f77_fail("InitOverlap must come before GetProFldIJKEF");
+  rflu_handle *h = g_f77.h;
+  const int N = h->mesh.nCells;
+  if (*i < 1 || *i > 2 || *j < 1 || *j > 2 || *k < 1 || *k > 2 || *e < 1 || *e > N)
+    return f77_fail("GetProFldIJKEF: node or element out of range");
+  if (*m < 1 || *m > 5)
+    return f77_fail("GetProFldIJKEF: projected field " + std::to_string(*m) +
+                    " is not built (PHIP, FX, FY, FZ, E are)");
+  if (!g_f77.proValid) {
+    g_f77.pro.resize(40 * (size_t)N);
+    if (int rc = picl_gpu_get_nodes(h, g_f77.pro.data()))
+      return f77_fail("GetProFldIJKEF: " + h->err + " (code " + std::to_string(rc) + ")");
+    g_f77.proValid = true;
+  }
+  *fld = g_f77.pro[40 * (size_t)(*e - 1) + 8 * (size_t)(*m - 1) + (*i - 1) + 2 * (*j - 1) + 4 * (*k - 1)];
 }
 
 }  // extern "C"

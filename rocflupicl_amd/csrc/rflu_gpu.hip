@@ -34,6 +34,7 @@
 
 #include "../../include/rflu_gpu.h"
 #include "../../include/picl_gpu.h"
+#include "../../include/ppiclf_gpu.h"
 #include "rflu_mesh.hpp"
 
 #define BLK 256

@@ -18,7 +18,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void|rflu_handle \*)\s*\*?\s*(\w+)\s*\(", text,
+                                 flags=re.M)))
 
 
 @pytest.fixture(scope="module")
@@ -32,9 +33,10 @@ def built_lib():
 def test_header_declarations_match_loader_lists():
     assert declared("rflu_gpu.h") == sorted(_lib.RFLU_SYMBOLS)
     assert declared("picl_gpu.h") == sorted(_lib.PICL_SYMBOLS)
+    assert declared("ppiclf_gpu.h") == sorted(_lib.PPICLF_SYMBOLS)
 
 
-@pytest.mark.parametrize("header", ["rflu_gpu.h", "picl_gpu.h"])
+@pytest.mark.parametrize("header", ["rflu_gpu.h", "picl_gpu.h", "ppiclf_gpu.h"])
 def test_library_exports_every_declared_symbol(header, built_lib):
     exported = _lib.exported_symbols()
     missing = [s for s in declared(header) if s not in exported]
@@ -43,5 +45,5 @@ def test_library_exports_every_declared_symbol(header, built_lib):
 
 def test_library_opens_without_a_gpu(built_lib):
     L = ctypes.CDLL(built_lib)
-    for s in declared("rflu_gpu.h") + declared("picl_gpu.h"):
+    for s in declared("rflu_gpu.h") + declared("picl_gpu.h") + declared("ppiclf_gpu.h"):
         assert hasattr(L, s)
