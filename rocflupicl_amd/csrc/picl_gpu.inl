@@ -96,7 +96,7 @@ struct PiclDev {
   int *binCnt = nullptr;    // particles per centroid bin (this stage)
   int *gBox = nullptr;      // [6] lowest x, y, z and highest x, y, z centroid bin holding particles (this stage)
   int *gvList = nullptr;    // vertices within reach of a bin holding particles (this stage), gvList[nV] = count
-  double *gq = nullptr;     // [nP][8] by sorted position: x, y, z, map * multfci (5)
+  double *gq = nullptr;     // by sorted position: [nP][4] x, y, z, map0 * multfci, then [nP][4] map1..4 * multfci
   double *xo = nullptr;     // [3][nP] position at the start of the stage, by current slot
   double *vproj = nullptr;  // [5][nV]
   int *vperm = nullptr;     // vertices in Morton order (the order of the deposit's vertex list)
@@ -1317,11 +1317,14 @@ __global__ void __launch_bounds__(DEP_T) k_deposit(PiclDev P, int posXo) {
     P.dest[t] = b + q;
     if (P.gauss) {
       const double *xs = posXo ? P.xo : C.st;
-      double2 *g = reinterpret_cast<double2 *>(P.gq + 8 * (size_t)(b + q));
+      // the position record is what the deposit's candidate tests read:
+      // 32 bytes, four to a 128-byte line
+      double2 *g = reinterpret_cast<double2 *>(P.gq + 4 * (size_t)(b + q));
+      double2 *gm = reinterpret_cast<double2 *>(P.gq + 4 * (nP + (size_t)(b + q)));
       g[0] = make_double2(xs[t], xs[nP + t]);
       g[1] = make_double2(xs[2 * nP + t], map[0] * P.gMult);
-      g[2] = make_double2(map[1] * P.gMult, map[2] * P.gMult);
-      g[3] = make_double2(map[3] * P.gMult, map[4] * P.gMult);
+      gm[0] = make_double2(map[1] * P.gMult, map[2] * P.gMult);
+      gm[1] = make_double2(map[3] * P.gMult, map[4] * P.gMult);
     }
   }
 #pragma unroll
@@ -1471,9 +1474,11 @@ __device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
 __device__ __forceinline__ void gauss_batch(const PiclDev &P, const int *Qp, const double *Qr, int n, int lane,
                                             double acc[5], double cmp[5]) {
   if (lane < n) {
-    const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)Qp[lane]);
+    const size_t at = Qp[lane];
+    const double2 *g = reinterpret_cast<const double2 *>(P.gq + 4 * at);
+    const double2 *gm = reinterpret_cast<const double2 *>(P.gq + 4 * ((size_t)P.nP + at));
     double rexp = exp(Qr[lane] * P.gRdum);
-    double2 g1 = g[1], g2 = g[2], g3 = g[3];
+    double2 g1 = g[1], g2 = gm[0], g3 = gm[1];
     two_sum_acc(acc[0], cmp[0], g1.y * rexp);
     two_sum_acc(acc[1], cmp[1], g2.x * rexp);
     two_sum_acc(acc[2], cmp[2], g2.y * rexp);
@@ -1502,7 +1507,7 @@ __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, cons
       double rdist2 = 0.0;
       if (p < cnt) {
         pos = first + p;
-        const double2 *g = reinterpret_cast<const double2 *>(P.gq + 8 * (size_t)pos);
+        const double2 *g = reinterpret_cast<const double2 *>(P.gq + 4 * (size_t)pos);
         double2 g0 = g[0], g1 = g[1];
         // the image's coordinates as ppiclF's ghost holds them, x_p + s
         double dx = xv[0] - (sh[0] == 0.0 ? g0.x : g0.x + sh[0]);
