@@ -15,6 +15,15 @@ post-processors) read back.
   CHARACTER(80), CHRLEN in modfloflu/ModDataTypes.F90:73; integers 4 bytes).
   Little-endian, as the reference's FORMAT_BINARY and FORMAT_BINARY_L write
   on x86-64.
+* ASCII dependent-variable file ``<case>.mixt.dva_...``, written beside
+  every ASCII flow file (RFLU_WriteFlowASCII, :2228-2230, 2323-2352): the
+  same header, dimensions (cells, 4) and four sections.  A perfect gas has
+  three dependent variables (p, T, c; nDv = 3,
+  libflu/RFLU_DerivedInputValues.F90:186-193), so the reference's fourth
+  section reads dv(4, j) past the end of each 3-row column of its
+  column-major array: dv(1, j+1), the next cell's pressure.  That is
+  written here too; for the last cell the read lies beyond the array and 0
+  is written.  The binary writer has no such companion.
 * Restart info ``<case>.rin``: one line per written solution, the physical
   time, list-directed (RFLU_WriteRestartInfo.F90), appended.
 
@@ -86,6 +95,23 @@ def write_flow_ascii(path, cv, time, res_init=0.0):
     lines.append("# End")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def dv_sections(dv):
+    """The four dv sections the reference writes for a perfect gas: p, T, c
+    and dv(4, j) read from the 3-row array's storage (the next cell's p;
+    0 past the last cell)."""
+    dv = np.asarray(dv, np.float64)
+    n = dv.shape[0]
+    flat = np.ascontiguousarray(dv).reshape(-1)  # (p, T, c) per cell: the Fortran column-major dv(3, n)
+    fourth = np.zeros(n)
+    fourth[:n - 1] = flat[3:3 * n:3]
+    return np.column_stack([dv[:, 0], dv[:, 1], dv[:, 2], fourth])
+
+
+def write_dv_ascii(path, dv, time, res_init=0.0):
+    """RFLU_WriteFlowASCII's .mixt.dva file for dv (nCells, 3) = (p, T, c)."""
+    write_flow_ascii(path, dv_sections(dv), time, res_init)
 
 
 def read_flow_ascii(path):

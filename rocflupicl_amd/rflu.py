@@ -271,12 +271,18 @@ class Region:
 
     # ---- flow files (RFLU_ModReadWriteFlow) ------------------------------
     def write_flow(self, case, directory="", region=1, binary=True):
-        """RFLU_WriteFlowBinary / RFLU_WriteFlowASCII of the current state and
-        RFLU_WriteRestartInfo; returns the file name."""
+        """RFLU_WriteFlowBinary / RFLU_WriteFlowASCII of the current state (the
+        ASCII form with its .mixt.dva dependent-variable companion) and
+        RFLU_WriteRestartInfo; returns the flow file's name."""
         from . import flowio
         t = self.time()[0]
         name = flowio.file_name(case, ".mixt.cv" if binary else ".mixt.cva", region, t, directory)
-        (flowio.write_flow_binary if binary else flowio.write_flow_ascii)(name, self.get_state()[0], t)
+        cv, dv = self.get_state()
+        if binary:
+            flowio.write_flow_binary(name, cv, t)
+        else:
+            flowio.write_flow_ascii(name, cv, t)
+            flowio.write_dv_ascii(flowio.file_name(case, ".mixt.dva", region, t, directory), dv, t)
         flowio.append_restart_info(case, t, directory)
         return name
 

@@ -73,3 +73,22 @@ def test_particle_vtu_layout_and_round_trip(tmp_path):
     assert np.array_equal(d["rprop"], rprop.astype(np.float32).astype(np.float64))
     assert np.array_equal(d["position"], y[:, :3].astype(np.float32).astype(np.float64))
     assert np.array_equal(d["tags"], tags)
+
+
+def test_dependent_variable_file_layout(tmp_path):
+    """The .mixt.dva companion: header as the flow file, dimensions (cells, 4),
+    p, T, c, then dv(4, j) read past each 3-row column (the next cell's p)."""
+    rng = np.random.default_rng(6)
+    n = 7
+    dv = np.column_stack([1e5 * (1 + rng.random(n)), 300 * (1 + rng.random(n)), 340 * (1 + rng.random(n))])
+    p = tmp_path / flowio.file_name("case", ".mixt.dva", 1, 2.5e-4)
+    flowio.write_dv_ascii(p, dv, 2.5e-4)
+    text = p.read_text().splitlines()
+    assert text[0] == "# ROCFLU flow file (v2)"
+    dims = text[text.index("# Dimensions") + 1]
+    assert dims == f"{n:16d}{4:16d}"
+    assert [ln for ln in text if ln.startswith("# ") and ln[2:].strip().isdigit()] == ["#  1", "#  2", "#  3", "#  4"]
+    d = flowio.read_flow_ascii(p)
+    assert d["cv"].shape == (n, 4) and d["time"] == 2.5e-4
+    assert np.allclose(d["cv"][:, :3], dv, rtol=1e-15, atol=0)
+    assert np.array_equal(d["cv"][:n - 1, 3], d["cv"][1:, 0]) and d["cv"][n - 1, 3] == 0.0

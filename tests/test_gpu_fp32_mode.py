@@ -8,11 +8,13 @@ components are zero but for the mesh jitter, and a component-wise ratio
 would divide by that noise (in FP64, perturbing the initial state by 1e-7
 already moves them by ~1e-5 of their own norm).
 
-Tolerances: 1e-6 on the Mach 1.5 tet workload; 3e-6 on the Mach 2 hex box
-after 40 steps.  The FP64 solution itself amplifies perturbations at the
-shock (a 1e-7 relative perturbation of the initial state grows to 3e-7 on
-the hex box over 40 steps), so FP32 rounding of every flux accumulates to
-~1e-6 there.
+The target is 1e-6 (TOL).  It holds on the Mach 1.5 tet workload after 40
+steps and on the Mach 2 hex box after 20 steps.  It is NOT met on the Mach 2
+hex box after 40 steps: measured 1.16e-6.  The FP64 solution itself amplifies
+perturbations at the shock (a 1e-7 relative perturbation of the initial
+state grows to 3e-7 on that box over 40 steps), so FP32 rounding of every
+flux accumulates past 1e-6 there.  That case is kept as a regression bound
+on the measured miss (MISS_HEX_MACH2_40), not as a tolerance the mode meets.
 """
 import numpy as np
 import pytest
@@ -22,7 +24,7 @@ from rocflupicl_amd import configs, rflu
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-TOL_HEX_MACH2 = 3e-6
+MISS_HEX_MACH2_40 = 1.5e-6  # Mach 2 hex box, 40 steps: the target is missed (measured 1.16e-6)
 
 
 def rel_l2(a, b):
@@ -50,13 +52,23 @@ def test_fp32_mode_tracks_fp64_on_tets(discr):
     assert err <= TOL, err
 
 
-def test_fp32_mode_on_hexes():
+def test_fp32_mode_on_hexes_meets_the_target_for_20_steps():
+    mesh, params, ic = configs.shock_hex(nx=40, ny=8, nz=8, limiterGuard=1)
+    a = run(mesh, params, ic, 20, 0)
+    b = run(mesh, params, ic, 20, 1)
+    err = rel_l2(b.get_state()[0], a.get_state()[0])
+    print(f"hex AUSM+ 20 steps: rel L2 {err:.3e}")
+    assert err <= TOL, err
+
+
+def test_fp32_mode_on_hexes_misses_the_target_at_40_steps():
+    """Documents the miss: after 40 steps the Mach 2 hex box is past 1e-6."""
     mesh, params, ic = configs.shock_hex(nx=40, ny=8, nz=8, limiterGuard=1)
     a = run(mesh, params, ic, 40, 0)
     b = run(mesh, params, ic, 40, 1)
     err = rel_l2(b.get_state()[0], a.get_state()[0])
-    print(f"hex AUSM+: rel L2 {err:.3e}")
-    assert err <= TOL_HEX_MACH2, err
+    print(f"hex AUSM+ 40 steps: rel L2 {err:.3e} (target 1e-6 not met)")
+    assert err <= MISS_HEX_MACH2_40, err
 
 
 def test_fp32_mode_graph_equals_eager():
