@@ -118,12 +118,21 @@ def profiled_traffic(kernel, workload, fp32=False):
 
 
 def profiled_valu(kernel, workload, fp32=False):
-    """Wave-level VALU instructions per launch of `kernel` (SQ_INSTS_VALU)
-    from the committed counter summary, or None."""
-    for cells in _profile_rows(workload, fp32, 8):
-        if cells[0].startswith(kernel):
+    """Wave-level VALU instructions per launch of `kernel` (the SQ_INSTS_VALU
+    column of the committed counter summary's shader-sequencer table), or
+    None."""
+    name = PROFILES.get((workload, fp32))
+    if name is None or not os.path.exists(os.path.join(ROOT, "profiles", name)):
+        return None
+    col = None
+    for line in open(os.path.join(ROOT, "profiles", name)):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if "SQ_INSTS_VALU" in cells:
+            col = cells.index("SQ_INSTS_VALU")
+            continue
+        if col is not None and len(cells) > col and cells[0].startswith(kernel):
             try:
-                return float(cells[2])
+                return float(cells[col])
             except ValueError:
                 return None
     return None
@@ -317,9 +326,42 @@ def run_batch(args):
     print(json.dumps(out))
 
 
+def run_serve_sweep(args):
+    """The serving trace at each arrival rate of --rates (the same seeded
+    requests, Poisson arrival times scaled to the rate): p50/p99 latency and
+    aggregate throughput per rate, one JSON line."""
+    from rocflupicl_amd import serve
+    rows = []
+    for rate in [float(r) for r in args.rates.split(",")]:
+        reqs = serve.trace(args.requests, rate, seed=args.seed)
+        srv = serve.Server(max_active=args.max_active, chunk=args.chunk, builders=args.builders)
+        results, wall = srv.run(reqs)
+        summ = serve.summarize(results, wall)
+        row = dict(rate_per_s=rate, requests=summ["requests"], failed=summ["failed"], wall_s=round(wall, 2),
+                   p50_latency_s=round(summ["p50_latency_s"], 3), p99_latency_s=round(summ["p99_latency_s"], 3),
+                   mean_latency_s=round(summ["mean_latency_s"], 3),
+                   mcell_updates_per_s=round(summ["mcell_updates_per_s"], 2),
+                   particle_updates_per_s=round(summ["particle_updates_per_s"], 1))
+        print(f"# rate {rate}: {json.dumps(row)}", file=sys.stderr, flush=True)
+        rows.append(row)
+    out = {
+        "metric": "Mcell-updates/s/GPU & % HBM roofline; served p50/p99 latency at fixed accuracy",
+        "unit": "Mcell-updates/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic trace (seeded meshes, particle clouds and shock states)",
+        "config": {"workload": "serve_trace_sweep", "requests_per_rate": args.requests, "seed": args.seed,
+                   "cells": "log-uniform 10k-2M", "particles": "0 or uniform 0-500k (half the requests)",
+                   "steps": "uniform 100-2000", "max_active": args.max_active, "chunk": args.chunk,
+                   "model": "RFLU 2nd-order AUSM+ (+ ppiclF QS/AM Parmar + PG)", "parallelism": "single GPU"},
+        "sweep": rows,
+    }
+    print(json.dumps(out))
+
+
 def run_serve(args):
     """Serving trace: Poisson arrivals, continuous batching (rocflupicl_amd.serve)."""
     from rocflupicl_amd import serve
+    if args.rates:
+        return run_serve_sweep(args)
     reqs = serve.trace(args.requests, args.rate, seed=args.seed)
     srv = serve.Server(max_active=args.max_active, chunk=args.chunk, builders=args.builders)
     results, wall = srv.run(reqs)
@@ -460,6 +502,7 @@ def main():
     ap.add_argument("--instances", type=int, default=256, help="batch_ensemble: number of instances")
     ap.add_argument("--requests", type=int, default=24, help="serve_trace: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve_trace: arrivals per second")
+    ap.add_argument("--rates", default="", help="serve_trace: comma-separated arrival rates to sweep")
     ap.add_argument("--seed", type=int, default=5, help="serve_trace: trace seed")
     ap.add_argument("--max-active", type=int, default=8, help="serve_trace: requests in flight")
     ap.add_argument("--chunk", type=int, default=10, help="serve_trace: steps per launch batch")
