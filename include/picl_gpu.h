@@ -42,7 +42,7 @@ typedef struct picl_params {
                         Parmar, 4 Gidaspow; 5 Schiller-Naumann (rocinteract's
                         INRT_DRAG_MODEL_SN law in the same slot, not a
                         ppiclF choice) */
-  int amFlag;        /* 0 off, 1 AM Parmar */
+  int amFlag;        /* 0 off, 1 AM Parmar, 2 AM Briney (neighborWidth below) */
   int pgFlag;        /* 0 off, 1 pressure-gradient force */
   int feedbackFlag;  /* 0 one-way, 1 two-way */
   int interpMode;    /* 0: IDW over the 27 nearest element centroids
@@ -85,6 +85,13 @@ typedef struct picl_params {
    * sets the damping. */
   int collisionalFlag;
   double ksp, erest;
+  /* amFlag 2 (AMFORCE 2, Briney's added mass,
+   * libpicl/user_files/ppiclf_user_AddedMass.f:164-400 with the pair terms of
+   * ppiclf_user_EvalNearestNeighbor.f:115-159): the neighbour width
+   * ppiclf_d2chk(3) within which other particles (and their periodic images)
+   * add pair terms.  rocpicl passes max(4 d_max, NEIGHBORWIDTH)
+   * (PICL_TEMP_InitSolver.F90:655-661); 0 here means 4 d_max. */
+  double neighborWidth;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

@@ -7,6 +7,7 @@ leg, as the checker.  Arrays are returned in the reference's Fortran layout
 import ctypes as C
 import math
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -34,7 +35,7 @@ class OrcPiclParams(C.Structure):
                 ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
                 ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
                 ("viscousUnsteady", C.c_int), ("collisionalFlag", C.c_int), ("ksp", C.c_double),
-                ("erest", C.c_double)]
+                ("erest", C.c_double), ("neighborWidth", C.c_double)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
@@ -55,6 +56,15 @@ def make_picl_params(**kw):
 
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def build_ref():
+    """Compile the reference's Briney added-mass helpers into oracle/_ref
+    (make ref) where the reference tree and amdflang are at hand; elsewhere
+    nothing is built and the golden vectors stand in for them."""
+    ref = os.environ.get("REF", "/root/reference")
+    if os.path.isdir(os.path.join(ref, "libpicl", "user_files", "Added_Mass")) and shutil.which("amdflang"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", "REF=" + ref], check=True)
 
 
 def lib():
@@ -113,6 +123,13 @@ def lib():
         D = C.c_double
         L.orc_picl_collision_force.argtypes = [P, P, D, D, D, P, P, D, D, D, D, D, D, P]
         L.orc_picl_collision_force2.argtypes = [P, P, D, D, D, D, D, D, D, D, D, P, P]
+        for n in ("orc_am_b11_11", "orc_am_b11_22", "orc_am_b12_11", "orc_am_b12_22", "orc_am_rdf"):
+            getattr(L, n).argtypes = [D, D]
+            getattr(L, n).restype = D
+        for n in ("orc_am_ia_analytical", "orc_am_ii_analytical", "orc_am_ia_numerical", "orc_am_ii_numerical"):
+            getattr(L, n).argtypes = [D, D, D]
+            getattr(L, n).restype = D
+        L.orc_am_resistance_pair.argtypes = [D, D, D, D, D, P]
         L.orc_set_threads.argtypes = [I]
         L.orc_get_threads.restype = I
         L.orc_test_rk3_linear.argtypes = [D, D, P]
@@ -404,6 +421,21 @@ def qs_beta(qsFlag, rmachp, rphip, rep, rmu, dp, rhof=1.0, vmag=1.0):
     capped inputs."""
     return lib().orc_picl_qs_beta(int(qsFlag), float(rmachp), float(rphip), float(rep), float(rmu), float(dp),
                                   float(rhof), float(vmag))
+
+
+def am_curve(name, *args):
+    """One function of Briney's added-mass model by its reference name:
+    b11_11, b11_22, b12_11, b12_22, rdf (d, alpha); ia_analytical,
+    ii_analytical, ia_numerical, ii_numerical (rmax, rad, alpha)."""
+    return getattr(lib(), "orc_am_" + name)(*[float(a) for a in args])
+
+
+def am_resistance_pair(x, y, z, alpha, rad):
+    """resistance_pair: the pair's (3, 6) block [B11 | B12] rotated onto the
+    pair axis (x, y, z) = x_j - x_i."""
+    R = np.zeros((3, 6))
+    lib().orc_am_resistance_pair(float(x), float(y), float(z), float(alpha), float(rad), _p(R))
+    return R
 
 
 def set_threads(n):

@@ -27,7 +27,7 @@ enum {
   R_JRHOP = 0, R_JRHOF = 1, R_JDP = 2, R_JVOLP = 3, R_JPHIP = 4, R_JUX = 5, R_JUY = 6, R_JUZ = 7,
   R_JCS = 8, R_JDPDX = 9, R_JDPDY = 10, R_JDPDZ = 11, R_JSDRX = 12, R_JSDRY = 13, R_JSDRZ = 14,
   R_JRHSR = 15, R_JPGCX = 16, R_JPGCY = 17, R_JPGCZ = 18, R_JSPL = 21, R_JT = 23,
-  R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
+  R_WDOTX = 27, R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
 };
 enum { P_JPHIP = 0, P_JFX, P_JFY, P_JFZ, P_JE, NPRO };
 #define LRS 12
@@ -64,6 +64,7 @@ struct orc_picl {
   int per[3];                  /* linear periodicity per axis */
   double perLo[3], perHi[3];   /* ppiclf_xdrange: the mesh's vertex bounds */
   double *fld;     /* ORC_PICL_NFLD interpolation inputs per element */
+  double nbW;      /* ppiclf_d2chk(3): the added-mass pair terms' reach */
 };
 
 void orc__picl_free(struct orc_picl *p) {
@@ -835,6 +836,244 @@ static void collision_sum(const struct orc_picl *p, int i, double rmass, double 
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Briney added mass (AMFORCE 2): libpicl/user_files/Added_Mass/            */
+/* ppiclf_user_AddedMass_functions.f, _subroutines.f; the unary and binary  */
+/* terms of ppiclf_user_AddedMass.f:164-400; the pair terms of              */
+/* ppiclf_user_EvalNearestNeighbor.f:115-159.                               */
+/* Integer powers as gcc expands them: x**5 = x^3 x^2, x**6 = x^3 x^3,      */
+/* x**8 = x^4 x^4, x**9 = x^6 x^3 (x^3 = x^2 x, x^4 = x^2 x^2).             */
+/* ------------------------------------------------------------------------ */
+static double ip6(double x) { double x3 = (x * x) * x; return x3 * x3; }
+static double ip8(double x) { double x2 = x * x, x4 = x2 * x2; return x4 * x4; }
+static double ip9(double x) { double x3 = (x * x) * x, x6 = x3 * x3; return x6 * x3; }
+
+/* the four fitted curves of the binary model (_functions.f:48-148) */
+double orc_am_b11_11(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * ((double)(3.0f / 64.0f) * ip6(t) + (double)(9.0f / 256.0f) * ip8(t));
+  double hot = F32(0.059) / (((d - F32(1.9098)) * ip2(d - F32(0.4782))) * ip3(d));
+  double alpha_corr = ((alpha * alpha - F32(0.0902) * alpha) * F32(70.7731)) / ip6(d + F32(2.0936));
+  return (asym + hot) + alpha_corr;
+}
+
+double orc_am_b11_22(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * ((double)(3.0f / 256.0f) * ip6(t) + (double)(3.0f / 256.0f) * ip8(t));
+  double A = (F32(0.0003) + (F32(0.0262) * alpha) * alpha) - F32(0.0012) * alpha;
+  double B = (F32(1.3127) + (F32(1.0401) * alpha) * alpha) - F32(1.2519) * alpha;
+  double hot = A / ip6(d - B);
+  return asym + hot;
+}
+
+double orc_am_b12_11(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * (-((double)(3.0f / 8.0f) * ip3(t)) - (double)(3.0f / 512.0f) * ip9(t));
+  double hot = -(F32(0.0006) / ip5(d - F32(1.5428)));
+  double alpha_corr = -((F32(0.7913) * alpha) * exp(-((F32(0.9801) - F32(0.1075) * alpha) * d)));
+  return (asym + hot) + alpha_corr;
+}
+
+double orc_am_b12_22(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * ((double)(3.0f / 16.0f) * ip3(t) + (double)(3.0f / 4096.0f) * ip9(t));
+  double hot = (-F32(0.1985) + F32(16.7372) * alpha) / ip6(d) + (F32(1.3907) - F32(48.2604) * alpha) / ip8(d);
+  return asym + hot;
+}
+
+/* distant-neighbour corrections with g(r) = 1 (_functions.f:160-245) */
+double orc_am_ia_analytical(double rmax, double rad, double alpha) {
+  double m = rmax / rad;
+  double term1 = (5.0 * ip2(m) + 9.0) / (10.0 * ip5(m));
+  double l1 = log(m - F32(1.9098)), l2 = log(m - F32(0.4782));
+  double num = (((F32(0.00720826) - F32(0.0150737) * m) * l1 - (F32(0.120023) * m) * l2) + F32(0.057395) * l2 +
+                (F32(0.135097) * m - F32(0.0646033)) * log(m)) - F32(0.0861828);
+  double term2 = num / (m - F32(0.4782));
+  double term3 = ((ip2(alpha) - F32(0.0902) * alpha) * ((F32(0.333333) * ip2(m) + F32(0.348933) * m) + F32(0.146105))) /
+                 ip5(m + F32(2.0936));
+  double B11 = (term1 + term2) + term3;
+  term1 = (5.0 * ip2(m) + 12.0) / (40.0 * ip5(m));
+  double numerator = (F32(0.0262) * ip2(alpha) - F32(0.0012) * alpha) + F32(0.0003);
+  double c = (-(F32(1.0401) * ip2(alpha)) + F32(1.2519) * alpha) - F32(1.3127);
+  term2 = (numerator * ((10.0 * ip2(m) + (5.0 * m) * c) + ip2(c))) / (30.0 * ip5(m + c));
+  double B22 = term1 + term2;
+  return alpha * (B11 + 2.0 * B22);
+}
+
+double orc_am_ii_analytical(double rmax, double rad, double alpha) {
+  double m = rmax / rad;
+  double term1 = -(1.0 / (4.0 * ip6(m)));
+  double term2 = -((F32(6.0e-4) * ((0.5 * ip2(m) - F32(0.516067) * m) + F32(0.199744))) / ip4(F32(1.5482) - m));
+  double prefactor = -(F32(0.7913) * alpha);
+  double c = F32(0.9801) - F32(0.1075) * alpha;
+  double term3 = ((prefactor * ((m * c) * ((m * c) + 2.0) + 2.0)) * exp(-(m * c))) / ip3(c);
+  double B11 = (term1 + term2) + term3;
+  term1 = 1.0 / (32.0 * ip6(m));
+  term2 = (-F32(0.1985) + F32(16.7372) * alpha) / (3.0 * ip3(m));
+  term3 = (F32(1.3907) - F32(48.2604) * alpha) / (5.0 * ip5(m));
+  double B22 = (term1 + term2) + term3;
+  return alpha * (B11 + 2.0 * B22);
+}
+
+/* radial distribution function of hard spheres (Trokhymchuk et al. 2005,
+ * with its erratum), r = distance / diameter (_functions.f:262-347) */
+double orc_am_rdf(double r, double alpha) {
+  double eta = alpha;
+  double d = pow((2.0 * eta) * (((eta * eta - 3.0 * eta) - 3.0) +
+                                sqrt(3.0 * ((((ip4(eta) - 2.0 * ip3(eta)) + ip2(eta)) + 6.0 * eta) + 3.0))),
+                 (double)(1.0f / 3.0f));
+  double f = (2.0 * eta) / (1.0 - eta);
+  double mu = f * ((-1.0 - d / (2.0 * eta)) + eta / d);
+  double alpha0 = f * ((-1.0 + d / (4.0 * eta)) - eta / (2.0 * d));
+  double beta0 = (f * (double)sqrtf(3.0f)) * (-(d / (4.0 * eta)) - eta / (2.0 * d));
+  double a2b2 = ip2(alpha0) + ip2(beta0);
+  double denom_gamma = ((a2b2 - (2.0 * mu) * alpha0) * (1.0 + 0.5 * eta)) - mu * (1.0 + 2.0 * eta);
+  double X = (alpha0 * a2b2 - mu * (ip2(alpha0) - ip2(beta0))) * (1.0 + 0.5 * eta) +
+             (a2b2 - mu * alpha0) * (1.0 + 2.0 * eta);
+  double gamma = atan(-(((1.0 / beta0) * X) / denom_gamma));
+  double omega = (-(F32(0.682) * exp(-(F32(24.697) * eta))) + F32(4.72)) + F32(4.45) * eta;
+  double kappa = F32(4.674) * exp(-(F32(3.935) * eta)) + F32(3.536) * exp(-(F32(56.27) * eta));
+  double alpha1 = ((F32(44.554) + F32(79.868) * eta) + (F32(116.432) * eta) * eta) - F32(44.652) * exp(2.0 * eta);
+  double beta = (-F32(5.022) + F32(5.857) * eta) + F32(5.089) * exp(-(4.0 * eta));
+  double rstar = (F32(2.0116) - F32(1.0647) * eta) + (F32(0.0538) * eta) * eta;
+  double gm = ((((F32(1.0286) - F32(0.6095) * eta) + F32(3.5781) * ip2(eta)) - F32(21.3651) * ip3(eta)) +
+               F32(42.6344) * ip4(eta)) - F32(33.8485) * ip5(eta);
+  const double f23 = (double)(2.0f / 3.0f);
+  double gsig = (1.0 / (4.0 * eta)) *
+                (((((1.0 + eta) + ip2(eta)) - f23 * ip3(eta)) - f23 * ip4(eta)) / ip3(1.0 - eta) - 1.0);
+  double Brad = ((gm - (gsig / rstar) * exp(mu * (rstar - 1.0))) * rstar) /
+                (cos(beta * (rstar - 1.0) + gamma) * exp(alpha1 * (rstar - 1.0)) - cos(gamma) * exp(mu * (rstar - 1.0)));
+  double Arad = gsig - Brad * cos(gamma);
+  double delta = -(omega * rstar) - atan((kappa * rstar + 1.0) / (omega * rstar));
+  double Crad = ((rstar * (gm - 1.0)) * exp(kappa * rstar)) / cos(omega * rstar + delta);
+  double g2 = (Arad / r) * exp(mu * (r - 1.0)) + ((Brad / r) * cos(beta * (r - 1.0) + gamma)) * exp(alpha1 * (r - 1.0));
+  if (r > rstar)
+    g2 = 1.0 + ((Crad / r) * cos(omega * r + delta)) * exp(-(kappa * r));
+  else if (r < 1.0)
+    g2 = 0.0;
+  return g2;
+}
+
+/* the integrands and Simpson sums used when rmax / rad < 3.49
+ * (_functions.f:356-513); pi is 4*ATAN(1.0) in single precision */
+static double am_integrand(int induced, double d, double rad, double alpha) {
+  const double pi = (double)(4.0f * atanf(1.0f));
+  double g = orc_am_rdf(d / (2.0 * rad), alpha);
+  double f11 = induced ? orc_am_b12_11(d / rad, alpha) : orc_am_b11_11(d / rad, alpha);
+  double f22 = induced ? orc_am_b12_22(d / rad, alpha) : orc_am_b11_22(d / rad, alpha);
+  double rho = alpha / ((double)(4.0f / 3.0f) * pi);
+  double w = ((((double)(1.0f / 3.0f) * rho) * 4.0) * pi) * ip2(d / rad);
+  return (w * g) * (f11 + 2.0 * f22);
+}
+
+static double am_numerical(int induced, double rmax, double rad, double alpha) {
+  const int npts = 1001;
+  const double coef[2] = {4.0, 2.0};
+  double maxr = rad * 20.0;
+  if (!(maxr > rmax)) return 0.0;
+  double dr = (maxr - rmax) / (double)(npts + 1);
+  double integral = 0.0, r = rmax;
+  integral = integral + am_integrand(induced, r, rad, alpha);
+  integral = integral + am_integrand(induced, maxr, rad, alpha);
+  for (int i = 2; i <= npts - 1; i++) {
+    r = r + dr;
+    double f = am_integrand(induced, r, rad, alpha);
+    integral = integral + coef[i % 2] * f; /* 4, 2, 4, 2, ... */
+  }
+  return (dr * integral) / 3.0;
+}
+
+double orc_am_ia_numerical(double rmax, double rad, double alpha) { return am_numerical(0, rmax, rad, alpha); }
+double orc_am_ii_numerical(double rmax, double rad, double alpha) { return am_numerical(1, rmax, rad, alpha); }
+
+/* resistance_pair (_subroutines.f:37-143): B11 and B12 of a pair at
+ * (x, y, z) = x_j - x_i rotated from the pair axis, Qt (B Q) with matmul's
+ * k-ascending sums.  R[k][l] = the (k, l) entry of B11 (l < 3) or B12 (l >= 3). */
+void orc_am_resistance_pair(double x, double y, double z, double alpha, double rad, double R[3][6]) {
+  double gam = atan2(y, x), bet = -atan2(z, sqrt(x * x + y * y));
+  double Q[3][3] = {{cos(bet) * cos(gam), cos(bet) * sin(gam), -sin(bet)},
+                    {-sin(gam), cos(gam), 0.0},
+                    {sin(bet) * cos(gam), sin(bet) * sin(gam), cos(bet)}};
+  double dist = sqrt(x * x + y * y + z * z) / rad;
+  double b1[3] = {orc_am_b11_11(dist, alpha), orc_am_b11_22(dist, alpha), 0.0};
+  double b2[3] = {orc_am_b12_11(dist, alpha), orc_am_b12_22(dist, alpha), 0.0};
+  b1[2] = b1[1];
+  b2[2] = b2[1];
+  for (int blk = 0; blk < 2; blk++) {
+    const double *b = blk ? b2 : b1;
+    double M[3][3]; /* matmul(B, Q): B diagonal */
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++) s = s + (k == i ? b[i] : 0.0) * Q[k][j];
+        M[i][j] = s;
+      }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++) s = s + Q[k][i] * M[k][j];
+        R[i][3 * blk + j] = s;
+      }
+  }
+}
+
+/* EvalNearestNeighbor's added-mass part for neighbour j of i: the pair
+ * terms into Fam and the neighbour's acceleration into Wsum. */
+static void am_pair(const double *yi, const double *rpi_, const double *yj, const double *rpj, double rphip,
+                    double Fam[3], double Wsum[3], int *nn) {
+  double rthresh = 0.5 * (rpi_[R_JDP] + rpj[R_JDP]);
+  double rx = yj[JX] - yi[JX], ry = yj[JY] - yi[JY], rz = yj[JZ] - yi[JZ];
+  double rdiff = sqrt((ip2(rx) + ip2(ry)) + ip2(rz));
+  if (rdiff < rthresh) rx = ry = rz = rthresh; /* overlapping: the reference's substitute */
+  double alpha_local = fmin(F32(0.4), rphip);
+  double rad = 0.5 * rpi_[R_JDP];
+  double R[3][6];
+  orc_am_resistance_pair(rx, ry, rz, alpha_local, rad, R);
+  *nn += 1;
+  for (int k = 0; k < 3; k++) {
+    for (int l = 0; l < 3; l++) {
+      Fam[k] = Fam[k] + R[k][l] * rpi_[R_WDOTX + l];
+      Fam[k] = Fam[k] + R[k][l + 3] * rpj[R_WDOTX + l];
+    }
+    Wsum[k] = Wsum[k] + rpj[R_WDOTX + k];
+  }
+}
+
+/* ppiclf_solve_NearestNeighbor (ppiclf_solve.f:470-560) for the added mass:
+ * the other particles by ascending index, then the periodic images, those
+ * within the neighbour width (the squared-distance filter, then
+ * EvalNearestNeighbor's rdiff <= d2chk(3)). */
+static int am_neighbours(const struct orc_picl *p, int i, double rphip, double Fam[3], double Wsum[3]) {
+  const double *yi = p->y + LRS * (size_t)i, *rpi_ = p->rprop + LRP * (size_t)i;
+  const double w = p->nbW, dist2 = w * w;
+  int nn = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int j = 0; j < p->n; j++) {
+      if (p->lost[j] || (pass == 0 && j == i)) continue;
+      for (int img = 0; img < (pass ? 27 : 1); img++) {
+        if (pass && img == 13) continue;
+        double yj[LRS];
+        memcpy(yj, p->y + LRS * (size_t)j, sizeof yj);
+        int skip = 0;
+        if (pass)
+          for (int d = 0, q = img; d < 3; d++, q /= 3) {
+            int sd = q % 3 - 1;
+            if (sd != 0 && !p->per[d]) skip = 1;
+            if (sd != 0) yj[d] = yj[d] + (double)sd * (p->perHi[d] - p->perLo[d]);
+          }
+        if (skip) continue;
+        double tot = (ip2(yi[JX] - yj[JX]) + ip2(yi[JY] - yj[JY])) + ip2(yi[JZ] - yj[JZ]);
+        if (tot > dist2) continue;
+        double rdiff = sqrt((ip2(yj[JX] - yi[JX]) + ip2(yj[JY] - yi[JY])) + ip2(yj[JZ] - yi[JZ]));
+        if (!(rdiff <= w)) continue;
+        am_pair(yi, rpi_, yj, p->rprop + LRP * (size_t)j, rphip, Fam, Wsum, &nn);
+      }
+    }
+    if (!(p->per[0] || p->per[1] || p->per[2])) break;
+  }
+  return nn;
+}
+
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
  * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124),
  * pressure-gradient, viscous-unsteady and normal collision forces and heat
@@ -872,6 +1111,33 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
   double fdpdx = 0.0, fdpdy = 0.0, fdpdz = 0.0;
   double fcx = 0.0, fcy = 0.0, fcz = 0.0;
   double fvux = 0.0, fvuy = 0.0, fvuz = 0.0;
+  double Fam[3] = {0.0, 0.0, 0.0}, Wsum[3] = {0.0, 0.0, 0.0};
+  int nneighbors = 0;
+  if (P->amFlag == 2) {
+    /* Step 1a: the unary term of Briney's model (ppiclf_user_AM_Briney_Unary,
+     * ppiclf_user_AddedMass.f:164-240): Mach correction only, no volume
+     * factor yet */
+    double rcd_am;
+    if (rmachp > F32(0.6)) {
+      rcd_am = 0x1.5104d6p+1; /* 1.0 + 1.8*((0.6)**2) + 7.6*((0.6)**4) folded in single precision */
+    } else {
+      double m2 = rmachp * rmachp;
+      rcd_am = 1.0 + F32(1.8) * m2 + F32(7.6) * (m2 * m2);
+    }
+    rcd_am = rcd_am * 0.5;
+    rmass_add = rhof * rp[R_JVOLP] * rcd_am;
+    double SDrho = rp[R_JRHSR] + y[JVX] * rp[R_JPGCX] + y[JVY] * rp[R_JPGCY] + y[JVZ] * rp[R_JPGCZ];
+    SDrho = SDrho / rphif;
+    double vgradrho = vx * rp[R_JRHOGX] + vy * rp[R_JRHOGY] + vz * rp[R_JRHOGZ];
+    famx = rcd_am * (vx * SDrho + rhof * rp[R_JSDRX] + rp[R_JUX] * vgradrho);
+    famy = rcd_am * (vy * SDrho + rhof * rp[R_JSDRY] + rp[R_JUY] * vgradrho);
+    famz = rcd_am * (vz * SDrho + rhof * rp[R_JSDRZ] + rp[R_JUZ] * vgradrho);
+    Fam[0] = famx;
+    Fam[1] = famy;
+    Fam[2] = famz;
+    /* Step 1b: the pair terms (ppiclf_user.f:411-468) */
+    nneighbors = am_neighbours(p, i, rphip, Fam, Wsum);
+  }
   double tqc[3] = {0.0, 0.0, 0.0}; /* collision torques (ydotc(JOX..JOZ)) */
   if (P->collisionalFlag >= 1) { /* Step 1b and Step 6, ppiclf_user.f:411-470, 587-600 */
     double fc[3];
@@ -913,6 +1179,32 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
     famx = rcd_am * rp[R_JVOLP] * (vx * SDrho + rhof * rp[R_JSDRX] + ug * vgradrho);
     famy = rcd_am * rp[R_JVOLP] * (vy * SDrho + rhof * rp[R_JSDRY] + vg * vgradrho);
     famz = rcd_am * rp[R_JVOLP] * (vz * SDrho + rhof * rp[R_JSDRZ] + wg * vgradrho);
+  } else if (P->amFlag == 2) { /* Step 4 (ppiclf_user.f:527-552) */
+    if (nneighbors > 0) {      /* ppiclf_user_AM_Briney_Binary (ppiclf_user_AddedMass.f:290-360) */
+      double rad = rp[R_JDP] * 0.5;
+      double dr_max = p->nbW;
+      double alpha = fmin(F32(0.4), rphip);
+      double IA, II;
+      if (dr_max / rad >= F32(3.49)) {
+        IA = orc_am_ia_analytical(dr_max, rad, alpha);
+        II = orc_am_ii_analytical(dr_max, rad, alpha);
+      } else {
+        IA = orc_am_ia_numerical(dr_max, rad, alpha);
+        II = orc_am_ii_numerical(dr_max, rad, alpha);
+      }
+      for (int c = 0; c < 3; c++) {
+        Fam[c] = Fam[c] + IA * rp[R_WDOTX + c];
+        Fam[c] = Fam[c] + II * Wsum[c] / (double)nneighbors;
+      }
+      for (int c = 0; c < 3; c++) Fam[c] = rp[R_JVOLP] * Fam[c];
+      famx = Fam[0];
+      famy = Fam[1];
+      famz = Fam[2];
+    } else {
+      famx = famx * rp[R_JVOLP];
+      famy = famy * rp[R_JVOLP];
+      famz = famz * rp[R_JVOLP];
+    }
   }
   if (P->pgFlag == 1) {
     fdpdx = -rp[R_JVOLP] * rp[R_JDPDX];
@@ -971,6 +1263,40 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
   }
 }
 
+/* ppiclf_user.f:934-990 (am_flag 2, after every particle's forces): each
+ * particle's density-weighted acceleration W = D(rho_g u_g)/Dt - rho_g dv/dt,
+ * read by the next stage's pair terms (rprop(WDOTX..WDOTZ)).  The loop reads
+ * two variables the particle loop left behind: rphif of the last particle
+ * (every particle's SDrho is divided by it), and vx, vy, vz, which it sets
+ * only after using them in vgradrhog, so particle i's gradient term takes the
+ * previous particle's relative velocity (the first particle, the last one's). */
+static void am_store_wdot(struct orc_picl *p) {
+  int last = -1;
+  for (int i = p->n - 1; i >= 0 && last < 0; i--)
+    if (!p->lost[i]) last = i;
+  if (last < 0) return;
+  const double *rl = p->rprop + LRP * (size_t)last, *yl = p->y + LRS * (size_t)last;
+  double vx = rl[R_JUX] - yl[JVX], vy = rl[R_JUY] - yl[JVY], vz = rl[R_JUZ] - yl[JVZ];
+  double rphif = 1.0 - rl[R_JPHIP]; /* ppiclf_user.f:324, then :360-361 unless the particle was skipped */
+  if (!(sqrt(vx * vx + vy * vy + vz * vz) <= 1.0e-3)) rphif = 1.0 - fmin(rl[R_JPHIP], 0.62);
+  for (int i = 0; i < p->n; i++) {
+    if (p->lost[i]) continue;
+    double *rp = p->rprop + LRP * (size_t)i;
+    const double *y = p->y + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i;
+    double SDrho = rp[R_JRHSR] + y[JVX] * rp[R_JPGCX] + y[JVY] * rp[R_JPGCY] + y[JVZ] * rp[R_JPGCZ];
+    SDrho = SDrho / rphif;
+    double vgradrhog = vx * rp[R_JRHOGX] + vy * rp[R_JRHOGY] + vz * rp[R_JRHOGZ];
+    double rhof = rp[R_JRHOF];
+    vx = rp[R_JUX] - y[JVX];
+    vy = rp[R_JUY] - y[JVY];
+    vz = rp[R_JUZ] - y[JVZ];
+    double ug = rp[R_JUX], vg = rp[R_JUY], wg = rp[R_JUZ];
+    rp[R_WDOTX] = vx * SDrho + rhof * rp[R_JSDRX] + ug * vgradrhog - rhof * yd[JVX];
+    rp[R_WDOTX + 1] = vy * SDrho + rhof * rp[R_JSDRY] + vg * vgradrhog - rhof * yd[JVY];
+    rp[R_WDOTX + 2] = vz * SDrho + rhof * rp[R_JSDRZ] + wg * vgradrhog - rhof * yd[JVZ];
+  }
+}
+
 /* ppiclf_solve_InvokeLinearPeriodic (ppiclf_solve.f:1319-1353), reached
  * from RemoveParticle (ppiclf.f:13140-13144) after every stage's update
  * (ppiclf_solve.f:1933): each axis on its own, a particle beyond one bound
@@ -996,6 +1322,7 @@ static void integrate_particle(struct orc_picl *p, int istage, double dt) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < p->n; i++)
     if (!p->lost[i]) set_ydot(p, i, dt);
+  if (p->pp.amFlag == 2) am_store_wdot(p);
   if (p->pp.viscousUnsteady == 1 && istage == 3) vu_shift(p, dt);
   if (istage == 1) memset(p->y1, 0, sizeof(double) * LRS * (size_t)p->n);
   const double *k = c[istage - 1];
@@ -1097,7 +1424,8 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
     if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4 || pp->viscousUnsteady < 0 ||
-      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 2)
+      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 2 || pp->amFlag < 0 ||
+      pp->amFlag > 2 || pp->neighborWidth < 0.0)
     return -1;
   if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0)) return -1;
   const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};
@@ -1144,6 +1472,11 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   p->lost = calloc(n, 1);
   p->hist = calloc(6 * VU_N * n, sizeof(double)); /* ppiclf_solve.f:432-434 */
   p->nTimeBH = 1;                                  /* PICL_TEMP_InitSolver.F90:290 */
+  if (pp->amFlag == 2) { /* PICL_TEMP_InitSolver.F90:655-661: max(4 d_max, NEIGHBORWIDTH) */
+    double dmax = 0.0;
+    for (int i = 0; i < nPart; i++) dmax = fmax(dmax, pconst[4 * (size_t)i + 1]);
+    p->nbW = pp->neighborWidth > 0.0 ? pp->neighborWidth : 4.0 * dmax;
+  }
   for (int i = 0; i < nPart; i++) {
     double *rp = p->rprop + LRP * (size_t)i;
     rp[R_JRHOP] = pconst[4 * (size_t)i];

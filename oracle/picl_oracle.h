@@ -29,7 +29,7 @@ typedef struct orc_picl_params {
   double rcpPart;    /* particle heat capacity (ppiclf_rcp_part) */
   int qsFlag;        /* 1 Parmar, 2 Osnes, 3 modified Parmar, 4 Gidaspow,
                         5 Schiller-Naumann (INRT_DRAG_MODEL_SN in ppiclF's slot) */
-  int amFlag;        /* 0: off, 1: AM Parmar */
+  int amFlag;        /* 0: off, 1: AM Parmar, 2: AM Briney (unary + binary terms) */
   int pgFlag;        /* 0: off, 1: pressure-gradient force */
   int feedbackFlag;  /* 0: one-way, 1: two-way coupling */
   int interpMode;    /* 0: IDW-27 (LocalInterp); 1: trilinear (findpts_eval_local) */
@@ -44,6 +44,8 @@ typedef struct orc_picl_params {
                           2: + tangential force, contact and rolling torques */
   double ksp;          /* KSP: collision time = ksp * dt sets the spring stiffness cap */
   double erest;        /* ERESTITUTION: restitution coefficient of the damping */
+  double neighborWidth; /* amFlag 2: ppiclf_d2chk(3), the pair terms' reach; 0 means 4 x the
+                           largest diameter (PICL_TEMP_InitSolver.F90:655-661 with NEIGHBORWIDTH 0) */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index
@@ -96,6 +98,21 @@ void orc_picl_collision_force2(const double *yi12, const double *yj12, double dp
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
 /* Elements LocalInterp sums for point xp, nearest first; returns how many. */
 int orc_picl_neighbors(const orc_region *r, const double *xp, int *ids, double *d2);
+
+/* Briney's added-mass model (libpicl/user_files/Added_Mass/): the four
+ * fitted curves, the distant-neighbour corrections (analytical, and the
+ * Simpson sums over the radial distribution function), and the pair's
+ * resistance matrix R[k][l] (l < 3: B11, l >= 3: B12, rotated to the pair). */
+double orc_am_b11_11(double d, double alpha);
+double orc_am_b11_22(double d, double alpha);
+double orc_am_b12_11(double d, double alpha);
+double orc_am_b12_22(double d, double alpha);
+double orc_am_ia_analytical(double rmax, double rad, double alpha);
+double orc_am_ii_analytical(double rmax, double rad, double alpha);
+double orc_am_rdf(double r, double alpha);
+double orc_am_ia_numerical(double rmax, double rad, double alpha);
+double orc_am_ii_numerical(double rmax, double rad, double alpha);
+void orc_am_resistance_pair(double x, double y, double z, double alpha, double rad, double R[3][6]);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,7 @@ struct PiclPhys {
   int vu;                     // viscous-unsteady force
   int col;                    // normal collision force
   double ksp, erest;          // collision time / dt, restitution coefficient
+  double nbW;                 // ppiclf_d2chk(3): the added-mass pair terms' reach (amFlag 2)
 };
 
 // Rows of PBuf::st
@@ -59,6 +60,7 @@ struct PBuf {
   double *pc = nullptr;  // [4][nP] rho_p, d_p, V_p, SPL
   int *pid = nullptr;    // original (ppiclF) index
   int *elem = nullptr;   // element (device cell) of the current stage, -1 lost
+  double *w = nullptr;   // [3][nP] W = D(rho_g u_g)/Dt - rho_g dv/dt (rprop WDOTX..Z; amFlag 2)
 };
 
 struct PiclDev {
@@ -114,6 +116,16 @@ struct PiclDev {
       *colBsum = nullptr;
   double *fc = nullptr;  // [3][nP] collision force by current slot
   double *tq = nullptr;  // [3][nP] collision torque by current slot (tangential collisions)
+  int grid = 0;          // the grid of cubes is built each stage (collisions or the added-mass pairs)
+  // Briney's added mass (amFlag 2), per stage
+  int am2 = 0;
+  double *vint = nullptr;    // [20][nP] interpolated values by current slot
+  double *famB = nullptr;    // [4][nP] famx, famy, famz, rmass_add by current slot
+  double *snum = nullptr;    // [nP] SDrho's numerator by current slot
+  double *vrelP = nullptr;   // [3][nP] relative velocity by original index
+  double *rphifP = nullptr;  // [nP] the particle loop's rphif by original index
+  int *actP = nullptr;       // [nP] located this stage, by original index
+  int *lastP = nullptr;      // [1] 1 + the largest original index located this stage
 };
 
 // ---------------------------------------------------------------------------
@@ -810,6 +822,349 @@ __global__ void __launch_bounds__(BLK) k_contacts(Dev d, PiclDev P, PiclPhys Ph)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Briney added mass (AMFORCE 2): libpicl/user_files/Added_Mass/
+// ppiclf_user_AddedMass_functions.f, _subroutines.f, the unary and binary
+// terms of ppiclf_user_AddedMass.f:164-400 and the pair terms of
+// ppiclf_user_EvalNearestNeighbor.f:115-159.  Same operations as the
+// oracle: real*4 literals widened, integer powers as gcc expands them.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ip6(double x) { double x3 = (x * x) * x; return x3 * x3; }
+__device__ __forceinline__ double ip8(double x) { double x2 = x * x, x4 = x2 * x2; return x4 * x4; }
+__device__ __forceinline__ double ip9(double x) { double x3 = (x * x) * x, x6 = x3 * x3; return x6 * x3; }
+
+__device__ double am_b11_11(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * (0.046875 * ip6(t) + 0.03515625 * ip8(t));  // 3.0/64.0, 9.0/256.0
+  double hot = F32(0.059) / (((d - F32(1.9098)) * ip2(d - F32(0.4782))) * ip3(d));
+  double alpha_corr = ((alpha * alpha - F32(0.0902) * alpha) * F32(70.7731)) / ip6(d + F32(2.0936));
+  return (asym + hot) + alpha_corr;
+}
+__device__ double am_b11_22(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * (0.01171875 * ip6(t) + 0.01171875 * ip8(t));  // 3.0/256.0
+  double A = (F32(0.0003) + (F32(0.0262) * alpha) * alpha) - F32(0.0012) * alpha;
+  double B = (F32(1.3127) + (F32(1.0401) * alpha) * alpha) - F32(1.2519) * alpha;
+  return asym + A / ip6(d - B);
+}
+__device__ double am_b12_11(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * (-(0.375 * ip3(t)) - 0.005859375 * ip9(t));  // 3.0/8.0, 3.0/512.0
+  double hot = -(F32(0.0006) / ip5(d - F32(1.5428)));
+  double alpha_corr = -((F32(0.7913) * alpha) * exp(-((F32(0.9801) - F32(0.1075) * alpha) * d)));
+  return (asym + hot) + alpha_corr;
+}
+__device__ double am_b12_22(double d, double alpha) {
+  double t = 2.0 / d;
+  double asym = 0.5 * (0.1875 * ip3(t) + 0.000732421875 * ip9(t));  // 3.0/16.0, 3.0/4096.0
+  double hot = (-F32(0.1985) + F32(16.7372) * alpha) / ip6(d) + (F32(1.3907) - F32(48.2604) * alpha) / ip8(d);
+  return asym + hot;
+}
+
+__device__ double am_ia_analytical(double rmax, double rad, double alpha) {
+  double m = rmax / rad;
+  double term1 = (5.0 * ip2(m) + 9.0) / (10.0 * ip5(m));
+  double l1 = log(m - F32(1.9098)), l2 = log(m - F32(0.4782));
+  double num = (((F32(0.00720826) - F32(0.0150737) * m) * l1 - (F32(0.120023) * m) * l2) + F32(0.057395) * l2 +
+                (F32(0.135097) * m - F32(0.0646033)) * log(m)) - F32(0.0861828);
+  double term2 = num / (m - F32(0.4782));
+  double term3 = ((ip2(alpha) - F32(0.0902) * alpha) * ((F32(0.333333) * ip2(m) + F32(0.348933) * m) + F32(0.146105))) /
+                 ip5(m + F32(2.0936));
+  double B11 = (term1 + term2) + term3;
+  term1 = (5.0 * ip2(m) + 12.0) / (40.0 * ip5(m));
+  double numerator = (F32(0.0262) * ip2(alpha) - F32(0.0012) * alpha) + F32(0.0003);
+  double c = (-(F32(1.0401) * ip2(alpha)) + F32(1.2519) * alpha) - F32(1.3127);
+  term2 = (numerator * ((10.0 * ip2(m) + (5.0 * m) * c) + ip2(c))) / (30.0 * ip5(m + c));
+  return alpha * (B11 + 2.0 * (term1 + term2));
+}
+__device__ double am_ii_analytical(double rmax, double rad, double alpha) {
+  double m = rmax / rad;
+  double term1 = -(1.0 / (4.0 * ip6(m)));
+  double term2 = -((F32(6.0e-4) * ((0.5 * ip2(m) - F32(0.516067) * m) + F32(0.199744))) / ip4(F32(1.5482) - m));
+  double prefactor = -(F32(0.7913) * alpha);
+  double c = F32(0.9801) - F32(0.1075) * alpha;
+  double term3 = ((prefactor * ((m * c) * ((m * c) + 2.0) + 2.0)) * exp(-(m * c))) / ip3(c);
+  double B11 = (term1 + term2) + term3;
+  term1 = 1.0 / (32.0 * ip6(m));
+  term2 = (-F32(0.1985) + F32(16.7372) * alpha) / (3.0 * ip3(m));
+  term3 = (F32(1.3907) - F32(48.2604) * alpha) / (5.0 * ip5(m));
+  return alpha * (B11 + 2.0 * ((term1 + term2) + term3));
+}
+
+// radial distribution function of hard spheres (Trokhymchuk et al. 2005 and
+// erratum), r = distance / diameter; sqrt(3.0) and 2/3 in single precision
+__device__ double am_rdf(double r, double alpha) {
+  double eta = alpha;
+  double d = pow((2.0 * eta) * (((eta * eta - 3.0 * eta) - 3.0) +
+                                sqrt(3.0 * ((((ip4(eta) - 2.0 * ip3(eta)) + ip2(eta)) + 6.0 * eta) + 3.0))),
+                 0x1.555556p-2);
+  double f = (2.0 * eta) / (1.0 - eta);
+  double mu = f * ((-1.0 - d / (2.0 * eta)) + eta / d);
+  double alpha0 = f * ((-1.0 + d / (4.0 * eta)) - eta / (2.0 * d));
+  double beta0 = (f * 0x1.bb67aep+0) * (-(d / (4.0 * eta)) - eta / (2.0 * d));
+  double a2b2 = ip2(alpha0) + ip2(beta0);
+  double denom_gamma = ((a2b2 - (2.0 * mu) * alpha0) * (1.0 + 0.5 * eta)) - mu * (1.0 + 2.0 * eta);
+  double X = (alpha0 * a2b2 - mu * (ip2(alpha0) - ip2(beta0))) * (1.0 + 0.5 * eta) +
+             (a2b2 - mu * alpha0) * (1.0 + 2.0 * eta);
+  double gamma = atan(-(((1.0 / beta0) * X) / denom_gamma));
+  double omega = (-(F32(0.682) * exp(-(F32(24.697) * eta))) + F32(4.72)) + F32(4.45) * eta;
+  double kappa = F32(4.674) * exp(-(F32(3.935) * eta)) + F32(3.536) * exp(-(F32(56.27) * eta));
+  double alpha1 = ((F32(44.554) + F32(79.868) * eta) + (F32(116.432) * eta) * eta) - F32(44.652) * exp(2.0 * eta);
+  double beta = (-F32(5.022) + F32(5.857) * eta) + F32(5.089) * exp(-(4.0 * eta));
+  double rstar = (F32(2.0116) - F32(1.0647) * eta) + (F32(0.0538) * eta) * eta;
+  double gm = ((((F32(1.0286) - F32(0.6095) * eta) + F32(3.5781) * ip2(eta)) - F32(21.3651) * ip3(eta)) +
+               F32(42.6344) * ip4(eta)) - F32(33.8485) * ip5(eta);
+  const double f23 = 0x1.555556p-1;
+  double gsig = (1.0 / (4.0 * eta)) *
+                (((((1.0 + eta) + ip2(eta)) - f23 * ip3(eta)) - f23 * ip4(eta)) / ip3(1.0 - eta) - 1.0);
+  double Brad = ((gm - (gsig / rstar) * exp(mu * (rstar - 1.0))) * rstar) /
+                (cos(beta * (rstar - 1.0) + gamma) * exp(alpha1 * (rstar - 1.0)) - cos(gamma) * exp(mu * (rstar - 1.0)));
+  double Arad = gsig - Brad * cos(gamma);
+  double delta = -(omega * rstar) - atan((kappa * rstar + 1.0) / (omega * rstar));
+  double Crad = ((rstar * (gm - 1.0)) * exp(kappa * rstar)) / cos(omega * rstar + delta);
+  double g2 = (Arad / r) * exp(mu * (r - 1.0)) + ((Brad / r) * cos(beta * (r - 1.0) + gamma)) * exp(alpha1 * (r - 1.0));
+  if (r > rstar)
+    g2 = 1.0 + ((Crad / r) * cos(omega * r + delta)) * exp(-(kappa * r));
+  else if (r < 1.0)
+    g2 = 0.0;
+  return g2;
+}
+
+// Simpson sums of the distant-neighbour corrections, used when rmax / rad <
+// 3.49; pi = 4*ATAN(1.0) in single precision
+__device__ double am_numerical(bool induced, double rmax, double rad, double alpha) {
+  const int npts = 1001;
+  double maxr = rad * 20.0;
+  if (!(maxr > rmax)) return 0.0;
+  double dr = (maxr - rmax) / (double)(npts + 1);
+  auto f = [&](double d) {
+    const double pi = 0x1.921fb6p+1;
+    double g = am_rdf(d / (2.0 * rad), alpha);
+    double f11 = induced ? am_b12_11(d / rad, alpha) : am_b11_11(d / rad, alpha);
+    double f22 = induced ? am_b12_22(d / rad, alpha) : am_b11_22(d / rad, alpha);
+    double rho = alpha / (0x1.555556p+0 * pi);
+    double w = (((0x1.555556p-2 * rho) * 4.0) * pi) * ip2(d / rad);
+    return (w * g) * (f11 + 2.0 * f22);
+  };
+  double integral = 0.0, r = rmax;
+  integral = integral + f(r);
+  integral = integral + f(maxr);
+  for (int i = 2; i <= npts - 1; i++) {
+    r = r + dr;
+    double fi = f(r);
+    integral = integral + (i % 2 == 0 ? 4.0 : 2.0) * fi;
+  }
+  return (dr * integral) / 3.0;
+}
+
+// resistance_pair: B11, B12 rotated onto the pair axis (x, y, z) = x_j - x_i,
+// Qt (B Q) with matmul's k-ascending sums; R[k][l]: l < 3 B11, l >= 3 B12
+__device__ void am_resistance_pair(double x, double y, double z, double alpha, double rad, double R[3][6]) {
+  double gam = atan2(y, x), bet = -atan2(z, sqrt(x * x + y * y));
+  double cb = cos(bet), sb = sin(bet), cg = cos(gam), sg = sin(gam);
+  double Q[3][3] = {{cb * cg, cb * sg, -sb}, {-sg, cg, 0.0}, {sb * cg, sb * sg, cb}};
+  double dist = sqrt(x * x + y * y + z * z) / rad;
+  double b[2][3];
+  b[0][0] = am_b11_11(dist, alpha);
+  b[0][1] = b[0][2] = am_b11_22(dist, alpha);
+  b[1][0] = am_b12_11(dist, alpha);
+  b[1][1] = b[1][2] = am_b12_22(dist, alpha);
+#pragma unroll
+  for (int blk = 0; blk < 2; blk++) {
+    double M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) s = s + (k == i ? b[blk][i] : 0.0) * Q[k][j];
+        M[i][j] = s;
+      }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) s = s + Q[k][i] * M[k][j];
+        R[i][3 * blk + j] = s;
+      }
+  }
+}
+
+// One thread per cube: its member list in ascending slot order, so the
+// scans that sum in member order (the added-mass pair terms) are
+// deterministic whatever order k_col_fill's atomics produced.
+__global__ void __launch_bounds__(BLK) k_col_sort(PiclDev P) {
+  int b = blockIdx.x * BLK + threadIdx.x;
+  if (b >= P.colNb) return;
+  int *L = P.colMem + P.colOff[b];
+  int n = P.colOff[b + 1] - P.colOff[b];
+  for (int a = 1; a < n; a++) {
+    int x = L[a], j = a - 1;
+    while (j >= 0 && L[j] > x) { L[j + 1] = L[j]; j--; }
+    L[j + 1] = x;
+  }
+}
+
+// Steps 1a, 1b and 4 of ppiclf_user_SetYdot for am_flag 2 (ppiclf_user.f:
+// 393-468, 527-552): the unary term, the pair terms of every particle within
+// the neighbour width (ppiclf_solve_NearestNeighbor's squared-distance filter,
+// then EvalNearestNeighbor's rdiff <= d2chk(3)) and the binary term, into
+// famB = (famx, famy, famz, rmass_add).  The pair terms are added in the
+// cubes' order (z, y, x, each cube's members by slot), not ppiclF's
+// particle-index order: the sums agree with the oracle's to rounding.
+__global__ void __launch_bounds__(BLK) k_am_pairs(Dev d, PiclDev P, PiclPhys Ph) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  if (C.elem[t] < 0) return;
+  const double *st = C.st;
+  double v[PICL_NFLD];
+#pragma unroll
+  for (int f = 0; f < PICL_NFLD; f++) v[f] = P.vint[f * nP + t];
+  double xi[3] = {st[t], st[nP + t], st[2 * nP + t]};
+  double yv[3] = {st[3 * nP + t], st[4 * nP + t], st[5 * nP + t]};
+  double vxr = v[1] - yv[0], vyr = v[2] - yv[1], vzr = v[3] - yv[2];
+  double vmag = sqrt(vxr * vxr + vyr * vyr + vzr * vzr);
+  if (vmag <= 1.0e-3) return;  // skipped by SetYdot
+  double rhof = v[0], rmachp = vmag / v[7];
+  double rphip = fmin(v[9], 0.62), rphif = 1.0 - rphip;
+  double dpi = C.pc[nP + t], volp = C.pc[2 * nP + t];
+  // Step 1a: ppiclf_user_AM_Briney_Unary (Mach correction only, no volume yet)
+  double rcd_am;
+  if (rmachp > F32(0.6)) {
+    rcd_am = 0x1.5104d6p+1;
+  } else {
+    double m2 = rmachp * rmachp;
+    rcd_am = 1.0 + F32(1.8) * m2 + F32(7.6) * (m2 * m2);
+  }
+  rcd_am = rcd_am * 0.5;
+  double rmass_add = rhof * volp * rcd_am;
+  double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
+  SDrho = SDrho / rphif;
+  double vgradrho = vxr * v[17] + vyr * v[18] + vzr * v[19];
+  double Fam[3] = {rcd_am * (vxr * SDrho + rhof * v[10] + v[1] * vgradrho),
+                   rcd_am * (vyr * SDrho + rhof * v[11] + v[2] * vgradrho),
+                   rcd_am * (vzr * SDrho + rhof * v[12] + v[3] * vgradrho)};
+  double Wi[3] = {C.w[t], C.w[nP + t], C.w[2 * nP + t]};
+  double Wsum[3] = {0.0, 0.0, 0.0};
+  int nn = 0;
+  // Step 1b: the neighbours within d2chk(3), periodic images included
+  const double w = Ph.nbW, dist2 = w * w, Rq = w * (1.0 + 1.0e-9);
+  const double alpha_local = fmin(F32(0.4), rphip), rad = 0.5 * dpi;
+  int nsd[3], sdv[3][3], clo[3][3], chi[3][3];
+  double shv[3][3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    nsd[l] = 0;
+    for (int sd = -1; sd <= 1; sd++) {
+      if (sd != 0 && !Ph.per[l]) continue;
+      double shl = sd == 0 ? 0.0 : (double)sd * (Ph.perHi[l] - Ph.perLo[l]);
+      double q = xi[l] - shl;
+      if (q + Rq < P.colLo[l] - Rq || q - Rq > P.colHi[l] + Rq) continue;
+      sdv[l][nsd[l]] = sd;
+      shv[l][nsd[l]] = shl;
+      clo[l][nsd[l]] = col_coord(P, q - Rq, l);
+      chi[l][nsd[l]] = col_coord(P, q + Rq, l);
+      nsd[l]++;
+    }
+  }
+  for (int iz = 0; iz < nsd[2]; iz++)
+  for (int iy = 0; iy < nsd[1]; iy++)
+  for (int ix = 0; ix < nsd[0]; ix++) {
+    const bool self = sdv[0][ix] == 0 && sdv[1][iy] == 0 && sdv[2][iz] == 0;
+    double sh[3] = {shv[0][ix], shv[1][iy], shv[2][iz]};
+    for (int cz = clo[2][iz]; cz <= chi[2][iz]; cz++)
+      for (int cy = clo[1][iy]; cy <= chi[1][iy]; cy++) {
+        int row = P.colN[0] * (cy + P.colN[1] * cz);
+        for (int q = P.colOff[row + clo[0][ix]]; q < P.colOff[row + chi[0][ix] + 1]; q++) {
+          int sj = P.colMem[q];
+          if (self && sj == t) continue;
+          double xj[3];
+#pragma unroll
+          for (int l = 0; l < 3; l++) xj[l] = sh[l] == 0.0 ? st[l * nP + sj] : st[l * nP + sj] + sh[l];
+          double tot = ((xi[0] - xj[0]) * (xi[0] - xj[0]) + (xi[1] - xj[1]) * (xi[1] - xj[1])) +
+                       (xi[2] - xj[2]) * (xi[2] - xj[2]);
+          if (tot > dist2) continue;
+          double rx = xj[0] - xi[0], ry = xj[1] - xi[1], rz = xj[2] - xi[2];
+          double rdiff = sqrt((rx * rx + ry * ry) + rz * rz);
+          if (!(rdiff <= w)) continue;
+          double rthresh = 0.5 * (dpi + C.pc[nP + sj]);
+          if (rdiff < rthresh) rx = ry = rz = rthresh;  // overlapping: the reference's substitute
+          double R[3][6];
+          am_resistance_pair(rx, ry, rz, alpha_local, rad, R);
+          double Wj[3] = {C.w[sj], C.w[nP + sj], C.w[2 * nP + sj]};
+          nn++;
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+              Fam[k] = Fam[k] + R[k][l] * Wi[l];
+              Fam[k] = Fam[k] + R[k][l + 3] * Wj[l];
+            }
+            Wsum[k] = Wsum[k] + Wj[k];
+          }
+        }
+      }
+  }
+  // Step 4: ppiclf_user_AM_Briney_Binary, or the unary term times the volume
+  if (nn > 0) {
+    double IA, II;
+    if (w / rad >= F32(3.49)) {
+      IA = am_ia_analytical(w, rad, alpha_local);
+      II = am_ii_analytical(w, rad, alpha_local);
+    } else {
+      IA = am_numerical(false, w, rad, alpha_local);
+      II = am_numerical(true, w, rad, alpha_local);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      Fam[c] = Fam[c] + IA * Wi[c];
+      Fam[c] = Fam[c] + II * Wsum[c] / (double)nn;
+      Fam[c] = volp * Fam[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) Fam[c] = Fam[c] * volp;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) P.famB[c * nP + t] = Fam[c];
+  P.famB[3 * nP + t] = rmass_add;
+}
+
+// ppiclf_user.f:934-990 after every particle's forces (am_flag 2): each
+// particle's W = D(rho_g u_g)/Dt - rho_g dv/dt.  It divides by the rphif the
+// particle loop left behind (the last particle's) and, in vgradrhog, uses
+// the relative velocity of the previous particle in ppiclF's order (the
+// first particle, the last one's): k_particles' first phase recorded both by
+// original index, with the pre-update velocities.
+__global__ void __launch_bounds__(BLK) k_wdot(PiclDev P) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= P.nAct[0]) return;
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  if (C.elem[t] < 0) return;
+  const int last = P.lastP[0] - 1;
+  int q = C.pid[t] - 1;
+  while (q >= 0 && !P.actP[q]) q--;
+  if (q < 0) q = last;
+  const double rphif = P.rphifP[last];
+  double SDrho = P.snum[t] / rphif;
+  double vgradrhog = P.vrelP[q] * P.vint[17 * nP + t] + P.vrelP[nP + q] * P.vint[18 * nP + t] +
+                     P.vrelP[2 * nP + q] * P.vint[19 * nP + t];
+  double rhof = P.vint[t];
+  const int pid = C.pid[t];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double vr = P.vrelP[c * nP + pid], ug = P.vint[(1 + c) * nP + t];
+    C.w[c * nP + t] = vr * SDrho + rhof * P.vint[(10 + c) * nP + t] + ug * vgradrhog -
+                      rhof * C.st[(ROW_YD + 3 + c) * nP + t];
+  }
+}
+
 // ppiclf_user_VU_Rocflu (ppiclf_user_ViscousUnsteady.f:19-90): the history
 // kernel (Mei and Adrian's, with ppiclF's volume-fraction factor) summed over
 // entries 2..nTimeBH-1, half weight on the oldest; entry 1 (this stage's)
@@ -848,19 +1203,31 @@ __global__ void k_vu_time(Dev d, PiclDev P) {
   T[VU_N] = (double)n;
 }
 
+// phase 0: the whole stage.  With Briney's added mass the stage runs in two
+// phases around k_am_pairs (whose pair terms need every particle's
+// interpolated volume fraction): phase 1 locates and interpolates into vint
+// and records what the W store reads, phase 2 takes vint and the pair
+// kernel's added-mass force and does the rest.
 template <int TRI, int QS>
-__global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage) {
+__global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage, int phase) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
   const size_t nP = P.nP;
   PBuf C = cur_buf(P);
   double *st = C.st;
+  double v[PICL_NFLD];
+  if (phase == 2) {
+    if (C.elem[t] < 0) return;
+#pragma unroll
+    for (int f = 0; f < PICL_NFLD; f++) v[f] = P.vint[f * nP + t];
+  } else {
   double xp[3] = {st[t], st[nP + t], st[2 * nP + t]};
   int e = picl_locate(P, xp, C.elem[t]);
   C.elem[t] = e;
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
     for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = 0.0;
+    if (phase == 1) P.actP[C.pid[t]] = 0;
     return;
   }
   atomicAdd(&P.cnt[e], 1);
@@ -868,7 +1235,6 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
 #pragma unroll
     for (int k = 0; k < 3; k++) P.xo[k * nP + t] = xp[k];
   }
-  double v[PICL_NFLD];
   if (TRI) {
     // gslib findpts_eval_local on the element's 2x2x2 nodal field (every node
     // holds the cell value): Lagrange weights (1-r)/2, (1+r)/2 on the nodes
@@ -975,6 +1341,25 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
 #pragma unroll
     for (int f = 0; f < PICL_NFLD; f++) P.dbg[f * nP + pid] = v[f];
   }
+  if (phase == 1) {
+#pragma unroll
+    for (int f = 0; f < PICL_NFLD; f++) P.vint[f * nP + t] = v[f];
+    // what ppiclf_user.f:934-990 reads: SDrho's numerator with the
+    // start-of-stage velocity, the relative velocity and the rphif that
+    // SetYdot's particle loop leaves (ppiclf_user.f:316-361), by index
+    const int pid = C.pid[t];
+    double y0 = st[3 * nP + t], y1 = st[4 * nP + t], y2 = st[5 * nP + t];
+    P.snum[t] = v[13] + y0 * v[14] + y1 * v[15] + y2 * v[16];
+    double vx = v[1] - y0, vy = v[2] - y1, vz = v[3] - y2;
+    P.vrelP[pid] = vx;
+    P.vrelP[nP + pid] = vy;
+    P.vrelP[2 * nP + pid] = vz;
+    P.rphifP[pid] = sqrt(vx * vx + vy * vy + vz * vz) <= 1.0e-3 ? 1.0 - v[9] : 1.0 - fmin(v[9], 0.62);
+    P.actP[pid] = 1;
+    atomicMax(P.lastP, pid + 1);
+    return;
+  }
+  }
   // slots: 0 rhof 1-3 u 4-6 dp/dx 7 c 8 T 9 phi 10-12 Du/Dt 13 rhsR 14-16 grad(rho phi) 17-19 grad rho_g
 
   // ---- SetYdot
@@ -1032,6 +1417,11 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       famx = rcd_am * volp * (vxr * SDrho + rhof * v[10] + v[1] * vgradrho);
       famy = rcd_am * volp * (vyr * SDrho + rhof * v[11] + v[2] * vgradrho);
       famz = rcd_am * volp * (vzr * SDrho + rhof * v[12] + v[3] * vgradrho);
+    } else if (Ph.amFlag == 2) {  // Briney: unary, pair and binary terms from k_am_pairs
+      famx = P.famB[t];
+      famy = P.famB[nP + t];
+      famz = P.famB[2 * nP + t];
+      rmass_add = P.famB[3 * nP + t];
     }
     if (Ph.pgFlag == 1) {
       fdpdx = -volp * v[4];
@@ -1680,6 +2070,9 @@ __global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
     for (int r = ROW_O; r < ROW_VU; r++) X.st[r * nP + pos] = C.st[r * nP + t];
   if (P.vu)
     for (int r = ROW_VU; r < NROW_VU; r++) X.st[r * nP + pos] = C.st[r * nP + t];
+  if (P.am2)
+#pragma unroll
+    for (int c = 0; c < 3; c++) X.w[c * nP + pos] = C.w[c * nP + t];
   X.pid[pos] = C.pid[t];
   X.elem[pos] = C.elem[t];
 }
@@ -1784,7 +2177,8 @@ void picl_free(PiclState *s) {
                   P.buf[1].elem, P.par, P.ycN, P.src, P.dest, P.cnt, P.off, P.bsum, P.nAct, P.lostCur,
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
                   P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
-                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq, P.gBox, P.gvList};
+                  P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq, P.gBox, P.gvList,
+                  P.buf[0].w, P.buf[1].w, P.vint, P.famB, P.snum, P.vrelP, P.rphifP, P.actP, P.lastP};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -1843,7 +2237,7 @@ void picl_particle_stage(rflu_handle *h, int istage) {
   PiclDev &P = s->P;
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nC + 1)), dim3(BLK), 0, h->stream, P.cnt, P.nC + 1);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lostCur, 1);
-  if (P.col) {  // the stage's neighbour search for the collision force
+  if (P.grid) {  // the stage's grid of cubes for the neighbour searches
     ProfScope ps(h, "k_contacts");
     int n = P.colNb + 1, nb = scan_blocks(n);
     hipLaunchKernelGGL(k_zero_int, dim3(nblk(n)), dim3(BLK), 0, h->stream, P.colCnt, n);
@@ -1853,16 +2247,34 @@ void picl_particle_stage(rflu_handle *h, int istage) {
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(SCAN_T), 0, h->stream, P.colBsum, nb);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(SCAN_T), 0, h->stream, P.colCnt, n, P.colBsum, P.colOff);
     hipLaunchKernelGGL(k_col_fill, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
-    hipLaunchKernelGGL(k_contacts, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
+    if (P.am2) hipLaunchKernelGGL(k_col_sort, dim3(nblk(P.colNb)), dim3(BLK), 0, h->stream, P);
+    if (P.col) hipLaunchKernelGGL(k_contacts, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
   }
   {
     ProfScope ps(h, "k_particles");
-    using KP = void (*)(Dev, PiclDev, PiclPhys, int);
+    using KP = void (*)(Dev, PiclDev, PiclPhys, int, int);
+    static const KP kp[2][5] = {
+        {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
+        {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
+    KP f = kp[s->phys.interpMode][s->phys.qsFlag - 1];
+    if (P.am2) {
+      hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lastP, 1);
+      hipLaunchKernelGGL(f, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys, istage, 1);
+    }
+  }
+  if (P.am2) {
+    ProfScope ps(h, "k_am_pairs");
+    hipLaunchKernelGGL(k_am_pairs, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
+  }
+  {
+    ProfScope ps(h, "k_particles");
+    using KP = void (*)(Dev, PiclDev, PiclPhys, int, int);
     static const KP kp[2][5] = {
         {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
         {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
     hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
-                       P, s->phys, istage);
+                       P, s->phys, istage, P.am2 ? 2 : 0);
+    if (P.am2) hipLaunchKernelGGL(k_wdot, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
   }
   if (s->phys.vu && istage == 3) hipLaunchKernelGGL(k_vu_time, dim3(1), dim3(64), 0, h->stream, h->d, P);
   picl_sort_and_deposit(h, 1);
@@ -1962,7 +2374,9 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (pp->projMode < 0 || pp->projMode > 1) return set_err(h, RFLU_GPU_EINVAL, "projMode must be 0 or 1");
   if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0))
     return set_err(h, RFLU_GPU_EINVAL, "the Gaussian filter needs filterWidth > 0 and 0 < filterAlpha < 1");
-  if (pp->amFlag < 0 || pp->amFlag > 1) return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 or 1");
+  if (pp->amFlag < 0 || pp->amFlag > 2)
+    return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 (off), 1 (Parmar) or 2 (Briney)");
+  if (!(pp->neighborWidth >= 0.0)) return set_err(h, RFLU_GPU_EINVAL, "neighborWidth must be >= 0");
   if (pp->viscousUnsteady < 0 || pp->viscousUnsteady > 1)
     return set_err(h, RFLU_GPU_EINVAL, "viscousUnsteady must be 0 or 1");
   if (pp->collisionalFlag < 0 || pp->collisionalFlag > 2)
@@ -2006,6 +2420,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   s->P.rot = pp->collisionalFlag >= 2 ? 1 : 0;
   s->phys.ksp = pp->ksp;
   s->phys.erest = pp->erest;
+  s->P.am2 = pp->amFlag == 2 ? 1 : 0;
+  s->P.grid = s->P.col || s->P.am2 ? 1 : 0;
   s->P.nRow = s->P.vu ? NROW_VU : (s->P.rot ? ROW_VU : NROW);
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
@@ -2095,13 +2511,23 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.buf[b].elem, (size_t)nP, minus1.data());
   }
   rc |= dalloc(h, &P.par, 1, &zero);
-  if (P.col) {  // grid of cubes of side just above the largest diameter over the bounding box
+  if (P.grid) {  // grid of cubes over the bounding box for the neighbour searches
     double dmax = 0.0;
     for (int p = 0; p < nP; p++) dmax = std::max(dmax, pconst[4 * (size_t)p + 1]);
-    if (!(dmax > 0.0)) { picl_free(s); return set_err(h, RFLU_GPU_EINVAL, "collisions need positive diameters"); }
+    if (!(dmax > 0.0)) {
+      picl_free(s);
+      return set_err(h, RFLU_GPU_EINVAL, "collisions and Briney's added mass need positive diameters");
+    }
     P.colR = dmax * (1.0 + 1.0e-9);
     P.colDmax = dmax;
+    // collisions reach (d + d_max) / 2: cubes of a third of d_max; the
+    // added-mass pairs reach d2chk(3): cubes of half of it
     P.colH = dmax / 3.0 * (1.0 + 1.0e-8);
+    if (P.am2) {
+      // PICL_TEMP_InitSolver.F90:655-661 passes max(4 d_max, NEIGHBORWIDTH)
+      s->phys.nbW = pp->neighborWidth > 0.0 ? pp->neighborWidth : 4.0 * dmax;
+      P.colH = std::max(P.colH, 0.5 * s->phys.nbW * (1.0 + 1.0e-8));
+    }
     for (int l = 0; l < 3; l++) { P.colLo[l] = vlo[l]; P.colHi[l] = vhi[l]; }
     // at most 2^25 cells and 8 per particle: coarser cubes find the same
     // contacts among more candidates (the span loops take any width)
@@ -2122,8 +2548,20 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.colMem, (size_t)nP);
     rc |= dalloc(h, &P.colB, (size_t)nP);
     rc |= dalloc(h, &P.colBsum, (size_t)scan_blocks(P.colNb + 1));
-    rc |= dalloc(h, &P.fc, 3 * (size_t)nP);
+    if (P.col) rc |= dalloc(h, &P.fc, 3 * (size_t)nP);
     if (P.rot) rc |= dalloc(h, &P.tq, 3 * (size_t)nP);
+  }
+  if (P.am2) {  // W starts at 0 (the zeroed rprop, ppiclf_solve.f:432-434)
+    std::vector<double> z3(3 * (size_t)nP, 0.0);
+    std::vector<int> zi((size_t)nP, 0);
+    for (int b = 0; b < 2; b++) rc |= dalloc(h, &P.buf[b].w, 3 * (size_t)nP, z3.data());
+    rc |= dalloc(h, &P.vint, PICL_NFLD * (size_t)nP);
+    rc |= dalloc(h, &P.famB, 4 * (size_t)nP);
+    rc |= dalloc(h, &P.snum, (size_t)nP);
+    rc |= dalloc(h, &P.vrelP, 3 * (size_t)nP, z3.data());
+    rc |= dalloc(h, &P.rphifP, (size_t)nP, z3.data());
+    rc |= dalloc(h, &P.actP, (size_t)nP, zi.data());
+    rc |= dalloc(h, &P.lastP, 1, zi.data());
   }
   if (P.vu) {  // ppiclf_nTimeBH = 1 (PICL_TEMP_InitSolver.F90:290), ages 0
     std::vector<double> t0(VU_N + 1, 0.0);
@@ -2576,7 +3014,8 @@ void ppiclf_solve_initparticle_(const int *imethod, const int *ndim, const int *
     g_f77.pc[4 * (size_t)p + 2] = r[3];
     g_f77.pc[4 * (size_t)p + 3] = r[21];
   }
-  (void)filt2; (void)filt3;
+  (void)filt2;
+  g_f77.nbw = *filt3;  // ppiclf_d2chk(3) until InitNeighborBin resets it (ppiclf_solve.f:224-226)
   g_f77.linit = true;
 }
 
@@ -2606,7 +3045,7 @@ void ppiclf_solve_initgaussianfilter_(const double *filt, const double *alpha, c
 void ppiclf_solve_initneighborbin_(const double *rwidth) {
   API_LOCK;
   if (!g_f77.linit) return f77_fail("InitParticle must be before InitNeighborBin");
-  g_f77.nbw = *rwidth;  // the collision search sizes its own cubes
+  g_f77.nbw = *rwidth;  // ppiclf_d2chk(3), the added-mass pairs' reach (the collision search sizes its own cubes)
 }
 
 // ppiclf_comm.f:926-1000: the element nodes must be the bound region's hex
@@ -2636,6 +3075,7 @@ void ppiclf_comm_initoverlapmesh_(const int *ncell, const int *lx1, const int *l
   p.periodicX = g_f77.per[0];
   p.periodicY = g_f77.per[1];
   p.periodicZ = g_f77.per[2];
+  p.neighborWidth = g_f77.nbw;
   if (int rc = picl_gpu_init(h, &p, g_f77.npart, g_f77.y.data(), g_f77.pc.data()))
     return f77_fail("InitOverlap: " + h->err + " (code " + std::to_string(rc) + ")");
   g_f77.fld.assign((size_t)PICL_NFLD * N, 0.0);
