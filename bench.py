@@ -388,8 +388,9 @@ def run_serve(args):
     print(json.dumps(out))
 
 
-KERNELS = ("k_dtmin", "k_grad", "k_rhog", "k_flux", "k_gather", "k_update", "k_picl_fields", "k_particles",
-           "k_scan", "k_fill", "k_deposit", "k_gauss", "k_permute", "k_feedback")
+KERNELS = ("k_dtmin", "k_grad", "k_rhog", "k_flux", "k_gather", "k_update", "k_fb_grad", "k_picl_fields",
+           "k_contacts", "k_interp", "k_am_pairs", "k_particles", "k_scan", "k_fill", "k_deposit", "k_gauss",
+           "k_permute", "k_feedback")
 
 
 def measure(workload, args, fp32=False):

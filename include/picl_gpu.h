@@ -79,8 +79,12 @@ typedef struct picl_params {
    * (ppiclf_user_EvalNearestNeighbor.f:118-233, Cundall and Strack), with
    * periodic images on periodic axes; 2 adds the tangential force (Coulomb
    * friction or dashpot, the smaller), the contact and rolling-resistance
-   * torques and the angular velocity y(JOX..JOZ) they drive.  3-4
-   * (hydrodynamic torque, lift) are rejected.  KSP and EREST (global%piclKsp, piclERest, :222-223):
+   * torques and the angular velocity y(JOX..JOZ) they drive; 3 adds the
+   * hydrodynamic torque and the undisturbed-flow torque
+   * (ppiclf_user_Torque.f:72-197) from the gas vorticity and its material
+   * derivative, interpolated like the other fields (PICL_TEMP_Runge.F90:
+   * 618-663, 747-809, with the feedback force's gradient); 4 adds the Saffman
+   * and Magnus lift (ppiclf_user_Lift.f:61-181).  KSP and EREST (global%piclKsp, piclERest, :222-223):
    * the spring stiffness is capped at rmass pi^2 / (ksp dt)^2, erest in (0, 1]
    * sets the damping. */
   int collisionalFlag;

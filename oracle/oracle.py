@@ -130,6 +130,9 @@ def lib():
             getattr(L, n).argtypes = [D, D, D]
             getattr(L, n).restype = D
         L.orc_am_resistance_pair.argtypes = [D, D, D, D, D, P]
+        L.orc_picl_torque_hydro.argtypes = [P, P, D, D, D, D, D, P]
+        L.orc_picl_torque_hydro.restype = I
+        L.orc_picl_lift.argtypes = [P, P, P, D, D, D, D, P]
         L.orc_set_threads.argtypes = [I]
         L.orc_get_threads.restype = I
         L.orc_test_rk3_linear.argtypes = [D, D, P]
@@ -436,6 +439,23 @@ def am_resistance_pair(x, y, z, alpha, rad):
     R = np.zeros((3, 6))
     lib().orc_am_resistance_pair(float(x), float(y), float(z), float(alpha), float(rad), _p(R))
     return R
+
+
+def torque_hydro(vort, spin, rhof, dp, rmu, rmass, rhop):
+    """Torque_Hydro: the torque vector, or None where ppiclF stops (Re_r >= 100)."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (vort, spin)]
+    t = np.zeros(3)
+    rc = lib().orc_picl_torque_hydro(_p(a[0]), _p(a[1]), float(rhof), float(dp), float(rmu), float(rmass),
+                                     float(rhop), _p(t))
+    return None if rc else t
+
+
+def lift(vort, spin, vrel, dp, rep, rmu, rhof):
+    """Saffman + Magnus lift on a particle (rep capped at 0.1 beforehand)."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (vort, spin, vrel)]
+    out = np.zeros(3)
+    lib().orc_picl_lift(_p(a[0]), _p(a[1]), _p(a[2]), float(dp), float(rep), float(rmu), float(rhof), _p(out))
+    return out
 
 
 def set_threads(n):

@@ -27,7 +27,7 @@ enum {
   R_JRHOP = 0, R_JRHOF = 1, R_JDP = 2, R_JVOLP = 3, R_JPHIP = 4, R_JUX = 5, R_JUY = 6, R_JUZ = 7,
   R_JCS = 8, R_JDPDX = 9, R_JDPDY = 10, R_JDPDZ = 11, R_JSDRX = 12, R_JSDRY = 13, R_JSDRZ = 14,
   R_JRHSR = 15, R_JPGCX = 16, R_JPGCY = 17, R_JPGCZ = 18, R_JSPL = 21, R_JT = 23,
-  R_WDOTX = 27, R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
+  R_WDOTX = 27, R_JXVOR = 30, R_JSDOX = 42, R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
 };
 enum { P_JPHIP = 0, P_JFX, P_JFY, P_JFZ, P_JE, NPRO };
 #define LRS 12
@@ -65,12 +65,17 @@ struct orc_picl {
   double perLo[3], perHi[3];   /* ppiclf_xdrange: the mesh's vertex bounds */
   double *fld;     /* ORC_PICL_NFLD interpolation inputs per element */
   double nbW;      /* ppiclf_d2chk(3): the added-mass pair terms' reach */
+  double *fldRot;  /* COLLISIONS >= 3: vorticity and D(omega)/Dt per element */
+  int rotErr;      /* a rotational Reynolds number reached 100 this stage */
 };
+
+/* The rotational fields (rprop JXVOR..JZVOR, JSDOX..JSDOZ, 0-based) */
+static const int ROT_RPROP[6] = {R_JXVOR, R_JXVOR + 1, R_JXVOR + 2, R_JSDOX, R_JSDOX + 1, R_JSDOX + 2};
 
 void orc__picl_free(struct orc_picl *p) {
   if (!p) return;
   void *ptrs[] = {p->y, p->y1, p->ydot, p->ydotc, p->rprop, p->elem, p->lost, p->xm1b,
-                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld, p->hist};
+                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld, p->hist, p->fldRot};
   for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
   free(p);
 }
@@ -272,6 +277,16 @@ static void local_interp(struct orc_picl *p, int ip) {
     }
     p->rprop[LRP * (size_t)ip + j] = v / wsum;
   }
+  if (p->fldRot)
+    for (int f = 0; f < 6; f++) {
+      double v = 0.0, wsum = 0.0;
+      for (int k = 0; k < nn; k++) {
+        double w = 1.0 / (sqrt(d2[k]) + eps);
+        v = v + w * p->fldRot[6 * (size_t)inear[k] + f];
+        wsum = wsum + w;
+      }
+      p->rprop[LRP * (size_t)ip + ROT_RPROP[f]] = v / wsum;
+    }
 }
 
 /* Trilinear mode: gslib findpts_eval_local, the evaluation ppiclF made before
@@ -296,6 +311,13 @@ static void trilinear_interp(struct orc_picl *p, int ip) {
     double w = h[1][0] * u + h[1][1] * u;
     p->rprop[LRP * (size_t)ip + orc_picl_field_rprop[f] - 1] = h[2][0] * w + h[2][1] * w;
   }
+  if (p->fldRot)
+    for (int f = 0; f < 6; f++) {
+      double v = p->fldRot[6 * (size_t)e + f];
+      double u = h[0][0] * v + h[0][1] * v;
+      double w = h[1][0] * u + h[1][1] * u;
+      p->rprop[LRP * (size_t)ip + ROT_RPROP[f]] = h[2][0] * w + h[2][1] * w;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1074,6 +1096,108 @@ static int am_neighbours(const struct orc_picl *p, int i, double rphip, double F
   return nn;
 }
 
+/* ------------------------------------------------------------------------ */
+/* COLLISIONS 3-4: hydrodynamic and undisturbed torque                       */
+/* (ppiclf_user_Torque.f:72-197), Saffman and Magnus lift                    */
+/* (ppiclf_user_Lift.f:61-181).  rp holds the interpolated vorticity         */
+/* (JXVOR..) and its material derivative (JSDOX..), y the angular velocity.  */
+/* ------------------------------------------------------------------------ */
+/* Torque_Hydro: -1 when the rotational Reynolds number reaches 100, where
+ * ppiclF stops ("Re rotational too large") */
+static int torque_hydro(const double *rp, const double *y, double rhof, double dp, double rmu, double rmass,
+                        double t[3]) {
+  const double rpi = acos(-1.0);
+  double omgrx = 0.5 * rp[R_JXVOR] - y[7];
+  double omgry = 0.5 * rp[R_JXVOR + 1] - y[8];
+  double omgrz = 0.5 * rp[R_JXVOR + 2] - y[9];
+  double omgr_mag = sqrt(omgrx * omgrx + omgry * omgry + omgrz * omgrz);
+  double reyr = rhof * dp * dp * omgr_mag / (4.0 * rmu);
+  reyr = fmax(reyr, 0.001);
+  double Ct1, Ct2, Ct3;
+  if (reyr < 1.0) {
+    Ct1 = 0.0; Ct2 = 16.0 * rpi; Ct3 = 0.0;
+  } else if (reyr < 10.0) {
+    Ct1 = 0.0; Ct2 = 16.0 * rpi; Ct3 = 0.0418;
+  } else if (reyr < 20.0) {
+    Ct1 = 5.32; Ct2 = 37.2; Ct3 = 0.0;
+  } else if (reyr < 50.0) {
+    Ct1 = 6.44; Ct2 = 32.2; Ct3 = 0.0;
+  } else if (reyr < 100.0) {
+    Ct1 = 6.45; Ct2 = 32.1; Ct3 = 0.0;
+  } else {
+    return -1;
+  }
+  double Ct = Ct1 / sqrt(reyr) + Ct2 / reyr + Ct3 * reyr;
+  double beta = rp[R_JRHOP] / rhof;
+  double rIp = rmass * dp * dp / 10.0;
+  double factor = rIp * 60.0 * Ct * omgr_mag / (64.0 * rpi * beta);
+  t[0] = factor * omgrx;
+  t[1] = factor * omgry;
+  t[2] = factor * omgrz;
+  return 0;
+}
+
+/* Lift_Saffman then Lift_Magnus, added into lift (vrel: the relative
+ * velocity vx, vy, vz; rep already capped at 0.1, rnu = rmu / rhof) */
+static void lift_forces(const double *rp, const double *y, const double vrel[3], double vmag, double dp, double rep,
+                        double rmu, double rhof, double rnu, double lift[3]) {
+  if (vmag < 1.0e-8) return;
+  double vx = vrel[0], vy = vrel[1], vz = vrel[2];
+  { /* Saffman, with Mei's correction */
+    double omgx = rp[R_JXVOR], omgy = rp[R_JXVOR + 1], omgz = rp[R_JXVOR + 2];
+    double omg_mag = sqrt(omgx * omgx + omgy * omgy + omgz * omgz);
+    double omg_star = omg_mag * dp / vmag;
+    double epi = sqrt(omg_star / rep);
+    double d1 = 1.0 + tanh(2.5 * (log10(epi) + 0.191));
+    double d2 = 0.667 + tanh(6.0 * epi - 1.92);
+    double Jepi = 0.3 * d1 * d2;
+    double factor = 1.615 * rmu * (dp * dp) * vmag * sqrt(omg_mag / rnu);
+    double elx = vy * omgz - vz * omgy, ely = vz * omgx - vx * omgz, elz = vx * omgy - vy * omgx;
+    double elm = sqrt(elx * elx + ely * ely + elz * elz);
+    elm = fmax(1.0e-20, elm);
+    double ielm = 1.0 / elm;
+    lift[0] = lift[0] + factor * Jepi * elx * ielm;
+    lift[1] = lift[1] + factor * Jepi * ely * ielm;
+    lift[2] = lift[2] + factor * Jepi * elz * ielm;
+  }
+  { /* Magnus, from the particle's spin */
+    double omgx = y[7], omgy = y[8], omgz = y[9];
+    double omg_mag = sqrt(omgx * omgx + omgy * omgy + omgz * omgz);
+    double epi = omg_mag * dp / vmag;
+    double d1 = 0.675 + 0.15 * (1.0 + tanh(0.28 * (epi - 2.0)));
+    double CL = 1.0 - d1 * tanh(F32(0.18) * sqrt(rep));
+    double factor = 0.125 * dp * dp * dp * rhof;
+    double elx = vy * omgz - vz * omgy, ely = vz * omgx - vx * omgz, elz = vx * omgy - vy * omgx;
+    lift[0] = lift[0] + factor * CL * elx;
+    lift[1] = lift[1] + factor * CL * ely;
+    lift[2] = lift[2] + factor * CL * elz;
+  }
+}
+
+/* test hooks: the torque and lift laws on given inputs (vort: the gas
+ * vorticity, spin: the particle's angular velocity) */
+int orc_picl_torque_hydro(const double *vort, const double *spin, double rhof, double dp, double rmu, double rmass,
+                          double rhop, double *t) {
+  double rp[LRP] = {0}, y[LRS] = {0};
+  for (int c = 0; c < 3; c++) {
+    rp[R_JXVOR + c] = vort[c];
+    y[7 + c] = spin[c];
+  }
+  rp[R_JRHOP] = rhop;
+  return torque_hydro(rp, y, rhof, dp, rmu, rmass, t);
+}
+void orc_picl_lift(const double *vort, const double *spin, const double *vrel, double dp, double rep, double rmu,
+                   double rhof, double *lift) {
+  double rp[LRP] = {0}, y[LRS] = {0};
+  for (int c = 0; c < 3; c++) {
+    rp[R_JXVOR + c] = vort[c];
+    y[7 + c] = spin[c];
+  }
+  double vmag = sqrt(vrel[0] * vrel[0] + vrel[1] * vrel[1] + vrel[2] * vrel[2]);
+  lift[0] = lift[1] = lift[2] = 0.0;
+  lift_forces(rp, y, vrel, vmag, dp, rep, rmu, rhof, rmu / rhof, lift);
+}
+
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
  * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124),
  * pressure-gradient, viscous-unsteady and normal collision forces and heat
@@ -1146,7 +1270,12 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
     fcy = fc[1];
     fcz = fc[2];
   }
-  const double liftx = 0.0, lifty = 0.0, liftz = 0.0;
+  double lift[3] = {0.0, 0.0, 0.0};
+  if (P->collisionalFlag == 4) { /* Step 9b (:640-646) */
+    double vrel[3] = {vx, vy, vz};
+    lift_forces(rp, y, vrel, vmag, dp, rep, rmu, rhof, rmu / rhof, lift);
+  }
+  const double liftx = lift[0], lifty = lift[1], liftz = lift[2];
   double beta; /* ppiclf_user.f:473-485 */
   switch (P->qsFlag) {
     case 1: beta = qs_parmar(rmachp, rphip, rep, rmu, dp); break;
@@ -1229,10 +1358,16 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
   }
   double rmass_omega = rmass * dp * dp / 10.0;
   double taux = 0.0, tauy = 0.0, tauz = 0.0;
-  if (P->collisionalFlag >= 2) { /* Step 9a, :638-646; Torque_driver adds nothing below flag 3 */
-    taux = tqc[0] + 0.0 + 0.0;
-    tauy = tqc[1] + 0.0 + 0.0;
-    tauz = tqc[2] + 0.0 + 0.0;
+  double tauh[3] = {0.0, 0.0, 0.0}, tauu[3] = {0.0, 0.0, 0.0};
+  if (P->collisionalFlag >= 2) { /* Step 9a, :638-646: Torque_driver */
+    if (P->collisionalFlag >= 3) {
+      if (torque_hydro(rp, y, rhof, dp, rmu, rmass, tauh)) p->rotErr = 1;
+      double rIf = rhof * dp * dp * rp[R_JVOLP] / 10.0; /* Torque_Undisturbed */
+      for (int c = 0; c < 3; c++) tauu[c] = 0.5 * rIf * rp[R_JSDOX + c];
+    }
+    taux = tqc[0] + tauh[0] + tauu[0];
+    tauy = tqc[1] + tauh[1] + tauu[1];
+    tauz = tqc[2] + tauh[2] + tauu[2];
   }
   yd[JX] = y[JVX];
   yd[JY] = y[JVY];
@@ -1248,7 +1383,7 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
   yd[11] = 0.0;
   if (P->viscousUnsteady >= 1) vu_update(p, i, rp, y, yd, vx, vy, vz, rhof, rphif); /* :669-672 */
   if (P->feedbackFlag == 1) {
-    const double tauxh = 0.0, tauyh = 0.0, tauzh = 0.0;
+    const double tauxh = tauh[0], tauyh = tauh[1], tauzh = tauh[2];
     yc[JVX] = rp[R_JSPL] * (yd[JVX] * rmass - fcx);
     yc[JVY] = rp[R_JSPL] * (yd[JVY] * rmass - fcy);
     yc[JVZ] = rp[R_JSPL] * (yd[JVZ] * rmass - fcz);
@@ -1361,6 +1496,21 @@ static double phip_of(const double *vfP8, double vol) {
 int orc__picl_runge(orc_region *r, int istage) {
   struct orc_picl *p = r->picl;
   int N = r->nCells;
+  double *fb = NULL, *fbg = NULL;
+  if (p->fldRot) {
+    /* :618-663: the feedback force per cell from the previous projection,
+     * its LSQ gradient, WENO-XYZ and RFLU_LimitGradCellsSimple with the
+     * variable flags (1, 2, 3): only the first (x) is VAR_INFO_POS */
+    fb = malloc(sizeof(double) * 3 * (size_t)N);
+    fbg = malloc(sizeof(double) * 9 * (size_t)N);
+    for (int i = 0; i < N; i++) {
+      const double *pf = p->proFld + 8 * NPRO * (size_t)i;
+      for (int k = 0; k < 3; k++) fb[3 * (size_t)i + k] = sum_nodes(pf + 8 * (P_JFX + k)) * 0.125 * r->vol[i];
+    }
+    for (int k = 0; k < 3; k++) orc__gradient_field(r, fb + k, 3, fbg + 3 * k, 9);
+    orc__weno_xyz(r, fbg, 9);
+    orc__limit_simple_field(r, fbg, 9, fb, 3);
+  }
   /* field assembly :712-842 (one value per element; LocalInterp's node
    * average of the 8 identical copies, :2363-2373, is sum8(v)/8) */
   for (int i = 0; i < N; i++) {
@@ -1395,8 +1545,41 @@ int orc__picl_runge(orc_region *r, int istage) {
      * the nodes themselves */
     for (int q = 0; q < ORC_PICL_NFLD; q++)
       p->fld[ORC_PICL_NFLD * (size_t)i + q] = p->pp.interpMode == 1 ? f[q] : sum8(f[q]) / 8;
+    if (p->fldRot) {
+      /* :747-809: vorticity and its material derivative (stretching,
+       * dilatation, baroclinic, the feedback force's curl and its
+       * misalignment with grad(rho phi)) */
+      double dudx = g[3], dudy = g[4], dudz = g[5], dvdx = g[6], dvdy = g[7], dvdz = g[8];
+      double dwdx = g[9], dwdy = g[10], dwdz = g[11];
+      double omgx = dwdy - dvdz, omgy = dudz - dwdx, omgz = dvdx - dudy;
+      double divu = dudx + dvdy + dwdz;
+      double dprdx = g[0], dprdy = g[1], dprdz = g[2], dpdx = g[12], dpdy = g[13], dpdz = g[14];
+      const double *fg = fbg + 9 * (size_t)i, *F = fb + 3 * (size_t)i;
+      double ir = 1.0 / cv[0], ir2 = ir * ir;
+      double dodx = omgx * dudx + omgy * dudy + omgz * dudz;
+      double dody = omgx * dvdx + omgy * dvdy + omgz * dvdz;
+      double dodz = omgx * dwdx + omgy * dwdy + omgz * dwdz;
+      dodx = dodx - omgx * divu;
+      dody = dody - omgy * divu;
+      dodz = dodz - omgz * divu;
+      dodx = dodx + (dprdy * dpdz - dprdz * dpdy) * ir2;
+      dody = dody + (dprdz * dpdx - dprdx * dpdz) * ir2;
+      dodz = dodz + (dprdx * dpdy - dprdy * dpdx) * ir2;
+      dodx = dodx + (fg[6 + 1] - fg[3 + 2]) * ir; /* dFz/dy - dFy/dz */
+      dody = dody + (fg[0 + 2] - fg[6 + 0]) * ir; /* dFx/dz - dFz/dx */
+      dodz = dodz + (fg[3 + 0] - fg[0 + 1]) * ir; /* dFy/dx - dFx/dy */
+      dodx = dodx + (dprdy * F[2] - dprdz * F[1]) * ir2;
+      dody = dody + (dprdz * F[0] - dprdx * F[2]) * ir2;
+      dodz = dodz + (dprdx * F[1] - dprdy * F[0]) * ir2;
+      double fr[6] = {omgx, omgy, omgz, dodx, dody, dodz};
+      for (int q = 0; q < 6; q++) p->fldRot[6 * (size_t)i + q] = p->pp.interpMode == 1 ? fr[q] : sum8(fr[q]) / 8;
+    }
   }
+  free(fb);
+  free(fbg);
+  p->rotErr = 0;
   integrate_particle(p, istage, r->dtMin);
+  if (p->rotErr) return -1; /* ppiclf_exittr: "Re rotational too large" */
   if (p->pp.feedbackFlag == 1) { /* :926-1038 */
     for (int i = 0; i < N; i++) {
       const double *pf = p->proFld + 8 * NPRO * (size_t)i;
@@ -1424,7 +1607,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
     if (r->c2v[q] < 0) return -1; /* all hexes */
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4 || pp->viscousUnsteady < 0 ||
-      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 2 || pp->amFlag < 0 ||
+      pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 4 || pp->amFlag < 0 ||
       pp->amFlag > 2 || pp->neighborWidth < 0.0)
     return -1;
   if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0)) return -1;
@@ -1485,6 +1668,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
     rp[R_JSPL] = pconst[4 * (size_t)i + 3];
   }
   setup_mesh(r, p);
+  if (pp->collisionalFlag >= 3) p->fldRot = calloc(6 * (size_t)p->ne, sizeof(double));
   r->picl = p;
   int N = r->nCells;
   r->phi = calloc((size_t)N, sizeof(double));

@@ -114,6 +114,13 @@ double orc_am_ia_numerical(double rmax, double rad, double alpha);
 double orc_am_ii_numerical(double rmax, double rad, double alpha);
 void orc_am_resistance_pair(double x, double y, double z, double alpha, double rad, double R[3][6]);
 
+/* COLLISIONS 3-4 laws on given inputs: Torque_Hydro (returns -1 where ppiclF
+ * stops, Re_r >= 100) and Saffman + Magnus lift (rep already capped). */
+int orc_picl_torque_hydro(const double *vort, const double *spin, double rhof, double dp, double rmu, double rmass,
+                          double rhop, double *t);
+void orc_picl_lift(const double *vort, const double *spin, const double *vrel, double dp, double rep, double rmu,
+                   double rhof, double *lift);
+
 #ifdef __cplusplus
 }
 #endif

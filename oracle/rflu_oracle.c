@@ -727,7 +727,9 @@ static void limiter_venkat(orc_region *r) {
 
 /* LSQ gradient of cv(1) alone: RFLU_ComputeGradCellsFast_3D with
  * iBegVar = iEndVar = 1 (libfloflu/CellGradientsMP.F90:375-378) */
-static void gradient_var1(orc_region *r, double *out) {
+/* The LSQ gradient (RFLU_ComputeGradCellsWrapper) of one cell field,
+ * v[vs * cell], into out[os * cell + 0..2]. */
+void orc__gradient_field(orc_region *r, const double *v, int vs, double *out, int os) {
   for (int icg = 0; icg < r->nCells; icg++) {
     const double *m = r->moms + 10 * (size_t)icg;
     double r11 = m[0], r12 = m[1], r22 = m[2], r13 = m[3], r23 = m[4], r33 = m[5];
@@ -736,10 +738,10 @@ static void gradient_var1(orc_region *r, double *out) {
     double c13 = -(c11 * r13 + c12 * c22 * r23);
     double c23 = -(c22 * r23);
     double c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
-    double *g = out + 3 * (size_t)icg;
+    double *g = out + (size_t)os * icg;
     g[0] = g[1] = g[2] = 0.0;
     const double *x0 = r->cofg + 3 * (size_t)icg;
-    double v0 = r->cv[5 * (size_t)icg];
+    double v0 = v[(size_t)vs * icg];
     for (int s = r->csOff[icg]; s < r->csOff[icg + 1]; s++) {
       int j = r->csMem[s];
       double dx = r->cofg[3 * (size_t)j] - x0[0];
@@ -753,13 +755,16 @@ static void gradient_var1(orc_region *r, double *out) {
       double wtx = term * (term1 + c12 * term2 + c13 * term3);
       double wty = term * (term2 + c23 * term3);
       double wtz = term * (term3);
-      double dVar = r->cv[5 * (size_t)j] - v0;
+      double dVar = v[(size_t)vs * j] - v0;
       g[0] = g[0] + wtx * dVar;
       g[1] = g[1] + wty * dVar;
       g[2] = g[2] + wtz * dVar;
     }
   }
 }
+
+/* the gradient of cv(1) (rho_g while CellGradientsMP holds it there) */
+static void gradient_var1(orc_region *r, double *out) { orc__gradient_field(r, r->cv, 5, out, 3); }
 
 /* RFLU_WENOGradCellsXYZFast_3D, one gradient (modflu/RFLU_ModWENO.F90:1402-1588):
  * per component, the average of the cell's own and its stencil members'
@@ -786,6 +791,7 @@ static void weno_xyz_n(orc_region *r, double *grad, int stride) {
   free(eno);
 }
 static void weno_xyz(orc_region *r, double *grad) { weno_xyz_n(r, grad, 3); }
+void orc__weno_xyz(orc_region *r, double *grad, int stride) { weno_xyz_n(r, grad, stride); }
 
 /* RFLU_LimitGradCellsSimple, one VAR_INFO_POS variable
  * (modflu/RFLU_ModLimiters.F90:793-921): the gradient is zeroed when the
@@ -812,6 +818,26 @@ static void limit_simple_var(orc_region *r, double *grad, int stride, int var) {
   free(zero);
 }
 static void limit_simple(orc_region *r, double *grad) { limit_simple_var(r, grad, 3, 0); }
+
+/* The same for a cell field that is not in cv: vals[vstride * cell]. */
+void orc__limit_simple_field(orc_region *r, double *grad, int stride, const double *vals, int vstride) {
+  int N = r->nCells;
+  char *zero = calloc((size_t)N, 1);
+  for (int f = 0; f < r->nFaces + r->nBFaces; f++) {
+    int nc = f < r->nFaces ? 2 : 1;
+    const double *fc = f < r->nFaces ? r->fc + 3 * (size_t)f : r->bfc + 3 * (size_t)(f - r->nFaces);
+    for (int s = 0; s < nc; s++) {
+      int c = f < r->nFaces ? r->f2c[2 * f + s] : r->bf2c[f - r->nFaces];
+      const double *x = r->cofg + 3 * (size_t)c, *g = grad + (size_t)stride * c;
+      double dx = fc[0] - x[0], dy = fc[1] - x[1], dz = fc[2] - x[2];
+      double varf = vals[(size_t)vstride * c] + g[0] * dx + g[1] * dy + g[2] * dz;
+      if (varf <= 0.0) zero[c] = 1;
+    }
+  }
+  for (int i = 0; i < N; i++)
+    if (zero[i]) grad[(size_t)stride * i] = grad[(size_t)stride * i + 1] = grad[(size_t)stride * i + 2] = 0.0;
+  free(zero);
+}
 
 /* CellGradientsMP: libfloflu/CellGradientsMP.F90:158-253, and the PICL
  * block :355-390 (gradient of rho_g = rho/(1-phi), WENO-XYZ, positivity
@@ -1595,7 +1621,7 @@ int orc_rk_stage(orc_region *r, int istage) {
   orc_residual(r);
   if (r->picl) {
     int e = orc__picl_runge(r, istage);
-    if (e) return e;
+    if (e) { snprintf(g_err, sizeof g_err, "ppiclF: Re rotational too large"); return e; }
   }
   rk_update(r, istage);
   set_dependent_vars(r);
@@ -1614,7 +1640,7 @@ int orc_step(orc_region *r, int nSteps) {
       orc_residual(r);
       if (r->picl) {                                                       /* RungeKuttaMP.F90:243-247 */
         int e = orc__picl_runge(r, istage);
-        if (e) return e;
+        if (e) { snprintf(g_err, sizeof g_err, "ppiclF: Re rotational too large"); return e; }
       }
       rk_update(r, istage);
       set_dependent_vars(r);

@@ -42,5 +42,9 @@ struct orc_region {
 void orc__set_dependent_vars(orc_region *r);
 int orc__picl_runge(orc_region *r, int istage); /* PICL_TEMP_Runge (picl_oracle.c) */
 void orc__picl_free(struct orc_picl *p);
+/* CellGradientsMP's building blocks, on any cell field (rflu_oracle.c) */
+void orc__gradient_field(orc_region *r, const double *v, int vs, double *out, int os);
+void orc__weno_xyz(orc_region *r, double *grad, int stride);
+void orc__limit_simple_field(orc_region *r, double *grad, int stride, const double *vals, int vstride);
 
 #endif

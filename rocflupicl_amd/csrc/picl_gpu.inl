@@ -126,6 +126,13 @@ struct PiclDev {
   double *rphifP = nullptr;  // [nP] the particle loop's rphif by original index
   int *actP = nullptr;       // [nP] located this stage, by original index
   int *lastP = nullptr;      // [1] 1 + the largest original index located this stage
+  // COLLISIONS >= 3: vorticity and its material derivative, interpolated
+  int rot3 = 0;
+  double *fb = nullptr;      // [3][nC] feedback force per cell (previous projection)
+  double *fbRaw = nullptr;   // [nC][9] its LSQ gradient
+  double *fbg = nullptr;     // [nC][9] after WENO-XYZ and the x-force positivity limiter
+  double *fldRot = nullptr;  // [nC][6] vorticity, D(omega)/Dt per element
+  int nVint = PICL_NFLD;     // values per particle in vint (20, 26 with the rotational fields)
 };
 
 // ---------------------------------------------------------------------------
@@ -393,6 +400,38 @@ __global__ void __launch_bounds__(BLK) k_picl_fields(Dev d, Gas G, PiclDev P, in
 #pragma unroll
   for (int k = 0; k < PICL_NFLD / 2; k++)
     o[k] = tri ? make_double2(f[2 * k], f[2 * k + 1]) : make_double2(sum8(f[2 * k]) / 8, sum8(f[2 * k + 1]) / 8);
+  if (P.rot3) {
+    // PICL_TEMP_Runge.F90:747-809: vorticity and its material derivative
+    // (stretching, dilatation, baroclinic, the feedback force's curl and its
+    // misalignment with grad(rho phi))
+    double dudx = gg[3], dudy = gg[4], dudz = gg[5], dvdx = gg[6], dvdy = gg[7], dvdz = gg[8];
+    double dwdx = gg[9], dwdy = gg[10], dwdz = gg[11];
+    double omgx = dwdy - dvdz, omgy = dudz - dwdx, omgz = dvdx - dudy;
+    double divu = dudx + dvdy + dwdz;
+    double dprdx = gg[0], dprdy = gg[1], dprdz = gg[2], dpdx = gg[12], dpdy = gg[13], dpdz = gg[14];
+    const double *fg = P.fbg + 9 * (size_t)i;
+    double F[3] = {P.fb[i], P.fb[(size_t)N + i], P.fb[2 * (size_t)N + i]};
+    double ir = 1.0 / cvB[0], ir2 = ir * ir;
+    double dodx = omgx * dudx + omgy * dudy + omgz * dudz;
+    double dody = omgx * dvdx + omgy * dvdy + omgz * dvdz;
+    double dodz = omgx * dwdx + omgy * dwdy + omgz * dwdz;
+    dodx = dodx - omgx * divu;
+    dody = dody - omgy * divu;
+    dodz = dodz - omgz * divu;
+    dodx = dodx + (dprdy * dpdz - dprdz * dpdy) * ir2;
+    dody = dody + (dprdz * dpdx - dprdx * dpdz) * ir2;
+    dodz = dodz + (dprdx * dpdy - dprdy * dpdx) * ir2;
+    dodx = dodx + (fg[7] - fg[5]) * ir;  // dFz/dy - dFy/dz
+    dody = dody + (fg[2] - fg[6]) * ir;  // dFx/dz - dFz/dx
+    dodz = dodz + (fg[3] - fg[1]) * ir;  // dFy/dx - dFx/dy
+    dodx = dodx + (dprdy * F[2] - dprdz * F[1]) * ir2;
+    dody = dody + (dprdz * F[0] - dprdx * F[2]) * ir2;
+    dodz = dodz + (dprdx * F[1] - dprdy * F[0]) * ir2;
+    double fr[6] = {omgx, omgy, omgz, dodx, dody, dodz};
+    double *ro = P.fldRot + 6 * (size_t)i;
+#pragma unroll
+    for (int q = 0; q < 6; q++) ro[q] = tri ? fr[q] : sum8(fr[q]) / 8;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,6 +1204,173 @@ __global__ void __launch_bounds__(BLK) k_wdot(PiclDev P) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// COLLISIONS 3-4: hydrodynamic and undisturbed torque
+// (ppiclf_user_Torque.f:72-197), Saffman and Magnus lift
+// (ppiclf_user_Lift.f:61-181).  vr: the interpolated vorticity, then its
+// material derivative; om: the particle's angular velocity.  Same
+// operations as the oracle.
+// ---------------------------------------------------------------------------
+// Torque_Hydro; false when the rotational Reynolds number reaches 100, where
+// ppiclF stops ("Re rotational too large")
+__device__ bool torque_hydro(const double vr[6], const double om[3], double rhof, double dp, double rmu,
+                             double rmass, double rhop, double t[3]) {
+  const double rpi = 3.141592653589793116;  // acos(-1.0d0)
+  double omgrx = 0.5 * vr[0] - om[0], omgry = 0.5 * vr[1] - om[1], omgrz = 0.5 * vr[2] - om[2];
+  double omgr_mag = sqrt(omgrx * omgrx + omgry * omgry + omgrz * omgrz);
+  double reyr = rhof * dp * dp * omgr_mag / (4.0 * rmu);
+  reyr = fmax(reyr, 0.001);
+  double Ct1, Ct2, Ct3;
+  if (reyr < 1.0) {
+    Ct1 = 0.0; Ct2 = 16.0 * rpi; Ct3 = 0.0;
+  } else if (reyr < 10.0) {
+    Ct1 = 0.0; Ct2 = 16.0 * rpi; Ct3 = 0.0418;
+  } else if (reyr < 20.0) {
+    Ct1 = 5.32; Ct2 = 37.2; Ct3 = 0.0;
+  } else if (reyr < 50.0) {
+    Ct1 = 6.44; Ct2 = 32.2; Ct3 = 0.0;
+  } else if (reyr < 100.0) {
+    Ct1 = 6.45; Ct2 = 32.1; Ct3 = 0.0;
+  } else {
+    t[0] = t[1] = t[2] = 0.0;
+    return false;
+  }
+  double Ct = Ct1 / sqrt(reyr) + Ct2 / reyr + Ct3 * reyr;
+  double beta = rhop / rhof;
+  double rIp = rmass * dp * dp / 10.0;
+  double factor = rIp * 60.0 * Ct * omgr_mag / (64.0 * rpi * beta);
+  t[0] = factor * omgrx;
+  t[1] = factor * omgry;
+  t[2] = factor * omgrz;
+  return true;
+}
+
+// Lift_Saffman then Lift_Magnus into lift (v: relative velocity; rep
+// already capped at 0.1; rnu = rmu / rhof)
+__device__ void lift_forces(const double vr[6], const double om[3], const double v[3], double vmag, double dp,
+                            double rep, double rmu, double rhof, double rnu, double lift[3]) {
+  if (vmag < 1.0e-8) return;
+  {
+    double ox = vr[0], oy = vr[1], oz = vr[2];
+    double omg_mag = sqrt(ox * ox + oy * oy + oz * oz);
+    double omg_star = omg_mag * dp / vmag;
+    double epi = sqrt(omg_star / rep);
+    double d1 = 1.0 + tanh(2.5 * (log10(epi) + 0.191));
+    double d2 = 0.667 + tanh(6.0 * epi - 1.92);
+    double Jepi = 0.3 * d1 * d2;
+    double factor = 1.615 * rmu * (dp * dp) * vmag * sqrt(omg_mag / rnu);
+    double elx = v[1] * oz - v[2] * oy, ely = v[2] * ox - v[0] * oz, elz = v[0] * oy - v[1] * ox;
+    double elm = sqrt(elx * elx + ely * ely + elz * elz);
+    elm = fmax(1.0e-20, elm);
+    double ielm = 1.0 / elm;
+    lift[0] = lift[0] + factor * Jepi * elx * ielm;
+    lift[1] = lift[1] + factor * Jepi * ely * ielm;
+    lift[2] = lift[2] + factor * Jepi * elz * ielm;
+  }
+  {
+    double ox = om[0], oy = om[1], oz = om[2];
+    double omg_mag = sqrt(ox * ox + oy * oy + oz * oz);
+    double epi = omg_mag * dp / vmag;
+    double d1 = 0.675 + 0.15 * (1.0 + tanh(0.28 * (epi - 2.0)));
+    double CL = 1.0 - d1 * tanh(F32(0.18) * sqrt(rep));
+    double factor = 0.125 * dp * dp * dp * rhof;
+    double elx = v[1] * oz - v[2] * oy, ely = v[2] * ox - v[0] * oz, elz = v[0] * oy - v[1] * ox;
+    lift[0] = lift[0] + factor * CL * elx;
+    lift[1] = lift[1] + factor * CL * ely;
+    lift[2] = lift[2] + factor * CL * elz;
+  }
+}
+
+// PICL_TEMP_Runge.F90:618-663 for COLLISIONS >= 3: the feedback force per
+// cell from the previous stage's projection (the sums k_feedback forms)...
+__global__ void __launch_bounds__(BLK) k_fb_cells(Dev d, PiclDev P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  double vol = d.vol[i];
+#pragma unroll
+  for (int k = 0; k < 3; k++) P.fb[k * (size_t)d.nC + i] = sum_nodes(P, 1 + k, i) * 0.125 * vol;
+}
+
+// ... its LSQ gradient over the cell stencil (RFLU_ComputeGradCellsWrapper,
+// k_grad2's weights and order) ...
+__global__ void __launch_bounds__(BLK) k_fb_grad(Dev d, PiclDev P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  const int N = d.nC;
+  double r11 = d.rm[i], r12 = d.rm[N + i], r22 = d.rm[2 * N + i];
+  double r13 = d.rm[3 * N + i], r23 = d.rm[4 * N + i], r33 = d.rm[5 * N + i];
+  double c11 = 1.0 / r11, c22 = 1.0 / r22, c33 = 1.0 / r33;
+  double c12 = -(c11 * r12);
+  double c13 = -(c11 * r13 + c12 * c22 * r23);
+  double c23 = -(c22 * r23);
+  double c1111 = c11 * c11, c2222 = c22 * c22, c3333 = c33 * c33;
+  const double xi = d.cofg[i], yi = d.cofg[N + i], zi = d.cofg[2 * N + i];
+  double fi[3], g[9];
+#pragma unroll
+  for (int k = 0; k < 3; k++) fi[k] = P.fb[k * (size_t)N + i];
+#pragma unroll
+  for (int q = 0; q < 9; q++) g[q] = 0.0;
+  const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
+  for (int s = d.csOff[i]; s < d.csOff[i + 1]; s++) {
+    int j = (int)(mem[s] & CS_IDX);
+    double dx = d.cofg[j] - xi, dy = d.cofg[N + j] - yi, dz = d.cofg[2 * N + j] - zi;
+    double term = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    dx = term * dx; dy = term * dy; dz = term * dz;
+    double term1 = c1111 * (dx);
+    double term2 = c2222 * (dy + c12 * dx);
+    double term3 = c3333 * (dz + c23 * dy + c13 * dx);
+    double wtx = term * (term1 + c12 * term2 + c13 * term3);
+    double wty = term * (term2 + c23 * term3);
+    double wtz = term * (term3);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double dVar = P.fb[k * (size_t)N + j] - fi[k];
+      g[3 * k] = g[3 * k] + wtx * dVar;
+      g[3 * k + 1] = g[3 * k + 1] + wty * dVar;
+      g[3 * k + 2] = g[3 * k + 2] + wtz * dVar;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; q++) P.fbRaw[9 * (size_t)i + q] = g[q];
+}
+
+// ... then WENO-XYZ per component (self first, then the stencil members) and
+// RFLU_LimitGradCellsSimple with the flags (1, 2, 3): only the x force is
+// VAR_INFO_POS, its gradient zeroed when an extrapolation to a face
+// centroid is <= 0
+__global__ void __launch_bounds__(BLK) k_fb_weno(Dev d, PiclDev P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  const int N = d.nC;
+  const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
+  double g[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    double term = P.fbRaw[9 * (size_t)i + q];
+    double sum = 1.0 / (term * term + 1.0E-15);
+    double loc = sum * term;
+    for (int s = d.csOff[i]; s < d.csOff[i + 1]; s++) {
+      term = P.fbRaw[9 * (size_t)(mem[s] & CS_IDX) + q];
+      double w = 1.0 / (term * term + 1.0E-15);
+      sum = sum + w;
+      loc = loc + w * term;
+    }
+    g[q] = loc / sum;
+  }
+  double varc = P.fb[i];
+  double xc = d.cofg[i], yc = d.cofg[N + i], zc = d.cofg[2 * N + i];
+  bool zero = false;
+  for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+    int slot = d.cfIdx[e] >> 1;
+    double dx = d.fcx[slot] - xc, dy = d.fcy[slot] - yc, dz = d.fcz[slot] - zc;
+    double varf = varc + g[0] * dx + g[1] * dy + g[2] * dz;
+    if (varf <= 0.0) zero = true;
+  }
+  if (zero) g[0] = g[1] = g[2] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 9; q++) P.fbg[9 * (size_t)i + q] = g[q];
+}
+
 // ppiclf_user_VU_Rocflu (ppiclf_user_ViscousUnsteady.f:19-90): the history
 // kernel (Mei and Adrian's, with ppiclF's volume-fraction factor) summed over
 // entries 2..nTimeBH-1, half weight on the oldest; entry 1 (this stage's)
@@ -1216,10 +1422,14 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   PBuf C = cur_buf(P);
   double *st = C.st;
   double v[PICL_NFLD];
+  double vr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // vorticity, D(omega)/Dt (COLLISIONS >= 3)
   if (phase == 2) {
     if (C.elem[t] < 0) return;
 #pragma unroll
     for (int f = 0; f < PICL_NFLD; f++) v[f] = P.vint[f * nP + t];
+    if (P.rot3)
+#pragma unroll
+      for (int q = 0; q < 6; q++) vr[q] = P.vint[(PICL_NFLD + q) * nP + t];
   } else {
   double xp[3] = {st[t], st[nP + t], st[2 * nP + t]};
   int e = picl_locate(P, xp, C.elem[t]);
@@ -1255,6 +1465,16 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       double wa = h[1][0] * ua + h[1][1] * ua, wb = h[1][0] * ub + h[1][1] * ub;
       v[2 * f] = h[2][0] * wa + h[2][1] * wa;
       v[2 * f + 1] = h[2][0] * wb + h[2][1] * wb;
+    }
+    if (P.rot3) {
+      const double *rp = P.fldRot + 6 * (size_t)e;
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        double a = rp[q];
+        double ua = h[0][0] * a + h[0][1] * a;
+        double wa = h[1][0] * ua + h[1][1] * ua;
+        vr[q] = h[2][0] * wa + h[2][1] * wa;
+      }
     }
   } else {
   // ---- the 27 nearest element centroids, ordered by (d2, element number)
@@ -1330,11 +1550,19 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
         v[2 * f] = v[2 * f] + w * a.x;
         v[2 * f + 1] = v[2 * f + 1] + w * a.y;
       }
+      if (P.rot3) {
+        const double *rp = P.fldRot + 6 * (size_t)P.inv[kr[k]];
+#pragma unroll
+        for (int q = 0; q < 6; q++) vr[q] = vr[q] + w * rp[q];
+      }
       wsum = wsum + w;
     }
   }
 #pragma unroll
   for (int f = 0; f < PICL_NFLD; f++) v[f] = v[f] / wsum;
+  if (P.rot3)
+#pragma unroll
+    for (int q = 0; q < 6; q++) vr[q] = vr[q] / wsum;
   }
   if (P.dbg) {
     int pid = C.pid[t];
@@ -1344,6 +1572,9 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   if (phase == 1) {
 #pragma unroll
     for (int f = 0; f < PICL_NFLD; f++) P.vint[f * nP + t] = v[f];
+    if (P.rot3)
+#pragma unroll
+      for (int q = 0; q < 6; q++) P.vint[(PICL_NFLD + q) * nP + t] = vr[q];
     // what ppiclf_user.f:934-990 reads: SDrho's numerator with the
     // start-of-stage velocity, the relative velocity and the rphif that
     // SetYdot's particle loop leaves (ppiclf_user.f:316-361), by index
@@ -1438,12 +1669,21 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
     }
     double fvu[3] = {0.0, 0.0, 0.0};
     if (Ph.vu) vu_force(st, nP, t, P.vuT, d.scal[S_DTMIN], rmu, rhof, dpp, vmag, rphip, fvu);
+    double om[3] = {0.0, 0.0, 0.0};  // y(JOX..JOZ) at the start of the stage
+    if (P.rot)
+#pragma unroll
+      for (int c = 0; c < 3; c++) om[c] = st[(ROW_O + c) * nP + t];
+    double lift[3] = {0.0, 0.0, 0.0};  // Step 9b (ppiclf_user.f:640-646)
+    if (Ph.col == 4) {
+      double vrel[3] = {vxr, vyr, vzr};
+      lift_forces(vr, om, vrel, vmag, dpp, rep, rmu, rhof, rmu / rhof, lift);
+    }
     ydn[0] = yv[0];
     ydn[1] = yv[1];
     ydn[2] = yv[2];
-    ydn[3] = (fqsx + famx + fdpdx + fvu[0] + 0.0 + fcv[0]) / (rmass + rmass_add);
-    ydn[4] = (fqsy + famy + fdpdy + fvu[1] + 0.0 + fcv[1]) / (rmass + rmass_add);
-    ydn[5] = (fqsz + famz + fdpdz + fvu[2] + 0.0 + fcv[2]) / (rmass + rmass_add);
+    ydn[3] = (fqsx + famx + fdpdx + fvu[0] + lift[0] + fcv[0]) / (rmass + rmass_add);
+    ydn[4] = (fqsy + famy + fdpdy + fvu[1] + lift[1] + fcv[1]) / (rmass + rmass_add);
+    ydn[5] = (fqsz + famz + fdpdz + fvu[2] + lift[2] + fcv[2]) / (rmass + rmass_add);
     if (Ph.vu) {  // ppiclf_user_UpdatePlag (ppiclf_user_ViscousUnsteady.f:146-206)
       double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
       SDrho = SDrho / rphif;
@@ -1461,11 +1701,19 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       ydT = qq / (rmass * Ph.rcpPart);
       st[ROW_TYD * nP + t] = ydT;
     }
+    double tauh[3] = {0.0, 0.0, 0.0};  // Torque_Hydro
     if (P.rot) {  // Step 9a (:638-646): ydot(JOX..JOZ) = tau / (rmass dp^2 / 10)
       double rmass_omega = rmass * dpp * dpp / 10.0;
+      double tauu[3] = {0.0, 0.0, 0.0};
+      if (P.rot3) {  // Torque_driver (ppiclf_user_Torque.f:23-62)
+        if (!torque_hydro(vr, om, rhof, dpp, rmu, rmass, rhop, tauh)) atomicOr(d.err, E_REROT);
+        double rIf = rhof * dpp * dpp * volp / 10.0;  // Torque_Undisturbed
+#pragma unroll
+        for (int c = 0; c < 3; c++) tauu[c] = 0.5 * rIf * vr[3 + c];
+      }
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        ydO[c] = (P.tq[c * nP + t] + 0.0 + 0.0) / rmass_omega;
+        ydO[c] = (P.tq[c * nP + t] + tauh[c] + tauu[c]) / rmass_omega;
         st[(ROW_OYD + c) * nP + t] = ydO[c];
       }
     }
@@ -1473,8 +1721,9 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       yc[0] = spl * (ydn[3] * rmass - fcv[0]);
       yc[1] = spl * (ydn[4] * rmass - fcv[1]);
       yc[2] = spl * (ydn[5] * rmass - fcv[2]);
-      yc[3] = spl * ((fqsx + fvu[0] + 0.0) * yv[0] + (fqsy + fvu[1] + 0.0) * yv[1] + (fqsz + fvu[2] + 0.0) * yv[2] +
-                     famx * v[1] + famy * v[2] + famz * v[3] + 0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0 + qq);
+      yc[3] = spl * ((fqsx + fvu[0] + lift[0]) * yv[0] + (fqsy + fvu[1] + lift[1]) * yv[1] +
+                     (fqsz + fvu[2] + lift[2]) * yv[2] + famx * v[1] + famy * v[2] + famz * v[3] + tauh[0] * om[0] +
+                     tauh[1] * om[1] + tauh[2] * om[2] + qq);
     } else {  // ydotc keeps what the collisions added into it
 #pragma unroll
       for (int c = 0; c < 3; c++) yc[c] = fcv[c];
@@ -2178,7 +2427,8 @@ void picl_free(PiclState *s) {
                   P.fld, P.gRho, P.proj, P.rexp, P.cen, P.X, P.bbox, P.refId, P.inv, P.lbOff, P.lbMem, P.cbOff,
                   P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
                   P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq, P.gBox, P.gvList,
-                  P.buf[0].w, P.buf[1].w, P.vint, P.famB, P.snum, P.vrelP, P.rphifP, P.actP, P.lastP};
+                  P.buf[0].w, P.buf[1].w, P.vint, P.famB, P.snum, P.vrelP, P.rphifP, P.actP, P.lastP,
+                  P.fb, P.fbRaw, P.fbg, P.fldRot};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -2250,17 +2500,15 @@ void picl_particle_stage(rflu_handle *h, int istage) {
     if (P.am2) hipLaunchKernelGGL(k_col_sort, dim3(nblk(P.colNb)), dim3(BLK), 0, h->stream, P);
     if (P.col) hipLaunchKernelGGL(k_contacts, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
   }
-  {
-    ProfScope ps(h, "k_particles");
+  if (P.am2) {  // the particle kernel's first phase: locate and interpolate
+    ProfScope ps(h, "k_interp");
     using KP = void (*)(Dev, PiclDev, PiclPhys, int, int);
     static const KP kp[2][5] = {
         {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
         {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
-    KP f = kp[s->phys.interpMode][s->phys.qsFlag - 1];
-    if (P.am2) {
-      hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lastP, 1);
-      hipLaunchKernelGGL(f, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys, istage, 1);
-    }
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lastP, 1);
+    hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
+                       P, s->phys, istage, 1);
   }
   if (P.am2) {
     ProfScope ps(h, "k_am_pairs");
@@ -2285,6 +2533,12 @@ void picl_particle_stage(rflu_handle *h, int istage) {
 int picl_stage(rflu_handle *h, int istage) {
   PiclState *s = h->picl;
   PiclDev &P = s->P;
+  if (P.rot3) {  // the feedback force's gradient for D(omega)/Dt
+    ProfScope ps(h, "k_fb_grad");
+    hipLaunchKernelGGL(k_fb_cells, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P);
+    hipLaunchKernelGGL(k_fb_grad, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P);
+    hipLaunchKernelGGL(k_fb_weno, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, P);
+  }
   {
     ProfScope ps(h, "k_picl_fields");
     hipLaunchKernelGGL(k_picl_fields, dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, P,
@@ -2379,10 +2633,10 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (!(pp->neighborWidth >= 0.0)) return set_err(h, RFLU_GPU_EINVAL, "neighborWidth must be >= 0");
   if (pp->viscousUnsteady < 0 || pp->viscousUnsteady > 1)
     return set_err(h, RFLU_GPU_EINVAL, "viscousUnsteady must be 0 or 1");
-  if (pp->collisionalFlag < 0 || pp->collisionalFlag > 2)
+  if (pp->collisionalFlag < 0 || pp->collisionalFlag > 4)
     return set_err(h, RFLU_GPU_EINVAL,
-                   "collisionalFlag must be 0 (off), 1 (normal force) or 2 (+ tangential force and contact "
-                   "torques); the hydrodynamic torque and lift (3-4) are not built");
+                   "collisionalFlag must be 0 (off), 1 (normal force), 2 (+ tangential force and contact torques), "
+                   "3 (+ hydrodynamic and undisturbed torque) or 4 (+ Saffman and Magnus lift)");
   if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0))
     return set_err(h, RFLU_GPU_EINVAL, "collisions need ksp > 0 and 0 < erest <= 1");
   if (nPart < 1) return set_err(h, RFLU_GPU_EINVAL, "no particles");
@@ -2421,6 +2675,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   s->phys.ksp = pp->ksp;
   s->phys.erest = pp->erest;
   s->P.am2 = pp->amFlag == 2 ? 1 : 0;
+  s->P.rot3 = pp->collisionalFlag >= 3 ? 1 : 0;
+  s->P.nVint = PICL_NFLD + (s->P.rot3 ? 6 : 0);
   s->P.grid = s->P.col || s->P.am2 ? 1 : 0;
   s->P.nRow = s->P.vu ? NROW_VU : (s->P.rot ? ROW_VU : NROW);
   s->y0.assign(y, y + 12 * (size_t)nP);
@@ -2555,7 +2811,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     std::vector<double> z3(3 * (size_t)nP, 0.0);
     std::vector<int> zi((size_t)nP, 0);
     for (int b = 0; b < 2; b++) rc |= dalloc(h, &P.buf[b].w, 3 * (size_t)nP, z3.data());
-    rc |= dalloc(h, &P.vint, PICL_NFLD * (size_t)nP);
+    rc |= dalloc(h, &P.vint, (size_t)P.nVint * nP);
     rc |= dalloc(h, &P.famB, 4 * (size_t)nP);
     rc |= dalloc(h, &P.snum, (size_t)nP);
     rc |= dalloc(h, &P.vrelP, 3 * (size_t)nP, z3.data());
@@ -2567,6 +2823,12 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     std::vector<double> t0(VU_N + 1, 0.0);
     t0[VU_N] = 1.0;
     rc |= dalloc(h, &P.vuT, t0.size(), t0.data());
+  }
+  if (P.rot3) {
+    rc |= dalloc(h, &P.fb, 3 * (size_t)N);
+    rc |= dalloc(h, &P.fbRaw, 9 * (size_t)N);
+    rc |= dalloc(h, &P.fbg, 9 * (size_t)N);
+    rc |= dalloc(h, &P.fldRot, 6 * (size_t)N);
   }
   rc |= dalloc(h, &P.ycN, 4 * (size_t)nP);
   rc |= dalloc(h, &P.src, (size_t)nP);

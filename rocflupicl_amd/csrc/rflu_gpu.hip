@@ -92,7 +92,7 @@ struct Dev {
 // scal[] slots
 enum { S_DTLOCAL = 0, S_DTMIN = 1, S_TIME = 2, S_TSR = 3 };
 // err bits
-enum { E_NAN = 1, E_NEG = 2, E_POSFIX = 4, E_BOUNDFIX = 8, E_INTERP = 16, E_CONTACT = 32 };
+enum { E_NAN = 1, E_NEG = 2, E_POSFIX = 4, E_BOUNDFIX = 8, E_INTERP = 16, E_CONTACT = 32, E_REROT = 64 };
 // boundary kinds on device: 0 slip wall, 1..3 outflow (1 + bcOpt)
 }  // namespace
 
@@ -2388,6 +2388,8 @@ int check_err(rflu_handle *h) {
   if (e & E_NEG) return set_err(h, RFLU_GPU_ENEGATIVE, "RFLU_CheckPositivity: negative positive-definite variables");
   if (e & E_INTERP) return set_err(h, RFLU_GPU_EINVAL, "ppiclf LocalInterp: a particle has no element centroid within reach");
   if (e & E_CONTACT) return set_err(h, RFLU_GPU_EINVAL, "collisions: a particle overlaps more than 32 neighbours");
+  if (e & E_REROT)
+    return set_err(h, RFLU_GPU_EINVAL, "ppiclf Torque_Hydro: Re rotational too large (ppiclf_user_Torque.f:125)");
   return 0;
 }
 

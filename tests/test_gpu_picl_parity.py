@@ -416,6 +416,36 @@ def test_coupled_steps_with_tangential_collisions(oracle_mod):
     assert ok, err
 
 
+@pytest.mark.parametrize("level", [3, 4])
+def test_coupled_steps_with_hydrodynamic_torque_and_lift(oracle_mod, level):
+    """COLLISIONS 3 adds the hydrodynamic and undisturbed torques, from the
+    gas vorticity and its material derivative (with the feedback force's
+    LSQ/WENO/limited gradient); 4 adds the Saffman and Magnus lift.  Random
+    spins, the dense curtain, periodic y and z, and a jittered mesh so the
+    vorticity is not zero."""
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=2000, jitter=0.05, seed=4, phi=0.3)
+    y = y.copy()
+    y[:, 7:10] = np.random.default_rng(9).uniform(-3e4, 3e4, (len(y), 3))
+    picl.update(collisionalFlag=level, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
+    g = rflu.Region(mesh, **params)
+    o = oracle_mod.Region(mesh, **params)
+    cv = ic(g.mesh_arrays()["cofg"])
+    g.set_state(cv)
+    o.set_cv(cv)
+    g.picl_init(y, pc, **picl)
+    o.picl_init(y, pc, **picl)
+    g.RFLU_TimeStepping(4, use_graph=True)
+    o.step(4)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(o.picl_rprop()[:, 30:33]).max() > 0.0  # vorticity reaches the particles
+    for k in ("y", "y1", "ydot", "ydotc"):
+        ok, err = close(gp[k], op[k])
+        assert ok, (k, err)
+    ok, err = close(g.get_state()[0], o.cv())
+    assert ok, err
+
+
 def test_collision_list_overflow_is_reported():
     """Forty particles packed within a tenth of a diameter of one point each
     overlap 39 neighbours, more than the 32 a particle's contact list holds:
@@ -432,12 +462,12 @@ def test_collision_list_overflow_is_reported():
         g.RFLU_TimeStepping(1, use_graph=False)
 
 
-@pytest.mark.parametrize("bad", [dict(collisionalFlag=3), dict(collisionalFlag=4), dict(viscousUnsteady=2),
+@pytest.mark.parametrize("bad", [dict(collisionalFlag=5), dict(amFlag=3), dict(amFlag=2, neighborWidth=-1.0),
+                                 dict(viscousUnsteady=2),
                                  dict(collisionalFlag=1, ksp=0.0), dict(collisionalFlag=1, erest=0.0),
                                  dict(collisionalFlag=1, erest=1.5), dict(periodicY=2), dict(heatTransfer=5)])
 def test_unbuilt_or_invalid_particle_physics_is_rejected(bad):
-    """Options outside what is built (the hydrodynamic torque and lift of
-    COLLISIONS 3-4) or outside their range fail at set-up with an error."""
+    """Options outside their range fail at set-up with an error."""
     mesh, params, ic, (y, pc), picl = configs.curtain(nx=8, ny=4, nz=4, npart=50, seed=4)
     picl.update(bad)
     g = rflu.Region(mesh, **params)
