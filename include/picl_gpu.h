@@ -96,6 +96,21 @@ typedef struct picl_params {
    * add pair terms.  rocpicl passes max(4 d_max, NEIGHBORWIDTH)
    * (PICL_TEMP_InitSolver.F90:655-661); 0 here means 4 d_max. */
   double neighborWidth;
+  /* QSFLUCT (qs_fluct_flag, ppiclf_user_Fluctuations.f:32-418): 0 off; 1 the
+   * Lattanzi and 2 the Osnes fluctuating quasi-steady force, an SDE step per
+   * RK stage from the granular temperature of the neighbours within a box of
+   * half-width fluctHalfWidth (ppiclf_d2chk(2); 0 means rocpicl's
+   * 2 x the last element's largest extent, PICL_TEMP_InitSolver.F90:598-646)
+   * inside the neighbour width.  qsFluctAdapt 1 (QSFLUCTFILTADAPT) widens the
+   * box to the particle's adaptive filter.  The box filter only: the
+   * Gaussian one (QSFLUCTFILT 1) reads a variable the reference never sets
+   * (ppiclf_user_EvalNearestNeighbor.f:50,364).  The normal draws come from
+   * a counter-based generator seeded by fluctSeed, not from gfortran's
+   * RANDOM_NUMBER stream. */
+  int qsFluct;
+  int qsFluctAdapt;
+  double fluctHalfWidth;
+  long long fluctSeed;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

@@ -35,7 +35,8 @@ class OrcPiclParams(C.Structure):
                 ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
                 ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
                 ("viscousUnsteady", C.c_int), ("collisionalFlag", C.c_int), ("ksp", C.c_double),
-                ("erest", C.c_double), ("neighborWidth", C.c_double)]
+                ("erest", C.c_double), ("neighborWidth", C.c_double), ("qsFluct", C.c_int),
+                ("qsFluctAdapt", C.c_int), ("fluctHalfWidth", C.c_double), ("fluctSeed", C.c_longlong)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,
@@ -114,6 +115,7 @@ def lib():
         L.orc_picl_locate.argtypes = [P, I, P, P]
         L.orc_picl_neighbors.argtypes = [P, P, P, P]
         L.orc_picl_neighbors.restype = I
+        L.orc_picl_fluct_means.argtypes = [P, I, P]
         L.orc_picl_get_nodes.argtypes = [P, P]
         L.orc_picl_qs_beta.argtypes = [I] + [C.c_double] * 7
         L.orc_picl_qs_beta.restype = C.c_double
@@ -133,6 +135,9 @@ def lib():
         L.orc_picl_torque_hydro.argtypes = [P, P, D, D, D, D, D, P]
         L.orc_picl_torque_hydro.restype = I
         L.orc_picl_lift.argtypes = [P, P, P, D, D, D, D, P]
+        L.orc_fluct_uniform.argtypes = [C.c_longlong, I, I, I]
+        L.orc_fluct_uniform.restype = D
+        L.orc_fluct_force.argtypes = [I, P, P, P] + [D] * 4 + [P, D, D, D, P]
         L.orc_set_threads.argtypes = [I]
         L.orc_get_threads.restype = I
         L.orc_test_rk3_linear.argtypes = [D, D, P]
@@ -345,6 +350,12 @@ class Region:
         lib().orc_picl_locate(self.h, xyz.shape[0], _p(xyz), _p(out))
         return out
 
+    def picl_fluct_means(self, i):
+        """Particle i's fluctuation box sums (u, v, w, u^2, v^2, w^2, count)."""
+        out = np.zeros(7)
+        lib().orc_picl_fluct_means(self.h, int(i), _p(out))
+        return out
+
     def picl_neighbors(self, xp):
         xp = np.ascontiguousarray(xp, np.float64)
         ids = np.zeros(27, np.int32)
@@ -455,6 +466,20 @@ def lift(vort, spin, vrel, dp, rep, rmu, rhof):
     a = [np.ascontiguousarray(v, dtype=np.float64) for v in (vort, spin, vrel)]
     out = np.zeros(3)
     lib().orc_picl_lift(_p(a[0]), _p(a[1]), _p(a[2]), float(dp), float(rep), float(rmu), float(rhof), _p(out))
+    return out
+
+
+def fluct_uniform(seed, call, pid, k):
+    """The uniform draw k of particle pid's SetYdot call (stands in for RANDOM_NUMBER)."""
+    return lib().orc_fluct_uniform(int(seed), int(call), int(pid), int(k))
+
+
+def fluct_force(flag, means, F, U, rphip, rmachp, rep, vmag, vrel, rmu, dp, fac):
+    """One SDE step of the fluctuating force (1 Lattanzi, 2 Osnes)."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (means, F, U, vrel)]
+    out = np.zeros(3)
+    lib().orc_fluct_force(int(flag), _p(a[0]), _p(a[1]), _p(a[2]), float(rphip), float(rmachp), float(rep),
+                          float(vmag), _p(a[3]), float(rmu), float(dp), float(fac), _p(out))
     return out
 
 

@@ -17,6 +17,7 @@
 #include "rflu_oracle_int.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,7 +28,7 @@ enum {
   R_JRHOP = 0, R_JRHOF = 1, R_JDP = 2, R_JVOLP = 3, R_JPHIP = 4, R_JUX = 5, R_JUY = 6, R_JUZ = 7,
   R_JCS = 8, R_JDPDX = 9, R_JDPDY = 10, R_JDPDZ = 11, R_JSDRX = 12, R_JSDRY = 13, R_JSDRZ = 14,
   R_JRHSR = 15, R_JPGCX = 16, R_JPGCY = 17, R_JPGCZ = 18, R_JSPL = 21, R_JT = 23,
-  R_WDOTX = 27, R_JXVOR = 30, R_JSDOX = 42, R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
+  R_FLUCTFX = 24, R_WDOTX = 27, R_JXVOR = 30, R_JSDOX = 42, R_JRHOGX = 36, R_JRHOGY = 37, R_JRHOGZ = 38
 };
 enum { P_JPHIP = 0, P_JFX, P_JFY, P_JFZ, P_JE, NPRO };
 #define LRS 12
@@ -67,6 +68,9 @@ struct orc_picl {
   double nbW;      /* ppiclf_d2chk(3): the added-mass pair terms' reach */
   double *fldRot;  /* COLLISIONS >= 3: vorticity and D(omega)/Dt per element */
   int rotErr;      /* a rotational Reynolds number reached 100 this stage */
+  double fluctW;   /* ppiclf_d2chk(2): the fluctuation box filter's half-width */
+  int nCalls;      /* SetYdot calls so far: the counter of the uniform draws */
+  int curStage;    /* the RK stage of the current SetYdot call */
 };
 
 /* The rotational fields (rprop JXVOR..JZVOR, JSDOX..JSDOZ, 0-based) */
@@ -1198,6 +1202,157 @@ void orc_picl_lift(const double *vort, const double *spin, const double *vrel, d
   lift_forces(rp, y, vrel, vmag, dp, rep, rmu, rhof, rmu / rhof, lift);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Quasi-steady force fluctuations (QSFLUCT 1 Lattanzi, 2 Osnes):           */
+/* ppiclf_user_Fluctuations.f:32-418, the neighbour means of                 */
+/* ppiclf_user.f:411-468 and ppiclf_user_EvalNearestNeighbor.f:318-360.      */
+/* ------------------------------------------------------------------------ */
+/* The six uniform draws of a particle's call stand in for RANDOM_NUMBER
+ * (libgfortran's generator and its call order, which no restatement can
+ * reproduce): a splitmix64 hash of (seed, call, original index, draw),
+ * uniform on (0, 1) with 53 bits. */
+double orc_fluct_uniform(long long seed, int call, int pid, int k) {
+  uint64_t n = ((((uint64_t)(uint32_t)call << 32) | (uint32_t)pid) << 3) + (uint64_t)(k + 1);
+  uint64_t z = (uint64_t)seed + 0x9E3779B97F4A7C15ull * n;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return ((double)(z >> 11) + 0.5) * 0x1.0p-53;
+}
+
+/* TwoSum accumulation: the rounded s + c does not depend on the order of
+ * the terms (in all but contrived cases), which the means need: their
+ * variance theta = <u^2> - <u>^2 cancels to rounding noise when the
+ * neighbours move together. */
+static void fl_acc(double *s, double *c, double x) {
+  double t = *s + x, z = t - *s;
+  *c = *c + ((*s - (t - z)) + (x - z));
+  *s = t;
+}
+
+/* The box-filter means' sums: the particle itself, then every neighbour
+ * (and periodic image) that passes ppiclf_solve_NearestNeighbor's filter
+ * (d2chk(3)) and lies within the box of half-width d2chk(2) (adaptive:
+ * max(d2chk(2), (10 d^3 / max(1e-4, phi))^(1/3) / 2)).  out = the sums of
+ * u, v, w, u^2, v^2, w^2, then the count. */
+static void fluct_means(const struct orc_picl *p, int i, double out[7]) {
+  const double *yi = p->y + LRS * (size_t)i, *rpi_ = p->rprop + LRP * (size_t)i;
+  const double w = p->nbW, dist2 = w * w;
+  double box = p->fluctW;
+  if (p->pp.qsFluctAdapt >= 1) {
+    double dpl = rpi_[R_JDP], phip = rpi_[R_JPHIP];
+    double adptfilter = pow(10.0 * ip3(dpl) / fmax(F32(1.e-4), phip), (double)(1.0f / 3.0f));
+    adptfilter = adptfilter / 2.0;
+    box = fmax(p->fluctW, adptfilter);
+  }
+  double S[6], Cc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 3; k++) {
+    S[k] = yi[JVX + k];
+    S[3 + k] = S[k] * S[k];
+  }
+  int cnt = 1;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int j = 0; j < p->n; j++) {
+      if (p->lost[j] || (pass == 0 && j == i)) continue;
+      for (int img = 0; img < (pass ? 27 : 1); img++) {
+        if (pass && img == 13) continue;
+        double yj[LRS];
+        memcpy(yj, p->y + LRS * (size_t)j, sizeof yj);
+        int skip = 0;
+        if (pass)
+          for (int d = 0, q = img; d < 3; d++, q /= 3) {
+            int sd = q % 3 - 1;
+            if (sd != 0 && !p->per[d]) skip = 1;
+            if (sd != 0) yj[d] = yj[d] + (double)sd * (p->perHi[d] - p->perLo[d]);
+          }
+        if (skip) continue;
+        double tot = (ip2(yi[JX] - yj[JX]) + ip2(yi[JY] - yj[JY])) + ip2(yi[JZ] - yj[JZ]);
+        if (tot > dist2) continue;
+        if (fabs(yi[JX] - yj[JX]) > box || fabs(yi[JY] - yj[JY]) > box || fabs(yi[JZ] - yj[JZ]) > box) continue;
+        for (int k = 0; k < 3; k++) {
+          fl_acc(&S[k], &Cc[k], yj[JVX + k]);
+          fl_acc(&S[3 + k], &Cc[3 + k], ip2(yj[JVX + k]));
+        }
+        cnt++;
+      }
+    }
+    if (!(p->per[0] || p->per[1] || p->per[2])) break;
+  }
+  for (int k = 0; k < 6; k++) out[k] = S[k] + Cc[k];
+  out[6] = (double)cnt;
+}
+
+/* ppiclf_user_QS_fluct_Lattanzi (flag 1) and _Osnes (flag 2): the SDE step
+ * of the fluctuating force from its previous value F (rprop FLUCTFX..Z),
+ * the means m, the six uniforms U and fac = rk3ark(stage) dt. */
+void orc_fluct_force(int flag, const double m[7], const double F[3], const double U[6], double rphip,
+                     double rmachp, double rep, double vmag, const double vrel[3], double rmu, double dp,
+                     double fac, double out[3]) {
+  const double rpi = acos(-1.0), TwoPi = 2.0 * acos(-1.0);
+  double denum = fmax(m[6], 1.0);
+  double upmean = m[0] / denum, vpmean = m[1] / denum, wpmean = m[2] / denum;
+  double u2pmean = m[3] / denum, v2pmean = m[4] / denum, w2pmean = m[5] / denum;
+  double fq = F32(6.52) * rphip - F32(22.56) * ip2(rphip) + F32(49.90) * ip3(rphip);
+  double Fs = 3.0 * rpi * rmu * dp * (1.0 + F32(0.15) * pow(rep * (1.0 - rphip), F32(0.687))) * (1.0 - rphip) * vmag;
+  double bq = fmin(sqrt(20.0 * rmachp), 1.0) * F32(0.55) * pow(rphip, F32(0.7)) *
+              (1.0 + tanh((rmachp - 0.5) / F32(0.2)));
+  double theta = ((u2pmean + v2pmean + w2pmean) - (ip2(upmean) + ip2(vpmean) + ip2(wpmean))) / 3.0;
+  if (theta <= 1.0e-12) theta = 0.0;
+  double chi = (1.0 + 2.50 * rphip + F32(4.51) * ip2(rphip) + F32(4.52) * ip3(rphip)) /
+               pow(1.0 - ip3(rphip / F32(0.64)), F32(0.68));
+  double tF_inv = (24.0 * rphip * chi / dp) * sqrt(theta / rpi);
+  double aSDE = tF_inv;
+  if (flag == 1) {
+    double sigF = (fq + bq) * Fs;
+    double bSDE = sigF * sqrt(2.0 * tF_inv);
+    for (int k = 0; k < 3; k++) {
+      double Z = sqrt(-2.0 * log(U[2 * k])) * cos(TwoPi * U[2 * k + 1]);
+      double dW = sqrt(fac) * Z;
+      out[k] = (1.0 - aSDE * fac) * F[k] + bSDE * dW;
+    }
+    return;
+  }
+  double sigD = (fq + bq) * Fs;
+  double bSDE_CD = sigD * sqrt(2.0 * tF_inv);
+  /* below 1e-8 m/s the axis is x and the transverse amplitude vanishes
+   * (bSDE_CD above keeps the unguarded sigD) (ppiclf_user_Fluctuations.f:317-328) */
+  double avec[3] = {1.0, 0.0, 0.0}, CD_frac = 0.0;
+  if (vmag > 1.0e-8) {
+    for (int k = 0; k < 3; k++) avec[k] = vrel[k] / vmag;
+    double CD_prime = F[0] * avec[0] + F[1] * avec[1] + F[2] * avec[2];
+    CD_frac = CD_prime / sigD;
+  } else {
+    sigD = 0.0;
+  }
+  double sigmoid_cf = 1.0 / (1.0 + exp(-CD_frac));
+  double f_CF = F32(0.39356905) * sigmoid_cf + F32(0.43758848);
+  double sigT = f_CF * sigD;
+  double bSDE_CL = sigT * sqrt(2.0 * tF_inv);
+  double eunit[3] = {1.0, 0.0, 0.0};
+  if (fabs(avec[1]) + fabs(avec[2]) <= 1.0e-8) {
+    eunit[0] = 0.0; eunit[1] = 1.0;
+  } else if (fabs(avec[0]) + fabs(avec[2]) <= 1.0e-8) {
+    eunit[0] = 0.0; eunit[2] = 1.0;
+  }
+  double bvec[3] = {avec[1] * eunit[2] - avec[2] * eunit[1], avec[2] * eunit[0] - avec[0] * eunit[2],
+                    avec[0] * eunit[1] - avec[1] * eunit[0]};
+  double dn = fmax(1.0e-8, sqrt(ip2(bvec[0]) + ip2(bvec[1]) + ip2(bvec[2])));
+  for (int k = 0; k < 3; k++) bvec[k] = bvec[k] / dn;
+  double cvec[3] = {avec[1] * bvec[2] - avec[2] * bvec[1], avec[2] * bvec[0] - avec[0] * bvec[2],
+                    avec[0] * bvec[1] - avec[1] * bvec[0]};
+  dn = fmax(1.0e-8, sqrt(ip2(cvec[0]) + ip2(cvec[1]) + ip2(cvec[2])));
+  for (int k = 0; k < 3; k++) cvec[k] = cvec[k] / dn;
+  double Z1 = sqrt(-2.0 * log(U[0])) * cos(TwoPi * U[1]);
+  double Z2 = sqrt(-2.0 * log(U[2])) * cos(TwoPi * U[3]);
+  double dW1 = sqrt(fac) * Z1, dW2 = sqrt(fac) * Z2;
+  double cosrand = cos(TwoPi * U[4]), sinrand = sin(TwoPi * U[4]);
+  double dvec[3];
+  for (int k = 0; k < 3; k++) dvec[k] = bvec[k] * cosrand + cvec[k] * sinrand;
+  dn = fmax(1.0e-8, sqrt(ip2(dvec[0]) + ip2(dvec[1]) + ip2(dvec[2])));
+  for (int k = 0; k < 3; k++) dvec[k] = dvec[k] / dn;
+  for (int k = 0; k < 3; k++) out[k] = (1.0 - aSDE * fac) * F[k] + bSDE_CD * dW1 * avec[k] + bSDE_CL * dW2 * dvec[k];
+}
+
 /* ppiclf_user_SetYdot (libpicl/user_files/ppiclf_user.f:39-1042) for one
  * particle with the quasi-steady, added-mass (ppiclf_user_AddedMass.f:16-124),
  * pressure-gradient, viscous-unsteady and normal collision forces and heat
@@ -1262,6 +1417,8 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
     /* Step 1b: the pair terms (ppiclf_user.f:411-468) */
     nneighbors = am_neighbours(p, i, rphip, Fam, Wsum);
   }
+  double fmeans[7];
+  if (P->qsFluct >= 1) fluct_means(p, i, fmeans); /* Step 1b, the fluctuation means */
   double tqc[3] = {0.0, 0.0, 0.0}; /* collision torques (ydotc(JOX..JOZ)) */
   if (P->collisionalFlag >= 1) { /* Step 1b and Step 6, ppiclf_user.f:411-470, 587-600 */
     double fc[3];
@@ -1285,9 +1442,18 @@ static void set_ydot(struct orc_picl *p, int i, double dt) {
     default: beta = qs_sn(rep, rmu, dp); break;
   }
   double fqsx = beta * vx, fqsy = beta * vy, fqsz = beta * vz;
-  fqsx = fqsx + 0.0; /* + fqs_fluct (zero) */
-  fqsy = fqsy + 0.0;
-  fqsz = fqsz + 0.0;
+  double fl[3] = {0.0, 0.0, 0.0};
+  if (P->qsFluct >= 1) { /* Step 3 (ppiclf_user.f:487-510) */
+    static const double rk3ark[3] = {8.0 / 15.0, 5.0 / 12.0, 0.75};
+    double fac = rk3ark[p->curStage - 1] * dt, U[6], vrel[3] = {vx, vy, vz};
+    for (int k = 0; k < 6; k++) U[k] = orc_fluct_uniform(P->fluctSeed, p->nCalls, i, k);
+    orc_fluct_force(P->qsFluct, fmeans, rp + R_FLUCTFX, U, rphip, rmachp, rep, vmag, vrel, rmu, dp, fac, fl);
+  }
+  fqsx = fqsx + fl[0];
+  fqsy = fqsy + fl[1];
+  fqsz = fqsz + fl[2];
+  if (P->qsFluct >= 1)
+    for (int k = 0; k < 3; k++) p->rprop[LRP * (size_t)i + R_FLUCTFX + k] = fl[k];
   if (P->amFlag == 1) {
     double rcd_am;
     if (rmachp > F32(0.6)) {
@@ -1454,9 +1620,11 @@ static void integrate_particle(struct orc_picl *p, int istage, double dt) {
                     {dt * 17.0 / 60.0, 1.0, dt * 5.0 / 12.0},
                     {dt * 5.0 / 12.0, 1.0, dt * 3.0 / 4.0}};
   init_solve(p, 1);
+  p->curStage = istage;
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < p->n; i++)
     if (!p->lost[i]) set_ydot(p, i, dt);
+  p->nCalls++;
   if (p->pp.amFlag == 2) am_store_wdot(p);
   if (p->pp.viscousUnsteady == 1 && istage == 3) vu_shift(p, dt);
   if (istage == 1) memset(p->y1, 0, sizeof(double) * LRS * (size_t)p->n);
@@ -1608,7 +1776,8 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   if (pp->qsFlag < 1 || pp->qsFlag > 5 || pp->interpMode < 0 || pp->interpMode > 1 || pp->projMode < 0 ||
       pp->projMode > 1 || pp->heatTransfer < 0 || pp->heatTransfer > 4 || pp->viscousUnsteady < 0 ||
       pp->viscousUnsteady > 1 || pp->collisionalFlag < 0 || pp->collisionalFlag > 4 || pp->amFlag < 0 ||
-      pp->amFlag > 2 || pp->neighborWidth < 0.0)
+      pp->amFlag > 2 || pp->neighborWidth < 0.0 || pp->qsFluct < 0 || pp->qsFluct > 2 || pp->qsFluctAdapt < 0 ||
+      pp->qsFluctAdapt > 1 || pp->fluctHalfWidth < 0.0)
     return -1;
   if (pp->collisionalFlag && !(pp->ksp > 0.0 && pp->erest > 0.0 && pp->erest <= 1.0)) return -1;
   const int perFlag[3] = {pp->periodicX, pp->periodicY, pp->periodicZ};
@@ -1655,7 +1824,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   p->lost = calloc(n, 1);
   p->hist = calloc(6 * VU_N * n, sizeof(double)); /* ppiclf_solve.f:432-434 */
   p->nTimeBH = 1;                                  /* PICL_TEMP_InitSolver.F90:290 */
-  if (pp->amFlag == 2) { /* PICL_TEMP_InitSolver.F90:655-661: max(4 d_max, NEIGHBORWIDTH) */
+  if (pp->amFlag == 2 || pp->qsFluct >= 1) { /* PICL_TEMP_InitSolver.F90:655-661: max(4 d_max, NEIGHBORWIDTH) */
     double dmax = 0.0;
     for (int i = 0; i < nPart; i++) dmax = fmax(dmax, pconst[4 * (size_t)i + 1]);
     p->nbW = pp->neighborWidth > 0.0 ? pp->neighborWidth : 4.0 * dmax;
@@ -1669,6 +1838,25 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   }
   setup_mesh(r, p);
   if (pp->collisionalFlag >= 3) p->fldRot = calloc(6 * (size_t)p->ne, sizeof(double));
+  if (pp->qsFluct >= 1) {
+    /* PICL_TEMP_InitSolver.F90:598-646: d2chk(2) = 4 MAXVAL(Max_EleLen) / 2,
+     * where Max_EleLen is reset for every element, so the last element's
+     * extents decide (those outside [1e-7, 1e-2] count as 1.23456789e-10) */
+    double mx = 0.0;
+    const double *X = p->xm1b + 24 * (size_t)(p->ne - 1);
+    for (int d = 0; d < 3; d++) {
+      double lo = 1.0e10, hi = -1.0e10;
+      for (int n = 0; n < 8; n++) {
+        if (X[3 * n + d] > hi) hi = X[3 * n + d];
+        if (X[3 * n + d] < lo) lo = X[3 * n + d];
+      }
+      double len = fabs(hi - lo), mel = 1.23456789e-10;
+      if (!(len > 1e-2 || len < 1e-7) && len > mel) mel = len;
+      if (d == 0 || mel > mx) mx = mel;
+    }
+    double filter = 4.0 * mx;
+    p->fluctW = pp->fluctHalfWidth > 0.0 ? pp->fluctHalfWidth : filter / 2.0;
+  }
   r->picl = p;
   int N = r->nCells;
   r->phi = calloc((size_t)N, sizeof(double));
@@ -1724,3 +1912,6 @@ void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem) {
 int orc_picl_neighbors(const orc_region *r, const double *xp, int *ids, double *d2) {
   return nearest27(r->picl, xp, ids, d2);
 }
+/* Test hook: particle i's fluctuation box sums (u, v, w, u^2, v^2, w^2,
+ * count) over the current positions and velocities. */
+void orc_picl_fluct_means(const orc_region *r, int i, double out[7]) { fluct_means(r->picl, i, out); }

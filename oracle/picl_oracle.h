@@ -46,6 +46,11 @@ typedef struct orc_picl_params {
   double erest;        /* ERESTITUTION: restitution coefficient of the damping */
   double neighborWidth; /* amFlag 2: ppiclf_d2chk(3), the pair terms' reach; 0 means 4 x the
                            largest diameter (PICL_TEMP_InitSolver.F90:655-661 with NEIGHBORWIDTH 0) */
+  int qsFluct;          /* QSFLUCT: 0 off, 1 Lattanzi, 2 Osnes quasi-steady force fluctuations */
+  int qsFluctAdapt;     /* QSFLUCTFILTADAPT: 1 widens the box to the particle's adaptive filter */
+  double fluctHalfWidth; /* ppiclf_d2chk(2), the box filter's half-width; 0 means rocpicl's
+                            2 x the last element's largest extent */
+  long long fluctSeed;  /* seed of the uniform draws that stand in for RANDOM_NUMBER */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index
@@ -98,6 +103,7 @@ void orc_picl_collision_force2(const double *yi12, const double *yj12, double dp
 void orc_picl_locate(const orc_region *r, int n, const double *xyz, int *elem);
 /* Elements LocalInterp sums for point xp, nearest first; returns how many. */
 int orc_picl_neighbors(const orc_region *r, const double *xp, int *ids, double *d2);
+void orc_picl_fluct_means(const orc_region *r, int i, double out[7]);
 
 /* Briney's added-mass model (libpicl/user_files/Added_Mass/): the four
  * fitted curves, the distant-neighbour corrections (analytical, and the
@@ -120,6 +126,14 @@ int orc_picl_torque_hydro(const double *vort, const double *spin, double rhof, d
                           double rhop, double *t);
 void orc_picl_lift(const double *vort, const double *spin, const double *vrel, double dp, double rep, double rmu,
                    double rhof, double *lift);
+
+/* QS fluctuations: the uniform draw k of a particle's call, and one SDE step
+ * (flag 1 Lattanzi, 2 Osnes) from the box means m (sums of u, v, w, u^2, v^2,
+ * w^2 and the count), the previous force F and the uniforms U. */
+double orc_fluct_uniform(long long seed, int call, int pid, int k);
+void orc_fluct_force(int flag, const double m[7], const double F[3], const double U[6], double rphip,
+                     double rmachp, double rep, double vmag, const double vrel[3], double rmu, double dp,
+                     double fac, double out[3]);
 
 #ifdef __cplusplus
 }

@@ -72,7 +72,8 @@ class PiclParams(C.Structure):
                 ("filterWidth", C.c_double), ("filterAlpha", C.c_double), ("heatTransfer", C.c_int),
                 ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
                 ("viscousUnsteady", C.c_int), ("collisionalFlag", C.c_int), ("ksp", C.c_double),
-                ("erest", C.c_double), ("neighborWidth", C.c_double)]
+                ("erest", C.c_double), ("neighborWidth", C.c_double), ("qsFluct", C.c_int),
+                ("qsFluctAdapt", C.c_int), ("fluctHalfWidth", C.c_double), ("fluctSeed", C.c_longlong)]
 
 
 _lib = None

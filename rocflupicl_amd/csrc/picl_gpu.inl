@@ -41,7 +41,10 @@ struct PiclPhys {
   int vu;                     // viscous-unsteady force
   int col;                    // normal collision force
   double ksp, erest;          // collision time / dt, restitution coefficient
-  double nbW;                 // ppiclf_d2chk(3): the added-mass pair terms' reach (amFlag 2)
+  double nbW;                 // ppiclf_d2chk(3): the neighbour width (Briney's pair terms, fluctuation means)
+  int qsFluct, qsFluctAdapt;  // QS fluctuations: 1 Lattanzi, 2 Osnes; adaptive box
+  double fluctW;              // ppiclf_d2chk(2): the fluctuation box's half-width
+  long long fluctSeed;        // seed of the uniform draws
 };
 
 // Rows of PBuf::st
@@ -61,6 +64,7 @@ struct PBuf {
   int *pid = nullptr;    // original (ppiclF) index
   int *elem = nullptr;   // element (device cell) of the current stage, -1 lost
   double *w = nullptr;   // [3][nP] W = D(rho_g u_g)/Dt - rho_g dv/dt (rprop WDOTX..Z; amFlag 2)
+  double *fq = nullptr;  // [3][nP] the fluctuating quasi-steady force (rprop FLUCTFX..Z; qsFluct)
 };
 
 struct PiclDev {
@@ -133,6 +137,11 @@ struct PiclDev {
   double *fbg = nullptr;     // [nC][9] after WENO-XYZ and the x-force positivity limiter
   double *fldRot = nullptr;  // [nC][6] vorticity, D(omega)/Dt per element
   int nVint = PICL_NFLD;     // values per particle in vint (20, 26 with the rotational fields)
+  // QS fluctuations: the box means by current slot, the SetYdot call counter
+  int fl = 0;
+  int pairs = 0;             // the two-phase stage with k_pairs (Briney or fluctuations)
+  double *fmean = nullptr;   // [7][nP] sums of u, v, w, u^2, v^2, w^2 and the count
+  int *nCall = nullptr;      // [1]
 };
 
 // ---------------------------------------------------------------------------
@@ -1047,14 +1056,119 @@ __global__ void __launch_bounds__(BLK) k_col_sort(PiclDev P) {
   }
 }
 
-// Steps 1a, 1b and 4 of ppiclf_user_SetYdot for am_flag 2 (ppiclf_user.f:
-// 393-468, 527-552): the unary term, the pair terms of every particle within
-// the neighbour width (ppiclf_solve_NearestNeighbor's squared-distance filter,
-// then EvalNearestNeighbor's rdiff <= d2chk(3)) and the binary term, into
-// famB = (famx, famy, famz, rmass_add).  The pair terms are added in the
-// cubes' order (z, y, x, each cube's members by slot), not ppiclF's
-// particle-index order: the sums agree with the oracle's to rounding.
-__global__ void __launch_bounds__(BLK) k_am_pairs(Dev d, PiclDev P, PiclPhys Ph) {
+// ---------------------------------------------------------------------------
+// Quasi-steady force fluctuations (QSFLUCT 1 Lattanzi, 2 Osnes):
+// ppiclf_user_Fluctuations.f:32-418.  Same operations as the oracle.
+// ---------------------------------------------------------------------------
+// the uniform draws that stand in for RANDOM_NUMBER: a splitmix64 hash of
+// (seed, call, original index, draw), uniform on (0, 1) with 53 bits
+__device__ __forceinline__ double fluct_uniform(long long seed, int call, int pid, int k) {
+  unsigned long long n = ((((unsigned long long)(unsigned)call << 32) | (unsigned)pid) << 3) + (unsigned long long)(k + 1);
+  unsigned long long z = (unsigned long long)seed + 0x9E3779B97F4A7C15ull * n;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return ((double)(z >> 11) + 0.5) * 0x1.0p-53;
+}
+
+__device__ __forceinline__ void fl_acc(double &s, double &c, double x) {
+  double t = s + x, z = t - s;
+  c = c + ((s - (t - z)) + (x - z));
+  s = t;
+}
+
+// one SDE step from the box means m (sums and count), the previous force F,
+// the uniforms U and fac = rk3ark(stage) dt
+__device__ void fluct_force(int flag, const double m[7], const double F[3], const double U[6], double rphip,
+                            double rmachp, double rep, double vmag, const double vrel[3], double rmu, double dp,
+                            double fac, double out[3]) {
+  const double rpi = 3.141592653589793116, TwoPi = 6.283185307179586232;
+  double denum = fmax(m[6], 1.0);
+  double upmean = m[0] / denum, vpmean = m[1] / denum, wpmean = m[2] / denum;
+  double u2pmean = m[3] / denum, v2pmean = m[4] / denum, w2pmean = m[5] / denum;
+  double fq = F32(6.52) * rphip - F32(22.56) * ip2(rphip) + F32(49.90) * ip3(rphip);
+  double Fs = 3.0 * rpi * rmu * dp * (1.0 + F32(0.15) * pow(rep * (1.0 - rphip), F32(0.687))) * (1.0 - rphip) * vmag;
+  double bq = fmin(sqrt(20.0 * rmachp), 1.0) * F32(0.55) * pow(rphip, F32(0.7)) *
+              (1.0 + tanh((rmachp - 0.5) / F32(0.2)));
+  double theta = ((u2pmean + v2pmean + w2pmean) - (ip2(upmean) + ip2(vpmean) + ip2(wpmean))) / 3.0;
+  if (theta <= 1.0e-12) theta = 0.0;
+  double chi = (1.0 + 2.50 * rphip + F32(4.51) * ip2(rphip) + F32(4.52) * ip3(rphip)) /
+               pow(1.0 - ip3(rphip / F32(0.64)), F32(0.68));
+  double tF_inv = (24.0 * rphip * chi / dp) * sqrt(theta / rpi);
+  double aSDE = tF_inv;
+  if (flag == 1) {
+    double sigF = (fq + bq) * Fs;
+    double bSDE = sigF * sqrt(2.0 * tF_inv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double Z = sqrt(-2.0 * log(U[2 * k])) * cos(TwoPi * U[2 * k + 1]);
+      double dW = sqrt(fac) * Z;
+      out[k] = (1.0 - aSDE * fac) * F[k] + bSDE * dW;
+    }
+    return;
+  }
+  double sigD = (fq + bq) * Fs;
+  double bSDE_CD = sigD * sqrt(2.0 * tF_inv);
+  // below 1e-8 m/s the axis is x and the transverse amplitude vanishes; bSDE_CD
+  // keeps the unguarded sigD (ppiclf_user_Fluctuations.f:317-328)
+  double avec[3] = {1.0, 0.0, 0.0}, CD_frac = 0.0;
+  if (vmag > 1.0e-8) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) avec[k] = vrel[k] / vmag;
+    double CD_prime = F[0] * avec[0] + F[1] * avec[1] + F[2] * avec[2];
+    CD_frac = CD_prime / sigD;
+  } else {
+    sigD = 0.0;
+  }
+  double sigmoid_cf = 1.0 / (1.0 + exp(-CD_frac));
+  double f_CF = F32(0.39356905) * sigmoid_cf + F32(0.43758848);
+  double sigT = f_CF * sigD;
+  double bSDE_CL = sigT * sqrt(2.0 * tF_inv);
+  double eunit[3] = {1.0, 0.0, 0.0};
+  if (fabs(avec[1]) + fabs(avec[2]) <= 1.0e-8) {
+    eunit[0] = 0.0; eunit[1] = 1.0;
+  } else if (fabs(avec[0]) + fabs(avec[2]) <= 1.0e-8) {
+    eunit[0] = 0.0; eunit[2] = 1.0;
+  }
+  double bvec[3] = {avec[1] * eunit[2] - avec[2] * eunit[1], avec[2] * eunit[0] - avec[0] * eunit[2],
+                    avec[0] * eunit[1] - avec[1] * eunit[0]};
+  double dn = fmax(1.0e-8, sqrt(ip2(bvec[0]) + ip2(bvec[1]) + ip2(bvec[2])));
+#pragma unroll
+  for (int k = 0; k < 3; k++) bvec[k] = bvec[k] / dn;
+  double cvec[3] = {avec[1] * bvec[2] - avec[2] * bvec[1], avec[2] * bvec[0] - avec[0] * bvec[2],
+                    avec[0] * bvec[1] - avec[1] * bvec[0]};
+  dn = fmax(1.0e-8, sqrt(ip2(cvec[0]) + ip2(cvec[1]) + ip2(cvec[2])));
+#pragma unroll
+  for (int k = 0; k < 3; k++) cvec[k] = cvec[k] / dn;
+  double Z1 = sqrt(-2.0 * log(U[0])) * cos(TwoPi * U[1]);
+  double Z2 = sqrt(-2.0 * log(U[2])) * cos(TwoPi * U[3]);
+  double dW1 = sqrt(fac) * Z1, dW2 = sqrt(fac) * Z2;
+  double cosrand = cos(TwoPi * U[4]), sinrand = sin(TwoPi * U[4]);
+  double dvec[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) dvec[k] = bvec[k] * cosrand + cvec[k] * sinrand;
+  dn = fmax(1.0e-8, sqrt(ip2(dvec[0]) + ip2(dvec[1]) + ip2(dvec[2])));
+#pragma unroll
+  for (int k = 0; k < 3; k++) dvec[k] = dvec[k] / dn;
+#pragma unroll
+  for (int k = 0; k < 3; k++) out[k] = (1.0 - aSDE * fac) * F[k] + bSDE_CD * dW1 * avec[k] + bSDE_CL * dW2 * dvec[k];
+}
+
+// Step 1b of ppiclf_user_SetYdot (ppiclf_user.f:393-468) over the
+// neighbours within the neighbour width d2chk(3) (ppiclf_solve_NearestNeighbor's
+// squared-distance filter), periodic images included, then:
+// * Briney's added mass (am_flag 2): the unary term (Step 1a), the pair
+//   terms of EvalNearestNeighbor (rdiff <= d2chk(3)) and the binary term
+//   (Step 4, ppiclf_user.f:527-552), into famB = (famx, famy, famz,
+//   rmass_add);
+// * the fluctuation means (qs_fluct_flag >= 1; EvalNearestNeighbor.f:
+//   318-360): the particle's own velocity, then every neighbour within the
+//   box of half-width d2chk(2) (or the adaptive one), TwoSum-accumulated so
+//   the result does not depend on the order, into fmean.
+// The pair terms are added in the cubes' order (z, y, x, each cube's members
+// by slot), not ppiclF's particle-index order: they agree with the oracle's
+// to rounding.
+__global__ void __launch_bounds__(BLK) k_pairs(Dev d, PiclDev P, PiclPhys Ph) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
   const size_t nP = P.nP;
@@ -1069,29 +1183,46 @@ __global__ void __launch_bounds__(BLK) k_am_pairs(Dev d, PiclDev P, PiclPhys Ph)
   double vxr = v[1] - yv[0], vyr = v[2] - yv[1], vzr = v[3] - yv[2];
   double vmag = sqrt(vxr * vxr + vyr * vyr + vzr * vzr);
   if (vmag <= 1.0e-3) return;  // skipped by SetYdot
+  const bool am = P.am2, fl = P.fl;
   double rhof = v[0], rmachp = vmag / v[7];
   double rphip = fmin(v[9], 0.62), rphif = 1.0 - rphip;
   double dpi = C.pc[nP + t], volp = C.pc[2 * nP + t];
-  // Step 1a: ppiclf_user_AM_Briney_Unary (Mach correction only, no volume yet)
-  double rcd_am;
-  if (rmachp > F32(0.6)) {
-    rcd_am = 0x1.5104d6p+1;
-  } else {
-    double m2 = rmachp * rmachp;
-    rcd_am = 1.0 + F32(1.8) * m2 + F32(7.6) * (m2 * m2);
-  }
-  rcd_am = rcd_am * 0.5;
-  double rmass_add = rhof * volp * rcd_am;
-  double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
-  SDrho = SDrho / rphif;
-  double vgradrho = vxr * v[17] + vyr * v[18] + vzr * v[19];
-  double Fam[3] = {rcd_am * (vxr * SDrho + rhof * v[10] + v[1] * vgradrho),
-                   rcd_am * (vyr * SDrho + rhof * v[11] + v[2] * vgradrho),
-                   rcd_am * (vzr * SDrho + rhof * v[12] + v[3] * vgradrho)};
-  double Wi[3] = {C.w[t], C.w[nP + t], C.w[2 * nP + t]};
-  double Wsum[3] = {0.0, 0.0, 0.0};
+  double Fam[3] = {0.0, 0.0, 0.0}, Wi[3] = {0.0, 0.0, 0.0}, Wsum[3] = {0.0, 0.0, 0.0}, rmass_add = 0.0;
   int nn = 0;
-  // Step 1b: the neighbours within d2chk(3), periodic images included
+  if (am) {  // Step 1a: ppiclf_user_AM_Briney_Unary (Mach correction only, no volume yet)
+    double rcd_am;
+    if (rmachp > F32(0.6)) {
+      rcd_am = 0x1.5104d6p+1;
+    } else {
+      double m2 = rmachp * rmachp;
+      rcd_am = 1.0 + F32(1.8) * m2 + F32(7.6) * (m2 * m2);
+    }
+    rcd_am = rcd_am * 0.5;
+    rmass_add = rhof * volp * rcd_am;
+    double SDrho = v[13] + yv[0] * v[14] + yv[1] * v[15] + yv[2] * v[16];
+    SDrho = SDrho / rphif;
+    double vgradrho = vxr * v[17] + vyr * v[18] + vzr * v[19];
+    Fam[0] = rcd_am * (vxr * SDrho + rhof * v[10] + v[1] * vgradrho);
+    Fam[1] = rcd_am * (vyr * SDrho + rhof * v[11] + v[2] * vgradrho);
+    Fam[2] = rcd_am * (vzr * SDrho + rhof * v[12] + v[3] * vgradrho);
+#pragma unroll
+    for (int c = 0; c < 3; c++) Wi[c] = C.w[c * nP + t];
+  }
+  double S[6], Cc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int cnt = 1;
+  double box = Ph.fluctW;
+  if (fl) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      S[k] = yv[k];
+      S[3 + k] = yv[k] * yv[k];
+    }
+    if (Ph.qsFluctAdapt >= 1) {  // EvalNearestNeighbor.f:325-335
+      double adptfilter = pow(10.0 * ip3(dpi) / fmax(F32(1.e-4), v[9]), 0x1.555556p-2);
+      adptfilter = adptfilter / 2.0;
+      box = fmax(Ph.fluctW, adptfilter);
+    }
+  }
   const double w = Ph.nbW, dist2 = w * w, Rq = w * (1.0 + 1.0e-9);
   const double alpha_local = fmin(F32(0.4), rphip), rad = 0.5 * dpi;
   int nsd[3], sdv[3][3], clo[3][3], chi[3][3];
@@ -1128,50 +1259,68 @@ __global__ void __launch_bounds__(BLK) k_am_pairs(Dev d, PiclDev P, PiclPhys Ph)
           double tot = ((xi[0] - xj[0]) * (xi[0] - xj[0]) + (xi[1] - xj[1]) * (xi[1] - xj[1])) +
                        (xi[2] - xj[2]) * (xi[2] - xj[2]);
           if (tot > dist2) continue;
-          double rx = xj[0] - xi[0], ry = xj[1] - xi[1], rz = xj[2] - xi[2];
-          double rdiff = sqrt((rx * rx + ry * ry) + rz * rz);
-          if (!(rdiff <= w)) continue;
-          double rthresh = 0.5 * (dpi + C.pc[nP + sj]);
-          if (rdiff < rthresh) rx = ry = rz = rthresh;  // overlapping: the reference's substitute
-          double R[3][6];
-          am_resistance_pair(rx, ry, rz, alpha_local, rad, R);
-          double Wj[3] = {C.w[sj], C.w[nP + sj], C.w[2 * nP + sj]};
-          nn++;
+          if (am) {
+            double rx = xj[0] - xi[0], ry = xj[1] - xi[1], rz = xj[2] - xi[2];
+            double rdiff = sqrt((rx * rx + ry * ry) + rz * rz);
+            if (rdiff <= w) {
+              double rthresh = 0.5 * (dpi + C.pc[nP + sj]);
+              if (rdiff < rthresh) rx = ry = rz = rthresh;  // overlapping: the reference's substitute
+              double R[3][6];
+              am_resistance_pair(rx, ry, rz, alpha_local, rad, R);
+              double Wj[3] = {C.w[sj], C.w[nP + sj], C.w[2 * nP + sj]};
+              nn++;
 #pragma unroll
-          for (int k = 0; k < 3; k++) {
+              for (int k = 0; k < 3; k++) {
 #pragma unroll
-            for (int l = 0; l < 3; l++) {
-              Fam[k] = Fam[k] + R[k][l] * Wi[l];
-              Fam[k] = Fam[k] + R[k][l + 3] * Wj[l];
+                for (int l = 0; l < 3; l++) {
+                  Fam[k] = Fam[k] + R[k][l] * Wi[l];
+                  Fam[k] = Fam[k] + R[k][l + 3] * Wj[l];
+                }
+                Wsum[k] = Wsum[k] + Wj[k];
+              }
             }
-            Wsum[k] = Wsum[k] + Wj[k];
+          }
+          if (fl && fabs(xi[0] - xj[0]) <= box && fabs(xi[1] - xj[1]) <= box && fabs(xi[2] - xj[2]) <= box) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+              double u = st[(3 + k) * nP + sj];
+              fl_acc(S[k], Cc[k], u);
+              fl_acc(S[3 + k], Cc[3 + k], u * u);
+            }
+            cnt++;
           }
         }
       }
   }
-  // Step 4: ppiclf_user_AM_Briney_Binary, or the unary term times the volume
-  if (nn > 0) {
-    double IA, II;
-    if (w / rad >= F32(3.49)) {
-      IA = am_ia_analytical(w, rad, alpha_local);
-      II = am_ii_analytical(w, rad, alpha_local);
+  if (am) {  // Step 4: ppiclf_user_AM_Briney_Binary, or the unary term times the volume
+    if (nn > 0) {
+      double IA, II;
+      if (w / rad >= F32(3.49)) {
+        IA = am_ia_analytical(w, rad, alpha_local);
+        II = am_ii_analytical(w, rad, alpha_local);
+      } else {
+        IA = am_numerical(false, w, rad, alpha_local);
+        II = am_numerical(true, w, rad, alpha_local);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Fam[c] = Fam[c] + IA * Wi[c];
+        Fam[c] = Fam[c] + II * Wsum[c] / (double)nn;
+        Fam[c] = volp * Fam[c];
+      }
     } else {
-      IA = am_numerical(false, w, rad, alpha_local);
-      II = am_numerical(true, w, rad, alpha_local);
+#pragma unroll
+      for (int c = 0; c < 3; c++) Fam[c] = Fam[c] * volp;
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      Fam[c] = Fam[c] + IA * Wi[c];
-      Fam[c] = Fam[c] + II * Wsum[c] / (double)nn;
-      Fam[c] = volp * Fam[c];
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; c++) Fam[c] = Fam[c] * volp;
+    for (int c = 0; c < 3; c++) P.famB[c * nP + t] = Fam[c];
+    P.famB[3 * nP + t] = rmass_add;
   }
+  if (fl) {
 #pragma unroll
-  for (int c = 0; c < 3; c++) P.famB[c * nP + t] = Fam[c];
-  P.famB[3 * nP + t] = rmass_add;
+    for (int k = 0; k < 6; k++) P.fmean[k * nP + t] = S[k] + Cc[k];
+    P.fmean[6 * nP + t] = (double)cnt;
+  }
 }
 
 // ppiclf_user.f:934-990 after every particle's forces (am_flag 2): each
@@ -1409,11 +1558,11 @@ __global__ void k_vu_time(Dev d, PiclDev P) {
   T[VU_N] = (double)n;
 }
 
-// phase 0: the whole stage.  With Briney's added mass the stage runs in two
-// phases around k_am_pairs (whose pair terms need every particle's
-// interpolated volume fraction): phase 1 locates and interpolates into vint
-// and records what the W store reads, phase 2 takes vint and the pair
-// kernel's added-mass force and does the rest.
+// phase 0: the whole stage.  With Briney's added mass or the QS
+// fluctuations the stage runs in two phases around k_pairs (whose sums need
+// every particle's interpolated values): phase 1 locates and interpolates
+// into vint and records what the W store reads, phase 2 takes vint and the
+// pair kernel's sums and does the rest.
 template <int TRI, int QS>
 __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage, int phase) {
   int t = blockIdx.x * BLK + threadIdx.x;
@@ -1437,7 +1586,7 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
     for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = 0.0;
-    if (phase == 1) P.actP[C.pid[t]] = 0;
+    if (phase == 1 && P.am2) P.actP[C.pid[t]] = 0;
     return;
   }
   atomicAdd(&P.cnt[e], 1);
@@ -1578,16 +1727,18 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
     // what ppiclf_user.f:934-990 reads: SDrho's numerator with the
     // start-of-stage velocity, the relative velocity and the rphif that
     // SetYdot's particle loop leaves (ppiclf_user.f:316-361), by index
-    const int pid = C.pid[t];
-    double y0 = st[3 * nP + t], y1 = st[4 * nP + t], y2 = st[5 * nP + t];
-    P.snum[t] = v[13] + y0 * v[14] + y1 * v[15] + y2 * v[16];
-    double vx = v[1] - y0, vy = v[2] - y1, vz = v[3] - y2;
-    P.vrelP[pid] = vx;
-    P.vrelP[nP + pid] = vy;
-    P.vrelP[2 * nP + pid] = vz;
-    P.rphifP[pid] = sqrt(vx * vx + vy * vy + vz * vz) <= 1.0e-3 ? 1.0 - v[9] : 1.0 - fmin(v[9], 0.62);
-    P.actP[pid] = 1;
-    atomicMax(P.lastP, pid + 1);
+    if (P.am2) {
+      const int pid = C.pid[t];
+      double y0 = st[3 * nP + t], y1 = st[4 * nP + t], y2 = st[5 * nP + t];
+      P.snum[t] = v[13] + y0 * v[14] + y1 * v[15] + y2 * v[16];
+      double vx = v[1] - y0, vy = v[2] - y1, vz = v[3] - y2;
+      P.vrelP[pid] = vx;
+      P.vrelP[nP + pid] = vy;
+      P.vrelP[2 * nP + pid] = vz;
+      P.rphifP[pid] = sqrt(vx * vx + vy * vy + vz * vz) <= 1.0e-3 ? 1.0 - v[9] : 1.0 - fmin(v[9], 0.62);
+      P.actP[pid] = 1;
+      atomicMax(P.lastP, pid + 1);
+    }
     return;
   }
   }
@@ -1628,9 +1779,28 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       default: beta = qs_sn(rep, rmu, dpp); break;
     }
     double fqsx = beta * vxr, fqsy = beta * vyr, fqsz = beta * vzr;
-    fqsx = fqsx + 0.0;
-    fqsy = fqsy + 0.0;
-    fqsz = fqsz + 0.0;
+    if (Ph.qsFluct >= 1) {  // Step 3 (ppiclf_user.f:487-510): one SDE step of the fluctuating force
+      const double rk3ark = istage == 1 ? 8.0 / 15.0 : (istage == 2 ? 5.0 / 12.0 : 0.75);
+      double fac = rk3ark * d.scal[S_DTMIN];
+      double m[7], F[3], U[6], fl[3], vrel[3] = {vxr, vyr, vzr};
+#pragma unroll
+      for (int k = 0; k < 7; k++) m[k] = P.fmean[k * nP + t];
+#pragma unroll
+      for (int k = 0; k < 3; k++) F[k] = C.fq[k * nP + t];
+      const int call = P.nCall[0], pid = C.pid[t];
+#pragma unroll
+      for (int k = 0; k < 6; k++) U[k] = fluct_uniform(Ph.fluctSeed, call, pid, k);
+      fluct_force(Ph.qsFluct, m, F, U, rphip, rmachp, rep, vmag, vrel, rmu, dpp, fac, fl);
+      fqsx = fqsx + fl[0];
+      fqsy = fqsy + fl[1];
+      fqsz = fqsz + fl[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) C.fq[k * nP + t] = fl[k];
+    } else {
+      fqsx = fqsx + 0.0;
+      fqsy = fqsy + 0.0;
+      fqsz = fqsz + 0.0;
+    }
     if (Ph.amFlag == 1) {
       double rcd_am;
       if (rmachp > F32(0.6)) {
@@ -1648,7 +1818,7 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       famx = rcd_am * volp * (vxr * SDrho + rhof * v[10] + v[1] * vgradrho);
       famy = rcd_am * volp * (vyr * SDrho + rhof * v[11] + v[2] * vgradrho);
       famz = rcd_am * volp * (vzr * SDrho + rhof * v[12] + v[3] * vgradrho);
-    } else if (Ph.amFlag == 2) {  // Briney: unary, pair and binary terms from k_am_pairs
+    } else if (Ph.amFlag == 2) {  // Briney: unary, pair and binary terms from k_pairs
       famx = P.famB[t];
       famy = P.famB[nP + t];
       famz = P.famB[2 * nP + t];
@@ -2322,8 +2492,16 @@ __global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
   if (P.am2)
 #pragma unroll
     for (int c = 0; c < 3; c++) X.w[c * nP + pos] = C.w[c * nP + t];
+  if (P.fl)
+#pragma unroll
+    for (int c = 0; c < 3; c++) X.fq[c * nP + pos] = C.fq[c * nP + t];
   X.pid[pos] = C.pid[t];
   X.elem[pos] = C.elem[t];
+}
+
+// the SetYdot call counter that keys the fluctuations' uniform draws
+__global__ void k_next_call(PiclDev P) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) P.nCall[0] = P.nCall[0] + 1;
 }
 
 __global__ void k_flip(PiclDev P) {
@@ -2428,7 +2606,7 @@ void picl_free(PiclState *s) {
                   P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
                   P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq, P.gBox, P.gvList,
                   P.buf[0].w, P.buf[1].w, P.vint, P.famB, P.snum, P.vrelP, P.rphifP, P.actP, P.lastP,
-                  P.fb, P.fbRaw, P.fbg, P.fldRot};
+                  P.fb, P.fbRaw, P.fbg, P.fldRot, P.buf[0].fq, P.buf[1].fq, P.fmean, P.nCall};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -2497,22 +2675,22 @@ void picl_particle_stage(rflu_handle *h, int istage) {
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(SCAN_T), 0, h->stream, P.colBsum, nb);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(SCAN_T), 0, h->stream, P.colCnt, n, P.colBsum, P.colOff);
     hipLaunchKernelGGL(k_col_fill, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
-    if (P.am2) hipLaunchKernelGGL(k_col_sort, dim3(nblk(P.colNb)), dim3(BLK), 0, h->stream, P);
+    if (P.pairs) hipLaunchKernelGGL(k_col_sort, dim3(nblk(P.colNb)), dim3(BLK), 0, h->stream, P);
     if (P.col) hipLaunchKernelGGL(k_contacts, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
   }
-  if (P.am2) {  // the particle kernel's first phase: locate and interpolate
+  if (P.pairs) {  // the particle kernel's first phase: locate and interpolate
     ProfScope ps(h, "k_interp");
     using KP = void (*)(Dev, PiclDev, PiclPhys, int, int);
     static const KP kp[2][5] = {
         {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
         {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lastP, 1);
+    if (P.am2) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lastP, 1);
     hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
                        P, s->phys, istage, 1);
   }
-  if (P.am2) {
-    ProfScope ps(h, "k_am_pairs");
-    hipLaunchKernelGGL(k_am_pairs, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
+  if (P.pairs) {
+    ProfScope ps(h, "k_pairs");
+    hipLaunchKernelGGL(k_pairs, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys);
   }
   {
     ProfScope ps(h, "k_particles");
@@ -2521,8 +2699,9 @@ void picl_particle_stage(rflu_handle *h, int istage) {
         {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
         {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
     hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
-                       P, s->phys, istage, P.am2 ? 2 : 0);
+                       P, s->phys, istage, P.pairs ? 2 : 0);
     if (P.am2) hipLaunchKernelGGL(k_wdot, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
+    if (P.fl) hipLaunchKernelGGL(k_next_call, dim3(1), dim3(64), 0, h->stream, P);
   }
   if (s->phys.vu && istage == 3) hipLaunchKernelGGL(k_vu_time, dim3(1), dim3(64), 0, h->stream, h->d, P);
   picl_sort_and_deposit(h, 1);
@@ -2631,6 +2810,10 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   if (pp->amFlag < 0 || pp->amFlag > 2)
     return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 (off), 1 (Parmar) or 2 (Briney)");
   if (!(pp->neighborWidth >= 0.0)) return set_err(h, RFLU_GPU_EINVAL, "neighborWidth must be >= 0");
+  if (pp->qsFluct < 0 || pp->qsFluct > 2)
+    return set_err(h, RFLU_GPU_EINVAL, "qsFluct must be 0 (off), 1 (Lattanzi) or 2 (Osnes)");
+  if (pp->qsFluctAdapt < 0 || pp->qsFluctAdapt > 1) return set_err(h, RFLU_GPU_EINVAL, "qsFluctAdapt must be 0 or 1");
+  if (!(pp->fluctHalfWidth >= 0.0)) return set_err(h, RFLU_GPU_EINVAL, "fluctHalfWidth must be >= 0");
   if (pp->viscousUnsteady < 0 || pp->viscousUnsteady > 1)
     return set_err(h, RFLU_GPU_EINVAL, "viscousUnsteady must be 0 or 1");
   if (pp->collisionalFlag < 0 || pp->collisionalFlag > 4)
@@ -2676,15 +2859,20 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   s->phys.erest = pp->erest;
   s->P.am2 = pp->amFlag == 2 ? 1 : 0;
   s->P.rot3 = pp->collisionalFlag >= 3 ? 1 : 0;
+  s->P.fl = pp->qsFluct >= 1 ? 1 : 0;
+  s->P.pairs = s->P.am2 || s->P.fl ? 1 : 0;
+  s->phys.qsFluct = pp->qsFluct;
+  s->phys.qsFluctAdapt = pp->qsFluctAdapt;
+  s->phys.fluctSeed = pp->fluctSeed;
   s->P.nVint = PICL_NFLD + (s->P.rot3 ? 6 : 0);
-  s->P.grid = s->P.col || s->P.am2 ? 1 : 0;
+  s->P.grid = s->P.col || s->P.am2 || s->P.fl ? 1 : 0;
   s->P.nRow = s->P.vu ? NROW_VU : (s->P.rot ? ROW_VU : NROW);
   s->y0.assign(y, y + 12 * (size_t)nP);
   // element geometry in device numbering
   std::vector<double> X(24 * (size_t)N), bbox(8 * (size_t)N, 0.0), cen(4 * (size_t)N, 0.0), rexp(N);
   std::vector<double> blo(3 * (size_t)N), bhi(3 * (size_t)N);
   std::vector<int> refId(N), inv(N);
-  double lastExt2[3] = {0, 0, 0}, maxExt[3] = {0, 0, 0}, glo[3] = {1e300, 1e300, 1e300}, ghi[3] = {-1e300, -1e300, -1e300};
+  double lastLen[3] = {0, 0, 0}, lastExt2[3] = {0, 0, 0}, maxExt[3] = {0, 0, 0}, glo[3] = {1e300, 1e300, 1e300}, ghi[3] = {-1e300, -1e300, -1e300};
   for (int i = 0; i < N; i++) {
     int o = h->perm[i];
     refId[i] = o;
@@ -2708,6 +2896,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
       maxExt[l] = std::max(maxExt[l], mx - mn);
       glo[l] = std::min(glo[l], mn);
       ghi[l] = std::max(ghi[l], mx);
+      if (o == N - 1) lastLen[l] = std::fabs(mx - mn);
       if (o == N - 1) {  // LocalInterp keeps the last element's extent (ppiclf_solve.f:2263-2289)
         double d2i = (mx - mn) * (mx - mn), d2m = 0.0;
         if (d2i > d2m) d2m = d2i;
@@ -2717,6 +2906,19 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rexp[i] = 1.0 / hex_volume(Xe);
   }
   for (int l = 0; l < 3; l++) P.d2Max[l] = lastExt2[l] * (1.5 * 1.5);
+  if (P.fl) {
+    // PICL_TEMP_InitSolver.F90:598-646: d2chk(2) = 4 MAXVAL(Max_EleLen) / 2,
+    // Max_EleLen reset for every element, so the last element's extents
+    // decide (those outside [1e-7, 1e-2] count as 1.23456789e-10)
+    double mxl = 0.0;
+    for (int l = 0; l < 3; l++) {
+      double mel = 1.23456789e-10;
+      if (!(lastLen[l] > 1e-2 || lastLen[l] < 1e-7) && lastLen[l] > mel) mel = lastLen[l];
+      if (l == 0 || mel > mxl) mxl = mel;
+    }
+    double filter = 4.0 * mxl;
+    s->phys.fluctW = pp->fluctHalfWidth > 0.0 ? pp->fluctHalfWidth : filter / 2.0;
+  }
   // location bins: one maximal element extent per bin, members in reference order
   std::vector<int> refOrder(N);
   for (int o = 0; o < N; o++) refOrder[o] = inv[o];
@@ -2779,7 +2981,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     // collisions reach (d + d_max) / 2: cubes of a third of d_max; the
     // added-mass pairs reach d2chk(3): cubes of half of it
     P.colH = dmax / 3.0 * (1.0 + 1.0e-8);
-    if (P.am2) {
+    if (P.pairs) {
       // PICL_TEMP_InitSolver.F90:655-661 passes max(4 d_max, NEIGHBORWIDTH)
       s->phys.nbW = pp->neighborWidth > 0.0 ? pp->neighborWidth : 4.0 * dmax;
       P.colH = std::max(P.colH, 0.5 * s->phys.nbW * (1.0 + 1.0e-8));
@@ -2807,17 +3009,24 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     if (P.col) rc |= dalloc(h, &P.fc, 3 * (size_t)nP);
     if (P.rot) rc |= dalloc(h, &P.tq, 3 * (size_t)nP);
   }
+  if (P.pairs) rc |= dalloc(h, &P.vint, (size_t)P.nVint * nP);
   if (P.am2) {  // W starts at 0 (the zeroed rprop, ppiclf_solve.f:432-434)
     std::vector<double> z3(3 * (size_t)nP, 0.0);
     std::vector<int> zi((size_t)nP, 0);
     for (int b = 0; b < 2; b++) rc |= dalloc(h, &P.buf[b].w, 3 * (size_t)nP, z3.data());
-    rc |= dalloc(h, &P.vint, (size_t)P.nVint * nP);
     rc |= dalloc(h, &P.famB, 4 * (size_t)nP);
     rc |= dalloc(h, &P.snum, (size_t)nP);
     rc |= dalloc(h, &P.vrelP, 3 * (size_t)nP, z3.data());
     rc |= dalloc(h, &P.rphifP, (size_t)nP, z3.data());
     rc |= dalloc(h, &P.actP, (size_t)nP, zi.data());
     rc |= dalloc(h, &P.lastP, 1, zi.data());
+  }
+  if (P.fl) {  // the fluctuating force starts at 0 (the zeroed rprop)
+    std::vector<double> z3(3 * (size_t)nP, 0.0);
+    int zero1 = 0;
+    for (int b = 0; b < 2; b++) rc |= dalloc(h, &P.buf[b].fq, 3 * (size_t)nP, z3.data());
+    rc |= dalloc(h, &P.fmean, 7 * (size_t)nP);
+    rc |= dalloc(h, &P.nCall, 1, &zero1);
   }
   if (P.vu) {  // ppiclf_nTimeBH = 1 (PICL_TEMP_InitSolver.F90:290), ages 0
     std::vector<double> t0(VU_N + 1, 0.0);
@@ -3170,7 +3379,7 @@ struct F77State {
   int npart = 0;
   std::vector<double> y, pc;
   int proj = 0;
-  double width = 0.0, alpha = 0.0, nbw = 0.0;
+  double width = 0.0, alpha = 0.0, nbw = 0.0, filt2 = 0.0;
   int stage = 0;  // ppiclf_solve_IntegrateRK3s_Rocflu's SAVE stage counter
   int icnt = 0;   // PPICLF_INT_ICNT
   std::vector<double> fld;  // [PICL_NFLD][nCells], the host's cell numbering
@@ -3276,7 +3485,7 @@ void ppiclf_solve_initparticle_(const int *imethod, const int *ndim, const int *
     g_f77.pc[4 * (size_t)p + 2] = r[3];
     g_f77.pc[4 * (size_t)p + 3] = r[21];
   }
-  (void)filt2;
+  g_f77.filt2 = *filt2;  // ppiclf_d2chk(2), the fluctuation box's half-width
   g_f77.nbw = *filt3;  // ppiclf_d2chk(3) until InitNeighborBin resets it (ppiclf_solve.f:224-226)
   g_f77.linit = true;
 }
@@ -3338,6 +3547,7 @@ void ppiclf_comm_initoverlapmesh_(const int *ncell, const int *lx1, const int *l
   p.periodicY = g_f77.per[1];
   p.periodicZ = g_f77.per[2];
   p.neighborWidth = g_f77.nbw;
+  p.fluctHalfWidth = g_f77.filt2;
   if (int rc = picl_gpu_init(h, &p, g_f77.npart, g_f77.y.data(), g_f77.pc.data()))
     return f77_fail("InitOverlap: " + h->err + " (code " + std::to_string(rc) + ")");
   g_f77.fld.assign((size_t)PICL_NFLD * N, 0.0);

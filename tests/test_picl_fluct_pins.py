@@ -1,0 +1,358 @@
+"""Pins of the oracle's quasi-steady force fluctuations (QSFLUCT 1 Lattanzi,
+2 Osnes; libpicl/user_files/ppiclf_user_Fluctuations.f:32-418 with the box
+means of ppiclf_user_EvalNearestNeighbor.f:318-360).
+
+ppiclF's user files cannot be compiled here (each includes PPICLF, whose
+DATA statements for its COMMON blocks flang rejects as re-initialisation),
+so the step is pinned by what the model is:
+* an Euler-Maruyama step of an Ornstein-Uhlenbeck process
+  dF = -F/tau dt + sigma sqrt(2/tau) dW: the old force decays by
+  (1 - fac/tau), the noise scales with sqrt(fac) and sqrt(2/tau), and the
+  stationary variance is sigma^2 / (1 - fac/(2 tau));
+* 1/tau is kinetic theory's collision frequency 24 phi g0 sqrt(theta/pi) / d
+  with the granular temperature theta of the neighbours' velocities (their
+  population variance) and Ma and Ahmadi's (1986) contact value g0, whose
+  published coefficients the reference rounds;
+* the normals are Box-Muller transforms of the uniforms;
+* Osnes' step puts the drag-direction noise along the relative velocity with
+  Lattanzi's amplitude, and the transverse noise in the plane across it, at
+  an angle 2 pi U5, scaled by a sigmoid of the drag fluctuation bounded by
+  0.4376 and 0.8311;
+* sigma vanishes in the dilute limit like 6.52 phi times Schiller and
+  Naumann's drag, grows with the Mach number and saturates;
+* the box means are those of a brute-force search (periodic images
+  included), and identical velocities give theta = 0 exactly (the means use
+  compensated sums).
+The amplitude polynomials' coefficients (fq, bq) and the adaptive box width
+are restated from the code: parity unpinned beyond the limits above.  The
+uniforms stand in for gfortran's RANDOM_NUMBER stream, which no restatement
+reproduces.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import oracle as orc
+from rocflupicl_amd import configs
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    orc.build()
+
+
+F32 = lambda v: float(np.float32(v))
+RMU, DP = 1.8e-5, 1.15e-4
+Z_ONE = (math.exp(-0.5), 0.0)  # sqrt(-2 log u1) cos(2 pi u2) = 1
+
+
+def means_of(vel):
+    """The box sums of a set of velocities (n, 3) as the oracle takes them."""
+    vel = np.asarray(vel, np.float64)
+    return np.concatenate([vel.sum(0), (vel ** 2).sum(0), [len(vel)]])
+
+
+def theta_of(vel):
+    return float(np.var(np.asarray(vel), axis=0).mean())
+
+
+def g0_ma_ahmadi(phi):
+    """Ma & Ahmadi (1986) radial distribution at contact, published form."""
+    return (1.0 + 2.5 * phi + 4.5904 * phi ** 2 + 4.515439 * phi ** 3) / (1.0 - (phi / 0.64356) ** 3) ** 0.67802
+
+
+def step(flag, vel, F, U, phi=0.2, mach=0.3, rep=50.0, vrel=(30.0, -4.0, 2.0), fac=2e-9):
+    vrel = np.asarray(vrel, np.float64)
+    return orc.fluct_force(flag, means_of(vel), F, U, phi, mach, rep, float(np.linalg.norm(vrel)), vrel, RMU, DP,
+                           fac)
+
+
+def rate(flag, vel, U, **kw):
+    """1/tau from the decay of two old forces under the same draws.  Their
+    difference lies across the relative velocity: Osnes' transverse noise
+    depends on the old force's drag component, which they then share."""
+    fac = kw.get("fac", 2e-9)
+    vrel = np.asarray(kw.get("vrel", (30.0, -4.0, 2.0)), np.float64)
+    if not np.linalg.norm(vrel) > 1e-8:
+        vrel = np.array([1.0, 0.0, 0.0])  # the guard's axis
+    F1 = np.array([3e-6, -1e-6, 2e-6])
+    dF = np.cross(vrel, [0.3, -0.5, 0.8])
+    dF *= 4e-6 / np.linalg.norm(dF)
+    d = step(flag, vel, F1, U, **kw) - step(flag, vel, F1 + dF, U, **kw)
+    lam = -(d @ dF) / (dF @ dF)
+    assert np.abs(d + lam * dF).max() <= 1e-9 * np.abs(dF).max()  # the difference only decays
+    return (1.0 - lam) / fac
+
+
+def test_uniforms_are_uniform_independent_and_keyed():
+    u = np.array([orc.fluct_uniform(7, c, p, k) for c in range(20) for p in range(500) for k in range(6)])
+    assert u.min() > 0.0 and u.max() < 1.0
+    # within 5 standard errors of the mean 1/2 and the variance 1/12
+    assert abs(u.mean() - 0.5) < 5.0 * math.sqrt(1.0 / 12.0 / u.size)
+    assert abs(u.var() - 1.0 / 12.0) < 5.0 * math.sqrt((1.0 / 80.0 - 1.0 / 144.0) / u.size)
+    assert len(np.unique(u)) == u.size
+    assert stats.kstest(u, "uniform").pvalue > 1e-4
+    assert orc.fluct_uniform(7, 3, 4, 5) == orc.fluct_uniform(7, 3, 4, 5)
+    base = orc.fluct_uniform(7, 3, 4, 5)
+    assert len({base, orc.fluct_uniform(8, 3, 4, 5), orc.fluct_uniform(7, 2, 4, 5), orc.fluct_uniform(7, 3, 5, 5),
+                orc.fluct_uniform(7, 3, 4, 4)}) == 5
+    # Box-Muller of consecutive pairs: standard normal, uncorrelated across pairs
+    uu = u.reshape(-1, 6)
+    z = np.sqrt(-2.0 * np.log(uu[:, 0::2])) * np.cos(2.0 * np.pi * uu[:, 1::2])
+    assert stats.kstest(z.ravel(), "norm").pvalue > 1e-4
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 4.0 / math.sqrt(len(z))
+
+
+@pytest.mark.parametrize("flag", [1, 2])
+@pytest.mark.parametrize("phi", [0.02, 0.2, 0.4])
+def test_decay_rate_is_the_kinetic_theory_collision_frequency(flag, phi):
+    rng = np.random.default_rng(int(phi * 100) + flag)
+    vel = rng.normal([40.0, 3.0, -2.0], [2.0, 1.5, 0.7], size=(37, 3))
+    U = [0.3, 0.7, 0.2, 0.9, 0.55, 0.1]
+    a = rate(flag, vel, U, phi=phi)
+    expect = 24.0 * phi * g0_ma_ahmadi(phi) * math.sqrt(theta_of(vel) / math.pi) / DP
+    assert abs(a / expect - 1.0) < 1e-2, (a, expect)
+    # theta is the population variance: 37 against 5 neighbours tells it from the n - 1 one
+    a5 = rate(flag, vel[:5], U, phi=phi)
+    assert abs(a / a5 - math.sqrt(theta_of(vel) / theta_of(vel[:5]))) < 1e-9
+
+
+@pytest.mark.parametrize("flag", [1, 2])
+def test_no_granular_temperature_leaves_the_force_unchanged(flag):
+    F = np.array([3e-6, -1e-6, 2e-6])
+    vel = np.tile([37.3, -2.1, 5.5], (40, 1))
+    out = step(flag, vel, F, [0.3, 0.7, 0.2, 0.9, 0.55, 0.1])
+    assert np.array_equal(out, F)
+
+
+def sigma_of(mach=0.3, phi=0.2, theta_scale=1.0, fac=2e-9, rep=50.0, vrel=(30.0, -4.0, 2.0)):
+    """sigma from the noise of one step from F = 0 with a unit normal draw."""
+    rng = np.random.default_rng(1)
+    vel = rng.normal(0.0, math.sqrt(theta_scale), size=(30, 3))
+    U = [*Z_ONE, *Z_ONE, *Z_ONE]
+    kw = dict(phi=phi, mach=mach, rep=rep, vrel=vrel, fac=fac)
+    a = rate(1, vel, U, **kw)
+    out = step(1, vel, np.zeros(3), U, **kw)
+    assert np.allclose(out, out[0], rtol=1e-15, atol=0.0)  # the same unit normal on each axis
+    return out[0] / math.sqrt(2.0 * a * fac)
+
+
+def test_noise_amplitude_does_not_depend_on_the_temperature_or_the_step():
+    s = [sigma_of(theta_scale=t, fac=f) for t in (0.5, 4.0, 30.0) for f in (1e-9, 5e-9)]
+    assert np.allclose(s, s[0], rtol=1e-7, atol=0.0), s
+
+
+def test_noise_amplitude_dilute_limit_is_schiller_naumann_drag():
+    rep, v = 50.0, np.linalg.norm([30.0, -4.0, 2.0])
+    fsn = 3.0 * math.pi * RMU * DP * (1.0 + 0.15 * rep ** 0.687) * v
+    for phi in (1e-6, 1e-7):
+        s = sigma_of(mach=0.0, phi=phi, rep=rep)
+        assert abs(s / (F32(6.52) * phi * fsn) - 1.0) < 1e-4, phi
+
+
+def test_noise_amplitude_grows_with_mach_and_saturates():
+    ms = np.linspace(0.0, 3.0, 31)
+    s = np.array([sigma_of(mach=m) for m in ms])
+    assert np.all(np.diff(s) >= -1e-12 * s.max())
+    assert s[-1] > 1.2 * s[0]
+    assert abs(sigma_of(mach=5.0) / s[-1] - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("flag", [1, 2])
+def test_stationary_variance_of_the_discrete_process(flag):
+    """Iterating the step with the oracle's own draws: Var F -> sigma^2 / (1 - a fac / 2)
+    along the relative velocity (and on every axis for Lattanzi)."""
+    rng = np.random.default_rng(2)
+    vel = rng.normal(0.0, 2.0, size=(30, 3))
+    vrel = np.array([30.0, 0.0, 0.0])
+    fac = 5e-7  # a fac ~ 0.04: a correlation time of some 25 steps, ~5000 independent samples
+    sig = sigma_of(vrel=vrel, fac=fac)
+    a = rate(1, vel, [*Z_ONE, *Z_ONE, *Z_ONE], vrel=vrel, fac=fac)
+    assert 0.01 < a * fac < 0.1
+    n, burn = 120000, 2000
+    F = np.zeros(3)
+    hist = np.empty((n, 3))
+    for c in range(n):
+        U = [orc.fluct_uniform(5, c, 0, k) for k in range(6)]
+        F = step(flag, vel, F, U, vrel=vrel, fac=fac)
+        hist[c] = F
+    h = hist[burn:]
+    want = sig ** 2 / (1.0 - a * fac / 2.0)
+    assert abs(h[:, 0].var() / want - 1.0) < 0.1, (h[:, 0].var(), want, a * fac)
+    if flag == 1:
+        for k in (1, 2):
+            assert abs(h[:, k].var() / want - 1.0) < 0.1
+    else:
+        # the transverse part: between the sigmoid's bounds squared
+        tv = (h[:, 1].var() + h[:, 2].var()) / want
+        assert F32(0.43758848) ** 2 * 0.9 < tv < (F32(0.39356905) + F32(0.43758848)) ** 2 * 1.1, tv
+
+
+def frame_case(vrel, U, F=np.zeros(3)):
+    rng = np.random.default_rng(3)
+    vel = rng.normal(0.0, 2.0, size=(30, 3))
+    o2 = step(2, vel, F, U, vrel=vrel)
+    o1 = step(1, vel, np.zeros(3), U, vrel=vrel)  # Lattanzi: b sqrt(fac) Z(U1,U2), Z(U3,U4), Z(U5,U6)
+    return o1, o2
+
+
+@pytest.mark.parametrize("vrel", [(30.0, -4.0, 2.0), (-5.0, 12.0, 33.0), (25.0, 0.0, 0.0), (0.0, -20.0, 0.0),
+                                  (0.0, 0.0, 15.0), (0.0, 3.0, 4.0)])
+def test_osnes_noise_frame(vrel):
+    vrel = np.asarray(vrel, np.float64)
+    ahat = vrel / np.linalg.norm(vrel)
+    U = [0.31, 0.12, 0.27, 0.05, 0.0, 0.5]
+    o1, r = frame_case(vrel, U)
+    par = r @ ahat
+    perp = r - par * ahat
+    scale = np.abs(o1).max()
+    assert abs(par - o1[0]) <= 1e-12 * scale  # drag-direction noise: Lattanzi's amplitude and first normal
+    assert abs(perp @ ahat) <= 1e-12 * scale
+    fcf0 = F32(0.39356905) * 0.5 + F32(0.43758848)  # CD' = 0: the sigmoid's midpoint
+    assert abs(np.linalg.norm(perp) - fcf0 * abs(o1[1])) <= 1e-12 * scale
+    # U5 = 0: along ahat x e, e the first unit vector not parallel to ahat (x, else y, else z)
+    e = np.eye(3)[0] if abs(ahat[1]) + abs(ahat[2]) > 1e-8 else np.eye(3)[1]
+    b = np.cross(ahat, e)
+    b /= np.linalg.norm(b)
+    assert np.abs(perp - fcf0 * o1[1] * b).max() <= 1e-12 * scale
+    # a quarter turn of U5 turns the transverse noise by a right angle about ahat
+    U2 = list(U)
+    U2[4] = 0.25
+    _, r2 = frame_case(vrel, U2)
+    perp2 = r2 - (r2 @ ahat) * ahat
+    assert abs(perp2 @ perp) <= 1e-12 * scale ** 2 / abs(o1[1])
+    assert np.abs(perp2 - np.cross(ahat, perp)).max() <= 1e-12 * scale
+
+
+def test_osnes_transverse_amplitude_follows_the_drag_fluctuation():
+    vrel = np.array([30.0, -4.0, 2.0])
+    ahat = vrel / np.linalg.norm(vrel)
+    U = [0.31, 0.12, 0.27, 0.05, 0.0, 0.5]
+    sig = sigma_of(vrel=vrel)
+    o1, _ = frame_case(vrel, U)
+    rng = np.random.default_rng(3)
+    vel = rng.normal(0.0, 2.0, size=(30, 3))
+    a = rate(2, vel, U, vrel=vrel)
+    ratios = []
+    for s in (-40.0, -2.0, -0.5, 0.0, 0.5, 2.0, 40.0):
+        F = s * sig * ahat
+        r = step(2, vel, F, U, vrel=vrel) - (1.0 - a * 2e-9) * F
+        perp = r - (r @ ahat) * ahat
+        ratios.append(np.linalg.norm(perp) / abs(o1[1]))
+    ratios = np.array(ratios)
+    assert np.all(np.diff(ratios) > 0.0)
+    lo, hi = F32(0.43758848), F32(0.39356905) + F32(0.43758848)
+    assert abs(ratios[0] - lo) < 1e-9 and abs(ratios[-1] - hi) < 1e-9
+
+
+@pytest.mark.parametrize("vmag", [0.0, 1e-9])
+def test_osnes_small_relative_velocity_guard(vmag):
+    """Below 1e-8 m/s the axis is x whatever the velocity's direction and
+    there is no transverse noise (ppiclf_user_Fluctuations.f:317-328).  Only
+    a direct call of the step reaches it: SetYdot skips particles below
+    1e-3 m/s (ppiclf_user.f:335) before any force."""
+    vrel = np.array([0.0, vmag, 0.0])
+    U = [0.31, 0.12, 0.27, 0.05, 0.2, 0.5]
+    F = np.array([3e-6, -1e-6, 2e-6])
+    rng = np.random.default_rng(3)
+    vel = rng.normal(0.0, 2.0, size=(30, 3))
+    out = step(2, vel, F, U, vrel=vrel)
+    a = rate(2, vel, U, vrel=vrel)
+    o1 = step(1, vel, np.zeros(3), U, vrel=vrel)
+    r = out - (1.0 - a * 2e-9) * F
+    assert np.all(np.isfinite(out))
+    assert abs(r[1]) <= 1e-12 * np.abs(F).max() and abs(r[2]) <= 1e-12 * np.abs(F).max()
+    assert abs(r[0] - o1[0]) <= 1e-12 * np.abs(F).max()
+
+
+def cloud(npart=400, vel_sd=3.0, **pp):
+    mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=npart, jitter=0.05, seed=4, phi=0.3)
+    rng = np.random.default_rng(9)
+    y = y.copy()
+    y[:, 3:6] = rng.normal([20.0, 1.0, -1.0], vel_sd, size=(npart, 3))
+    picl.update(qsFluct=1, fluctSeed=3)
+    picl.update(pp)
+    o = orc.Region(mesh, **params)
+    o.set_cv(ic(o.cells()[1]))
+    o.picl_init(y, pc, **picl)
+    return o, mesh
+
+
+def brute_sums(y, i, nbw, box, per, lengths):
+    shifts = [np.zeros(3)]
+    for sx in ((-1, 0, 1) if per[0] else (0,)):
+        for sy in ((-1, 0, 1) if per[1] else (0,)):
+            for sz in ((-1, 0, 1) if per[2] else (0,)):
+                if sx or sy or sz:
+                    shifts.append(np.array([sx, sy, sz]) * lengths)
+    S = np.zeros(7)
+    S[:3] += y[i, 3:6]
+    S[3:6] += y[i, 3:6] ** 2
+    S[6] += 1
+    for sh in shifts:
+        d = y[:, :3] + sh - y[i, :3]
+        ok = ((d ** 2).sum(1) <= nbw ** 2) & (np.abs(d) <= box).all(1)
+        if not sh.any():
+            ok[i] = False
+        v = y[ok, 3:6]
+        S[:3] += v.sum(0)
+        S[3:6] += (v ** 2).sum(0)
+        S[6] += ok.sum()
+    return S
+
+
+@pytest.mark.parametrize("per", [(0, 0, 0), (0, 1, 1)])
+def test_box_means_equal_a_brute_force_search(per):
+    nbw, box = 4.0 * 115e-6 * 1.6, 2.2e-4
+    o, mesh = cloud(neighborWidth=nbw, fluctHalfWidth=box, periodicY=per[1], periodicZ=per[2])
+    y = o.picl_particles()["y"]
+    lengths = np.asarray(mesh["lengths"])
+    for i in range(0, len(y), 7):
+        got, want = o.picl_fluct_means(i), brute_sums(y, i, nbw, box, per, lengths)
+        assert got[6] == want[6], i
+        assert np.allclose(got[:6], want[:6], rtol=1e-13, atol=0.0), i
+    # enough neighbours to be a real test
+    assert np.mean([o.picl_fluct_means(i)[6] for i in range(0, len(y), 7)]) > 5.0
+
+
+def test_default_box_is_twice_the_last_elements_largest_extent():
+    o_default, mesh = cloud(neighborWidth=1e-3)
+    X = mesh["xyz"][mesh["c2v"][-1]]
+    ext = X.max(0) - X.min(0)
+    ext = np.where((ext > 1e-2) | (ext < 1e-7), 1.23456789e-10, ext)
+    o_explicit, _ = cloud(neighborWidth=1e-3, fluctHalfWidth=2.0 * ext.max())
+    n = o_default.picl_particles()["y"].shape[0]
+    for i in range(0, n, 5):
+        assert np.array_equal(o_default.picl_fluct_means(i), o_explicit.picl_fluct_means(i))
+    # and it is not the first element's (the jittered cells differ)
+    X0 = mesh["xyz"][mesh["c2v"][0]]
+    assert (X0.max(0) - X0.min(0)).max() != ext.max()
+
+
+def test_identical_velocities_give_zero_temperature_in_a_cloud():
+    o, _ = cloud(vel_sd=0.0, neighborWidth=1e-3, fluctHalfWidth=3e-4)
+    for i in range(0, 400, 3):
+        m = o.picl_fluct_means(i)
+        n = m[6]
+        th = ((m[3] + m[4] + m[5]) / n - ((m[0] / n) ** 2 + (m[1] / n) ** 2 + (m[2] / n) ** 2)) / 3.0
+        assert n > 1 and th <= 1e-12, (i, n, th)
+
+
+def test_coupled_fluctuations_are_seeded_and_finite():
+    runs = {}
+    for key, seed in (("a", 3), ("b", 3), ("c", 4)):
+        o, _ = cloud(qsFluct=2, fluctSeed=seed)
+        o.step(2)
+        runs[key] = (o.picl_particles()["y"], o.picl_rprop()[:, 24:27])
+    assert np.array_equal(runs["a"][0], runs["b"][0]) and np.array_equal(runs["a"][1], runs["b"][1])
+    assert not np.array_equal(runs["a"][1], runs["c"][1])
+    assert np.all(np.isfinite(runs["a"][1])) and np.abs(runs["a"][1]).max() > 0.0
+
+
+@pytest.mark.parametrize("bad", [dict(qsFluct=3), dict(qsFluct=-1), dict(qsFluct=1, qsFluctAdapt=2),
+                                 dict(qsFluct=1, fluctHalfWidth=-1.0)])
+def test_invalid_fluctuation_settings_are_rejected(bad):
+    with pytest.raises(Exception):
+        cloud(**bad)
