@@ -99,9 +99,12 @@ typedef struct picl_params {
   /* QSFLUCT (qs_fluct_flag, ppiclf_user_Fluctuations.f:32-418): 0 off; 1 the
    * Lattanzi and 2 the Osnes fluctuating quasi-steady force, an SDE step per
    * RK stage from the granular temperature of the neighbours within a box of
-   * half-width fluctHalfWidth (ppiclf_d2chk(2); 0 means rocpicl's
-   * 2 x the last element's largest extent, PICL_TEMP_InitSolver.F90:598-646)
-   * inside the neighbour width.  qsFluctAdapt 1 (QSFLUCTFILTADAPT) widens the
+   * half-width fluctHalfWidth (ppiclf_d2chk(2); 0 means what rocpicl's
+   * filter set-up leaves there: with the box projection the last element's
+   * largest extent, rocpicl's filter 4 MAXVAL(Max_EleLen)/2
+   * (PICL_TEMP_InitSolver.F90:598-646) halved by InitBoxFilter
+   * (ppiclf_solve.f:1611); with the Gaussian one its cut-off radius
+   * (ppiclf_solve.f:1558-1559)) inside the neighbour width.  qsFluctAdapt 1 (QSFLUCTFILTADAPT) widens the
    * box to the particle's adaptive filter.  The box filter only: the
    * Gaussian one (QSFLUCTFILT 1) reads a variable the reference never sets
    * (ppiclf_user_EvalNearestNeighbor.f:50,364).  The normal draws come from
@@ -143,6 +146,11 @@ int picl_gpu_get_cells(rflu_handle *h, double *phi, double *gradRhog, double *pr
  * GetProFldIJKEF): 40 per cell, (PHIP, FX, FY, FZ, E) x ppiclF node
  * n = i + 2j + 4k, field-major. */
 int picl_gpu_get_nodes(rflu_handle *h, double *proNode);
+/* With collisionalFlag >= 3: the vorticity and its material derivative per
+ * cell at the last stage, 6 per cell (rprop JXVOR..JZVOR, JSDOX..JSDOZ: the
+ * fields PICL_TEMP_Runge.F90:747-809 assembles and :880-882,890-892 hand to
+ * ppiclf_solve_InterpFieldUser). */
+int picl_gpu_get_rot_fields(rflu_handle *h, double *rot6);
 /* Values LocalInterp produced at the last stage, 20 per particle in the
  * order above (kept only while picl_gpu_set_debug(h, 1) is on). */
 int picl_gpu_get_interp(rflu_handle *h, double *rprop20);

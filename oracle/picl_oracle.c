@@ -1839,7 +1839,7 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   setup_mesh(r, p);
   if (pp->collisionalFlag >= 3) p->fldRot = calloc(6 * (size_t)p->ne, sizeof(double));
   if (pp->qsFluct >= 1) {
-    /* PICL_TEMP_InitSolver.F90:598-646: d2chk(2) = 4 MAXVAL(Max_EleLen) / 2,
+    /* PICL_TEMP_InitSolver.F90:598-646: filter = 4 MAXVAL(Max_EleLen) / 2,
      * where Max_EleLen is reset for every element, so the last element's
      * extents decide (those outside [1e-7, 1e-2] count as 1.23456789e-10) */
     double mx = 0.0;
@@ -1855,7 +1855,16 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
       if (d == 0 || mel > mx) mx = mel;
     }
     double filter = 4.0 * mx;
-    p->fluctW = pp->fluctHalfWidth > 0.0 ? pp->fluctHalfWidth : filter / 2.0;
+    filter = filter / 2.0;
+    /* ppiclf_solve_InitBoxFilter(filter) halves it (ppiclf_solve.f:1611);
+     * the Gaussian filter's set-up puts its cut-off radius there instead
+     * (:1558-1559) */
+    double w2 = filter / 2.0;
+    if (pp->projMode == 1) {
+      double rsig = pp->filterWidth / (2.0 * sqrt(2.0 * log(2.0)));
+      w2 = rsig * sqrt(-2.0 * log(pp->filterAlpha));
+    }
+    p->fluctW = pp->fluctHalfWidth > 0.0 ? pp->fluctHalfWidth : w2;
   }
   r->picl = p;
   int N = r->nCells;

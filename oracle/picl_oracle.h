@@ -48,8 +48,9 @@ typedef struct orc_picl_params {
                            largest diameter (PICL_TEMP_InitSolver.F90:655-661 with NEIGHBORWIDTH 0) */
   int qsFluct;          /* QSFLUCT: 0 off, 1 Lattanzi, 2 Osnes quasi-steady force fluctuations */
   int qsFluctAdapt;     /* QSFLUCTFILTADAPT: 1 widens the box to the particle's adaptive filter */
-  double fluctHalfWidth; /* ppiclf_d2chk(2), the box filter's half-width; 0 means rocpicl's
-                            2 x the last element's largest extent */
+  double fluctHalfWidth; /* ppiclf_d2chk(2), the box filter's half-width; 0 means what the filter's
+                            set-up leaves: the last element's largest extent (box projection) or
+                            the Gaussian filter's cut-off radius */
   long long fluctSeed;  /* seed of the uniform draws that stand in for RANDOM_NUMBER */
 } orc_picl_params;
 

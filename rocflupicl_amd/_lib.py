@@ -30,7 +30,7 @@ RFLU_SYMBOLS = [
 PICL_SYMBOLS = [
     "picl_gpu_init", "picl_gpu_runge", "picl_gpu_sizes", "picl_gpu_get_particles", "picl_gpu_get_cells",
     "picl_gpu_get_interp", "picl_gpu_set_debug", "picl_gpu_locate", "picl_gpu_get_nodes",
-    "picl_gpu_locate_static",
+    "picl_gpu_locate_static", "picl_gpu_get_rot_fields",
 ]
 
 # Every symbol include/ppiclf_gpu.h declares (ppiclF's F77 entry points and
@@ -132,6 +132,7 @@ def load():
     L.picl_gpu_get_cells.argtypes = [P, P, P, P, P]
     L.picl_gpu_get_interp.argtypes = [P, P]
     L.picl_gpu_get_nodes.argtypes = [P, P]
+    L.picl_gpu_get_rot_fields.argtypes = [P, P]
     L.picl_gpu_set_debug.argtypes = [P, I]
     L.picl_gpu_locate.argtypes = [P, I, P, P]
     L.picl_gpu_locate_static.argtypes = [P, I, P, P]

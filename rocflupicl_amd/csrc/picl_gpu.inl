@@ -2907,17 +2907,28 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   }
   for (int l = 0; l < 3; l++) P.d2Max[l] = lastExt2[l] * (1.5 * 1.5);
   if (P.fl) {
-    // PICL_TEMP_InitSolver.F90:598-646: d2chk(2) = 4 MAXVAL(Max_EleLen) / 2,
+    // d2chk(2) as the filter's set-up leaves it.  The box filter: rocpicl's
+    // filter = 4 MAXVAL(Max_EleLen) / 2 (PICL_TEMP_InitSolver.F90:598-646,
     // Max_EleLen reset for every element, so the last element's extents
-    // decide (those outside [1e-7, 1e-2] count as 1.23456789e-10)
-    double mxl = 0.0;
-    for (int l = 0; l < 3; l++) {
-      double mel = 1.23456789e-10;
-      if (!(lastLen[l] > 1e-2 || lastLen[l] < 1e-7) && lastLen[l] > mel) mel = lastLen[l];
-      if (l == 0 || mel > mxl) mxl = mel;
+    // decide; those outside [1e-7, 1e-2] count as 1.23456789e-10), which
+    // ppiclf_solve_InitBoxFilter halves (ppiclf_solve.f:1611).  The Gaussian
+    // filter: its cut-off radius (ppiclf_solve.f:1558-1559).
+    double w2;
+    if (pp->projMode == 1) {
+      double rsig = pp->filterWidth / (2.0 * sqrt(2.0 * log(2.0)));
+      w2 = rsig * sqrt(-2.0 * log(pp->filterAlpha));
+    } else {
+      double mxl = 0.0;
+      for (int l = 0; l < 3; l++) {
+        double mel = 1.23456789e-10;
+        if (!(lastLen[l] > 1e-2 || lastLen[l] < 1e-7) && lastLen[l] > mel) mel = lastLen[l];
+        if (l == 0 || mel > mxl) mxl = mel;
+      }
+      double filter = 4.0 * mxl;
+      filter = filter / 2.0;
+      w2 = filter / 2.0;
     }
-    double filter = 4.0 * mxl;
-    s->phys.fluctW = pp->fluctHalfWidth > 0.0 ? pp->fluctHalfWidth : filter / 2.0;
+    s->phys.fluctW = pp->fluctHalfWidth > 0.0 ? pp->fluctHalfWidth : w2;
   }
   // location bins: one maximal element extent per bin, members in reference order
   std::vector<int> refOrder(N);
@@ -3279,6 +3290,14 @@ int picl_gpu_get_nodes(rflu_handle *h, double *proNode) {
   return 0;
 }
 
+int picl_gpu_get_rot_fields(rflu_handle *h, double *rot6) {
+  API_LOCK;
+  if (!h || !h->picl || !rot6) return RFLU_GPU_EINVAL;
+  PiclDev &P = h->picl->P;
+  if (!P.rot3) return set_err(h, RFLU_GPU_EINVAL, "the rotational fields exist only with collisionalFlag >= 3");
+  return from_device_cells(h, rot6, P.fldRot, 6, true);
+}
+
 int picl_gpu_set_debug(rflu_handle *h, int on) {
   API_LOCK;
   if (!h || !h->picl) return RFLU_GPU_EINVAL;
@@ -3382,7 +3401,7 @@ struct F77State {
   double width = 0.0, alpha = 0.0, nbw = 0.0, filt2 = 0.0;
   int stage = 0;  // ppiclf_solve_IntegrateRK3s_Rocflu's SAVE stage counter
   int icnt = 0;   // PPICLF_INT_ICNT
-  std::vector<double> fld;  // [PICL_NFLD][nCells], the host's cell numbering
+  std::vector<double> fld;  // [PICL_NFLD + 6][nCells], the host's cell numbering
   std::vector<char> have;
   std::vector<double> pro;  // [nCells][5][8] projection read back
   bool proValid = false;
@@ -3393,6 +3412,9 @@ F77State g_f77;
 
 // rprop index (PPICLF_USER.h) of each field the device interpolates, in its order
 const int F77_FIELD_RPROP[PICL_NFLD] = {2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39};
+// with COLLISIONS >= 3: the vorticity and its material derivative (JXVOR..JZVOR,
+// JSDOX..JSDOZ; PICL_TEMP_Runge.F90:880-882,890-892)
+const int F77_ROT_RPROP[6] = {31, 32, 33, 43, 44, 45};
 
 void f77_fail(const std::string &m) {
   g_f77.err = m;
@@ -3485,7 +3507,7 @@ void ppiclf_solve_initparticle_(const int *imethod, const int *ndim, const int *
     g_f77.pc[4 * (size_t)p + 2] = r[3];
     g_f77.pc[4 * (size_t)p + 3] = r[21];
   }
-  g_f77.filt2 = *filt2;  // ppiclf_d2chk(2), the fluctuation box's half-width
+  g_f77.filt2 = *filt2;  // ppiclf_d2chk(2) until the filter's set-up resets it
   g_f77.nbw = *filt3;  // ppiclf_d2chk(3) until InitNeighborBin resets it (ppiclf_solve.f:224-226)
   g_f77.linit = true;
 }
@@ -3493,10 +3515,10 @@ void ppiclf_solve_initparticle_(const int *imethod, const int *ndim, const int *
 // ppiclf_solve.f:1579-1640
 void ppiclf_solve_initboxfilter_(const double *filt, const int *iwallm, const int *sngl_elem) {
   API_LOCK;
-  (void)filt;
   if (!g_f77.linit) return f77_fail("InitParticle must be before InitFilter");
   if (*iwallm != 0) return f77_fail("the wall mirror (iwallm 1) is not built");
   if (*sngl_elem != 1) return f77_fail("the multi-element box filter (sngl_elem 0) is not built");
+  g_f77.filt2 = *filt / 2.0;  // d2chk(2) (ppiclf_solve.f:1611)
   g_f77.proj = 0;
   g_f77.lfilt = true;
 }
@@ -3509,6 +3531,8 @@ void ppiclf_solve_initgaussianfilter_(const double *filt, const double *alpha, c
   g_f77.proj = 1;
   g_f77.width = *filt;
   g_f77.alpha = *alpha;
+  const double rsig = *filt / (2.0 * sqrt(2.0 * log(2.0)));
+  g_f77.filt2 = rsig * sqrt(-2.0 * log(*alpha));  // d2chk(2), the cut-off (ppiclf_solve.f:1558-1559)
   g_f77.lfilt = true;
 }
 
@@ -3550,8 +3574,8 @@ void ppiclf_comm_initoverlapmesh_(const int *ncell, const int *lx1, const int *l
   p.fluctHalfWidth = g_f77.filt2;
   if (int rc = picl_gpu_init(h, &p, g_f77.npart, g_f77.y.data(), g_f77.pc.data()))
     return f77_fail("InitOverlap: " + h->err + " (code " + std::to_string(rc) + ")");
-  g_f77.fld.assign((size_t)PICL_NFLD * N, 0.0);
-  g_f77.have.assign(PICL_NFLD, 0);
+  g_f77.fld.assign((size_t)(PICL_NFLD + 6) * N, 0.0);
+  g_f77.have.assign(PICL_NFLD + 6, 0);
   g_f77.overlap = true;
   g_f77.stage = 0;
   g_f77.icnt = 0;
@@ -3572,6 +3596,9 @@ void ppiclf_solve_interpfielduser_(const int *jp, const double *infld) {
   int k = -1;
   for (int q = 0; q < PICL_NFLD; q++)
     if (F77_FIELD_RPROP[q] == *jp) k = q;
+  if (g_f77.h->picl->P.rot3)
+    for (int q = 0; q < 6; q++)
+      if (F77_ROT_RPROP[q] == *jp) k = PICL_NFLD + q;
   if (k < 0) return;
   const int N = g_f77.h->mesh.nCells;
   const bool tri = g_f77.h->picl->phys.interpMode == 1;
@@ -3600,17 +3627,24 @@ void ppiclf_solve_integrateparticle_(const int *istep, const int *iostep, const 
   (void)istep; (void)iostep; (void)time;
   if (!g_f77.overlap) return f77_fail("InitOverlap must come before IntegrateParticle");
   rflu_handle *h = g_f77.h;
-  for (int k = 0; k < PICL_NFLD; k++)
+  PiclDev &P = h->picl->P;
+  const int nf = PICL_NFLD + (P.rot3 ? 6 : 0);
+  for (int k = 0; k < nf; k++)
     if (!g_f77.have[k])
-      return f77_fail("IntegrateParticle: field rprop " + std::to_string(F77_FIELD_RPROP[k]) +
+      return f77_fail("IntegrateParticle: field rprop " +
+                      std::to_string(k < PICL_NFLD ? F77_FIELD_RPROP[k] : F77_ROT_RPROP[k - PICL_NFLD]) +
                       " was not given by InterpFieldUser this stage");
   const int N = h->mesh.nCells;
-  std::vector<double> dev((size_t)PICL_NFLD * N);
-  for (int i = 0; i < N; i++)
+  std::vector<double> dev((size_t)PICL_NFLD * N), rot(P.rot3 ? 6 * (size_t)N : 0);
+  for (int i = 0; i < N; i++) {
     for (int k = 0; k < PICL_NFLD; k++) dev[(size_t)PICL_NFLD * i + k] = g_f77.fld[(size_t)k * N + h->perm[i]];
-  PiclDev &P = h->picl->P;
+    for (int q = 0; q < (int)(rot.size() / N); q++)
+      rot[6 * (size_t)i + q] = g_f77.fld[(size_t)(PICL_NFLD + q) * N + h->perm[i]];
+  }
   g_f77.stage = g_f77.stage % 3 + 1;
   if (hipMemcpyAsync(P.fld, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      (P.rot3 && hipMemcpyAsync(P.fldRot, rot.data(), rot.size() * sizeof(double), hipMemcpyHostToDevice,
+                                h->stream) != hipSuccess) ||
       hipMemcpyAsync(h->d.scal + S_DTMIN, dt, sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
     return f77_fail("IntegrateParticle: upload failed");
   picl_particle_stage(h, g_f77.stage);

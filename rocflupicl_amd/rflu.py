@@ -244,6 +244,13 @@ class Region:
         self._chk(self._L.picl_gpu_get_nodes(self.h, _p(a)))
         return a
 
+    def picl_rot_fields(self):
+        """Vorticity and its material derivative per cell at the last stage
+        (collisionalFlag >= 3), (nCells, 6)."""
+        a = np.zeros((self.nCells, 6))
+        self._chk(self._L.picl_gpu_get_rot_fields(self.h, _p(a)))
+        return a
+
     def picl_set_debug(self, on):
         self._chk(self._L.picl_gpu_set_debug(self.h, 1 if on else 0))
 

@@ -43,10 +43,14 @@ def same_bits(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-def case(**picl_over):
+def case(mono=False, **picl_over):
     xs = 0.1 * 12 * H
     mesh, params, ic, (y, pc), picl = configs.curtain(nx=12, ny=4, nz=4, npart=500, jitter=0.05, x_shock=xs,
                                                       x_curtain=xs - H, limiterGuard=1)
+    if mono:  # Briney's added mass holds for equal spheres
+        pc = pc.copy()
+        pc[:, 1] = 115e-6
+        pc[:, 2] = np.pi / 6.0 * 115e-6 ** 3
     picl.update(picl_over)
     return mesh, params, ic, y, pc, picl
 
@@ -102,10 +106,10 @@ def pro_fld(L, nCells):
     return out
 
 
-def run_pair(picl_over, steps=2):
+def run_pair(picl_over, steps=2, mono=False):
     L = _lib.load()
     L.ppiclf_gpu_set_abort(0)
-    mesh, params, ic, y, pc, picl = case(**picl_over)
+    mesh, params, ic, y, pc, picl = case(mono=mono, **picl_over)
     a = fused(mesh, params, ic, y, pc, picl)
     b = f77_init(L, mesh, params, ic, y, pc, picl)
     assert np.array_equal(pro_fld(L, b.nCells), a.picl_nodes())  # InitSolve's projection
@@ -119,8 +123,14 @@ def run_pair(picl_over, steps=2):
             for k, jp in enumerate(rflu.PICL_FIELD_RPROP):
                 infld = np.ascontiguousarray(np.repeat(f[:, k][:, None], 8, axis=1))
                 L.ppiclf_solve_interpfielduser_(ri(jp), ptr(infld))
-            zero = np.zeros((b.nCells, 8))
-            L.ppiclf_solve_interpfielduser_(ri(31), ptr(zero))  # vorticity: read by no built model
+            if picl.get("collisionalFlag", 0) >= 3:  # vorticity and D(omega)/Dt (PICL_TEMP_Runge.F90:880-892)
+                r = a.picl_rot_fields()
+                for k, jp in enumerate((31, 32, 33, 43, 44, 45)):
+                    infld = np.ascontiguousarray(np.repeat(r[:, k][:, None], 8, axis=1))
+                    L.ppiclf_solve_interpfielduser_(ri(jp), ptr(infld))
+            else:
+                zero = np.zeros((b.nCells, 8))
+                L.ppiclf_solve_interpfielduser_(ri(31), ptr(zero))  # vorticity: read by no model that is on
             L.ppiclf_solve_integrateparticle_(ri(1), ri(0), rd(dt), rd(a.time()[0]))
             assert L.ppiclf_gpu_last_error().decode() == ""
             history.append((a.picl_particles(), b.picl_particles(), a.picl_nodes(), pro_fld(L, b.nCells)))
@@ -133,6 +143,30 @@ def run_pair(picl_over, steps=2):
 def test_f77_stages_equal_the_fused_stage_trilinear(proj):
     over = dict(interpMode=1, qsFlag=5, projMode=proj, filterWidth=2 * H, filterAlpha=1e-2)
     for pa, pb, na, nb in run_pair(over):
+        for k in ("y", "y1", "ydot", "ydotc", "elem"):
+            assert same_bits(pa[k], pb[k]) if k != "elem" else np.array_equal(pa[k], pb[k]), k
+        assert same_bits(na, nb)
+    assert np.abs(pa["y"][:, 3]).max() > 0.0
+
+
+def test_f77_stages_with_torque_and_lift_equal_the_fused_stage():
+    """COLLISIONS 4: the host hands the vorticity and its material
+    derivative over as six more fields, and the rotational state follows."""
+    over = dict(interpMode=1, qsFlag=5, collisionalFlag=4, ksp=10.0, erest=0.7)
+    for pa, pb, na, nb in run_pair(over):
+        for k in ("y", "y1", "ydot", "ydotc", "elem"):
+            assert same_bits(pa[k], pb[k]) if k != "elem" else np.array_equal(pa[k], pb[k]), k
+        assert same_bits(na, nb)
+    assert np.abs(pa["ydot"][:, 7:10]).max() > 0.0  # the torques act
+
+
+def test_f77_stages_with_briney_and_fluctuations_equal_the_fused_stage():
+    """Briney's added mass and Osnes' fluctuations through the F77 route:
+    InitNeighborBin's width (4 d_p) is the fused route's default, and
+    InitBoxFilter(2 dx) leaves d2chk(2) = dx, which the fused route is given."""
+    over = dict(interpMode=1, qsFlag=5, amFlag=2, qsFluct=2, fluctSeed=5, fluctHalfWidth=H, collisionalFlag=2,
+                ksp=10.0, erest=0.7)
+    for pa, pb, na, nb in run_pair(over, mono=True):
         for k in ("y", "y1", "ydot", "ydotc", "elem"):
             assert same_bits(pa[k], pb[k]) if k != "elem" else np.array_equal(pa[k], pb[k]), k
         assert same_bits(na, nb)

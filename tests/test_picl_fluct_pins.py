@@ -317,18 +317,37 @@ def test_box_means_equal_a_brute_force_search(per):
     assert np.mean([o.picl_fluct_means(i)[6] for i in range(0, len(y), 7)]) > 5.0
 
 
-def test_default_box_is_twice_the_last_elements_largest_extent():
+def same_means(a, b, n):
+    return all(np.array_equal(a.picl_fluct_means(i), b.picl_fluct_means(i)) for i in range(0, n, 5))
+
+
+def test_default_box_is_the_last_elements_largest_extent():
+    """rocpicl's filter is 4 x the largest extent / 2 and InitBoxFilter
+    halves it (PICL_TEMP_InitSolver.F90:598-688, ppiclf_solve.f:1611)."""
     o_default, mesh = cloud(neighborWidth=1e-3)
     X = mesh["xyz"][mesh["c2v"][-1]]
     ext = X.max(0) - X.min(0)
     ext = np.where((ext > 1e-2) | (ext < 1e-7), 1.23456789e-10, ext)
-    o_explicit, _ = cloud(neighborWidth=1e-3, fluctHalfWidth=2.0 * ext.max())
     n = o_default.picl_particles()["y"].shape[0]
-    for i in range(0, n, 5):
-        assert np.array_equal(o_default.picl_fluct_means(i), o_explicit.picl_fluct_means(i))
-    # and it is not the first element's (the jittered cells differ)
+    assert same_means(o_default, cloud(neighborWidth=1e-3, fluctHalfWidth=ext.max())[0], n)
+    # the box binds: twice the extent, or the first element's, selects other neighbours
+    assert not same_means(o_default, cloud(neighborWidth=1e-3, fluctHalfWidth=2.0 * ext.max())[0], n)
     X0 = mesh["xyz"][mesh["c2v"][0]]
-    assert (X0.max(0) - X0.min(0)).max() != ext.max()
+    assert not same_means(o_default, cloud(neighborWidth=1e-3, fluctHalfWidth=(X0.max(0) - X0.min(0)).max())[0], n)
+
+
+def test_default_box_with_the_gaussian_filter_is_its_cut_off_radius():
+    """InitGaussianFilter puts the radius where the Gaussian of full width
+    at half maximum filterWidth falls to filterAlpha of its peak into
+    d2chk(2) (ppiclf_solve.f:1558-1559)."""
+    fw, alpha = 1.2e-3, 1e-2
+    sig = fw / (2.0 * math.sqrt(2.0 * math.log(2.0)))  # FWHM = 2 sqrt(2 ln 2) sigma
+    r = sig * math.sqrt(-2.0 * math.log(alpha))  # exp(-r^2 / (2 sigma^2)) = alpha
+    g = dict(projMode=1, filterWidth=fw, filterAlpha=alpha, neighborWidth=2e-3)
+    o_default, _ = cloud(**g)
+    n = o_default.picl_particles()["y"].shape[0]
+    assert same_means(o_default, cloud(fluctHalfWidth=r, **g)[0], n)
+    assert not same_means(o_default, cloud(fluctHalfWidth=0.8 * r, **g)[0], n)
 
 
 def test_identical_velocities_give_zero_temperature_in_a_cloud():
