@@ -173,23 +173,24 @@ def build_workload(name):
                                               "4M particles", n=None, particles=parts, picl=picl)
     if name == "curtain_deck":
         # the shock-particle-curtain with the particle physics of the AirShock
-        # input deck (input_files/AirShock/rectshktb.inp:97-116) as far as it
-        # is built: Osnes quasi-steady drag, pressure gradient, viscous-unsteady
-        # force, collisions with tangential force and torques (the deck's
-        # level 4 also adds hydrodynamic torque and lift), KSP 10, EREST 0.7,
-        # periodic y and z; added mass Parmar (the deck's Briney model and
-        # its QS fluctuations are not built)
-        mesh, params, ic, parts, picl = configs.curtain(limiterGuard=1)
-        picl.update(CURTAIN_DECK)
-        return (mesh, params, ic), dict(model="RFLU 2nd-order AUSM+ + ppiclF (IDW-27, QS Osnes, AM Parmar, PG, "
-                                              "viscous unsteady, collisions level 2, periodic y/z, box projection), "
-                                              "400x80x80 hex, 4M particles", n=None, particles=parts, picl=picl)
+        # input deck (input_files/AirShock/rectshktb.inp:97-116): Osnes drag,
+        # Briney added mass, pressure gradient, collisions level 4 with torques
+        # and lift, viscous-unsteady force, Osnes fluctuations, periodic y and
+        # z.  Briney's model holds for equal spheres, so the cloud is
+        # monodisperse (d_p = 115 um), and, like a particle file's cloud, no
+        # two particles touch at the start (the deck's file is not shipped)
+        mesh, params, ic, parts, picl = configs.curtain(limiterGuard=1, sigma_ln=0.0, placement="lattice")
+        picl.update(configs.AIRSHOCK_PICL)
+        return (mesh, params, ic), dict(model="RFLU 2nd-order AUSM+ + ppiclF with the AirShock deck's particle "
+                                              "physics (IDW-27, QS Osnes + fluctuations, AM Briney, PG, viscous "
+                                              "unsteady, collisions level 4, periodic y/z, box projection), "
+                                              "400x80x80 hex, 4M monodisperse particles", n=None, particles=parts,
+                                        picl=picl)
     raise SystemExit(f"unknown workload {name}")
 
 
 # h = 0.0005 is the curtain's spacing
 CURTAIN_SPEC = dict(interpMode=1, qsFlag=5, projMode=1, filterWidth=2 * 0.0005, filterAlpha=1e-2)
-CURTAIN_DECK = dict(qsFlag=2, viscousUnsteady=1, collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
 
 
 def cpu_model():
@@ -208,11 +209,14 @@ def _oracle_sample(name):
     from oracle import oracle as orc
     from rocflupicl_amd import configs
     if name.startswith("curtain"):
-        mesh, params, ic, (y, pc), picl = configs.curtain(nx=40, ny=8, nz=8, npart=4000, limiterGuard=1)
+        deck = name == "curtain_deck"
+        mesh, params, ic, (y, pc), picl = configs.curtain(nx=40, ny=8, nz=8, npart=4000, limiterGuard=1,
+                                                          sigma_ln=0.0 if deck else 0.2,
+                                                          placement="lattice" if deck else "random")
         if name == "curtain_spec":
             picl.update(CURTAIN_SPEC)
         if name == "curtain_deck":
-            picl.update(CURTAIN_DECK)
+            picl.update(configs.AIRSHOCK_PICL)
         o = orc.Region(mesh, **params)
         o.set_cv(ic(o.cells()[1]))
         o.picl_init(y, pc, **picl)

@@ -91,14 +91,27 @@ def shock_hex(nx=400, ny=80, nz=80, lx=0.2, ly=0.04, lz=0.04, mach=2.0, x_shock=
 PICL_AIR = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1, amFlag=1,
                 pgFlag=1, feedbackFlag=1)
 
+# The particle physics of the AirShock input deck
+# (input_files/AirShock/rectshktb.inp:97-116): Osnes quasi-steady drag, Briney
+# added mass, pressure gradient, collisions level 4 (normal and tangential
+# forces, contact, rolling, hydrodynamic and undisturbed torques, Saffman and
+# Magnus lift) with KSP 10 and EREST 0.7, the viscous-unsteady force, Osnes'
+# quasi-steady fluctuations with the fixed box filter, periodic in y and z.
+AIRSHOCK_PICL = dict(qsFlag=2, amFlag=2, pgFlag=1, collisionalFlag=4, ksp=10.0, erest=0.7, viscousUnsteady=1,
+                     heatTransfer=0, feedbackFlag=1, qsFluct=2, qsFluctAdapt=0, periodicY=1, periodicZ=1)
+
 
 def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=None, x_curtain=None,
             phi=0.2, d_median=115e-6, sigma_ln=0.2, rho_p=2500.0, discr=4, jitter=0.0, seed=4,
-            nparts_cap=None, bc=None, **kw):
+            nparts_cap=None, bc=None, placement="random", **kw):
     """Shock-particle-curtain: hex box nx*ny*nz of spacing h, Mach `mach`
     shock at x_shock moving +x into quiescent air, and a slab of `npart`
-    particles (lognormal diameters, uniform positions) whose thickness gives
-    the volume fraction `phi` across the box section.
+    particles (lognormal diameters) whose thickness gives the volume fraction
+    `phi` across the box section.  placement "random": uniform positions
+    (particles may overlap); "lattice": a jittered lattice whose spacing
+    exceeds the largest diameter, so no two particles touch at the start (a
+    particle file's cloud), fitted to the section so periodic images keep
+    the spacing.
 
     Returns (mesh, params, ic, particles, picl_params) where particles =
     (y[n,12], pconst[n,4] = (rho_p, d_p, V_p, SPL)).  ic(cofg) gives the
@@ -128,9 +141,26 @@ def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=N
     area = (ny * h) * (nz * h)
     thick = vp.sum() / (phi * area)
     y = np.zeros((npart, 12))
-    y[:, 0] = x0 + thick * rng.random(npart)
-    y[:, 1] = ny * h * rng.random(npart)
-    y[:, 2] = nz * h * rng.random(npart)
+    if placement == "lattice":
+        a = (vp.mean() / phi) ** (1.0 / 3.0)
+        nl = [max(1, int(ny * h // a)), max(1, int(nz * h // a))]
+        per_layer = nl[0] * nl[1]
+        nxl = -(-npart // per_layer)
+        sp = np.array([thick / nxl, ny * h / nl[0], nz * h / nl[1]])
+        if sp.min() <= dp.max():
+            raise ValueError("the lattice spacing does not exceed the largest diameter")
+        k = np.arange(npart)
+        idx = np.stack([k // per_layer, (k % per_layer) // nl[1], k % nl[1]], axis=1)
+        wob = 0.45 * (sp - dp.max()) * (2.0 * rng.random((npart, 3)) - 1.0)
+        xyz = (idx + 0.5) * sp + wob
+        xyz[:, 0] += x0
+        y[:, :3] = xyz[rng.permutation(npart)]
+    elif placement == "random":
+        y[:, 0] = x0 + thick * rng.random(npart)
+        y[:, 1] = ny * h * rng.random(npart)
+        y[:, 2] = nz * h * rng.random(npart)
+    else:
+        raise ValueError(f"unknown placement {placement!r}")
     y[:, 6] = 300.0
     pconst = np.stack([np.full(npart, rho_p), dp, vp, np.ones(npart)], axis=1)
     return mesh, params, ic, (y, pconst), dict(PICL_AIR)

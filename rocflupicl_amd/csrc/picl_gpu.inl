@@ -1492,20 +1492,27 @@ __global__ void __launch_bounds__(BLK) k_fb_weno(Dev d, PiclDev P) {
   if (i >= d.nC) return;
   const int N = d.nC;
   const unsigned *mem = reinterpret_cast<const unsigned *>(d.csMem);
-  double g[9];
+  // each member's 9 gradient components are read once (one 72-byte row) and
+  // feed the 9 sums, each summed in the reference's member order
+  double g[9], sum[9], loc[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) {
     double term = P.fbRaw[9 * (size_t)i + q];
-    double sum = 1.0 / (term * term + 1.0E-15);
-    double loc = sum * term;
-    for (int s = d.csOff[i]; s < d.csOff[i + 1]; s++) {
-      term = P.fbRaw[9 * (size_t)(mem[s] & CS_IDX) + q];
-      double w = 1.0 / (term * term + 1.0E-15);
-      sum = sum + w;
-      loc = loc + w * term;
-    }
-    g[q] = loc / sum;
+    sum[q] = 1.0 / (term * term + 1.0E-15);
+    loc[q] = sum[q] * term;
   }
+  for (int s = d.csOff[i]; s < d.csOff[i + 1]; s++) {
+    const double *row = P.fbRaw + 9 * (size_t)(mem[s] & CS_IDX);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double term = row[q];
+      double w = 1.0 / (term * term + 1.0E-15);
+      sum[q] = sum[q] + w;
+      loc[q] = loc[q] + w * term;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; q++) g[q] = loc[q] / sum[q];
   double varc = P.fb[i];
   double xc = d.cofg[i], yc = d.cofg[N + i], zc = d.cofg[2 * N + i];
   bool zero = false;
@@ -1563,8 +1570,10 @@ __global__ void k_vu_time(Dev d, PiclDev P) {
 // every particle's interpolated values): phase 1 locates and interpolates
 // into vint and records what the W store reads, phase 2 takes vint and the
 // pair kernel's sums and does the rest.
-template <int TRI, int QS>
-__global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage, int phase) {
+// PH1 = 1 instantiates phase 1 alone, without the force code's registers.
+template <int TRI, int QS, int PH1 = 0>
+__global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys Ph, int istage, int phaseArg) {
+  const int phase = PH1 ? 1 : phaseArg;
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
   const size_t nP = P.nP;
@@ -1737,7 +1746,10 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       P.vrelP[2 * nP + pid] = vz;
       P.rphifP[pid] = sqrt(vx * vx + vy * vy + vz * vz) <= 1.0e-3 ? 1.0 - v[9] : 1.0 - fmin(v[9], 0.62);
       P.actP[pid] = 1;
-      atomicMax(P.lastP, pid + 1);
+      // one address for every particle: read first, so only the few that
+      // raise the maximum issue the atomic (a stale read costs one more)
+      if (pid + 1 > __hip_atomic_load(P.lastP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(P.lastP, pid + 1);
     }
     return;
   }
@@ -2681,12 +2693,10 @@ void picl_particle_stage(rflu_handle *h, int istage) {
   if (P.pairs) {  // the particle kernel's first phase: locate and interpolate
     ProfScope ps(h, "k_interp");
     using KP = void (*)(Dev, PiclDev, PiclPhys, int, int);
-    static const KP kp[2][5] = {
-        {k_particles<0, 1>, k_particles<0, 2>, k_particles<0, 3>, k_particles<0, 4>, k_particles<0, 5>},
-        {k_particles<1, 1>, k_particles<1, 2>, k_particles<1, 3>, k_particles<1, 4>, k_particles<1, 5>}};
+    static const KP kp[2] = {k_particles<0, 1, 1>, k_particles<1, 1, 1>};
     if (P.am2) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(BLK), 0, h->stream, P.lastP, 1);
-    hipLaunchKernelGGL(kp[s->phys.interpMode][s->phys.qsFlag - 1], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d,
-                       P, s->phys, istage, 1);
+    hipLaunchKernelGGL(kp[s->phys.interpMode], dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, h->d, P, s->phys, istage,
+                       1);
   }
   if (P.pairs) {
     ProfScope ps(h, "k_pairs");

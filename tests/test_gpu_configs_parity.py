@@ -180,6 +180,29 @@ def test_c3_curtain_as_specified_matches_oracle(oracle_mod, jitter):
     assert max(errs.values()) <= TOL, errs
 
 
+def test_airshock_deck_physics_matches_oracle(oracle_mod):
+    """The curtain with the AirShock deck's particle physics (Osnes drag and
+    fluctuations, Briney added mass, pressure gradient, viscous-unsteady
+    force, collisions level 4 with torques and lift, periodic y and z) on a
+    monodisperse cloud, jittered mesh: 16 x 4 x 4 hexes, 1,500 particles, 4
+    steps (the oracle's neighbour searches are brute force)."""
+    xs = 0.1 * 16 * H
+    mesh, params, ic, parts, picl = configs.curtain(nx=16, ny=4, nz=4, npart=1500, jitter=0.05, x_shock=xs,
+                                                    x_curtain=xs - H, limiterGuard=1, sigma_ln=0.0)
+    picl.update(configs.AIRSHOCK_PICL)
+    g = gpu_alone(mesh, params, ic, 4, parts, picl)
+    o = oracle_alone(oracle_mod, mesh, params, ic, 4, parts, picl)
+    gp, op = g.picl_particles(), o.picl_particles()
+    assert np.array_equal(gp["elem"], op["elem"])
+    assert np.abs(op["y"][:, 3]).max() > 0.1
+    errs = {k: elem_err(gp[k], op[k], vectors=((0, 1, 2), (3, 4, 5), (7, 8, 9))) for k in ("y", "y1", "ydot", "ydotc")}
+    gcv, gdv = g.get_state()
+    errs["cv"] = elem_err(gcv, o.cv(), vectors=((1, 2, 3),))
+    errs["dv"] = elem_err(gdv, o.dv())
+    print("AirShock deck element-wise relative errors:", errs)
+    assert max(errs.values()) <= TOL, errs
+
+
 def test_c3_full_size_closed_box_conserves_gas_mass():
     """400 x 80 x 80 hexes (2,560,000 cells) and 4,000,000 particles with the
     specified physics, every side a slip wall."""
