@@ -79,6 +79,7 @@ struct PiclDev {
   double *ycN = nullptr;  // [4][nP] ydotc of this stage, by current slot
   int *src = nullptr;     // new position -> current slot (active particles)
   int *dest = nullptr;    // current slot -> new position
+  int *srt = nullptr;     // new position -> current slot, each element's particles by original index
   int *cnt = nullptr, *off = nullptr, *bsum = nullptr, *nAct = nullptr, *lostCur = nullptr;
   double *fld = nullptr;   // [nC][20]
   double *gRho = nullptr;  // [nC][4]
@@ -2090,63 +2091,67 @@ __global__ void __launch_bounds__(BLK) k_fill(PiclDev P) {
 // stage's ydotc (ppiclf_comm.f:1190-1217 runs before SetYdot).  The segment,
 // sorted by original index, fixes the particles' positions in the next order.
 // ---------------------------------------------------------------------------
-#define DEP_T 128
-#define DEP_CAP 64
 // Gaussian mode (P.gauss): the sorted segments instead receive one record per
 // particle, position at the start of the stage and map * multfci, read by
 // k_gauss.  posXo = 0 reads the position from the particle rows (set-up).
-__global__ void __launch_bounds__(DEP_T) k_deposit(PiclDev P, int posXo) {
-  __shared__ long long sk[DEP_T * DEP_CAP];
-  int e = blockIdx.x * DEP_T + threadIdx.x;
+// Each particle's place in its element's segment: its rank by original index
+// among the segment's particles (ppiclF's order within an element, which the
+// box sums follow).  One thread per segment slot; the segment's threads read
+// its (short) list together.
+__global__ void __launch_bounds__(BLK) k_rank(PiclDev P) {
+  int q = blockIdx.x * BLK + threadIdx.x;
+  if (q >= P.off[P.nC]) return;
+  PBuf C = cur_buf(P);
+  const int t = P.src[q], e = C.elem[t], b = P.off[e], n = P.off[e + 1] - b, my = C.pid[t];
+  int rank = 0;
+  for (int j = 0; j < n; j++) rank += C.pid[P.src[b + j]] < my ? 1 : 0;
+  P.srt[b + rank] = t;
+  P.dest[t] = b + rank;
+}
+
+// The Gaussian filter's particle records, in the sorted order: position at
+// the start of the stage and the five deposited values times multfci.
+__global__ void __launch_bounds__(BLK) k_gq(PiclDev P, int posXo) {
+  int q = blockIdx.x * BLK + threadIdx.x;
+  if (q >= P.off[P.nC]) return;
+  const size_t nP = P.nP;
+  PBuf C = cur_buf(P);
+  const int t = P.srt[q];
+  const double spl = C.pc[3 * nP + t];
+  double map[5];
+  map[0] = C.pc[2 * nP + t] * spl;
+#pragma unroll
+  for (int k = 0; k < 4; k++) map[1 + k] = C.st[(ROW_YCP + k) * nP + t] * spl;
+  const double *xs = posXo ? P.xo : C.st;
+  // the position record is what the deposit's candidate tests read: 32
+  // bytes, four to a 128-byte line
+  double2 *g = reinterpret_cast<double2 *>(P.gq + 4 * (size_t)q);
+  double2 *gm = reinterpret_cast<double2 *>(P.gq + 4 * (nP + (size_t)q));
+  g[0] = make_double2(xs[t], xs[nP + t]);
+  g[1] = make_double2(xs[2 * nP + t], map[0] * P.gMult);
+  gm[0] = make_double2(map[1] * P.gMult, map[2] * P.gMult);
+  gm[1] = make_double2(map[3] * P.gMult, map[4] * P.gMult);
+}
+
+// The box filter (ppiclf_solve.f:3285-3447 with the single-element box): the
+// element's particles summed in the sorted order.
+__global__ void __launch_bounds__(BLK) k_deposit_box(PiclDev P) {
+  int e = blockIdx.x * BLK + threadIdx.x;
   if (e >= P.nC) return;
   const size_t nP = P.nP;
   PBuf C = cur_buf(P);
-  int b = P.off[e], n = P.off[e + 1] - b;
-  int *L = P.src + b;
-  // sort the segment by original index: key = (pid << 32) | slot
-  if (n <= DEP_CAP) {
-    long long *K = sk + (size_t)threadIdx.x * DEP_CAP;
-    for (int q = 0; q < n; q++) {
-      int t = L[q];
-      K[q] = ((long long)C.pid[t] << 32) | (unsigned)t;
-    }
-    for (int a = 1; a < n; a++) {
-      long long x = K[a];
-      int j = a - 1;
-      while (j >= 0 && K[j] > x) { K[j + 1] = K[j]; j--; }
-      K[j + 1] = x;
-    }
-    for (int q = 0; q < n; q++) L[q] = (int)(unsigned)(K[q] & 0xffffffffll);
-  } else {
-    for (int a = 1; a < n; a++) {
-      int x = L[a], px = C.pid[x], j = a - 1;
-      while (j >= 0 && C.pid[L[j]] > px) { L[j + 1] = L[j]; j--; }
-      L[j + 1] = x;
-    }
-  }
+  const int b = P.off[e], n = P.off[e + 1] - b;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  double rexp = P.rexp[e];
+  const double rexp = P.rexp[e];
   for (int q = 0; q < n; q++) {
-    int t = L[q];
-    double spl = C.pc[3 * nP + t];
+    const int t = P.srt[b + q];
+    const double spl = C.pc[3 * nP + t];
     double map[5];
     map[0] = C.pc[2 * nP + t] * spl;
 #pragma unroll
     for (int k = 0; k < 4; k++) map[1 + k] = C.st[(ROW_YCP + k) * nP + t] * spl;
 #pragma unroll
     for (int k = 0; k < 5; k++) acc[k] = acc[k] + (map[k] * 1.0) * rexp;
-    P.dest[t] = b + q;
-    if (P.gauss) {
-      const double *xs = posXo ? P.xo : C.st;
-      // the position record is what the deposit's candidate tests read:
-      // 32 bytes, four to a 128-byte line
-      double2 *g = reinterpret_cast<double2 *>(P.gq + 4 * (size_t)(b + q));
-      double2 *gm = reinterpret_cast<double2 *>(P.gq + 4 * (nP + (size_t)(b + q)));
-      g[0] = make_double2(xs[t], xs[nP + t]);
-      g[1] = make_double2(xs[2 * nP + t], map[0] * P.gMult);
-      gm[0] = make_double2(map[1] * P.gMult, map[2] * P.gMult);
-      gm[1] = make_double2(map[3] * P.gMult, map[4] * P.gMult);
-    }
   }
 #pragma unroll
   for (int k = 0; k < 5; k++) P.proj[k * (size_t)P.nC + e] = 0.0 + acc[k];
@@ -2618,7 +2623,7 @@ void picl_free(PiclState *s) {
                   P.cbRec, P.dbg, P.vX, P.ev, P.ecb, P.binCnt, P.gq, P.xo, P.vproj, P.vperm, P.vuT,
                   P.colCnt, P.colOff, P.colFill, P.colMem, P.colB, P.colBsum, P.fc, P.tq, P.gBox, P.gvList,
                   P.buf[0].w, P.buf[1].w, P.vint, P.famB, P.snum, P.vrelP, P.rphifP, P.actP, P.lastP,
-                  P.fb, P.fbRaw, P.fbg, P.fldRot, P.buf[0].fq, P.buf[1].fq, P.fmean, P.nCall};
+                  P.fb, P.fbRaw, P.fbg, P.fldRot, P.buf[0].fq, P.buf[1].fq, P.fmean, P.nCall, P.srt};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   delete s;
 }
@@ -2643,7 +2648,11 @@ void picl_sort_and_deposit(rflu_handle *h, int posXo) {
   }
   {
     ProfScope ps(h, "k_deposit");
-    hipLaunchKernelGGL(k_deposit, dim3((P.nC + DEP_T - 1) / DEP_T), dim3(DEP_T), 0, h->stream, P, posXo);
+    hipLaunchKernelGGL(k_rank, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P);
+    if (P.gauss)
+      hipLaunchKernelGGL(k_gq, dim3(nblk(P.nP)), dim3(BLK), 0, h->stream, P, posXo);
+    else
+      hipLaunchKernelGGL(k_deposit_box, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
   }
   if (P.gauss) {
     ProfScope ps(h, "k_gauss");
@@ -3063,6 +3072,7 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
   rc |= dalloc(h, &P.ycN, 4 * (size_t)nP);
   rc |= dalloc(h, &P.src, (size_t)nP);
   rc |= dalloc(h, &P.dest, (size_t)nP);
+  rc |= dalloc(h, &P.srt, (size_t)nP);
   rc |= dalloc(h, &P.lostCur, 1, &zero);
   rc |= dalloc(h, &P.cnt, (size_t)N + 1);
   rc |= dalloc(h, &P.off, (size_t)N + 1);
