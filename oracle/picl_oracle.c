@@ -177,7 +177,7 @@ static int setup_mesh(orc_region *r, struct orc_picl *p) {
 /* |r|,|s|,|t| <= 1 + 1e-10 (Newton from the element centre).                */
 /* ------------------------------------------------------------------------ */
 static int trilinear_inside(const double *X, const double *xp, double *rst) {
-  double r = 0.0, s = 0.0, t = 0.0;
+  double r = 0.0, s = 0.0, t = 0.0, prev = 0.0;
   for (int it = 0; it < 25; it++) {
     double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
     double b[2] = {0.5 * (1.0 - s), 0.5 * (1.0 + s)};
@@ -206,7 +206,11 @@ static int trilinear_inside(const double *X, const double *xp, double *rst) {
     double ds = (J[0][0] * (f1 * J[2][2] - J[1][2] * f2) - f0 * m01 + J[0][2] * (J[1][0] * f2 - f1 * J[2][0])) / det;
     double dt = (J[0][0] * (J[1][1] * f2 - f1 * J[2][1]) - J[0][1] * (J[1][0] * f2 - f1 * J[2][0]) + f0 * m02) / det;
     r = r + dr; s = s + ds; t = t + dt;
-    if (fabs(dr) + fabs(ds) + fabs(dt) < 1.0e-14) break;
+    /* converged, or the steps no longer shrink: they are rounding noise,   */
+    /* about ulp(x) / (element size) in reference coordinates              */
+    double step = fabs(dr) + fabs(ds) + fabs(dt);
+    if (step < 1.0e-12 || (it >= 2 && step >= 0.5 * prev)) break;
+    prev = step;
   }
   if (rst) { rst[0] = r; rst[1] = s; rst[2] = t; }
   const double lim = 1.0 + 1.0e-10;

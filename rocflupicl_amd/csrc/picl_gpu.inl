@@ -108,6 +108,10 @@ struct PiclDev {
   double *vproj = nullptr;  // [5][nV]
   int *vperm = nullptr;     // vertices in Morton order (the order of the deposit's vertex list)
   double gRdum = 0, gMult = 0, gD2chk = 0, gReach = 0, gReach2 = 0;
+  // element test of the deposit: an element's particles lie within gHalf of
+  // its centroid per axis; it is a candidate when that box comes within
+  // the cut-off (gCut2 = the squared cut-off with a relative margin)
+  double gHalf[3] = {0, 0, 0}, gCut2 = 0;
   // periodic images of the Gaussian deposit: shift hi - lo per periodic
   // axis (0 elsewhere); cbTop = the largest centroid bin coordinate
   double gShift[3] = {0, 0, 0}, cbTop[3] = {0, 0, 0};
@@ -152,7 +156,7 @@ struct PiclDev {
 // ---------------------------------------------------------------------------
 __device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3], double *maxrst = nullptr,
                                  double *rst = nullptr) {
-  double r = 0.0, s = 0.0, t = 0.0;
+  double r = 0.0, s = 0.0, t = 0.0, prev = 0.0;
   for (int it = 0; it < 25; it++) {
     double a[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
     double b[2] = {0.5 * (1.0 - s), 0.5 * (1.0 + s)};
@@ -184,7 +188,11 @@ __device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3
     double ds = (J[0][0] * (f1 * J[2][2] - J[1][2] * f2) - f0 * m01 + J[0][2] * (J[1][0] * f2 - f1 * J[2][0])) / det;
     double dt = (J[0][0] * (J[1][1] * f2 - f1 * J[2][1]) - J[0][1] * (J[1][0] * f2 - f1 * J[2][0]) + f0 * m02) / det;
     r = r + dr; s = s + ds; t = t + dt;
-    if (fabs(dr) + fabs(ds) + fabs(dt) < 1.0e-14) break;
+    // converged, or the steps no longer shrink: they are rounding noise,
+    // about ulp(x) / (element size) in reference coordinates
+    double step = fabs(dr) + fabs(ds) + fabs(dt);
+    if (step < 1.0e-12 || (it >= 2 && step >= 0.5 * prev)) break;
+    prev = step;
   }
   const double lim = 1.0 + 1.0e-10;
   if (maxrst) *maxrst = fmax(fabs(r), fmax(fabs(s), fabs(t)));
@@ -196,11 +204,13 @@ __device__ bool trilinear_inside(const double *__restrict__ X, const double xp[3
 // the previous stage).  A point well inside it (|r|,|s|,|t| <= 1 - 1e-8) lies
 // in no other element, not even within the 1e-10 tolerance, so the rule's
 // answer is the hint itself; otherwise the full search decides.
-__device__ int picl_locate(const PiclDev &P, const double xp[3], int hint = -1) {
+// rst (optional): the point's reference coordinates in the element found
+// (the same Newton solution a separate call on that element gives).
+__device__ int picl_locate(const PiclDev &P, const double xp[3], int hint = -1, double *rst = nullptr) {
   if (!(isfinite(xp[0]) && isfinite(xp[1]) && isfinite(xp[2]))) return -1;
   if (hint >= 0) {
     double m = 2.0;
-    if (trilinear_inside(P.X + 24 * (size_t)hint, xp, &m) && m <= 1.0 - 1.0e-8) return hint;
+    if (trilinear_inside(P.X + 24 * (size_t)hint, xp, &m, rst) && m <= 1.0 - 1.0e-8) return hint;
   }
   int b[3];
 #pragma unroll
@@ -219,7 +229,7 @@ __device__ int picl_locate(const PiclDev &P, const double xp[3], int hint = -1) 
       double tol = 1.0e-9 * (bb[4 + d] - bb[d]);
       if (xp[d] < bb[d] - tol || xp[d] > bb[4 + d] + tol) in = false;
     }
-    if (in && trilinear_inside(P.X + 24 * (size_t)e, xp)) return e;
+    if (in && trilinear_inside(P.X + 24 * (size_t)e, xp, nullptr, rst)) return e;
   }
   return -1;
 }
@@ -1591,7 +1601,8 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
       for (int q = 0; q < 6; q++) vr[q] = P.vint[(PICL_NFLD + q) * nP + t];
   } else {
   double xp[3] = {st[t], st[nP + t], st[2 * nP + t]};
-  int e = picl_locate(P, xp, C.elem[t]);
+  double rst[3];
+  int e = picl_locate(P, xp, C.elem[t], TRI ? rst : nullptr);
   C.elem[t] = e;
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
@@ -1608,8 +1619,7 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
     // gslib findpts_eval_local on the element's 2x2x2 nodal field (every node
     // holds the cell value): Lagrange weights (1-r)/2, (1+r)/2 on the nodes
     // -1, 1, contracted along r, then s, then t
-    double rst[3];
-    trilinear_inside(P.X + 24 * (size_t)e, xp, nullptr, rst);
+    // rst: the location's Newton solution in element e
     double h[3][2];
 #pragma unroll
     for (int l = 0; l < 3; l++) {
@@ -2283,6 +2293,9 @@ __global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
 #define GW_CAP 256
 #define GW_WAVES (BLK / 64)
 #define GW_BLOCKS 8192  // a multiple of 8: 32 workgroups per CU
+#ifndef GW_OCC
+#define GW_OCC 4  // waves per SIMD the register budget allows
+#endif
 
 // Compensated accumulation, branch-free: Knuth's TwoSum gives the exact
 // rounding error of s + x whatever their magnitudes (the same error the
@@ -2298,12 +2311,12 @@ __device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
 // One batch of queued (particle slot, r^2) pairs within the cut-off: lane l
 // takes entry l (l < n).
 __device__ __forceinline__ void gauss_batch(const PiclDev &P, const int *Qp, const double *Qr, int n, int lane,
-                                            double acc[5], double cmp[5]) {
+                                            const double *T, double acc[5], double cmp[5]) {
   if (lane < n) {
     const size_t at = Qp[lane];
     const double2 *g = reinterpret_cast<const double2 *>(P.gq + 4 * at);
     const double2 *gm = reinterpret_cast<const double2 *>(P.gq + 4 * ((size_t)P.nP + at));
-    double rexp = exp(Qr[lane] * P.gRdum);
+    double rexp = gx::exp_tab(Qr[lane] * P.gRdum, T);  // the argument lies in [ln alpha, 0]
     double2 g1 = g[1], g2 = gm[0], g3 = gm[1];
     two_sum_acc(acc[0], cmp[0], g1.y * rexp);
     two_sum_acc(acc[1], cmp[1], g2.x * rexp);
@@ -2320,7 +2333,7 @@ __device__ __forceinline__ void gauss_batch(const PiclDev &P, const int *Qp, con
 // of them are summed as one batch.
 __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, const int *C, int n, int lane,
                                             const double xv[3], const double sh[3], int *Qp, double *Qr, int &qn,
-                                            double acc[5], double cmp[5]) {
+                                            const double *T, double acc[5], double cmp[5]) {
   const int half = lane >> 5, hl = lane & 31;
   for (int k = 0; k < n; k += 2) {
     const int j = k + half;
@@ -2336,9 +2349,10 @@ __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, cons
         const double2 *g = reinterpret_cast<const double2 *>(P.gq + 4 * (size_t)pos);
         double2 g0 = g[0], g1 = g[1];
         // the image's coordinates as ppiclF's ghost holds them, x_p + s
-        double dx = xv[0] - (sh[0] == 0.0 ? g0.x : g0.x + sh[0]);
-        double dy = xv[1] - (sh[1] == 0.0 ? g0.y : g0.y + sh[1]);
-        double dz = xv[2] - (sh[2] == 0.0 ? g1.x : g1.x + sh[2]);
+        // (x_p + 0 is x_p, or +0 for -0, which gives the same difference)
+        double dx = xv[0] - (g0.x + sh[0]);
+        double dy = xv[1] - (g0.y + sh[1]);
+        double dz = xv[2] - (g1.x + sh[2]);
         rdist2 = dx * dx + dy * dy;
         rdist2 = rdist2 + dz * dz;
         take = rdist2 <= P.gD2chk;
@@ -2353,7 +2367,7 @@ __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, cons
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       if (qn >= 64) {
-        gauss_batch(P, Qp, Qr, 64, lane, acc, cmp);
+        gauss_batch(P, Qp, Qr, 64, lane, T, acc, cmp);
         // the remainder moves to the front
         int rest = qn - 64, mp = lane < rest ? Qp[64 + lane] : 0;
         double mr = lane < rest ? Qr[64 + lane] : 0.0;
@@ -2373,7 +2387,8 @@ __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, cons
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-__device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, int *E, int *C, int *Qp, double *Qr) {
+__device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, int *E, int *C, int *Qp, double *Qr,
+                                             const double *T) {
   const size_t nV = P.nV;
   const double xv[3] = {P.vX[v], P.vX[nV + v], P.vX[2 * nV + v]};
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, cmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -2411,8 +2426,11 @@ __device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, 
         if (r < nrec) {
           const double2 *cp = reinterpret_cast<const double2 *>(P.cbRec + 4 * (size_t)(q0 + r));
           double2 c01 = cp[0], c23 = cp[1];
-          double t0 = c01.x - (xv[0] - sh[0]), t1 = c01.y - (xv[1] - sh[1]), t2 = c23.x - (xv[2] - sh[2]);
-          if (t0 * t0 + t1 * t1 + t2 * t2 <= P.gReach2) {
+          // the element's box of particle positions against the cut-off
+          double t0 = fmax(fabs(c01.x - (xv[0] - sh[0])) - P.gHalf[0], 0.0);
+          double t1 = fmax(fabs(c01.y - (xv[1] - sh[1])) - P.gHalf[1], 0.0);
+          double t2 = fmax(fabs(c23.x - (xv[2] - sh[2])) - P.gHalf[2], 0.0);
+          if (t0 * t0 + t1 * t1 + t2 * t2 <= P.gCut2) {
             int e = (int)(unsigned)(__double_as_longlong(c23.y) & 0xffffffffll);
             first = P.off[e];
             cnt = P.off[e + 1] - first;
@@ -2422,7 +2440,7 @@ __device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, 
         const unsigned long long m = __ballot(take);
         const int nm = __popcll(m);
         if (nList + nm > GW_CAP) {
-          gauss_flush(P, E, C, nList, lane, xv, sh, Qp, Qr, qn, acc, cmp);
+          gauss_flush(P, E, C, nList, lane, xv, sh, Qp, Qr, qn, T, acc, cmp);
           nList = 0;
         }
         if (take) {
@@ -2435,8 +2453,8 @@ __device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, 
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
     }
-    if (nList) gauss_flush(P, E, C, nList, lane, xv, sh, Qp, Qr, qn, acc, cmp);
-    gauss_batch(P, Qp, Qr, qn, lane, acc, cmp);  // what is left of this image's queue
+    if (nList) gauss_flush(P, E, C, nList, lane, xv, sh, Qp, Qr, qn, T, acc, cmp);
+    gauss_batch(P, Qp, Qr, qn, lane, T, acc, cmp);  // what is left of this image's queue
     qn = 0;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -2462,7 +2480,8 @@ __device__ __forceinline__ void gauss_vertex(const PiclDev &P, int v, int lane, 
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_gauss_w(PiclDev P) {
+__global__ void __launch_bounds__(BLK, GW_OCC) k_gauss_w(PiclDev P) {
+  __shared__ double sT[32];
   __shared__ int sE[GW_WAVES][GW_CAP];
   __shared__ int sC[GW_WAVES][GW_CAP];
   __shared__ int sQp[GW_WAVES][128];
@@ -2470,6 +2489,8 @@ __global__ void __launch_bounds__(BLK) k_gauss_w(PiclDev P) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int *E = sE[w], *C = sC[w], *Qp = sQp[w];
   double *Qr = sQr[w];
+  if (threadIdx.x < 32) sT[threadIdx.x] = gx::kTab[threadIdx.x];
+  __syncthreads();
   const size_t nV = P.nV;
   // the listed vertices are split into 8 contiguous ranges, one per XCD
   // (hardware block b runs on XCD b % 8), walked by that XCD's waves
@@ -2479,7 +2500,7 @@ __global__ void __launch_bounds__(BLK) k_gauss_w(PiclDev P) {
   if (blockIdx.x >= (gridDim.x / 8) * 8) return;  // blocks beyond a multiple of 8 have no range
   for (int vt = lo + (int)(blockIdx.x / 8) * GW_WAVES + w; vt < hi; vt += wavesX) {
     const int v = P.gvList[vt];
-    gauss_vertex(P, v, lane, E, C, Qp, Qr);
+    gauss_vertex(P, v, lane, E, C, Qp, Qr, sT);
   }
 }
 
@@ -2824,8 +2845,8 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
                                        "(Schiller-Naumann)");
   if (pp->interpMode < 0 || pp->interpMode > 1) return set_err(h, RFLU_GPU_EINVAL, "interpMode must be 0 or 1");
   if (pp->projMode < 0 || pp->projMode > 1) return set_err(h, RFLU_GPU_EINVAL, "projMode must be 0 or 1");
-  if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0))
-    return set_err(h, RFLU_GPU_EINVAL, "the Gaussian filter needs filterWidth > 0 and 0 < filterAlpha < 1");
+  if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha >= 1.0e-300 && pp->filterAlpha < 1.0))
+    return set_err(h, RFLU_GPU_EINVAL, "the Gaussian filter needs filterWidth > 0 and 1e-300 <= filterAlpha < 1");
   if (pp->amFlag < 0 || pp->amFlag > 2)
     return set_err(h, RFLU_GPU_EINVAL, "amFlag must be 0 (off), 1 (Parmar) or 2 (Briney)");
   if (!(pp->neighborWidth >= 0.0)) return set_err(h, RFLU_GPU_EINVAL, "neighborWidth must be >= 0");
@@ -3123,6 +3144,13 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
       }
     P.gReach = (d2chk + hmax) * (1.0 + 1.0e-6);
     P.gReach2 = P.gReach * P.gReach;
+    for (int l = 0; l < 3; l++) {
+      double hl = 0.0;
+      for (int i = 0; i < N; i++)
+        for (int n = 0; n < 8; n++) hl = std::max(hl, std::fabs(X[24 * (size_t)i + 3 * n + l] - cen[4 * (size_t)i + l]));
+      P.gHalf[l] = hl * (1.0 + 1.0e-6);
+    }
+    P.gCut2 = (d2chk * (1.0 + 1.0e-6)) * (d2chk * (1.0 + 1.0e-6));
     P.gauss = 1;
     // ghost particles across periodic bounds (ppiclf_comm.f:1171-1185,
     // CheckPeriodicBC :2136-2206): images shifted by +-(hi - lo)

@@ -32,6 +32,8 @@
 #include <string>
 #include <vector>
 
+#include "gauss_exp.h"
+
 #include "../../include/rflu_gpu.h"
 #include "../../include/picl_gpu.h"
 #include "../../include/ppiclf_gpu.h"
