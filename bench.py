@@ -282,8 +282,9 @@ def run_batch(args):
         b = rflu.Batch([it[0] for it in items], **dict(items[0][1], fluxPrecision=prec))
         for k, (mesh, params, ic) in enumerate(items):
             b.set_state(k, ic(b.cofg(k)))
-        b.autotune(reps=3)
+        b.autotune(reps=3)  # gradient kernel and stage fusion, both bit-identical choices
         batches.append((key, b, items))
+    fused = [(key, b.fused) for key, b, _ in batches]
     cells = sum(b.nCells for _, b, _ in batches)
     setup_s = time.perf_counter() - t0
     print(f"# ensemble: {args.instances} instances, {cells} cells in {len(batches)} batches, setup {setup_s:.1f}s",
@@ -298,6 +299,7 @@ def run_batch(args):
     for _, b, _ in batches:
         b.wait()
     dt_s = time.perf_counter() - t0
+    batches_cfg = [dict(cell_type=k[0], gamma=k[1], instances=b.nInst, cells=b.nCells) for k, b, _ in batches]
     err = None
     if args.fp32:
         errs = []
@@ -311,7 +313,32 @@ def run_batch(args):
                 errs.append(max(np.linalg.norm(a[:, g] - r[:, g]) / np.linalg.norm(r[:, g])
                                 for g in ([0], [1, 2, 3], [4])))
             ref.close()
-        err = dict(max=float(np.max(errs)), median=float(np.median(errs)))
+        err = dict(max=float(np.max(errs)), median=float(np.median(errs)),
+                   within_1e6=f"{int(np.sum(np.asarray(errs) <= 1e-6))}/{len(errs)}")
+    # the plain FP64 path on the same ensemble: a sample of the instances, each
+    # a region of its own stepped alone (graph replays), no batching
+    plain = None
+    if args.plain_sample > 0:
+        for _, b, _ in batches:
+            b.close()
+        pick = ens[::max(1, len(ens) // args.plain_sample)][:args.plain_sample]
+        cu, secs, pc = 0.0, 0.0, 0
+        for key, mesh, params, ic in pick:
+            r = rflu.Region(mesh, **params)
+            r.set_state(ic(r.mesh_arrays()["cofg"]))
+            r.RFLU_TimeStepping(args.warmup, use_graph=True)
+            r.sync()
+            t1 = time.perf_counter()
+            r.RFLU_TimeStepping(args.steps, use_graph=True)
+            r.sync()
+            secs += time.perf_counter() - t1
+            cu += r.nCells * args.steps
+            pc += r.nCells
+            r.close()
+        pv = cu / secs / 1e6
+        plain = dict(value=round(pv, 2), unit="Mcell-updates/s", instances=len(pick), cells=pc,
+                     note="each sampled instance alone in the plain FP64 path (one region, graph replays), "
+                          "aggregate = sampled cell-updates / summed wall time")
     out = {
         "metric": "Mcell-updates/s/GPU & % HBM roofline",
         "value": round(cells * args.steps / dt_s / 1e6, 2),
@@ -322,11 +349,14 @@ def run_batch(args):
         "dtype": "f32-flux/f64-accumulate" if args.fp32 else "f64",
         "data": "synthetic (seeded meshes, shock initial states)",
         "config": {"workload": "batch_ensemble", "instances": args.instances, "cells": cells,
-                   "batches": [dict(cell_type=k[0], gamma=k[1], instances=b.nInst, cells=b.nCells)
-                               for k, b, _ in batches],
+                   "batches": batches_cfg,
                    "model": "RFLU 2nd-order AUSM+ + Venkat(sign-safe), per-instance dt", "parallelism": "single GPU"},
         "fp32_rel_l2_vs_fp64": err,
+        "fused_stages": [dict(cell_type=k[0], gamma=k[1], fused=f) for k, f in fused],
     }
+    if plain:
+        out["plain_fp64_path"] = plain
+        out["vs_plain_fp64_path"] = round(out["value"] / plain["value"], 3)
     print(json.dumps(out))
 
 
@@ -505,6 +535,8 @@ def main():
                     help="FP32 gradients and face fluxes, FP64 accumulation and update; also reports the "
                          "relative L2 error against the FP64 path after the same steps")
     ap.add_argument("--instances", type=int, default=256, help="batch_ensemble: number of instances")
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="batch_ensemble: instances run alone in the plain FP64 path for comparison (0: skip)")
     ap.add_argument("--requests", type=int, default=24, help="serve_trace: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve_trace: arrivals per second")
     ap.add_argument("--rates", default="", help="serve_trace: comma-separated arrival rates to sweep")

@@ -207,9 +207,21 @@ int rflu_batch_get_time(rflu_handle *h, int inst, double *currentTime, double *d
 
 /* Autotuner (no reference counterpart): times every gradient kernel that
  * gives the same bits on this handle's own mesh (reps launches each) and
- * keeps the fastest for later stages; the captured step graph is rebuilt.
- * Reports the choice and its time per launch. */
+ * keeps the fastest for later stages; then, on a region without particles,
+ * times the stage after the gradients as two kernels (face fluxes, then the
+ * gather and RK update) against the fused kernel (below) and keeps the
+ * faster (the state is saved and put back).  The captured step graph is
+ * rebuilt.  Reports the gradient choice and its time per launch. */
 int rflu_gpu_autotune(rflu_handle *h, int reps, int *gradVariant, double *msPerLaunch);
+
+/* Fused stages (no reference counterpart): on = 1 runs stages 2 and 3 of
+ * every rflu_gpu_step as the gradients plus one kernel in which each cell
+ * evaluates its own faces' fluxes (every interior face twice) and updates,
+ * so no per-face flux array passes through HBM.  Bit-identical to on = 0.
+ * Regions without particles only (RFLU_GPU_EINVAL otherwise); the
+ * routine-by-routine calls are unaffected. */
+int rflu_gpu_set_fusion(rflu_handle *h, int on);
+int rflu_gpu_get_fusion(const rflu_handle *h, int *on);
 
 /* Mean kernel duration (ms) per launch of the named kernel over the last
  * rflu_gpu_step call with profiling enabled; for measurement only. */

@@ -23,7 +23,8 @@ RFLU_SYMBOLS = [
     "rflu_gpu_sync", "rflu_gpu_set_profiling", "rflu_gpu_kernel_time", "rflu_gpu_bandwidth_probe",
     "rflu_gpu_step_async", "rflu_gpu_query", "rflu_gpu_wait", "rflu_gpu_device_bytes", "rflu_gpu_mem_info",
     "rflu_batch_create", "rflu_batch_sizes", "rflu_batch_set_state", "rflu_batch_get_state",
-    "rflu_batch_get_time", "rflu_gpu_autotune", "rflu_gpu_set_time",
+    "rflu_batch_get_time", "rflu_gpu_autotune", "rflu_gpu_set_time", "rflu_gpu_set_fusion",
+    "rflu_gpu_get_fusion",
 ]
 
 # Every symbol include/picl_gpu.h declares.

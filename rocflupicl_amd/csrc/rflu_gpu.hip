@@ -83,6 +83,9 @@ struct Dev {
   float *gradF = nullptr, *fluxF = nullptr;
   // WENO-XYZ reconstruction (reconst 2): the unlimited LSQ gradients
   double *gradRaw = nullptr;
+  // fused stages (k_fu): the records written by stage 2 and read by stage 3
+  // (cells read their neighbours' records while others update theirs)
+  double *recAlt = nullptr;
   // batched ensemble (nInst > 0): instance k owns device cells
   // [iOff[k], iOff[k+1]); cInst gives each cell's instance; iScal holds each
   // instance's scal[] slots (4 per instance)
@@ -111,6 +114,7 @@ struct rflu_handle {
   int device = 0;  // the device current at creation; the handle's memory and stream live there
   std::string err;
   hipGraphExec_t graph = nullptr;
+  int fuse = 0;         // stages 2 and 3 of a step as one fused kernel after the gradients (k_fu)
   int gradVariant = 4;  // 4: LDS tiles (default when they fit), 2: one thread per cell, 0: plain, 1/3: cooperative
   size_t tileLds = 0;   // LDS bytes per gradient tile (0: tiles not built)
   int tileCells = BLK;  // cells per gradient tile (= threads per tile workgroup)
@@ -1685,15 +1689,17 @@ __device__ __forceinline__ void rind_totang(const Gas &G, int opt, const double 
   for (int k = 0; k < 6; k++) b[k] = pc[PC_TT_SUP + k];
 }
 
-// F32 = 1: interior second-order faces in FP32 from the FP32 gradients;
-// boundary faces stay FP64 (their gradients widened from FP32).
+// The flux of face slot s (interior faces, then boundary faces by patch) as
+// the face loop of RFLU_ComputeFluxInv and the patch loops compute it; srad
+// (optional) receives the face's spectral radius for RFLU_TimeStepInviscid.
+// F32 = 1: interior second-order faces in FP32 from the FP32 gradients
+// (returned widened, exactly); boundary faces stay FP64 (their gradients
+// widened from FP32).
 template <int F32>
-__global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
-  int s = xcd_block() * BLK + threadIdx.x;
-  if (s >= d.S) return;
+__device__ __forceinline__ void face_flux(const Dev &d, const Gas &G, const FluxPar &P, int s, double f[5],
+                                          double *srad) {
   const double fs = 0.0, ksg = 0.0;
   double nx = d.fnx[s], ny = d.fny[s], nz = d.fnz[s], nm = d.fnm[s];
-  double f[5];
   if (s < d.nF) {
     int c1 = d.f2c[2 * s], c2 = d.f2c[2 * s + 1];
     double L[8], R[8];
@@ -1707,7 +1713,7 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
       rpL = L[0] / vl * vl;
       rpR = R[0] / vr * vr;
     }
-    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, R, nx, ny, nz, nm, phL, phR);
+    if (srad) *srad = face_srad(G, P.order, L, R, nx, ny, nz, nm, phL, phR);
     if (P.order == 1) {
       double rl = L[0], irl = 1.0 / rl, ul = L[1] * irl, vl = L[2] * irl, wl = L[3] * irl;
       double pl = L[5], tl = L[6], al = L[7];
@@ -1758,9 +1764,8 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
         hllc_flux_f(g - 1.0f, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, el, al, ql, Hl, B.r, B.u, B.v, B.w, B.p, er,
                     ar, qr, Hr, ff);
       }
-      float *of = d.fluxF + 5 * (size_t)s;
 #pragma unroll
-      for (int k = 0; k < 5; k++) of[k] = ff[k];
+      for (int k = 0; k < 5; k++) f[k] = (double)ff[k];
       return;
     } else {
       double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
@@ -1810,7 +1815,7 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
     double L[8];
     load_rec(d.rec, c1, L);
     double phL = d.phi ? d.phi[c1] : 0.0;
-    if (P.doSrad) d.srad[s] = face_srad(G, P.order, L, nullptr, nx, ny, nz, nm, phL);
+    if (srad) *srad = face_srad(G, P.order, L, nullptr, nx, ny, nz, nm, phL);
     double rl, rul, rvl, rwl, rel, pl;
     if (P.patchOrder == 1) {
       if (P.order == 1) {
@@ -1914,6 +1919,20 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
       f[4] = 0.5 * (ql * (rel + pl) + pl * fs + qr * (rer + pr) + pr * fs) * nm;
     }
   }
+}
+
+template <int F32>
+__global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
+  int s = xcd_block() * BLK + threadIdx.x;
+  if (s >= d.S) return;
+  double f[5];
+  face_flux<F32>(d, G, P, s, f, P.doSrad ? d.srad + s : nullptr);
+  if (F32 && P.order == 2 && s < d.nF) {  // interior FP32 fluxes stay FP32 (the widening is exact)
+    float *of = d.fluxF + 5 * (size_t)s;
+#pragma unroll
+    for (int k = 0; k < 5; k++) of[k] = (float)f[k];
+    return;
+  }
   double *o = d.flux + 5 * (size_t)s;
 #pragma unroll
   for (int k = 0; k < 5; k++) o[k] = f[k];
@@ -1968,24 +1987,11 @@ __device__ __forceinline__ void gravity(const Gas &G, int order, const double r[
   rhs[4] = rhs[4] - vol * (G.accel[0] * m1 + G.accel[1] * m2 + G.accel[2] * m3);
 }
 
-template <int F32>
-__global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
-  int i = xcd_block() * BLK + threadIdx.x;
-  if (i >= d.nC) return;
-  int N = d.nC;
-  double rhs[5];
-  double vol = d.vol[i];
-  if (P.fused) {
-    gather_rhs<F32>(d, i, rhs);
-    if (G.accelOn) {
-      double r[8];
-      load_rec(d.rec, i, r);
-      gravity(G, P.order, r, vol, rhs);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 5; k++) rhs[k] = d.rhs[k * N + i];
-  }
+// RKUpdateMP, RFLU_SetDependentVars and AfterUpdateMP for cell i, given its
+// residual.
+__device__ __forceinline__ void rk_update_cell(const Dev &d, const Gas &G, const UpdPar &P, int i, double vol,
+                                               double rhs[5], double *recOut) {
+  const int N = d.nC;
   // libfloflu/RkUpdateGeneric.F90:285-314 (Wray RK3, static grid)
   double fac = P.ark * (d.nInst ? d.iScal[4 * (size_t)d.cInst[i] + S_DTMIN] : d.scal[S_DTMIN]);
   double adtv = fac / vol;
@@ -2040,8 +2046,62 @@ __global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
   for (int k = 0; k < 5; k++) d.cv[k * N + i] = cv[k];
   double r[8];
   make_rec(P.order, cv[0], cv[1], cv[2], cv[3], cv[4], dv, d.cofg[i], d.cofg[N + i], d.cofg[2 * N + i], r);
-  store_rec(d.rec, i, r);
+  store_rec(recOut, i, r);
   if (d.rhog) d.rhog[i] = cv[0] / (1.0 - d.phi[i]);
+}
+
+template <int F32>
+__global__ void __launch_bounds__(BLK) k_update(Dev d, Gas G, UpdPar P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  int N = d.nC;
+  double rhs[5];
+  double vol = d.vol[i];
+  if (P.fused) {
+    gather_rhs<F32>(d, i, rhs);
+    if (G.accelOn) {
+      double r[8];
+      load_rec(d.rec, i, r);
+      gravity(G, P.order, r, vol, rhs);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; k++) rhs[k] = d.rhs[k * N + i];
+  }
+  rk_update_cell(d, G, P, i, vol, rhs, d.rec);
+}
+
+// One fused stage after the gradients (ConvectiveFluxesMP + SourceTermsMP +
+// RKUpdateMP for a stage without the dt evaluation, no particles): each cell
+// evaluates its own faces' fluxes in its slot order and updates at once, so
+// no per-face flux array passes through HBM.  Every interior face is
+// evaluated twice (once per cell) by the same function from the same inputs,
+// so the residual's bits are those of k_flux followed by k_update.  The
+// records are read from rec and the updated ones written to recAlt.
+template <int F32>
+__global__ void __launch_bounds__(BLK) k_fu(Dev d, Gas G, FluxPar FP, UpdPar P) {
+  int i = xcd_block() * BLK + threadIdx.x;
+  if (i >= d.nC) return;
+  double rhs[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) rhs[k] = -0.0;
+  for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
+    int code = d.cfIdx[e];
+    double f[5];
+    face_flux<F32>(d, G, FP, code >> 1, f, nullptr);
+    if (code & 1) {
+      rhs[0] = rhs[0] - f[0]; rhs[1] = rhs[1] - f[1]; rhs[2] = rhs[2] - f[2]; rhs[3] = rhs[3] - f[3]; rhs[4] = rhs[4] - f[4];
+    } else {
+      rhs[0] = rhs[0] + f[0]; rhs[1] = rhs[1] + f[1]; rhs[2] = rhs[2] + f[2]; rhs[3] = rhs[3] + f[3]; rhs[4] = rhs[4] + f[4];
+    }
+  }
+  double vol = d.vol[i];
+  if (G.accelOn) {
+    double r[8];
+    load_rec(d.rec, i, r);
+    gravity(G, P.order, r, vol, rhs);
+  }
+  rk_update_cell(d, G, P, i, vol, rhs, d.recAlt);
 }
 
 // ConvectiveFluxesMP / SourceTermsMP as separate steps (rhs kept in HBM).
@@ -2343,6 +2403,16 @@ int launch_update(rflu_handle *h, int istage, int fused) {
   return 0;
 }
 
+int launch_fu(rflu_handle *h, int istage) {
+  ProfScope ps(h, "k_fu");
+  FluxPar FP{h->prm.spaceOrder, h->prm.spaceDiscr, h->prm.patchOrder, 0};
+  if (h->prm.fluxPrecision == 1)
+    hipLaunchKernelGGL((k_fu<1>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, FP, updpar(h, istage, 1));
+  else
+    hipLaunchKernelGGL((k_fu<0>), dim3(h->d.nBlk), dim3(BLK), 0, h->stream, h->d, h->gas, FP, updpar(h, istage, 1));
+  return 0;
+}
+
 int picl_stage(rflu_handle *h, int istage);  // picl_gpu.inl
 int launch_rhog(rflu_handle *h);
 void picl_free(PiclState *s);
@@ -2373,9 +2443,26 @@ void enqueue_stage(rflu_handle *h, int st, bool withDt) {
   launch_update(h, st, 1);
 }
 
+// Stage 2 or 3 fused (h->fuse): the gradients, then k_fu.  Stage 2 reads the
+// records in rec and writes recAlt, stage 3 the other way round, so a step
+// ends with the records where it found them.
+void enqueue_fused_stage(rflu_handle *h, int st) {
+  double *a = h->d.rec, *b = h->d.recAlt;
+  if (st == 3) { h->d.rec = b; h->d.recAlt = a; }
+  launch_grad(h);
+  launch_fu(h, st);
+  h->d.rec = a;
+  h->d.recAlt = b;
+}
+
 // One RFLU_TimeStepping iteration.
 int enqueue_step(rflu_handle *h) {
-  for (int st = 1; st <= 3; st++) enqueue_stage(h, st, st == 1);
+  for (int st = 1; st <= 3; st++) {
+    if (h->fuse && !h->picl && st >= 2)
+      enqueue_fused_stage(h, st);
+    else
+      enqueue_stage(h, st, st == 1);
+  }
   hipLaunchKernelGGL(k_advance_time, dim3(h->d.nInst ? (h->d.nInst + 255) / 256 : 1), dim3(h->d.nInst ? 256 : 64), 0,
                      h->stream, h->d);
   return 0;
@@ -2840,7 +2927,7 @@ int rflu_gpu_destroy(rflu_handle *h) {
   for (void *p : ptrs) if (p) hipFree(p);
   for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot,
                   (void *)d.gradF, (void *)d.fluxF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal,
-                  (void *)d.gradRaw, (void *)d.cfSlot})
+                  (void *)d.gradRaw, (void *)d.cfSlot, (void *)d.recAlt})
     if (p) hipFree(p);
   picl_free(h->picl);
   if (h->graph) hipGraphExecDestroy(h->graph);
@@ -3156,14 +3243,16 @@ int rflu_gpu_autotune(rflu_handle *h, int reps, int *variant, double *msPerLaunc
   if (!h || reps < 1) return RFLU_GPU_EINVAL;
   if (variant) *variant = h->gradVariant;
   if (msPerLaunch) *msPerLaunch = 0.0;
-  if (h->prm.spaceOrder < 2 || h->prm.fluxPrecision == 1) return 0;  // a single gradient kernel
   // gradient kernels that give the same bits: LDS tiles (4), one thread per
   // cell at 1, 4, 5, 6 waves per SIMD (2, 24, 25, 26), cooperative (3);
-  // with particles the tile kernel (4) or the per-cell one (2)
+  // with particles the tile kernel (4) or the per-cell one (2); FP32 mode
+  // and first order have a single gradient kernel (or none)
   std::vector<int> cand;
-  if (h->tileLds) cand.push_back(4);
-  cand.push_back(2);
-  if (!h->picl) { cand.push_back(24); cand.push_back(25); cand.push_back(26); cand.push_back(3); }
+  if (h->prm.spaceOrder >= 2 && h->prm.fluxPrecision == 0) {
+    if (h->tileLds) cand.push_back(4);
+    cand.push_back(2);
+    if (!h->picl) { cand.push_back(24); cand.push_back(25); cand.push_back(26); cand.push_back(3); }
+  }
   bool prof = h->profiling;
   h->profiling = false;
   hipEvent_t a, b;
@@ -3183,14 +3272,74 @@ int rflu_gpu_autotune(rflu_handle *h, int reps, int *variant, double *msPerLaunc
     ms /= (float)reps;
     if (ms < bestMs) { bestMs = ms; best = c; }
   }
+  HIPCHK(h, hipGetLastError());
+  h->gradVariant = best;
+  // the stage after the gradients: k_flux + k_update against k_fu (same
+  // bits), timed as stage 2 on this mesh; the state is saved first and
+  // put back afterwards
+  if (!h->picl) {
+    int rc = 0;
+    const size_t nC = h->d.nC;
+    double *save = nullptr;
+    int errSave = 0;
+    if (!h->d.recAlt) rc = dalloc(h, &h->d.recAlt, 8 * nC);
+    if (!rc) rc = dalloc(h, &save, 18 * nC);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(save, h->d.cv, 5 * nC * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(save + 5 * nC, h->d.rhsSum, 5 * nC * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(save + 10 * nC, h->d.rec, 8 * nC * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&errSave, h->d.err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float ms2[2] = {0.0f, 0.0f};
+    for (int f = 0; f < 2; f++) {
+      for (int r = 0; r <= reps; r++) {
+        if (r == 1) HIPCHK(h, hipEventRecord(a, h->stream));
+        if (f) {
+          launch_fu(h, 2);
+        } else {
+          launch_flux(h, 0);
+          launch_update(h, 2, 1);
+        }
+      }
+      HIPCHK(h, hipEventRecord(b, h->stream));
+      HIPCHK(h, hipEventSynchronize(b));
+      HIPCHK(h, hipEventElapsedTime(&ms2[f], a, b));
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d.cv, save, 5 * nC * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.rhsSum, save + 5 * nC, 5 * nC * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.rec, save + 10 * nC, 8 * nC * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.err, &errSave, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipFree(save));
+    h->devBytes -= 18 * nC * sizeof(double);
+    h->fuse = ms2[1] < ms2[0] ? 1 : 0;
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   HIPCHK(h, hipGetLastError());
-  h->gradVariant = best;
   h->profiling = prof;
   if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }  // re-captured with the choice
   if (variant) *variant = best;
-  if (msPerLaunch) *msPerLaunch = bestMs;
+  if (msPerLaunch) *msPerLaunch = cand.empty() ? 0.0 : bestMs;
+  return 0;
+}
+
+int rflu_gpu_set_fusion(rflu_handle *h, int on) {
+  API_LOCK;
+  if (!h || (on != 0 && on != 1)) return RFLU_GPU_EINVAL;
+  if (on && h->picl) return set_err(h, RFLU_GPU_EINVAL, "fused stages need a region without particles");
+  if (on && !h->d.recAlt) {
+    int rc = dalloc(h, &h->d.recAlt, 8 * (size_t)h->d.nC);
+    if (rc) return rc;
+  }
+  h->fuse = on;
+  if (h->graph) { (void)hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+  return 0;
+}
+
+int rflu_gpu_get_fusion(const rflu_handle *h, int *on) {
+  if (!h || !on) return RFLU_GPU_EINVAL;
+  *on = h->fuse;
   return 0;
 }
 

@@ -324,10 +324,22 @@ class Region:
 
     def autotune(self, reps=5):
         """Pick the fastest of the bit-identical gradient kernels on this
-        mesh; returns (variant, ms per launch)."""
+        mesh, and (without particles) whether stages 2 and 3 run fused;
+        returns (variant, ms per launch)."""
         v, ms = C.c_int(), C.c_double()
         self._chk(self._L.rflu_gpu_autotune(self.h, int(reps), C.byref(v), C.byref(ms)))
         return v.value, ms.value
+
+    @property
+    def fused(self):
+        """Stages 2 and 3 of each step as one fused kernel after the gradients."""
+        on = C.c_int()
+        self._chk(self._L.rflu_gpu_get_fusion(self.h, C.byref(on)))
+        return bool(on.value)
+
+    @fused.setter
+    def fused(self, on):
+        self._chk(self._L.rflu_gpu_set_fusion(self.h, 1 if on else 0))
 
     def device_bytes(self):
         b = C.c_size_t()
