@@ -423,7 +423,7 @@ def run_serve(args):
 
 
 KERNELS = ("k_dtmin", "k_grad", "k_rhog", "k_flux", "k_gather", "k_update", "k_fb_grad", "k_picl_fields",
-           "k_contacts", "k_interp", "k_pairs", "k_particles", "k_scan", "k_fill", "k_deposit", "k_gauss",
+           "k_contacts", "k_interp", "k_pairs", "k_particles", "k_scan", "k_fill", "k_deposit", "k_gsel", "k_gauss",
            "k_permute", "k_feedback")
 
 

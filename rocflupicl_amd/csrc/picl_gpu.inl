@@ -2676,11 +2676,14 @@ void picl_sort_and_deposit(rflu_handle *h, int posXo) {
       hipLaunchKernelGGL(k_deposit_box, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
   }
   if (P.gauss) {
-    ProfScope ps(h, "k_gauss");
-    hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nCB)), dim3(BLK), 0, h->stream, P.binCnt, P.nCB);
-    hipLaunchKernelGGL(k_gbox_reset, dim3(1), dim3(64), 0, h->stream, P);
-    hipLaunchKernelGGL(k_gcount, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
-    hipLaunchKernelGGL(k_gvsel, dim3(nblk(P.nV)), dim3(BLK), 0, h->stream, P);
+    {
+      ProfScope ps(h, "k_gsel");  // the bins holding particles and the vertices within reach
+      hipLaunchKernelGGL(k_zero_int, dim3(nblk(P.nCB)), dim3(BLK), 0, h->stream, P.binCnt, P.nCB);
+      hipLaunchKernelGGL(k_gbox_reset, dim3(1), dim3(64), 0, h->stream, P);
+      hipLaunchKernelGGL(k_gcount, dim3(nblk(P.nC)), dim3(BLK), 0, h->stream, P);
+      hipLaunchKernelGGL(k_gvsel, dim3(nblk(P.nV)), dim3(BLK), 0, h->stream, P);
+    }
+    ProfScope ps(h, "k_gauss");  // the deposit itself (k_gauss_w)
     hipLaunchKernelGGL(k_gauss_w, dim3(GW_BLOCKS), dim3(BLK), 0, h->stream, P);
   }
   ProfScope ps(h, "k_permute");
