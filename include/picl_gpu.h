@@ -114,6 +114,23 @@ typedef struct picl_params {
   int qsFluctAdapt;
   double fluctHalfWidth;
   long long fluctSeed;
+  /* Angular periodicity about the z axis (ANGULARPERIODIC 1; ppiclF's
+   * ang_per_* arguments of ppiclf_solve_Initialize, ppiclf_solve.f:1-104,
+   * and ppiclf_solve_InitAngularPeriodic, :1172-1230): a wedge from the polar
+   * angle angXAngle to angXAngle + angAngle (radians, as ppiclF receives
+   * them; the deck's degrees times acos(-1)/180, PICL_TEMP_InitSolver.F90:
+   * 271-273), an opening of less than 90, of 90 or of 180 degrees, radii
+   * angRin < angRout.  After each stage's update a particle whose polar angle
+   * left the wedge has its position and velocity rotated back
+   * (ppiclf_solve_InvokeAngularPeriodic with the reference's rotation
+   * matrix, :2852-2881); with periodicZ the z axis wraps and a particle whose
+   * x lies outside [angRin, angRout] ([-angRout, angRout] for the half
+   * cylinder) is removed at the next stage (InvokeLinearPeriodic's second
+   * case, :1362-1394).  Excludes periodicX and periodicY; needs the box
+   * projection and no pair models (ppiclF's rotated ghost particles,
+   * AngularCreateGhost, are not built). */
+  int angularPeriodic;
+  double angAngle, angXAngle, angRin, angRout;
 } picl_params;
 
 /* PICL_TEMP_InitSolver (rocpicl/PICL_TEMP_InitSolver.F90:538-835):

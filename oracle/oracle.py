@@ -36,7 +36,9 @@ class OrcPiclParams(C.Structure):
                 ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
                 ("viscousUnsteady", C.c_int), ("collisionalFlag", C.c_int), ("ksp", C.c_double),
                 ("erest", C.c_double), ("neighborWidth", C.c_double), ("qsFluct", C.c_int),
-                ("qsFluctAdapt", C.c_int), ("fluctHalfWidth", C.c_double), ("fluctSeed", C.c_longlong)]
+                ("qsFluctAdapt", C.c_int), ("fluctHalfWidth", C.c_double), ("fluctSeed", C.c_longlong),
+                ("angularPeriodic", C.c_int), ("angAngle", C.c_double), ("angXAngle", C.c_double),
+                ("angRin", C.c_double), ("angRout", C.c_double)]
 
 
 PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPart=840.0, qsFlag=1,

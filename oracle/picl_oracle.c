@@ -71,6 +71,12 @@ struct orc_picl {
   double fluctW;   /* ppiclf_d2chk(2): the fluctuation box filter's half-width */
   int nCalls;      /* SetYdot calls so far: the counter of the uniform draws */
   int curStage;    /* the RK stage of the current SetYdot call */
+  /* angular periodicity (ang_per_flag 1): ppiclF's case, the wedge's polar
+   * angles, the sign switch 0.5 (xangle + angle), ppiclf_xdrange(:,1), and
+   * the rotation matrices of the local angles +angle, -angle, 0 */
+  int ang, angCase;
+  double angX, angA, angXd[2], angR[3][3][3];
+  char *mark;      /* removed by the x range test: gone from the next stage */
 };
 
 /* The rotational fields (rprop JXVOR..JZVOR, JSDOX..JSDOZ, 0-based) */
@@ -79,7 +85,7 @@ static const int ROT_RPROP[6] = {R_JXVOR, R_JXVOR + 1, R_JXVOR + 2, R_JSDOX, R_J
 void orc__picl_free(struct orc_picl *p) {
   if (!p) return;
   void *ptrs[] = {p->y, p->y1, p->ydot, p->ydotc, p->rprop, p->elem, p->lost, p->xm1b,
-                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld, p->hist, p->fldRot};
+                  p->centeri, p->bbox, p->rexp, p->proFld, p->fld, p->hist, p->fldRot, p->mark};
   for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
   free(p);
 }
@@ -338,6 +344,11 @@ static void init_solve(struct orc_picl *p, int interp) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int ip = 0; ip < p->n; ip++) {
     if (p->lost[ip]) continue;
+    if (p->mark && p->mark[ip]) { /* RemoveParticle: marked in the last stage (iprop 3), gone now */
+      p->lost[ip] = 1;
+      p->mark[ip] = 0;
+      continue;
+    }
     int e = locate(p, p->y + LRS * (size_t)ip);
     p->elem[ip] = e;
     if (e < 0) p->lost[ip] = 1;
@@ -1617,6 +1628,91 @@ static void linear_periodic(const struct orc_picl *p, double *y) {
   }
 }
 
+/* ppiclf_solve_Initialize's angular case (ppiclf_solve.f:60-104, with the
+ * REAL*4 pi = ACOS(-1.0): angled = angle * 180 / pi; below 90 case 1, NINT 90
+ * case 2, NINT 180 case 3, anything else stops in InitAngularPeriodic) and
+ * ppiclf_solve_InitAngularPeriodic (:1172-1230: rin < rout; xdrange(:,1) =
+ * (rin, rout), for the half cylinder (-rout, rout)).  The rotation matrices of
+ * ppiclf_solve_InvokeAngularPeriodic (:1396-1500) for the local angles
+ * +angle, -angle and 0, axis e_z, written as the reference writes them. */
+static int angular_init(struct orc_picl *p, const orc_picl_params *pp) {
+  double pi = (double)acosf(-1.0f);
+  double angled = pp->angAngle * 180.0 / pi;
+  int cs = 0;
+  if (angled < 90.0) cs = 1;
+  if (lround(angled) == 90) cs = 2;
+  if (lround(angled) == 180) cs = 3;
+  if (cs == 0 || !(pp->angAngle > 0.0) || !(pp->angRin < pp->angRout)) return -1;
+  p->ang = 1;
+  p->angCase = cs;
+  p->angX = pp->angXAngle;
+  p->angA = pp->angAngle;
+  p->angXd[0] = cs == 3 ? -1.0 * pp->angRout : pp->angRin;
+  p->angXd[1] = pp->angRout;
+  const double ex = 0.0, ey = 0.0, ez = 1.0;
+  for (int k = 0; k < 3; k++) {
+    double local_angle = k == 0 ? pp->angAngle : (k == 1 ? -1.0 * pp->angAngle : 0.0);
+    double ct = cos(local_angle), st = sin(local_angle);
+    double (*R)[3] = p->angR[k];
+    R[0][0] = ct + (1.0 - ct) * ex * ex;
+    R[0][1] = (1.0 - ct) * ex * ey - st * ez;
+    R[0][2] = (1.0 - ct) * ex * ez + st * ey;
+    R[1][0] = (1.0 - ct) * ey * ex + st * ez;
+    R[1][1] = ct + (1.0 - ct) * ey * ey;
+    R[1][2] = (1.0 - ct) * ey * ez - st * ex;
+    R[2][0] = (1.0 - ct) * ez * ex - st * ey;
+    R[2][1] = (1.0 - ct) * ez * ey + st * ex;
+    R[2][2] = ct + (1.0 - ct) * ez * ez;
+  }
+  return 0;
+}
+
+/* MATMUL(rotmat, v) as gfortran forms it: each row summed over the columns
+ * in order from zero */
+static void rotate3(double R[3][3], double *v) {
+  double o[3];
+  for (int i = 0; i < 3; i++) {
+    double sum = 0.0;
+    for (int j = 0; j < 3; j++) sum = sum + R[i][j] * v[j];
+    o[i] = sum;
+  }
+  for (int i = 0; i < 3; i++) v[i] = o[i];
+}
+
+/* ppiclf_solve_RemoveParticle for one particle after the update (:2826-2962):
+ * with angular periodicity, a polar angle per_alpha = atan2(y, x) outside
+ * [xangle, xangle + angle] calls InvokeAngularPeriodic (register 1): local
+ * angle +angle, -angle when per_alpha > 0.5 (xangle + angle), 0 for the half
+ * cylinder when per_alpha < xangle; position and velocity rotated.  Then
+ * InvokeLinearPeriodic (:1319-1394): without angular periodicity each
+ * periodic axis wraps; with it and periodicZ, an x outside xdrange(:,1)
+ * marks the particle for removal (MarkForRemoval, iprop 3), otherwise z
+ * wraps. */
+static void remove_particle(struct orc_picl *p, int i, double *y) {
+  if (!p->ang) {
+    linear_periodic(p, y);
+    return;
+  }
+  double per_alpha = atan2(y[1], y[0]);
+  if (per_alpha < p->angX || per_alpha > p->angX + p->angA) {
+    int k = 0;
+    if (per_alpha > 0.5 * (p->angX + p->angA)) k = 1;
+    if (p->angCase == 3 && per_alpha < p->angX) k = 2;
+    rotate3(p->angR[k], y);
+    rotate3(p->angR[k], y + 3);
+  }
+  if (p->per[2]) {
+    if (y[0] < p->angXd[0] || y[0] > p->angXd[1]) {
+      p->mark[i] = 1;
+      return;
+    }
+    if (y[2] < p->perLo[2])
+      y[2] = p->perHi[2] - fabs(p->perLo[2] - y[2]);
+    else if (y[2] > p->perHi[2])
+      y[2] = p->perLo[2] + fabs(y[2] - p->perHi[2]);
+  }
+}
+
 /* ppiclf_solve_IntegrateRK3s_Rocflu + SetRK3Coeff (imethod 2):
  * ppiclf_solve.f:1845-1936, 2774-2793 */
 static void integrate_particle(struct orc_picl *p, int istage, double dt) {
@@ -1638,7 +1734,7 @@ static void integrate_particle(struct orc_picl *p, int istage, double dt) {
     double *y = p->y + LRS * (size_t)i, *y1 = p->y1 + LRS * (size_t)i, *yd = p->ydot + LRS * (size_t)i;
     for (int j = 0; j < LRS; j++) y[j] = -(k[0] * y1[j]) + k[1] * y[j] + k[2] * yd[j];
     for (int j = 0; j < LRS; j++) y1[j] = yd[j];
-    linear_periodic(p, y);
+    remove_particle(p, i, y);
   }
 }
 
@@ -1795,8 +1891,21 @@ int orc_picl_init(orc_region *r, const orc_picl_params *pp, int nPart, const dou
   for (int d = 0; d < 3; d++) /* ppiclf_solve_InitPeriodicX/Y/Z: xl < xr */
     if (perFlag[d] < 0 || perFlag[d] > 1 || (perFlag[d] && !(vlo[d] < vhi[d]))) return -1;
   if (pp->projMode == 1 && !(pp->filterWidth > 0.0 && pp->filterAlpha > 0.0 && pp->filterAlpha < 1.0)) return -1;
+  /* angular periodicity: not with periodicX/Y (PICL_TEMP_InitSolver.F90:
+   * 700-705); the box projection and no pair models (no rotated ghosts) */
+  if (pp->angularPeriodic < 0 || pp->angularPeriodic > 1) return -1;
+  if (pp->angularPeriodic && (perFlag[0] || perFlag[1] || pp->projMode != 0 || pp->collisionalFlag ||
+                              pp->amFlag == 2 || pp->qsFluct))
+    return -1;
   struct orc_picl *p = calloc(1, sizeof *p);
   p->pp = *pp;
+  if (pp->angularPeriodic) {
+    if (angular_init(p, pp)) {
+      free(p);
+      return -1;
+    }
+    p->mark = calloc((size_t)nPart, 1);
+  }
   /* PICL_TEMP_InitSolver.F90:234-262: the range of a periodic axis is the
    * mesh's vertex minimum and maximum on it */
   for (int d = 0; d < 3; d++) {
@@ -1901,7 +2010,7 @@ void orc_picl_get_particles(const orc_region *r, double *y, double *y1, double *
   if (ydot) memcpy(ydot, p->ydot, b);
   if (ydotc) memcpy(ydotc, p->ydotc, b);
   if (elem)
-    for (int i = 0; i < p->n; i++) elem[i] = p->lost[i] ? -1 : p->elem[i];
+    for (int i = 0; i < p->n; i++) elem[i] = (p->lost[i] || (p->mark && p->mark[i])) ? -1 : p->elem[i];
 }
 void orc_picl_get_rprop(const orc_region *r, double *rprop) {
   memcpy(rprop, r->picl->rprop, sizeof(double) * LRP * (size_t)r->picl->n);

@@ -52,6 +52,9 @@ typedef struct orc_picl_params {
                             set-up leaves: the last element's largest extent (box projection) or
                             the Gaussian filter's cut-off radius */
   long long fluctSeed;  /* seed of the uniform draws that stand in for RANDOM_NUMBER */
+  int angularPeriodic;  /* ANGULARPERIODIC: particles wrap across a wedge about the z axis */
+  double angAngle, angXAngle; /* the wedge's opening and lower face angle (radians) */
+  double angRin, angRout;     /* its radii (the removal range with periodicZ) */
 } orc_picl_params;
 
 /* Interpolated fields, in slot order, with their ppiclF rprop index

@@ -74,7 +74,9 @@ class PiclParams(C.Structure):
                 ("periodicX", C.c_int), ("periodicY", C.c_int), ("periodicZ", C.c_int),
                 ("viscousUnsteady", C.c_int), ("collisionalFlag", C.c_int), ("ksp", C.c_double),
                 ("erest", C.c_double), ("neighborWidth", C.c_double), ("qsFluct", C.c_int),
-                ("qsFluctAdapt", C.c_int), ("fluctHalfWidth", C.c_double), ("fluctSeed", C.c_longlong)]
+                ("qsFluctAdapt", C.c_int), ("fluctHalfWidth", C.c_double), ("fluctSeed", C.c_longlong),
+                ("angularPeriodic", C.c_int), ("angAngle", C.c_double), ("angXAngle", C.c_double),
+                ("angRin", C.c_double), ("angRout", C.c_double)]
 
 
 _lib = None

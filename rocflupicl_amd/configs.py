@@ -166,6 +166,52 @@ def curtain(nx=400, ny=80, nz=80, h=0.0005, npart=4_000_000, mach=2.0, x_shock=N
     return mesh, params, ic, (y, pconst), dict(PICL_AIR)
 
 
+def wedge(nr=6, nt=8, nz=4, rin=0.02, rout=0.05, angle_deg=90.0, xangle_deg=0.0, lz=0.01, npart=240,
+          swirl=200.0, near=5e-7, seed=8, periodicZ=0, discr=4, **kw):
+    """Particles in a wedge about the z axis (ANGULARPERIODIC): nr x nt x nz
+    hexes over radii rin..rout, polar angles xangle..xangle + angle (given in
+    degrees, converted as PICL_TEMP_InitSolver.F90:271-273 does), z in [0, lz],
+    slip walls, quiescent air.  Half the particles sit within `near` of the
+    lower or the upper wedge face and swirl across it at `swirl` m/s (so the
+    first stage carries them out), the others anywhere inside, some moving
+    in z.  Returns (mesh, params, ic, particles, picl_params) like curtain."""
+    pi = float(np.arccos(-1.0))
+    angle, xangle = angle_deg * pi / 180.0, xangle_deg * pi / 180.0
+    mesh = mg.wedge_hex(nr, nt, nz, rin, rout, angle, xangle, lz)
+    params = dict(spaceDiscr=discr, spaceOrder=2, reconst=11, cfl=0.5, dtImposed=1.0e-8,
+                  dtMinLimit=0.0, limiterGuard=1, **AIR)
+    params.update(kw)
+
+    def ic(cofg):
+        rho = np.full(len(cofg), 1.2)
+        z = np.zeros_like(rho)
+        return mg.prim_to_cons(rho, z, z, z, np.full(len(cofg), 1.0e5), 1.4).T.copy()
+
+    rng = np.random.default_rng(seed)
+    r = rin + (rout - rin) * (0.1 + 0.8 * rng.random(npart))
+    side = rng.integers(0, 3, npart)  # 0 lower face, 1 upper face, 2 inside
+    dth = near / r
+    th = np.where(side == 0, xangle + dth, np.where(side == 1, xangle + angle - dth,
+                                                  xangle + angle * (0.1 + 0.8 * rng.random(npart))))
+    vt = np.where(side == 0, -swirl, np.where(side == 1, swirl, swirl * (2.0 * rng.random(npart) - 1.0)))
+    y = np.zeros((npart, 12))
+    y[:, 0] = r * np.cos(th)
+    y[:, 1] = r * np.sin(th)
+    y[:, 2] = lz * (0.05 + 0.9 * rng.random(npart))
+    near_z = rng.random(npart) < 0.25
+    y[near_z, 2] = np.where(rng.random(near_z.sum()) < 0.5, near, lz - near)
+    y[:, 3] = -vt * np.sin(th)
+    y[:, 4] = vt * np.cos(th)
+    y[:, 5] = np.where(near_z, np.where(y[:, 2] < 0.5 * lz, -swirl, swirl), 0.0)
+    y[:, 6] = 300.0
+    dp = 60e-6 * np.exp(0.2 * rng.standard_normal(npart))
+    vp = np.pi / 6.0 * dp ** 3
+    pconst = np.stack([np.full(npart, 2500.0), dp, vp, np.ones(npart)], axis=1)
+    picl = dict(PICL_AIR, angularPeriodic=1, angAngle=angle, angXAngle=xangle, angRin=rin, angRout=rout,
+                periodicZ=periodicZ)
+    return mesh, params, ic, (y, pconst), picl
+
+
 def ensemble(n_inst=256, cells=(5_000, 500_000), mach=(1.2, 3.0), gammas=(1.4, 1.67), seed=7):
     """The batched ensemble: n_inst independent shock tubes with cell counts
     log-uniform in `cells`, half 5-tet boxes ([0,1]^3) and half 5:1:1 hex

@@ -45,7 +45,17 @@ struct PiclPhys {
   int qsFluct, qsFluctAdapt;  // QS fluctuations: 1 Lattanzi, 2 Osnes; adaptive box
   double fluctW;              // ppiclf_d2chk(2): the fluctuation box's half-width
   long long fluctSeed;        // seed of the uniform draws
+  // angular periodicity about z: the wedge [angX, angUp] of polar angles,
+  // the sign switch angMid = 0.5 (xangle + angle), ppiclF's case (1 wedge
+  // below 90 degrees, 2 quarter, 3 half cylinder), the rotation matrices of
+  // +angle, -angle and 0 (row-major), and the x range of the removal test
+  int ang = 0, angCase = 0;
+  double angX = 0, angUp = 0, angMid = 0, angR[3][9] = {}, angXd[2] = {0, 0};
 };
+
+// the element a particle removed by the angular-periodic range test carries
+// into the next stage
+#define E_REMOVED (-3)
 
 // Rows of PBuf::st
 // particle state rows: x, y, z, u, v, w (and their y1, ydot), the feedback
@@ -142,6 +152,10 @@ struct PiclDev {
   double *fbg = nullptr;     // [nC][9] after WENO-XYZ and the x-force positivity limiter
   double *fldRot = nullptr;  // [nC][6] vorticity, D(omega)/Dt per element
   int nVint = PICL_NFLD;     // values per particle in vint (20, 26 with the rotational fields)
+  // angular periodicity with z periodic: a particle whose x left the wedge's
+  // range after the update is marked (rmMark by current slot) and enters the
+  // next stage with element E_REMOVED, which the location turns into lost
+  int *rmMark = nullptr;
   // QS fluctuations: the box means by current slot, the SetYdot call counter
   int fl = 0;
   int pairs = 0;             // the two-phase stage with k_pairs (Briney or fluctuations)
@@ -1576,6 +1590,28 @@ __global__ void k_vu_time(Dev d, PiclDev P) {
   T[VU_N] = (double)n;
 }
 
+// ppiclf_solve_RemoveParticle's angular test (ppiclf_solve.f:2852-2881) and
+// ppiclf_solve_InvokeAngularPeriodic (:1396-1500) with register = 1: a
+// particle whose polar angle left [xangle, xangle + angle] is rotated about z
+// by -angle when it is past 0.5 (xangle + angle), by +angle otherwise, and
+// not at all (case 3) below xangle; position and velocity, each as gfortran's
+// MATMUL of the 3x3 matrix with the vector (sums over the columns in order).
+__device__ __forceinline__ void rot_apply(const double *R, double *v) {
+  double o[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) o[i] = ((0.0 + R[3 * i] * v[0]) + R[3 * i + 1] * v[1]) + R[3 * i + 2] * v[2];
+#pragma unroll
+  for (int i = 0; i < 3; i++) v[i] = o[i];
+}
+__device__ __forceinline__ void angular_wrap(const PiclPhys &Ph, double yn[6]) {
+  double alpha = atan2(yn[1], yn[0]);
+  if (!(alpha < Ph.angX || alpha > Ph.angUp)) return;
+  int c = alpha > Ph.angMid ? 1 : 0;
+  if (Ph.angCase == 3 && alpha < Ph.angX) c = 2;
+  rot_apply(Ph.angR[c], yn);
+  rot_apply(Ph.angR[c], yn + 3);
+}
+
 // phase 0: the whole stage.  With Briney's added mass or the QS
 // fluctuations the stage runs in two phases around k_pairs (whose sums need
 // every particle's interpolated values): phase 1 locates and interpolates
@@ -1602,7 +1638,9 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   } else {
   double xp[3] = {st[t], st[nP + t], st[2 * nP + t]};
   double rst[3];
-  int e = picl_locate(P, xp, C.elem[t], TRI ? rst : nullptr);
+  // a particle marked by the angular-periodic range test in the last stage
+  // is gone (ppiclf_solve_RemoveParticle at the start of InitSolve)
+  int e = C.elem[t] == E_REMOVED ? -1 : picl_locate(P, xp, C.elem[t], TRI ? rst : nullptr);
   C.elem[t] = e;
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
@@ -1950,20 +1988,39 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   if (istage == 1) { k0 = 0.0; k2 = dt * 8.0 / 15.0; }
   else if (istage == 2) { k0 = dt * 17.0 / 60.0; k2 = dt * 5.0 / 12.0; }
   else { k0 = dt * 5.0 / 12.0; k2 = dt * 3.0 / 4.0; }
+  double yn[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     double y1 = istage == 1 ? 0.0 : st[(ROW_Y1 + k) * nP + t];
     double y = st[(ROW_X + k) * nP + t];
-    double yn = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
-    // ppiclf_solve_InvokeLinearPeriodic (ppiclf_solve.f:1319-1353) after the
-    // update (:1933): beyond one bound, re-enter as far inside the other
-    if (k < 3 && Ph.per[k]) {
-      if (yn < Ph.perLo[k])
-        yn = Ph.perHi[k] - fabs(Ph.perLo[k] - yn);
-      else if (yn > Ph.perHi[k])
-        yn = Ph.perLo[k] + fabs(yn - Ph.perHi[k]);
+    yn[k] = -(k0 * y1) + 1.0 * y + k2 * ydn[k];
+  }
+  // ppiclf_solve_RemoveParticle after the update (ppiclf_solve.f:1933): the
+  // angular wrap, then InvokeLinearPeriodic (:1319-1394)
+  if (Ph.ang) {
+    angular_wrap(Ph, yn);
+    if (Ph.per[2]) {  // the second case: the x range, then the z wrap
+      if (yn[0] < Ph.angXd[0] || yn[0] > Ph.angXd[1])
+        P.rmMark[t] = 1;
+      else if (yn[2] < Ph.perLo[2])
+        yn[2] = Ph.perHi[2] - fabs(Ph.perLo[2] - yn[2]);
+      else if (yn[2] > Ph.perHi[2])
+        yn[2] = Ph.perLo[2] + fabs(yn[2] - Ph.perHi[2]);
     }
-    st[(ROW_X + k) * nP + t] = yn;
+  } else {
+    // beyond one bound, re-enter as far inside the other, axis by axis
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if (Ph.per[k]) {
+        if (yn[k] < Ph.perLo[k])
+          yn[k] = Ph.perHi[k] - fabs(Ph.perLo[k] - yn[k]);
+        else if (yn[k] > Ph.perHi[k])
+          yn[k] = Ph.perLo[k] + fabs(yn[k] - Ph.perHi[k]);
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    st[(ROW_X + k) * nP + t] = yn[k];
     st[(ROW_Y1 + k) * nP + t] = ydn[k];
   }
   if (Ph.heatTransfer) {
@@ -2534,7 +2591,12 @@ __global__ void __launch_bounds__(BLK) k_permute(PiclDev P) {
 #pragma unroll
     for (int c = 0; c < 3; c++) X.fq[c * nP + pos] = C.fq[c * nP + t];
   X.pid[pos] = C.pid[t];
-  X.elem[pos] = C.elem[t];
+  int el = C.elem[t];
+  if (P.rmMark && t < nA && P.rmMark[t]) {  // removed by the angular-periodic range test
+    el = E_REMOVED;
+    P.rmMark[t] = 0;
+  }
+  X.elem[pos] = el;
 }
 
 // the SetYdot call counter that keys the fluctuations' uniform draws
@@ -2829,6 +2891,49 @@ static int picl_scratch_init(rflu_handle *h) {
 
 extern "C" {
 
+// ppiclf_solve_Initialize's angular case (ppiclf_solve.f:60-104, its pi the
+// single-precision ACOS(-1.0)) and ppiclf_solve_InitAngularPeriodic
+// (:1172-1230), then InvokeAngularPeriodic's rotation matrices (about e_z,
+// Rodrigues' form with the reference's expressions and cos/sin of the three
+// local angles +angle, -angle, 0) computed once here.
+static int angular_setup(rflu_handle *h, const picl_params *pp, PiclPhys &A) {
+  const double angle = pp->angAngle, xangle = pp->angXAngle;
+  const double pif = (double)acosf(-1.0f);
+  const double angled = angle * 180.0 / pif;
+  int c = 0;
+  if (angled < 90.0) c = 1;
+  if (std::lround(angled) == 90) c = 2;
+  if (std::lround(angled) == 180) c = 3;
+  if (!(pp->angRin < pp->angRout))
+    return set_err(h, RFLU_GPU_EINVAL, "Angular Per must have rin < rout (ppiclf_solve_InitAngularPeriodic)");
+  if (c == 0 || !(angle > 0.0))
+    return set_err(h, RFLU_GPU_EINVAL, "Invalid Rotational Case! (the wedge must open by less than 90, by 90 or by "
+                                       "180 degrees)");
+  A.ang = 1;
+  A.angCase = c;
+  A.angX = xangle;
+  A.angUp = xangle + angle;
+  A.angMid = 0.5 * (xangle + angle);
+  A.angXd[0] = c == 3 ? -1.0 * pp->angRout : pp->angRin;
+  A.angXd[1] = pp->angRout;
+  const double ex = 0.0, ey = 0.0, ez = 1.0;
+  const double la[3] = {angle, -1.0 * angle, 0.0};
+  for (int k = 0; k < 3; k++) {
+    double ct = cos(la[k]), st = sin(la[k]);
+    double *R = A.angR[k];
+    R[0] = ct + (1.0 - ct) * ex * ex;
+    R[1] = (1.0 - ct) * ex * ey - st * ez;
+    R[2] = (1.0 - ct) * ex * ez + st * ey;
+    R[3] = (1.0 - ct) * ey * ex + st * ez;
+    R[4] = ct + (1.0 - ct) * ey * ey;
+    R[5] = (1.0 - ct) * ey * ez - st * ex;
+    R[6] = (1.0 - ct) * ez * ex - st * ey;
+    R[7] = (1.0 - ct) * ez * ey + st * ex;
+    R[8] = ct + (1.0 - ct) * ez * ez;
+  }
+  return 0;
+}
+
 int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double *y, const double *pconst) {
   API_LOCK;
   if (!h || !pp || nPart < 0 || (nPart > 0 && (!y || !pconst))) return RFLU_GPU_EINVAL;
@@ -2881,6 +2986,18 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     if (perFlag[l] && !(vlo[l] < vhi[l]))
       return set_err(h, RFLU_GPU_EINVAL, "a periodic axis needs a mesh of positive extent along it");
   }
+  PiclPhys angPh;
+  if (pp->angularPeriodic < 0 || pp->angularPeriodic > 1)
+    return set_err(h, RFLU_GPU_EINVAL, "angularPeriodic must be 0 or 1");
+  if (pp->angularPeriodic) {
+    // PICL_TEMP_InitSolver.F90:700-705 and ppiclf_solve.f:75-78
+    if (perFlag[0] || perFlag[1])
+      return set_err(h, RFLU_GPU_EINVAL, "PPICLF: Invalid Periodicity choice (angular with periodicX or periodicY)");
+    if (pp->projMode != 0 || pp->collisionalFlag || pp->amFlag == 2 || pp->qsFluct)
+      return set_err(h, RFLU_GPU_EINVAL, "angular periodicity needs the box projection and no pair models "
+                                         "(ppiclF's rotated ghost particles are not built)");
+    if (int e = angular_setup(h, pp, angPh)) return e;
+  }
   if (int e = picl_scratch_init(h)) return e;  // the static staging of picl_gpu_locate_static
   const int N = h->d.nC, nP = nPart;
   PiclState *s = new PiclState();
@@ -2893,6 +3010,16 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     s->phys.per[l] = perFlag[l];
     s->phys.perLo[l] = vlo[l];
     s->phys.perHi[l] = vhi[l];
+  }
+  if (pp->angularPeriodic) {
+    s->phys.ang = 1;
+    s->phys.angCase = angPh.angCase;
+    s->phys.angX = angPh.angX;
+    s->phys.angUp = angPh.angUp;
+    s->phys.angMid = angPh.angMid;
+    memcpy(s->phys.angR, angPh.angR, sizeof angPh.angR);
+    s->phys.angXd[0] = angPh.angXd[0];
+    s->phys.angXd[1] = angPh.angXd[1];
   }
   s->P.ht = pp->heatTransfer != 0 ? 1 : 0;
   s->phys.vu = s->P.vu = pp->viscousUnsteady;
@@ -3094,6 +3221,10 @@ int picl_gpu_init(rflu_handle *h, const picl_params *pp, int nPart, const double
     rc |= dalloc(h, &P.fldRot, 6 * (size_t)N);
   }
   rc |= dalloc(h, &P.ycN, 4 * (size_t)nP);
+  if (pp->angularPeriodic && perFlag[2]) {
+    std::vector<int> z0(nP, 0);
+    rc |= dalloc(h, &P.rmMark, (size_t)nP, z0.data());
+  }
   rc |= dalloc(h, &P.src, (size_t)nP);
   rc |= dalloc(h, &P.dest, (size_t)nP);
   rc |= dalloc(h, &P.srt, (size_t)nP);
@@ -3446,6 +3577,8 @@ struct F77State {
   picl_params pp{};
   bool init = false, linit = false, lfilt = false, overlap = false;
   int per[3] = {0, 0, 0};
+  int ang = 0;                                             // ang_per_flag
+  double angA = 0.0, angX = 0.0, angRin = 0.0, angRout = 0.0;  // radians, radii
   int npart = 0;
   std::vector<double> y, pc;
   int proj = 0;
@@ -3514,9 +3647,14 @@ void ppiclf_solve_initialize_(const int *xi1, const double *xpmin, const double 
                               const double *zpmax, const int *ai1, const double *apa, const double *apxa,
                               const double *aprin, const double *aprout) {
   API_LOCK;
-  (void)apa; (void)apxa; (void)aprin; (void)aprout;
   if (!g_f77.h) return f77_fail("ppiclf_gpu_bind must come before ppiclf_solve_Initialize");
-  if (*ai1 == 1) return f77_fail("angular periodicity (ang_per_flag 1) is not built");
+  if (*ai1 != 0 && *ai1 != 1) return f77_fail("ang_per_flag must be 0 or 1");
+  if (*ai1 == 1 && (*xi1 == 1 || *yi1 == 1)) return f77_fail("PPICLF: Invalid Periodicity choice");
+  g_f77.ang = *ai1;
+  g_f77.angA = *apa;
+  g_f77.angX = *apxa;
+  g_f77.angRin = *aprin;
+  g_f77.angRout = *aprout;
   const int fl[3] = {*xi1, *yi1, *zi1};
   const double lo[3] = {*xpmin, *ypmin, *zpmin}, hi[3] = {*xpmax, *ypmax, *zpmax};
   double vlo[3] = {1e300, 1e300, 1e300}, vhi[3] = {-1e300, -1e300, -1e300};
@@ -3621,6 +3759,11 @@ void ppiclf_comm_initoverlapmesh_(const int *ncell, const int *lx1, const int *l
   p.periodicX = g_f77.per[0];
   p.periodicY = g_f77.per[1];
   p.periodicZ = g_f77.per[2];
+  p.angularPeriodic = g_f77.ang;
+  p.angAngle = g_f77.angA;
+  p.angXAngle = g_f77.angX;
+  p.angRin = g_f77.angRin;
+  p.angRout = g_f77.angRout;
   p.neighborWidth = g_f77.nbw;
   p.fluctHalfWidth = g_f77.filt2;
   if (int rc = picl_gpu_init(h, &p, g_f77.npart, g_f77.y.data(), g_f77.pc.data()))

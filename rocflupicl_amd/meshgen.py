@@ -139,6 +139,25 @@ def hex_box(nx, ny, nz, lx, ly, lz, jitter=0.0, seed=0, bc=None):
     return dict(xyz=xyz, c2v=c2v, nvc=8, patches=patches, dims=(nx, ny, nz), lengths=(lx, ly, lz))
 
 
+def wedge_hex(nr, nt, nz, rin, rout, angle, xangle, lz, bc=None):
+    """A wedge of hexes about the z axis: radii rin..rout (rin > 0), polar
+    angles xangle..xangle + angle (radians), z in [0, lz]; node (i, j, k) at
+    r = rin + i (rout - rin)/nr, theta = xangle + j angle/nt, z = k lz/nz.
+    The (r, theta, z) lattice maps the box's (x, y, z) with its orientation,
+    so the hexes are the box's.  Patches in the box's order: r = rin,
+    r = rout, theta = xangle, theta = xangle + angle, z = 0, z = lz (slip
+    walls by default)."""
+    g = _node_grid(nr, nt, nz, 1.0, 1.0, 1.0, 0.0, 0)
+    r = rin + g[:, 0] * (rout - rin)
+    th = xangle + g[:, 1] * angle
+    xyz = np.ascontiguousarray(np.stack([r * np.cos(th), r * np.sin(th), g[:, 2] * lz], axis=1))
+    c2v, i, j, k = _hex_c2v(nr, nt, nz)
+    if bc is None:
+        bc = [(BC_SLIPWALL, 0, 0.0)] * 6
+    patches = _box_patches(c2v, i, j, k, nr, nt, nz, 8, bc)
+    return dict(xyz=xyz, c2v=c2v, nvc=8, patches=patches, dims=(nr, nt, nz), lengths=(rout - rin, angle, lz))
+
+
 def tet_box(nx, ny, nz, lx, ly, lz, jitter=0.0, seed=0, bc=None):
     """Hex box split into 5 tets per hex with (i+j+k)-alternating diagonals."""
     xyz = _node_grid(nx, ny, nz, lx, ly, lz, jitter, seed)

@@ -27,7 +27,14 @@ PICL_DEFAULTS = dict(rmuFlag=1, rmuRef=1.716e-5, tref=273.15, suth=110.4, rcpPar
                      amFlag=1, pgFlag=1, feedbackFlag=1, interpMode=0, projMode=0, filterWidth=0.0,
                      filterAlpha=1e-2, heatTransfer=0, periodicX=0, periodicY=0,
                      periodicZ=0, viscousUnsteady=0, collisionalFlag=0, ksp=10.0, erest=0.9,
-                     neighborWidth=0.0, qsFluct=0, qsFluctAdapt=0, fluctHalfWidth=0.0, fluctSeed=0)
+                     neighborWidth=0.0, qsFluct=0, qsFluctAdapt=0, fluctHalfWidth=0.0, fluctSeed=0,
+                     angularPeriodic=0, angAngle=0.0, angXAngle=0.0, angRin=0.0, angRout=0.0)
+
+
+def angular_radians(deg):
+    """A wedge angle of the deck (degrees) in radians, as
+    PICL_TEMP_InitSolver.F90:271-273 converts it: deg * acos(-1) / 180."""
+    return float(deg) * float(np.arccos(-1.0)) / 180.0
 # rprop index (1-based, PPICLF_USER.h) of each interpolated field slot
 PICL_FIELD_RPROP = [2, 6, 7, 8, 10, 11, 12, 9, 24, 5, 13, 14, 15, 16, 17, 18, 19, 37, 38, 39]
 

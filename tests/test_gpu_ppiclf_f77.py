@@ -196,9 +196,10 @@ def test_f77_errors_are_reported_as_ppiclf_reports_them():
         d = dict(rflu.PICL_DEFAULTS)
         pp = _lib.PiclParams(**d)
         assert L.ppiclf_gpu_bind(b.h, C.byref(pp)) == 0
-        L.ppiclf_solve_initialize_(ri(0), rd(0), rd(1), ri(0), rd(0), rd(1), ri(0), rd(0), rd(1), ri(1), rd(0),
+        # angular periodicity with linear x periodicity (ppiclf_solve.f:75-78)
+        L.ppiclf_solve_initialize_(ri(1), rd(0), rd(1), ri(0), rd(0), rd(1), ri(0), rd(0), rd(1), ri(1), rd(0),
                                    rd(0), rd(0), rd(0))
-        assert "angular periodicity" in L.ppiclf_gpu_last_error().decode()
+        assert "Invalid Periodicity choice" in L.ppiclf_gpu_last_error().decode()
         L.ppiclf_solve_initialize_(ri(1), rd(0), rd(1), ri(0), rd(0), rd(1), ri(0), rd(0), rd(1), ri(0), rd(0),
                                    rd(0), rd(0), rd(0))
         assert "vertex minimum and maximum" in L.ppiclf_gpu_last_error().decode()
