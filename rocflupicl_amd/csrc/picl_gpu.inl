@@ -2387,58 +2387,88 @@ __device__ __forceinline__ void gauss_batch(const PiclDev &P, const int *Qp, con
 // slot in gq, C[j] = its particle count): each half-wave takes one element,
 // its 32 lanes its particles.  Those within the cut-off go to the queue (Qp,
 // Qr; *qn entries, fewer than 64 between calls) in lane order, and every 64
-// of them are summed as one batch.
+// of them are summed as one batch.  The chunks (element pair k, particles pb
+// to pb + 31) are walked in order with the next chunk's positions loaded
+// before the current one is tested, so a gather's latency overlaps a test.
 __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, const int *C, int n, int lane,
                                             const double xv[3], const double sh[3], int *Qp, double *Qr, int &qn,
                                             const double *T, double acc[5], double cmp[5]) {
   const int half = lane >> 5, hl = lane & 31;
-  for (int k = 0; k < n; k += 2) {
-    const int j = k + half;
-    const int cnt = j < n ? C[j] : 0, first = j < n ? E[j] : 0;
-    const int maxc = max(cnt, __shfl_xor(cnt, 32, 64));
-    for (int pb = 0; pb < maxc; pb += 32) {
-      const int p = pb + hl;
-      bool take = false;
-      int pos = 0;
-      double rdist2 = 0.0;
-      if (p < cnt) {
-        pos = first + p;
-        const double2 *g = reinterpret_cast<const double2 *>(P.gq + 4 * (size_t)pos);
-        double2 g0 = g[0], g1 = g[1];
-        // the image's coordinates as ppiclF's ghost holds them, x_p + s
-        // (x_p + 0 is x_p, or +0 for -0, which gives the same difference)
-        double dx = xv[0] - (g0.x + sh[0]);
-        double dy = xv[1] - (g0.y + sh[1]);
-        double dz = xv[2] - (g1.x + sh[2]);
-        rdist2 = dx * dx + dy * dy;
-        rdist2 = rdist2 + dz * dz;
-        take = rdist2 <= P.gD2chk;
-      }
-      const unsigned long long m = __ballot(take);
-      if (take) {
-        int at = qn + lanes_below(m);
-        Qp[at] = pos;
-        Qr[at] = rdist2;
-      }
-      qn += __popcll(m);
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (qn >= 64) {
-        gauss_batch(P, Qp, Qr, 64, lane, T, acc, cmp);
-        // the remainder moves to the front
-        int rest = qn - 64, mp = lane < rest ? Qp[64 + lane] : 0;
-        double mr = lane < rest ? Qr[64 + lane] : 0.0;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane < rest) {
-          Qp[lane] = mp;
-          Qr[lane] = mr;
-        }
-        qn = rest;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  int k = 0, pb = 0;
+  int cnt = half < n ? C[half] : 0, first = half < n ? E[half] : 0;
+  int maxc = max(cnt, __shfl_xor(cnt, 32, 64));
+  // the first chunk's positions
+  bool nin = hl < cnt;
+  int npos = first + hl;
+  double2 nxy = make_double2(0.0, 0.0);
+  double nz = 0.0;
+  if (nin) {
+    nxy = reinterpret_cast<const double2 *>(P.gq + 4 * (size_t)npos)[0];
+    nz = P.gq[4 * (size_t)npos + 2];
+  }
+  while (true) {
+    const bool in = nin;
+    const int pos = npos;
+    const double2 gxy = nxy;
+    const double gz = nz;
+    // advance to the next chunk (uniform over the wave) and start its loads
+    pb += 32;
+    if (pb >= maxc) {
+      k += 2;
+      pb = 0;
+      if (k < n) {
+        const int j = k + half;
+        cnt = j < n ? C[j] : 0;
+        first = j < n ? E[j] : 0;
+        maxc = max(cnt, __shfl_xor(cnt, 32, 64));
       }
     }
+    const bool more = k < n;
+    if (more) {
+      nin = pb + hl < cnt;
+      npos = first + pb + hl;
+      if (nin) {
+        nxy = reinterpret_cast<const double2 *>(P.gq + 4 * (size_t)npos)[0];
+        nz = P.gq[4 * (size_t)npos + 2];
+      }
+    }
+    bool take = false;
+    double rdist2 = 0.0;
+    if (in) {
+      // the image's coordinates as ppiclF's ghost holds them, x_p + s
+      // (x_p + 0 is x_p, or +0 for -0, which gives the same difference)
+      double dx = xv[0] - (gxy.x + sh[0]);
+      double dy = xv[1] - (gxy.y + sh[1]);
+      double dz = xv[2] - (gz + sh[2]);
+      rdist2 = dx * dx + dy * dy;
+      rdist2 = rdist2 + dz * dz;
+      take = rdist2 <= P.gD2chk;
+    }
+    const unsigned long long m = __ballot(take);
+    if (take) {
+      int at = qn + lanes_below(m);
+      Qp[at] = pos;
+      Qr[at] = rdist2;
+    }
+    qn += __popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (qn >= 64) {
+      gauss_batch(P, Qp, Qr, 64, lane, T, acc, cmp);
+      // the remainder moves to the front
+      int rest = qn - 64, mp = lane < rest ? Qp[64 + lane] : 0;
+      double mr = lane < rest ? Qr[64 + lane] : 0.0;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (lane < rest) {
+        Qp[lane] = mp;
+        Qr[lane] = mr;
+      }
+      qn = rest;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (!more) break;
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
