@@ -1612,6 +1612,23 @@ __device__ __forceinline__ void angular_wrap(const PiclPhys &Ph, double yn[6]) {
   rot_apply(Ph.angR[c], yn + 3);
 }
 
+// cnt[e] += 1 for every active lane with e >= 0, one atomic per run of
+// equal elements in consecutive lanes (the particles are mostly in element
+// order from the last stage's sort).  Every active lane must call it;
+// lanes that already left the kernel end any run.
+__device__ __forceinline__ void count_by_runs(int *cnt, int e) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long act = __ballot(true);
+  const int below = __shfl_up(e, 1, 64);
+  const bool starts = e >= 0 && (lane == 0 || below != e);
+  const unsigned long long brk = __ballot(e < 0 || lane == 0 || below != e) | ~act;
+  if (starts) {
+    const unsigned long long after = brk & ~((2ull << lane) - 1ull);
+    const int end = after ? __builtin_ctzll(after) : 64;
+    atomicAdd(&cnt[e], end - lane);
+  }
+}
+
 // phase 0: the whole stage.  With Briney's added mass or the QS
 // fluctuations the stage runs in two phases around k_pairs (whose sums need
 // every particle's interpolated values): phase 1 locates and interpolates
@@ -1642,13 +1659,13 @@ __global__ void __launch_bounds__(BLK, 3) k_particles(Dev d, PiclDev P, PiclPhys
   // is gone (ppiclf_solve_RemoveParticle at the start of InitSolve)
   int e = C.elem[t] == E_REMOVED ? -1 : picl_locate(P, xp, C.elem[t], TRI ? rst : nullptr);
   C.elem[t] = e;
+  count_by_runs(P.cnt, e);
   if (e < 0) {  // left every element: skipped from now on (ppiclF would remove it)
 #pragma unroll
     for (int k = 0; k < 4; k++) P.ycN[k * nP + t] = 0.0;
     if (phase == 1 && P.am2) P.actP[C.pid[t]] = 0;
     return;
   }
-  atomicAdd(&P.cnt[e], 1);
   if (P.gauss) {  // the Gaussian projection reads the position at the start of the stage
 #pragma unroll
     for (int k = 0; k < 3; k++) P.xo[k * nP + t] = xp[k];
@@ -2138,17 +2155,33 @@ __global__ void __launch_bounds__(SCAN_T) k_scan3(int *cnt, int n, const int *bs
 // Particles into their element's new positions (arbitrary order within an
 // element; k_deposit sorts each segment); particles that left every element
 // go after the active ones.
+// The particles are mostly in element order already (the last stage's
+// sort), so a wave's lanes form a few runs of one element each: one atomic
+// per run reserves the run's slots.  The order inside a segment is fixed
+// afterwards by k_rank, so the reservation order does not matter.
 __global__ void __launch_bounds__(BLK) k_fill(PiclDev P) {
-  int t = blockIdx.x * BLK + threadIdx.x;
-  if (t >= P.nAct[0]) return;
+  const int t = blockIdx.x * BLK + threadIdx.x, lane = threadIdx.x & 63;
+  const int nA = P.nAct[0];
   PBuf C = cur_buf(P);
-  int e = C.elem[t];
-  if (e < 0) {
-    P.dest[t] = P.off[P.nC] + atomicAdd(P.lostCur, 1);
-    return;
+  const int e = t < nA ? C.elem[t] : -2;
+  if (e == -1) P.dest[t] = P.off[P.nC] + atomicAdd(P.lostCur, 1);
+  // runs of equal elements: a lane heads a run when its element differs
+  // from the lane below's
+  const int below = __shfl_up(e, 1, 64);
+  const bool head = e >= 0 && (lane == 0 || below != e);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int h = 63 - __builtin_clzll(heads & upto | 1ull);  // this lane's run head (0 when none)
+  // the run ends at the next head or at the first lane of another element
+  const unsigned long long diff = __ballot(!(e >= 0) || (lane > 0 && below != e));
+  int base = 0;
+  if (head) {
+    const unsigned long long after = diff & ~((2ull << lane) - 1ull);
+    const int end = after ? __builtin_ctzll(after) : 64;
+    base = atomicAdd(&P.cnt[e], end - lane);
   }
-  int slot = atomicAdd(&P.cnt[e], 1);
-  P.src[P.off[e] + slot] = t;
+  base = __shfl(base, h, 64);
+  if (e >= 0) P.src[P.off[e] + base + (lane - h)] = t;
 }
 
 // ---------------------------------------------------------------------------
