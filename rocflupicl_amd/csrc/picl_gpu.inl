@@ -695,6 +695,44 @@ __device__ __forceinline__ int col_coord(const PiclDev &P, double x, int l) {
 
 // The stage's location (k_particles repeats it and gets the same element)
 // and the bucket counts
+// cnt[e] += 1 for every active lane with e >= 0, one atomic per run of
+// equal elements in consecutive lanes (the particles are mostly in element
+// order from the last stage's sort).  Every active lane must call it;
+// lanes that already left the kernel end any run.
+__device__ __forceinline__ void count_by_runs(int *cnt, int e) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long act = __ballot(true);
+  const int below = __shfl_up(e, 1, 64);
+  const bool starts = e >= 0 && (lane == 0 || below != e);
+  const unsigned long long brk = __ballot(e < 0 || lane == 0 || below != e) | ~act;
+  if (starts) {
+    const unsigned long long after = brk & ~((2ull << lane) - 1ull);
+    const int end = after ? __builtin_ctzll(after) : 64;
+    atomicAdd(&cnt[e], end - lane);
+  }
+}
+
+// Slots from a per-key counter, one atomic per run of equal keys in
+// consecutive lanes: returns ctr[key]'s old value plus the lane's place in
+// its run (-1 for key < 0).  Every active lane must call it.
+__device__ __forceinline__ int reserve_by_runs(int *ctr, int key) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long act = __ballot(true);
+  const int below = __shfl_up(key, 1, 64);
+  const bool starts = key >= 0 && (lane == 0 || below != key);
+  const unsigned long long heads = __ballot(starts);
+  const unsigned long long brk = __ballot(key < 0 || lane == 0 || below != key) | ~act;
+  const int h = 63 - __builtin_clzll((heads & ((2ull << lane) - 1ull)) | 1ull);
+  int base = 0;
+  if (starts) {
+    const unsigned long long after = brk & ~((2ull << lane) - 1ull);
+    const int end = after ? __builtin_ctzll(after) : 64;
+    base = atomicAdd(&ctr[key], end - lane);
+  }
+  base = __shfl(base, h, 64);
+  return key >= 0 ? base + (lane - h) : -1;
+}
+
 __global__ void __launch_bounds__(BLK) k_col_count(PiclDev P) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
@@ -703,21 +741,23 @@ __global__ void __launch_bounds__(BLK) k_col_count(PiclDev P) {
   double xp[3] = {C.st[t], C.st[nP + t], C.st[2 * nP + t]};
   int e = picl_locate(P, xp, C.elem[t]);
   C.elem[t] = e;
-  if (e < 0) { P.colB[t] = -1; return; }
-  int c[3];
+  int b = -1;
+  if (e >= 0) {
+    int c[3];
 #pragma unroll
-  for (int l = 0; l < 3; l++) c[l] = col_coord(P, xp[l], l);
-  int b = c[0] + P.colN[0] * (c[1] + P.colN[1] * c[2]);
+    for (int l = 0; l < 3; l++) c[l] = col_coord(P, xp[l], l);
+    b = c[0] + P.colN[0] * (c[1] + P.colN[1] * c[2]);
+  }
   P.colB[t] = b;
-  atomicAdd(&P.colCnt[b], 1);
+  count_by_runs(P.colCnt, b);
 }
 
 __global__ void __launch_bounds__(BLK) k_col_fill(PiclDev P) {
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= P.nAct[0]) return;
   int b = P.colB[t];
-  if (b < 0) return;
-  P.colMem[P.colOff[b] + atomicAdd(&P.colFill[b], 1)] = t;
+  int q = reserve_by_runs(P.colFill, b);  // the cube's order is fixed by k_col_sort
+  if (b >= 0) P.colMem[P.colOff[b] + q] = t;
 }
 
 // ppiclf_user_EvalNearestNeighbor, collisional part (:118-233), for
@@ -1612,23 +1652,6 @@ __device__ __forceinline__ void angular_wrap(const PiclPhys &Ph, double yn[6]) {
   rot_apply(Ph.angR[c], yn + 3);
 }
 
-// cnt[e] += 1 for every active lane with e >= 0, one atomic per run of
-// equal elements in consecutive lanes (the particles are mostly in element
-// order from the last stage's sort).  Every active lane must call it;
-// lanes that already left the kernel end any run.
-__device__ __forceinline__ void count_by_runs(int *cnt, int e) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long act = __ballot(true);
-  const int below = __shfl_up(e, 1, 64);
-  const bool starts = e >= 0 && (lane == 0 || below != e);
-  const unsigned long long brk = __ballot(e < 0 || lane == 0 || below != e) | ~act;
-  if (starts) {
-    const unsigned long long after = brk & ~((2ull << lane) - 1ull);
-    const int end = after ? __builtin_ctzll(after) : 64;
-    atomicAdd(&cnt[e], end - lane);
-  }
-}
-
 // phase 0: the whole stage.  With Briney's added mass or the QS
 // fluctuations the stage runs in two phases around k_pairs (whose sums need
 // every particle's interpolated values): phase 1 locates and interpolates
@@ -2160,28 +2183,13 @@ __global__ void __launch_bounds__(SCAN_T) k_scan3(int *cnt, int n, const int *bs
 // per run reserves the run's slots.  The order inside a segment is fixed
 // afterwards by k_rank, so the reservation order does not matter.
 __global__ void __launch_bounds__(BLK) k_fill(PiclDev P) {
-  const int t = blockIdx.x * BLK + threadIdx.x, lane = threadIdx.x & 63;
+  const int t = blockIdx.x * BLK + threadIdx.x;
   const int nA = P.nAct[0];
   PBuf C = cur_buf(P);
   const int e = t < nA ? C.elem[t] : -2;
   if (e == -1) P.dest[t] = P.off[P.nC] + atomicAdd(P.lostCur, 1);
-  // runs of equal elements: a lane heads a run when its element differs
-  // from the lane below's
-  const int below = __shfl_up(e, 1, 64);
-  const bool head = e >= 0 && (lane == 0 || below != e);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int h = 63 - __builtin_clzll(heads & upto | 1ull);  // this lane's run head (0 when none)
-  // the run ends at the next head or at the first lane of another element
-  const unsigned long long diff = __ballot(!(e >= 0) || (lane > 0 && below != e));
-  int base = 0;
-  if (head) {
-    const unsigned long long after = diff & ~((2ull << lane) - 1ull);
-    const int end = after ? __builtin_ctzll(after) : 64;
-    base = atomicAdd(&P.cnt[e], end - lane);
-  }
-  base = __shfl(base, h, 64);
-  if (e >= 0) P.src[P.off[e] + base + (lane - h)] = t;
+  const int slot = reserve_by_runs(P.cnt, e);
+  if (e >= 0) P.src[P.off[e] + slot] = t;
 }
 
 // ---------------------------------------------------------------------------
