@@ -2390,7 +2390,9 @@ __global__ void __launch_bounds__(BLK) k_gcount(PiclDev P) {
 // order at the end, so the result does not depend on the schedule.
 #define GW_CAP 256
 #define GW_WAVES (BLK / 64)
+#ifndef GW_BLOCKS
 #define GW_BLOCKS 8192  // a multiple of 8: 32 workgroups per CU
+#endif
 #ifndef GW_OCC
 #define GW_OCC 4  // waves per SIMD the register budget allows
 #endif
