@@ -78,9 +78,7 @@ struct Dev {
   unsigned short *sSlot = nullptr;
   unsigned short *cfSlot = nullptr;  // per interior cell-face entry: the LDS slot of the face neighbour
   // FP32 evaluation mode: limited gradients as 16 floats per cell (15 + pad)
-  // and the interior faces' fluxes as 5 floats per face (boundary faces keep
-  // their FP64 fluxes in flux[])
-  float *gradF = nullptr, *fluxF = nullptr;
+  float *gradF = nullptr;
   // WENO-XYZ reconstruction (reconst 2): the unlimited LSQ gradients
   double *gradRaw = nullptr;
   // fused stages (k_fu): the records written by stage 2 and read by stage 3
@@ -1689,11 +1687,42 @@ __device__ __forceinline__ void rind_totang(const Gas &G, int opt, const double 
   for (int k = 0; k < 6; k++) b[k] = pc[PC_TT_SUP + k];
 }
 
+// The interior face flux of the FP32 mode: AUSM+, Roe or HLLC on FP32 face
+// states.  irA, irB: 1/rho of the cells the states belong to (HLLC's
+// H = e + p / rho takes the cell's rho, as the FP64 path).
+__device__ __forceinline__ void face_flux_f(const Gas &G, int discr, float fx, float fy, float fz, float fm,
+                                            const FaceStateF &A, const FaceStateF &B, float irA, float irB,
+                                            float ff[5]) {
+  float g = (float)G.g, gc = (float)G.gc, cp = (float)G.cp;
+  if (discr == RFLU_DISCR_AUSM) {
+    float tl = A.p / (A.r * gc), al = sqrtf(g * gc * tl);
+    float Hl = cp * tl + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w);
+    float tr = B.p / (B.r * gc), ar = sqrtf(g * gc * tr);
+    float Hr = cp * tr + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w);
+    ausm_flux_f(fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, Hl, al, B.r, B.u, B.v, B.w, B.p, Hr, ar, ff);
+  } else if (discr == RFLU_DISCR_ROE) {
+    float Hl = cp * (A.p / (A.r * gc)) + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w);
+    float ql = A.u * fx + A.v * fy + A.w * fz;
+    float Hr = cp * (B.p / (B.r * gc)) + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w);
+    float qr = B.u * fx + B.v * fy + B.w * fz;
+    roe_flux_f(g - 1.0f, (float)G.epsentr, (float)G.dissFact, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, Hl, ql, B.r,
+               B.u, B.v, B.w, B.p, Hr, qr, ff);
+  } else {
+    float el = A.p / ((g - 1.0f) * A.r) + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w), al = sqrtf(g * A.p / A.r);
+    float ql = A.u * fx + A.v * fy + A.w * fz, Hl = el + A.p * irA;
+    float er = B.p / ((g - 1.0f) * B.r) + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w), ar = sqrtf(g * B.p / B.r);
+    float qr = B.u * fx + B.v * fy + B.w * fz, Hr = er + B.p * irB;
+    hllc_flux_f(g - 1.0f, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, el, al, ql, Hl, B.r, B.u, B.v, B.w, B.p, er, ar, qr,
+                Hr, ff);
+  }
+}
+
 // The flux of face slot s (interior faces, then boundary faces by patch) as
 // the face loop of RFLU_ComputeFluxInv and the patch loops compute it; srad
 // (optional) receives the face's spectral radius for RFLU_TimeStepInviscid.
-// F32 = 1: interior second-order faces in FP32 from the FP32 gradients
-// (returned widened, exactly); boundary faces stay FP64 (their gradients
+// F32 = 1: interior second-order faces from the FP32 gradients, the flux
+// evaluated in FP32 as a difference to the left cell's own state, whose
+// physical flux is added in FP64; boundary faces stay FP64 (their gradients
 // widened from FP32).
 template <int F32>
 __device__ __forceinline__ void face_flux(const Dev &d, const Gas &G, const FluxPar &P, int s, double f[5],
@@ -1741,31 +1770,34 @@ __device__ __forceinline__ void face_flux(const Dev &d, const Gas &G, const Flux
       FaceStateF A = reconstruct_f(L, d.gradF + 16 * (size_t)c1, xc, yc, zc);
       FaceStateF B = reconstruct_f(R, d.gradF + 16 * (size_t)c2, xc, yc, zc);
       float fx = (float)nx, fy = (float)ny, fz = (float)nz, fm = (float)nm;
-      float g = (float)G.g, gc = (float)G.gc, cp = (float)G.cp, ff[5];
-      if (P.discr == RFLU_DISCR_AUSM) {
-        float tl = A.p / (A.r * gc), al = sqrtf(g * gc * tl);
-        float Hl = cp * tl + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w);
-        float tr = B.p / (B.r * gc), ar = sqrtf(g * gc * tr);
-        float Hr = cp * tr + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w);
-        ausm_flux_f(fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, Hl, al, B.r, B.u, B.v, B.w, B.p, Hr, ar, ff);
-      } else if (P.discr == RFLU_DISCR_ROE) {
-        float Hl = cp * (A.p / (A.r * gc)) + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w);
-        float ql = A.u * fx + A.v * fy + A.w * fz;
-        float Hr = cp * (B.p / (B.r * gc)) + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w);
-        float qr = B.u * fx + B.v * fy + B.w * fz;
-        roe_flux_f(g - 1.0f, (float)G.epsentr, (float)G.dissFact, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, Hl, ql,
-                   B.r, B.u, B.v, B.w, B.p, Hr, qr, ff);
-      } else {
-        // H = e + p / rho with the cell's rho, as the FP64 path (FaceState::ir)
-        float el = A.p / ((g - 1.0f) * A.r) + 0.5f * (A.u * A.u + A.v * A.v + A.w * A.w), al = sqrtf(g * A.p / A.r);
-        float ql = A.u * fx + A.v * fy + A.w * fz, Hl = el + A.p * (float)(1.0 / L[0]);
-        float er = B.p / ((g - 1.0f) * B.r) + 0.5f * (B.u * B.u + B.v * B.v + B.w * B.w), ar = sqrtf(g * B.p / B.r);
-        float qr = B.u * fx + B.v * fy + B.w * fz, Hr = er + B.p * (float)(1.0 / R[0]);
-        hllc_flux_f(g - 1.0f, fx, fy, fz, fm, A.r, A.u, A.v, A.w, A.p, el, al, ql, Hl, B.r, B.u, B.v, B.w, B.p, er,
-                    ar, qr, Hr, ff);
+      // The face flux is the physical flux of the left cell's own state U in
+      // FP64 plus the FP32 difference F(A, B) - F(U, U).  Near a uniform
+      // state the two FP32 evaluations round almost alike, so the difference
+      // is accurate well below FP32 rounding of the flux itself, and the
+      // faces of a closed cell cancel to FP64 rounding whatever their
+      // orientation.  Where both face states equal U (zero gradients) the
+      // difference is exactly zero and is not evaluated.
+      const FaceStateF U = {(float)L[0], (float)L[1], (float)L[2], (float)L[3], (float)L[4]};
+      {
+        const double r = L[0], u = L[1], v = L[2], w = L[3], p = L[4];
+        const double mf = r * (u * nx + v * ny + w * nz) * nm;
+        const double H = G.g / (G.g - 1.0) * (p / r) + 0.5 * (u * u + v * v + w * w);
+        f[0] = mf;
+        f[1] = mf * u + p * nx * nm;
+        f[2] = mf * v + p * ny * nm;
+        f[3] = mf * w + p * nz * nm;
+        f[4] = mf * H;
       }
+      const bool uniform = A.r == U.r && A.u == U.u && A.v == U.v && A.w == U.w && A.p == U.p && B.r == U.r &&
+                           B.u == U.u && B.v == U.v && B.w == U.w && B.p == U.p;
+      if (!uniform) {
+        const float irL = (float)(1.0 / L[0]), irR = (float)(1.0 / R[0]);
+        float ff[5], fr[5];
+        face_flux_f(G, P.discr, fx, fy, fz, fm, A, B, irL, irR, ff);
+        face_flux_f(G, P.discr, fx, fy, fz, fm, U, U, irL, irL, fr);
 #pragma unroll
-      for (int k = 0; k < 5; k++) f[k] = (double)ff[k];
+        for (int k = 0; k < 5; k++) f[k] = f[k] + (double)(ff[k] - fr[k]);
+      }
       return;
     } else {
       double xc = d.fcx[s], yc = d.fcy[s], zc = d.fcz[s];
@@ -1927,12 +1959,6 @@ __global__ void __launch_bounds__(BLK) k_flux(Dev d, Gas G, FluxPar P) {
   if (s >= d.S) return;
   double f[5];
   face_flux<F32>(d, G, P, s, f, P.doSrad ? d.srad + s : nullptr);
-  if (F32 && P.order == 2 && s < d.nF) {  // interior FP32 fluxes stay FP32 (the widening is exact)
-    float *of = d.fluxF + 5 * (size_t)s;
-#pragma unroll
-    for (int k = 0; k < 5; k++) of[k] = (float)f[k];
-    return;
-  }
   double *o = d.flux + 5 * (size_t)s;
 #pragma unroll
   for (int k = 0; k < 5; k++) o[k] = f[k];
@@ -1945,7 +1971,7 @@ struct UpdPar { int order, istage, fused, storeRhs; double ark, grk; };
 
 // rhs = -diss (diss = 0) + face fluxes in ascending slot order
 // (libfloflu/ConvectiveFluxes.F90:133-141 and the face/patch loops).
-// F32: interior faces' fluxes come from the FP32 array (FP32 evaluation mode)
+// Every face's flux is FP64 in flux[], in the FP32 evaluation mode too.
 template <int F32>
 __device__ __forceinline__ void gather_rhs(const Dev &d, int i, double rhs[5]) {
 #pragma unroll
@@ -1953,14 +1979,8 @@ __device__ __forceinline__ void gather_rhs(const Dev &d, int i, double rhs[5]) {
   for (int e = d.cfOff[i]; e < d.cfOff[i + 1]; e++) {
     int code = d.cfIdx[e];
     int slot = code >> 1;
-    double f0, f1, f2, f3, f4;
-    if (F32 && slot < d.nF) {  // interior fluxes widened, summed in FP64
-      const float *f = d.fluxF + 5 * (size_t)slot;
-      f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4];
-    } else {
-      const double *f = d.flux + 5 * (size_t)slot;
-      f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4];
-    }
+    const double *f = d.flux + 5 * (size_t)slot;
+    double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
     if (code & 1) {
       rhs[0] = rhs[0] - f0; rhs[1] = rhs[1] - f1; rhs[2] = rhs[2] - f2; rhs[3] = rhs[3] - f3; rhs[4] = rhs[4] - f4;
     } else {
@@ -2769,7 +2789,6 @@ static int create_impl(const rflu_mesh_desc *md, const rflu_params *prm, rflu_ha
       if (!h->tileLds) rc |= set_err(h, RFLU_GPU_EINVAL, "fluxPrecision 1 needs gradient tiles that fit in LDS");
       else {
         rc |= dalloc(h, &d.gradF, 16 * (size_t)N);
-        rc |= dalloc(h, &d.fluxF, 5 * (size_t)m.nFaces);
       }
     }
     if (h->tileLds) {
@@ -2926,7 +2945,7 @@ int rflu_gpu_destroy(rflu_handle *h) {
                   d.err};
   for (void *p : ptrs) if (p) hipFree(p);
   for (void *p : {(void *)d.phi, (void *)d.rhog, (void *)d.gRaw, (void *)d.tOff, (void *)d.tMem, (void *)d.sSlot,
-                  (void *)d.gradF, (void *)d.fluxF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal,
+                  (void *)d.gradF, (void *)d.cInst, (void *)d.iOff, (void *)d.iScal,
                   (void *)d.gradRaw, (void *)d.cfSlot, (void *)d.recAlt})
     if (p) hipFree(p);
   picl_free(h->picl);
