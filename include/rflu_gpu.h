@@ -113,10 +113,12 @@ typedef struct rflu_params {
    * denom (a deliberate deviation, off by default). */
   int limiterGuard;
   /* 0 (reference): FP64 throughout.  1: gradients (LSQ + limiter) and
-   * interior second-order face fluxes evaluated in FP32, with FP64 kept for
-   * the residual accumulation, the conserved-state RK update, p/T recovery,
-   * boundary patches and the dt reduction.  Needs spaceOrder 2, no
-   * particles.  No reference counterpart: an added precision mode. */
+   * interior second-order face fluxes evaluated in FP32 (each face flux is
+   * the FP64 physical flux of the left cell's state plus an FP32 difference
+   * of the scheme's flux to it), with FP64 kept for the residual
+   * accumulation, the conserved-state RK update, p/T recovery, boundary
+   * patches and the dt reduction.  Needs spaceOrder 2, no particles.  No
+   * reference counterpart: an added precision mode. */
   int fluxPrecision;
   /* AUSM+up constants (NUMERICS SIGMA, KP, KU, MINF; the reference's
    * defaults are 1, 0.25, 0.75, 0.1, libfloflu/ReadNumericsSection.F90:236-275).
