@@ -2427,22 +2427,23 @@ __device__ __forceinline__ void gauss_batch(const PiclDev &P, const int *Qp, con
 }
 
 // The particles of the listed elements (E[j] = the element's first particle
-// slot in gq, C[j] = its particle count): each half-wave takes one element,
-// its 32 lanes its particles.  Those within the cut-off go to the queue (Qp,
-// Qr; *qn entries, fewer than 64 between calls) in lane order, and every 64
-// of them are summed as one batch.  The chunks (element pair k, particles pb
-// to pb + 31) are walked in order with the next chunk's positions loaded
-// before the current one is tested, so a gather's latency overlaps a test.
+// slot in gq, C[j] = its particle count), two elements at a time as one
+// sequence over the 64 lanes (elements hold about 26 particles, so a pair
+// mostly fits one pass).  Those within the cut-off go to the queue (Qp, Qr;
+// *qn entries, fewer than 64 between calls) in lane order, and every 64 of
+// them are summed as one batch.  The chunks (element pair k, k + 1, sequence
+// positions pb to pb + 63) are walked in order with the next chunk's
+// positions loaded before the current one is tested, so a gather's latency
+// overlaps a test.
 __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, const int *C, int n, int lane,
                                             const double xv[3], const double sh[3], int *Qp, double *Qr, int &qn,
                                             const double *T, double acc[5], double cmp[5]) {
-  const int half = lane >> 5, hl = lane & 31;
   int k = 0, pb = 0;
-  int cnt = half < n ? C[half] : 0, first = half < n ? E[half] : 0;
-  int maxc = max(cnt, __shfl_xor(cnt, 32, 64));
-  // the first chunk's positions
-  bool nin = hl < cnt;
-  int npos = first + hl;
+  // the pair's counts and first slots (the same for every lane)
+  int c0 = C[0], e0 = E[0], c1 = 1 < n ? C[1] : 0, e1 = 1 < n ? E[1] : 0;
+  // the first chunk's positions: lane l holds sequence position l
+  bool nin = lane < c0 + c1;
+  int npos = lane < c0 ? e0 + lane : e1 + (lane - c0);
   double2 nxy = make_double2(0.0, 0.0);
   double nz = 0.0;
   if (nin) {
@@ -2455,21 +2456,22 @@ __device__ __forceinline__ void gauss_flush(const PiclDev &P, const int *E, cons
     const double2 gxy = nxy;
     const double gz = nz;
     // advance to the next chunk (uniform over the wave) and start its loads
-    pb += 32;
-    if (pb >= maxc) {
+    pb += 64;
+    if (pb >= c0 + c1) {
       k += 2;
       pb = 0;
       if (k < n) {
-        const int j = k + half;
-        cnt = j < n ? C[j] : 0;
-        first = j < n ? E[j] : 0;
-        maxc = max(cnt, __shfl_xor(cnt, 32, 64));
+        c0 = C[k];
+        e0 = E[k];
+        c1 = k + 1 < n ? C[k + 1] : 0;
+        e1 = k + 1 < n ? E[k + 1] : 0;
       }
     }
     const bool more = k < n;
     if (more) {
-      nin = pb + hl < cnt;
-      npos = first + pb + hl;
+      const int q = pb + lane;
+      nin = q < c0 + c1;
+      npos = q < c0 ? e0 + q : e1 + (q - c0);
       if (nin) {
         nxy = reinterpret_cast<const double2 *>(P.gq + 4 * (size_t)npos)[0];
         nz = P.gq[4 * (size_t)npos + 2];
