@@ -10,8 +10,8 @@ every particle has some 150 neighbours within the neighbour width
 
 The GPU adds a particle's pair terms in the order its cube search meets the
 neighbours, the oracle in ppiclF's particle-index order, and the forces call
-pow/exp/log/atan2/sin/cos: states are held to 1e-12 of each column's largest
-magnitude, as the other particle models are.  The GPU's own order is fixed
+pow/exp/log/atan2/sin/cos: states are held to 1e-12 per element (positions,
+velocities and momenta as vectors), as the other particle models are.  The GPU's own order is fixed
 (the cube member lists are sorted), so graph replays equal eager steps bit
 for bit.
 """
@@ -23,14 +23,27 @@ from rocflupicl_amd import configs, rflu
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+FLOOR = 1e-9
 D0 = 115e-6
 
 
-def close(a, b, tol=TOL):
+def close(a, b, tol=TOL, vectors=(), floor=FLOOR):
+    """Largest element-wise relative error, |a - b| / max(|b|, FLOOR * the
+    column's largest magnitude); each group of columns in `vectors` (a
+    position, a velocity, a momentum) is measured as one vector, the error
+    vector's length over the vector's length, with the same floor."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
-    err = (np.abs(a - b) / scale).max()
+    if b.ndim == 1:
+        a, b = a[:, None], b[:, None]
+    grouped = {c for g in vectors for c in g}
+    groups = [list(g) for g in vectors] + [[c] for c in range(b.shape[1]) if c not in grouped]
+    err = 0.0
+    for g in groups:
+        num = np.linalg.norm(a[:, g] - b[:, g], axis=1)
+        mag = np.linalg.norm(b[:, g], axis=1)
+        scale = np.maximum(np.maximum(mag, floor * mag.max()), 1e-300)
+        err = max(err, float((num / scale).max()))
     return err <= tol, err
 
 
@@ -60,13 +73,13 @@ def oracle_run(oracle_mod, mesh, params, ic, y, pc, picl, steps):
     return o
 
 
-def check(g, o):
+def check(g, o, floor=FLOOR):
     gp, op = g.picl_particles(), o.picl_particles()
     assert np.array_equal(gp["elem"], op["elem"])
     for k in ("y", "y1", "ydot", "ydotc"):
-        ok, err = close(gp[k], op[k])
+        ok, err = close(gp[k], op[k], vectors=([0, 1, 2], [3, 4, 5]), floor=floor)
         assert ok, (k, err)
-    ok, err = close(g.get_state()[0], o.cv())
+    ok, err = close(g.get_state()[0], o.cv(), vectors=([1, 2, 3],), floor=floor)
     assert ok, err
     return op
 
@@ -93,7 +106,12 @@ def test_briney_with_periodic_images_and_tangential_collisions(oracle_mod):
     case = cloud(collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
     g = gpu_run(*case, steps=3)
     o = oracle_run(oracle_mod, *case, steps=3)
-    check(g, o)
+    # with collisions the pair forces come in opposite pairs and partly
+    # cancel, and the GPU adds them in cube order, the oracle in particle-
+    # index order: a few sums end far below their terms, so this case is
+    # held to 1e-12 of each column's largest magnitude (per element it
+    # measured up to 3e-12 on ydotc and 5e-11 on the cell momenta)
+    check(g, o, floor=1.0)
 
 
 def test_briney_numerical_correction_branch(oracle_mod):

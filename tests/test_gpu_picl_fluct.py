@@ -7,7 +7,8 @@ cube-grid neighbour search as Briney's pair terms, summed with compensated
 (TwoSum) sums so that their order does not show; the six uniforms of a
 particle's call are a counter-based hash of (seed, call, particle, draw) on
 both sides.  The SDE step calls log/cos/sin/exp/pow, so states are held to
-1e-12 of each column's largest magnitude, as the other particle models are.
+1e-12 per element (positions, velocities and momenta as vectors), as the
+other particle models are.
 The particles carry a seeded velocity scatter of a few m/s, so the granular
 temperature, and with it the fluctuations, is far from zero.
 """
@@ -19,14 +20,27 @@ from rocflupicl_amd import configs, rflu
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+FLOOR = 1e-9
 D0 = 115e-6
 
 
-def close(a, b, tol=TOL):
+def close(a, b, tol=TOL, vectors=(), floor=FLOOR):
+    """Largest element-wise relative error, |a - b| / max(|b|, FLOOR * the
+    column's largest magnitude); each group of columns in `vectors` (a
+    position, a velocity, a momentum) is measured as one vector, the error
+    vector's length over the vector's length, with the same floor."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
-    err = (np.abs(a - b) / scale).max()
+    if b.ndim == 1:
+        a, b = a[:, None], b[:, None]
+    grouped = {c for g in vectors for c in g}
+    groups = [list(g) for g in vectors] + [[c] for c in range(b.shape[1]) if c not in grouped]
+    err = 0.0
+    for g in groups:
+        num = np.linalg.norm(a[:, g] - b[:, g], axis=1)
+        mag = np.linalg.norm(b[:, g], axis=1)
+        scale = np.maximum(np.maximum(mag, floor * mag.max()), 1e-300)
+        err = max(err, float((num / scale).max()))
     return err <= tol, err
 
 
@@ -60,13 +74,13 @@ def oracle_run(oracle_mod, mesh, params, ic, y, pc, picl, steps):
     return o
 
 
-def check(g, o):
+def check(g, o, floor=FLOOR):
     gp, op = g.picl_particles(), o.picl_particles()
     assert np.array_equal(gp["elem"], op["elem"])
     for k in ("y", "y1", "ydot", "ydotc"):
-        ok, err = close(gp[k], op[k])
+        ok, err = close(gp[k], op[k], vectors=([0, 1, 2], [3, 4, 5]), floor=floor)
         assert ok, (k, err)
-    ok, err = close(g.get_state()[0], o.cv())
+    ok, err = close(g.get_state()[0], o.cv(), vectors=([1, 2, 3],), floor=floor)
     assert ok, err
     return op
 
@@ -123,7 +137,12 @@ def test_fluctuations_with_briney_added_mass_and_collisions(oracle_mod):
     case = cloud(mono=True, qsFluct=2, amFlag=2, collisionalFlag=2, ksp=10.0, erest=0.7, periodicY=1, periodicZ=1)
     g = gpu_run(*case, steps=3)
     o = oracle_run(oracle_mod, *case, steps=3)
-    check(g, o)
+    # with collisions the pair forces come in opposite pairs and partly
+    # cancel, and the GPU adds them in cube order, the oracle in particle-
+    # index order: a few sums end far below their terms, so this case is
+    # held to 1e-12 of each column's largest magnitude (per element it
+    # measured up to 3e-12 on ydotc and 5e-11 on the cell momenta)
+    check(g, o, floor=1.0)
 
 
 def test_fluctuation_graph_replays_equal_eager_steps():
